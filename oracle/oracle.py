@@ -1,0 +1,315 @@
+"""ctypes front end of the CPU oracle (``tsph_oracle.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke``
+and by bench.py's CPU-baseline leg, never by the product package.  Arrays are
+numpy, row-major (AoS), float64 unless a ``_f32`` entry point is named.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libtsph_oracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        _lib = C.CDLL(str(LIB_PATH))
+        _declare(_lib)
+    return _lib
+
+
+_dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+_lp = C.POINTER(C.c_int64)
+
+
+class MorrisParams(C.Structure):
+    _fields_ = [("dim", C.c_int), ("kernel3d", C.c_int),
+                ("h", C.c_double), ("rho0", C.c_double), ("c0", C.c_double),
+                ("gamma", C.c_double), ("gravity", C.c_double), ("nu", C.c_double),
+                ("wall_friction", C.c_double), ("dt", C.c_double),
+                ("diff_scale", C.c_double), ("diff_form", C.c_int),
+                ("eig_floor", C.c_double), ("reaction_style", C.c_int)]
+
+
+class DPParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("dx", "c0", "ref_speed", "dt", "rho0", "gravity",
+                                          "nu", "diffusion", "alpha")]
+
+
+def _declare(L):
+    for suf, ptr in (("f64", _dp), ("f32", _fp)):
+        f = getattr(L, f"orc_cell_linked_pairs_{suf}")
+        f.restype = C.c_int64
+        f.argtypes = [ptr, C.c_int64, C.c_int, C.c_double, C.POINTER(_lp), _lp, _lp, _lp]
+        f = getattr(L, f"orc_brute_force_pairs_{suf}")
+        f.restype = C.c_int64
+        f.argtypes = [ptr, C.c_int64, C.c_int, C.c_double, C.POINTER(_lp)]
+    L.orc_periodic_pairs_f64.restype = C.c_int64
+    L.orc_periodic_pairs_f64.argtypes = [_dp, C.c_int64, C.c_int, _dp, C.c_double, C.POINTER(_lp)]
+    L.orc_free.argtypes = [C.c_void_p]
+    L.orc_kernel_eval.argtypes = [C.c_int, _dp, C.c_int64, C.c_double, _dp, _dp, _dp]
+    L.orc_sym_eig.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_tait2d.argtypes = [_dp, C.c_int64, C.c_double, C.c_double, C.c_double, _dp]
+    L.orc_rhs_morris.argtypes = [C.POINTER(MorrisParams), C.c_int64, C.c_int64, _dp, _dp, _dp,
+                                 _dp, _lp, C.c_int64, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_threed_accelerations.argtypes = [C.c_int64, C.c_int64, _dp, _dp, _dp, C.c_double, _lp,
+                                           C.c_int64, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, _dp, _dp, C.c_int, C.c_double,
+                                           C.c_double, C.c_int, C.c_double, _dp, _dp, _dp]
+    L.orc_apply_boundary.restype = C.c_int64
+    L.orc_apply_boundary.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double]
+    L.orc_delta_plus_step.restype = C.c_int
+    L.orc_delta_plus_step.argtypes = [C.POINTER(DPParams), C.c_int64, C.c_int64, _dp, _dp, _dp,
+                                      _dp, _dp, _dp, _dp, _lp, C.c_int64, _dp, _dp, _dp, _dp,
+                                      _dp, _dp]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _l(a):
+    return a.ctypes.data_as(_lp) if a is not None else None
+
+
+def _take_pairs(ptr, count):
+    out = np.ctypeslib.as_array(ptr, shape=(max(count, 0) * 2,)).copy().reshape(-1, 2) \
+        if count > 0 else np.zeros((0, 2), np.int64)
+    lib().orc_free(ptr)
+    return out
+
+
+def cell_linked_pairs(pos, cutoff, with_cells=False):
+    """neighbor_search.cell_linked_pairs (neighbor_search.py:44-120)."""
+    pos = np.ascontiguousarray(pos)
+    n, dim = pos.shape
+    f64 = pos.dtype == np.float64
+    if not f64:
+        pos = pos.astype(np.float32)
+    cells = np.zeros((n, dim), np.int64)
+    keys = np.zeros(n, np.int64)
+    order = np.zeros(n, np.int64)
+    out = _lp()
+    fn = lib().orc_cell_linked_pairs_f64 if f64 else lib().orc_cell_linked_pairs_f32
+    data = pos.ctypes.data_as(_dp if f64 else _fp)
+    cnt = fn(data, n, dim, float(cutoff), C.byref(out), _l(cells), _l(keys), _l(order))
+    pairs = _take_pairs(out, cnt)
+    if with_cells:
+        return pairs, cells, keys, order
+    return pairs
+
+
+def brute_force_pairs(pos, cutoff):
+    pos = np.ascontiguousarray(pos)
+    n, dim = pos.shape
+    f64 = pos.dtype == np.float64
+    if not f64:
+        pos = pos.astype(np.float32)
+    out = _lp()
+    fn = lib().orc_brute_force_pairs_f64 if f64 else lib().orc_brute_force_pairs_f32
+    cnt = fn(pos.ctypes.data_as(_dp if f64 else _fp), n, dim, float(cutoff), C.byref(out))
+    return _take_pairs(out, cnt)
+
+
+def periodic_pairs(pos, box, cutoff):
+    """threed.neighbors3.periodic_pairs (neighbors3.py:71-142)."""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n, dim = pos.shape
+    b = np.broadcast_to(np.asarray(box, np.float64), (dim,)).copy()
+    out = _lp()
+    cnt = lib().orc_periodic_pairs_f64(_d(pos), n, dim, _d(b), float(cutoff), C.byref(out))
+    return _take_pairs(out, cnt)
+
+
+def kernel_eval(kind, r, h):
+    """kind 0: 2-D cubic, 1: 3-D cubic (+Laplacian), 2: 2-D Wendland C2."""
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    w = np.zeros_like(r); s = np.zeros_like(r); lap = np.zeros_like(r)
+    lib().orc_kernel_eval(kind, _d(r), r.size, float(h), _d(w), _d(s), _d(lap))
+    return w, s, lap
+
+
+def sym_eig(m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    n = m.shape[0]
+    w = np.zeros(n); v = np.zeros((n, n))
+    lib().orc_sym_eig(n, _d(m), _d(w), _d(v))
+    return w, v
+
+
+def tait2d(rho, rho0, c0, gamma):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    p = np.zeros_like(rho)
+    lib().orc_tait2d(_d(rho), rho.size, rho0, c0, gamma, _d(p))
+    return p
+
+
+def rhs_morris(pos, vel, mass, wall, pairs, *, dim, h, rho0, c0, gamma, gravity, nu,
+               wall_friction=0.0, dt=0.0, diff_scale=0.0, diff_form=0, eig_floor=0.1,
+               wall_mass=None, wall_vel=None, wall_accel=None, kernel3d=False,
+               reaction_style=0):
+    """droplet_impact.rhs (2-D) / dam_break_3d.rhs_3d (3-D) in FP64.
+
+    Returns (rho, p, acc, wall_p, wall_rho, reaction)."""
+    pos = np.asarray(pos, np.float64); vel = np.asarray(vel, np.float64)
+    mass = np.asarray(mass, np.float64)
+    n = pos.shape[0]
+    wall = np.zeros((0, dim)) if wall is None else np.asarray(wall, np.float64)
+    nw = wall.shape[0]
+    pos_all = np.ascontiguousarray(np.concatenate([pos, wall]))
+    wv = np.zeros((nw, dim)) if wall_vel is None else np.asarray(wall_vel, np.float64)
+    vel_all = np.ascontiguousarray(np.concatenate([vel, wv]))
+    wm = np.full(nw, 0.0 if wall_mass is None else float(wall_mass))
+    mass_all = np.ascontiguousarray(np.concatenate([mass, wm]))
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    P = MorrisParams(dim, 1 if kernel3d else 0, h, rho0, c0, gamma, gravity, nu,
+                     wall_friction, dt, diff_scale, diff_form, eig_floor, reaction_style)
+    rho = np.zeros(n); p = np.zeros(n); acc = np.zeros((n, dim))
+    wp = np.zeros(nw); wr = np.zeros(nw); reac = np.zeros((nw, dim))
+    wa = None if wall_accel is None else np.ascontiguousarray(wall_accel, np.float64)
+    lib().orc_rhs_morris(C.byref(P), n, nw, _d(pos_all), _d(vel_all), _d(mass_all), _d(wa),
+                         _l(pairs), pairs.shape[0], _d(rho), _d(p), _d(acc), _d(wp), _d(wr),
+                         _d(reac) if reaction_style else None)
+    return rho, p, acc, wp, wr, reac
+
+
+def threed_accelerations(pos, vel, mass, h, pairs, *, rho0, sound_speed, gamma=7.0,
+                         viscosity=0.0, gravity=None, box=None, n_fluid=None,
+                         renormalise=True, density_diffusion=0.0, dt=None,
+                         density_diffusion_form="antuono",
+                         density_diffusion_eigenvalue_floor=0.1):
+    """threed.operators3.accelerations (operators3.py:212-256), FP64."""
+    pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
+    mass = np.ascontiguousarray(mass, np.float64)
+    n = pos.shape[0]
+    nf = n if n_fluid is None else int(n_fluid)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    g = None if gravity is None else np.ascontiguousarray(gravity, np.float64)
+    b = None if box is None else np.broadcast_to(np.asarray(box, np.float64), (3,)).copy()
+    acc = np.zeros((n, 3)); rho = np.zeros(n); p = np.zeros(n)
+    form = 1 if density_diffusion_form == "antuono" else 0
+    lib().orc_threed_accelerations(n, nf, _d(pos), _d(vel), _d(mass), h, _l(pairs),
+                                   pairs.shape[0], rho0, sound_speed, gamma, viscosity,
+                                   _d(g), _d(b), 1 if renormalise else 0,
+                                   float(density_diffusion), float(dt or 0.0), form,
+                                   density_diffusion_eigenvalue_floor, _d(acc), _d(rho), _d(p))
+    return acc, rho, p
+
+
+def apply_boundary(pos, vel, lo, hi, tol):
+    pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
+    dim = pos.shape[1]
+    lo = np.ascontiguousarray(lo, np.float64); hi = np.ascontiguousarray(hi, np.float64)
+    cnt = lib().orc_apply_boundary(dim, pos.shape[0], _d(pos), _d(vel), _d(lo), _d(hi), tol)
+    return pos, vel, int(cnt)
+
+
+def delta_plus_step(pos, vel, rho, mass, *, dx, sound_speed, reference_speed, dt, rho0=1000.0,
+                    gravity=9.81, viscosity=1e-6, diffusion=0.1, alpha=0.02, wall=None,
+                    wall_velocity=None, wall_normal=None, pairs=None):
+    """delta_plus.delta_plus_step (delta_plus.py:105-229), FP64.
+
+    Returns dict(position, velocity, density, acceleration, wall_reaction, shift)."""
+    pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
+    rho = np.ascontiguousarray(rho, np.float64); mass = np.ascontiguousarray(mass, np.float64)
+    N = pos.shape[0]
+    wall = np.zeros((0, 2)) if wall is None else np.ascontiguousarray(wall, np.float64)
+    W = wall.shape[0]
+    wv = np.zeros((W, 2)) if wall_velocity is None else np.ascontiguousarray(wall_velocity, np.float64)
+    wn = np.zeros((W, 2)) if wall_normal is None else np.asarray(wall_normal, np.float64)
+    if W:
+        wn = np.ascontiguousarray(wn / np.linalg.norm(wn, axis=1)[:, None])
+    if pairs is None:
+        pairs = cell_linked_pairs(np.concatenate([pos, wall]), 4 * dx)
+    pairs = np.ascontiguousarray(pairs, np.int64).reshape(-1, 2)
+    P = DPParams(dx, sound_speed, reference_speed, dt, rho0, gravity, viscosity, diffusion, alpha)
+    out = {k: np.zeros((N, 2)) for k in ("position", "velocity", "acceleration", "shift")}
+    out["density"] = np.zeros(N)
+    out["wall_reaction"] = np.zeros((W, 2))
+    st = lib().orc_delta_plus_step(C.byref(P), N, W, _d(pos), _d(vel), _d(rho), _d(mass),
+                                   _d(wall), _d(wv), _d(wn), _l(pairs), pairs.shape[0],
+                                   _d(out["position"]), _d(out["velocity"]), _d(out["density"]),
+                                   _d(out["acceleration"]), _d(out["wall_reaction"]),
+                                   _d(out["shift"]))
+    if st == 1:
+        raise FloatingPointError("nonpositive extrapolated wall density")
+    if st == 2:
+        raise FloatingPointError("delta-plus density became nonpositive or nonfinite")
+    return out
+
+
+# --------------------------------------------------------------------------
+# Step drivers (2-D WCSPH loop of droplet_impact.run :1674-1763), used as the
+# CPU baseline and for trajectory parity.
+# --------------------------------------------------------------------------
+
+def lattice_kernel_sum_2d(h, dx):
+    """droplet_impact.lattice_kernel_sum (droplet_impact.py:577-598)."""
+    reach = int(math.ceil(2.0 * h / dx)) + 1
+    off = np.arange(-reach, reach + 1, dtype=np.float64) * dx
+    gx, gy = np.meshgrid(off, off, indexing="xy")
+    r = np.sqrt(gx ** 2 + gy ** 2).reshape(-1)
+    w, _, _ = kernel_eval(0, r, h)
+    return float(w.sum())
+
+
+class Stepper2D:
+    """Symplectic-Euler WCSPH loop with the Verlet pair cache (FP64)."""
+
+    def __init__(self, pos, vel, mass, wall, *, h, rho0, c0, gamma, gravity, nu, dt,
+                 wall_mass, skin, lo, hi, tol, wall_friction=0.0, diff_scale=0.0,
+                 diff_form=0, eig_floor=0.1):
+        self.pos = np.array(pos, np.float64); self.vel = np.array(vel, np.float64)
+        self.mass = np.asarray(mass, np.float64); self.wall = np.asarray(wall, np.float64)
+        self.kw = dict(dim=2, h=h, rho0=rho0, c0=c0, gamma=gamma, gravity=gravity, nu=nu,
+                       wall_friction=wall_friction, dt=dt, diff_scale=diff_scale,
+                       diff_form=diff_form, eig_floor=eig_floor, wall_mass=wall_mass)
+        self.h, self.dt, self.skin = h, dt, skin
+        self.lo, self.hi, self.tol = lo, hi, tol
+        self.anchor = None
+        self.pairs = None
+        self.rebuilds = 0
+        self.escaped = 0
+
+    def pairs_for_step(self):
+        n = self.pos.shape[0]
+        allp = np.concatenate([self.pos, self.wall])
+        if self.anchor is not None:
+            disp = float(np.sqrt(((allp - self.anchor) ** 2).sum(1)).max()) if len(allp) else 0.0
+        else:
+            disp = math.inf
+        if self.pairs is None or disp > 0.5 * self.skin:
+            pairs = cell_linked_pairs(allp, 2.0 * self.h + self.skin)
+            keep = (pairs[:, 0] < n) | (pairs[:, 1] < n)
+            self.pairs = pairs[keep]
+            self.anchor = allp.copy()
+            self.rebuilds += 1
+        return self.pairs
+
+    def step(self):
+        pairs = self.pairs_for_step()
+        rho, p, acc, _, _, _ = rhs_morris(self.pos, self.vel, self.mass, self.wall, pairs,
+                                          **self.kw)
+        self.vel = self.vel + self.dt * acc
+        self.pos = self.pos + self.dt * self.vel
+        self.pos, self.vel, cnt = apply_boundary(self.pos, self.vel, self.lo, self.hi, self.tol)
+        self.escaped += cnt
+        return rho, p, acc

@@ -1,0 +1,335 @@
+"""Write the golden vectors that pin the oracle to the reference.
+
+Run in the build container only (the reference is not on the GPU box):
+
+    python tests/golden/make_golden.py [/path/to/reference]
+
+It imports the reference package from ``<reference>/src`` and records, for
+small seeded inputs, the outputs of the reference functions the oracle
+restates.  Every fixture is data: inputs and the reference's outputs, stored
+as ``.npz`` (no pickles) next to this script.
+
+Fixtures
+  pairs.npz          cell_linked_pairs / brute_force_pairs / periodic_pairs
+                     on seeded clouds (1-D..3-D, FP32 and FP64, duplicates
+                     and at-cutoff points) - neighbor_search.py:29-120,
+                     threed/neighbors3.py:71-170
+  rhs2d.npz          droplet_impact.rhs on the small impact state with walls,
+                     friction, Molteni/Antuono diffusion, split loads, FP64
+                     and FP32 - droplet_impact.py:810-1228
+  lattices.npz       make_particles / wall_lattice / make_column and the 3-D
+                     lattices - droplet_impact.py:625-754, dam_break.py:143-169,
+                     dam_break_3d.py:185-283, kleefsman_obstacle_3d.py:187-205
+  run2d.npz          droplet_impact.run rows with output_every = 1 (meters per
+                     step) and the final snapshot - droplet_impact.py:1607-1795
+  threed.npz         threed.operators3.accelerations (periodic, renormalised,
+                     walls + Adami, Antuono diffusion) - operators3.py:51-295
+  rhs3d.npz          dam_break_3d.rhs_3d with wall forces - dam_break_3d.py:290-422
+  deltaplus.npz      delta_plus_step with and without walls - delta_plus.py:105-229
+"""
+from __future__ import annotations
+
+import math
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def main(reference: str = "/root/reference") -> None:
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    torch.set_num_threads(1)
+    from tensorsph.neighbor_search import (brute_force_pairs, cell_linked_pairs,
+                                            PairListCache)
+    from tensorsph.threed.neighbors3 import periodic_pairs
+    from tensorsph import droplet_impact as di
+    from tensorsph import dam_break as db
+    from tensorsph import dam_break_3d as db3
+    from tensorsph import kleefsman_obstacle_3d as kl
+    from tensorsph.threed import operators3 as op3
+    from tensorsph.threed import kernel3 as k3
+    from tensorsph.threed.solver import cubic_lattice, wall_shell
+    from tensorsph.delta_plus import delta_plus_step
+
+    # ------------------------------------------------------------------ pairs
+    out = {}
+    case = 0
+    for dim in (1, 2, 3):
+        for seed in (0, 7, 42):
+            for dtype in (torch.float64, torch.float32):
+                g = torch.Generator().manual_seed(seed)
+                pos = 4.0 * torch.randn((137, dim), generator=g, dtype=torch.float64)
+                pos[1] = pos[0]
+                pos[2] = pos[0]
+                pos[2, 0] += 0.25
+                pos = pos.to(dtype)
+                out[f"c{case}_pos"] = _np(pos)
+                out[f"c{case}_cutoff"] = np.array(0.25)
+                out[f"c{case}_cell"] = _np(cell_linked_pairs(pos, 0.25))
+                out[f"c{case}_brute"] = _np(brute_force_pairs(pos, 0.25))
+                case += 1
+    # lattice exactly at the cutoff (ties at d^2 == cutoff^2) in both dtypes
+    for dtype in (torch.float64, torch.float32):
+        ax = torch.arange(12, dtype=torch.float64) * 0.1
+        pos = torch.cartesian_prod(ax, ax).to(dtype)
+        out[f"c{case}_pos"] = _np(pos)
+        out[f"c{case}_cutoff"] = np.array(0.1)
+        out[f"c{case}_cell"] = _np(cell_linked_pairs(pos, 0.1))
+        out[f"c{case}_brute"] = _np(brute_force_pairs(pos, 0.1))
+        case += 1
+    # uniform cloud, the reference's timing case shape (neighbor-search.md)
+    g = torch.Generator().manual_seed(21)
+    pos = torch.rand((1000, 2), generator=g, dtype=torch.float64)
+    out[f"c{case}_pos"] = _np(pos)
+    out[f"c{case}_cutoff"] = np.array(0.1)
+    out[f"c{case}_cell"] = _np(cell_linked_pairs(pos, 0.1))
+    out[f"c{case}_brute"] = _np(brute_force_pairs(pos, 0.1))
+    case += 1
+    out["ncases"] = np.array(case)
+    # periodic (minimum image), jittered lattice
+    n = 7
+    dx = 1.0 / n
+    lat = cubic_lattice((n, n, n), dx) + 0.5 * dx
+    rng = np.random.Generator(np.random.PCG64(20261203))
+    jit = torch.from_numpy(rng.uniform(-0.01 * dx, 0.01 * dx, size=lat.shape))
+    lat = (lat + jit).remainder(1.0)
+    cut = 2.0 * 1.3 * dx * 1.25
+    out["per_pos"] = _np(lat)
+    out["per_cutoff"] = np.array(cut)
+    out["per_pairs"] = _np(periodic_pairs(lat, 1.0, cut))
+    np.savez_compressed(HERE / "pairs.npz", **out)
+
+    # ------------------------------------------------------------------ rhs2d
+    out = {}
+    cpu = torch.device("cpu")
+    for tag, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        torch.set_default_dtype(dtype)
+        variants = {
+            "base": {},
+            "fric": {"wall_friction": 0.5},
+            "molt": {"density_diffusion": 0.1},
+            "antu": {"density_diffusion": 0.1, "density_diffusion_form": "antuono"},
+        }
+        for name, over in variants.items():
+            cfg = di.Config(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05,
+                            drop_center_y=0.25, device="cpu", viscosity=0.01, **over)
+            pos, mass = di.make_particles(cfg, cpu)
+            wall = di.wall_lattice(cfg, cpu)
+            g = torch.Generator().manual_seed(4)
+            vel = torch.randn(pos.shape, generator=g, dtype=dtype) * 0.05
+            # compress the lattice a little so pressure is non-zero everywhere
+            pos = pos * 0.97
+            h = cfg.h_factor * cfg.dx
+            cache = PairListCache(2.0 * h, 0.25 * h, fluid_count=pos.shape[0])
+            pairs, _ = cache.get(torch.cat((pos, wall)))
+            rho, p, acc, loads = di.rhs(pos, vel, mass, cfg, pairs=pairs, wall=wall,
+                                        split=True)
+            key = f"{tag}_{name}"
+            out[key + "_pos"] = _np(pos); out[key + "_vel"] = _np(vel)
+            out[key + "_mass"] = _np(mass); out[key + "_wall"] = _np(wall)
+            out[key + "_pairs"] = _np(pairs)
+            out[key + "_rho"] = _np(rho); out[key + "_p"] = _np(p); out[key + "_acc"] = _np(acc)
+            out[key + "_reaction"] = _np(loads["reaction"])
+            out[key + "_face"] = _np(loads["face_pressure"])
+            out[key + "_params"] = np.array([cfg.dx, cfg.h_factor, cfg.rho0, cfg.sound_speed,
+                                             cfg.gamma, cfg.gravity, cfg.viscosity,
+                                             cfg.wall_friction, cfg.dt, cfg.density_diffusion,
+                                             1.0 if cfg.density_diffusion_form == "antuono" else 0.0,
+                                             cfg.density_diffusion_eigenvalue_floor,
+                                             float(di.lattice_mass_unit(cfg))])
+            if name == "base":
+                rho2, p2, acc2 = di.rhs(pos, vel, mass, cfg, wall=wall)   # pairs=None
+                out[key + "_nopairs_rho"] = _np(rho2)
+                out[key + "_nopairs_acc"] = _np(acc2)
+        torch.set_default_dtype(torch.float64)
+    # dense-reference case (no walls), test_pair_based_rhs.py:30-42
+    g = torch.Generator().manual_seed(5)
+    cfg = di.Config(dx=0.1, h_factor=1.3, rho0=1000.0, sound_speed=8.0, gamma=7.0,
+                    gravity=9.81, viscosity=0.03, device="cpu")
+    pos = torch.randn((61, 2), generator=g, dtype=torch.float64) * 0.12
+    vel = torch.randn((61, 2), generator=g, dtype=torch.float64) * 0.2
+    mass = (torch.rand((61,), generator=g, dtype=torch.float64) + 0.2) * 0.01
+    rho, p, acc = di.rhs(pos, vel, mass, cfg)
+    out["dense_pos"] = _np(pos); out["dense_vel"] = _np(vel); out["dense_mass"] = _np(mass)
+    out["dense_rho"] = _np(rho); out["dense_p"] = _np(p); out["dense_acc"] = _np(acc)
+    np.savez_compressed(HERE / "rhs2d.npz", **out)
+
+    # --------------------------------------------------------------- lattices
+    out = {}
+    torch.set_default_dtype(torch.float64)
+    cfg1 = di.Config(dx=0.005, pool_width=1.0, pool_depth=0.2, drop_radius=0.05,
+                     drop_center_y=0.35, device="cpu", viscosity=1e-6,
+                     sound_speed=10.0 * math.sqrt(2 * 9.81 * 0.2), dt=9.84e-5)
+    pos, mass = di.make_particles(cfg1, cpu)
+    out["impact_pos"] = _np(pos); out["impact_mass"] = _np(mass)
+    out["impact_wall_top06"] = _np(di.wall_lattice(cfg1, cpu, top=0.6))
+    out["impact_wall_default"] = _np(di.wall_lattice(cfg1, cpu))
+    out["impact_mass_unit"] = np.array(di.lattice_mass_unit(cfg1))
+    out["lattice_sum_13"] = np.array(di.lattice_kernel_sum(1.3 * 0.005, 0.005))
+    cfg2 = db.DamBreakConfig(dx=0.05, tank_height=3.0, column_width=1.0, column_height=2.0,
+                             pool_width=4.0, sound_speed=10.0 * math.sqrt(2 * 9.81 * 2.0))
+    pos, mass = db.make_column(cfg2, cpu)
+    out["dam_pos"] = _np(pos); out["dam_mass"] = _np(mass)
+    out["dam_wall"] = _np(di.wall_lattice(cfg2, cpu, top=cfg2.tank_top()))
+    cfg3 = db3.DamBreak3DConfig(dx=0.05, tank_height=0.6, tank_depth=0.3, column_width=0.15,
+                                column_depth=0.15, column_height=0.3, pool_width=1.0)
+    pos, mass = db3.make_column_3d(cfg3, cpu)
+    out["dam3_pos"] = _np(pos); out["dam3_mass"] = _np(mass)
+    out["dam3_wall"] = _np(db3.wall_lattice_3d(cfg3, cpu))
+    out["k3_lattice_sum"] = np.array(k3.lattice_kernel_sum(0.1, 0.13))
+    cfgk = kl.Kleefsman3DConfig(dx=0.04)
+    out["kl_obstacle"] = _np(kl.obstacle_lattice_3d(cfgk, cpu))
+    np.savez_compressed(HERE / "lattices.npz", **out)
+
+    # ------------------------------------------------------------------ run2d
+    out = {}
+    torch.set_default_dtype(torch.float64)
+    cfg = di.Config(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05,
+                    drop_center_y=0.25, device="cpu", viscosity=0.01, steps=40,
+                    output_every=1, fail_fast=False, dt=9.0e-4)
+    with tempfile.TemporaryDirectory() as tmp:
+        rows = di.run(cfg, tmp, plot=False)
+        snap = torch.load(Path(tmp) / f"particles_{cfg.steps:06d}.pt", weights_only=True)
+    keys = ["step", "time", "mass_total", "momentum_x", "momentum_y", "rho_min", "rho_max",
+            "rho_mean", "pressure_min", "pressure_max", "max_speed", "max_acceleration",
+            "kinetic_energy", "y_min", "y_max"]
+    out["row_keys"] = np.array(keys)
+    out["rows"] = np.array([[float(r[k]) for k in keys] for r in rows])
+    out["final_pos"] = _np(snap["position"]); out["final_rho"] = _np(snap["density"])
+    out["params"] = np.array([cfg.dx, cfg.steps, cfg.dt, cfg.viscosity])
+    np.savez_compressed(HERE / "run2d.npz", **out)
+
+    # ----------------------------------------------------------------- threed
+    out = {}
+    n = 8
+    dx = 1.0 / n
+    h = 1.3 * dx
+    lat = cubic_lattice((n, n, n), dx) + 0.5 * dx
+    rng = np.random.Generator(np.random.PCG64(20261203))
+    lat = (lat + torch.from_numpy(rng.uniform(-0.01 * dx, 0.01 * dx, size=lat.shape))).remainder(1.0)
+    mass = torch.full((lat.shape[0],), k3.calibrated_particle_mass(dx, h, 1.0))
+    vel = torch.zeros_like(lat)
+    vel[:, 0] = 1e-3 * torch.sin(2 * math.pi * lat[:, 2])
+    vel[:, 1] = 3e-4 * torch.cos(2 * math.pi * lat[:, 0])
+    pairs = periodic_pairs(lat, 1.0, 2.0 * h * 1.25)
+    acc, rho, p = op3.accelerations(lat, vel, mass, h, pairs, rho0=1.0, sound_speed=10.0,
+                                    gamma=7.0, viscosity=0.01, box=1.0, renormalise=True)
+    out.update(per_pos=_np(lat), per_vel=_np(vel), per_mass=_np(mass), per_pairs=_np(pairs),
+               per_acc=_np(acc), per_rho=_np(rho), per_p=_np(p), per_h=np.array(h))
+    acc, rho, p = op3.accelerations(lat, vel, mass, h, pairs, rho0=1.0, sound_speed=10.0,
+                                    gamma=7.0, viscosity=0.01, box=1.0, renormalise=False)
+    out.update(per_raw_acc=_np(acc))
+    # walls + Adami + gravity + Antuono diffusion (hydrostatic-like block)
+    nf = (5, 5, 4)
+    dxw = 0.1
+    hw = 1.3 * dxw
+    fluid = cubic_lattice(nf, dxw, origin=(3 * dxw, 3 * dxw, 3 * dxw))
+    shell = wall_shell(nf, 3, dxw)
+    g2 = torch.Generator().manual_seed(9)
+    fluid = fluid + 0.02 * dxw * torch.randn(fluid.shape, generator=g2)
+    allp = torch.cat((fluid, shell))
+    m = torch.full((allp.shape[0],), k3.calibrated_particle_mass(dxw, hw, 1000.0))
+    m[: fluid.shape[0]] *= 1.0 + 0.01 * torch.rand(fluid.shape[0], generator=g2)
+    v = torch.zeros_like(allp)
+    v[: fluid.shape[0]] = 0.05 * torch.randn(fluid.shape, generator=g2)
+    pw = periodic_pairs  # noqa: F841  (walls use the open search)
+    pairs = cell_linked_pairs(allp, 2.0 * hw)
+    grav = [0.0, 0.0, -9.81]
+    acc, rho, p = op3.accelerations(allp, v, m, hw, pairs, rho0=1000.0, sound_speed=20.0,
+                                    gamma=7.0, viscosity=1e-3, gravity=grav,
+                                    n_fluid=fluid.shape[0], renormalise=True,
+                                    density_diffusion=0.1, dt=1e-4,
+                                    density_diffusion_form="antuono")
+    out.update(wall_pos=_np(allp), wall_vel=_np(v), wall_mass=_np(m), wall_pairs=_np(pairs),
+               wall_nf=np.array(fluid.shape[0]), wall_h=np.array(hw),
+               wall_acc=_np(acc), wall_rho=_np(rho), wall_p=_np(p))
+    np.savez_compressed(HERE / "threed.npz", **out)
+
+    # ------------------------------------------------------------------ rhs3d
+    out = {}
+    cfg3 = db3.DamBreak3DConfig(dx=0.05, tank_height=0.4, tank_depth=0.2, column_width=0.2,
+                                column_depth=0.2, column_height=0.25, pool_width=0.4,
+                                viscosity=1e-3, wall_friction=0.3)
+    pos, mass = db3.make_column_3d(cfg3, cpu)
+    wall = db3.wall_lattice_3d(cfg3, cpu)
+    g3 = torch.Generator().manual_seed(13)
+    vel = 0.1 * torch.randn(pos.shape, generator=g3)
+    pos = pos + 0.01 * cfg3.dx * torch.randn(pos.shape, generator=g3)
+    h = cfg3.h_factor * cfg3.dx
+    cache = PairListCache(2.0 * h, 0.25 * h, fluid_count=pos.shape[0])
+    pairs, _ = cache.get(torch.cat((pos, wall)))
+    rho, p, acc, reaction = db3.rhs_3d(pos, vel, mass, cfg3, pairs=pairs, wall=wall,
+                                       return_wall_force=True)
+    out.update(pos=_np(pos), vel=_np(vel), mass=_np(mass), wall=_np(wall), pairs=_np(pairs),
+               rho=_np(rho), p=_np(p), acc=_np(acc), reaction=_np(reaction),
+               params=np.array([cfg3.dx, cfg3.h_factor, cfg3.rho0, cfg3.sound_speed,
+                                cfg3.gamma, cfg3.gravity, cfg3.viscosity, cfg3.wall_friction,
+                                cfg3.dt, db3._mass_3d(cfg3.dx, cfg3.h_factor, cfg3.rho0, True)]))
+    np.savez_compressed(HERE / "rhs3d.npz", **out)
+
+    # -------------------------------------------------------------- deltaplus
+    out = {}
+    DX = 0.01
+    axis = (torch.arange(15, dtype=torch.float64) + 0.5) * DX
+    pos = torch.cartesian_prod(axis, axis)
+    pos[7 * 15 + 7, 0] += 0.23 * DX
+    pos[7 * 15 + 13, 0] -= 0.2 * DX
+    pos[99] += 0.2 * DX
+    g4 = torch.Generator().manual_seed(17)
+    vel = 0.05 * torch.randn(pos.shape, generator=g4, dtype=torch.float64)
+    rho = 1000.0 + 0.5 * torch.randn(pos.shape[0], generator=g4, dtype=torch.float64)
+    mass = rho * DX ** 2
+    # floor and left wall ghost stations with normals (spheric19.py:124-131 style)
+    xs = (torch.arange(-3, 16, dtype=torch.float64) + 0.5) * DX
+    floor = torch.stack((xs, torch.full_like(xs, -0.5 * DX)), 1)
+    ys = (torch.arange(0, 16, dtype=torch.float64) + 0.5) * DX
+    left = torch.stack((torch.full_like(ys, -0.5 * DX), ys), 1)
+    wall = torch.cat((floor, left))
+    wn = torch.cat((torch.tensor([[0.0, 1.0]]).expand(len(floor), 2),
+                    torch.tensor([[1.0, 0.0]]).expand(len(left), 2)))
+    wv = torch.zeros_like(wall)
+    wv[len(floor):, 1] = 0.1
+    params = dict(dx=DX, sound_speed=20.0, reference_speed=0.5, dt=1e-4, rho0=1000.0,
+                  gravity=9.81, viscosity=1e-6, diffusion=0.1, alpha=0.02)
+    pairs = cell_linked_pairs(torch.cat((pos, wall)), 4 * DX * 1.1)
+    step = delta_plus_step(pos, vel, rho, mass, wall=wall, wall_velocity=wv,
+                           wall_normal=wn, pairs=pairs, **params)
+    out.update(pos=_np(pos), vel=_np(vel), rho=_np(rho), mass=_np(mass), wall=_np(wall),
+               wall_vel=_np(wv), wall_normal=_np(wn), pairs=_np(pairs),
+               params=np.array([params[k] for k in ("dx", "sound_speed", "reference_speed",
+                                                    "dt", "rho0", "gravity", "viscosity",
+                                                    "diffusion", "alpha")]),
+               out_pos=_np(step.position), out_vel=_np(step.velocity),
+               out_rho=_np(step.density), out_acc=_np(step.acceleration),
+               out_reaction=_np(step.wall_reaction), out_shift=_np(step.shift))
+    # free patch (no walls): the elliptical-drop initial state at 8 cells/radius
+    cells = 8
+    dxe = 1 / cells
+    ax = (torch.arange(-cells, cells, dtype=torch.float64) + 0.5) * dxe
+    pe = torch.cartesian_prod(ax, ax)
+    r2 = pe.square().sum(1)
+    pe = pe[r2 < 1]
+    de = 1000 * (1 + 0.5 * (1 - r2[r2 < 1]) / 20.0 ** 2)
+    me = de * dxe ** 2
+    ve = pe * pe.new_tensor([1.0, -1.0])
+    pairs_e = cell_linked_pairs(pe, 4 * dxe + 0.4 * dxe)
+    st = delta_plus_step(pe, ve, de, me, dx=dxe, dt=1e-3, sound_speed=20.0,
+                         reference_speed=1.0, gravity=0.0, viscosity=0.0, pairs=pairs_e)
+    out.update(free_pos=_np(pe), free_vel=_np(ve), free_rho=_np(de), free_mass=_np(me),
+               free_pairs=_np(pairs_e), free_out_pos=_np(st.position),
+               free_out_vel=_np(st.velocity), free_out_rho=_np(st.density),
+               free_out_shift=_np(st.shift))
+    np.savez_compressed(HERE / "deltaplus.npz", **out)
+    print("golden fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])
