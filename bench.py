@@ -1,0 +1,193 @@
+"""Benchmark: 2-D WCSPH dam break on one MI355X, FP32-compensated vs FP64.
+
+Workload (the 400k-1.6M-particle dam break the metric is quoted on): a
+1 x 2 m water column at the left of a 4 x 3 m tank with three wall layers on
+a square lattice, h = 1.3 dx, hydrostatic initial masses, fluid at rest,
+c0 = 10 sqrt(2 g H) with H = 2 m, dt at acoustic Courant 0.3, nu = 0.01,
+no density diffusion, Verlet skin 0.25 h.  dx = sqrt(2 / N) with N = 1.6 M
+gives 1,598,472 fluid + 26,847 wall particles.
+
+One step = one full pass of the particle pipeline (density + Tait, Adami wall
+pressure, pressure + viscous force, symplectic Euler + clamp, and the
+amortised neighbour-list rebuilds) over all fluid particles.  The state is
+resident in HBM when the timed region starts.
+
+Prints one JSON line; ``value`` is fluid particle-updates/s of the
+FP32-compensated pipeline.  ``roofline`` prices the force sweep (the
+dominant kernel) at its algorithmic bytes over its HIP-event time;
+``cpu_baseline`` times the FP64 CPU restatement on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--fluid", type=float, default=1.6e6, help="target fluid particle count")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def dam_break_state(n_target: float):
+    from tensorsph_amd.config import DamBreakConfig
+    from tensorsph_amd.cases import make_column, wall_lattice, lattice_mass_unit
+    dx = math.sqrt(2.0 / n_target)
+    c0 = 10.0 * math.sqrt(2.0 * 9.81 * 2.0)
+    h = 1.3 * dx
+    cfg = DamBreakConfig(dx=dx, tank_height=3.0, column_width=1.0, column_height=2.0,
+                         pool_width=4.0, sound_speed=c0, dt=0.3 * h / c0, viscosity=0.01)
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=cfg.tank_top())
+    return cfg, pos, mass, wall, lattice_mass_unit(cfg)
+
+
+def make_sim(cfg, pos, mass, wall, wmass, precision):
+    import numpy as np
+    from tensorsph_amd.engine import WcsphSimulation
+    h = cfg.h_factor * cfg.dx
+    return WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+                           sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+                           viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=wmass,
+                           skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0],
+                           hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx,
+                           precision=precision)
+
+
+def time_gpu(sim, steps, warmup):
+    import torch
+    sim.step(warmup)
+    torch.cuda.synchronize()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    sweep_ms, nsteps = sim.sweep_times()
+    sim.set_timing(False)
+    return wall, sweep_ms, nsteps
+
+
+def cpu_baseline(seconds: float):
+    """FP64 oracle loop (C, one thread) on the same dam break at a 25k-particle sample."""
+    import numpy as np
+    from oracle import oracle as orc
+    cfg, pos, mass, wall, wmass = dam_break_state(2.5e4)
+    h = cfg.h_factor * cfg.dx
+    st = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+                       c0=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+                       nu=cfg.viscosity, dt=cfg.dt, wall_mass=wmass,
+                       skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0],
+                       hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx)
+    st.step()                      # first list build outside the timing
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < seconds:
+        st.step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": pos.shape[0] * steps / dt, "unit": "particle-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"FP64 C restatement, 1 thread, dam break at {pos.shape[0]} fluid + "
+                      f"{wall.shape[0]} wall particles (dx={cfg.dx:.4g}), {steps} steps in "
+                      f"{dt:.1f} s, list rebuilds included"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
+                                   "slab decomposition is a design only, see DESIGN.md)"}))
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+    cfg, pos, mass, wall, wmass = dam_break_state(args.fluid)
+    nf, nw = pos.shape[0], wall.shape[0]
+    h = cfg.h_factor * cfg.dx
+
+    sim = make_sim(cfg, pos, mass, wall, wmass, "fp32")
+    wall_s, sweep_ms, nsteps = time_gpu(sim, args.steps, args.warmup)
+    pairs = sim.count_pairs(2.0 * h)
+    stats = sim.stats()
+    sim.close()
+    del sim
+    value = nf * args.steps / wall_s
+    ms_step = 1000.0 * wall_s / args.steps
+
+    # algorithmic bytes per launch of the force sweep (sweep B): reads x, v (FP64,
+    # 2 x 16 B), m, rho (4 B each), writes x, v (2 x 16 B) per fluid particle,
+    # plus each wall particle's x, m, rho, p read once (16 + 3 x 4 B)
+    s = 4
+    b_force = nf * (32 * 2 + 2 * s) + nw * (16 + 3 * s)
+    b_density = nf * (16 + 2 * s) + nw * (16 + s)
+    force_ms = sweep_ms[2] / max(nsteps, 1)
+    achieved = b_force / (force_ms * 1e-3) / 1e9 if force_ms > 0 else 0.0
+    step_bytes = nf * (40 * 2 + 4 * s) + nw * (24 * 2 + 2 * s)
+
+    result = {
+        "metric": "particle-updates/s (2-D WCSPH dam break, FP32-compensated) + HBM GB/s",
+        "value": value,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: deterministic dam-break lattice (no dataset)",
+        "config": {"workload": f"2-D dam break, {nf} fluid + {nw} wall particles, "
+                               f"dx={cfg.dx:.5g} m, h=1.3dx, c0={cfg.sound_speed:.4g} m/s, "
+                               f"dt={cfg.dt:.4g} s, nu={cfg.viscosity}",
+                   "precision": "pair arithmetic FP32 with Kahan sums; positions, velocities "
+                                "and integration FP64",
+                   "fluid": nf, "wall": nw, "parallelism": "single GPU"},
+        "pair_interactions_per_s": pairs * args.steps / wall_s,
+        "pairs_within_2h": pairs,
+        "list_rebuilds_in_run": stats["rebuilds"],
+        "algorithmic_GBps_step": step_bytes * args.steps / wall_s / 1e9,
+        "sweep_ms_per_step": {"density": sweep_ms[0] / max(nsteps, 1),
+                              "wall": sweep_ms[1] / max(nsteps, 1),
+                              "force_integrate": force_ms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_step_force (force + viscosity + integrate)",
+                     "algorithmic_bytes_per_launch": b_force},
+    }
+    if not args.no_fp64:
+        sim = make_sim(cfg, pos, mass, wall, wmass, "fp64")
+        w64, sw64, n64 = time_gpu(sim, args.steps, args.warmup)
+        sim.close()
+        del sim
+        v64 = nf * args.steps / w64
+        f64_ms = sw64[2] / max(n64, 1)
+        b64 = nf * (32 * 2 + 2 * 8) + nw * (16 + 3 * 8)
+        result["fp64"] = {"value": v64, "ms_per_step": 1000.0 * w64 / args.steps,
+                          "force_sweep_ms": f64_ms,
+                          "force_sweep_GBps": b64 / (f64_ms * 1e-3) / 1e9 if f64_ms > 0 else 0.0}
+        result["fp32c_over_fp64"] = value / v64
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * tsph.h -- C ABI of libtsph, the MI355X (gfx950) particle pipeline of
+ * tensorsph_amd.
+ *
+ * Plain C: status codes, POD parameter structs, raw device pointers, counts
+ * and a hipStream_t.  No C++ types or exceptions cross this boundary.  Every
+ * call is asynchronous on the given stream unless its comment says it
+ * synchronises (the *_search / *_export calls that return a count, and
+ * tsph_sim_step's Verlet-skin decision).
+ *
+ * Layouts.  Coordinates and velocities are FP64 (N, D) row-major, the layout
+ * of an (N, D) torch tensor.  Per-particle scalars (mass, density, pressure)
+ * and accelerations are FP64 in TSPH_F64 mode and FP32 in TSPH_F32 mode
+ * ("FP32 compensated": pair deltas formed in FP64, kernel arithmetic in FP32,
+ * per-particle sums Kahan-compensated).  Particle sets are fluid first, wall
+ * after, as the reference orders them.
+ *
+ * The reference (cxs503/TensorSPH) is pure Python; the functions below
+ * replace the Python call sites cited next to each declaration.  The Python
+ * package tensorsph_amd binds them with ctypes (INTEGRATION.md shows the
+ * binding).
+ */
+#ifndef TSPH_H
+#define TSPH_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSPH_ABI_VERSION 1
+
+/* status codes; the Python layer maps them to the reference's exceptions */
+#define TSPH_OK 0
+#define TSPH_ERR_ARG 1      /* ValueError */
+#define TSPH_ERR_HIP 2      /* RuntimeError (HIP failure) */
+#define TSPH_ERR_OOM 3      /* MemoryError */
+#define TSPH_ERR_NUMERIC 4  /* FloatingPointError */
+#define TSPH_ERR_TYPE 5     /* TypeError */
+
+/* precision / dtype selectors */
+#define TSPH_F64 0
+#define TSPH_F32 1
+
+typedef struct tsph_ctx tsph_ctx;
+
+/* Message of the last failing call on this thread. */
+const char* tsph_last_error(void);
+int tsph_abi_version(void);
+
+/* Context: owns device scratch (cell grid, neighbour lists) that grows on
+ * demand; one per simulation / stream.  Not thread-safe across threads. */
+int tsph_ctx_create(tsph_ctx** out);
+int tsph_ctx_destroy(tsph_ctx* ctx);
+
+/* ---------------------------------------------------------------- pair search
+ * Replaces neighbor_search.cell_linked_pairs (neighbor_search.py:44-120) and,
+ * with box != NULL, threed.neighbors3.periodic_pairs (neighbors3.py:71-142).
+ * pos: device (n, dim) in `dtype` (TSPH_F64 / TSPH_F32; the search runs in that
+ * precision, as the reference does).  box: host array of dim edges or NULL.
+ * fluid_count >= 0 drops pairs with both ends >= fluid_count (the
+ * PairListCache prune, neighbor_search.py:208-217); -1 keeps all.
+ * Synchronises once and returns the number of unordered pairs in *npairs. */
+int tsph_pairs_search(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int dim,
+                      double cutoff, const double* box, int64_t fluid_count,
+                      hipStream_t stream, int64_t* npairs);
+
+/* Writes the pairs of the last search as (npairs, 2) int64, i < j, sorted
+ * lexicographically: the reference's output format. */
+int tsph_pairs_export(tsph_ctx* ctx, int64_t* pairs, hipStream_t stream);
+
+/* Cell bookkeeping of the last search (neighbor_search.py:65-79): cells
+ * (n, dim) int64 normalised by the per-axis minimum, row-major keys (n),
+ * stable cell order (n) = argsort(keys, stable).  Any pointer may be NULL.
+ * *ncells receives the number of occupied cells (host). */
+int tsph_cells_export(tsph_ctx* ctx, int64_t* cells, int64_t* keys, int64_t* order,
+                      int64_t* ncells, hipStream_t stream);
+
+/* max_i |pos_i - anchor_i| (PairListCache.displacement, neighbor_search.py:
+ * 203-206), written to *out (device scalar of `dtype`). */
+int tsph_max_displacement(const void* pos, const void* anchor, int dtype, int64_t n,
+                          int dim, void* out, hipStream_t stream);
+
+/* ---------------------------------------------------------------- operators
+ * Neighbour lists for the operators below, from a reference pair list
+ * (P, 2) int64 (any order, i != j).  Targets are the fluid particles
+ * [0, n_fluid) (all their neighbours) and the wall particles
+ * [n_fluid, n_total) (their fluid neighbours).  Each list is sorted by
+ * neighbour index, so every gather below is deterministic. */
+int tsph_nlist_from_pairs(tsph_ctx* ctx, const int64_t* pairs, int64_t npairs,
+                          int64_t n_fluid, int64_t n_total, int all_targets,
+                          hipStream_t stream);
+
+typedef struct tsph_wcsph_params {
+  int32_t dim;               /* 2 or 3 */
+  int32_t precision;         /* TSPH_F64 or TSPH_F32 */
+  int64_t n_fluid, n_wall;
+  double h, rho0, sound_speed, gamma, gravity, viscosity, wall_friction, dt;
+  double diffusion_scale;    /* 2 delta c0 h; 0 disables density diffusion */
+  int32_t diffusion_form;    /* 0 Molteni-Colagrossi, 1 Antuono (renormalised) */
+  double eigenvalue_floor;   /* Antuono eigenvalue floor (fraction of trace/D) */
+  int32_t reaction_style;    /* 0 none; 1 2-D split loads; 2 rhs_3d wall force */
+  int32_t has_wall_accel;
+} tsph_wcsph_params;
+
+/* Replaces droplet_impact.rhs (droplet_impact.py:810-1228; dim 2) and
+ * dam_break_3d.rhs_3d (dam_break_3d.py:290-422; dim 3) on the lists of the
+ * last tsph_nlist_from_pairs call.  pos/vel: (n_fluid + n_wall, dim) FP64
+ * (wall rows of vel = wall velocity).  mass: (n_fluid + n_wall) scalar.
+ * wall_accel: (n_wall, dim) FP64 or NULL.  Outputs (scalar type): rho, p
+ * (n_fluid), acc (n_fluid, dim), wall_p, wall_rho (n_wall, may be NULL),
+ * reaction (n_wall, dim; NULL unless reaction_style != 0). */
+int tsph_wcsph_rhs(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* pos,
+                   const double* vel, const void* mass, const double* wall_accel,
+                   void* rho, void* p, void* acc, void* wall_p, void* wall_rho,
+                   void* reaction, hipStream_t stream);
+
+/* Tait EOS p = max(0, B((max(rho/rho0, ratio_floor))^gamma - 1)), B = rho0 c0^2/gamma:
+ * droplet_impact.tait_pressure (droplet_impact.py:415-427; ratio_floor 0.01)
+ * and threed.operators3.tait_pressure (operators3.py:82-86; ratio_floor 0). */
+int tsph_eos_tait(const void* rho, int64_t n, int precision, double rho0, double sound_speed,
+                  double gamma, double ratio_floor, void* p, hipStream_t stream);
+
+/* Kernel values for radii r: kind 0 = 2-D cubic spline, 1 = 3-D cubic spline
+ * (g and the radial Laplacian lap), 2 = 2-D Wendland C2.  g is the gradient
+ * scale (grad_i W = delta * g).  Replaces cubic_spline_kernel/_gradient
+ * (droplet_impact.py:276-304), wendland_c2_* (:320-346) and
+ * threed.kernel3.pair_kernels (kernel3.py:167-181).  g, lap may be NULL. */
+int tsph_kernel_eval(int kind, const void* r, int64_t n, int precision, double h, void* w,
+                     void* g, void* lap, hipStream_t stream);
+
+/* Safety clamp of apply_boundary (droplet_impact.py:1293-1312, 3-D:
+ * dam_break_3d.py:429-452), in place on FP64 (n, dim) pos/vel: positions
+ * clamped to [lo, hi], the velocity component zeroed where a particle was
+ * beyond the box by more than tol.  *count (device uint64) receives the
+ * number of particles found beyond the tolerance. */
+int tsph_apply_boundary(double* pos, double* vel, int64_t n, int dim, const double* lo,
+                        const double* hi, double tol, unsigned long long* count,
+                        hipStream_t stream);
+
+/* ---------------------------------------------------------------- resident stepper
+ * The step loop of droplet_impact.run (droplet_impact.py:1674-1763) and
+ * dam_break.run_case (dam_break.py:374-465): Verlet-skin pair cache
+ * (neighbor_search.py:166-227), rhs, symplectic Euler (:1745-1756) and the
+ * apply_boundary clamp (:1293-1312), with the state resident in HBM.  The
+ * caller's particle order is restored by tsph_sim_read. */
+typedef struct tsph_sim tsph_sim;
+
+typedef struct tsph_sim_params {
+  tsph_wcsph_params op;      /* reaction_style and has_wall_accel must be 0 */
+  double skin;               /* Verlet skin (absolute length, > 0) */
+  double lo[3], hi[3], tol;  /* clamp box and crossing tolerance */
+  int32_t clamp;             /* 1: apply the boundary clamp after every step */
+} tsph_sim_params;
+
+/* pos: (n_fluid + n_wall, dim) FP64 device, fluid then wall; vel: (n_fluid,
+ * dim) FP64; mass: (n_fluid + n_wall) in the scalar type. */
+int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double* vel,
+                    const void* mass, hipStream_t stream, tsph_sim** out);
+/* Advances nsteps steps.  meter_every > 0 records a meter row on the steps
+ * whose index is a multiple of it.  Returns when the last step has finished. */
+int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stream);
+/* State in the caller's order: pos, vel (n_fluid, dim) FP64; rho, p (n_fluid)
+ * and acc (n_fluid, dim) of the last step.  Any pointer may be NULL. */
+int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc,
+                  hipStream_t stream);
+/* counters: steps done, list builds, boundary crossings, list entries, meter
+ * rows; peaks: max density and max speed over all steps.  Host pointers. */
+int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
+/* Meter rows (host): step, sums[mass, momentum x/y/z, kinetic energy, sum rho,
+ * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, peak rho,
+ * peak speed], minima[rho, p, y]: 20 doubles per row. */
+int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
+/* Sweep timing with HIP events on the stepping stream.  Returns the summed
+ * milliseconds of the density, wall and force+integrate sweeps and the number
+ * of timed steps since the last reset; enable = 1/0 resets the sums and turns
+ * timing on/off, enable = -1 only reads. */
+int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps);
+/* Number of unordered pairs with separation <= radius in the current state
+ * (radius <= the list's build radius); synchronises. */
+int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream);
+int tsph_sim_destroy(tsph_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSPH_H */

@@ -1,0 +1,174 @@
+// Shared device helpers for the tensorsph_amd particle pipeline (gfx950).
+//
+// Conventions used by every kernel:
+//   * particle coordinates and velocities are FP64, array-of-structs (N, D),
+//     the layout of the (N, D) tensors the Python API hands over;
+//   * per-particle scalars (mass, density, pressure) and pair arithmetic use
+//     the precision type S: double ("plain FP64") or float ("FP32
+//     compensated": pair deltas formed in FP64 then rounded, kernel math in
+//     FP32, sums Kahan-compensated);
+//   * every per-particle sum is a gather over that particle's own neighbour
+//     list in a fixed order, so results never depend on thread scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TSPH_HD __host__ __device__ __forceinline__
+#define TSPH_D __device__ __forceinline__
+
+namespace tsph {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- accumulators
+template <typename S> struct Sum;
+template <> struct Sum<double> {
+  double s = 0.0;
+  TSPH_D void add(double x) { s += x; }
+  TSPH_D double get() const { return s; }
+};
+// Kahan summation: the running compensation c carries the low-order bits that
+// s + x drops.  Built without fast-math, so the compiler keeps the order.
+template <> struct Sum<float> {
+  float s = 0.0f, c = 0.0f;
+  TSPH_D void add(float x) {
+    float y = x - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  TSPH_D float get() const { return s; }
+};
+
+template <int D, typename S> struct VSum {
+  Sum<S> v[D];
+  TSPH_D void add(const S* x) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) v[d].add(x[d]);
+  }
+};
+
+// ---------------------------------------------------------------- kernels
+// Cubic spline (Monaghan 1992), q = r/h, support 2h.
+//   2-D sigma = 10/(7 pi h^2)   (droplet_impact.py:276-304)
+//   3-D sigma = 1/(pi h^3)      (threed/kernel3.py:48-123)
+// Returned: W, the gradient scale g with grad_i W = delta * g, and (3-D) the
+// radial Laplacian f'' + 2 f'/q with its q -> 0 limit -6.
+template <typename S> struct KernelConst {
+  S h, inv_h, sigma, sigma_over_h, w0;
+};
+
+template <typename S> TSPH_D void cubic_wg(S r, const KernelConst<S>& k, S& w, S& g) {
+  const S q = r * k.inv_h;
+  S shape, dshape;
+  if (q < S(1)) {
+    shape = S(1) - S(1.5) * (q * q) + S(0.75) * (q * q * q);
+    dshape = S(-3) * q + S(2.25) * (q * q);
+  } else if (q < S(2)) {
+    const S t = S(2) - q;
+    shape = S(0.25) * (t * t * t);
+    dshape = S(-0.75) * (t * t);
+  } else {
+    shape = S(0);
+    dshape = S(0);
+  }
+  w = k.sigma * (shape > S(0) ? shape : S(0));
+  const S rc = r > S(1e-12) ? r : S(1e-12);
+  g = k.sigma_over_h * dshape / rc;
+}
+
+template <typename S> TSPH_D void cubic_wgl(S r, const KernelConst<S>& k, S& w, S& g, S& lap) {
+  const S q = r * k.inv_h;
+  S shape, fp, fpp;
+  if (q < S(1)) {
+    shape = S(1) - S(1.5) * (q * q) + S(0.75) * (q * q * q);
+    fp = S(-3) * q + S(2.25) * (q * q);
+    fpp = S(-3) + S(4.5) * q;
+  } else if (q < S(2)) {
+    const S t = S(2) - q;
+    shape = S(0.25) * (t * t * t);
+    fp = S(-0.75) * (t * t);
+    fpp = S(1.5) * t;
+  } else {
+    shape = S(0); fp = S(0); fpp = S(0);
+  }
+  w = k.sigma * (shape > S(0) ? shape : S(0));
+  const S rc = r > S(1e-12) ? r : S(1e-12);
+  g = k.sigma_over_h * fp / rc;
+  const S qc = q > S(1e-12) ? q : S(1e-12);
+  const S twice = q > S(1e-10) ? S(2) * fp / qc : S(-6);
+  lap = q < S(2) ? k.sigma * (fpp + twice) * (k.inv_h * k.inv_h) : S(0);
+}
+
+// Wendland C2, 2-D sigma = 7/(4 pi h^2) (droplet_impact.py:320-346).
+template <typename S> TSPH_D void wendland_wg(S r, const KernelConst<S>& k, S& w, S& g) {
+  const S q = r * k.inv_h;
+  S t = S(1) - S(0.5) * q;
+  t = t > S(0) ? t : S(0);
+  const S t2 = t * t;
+  w = k.sigma * (t2 * t2) * (S(2) * q + S(1));
+  const S rc = r > S(1e-12) ? r : S(1e-12);
+  g = k.sigma_over_h * (S(-5) * q * (t2 * t)) / rc;
+}
+
+template <typename S>
+inline KernelConst<S> make_cubic(double h, int dim) {
+  KernelConst<S> k;
+  double sigma = dim == 3 ? 1.0 / (kPi * h * h * h) : 10.0 / (7.0 * kPi * h * h);
+  k.h = S(h); k.inv_h = S(1.0 / h); k.sigma = S(sigma); k.sigma_over_h = S(sigma / h);
+  k.w0 = S(sigma);
+  return k;
+}
+
+template <typename S>
+inline KernelConst<S> make_wendland2d(double h) {
+  KernelConst<S> k;
+  double sigma = 7.0 / (4.0 * kPi * h * h);
+  k.h = S(h); k.inv_h = S(1.0 / h); k.sigma = S(sigma); k.sigma_over_h = S(sigma / h);
+  k.w0 = S(sigma);
+  return k;
+}
+
+// ---------------------------------------------------------------- small math
+template <typename S> TSPH_D S sqrt_s(S x);
+template <> TSPH_D double sqrt_s<double>(double x) { return sqrt(x); }
+template <> TSPH_D float sqrt_s<float>(float x) { return sqrtf(x); }
+template <typename S> TSPH_D S pow_s(S x, S y);
+template <> TSPH_D double pow_s<double>(double x, double y) { return pow(x, y); }
+template <> TSPH_D float pow_s<float>(float x, float y) { return powf(x, y); }
+
+// Tait EOS of droplet_impact.tait_pressure (:415-427): ratio floored at 0.01,
+// pressure clamped at zero.  ``ratio_floor`` < 0 gives the 3-D form
+// (threed/operators3.py:82-86) without the floor.
+template <typename S>
+TSPH_D S tait(S rho, S rho0, S bulk, S gamma, S ratio_floor) {
+  S ratio = rho / rho0;
+  if (ratio_floor > S(0) && ratio < ratio_floor) ratio = ratio_floor;
+  S p = bulk * (pow_s<S>(ratio, gamma) - S(1));
+  return p > S(0) ? p : S(0);
+}
+
+// ---------------------------------------------------------------- launch helpers
+constexpr int kBlock = 256;
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  return (unsigned)(g > 0 ? g : 1);
+}
+
+// Chunked block -> work mapping for the 8 XCDs of MI355X.  Consecutive
+// workgroup ids are dispatched round-robin over the XCDs; remapping so XCD x
+// gets the x-th contiguous chunk of blocks keeps spatially adjacent particles
+// (and so their shared neighbours) in one XCD's L2.  The grid is padded to a
+// multiple of 8 (grid_xcd), which makes the map a bijection; kernels bound-
+// check the particle index as usual.
+TSPH_D int64_t xcd_block(int64_t bid, int64_t nblocks_padded) {
+  const int64_t per = nblocks_padded >> 3;
+  return (bid & 7) * per + (bid >> 3);
+}
+inline unsigned grid_xcd(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  g = ((g + 7) / 8) * 8;
+  return (unsigned)(g > 0 ? g : 8);
+}
+
+}  // namespace tsph

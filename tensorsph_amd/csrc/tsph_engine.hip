@@ -1,0 +1,960 @@
+// Resident WCSPH stepper: the per-step loop of droplet_impact.run
+// (droplet_impact.py:1674-1763), dam_break.run_case (dam_break.py:374-465)
+// and, in 3-D, kleefsman_obstacle_3d.run_case (:644-706), with the particle
+// state kept in HBM between steps.
+//
+// Layout.  Fluid rows [0, n) then wall rows [n, n + nw), positions and
+// velocities FP64 (nt, D), ping-ponged between two buffers.  At every
+// neighbour-list build the rows are re-ordered by cell key (fluid among
+// fluid, wall among wall), so a particle's neighbours sit close to it in
+// memory; orig[] maps every row back to the caller's index.
+//
+// Verlet skin without a per-step host round trip.  The list is built at
+// 2h + skin and stays valid while no particle has moved more than skin/2
+// since the build (neighbor_search.py:219-227).  Step k's force sweep
+// raises a device flag when its new positions break that bound; every sweep
+// of step k+1 reads the flag first and, when set, does nothing (the force
+// sweep copies the state through).  The host learns of the flag one step
+// later from a pinned-memory copy, rebuilds, and re-runs the skipped steps,
+// so the schedule of rebuilds is exactly the reference's while the host
+// never blocks the GPU queue.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <string.h>
+#include <vector>
+
+#include "tsph_cells.hpp"
+#include "tsph_common.hpp"
+#include "tsph_internal.hpp"
+#include "tsph_wcsph.hpp"
+
+namespace tsph {
+
+// device flag block
+struct SimFlags {
+  unsigned long long maxes[MM_NMAX];
+  unsigned long long escaped;
+  unsigned long long stale[2];      // slot k&1 = input of step k, slot (k+1)&1 = output
+  unsigned long long mins[MN_NMIN]; // ordered keys, atomicMin
+};
+
+// one entry of the host ring, packed on the device after every step and
+// copied to pinned memory in one transfer
+struct StepRecord {
+  unsigned long long stale_in, stale_out, escaped;
+  unsigned long long maxes[MM_NMAX];
+  unsigned long long mins[MN_NMIN];
+  double sums[MS_NSUM];
+};
+
+__global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __restrict__ sums, int slot_in,
+                              int with_sums, StepRecord* __restrict__ rec) {
+  const int t = threadIdx.x;
+  if (t == 0) { rec->stale_in = fl->stale[slot_in]; rec->stale_out = fl->stale[slot_in ^ 1]; rec->escaped = fl->escaped; }
+  if (t < MM_NMAX) rec->maxes[t] = fl->maxes[t];
+  if (t < MN_NMIN) rec->mins[t] = fl->mins[t];
+  if (t < MS_NSUM) rec->sums[t] = with_sums ? sums[t] : 0.0;
+}
+
+constexpr int kRing = 4;
+
+// ------------------------------------------------------------------ step sweeps
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                         const int32_t* __restrict__ idx, SimFlags* fl,
+                                                         int slot_in, int write_eos) {
+  if (fl->stale[slot_in]) return;
+  const int64_t nb = gridDim.x;
+  const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) fl->maxes[MM_DISP] = 0;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  Sum<S> acc;
+  acc.add(a.mass[i] * a.k.w0);
+  const int64_t e1 = off[i + 1];
+  for (int64_t e = off[i]; e < e1; ++e) {
+    const int64_t j = idx[e];
+    S r2 = S(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const S df = (S)(xi[d] - a.pos[j * D + d]);
+      r2 += df * df;
+    }
+    if (r2 > a.skip2) continue;
+    S w, g;
+    cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+    acc.add(a.mass[j] * w);
+  }
+  S rho = acc.get();
+  rho = rho > a.rho_floor ? rho : a.rho_floor;
+  a.rho[i] = rho;
+  if (write_eos) {
+    const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+    a.p[i] = p;
+    a.pr2[i] = p / (rho * rho);
+    a.vol[i] = a.mass[i] / rho;
+  }
+}
+
+template <int D, typename S, int FORM>
+__global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                           const int32_t* __restrict__ idx,
+                                                           const S* __restrict__ rho_in,
+                                                           const S* __restrict__ grad, SimFlags* fl,
+                                                           int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  const S rhoi = rho_in[i];
+  S gi[D];
+  if (FORM == 1) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) gi[d] = grad[i * D + d];
+  }
+  Sum<S> rate;
+  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+    const int64_t j = idx[e];
+    if (j >= a.n) continue;
+    S df[D], r2 = S(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+    if (r2 > a.skip2) continue;
+    S w, g;
+    cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+    S radial = S(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) radial += df[d] * (df[d] * g);
+    radial = -radial;
+    S contrast = rhoi - rho_in[j];
+    if (FORM == 1) {
+      S proj = S(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) proj += (gi[d] + grad[j * D + d]) * df[d];
+      contrast = contrast - S(0.5) * proj;
+    }
+    rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * (contrast * radial / (r2 + S(1e-12))));
+  }
+  S rho = rhoi + a.dt * rate.get();
+  rho = rho > a.rho_floor ? rho : a.rho_floor;
+  const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+  a.rho[i] = rho;
+  a.p[i] = p;
+  a.pr2[i] = p / (rho * rho);
+  a.vol[i] = a.mass[i] / rho;
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                          const int32_t* __restrict__ idx,
+                                                          S* __restrict__ grad_out, SimFlags* fl,
+                                                          int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  const S rhoi = a.rho[i];
+  Sum<S> G[D], M[D * D];
+  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+    const int64_t j = idx[e];
+    if (j >= a.n) continue;
+    S df[D], r2 = S(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+    if (r2 > a.skip2) continue;
+    S w, g;
+    cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+    const S vj = a.mass[j] / a.rho[j];
+    const S diff = a.rho[j] - rhoi;
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      G[r].add((vj * diff) * (df[r] * g));
+#pragma unroll
+      for (int c = 0; c < D; ++c) M[r * D + c].add(vj * (-(df[r] * (df[c] * g))));
+    }
+  }
+  double m[9], gv[3], out[3];
+  double tr = 0.0;
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    gv[r] = (double)G[r].get();
+#pragma unroll
+    for (int c = 0; c < D; ++c) m[r * D + c] = (double)M[r * D + c].get();
+    tr += m[r * D + r];
+  }
+  if (tr <= 0.0) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) grad_out[i * D + d] = S(0);
+    return;
+  }
+  regularised_solve<D>(m, gv, (double)a.eig_floor, out);
+#pragma unroll
+  for (int d = 0; d < D; ++d) grad_out[i * D + d] = (S)out[d];
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, const int64_t* __restrict__ woff,
+                                                      const int32_t* __restrict__ widx, SimFlags* fl,
+                                                      int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= a.nw) return;
+  const int64_t iw = a.n + w;
+  double xw[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xw[d] = a.pos[iw * D + d];
+  Sum<S> num, den;
+  for (int64_t e = woff[w]; e < woff[w + 1]; ++e) {
+    const int64_t f = widx[e];
+    S rw[D], r2 = S(0);
+#pragma unroll
+    for (int d = 0; d < D; ++d) { rw[d] = (S)(xw[d] - a.pos[f * D + d]); r2 += rw[d] * rw[d]; }
+    if (r2 > a.skip2) continue;
+    S wgt, g;
+    cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
+    S src = a.p[f];
+    if (a.gravity != S(0)) src = src + a.rho[f] * (-a.gravity * rw[1]);
+    num.add(src * wgt);
+    den.add(wgt);
+  }
+  const S dd = den.get() > S(1e-30) ? den.get() : S(1e-30);
+  S pw = num.get() / dd;
+  pw = pw > S(0) ? pw : S(0);
+  S rwv = a.rho0 * pow_s<S>(pw / a.bulk + S(1), a.inv_gamma);
+  rwv = rwv > a.rho0 ? rwv : a.rho0;
+  a.p[iw] = pw;
+  a.rho[iw] = rwv;
+  a.pr2[iw] = pw / (rwv * rwv);
+  a.vol[iw] = a.mass[iw] / rwv;
+}
+
+template <typename T> TSPH_D T wave_max(T v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+template <typename T> TSPH_D T wave_min(T v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+TSPH_D double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int D, typename S, bool METERS>
+__global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                       const int32_t* __restrict__ idx, S* __restrict__ acc_out,
+                                                       StepArgs st, SimFlags* fl, int slot_in, double half_skin) {
+  const int64_t nb = gridDim.x;
+  const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+  if (fl->stale[slot_in]) {
+    // list no longer valid: carry the state through unchanged, keep the flag up
+    if (i < a.n) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        st.pos_out[i * D + d] = a.pos[i * D + d];
+        st.vel_out[i * D + d] = a.vel[i * D + d];
+      }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) fl->stale[slot_in ^ 1] = 1;
+    return;
+  }
+  double xi[D] = {}, vi[D] = {};
+  S acc[D];
+  double mi = 0.0, rhoi = 0.0, pi_ = 0.0;
+  const bool live = i < a.n;
+  if (live) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
+    const S pri = a.pr2[i];
+    Sum<S> ap[D], av[D];
+    const int64_t e1 = off[i + 1];
+    for (int64_t e = off[i]; e < e1; ++e) {
+      const int64_t j = idx[e];
+      S df[D], r2 = S(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+      if (r2 > a.skip2) continue;
+      S w, g;
+      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+      const S cp = -a.mass[j] * (pri + a.pr2[j]);
+#pragma unroll
+      for (int d = 0; d < D; ++d) ap[d].add(cp * (df[d] * g));
+      const S blend = j < a.n ? S(1) : a.friction;
+      if (blend != S(0)) {
+        S dg = S(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
+        const S relu = -dg > S(0) ? -dg : S(0);
+        const S cv = -(a.nu2 * blend) * (a.vol[j] * (relu / (r2 + a.eta2)));
+#pragma unroll
+        for (int d = 0; d < D; ++d) av[d].add(cv * (S)(vi[d] - a.vel[j * D + d]));
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
+    if (acc_out) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc_out[i * D + d] = acc[d];
+    }
+    mi = (double)a.mass[i]; rhoi = (double)a.rho[i]; pi_ = (double)a.p[i];
+  }
+  // ---- integrate + clamp (droplet_impact.py:1745-1756, :1293-1312)
+  double disp = 0.0, spd2 = 0.0;
+  unsigned long long hits = 0;
+  if (live) {
+    double xn[D], vn[D];
+    int hit_any = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      vn[d] = vi[d] + st.dt * (double)acc[d];
+      double x = xi[d] + st.dt * vn[d];
+      const int hit = (x < st.lo[d] - st.tol) || (x > st.hi[d] + st.tol);
+      hit_any |= hit;
+      x = x < st.lo[d] ? st.lo[d] : (x > st.hi[d] ? st.hi[d] : x);
+      if (hit) vn[d] = 0.0;
+      xn[d] = x;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) { st.pos_out[i * D + d] = xn[d]; st.vel_out[i * D + d] = vn[d]; }
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) { const double df = xn[d] - st.anchor[i * D + d]; d2 += df * df; }
+    disp = sqrt(d2);
+#pragma unroll
+    for (int d = 0; d < D; ++d) spd2 += vi[d] * vi[d];
+    hits = hit_any;
+  }
+  // ---- per-step maxima (peak compression, peak speed, displacement)
+  const double wdisp = wave_max(disp);
+  const double wspd = sqrt(wave_max(spd2));
+  const double wrho = wave_max(live ? rhoi : -1e300);
+  unsigned long long whits = hits;
+  for (int o = 32; o > 0; o >>= 1) whits += __shfl_xor(whits, o, 64);
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&fl->maxes[MM_DISP], ord_enc(wdisp));
+    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(wspd));
+    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(wrho));
+    if (METERS) {
+      atomicMax(&fl->maxes[MM_SPEED], ord_enc(wspd));
+      atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(wrho));
+    }
+    if (whits) atomicAdd(&fl->escaped, whits);
+    if (wdisp > half_skin) fl->stale[slot_in ^ 1] = 1;
+  }
+  if (METERS) {
+    // sums of the pre-integration state (the rows of droplet_impact.py:1709-1733)
+    double accn2 = 0.0;
+    if (live) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) accn2 += (double)acc[d] * (double)acc[d];
+    }
+    double s[MS_NSUM];
+    s[MS_MASS] = mi;
+    s[MS_MX] = mi * vi[0];
+    s[MS_MY] = D > 1 ? mi * vi[1 % D] : 0.0;
+    s[MS_MZ] = D > 2 ? mi * vi[2 % D] : 0.0;
+    s[MS_KE] = live ? 0.5 * (mi * spd2) : 0.0;
+    s[MS_RHO] = live ? rhoi : 0.0;
+    s[MS_PE] = live ? mi * st.gravity * xi[1 % D] : 0.0;
+    __shared__ double red[MS_NSUM][kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < MS_NSUM; ++k) {
+      const double v = wave_sum(s[k]);
+      if (lane == 0) red[k][wave] = v;
+    }
+    const double big = 1e300;
+    const double wpmax = wave_max(live ? pi_ : -big), wpmin = wave_min(live ? pi_ : big);
+    const double wrmin = wave_min(live ? rhoi : big), wacc = wave_max(sqrt(accn2));
+    const double wymin = wave_min(live ? xi[1 % D] : big), wymax = wave_max(live ? xi[1 % D] : -big);
+    const double wxmax = wave_max(live ? xi[0] : -big);
+    if (lane == 0) {
+      atomicMax(&fl->maxes[MM_P_MAX], ord_enc(wpmax));
+      atomicMax(&fl->maxes[MM_ACC], ord_enc(wacc));
+      atomicMax(&fl->maxes[MM_Y_MAX], ord_enc(wymax));
+      atomicMax(&fl->maxes[MM_X_MAX], ord_enc(wxmax));
+      atomicMin(&fl->mins[MN_RHO_MIN], ord_enc(wrmin));
+      atomicMin(&fl->mins[MN_P_MIN], ord_enc(wpmin));
+      atomicMin(&fl->mins[MN_Y_MIN], ord_enc(wymin));
+    }
+    __syncthreads();
+    if (threadIdx.x < MS_NSUM) {
+      double v = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
+      st.block_sums[(int64_t)blockIdx.x * MS_NSUM + threadIdx.x] = v;
+    }
+  }
+}
+
+// fixed-order reduction of the per-block partial sums
+__global__ void k_reduce_sums(const double* __restrict__ part, int64_t nblocks, double* __restrict__ out,
+                              SimFlags* fl, int slot_in) {
+  if (fl->stale[slot_in]) return;
+  __shared__ double red[MS_NSUM][256];
+  for (int k = 0; k < MS_NSUM; ++k) {
+    double v = 0.0;
+    for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) v += part[b * MS_NSUM + k];
+    red[k][threadIdx.x] = v;
+  }
+  __syncthreads();
+  for (int stride = blockDim.x / 2; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+      for (int k = 0; k < MS_NSUM; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + stride];
+    __syncthreads();
+  }
+  if (threadIdx.x < MS_NSUM) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_reset_meters(SimFlags* fl, int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int t = threadIdx.x;
+  if (t < MM_NMAX && t != MM_DISP && t != MM_PEAK_SPEED && t != MM_PEAK_RHO) fl->maxes[t] = 0;
+  if (t < MN_NMIN) fl->mins[t] = ~0ull;
+}
+
+// ------------------------------------------------------------------ rebuild helpers
+__global__ void k_rank_flags(const int64_t* __restrict__ order, int64_t nt, int64_t n,
+                             unsigned long long* __restrict__ fflag) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nt) fflag[s] = order[s] < n ? 1ull : 0ull;
+}
+
+// new row of sorted slot s: fluid rank, or n + wall rank
+__global__ void k_new_index(const int64_t* __restrict__ order, const int64_t* __restrict__ fscan,
+                            int64_t nt, int64_t n, int64_t* __restrict__ old2new,
+                            int64_t* __restrict__ new2old) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nt) return;
+  const int64_t old = order[s];
+  const int64_t fr = fscan[s];                 // fluid rows before slot s
+  const int64_t nw_new = old < n ? fr : n + (s - fr);
+  old2new[old] = nw_new;
+  new2old[nw_new] = old;
+}
+
+template <int D, typename S>
+__global__ void k_permute(const int64_t* __restrict__ new2old, int64_t nt, const double* __restrict__ pos,
+                          const double* __restrict__ vel, const S* __restrict__ mass,
+                          const int64_t* __restrict__ orig, const int64_t* __restrict__ cells_old,
+                          double* __restrict__ pos2, double* __restrict__ vel2, S* __restrict__ mass2,
+                          int64_t* __restrict__ orig2, int64_t* __restrict__ cells_new) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  const int64_t o = new2old[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    pos2[k * D + d] = pos[o * D + d];
+    vel2[k * D + d] = vel[o * D + d];
+    cells_new[k * D + d] = cells_old[o * D + d];
+  }
+  mass2[k] = mass[o];
+  orig2[k] = orig[o];
+}
+
+__global__ void k_remap_order(int64_t* __restrict__ order, const int64_t* __restrict__ old2new, int64_t nt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nt) order[s] = old2new[order[s]];
+}
+
+// MODE 0 count / 1 fill.  Fluid targets list every neighbour (fluid or wall)
+// within the build radius; wall targets list their fluid neighbours.
+template <int D, int MODE>
+__global__ void k_build_lists(const double* __restrict__ pos, int64_t n, int64_t nt,
+                              const int64_t* __restrict__ cells, GridDesc g,
+                              const int64_t* __restrict__ ukeys, const int64_t* __restrict__ ustart,
+                              const int64_t* __restrict__ order, int64_t ncells, double c2,
+                              unsigned long long* __restrict__ cnt, const int64_t* __restrict__ off,
+                              int32_t* __restrict__ idx, unsigned long long* __restrict__ wcnt,
+                              const int64_t* __restrict__ woff, int32_t* __restrict__ widx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  const double box[3] = {0, 0, 0};
+  int64_t k = 0;
+  const bool fluid = i < n;
+  int64_t base = 0;
+  if (MODE == 1) base = fluid ? off[i] : woff[i - n];
+  for_neighbours<double, D, false>(pos, i, cells, g, ukeys, ustart, order, ncells, c2, box,
+                                   [&](int64_t j) {
+                                     if (!fluid && j >= n) return;
+                                     if (MODE == 1) {
+                                       if (fluid) idx[base + k] = (int32_t)j;
+                                       else widx[base + k] = (int32_t)j;
+                                     }
+                                     ++k;
+                                   });
+  if (MODE == 0) {
+    if (fluid) cnt[i] = k; else wcnt[i - n] = k;
+  }
+}
+
+// ------------------------------------------------------------------ scatter back
+template <int D, typename S>
+__global__ void k_unpermute(const int64_t* __restrict__ orig, int64_t n, const double* __restrict__ pos,
+                            const double* __restrict__ vel, const S* __restrict__ rho,
+                            const S* __restrict__ p, const S* __restrict__ acc, double* __restrict__ pos_o,
+                            double* __restrict__ vel_o, S* __restrict__ rho_o, S* __restrict__ p_o,
+                            S* __restrict__ acc_o) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t o = orig[k];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (pos_o) pos_o[o * D + d] = pos[k * D + d];
+    if (vel_o) vel_o[o * D + d] = vel[k * D + d];
+    if (acc_o) acc_o[o * D + d] = acc[k * D + d];
+  }
+  if (rho_o) rho_o[o] = rho[k];
+  if (p_o) p_o[o] = p[k];
+}
+
+// unordered pairs with |x_i - x_j| <= radius: fluid-fluid pairs are listed
+// from both ends (counted at the lower row), fluid-wall pairs once
+template <int D>
+__global__ void k_count_close(const double* __restrict__ pos, int64_t n, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ idx, double r2max, unsigned long long* cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (i < n) {
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+      const int64_t j = idx[e];
+      if (j < n && j < i) continue;
+      double r2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) { const double df = pos[i * D + d] - pos[j * D + d]; r2 += df * df; }
+      if (r2 <= r2max) ++c;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
+}
+
+__global__ void k_iota(int64_t* __restrict__ v, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+}  // namespace tsph
+
+using namespace tsph;
+
+struct tsph_sim {
+  tsph_sim_params prm{};
+  int dim = 2, prec = 0;
+  int64_t n = 0, nw = 0, nt = 0;
+  tsph_ctx* ctx = nullptr;
+  DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
+  DevBuf rho, p, pr2, vol, rho_raw, grad, acc, anchor;
+  DevBuf flags, block_sums, sums_out, rec_dev;
+  DevBuf old2new, new2old, fscan, cells_new;
+  StepRecord* ring = nullptr;       // pinned
+  hipEvent_t ev[kRing] = {};
+  // per-sweep timing (HIP events around each sweep on the stepping stream)
+  int timing = 0;
+  hipEvent_t tev[kRing][4] = {};
+  double sweep_ms[3] = {0, 0, 0};
+  int64_t sweep_count = 0;
+  int64_t pairs_2h = -1;
+  int64_t ring_step[kRing];
+  int cur = 0;
+  int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
+  bool built = false;
+  // host-side meter log
+  std::vector<double> rows;         // per recorded step: step, sums..., maxes..., mins...
+  double peak_rho = 0.0, peak_speed = 0.0;
+  size_t elt = 8;
+};
+
+namespace tsph {
+
+template <int D, typename S>
+int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
+  tsph_ctx* ctx = sim->ctx;
+  const int64_t n = sim->n, nt = sim->nt;
+  const double rc = 2.0 * sim->prm.op.h + sim->prm.skin;
+  const double* pos = sim->pos[sim->cur].as<double>();
+  TSPH_TRY((run_search<double, D>(ctx, pos, nt, rc, nullptr, stream)));
+  CellSearch& cs = ctx->search;
+  const unsigned g = grid_for(nt);
+  // new row of every particle: fluid by cell order, walls by cell order
+  k_rank_flags<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), nt, n, sim->fscan.as<unsigned long long>());
+  TSPH_CHECK_LAUNCH();
+  {
+    size_t bytes = 0;
+    unsigned long long* f = sim->fscan.as<unsigned long long>();
+    TSPH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, f, f, (int)nt, stream));
+    TSPH_TRY(ctx->scan_tmp.ensure(bytes));
+    TSPH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.ptr, bytes, f, f, (int)nt, stream));
+  }
+  k_new_index<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), sim->fscan.as<int64_t>(), nt, n,
+                                        sim->old2new.as<int64_t>(), sim->new2old.as<int64_t>());
+  const int other = sim->cur ^ 1;
+  k_permute<D, S><<<g, kBlock, 0, stream>>>(sim->new2old.as<int64_t>(), nt, pos, sim->vel[sim->cur].as<double>(),
+                                            sim->mass.as<S>(), sim->orig.as<int64_t>(), cs.cells.as<int64_t>(),
+                                            sim->pos[other].as<double>(), sim->vel[other].as<double>(),
+                                            sim->mass2.as<S>(), sim->orig2.as<int64_t>(),
+                                            sim->cells_new.as<int64_t>());
+  k_remap_order<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), sim->old2new.as<int64_t>(), nt);
+  TSPH_CHECK_LAUNCH();
+  std::swap(sim->mass, sim->mass2);
+  std::swap(sim->orig, sim->orig2);
+  sim->cur = other;
+  const double* npos = sim->pos[sim->cur].as<double>();
+  // neighbour lists on the re-ordered rows
+  GridDesc gd;
+  for (int d = 0; d < 3; ++d) { gd.ext[d] = cs.ext[d]; gd.stride[d] = cs.stride[d]; }
+  ctx->nb_targets = n; ctx->wnb_targets = nt - n;
+  TSPH_TRY(ctx->nb_off.ensure(sizeof(int64_t) * (n + 1)));
+  TSPH_TRY(ctx->wnb_off.ensure(sizeof(int64_t) * (nt - n + 1)));
+  TSPH_TRY(ctx->nb_cursor.ensure(sizeof(int64_t) * (n + 1)));
+  TSPH_TRY(ctx->wnb_cursor.ensure(sizeof(int64_t) * (nt - n + 1)));
+  unsigned long long* cnt = ctx->nb_cursor.as<unsigned long long>();
+  unsigned long long* wcnt = ctx->wnb_cursor.as<unsigned long long>();
+  k_build_lists<D, 0><<<g, kBlock, 0, stream>>>(npos, n, nt, sim->cells_new.as<int64_t>(), gd, cs.ukeys.as<int64_t>(),
+                                                cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells,
+                                                rc * rc, cnt, nullptr, nullptr, wcnt, nullptr, nullptr);
+  TSPH_CHECK_LAUNCH();
+  TSPH_TRY(exclusive_scan(ctx, cnt, n, ctx->nb_off.as<int64_t>(), stream, &ctx->nb_entries));
+  TSPH_TRY(exclusive_scan(ctx, wcnt, nt - n, ctx->wnb_off.as<int64_t>(), stream, &ctx->wnb_entries));
+  TSPH_REQUIRE(ctx->nb_entries < (1ll << 40), TSPH_ERR_OOM, "neighbour list too large");
+  TSPH_TRY(ctx->nb_idx.ensure(sizeof(int32_t) * (ctx->nb_entries + 1)));
+  TSPH_TRY(ctx->wnb_idx.ensure(sizeof(int32_t) * (ctx->wnb_entries + 1)));
+  k_build_lists<D, 1><<<g, kBlock, 0, stream>>>(npos, n, nt, sim->cells_new.as<int64_t>(), gd, cs.ukeys.as<int64_t>(),
+                                                cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells,
+                                                rc * rc, nullptr, ctx->nb_off.as<int64_t>(),
+                                                ctx->nb_idx.as<int32_t>(), nullptr, ctx->wnb_off.as<int64_t>(),
+                                                ctx->wnb_idx.as<int32_t>());
+  TSPH_CHECK_LAUNCH();
+  // both ping-pong buffers carry the (static) wall rows
+  if (nt > n) {
+    TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[sim->cur ^ 1].as<double>() + n * D, npos + n * D,
+                                  sizeof(double) * (nt - n) * D, hipMemcpyDeviceToDevice, stream));
+    TSPH_CHECK_HIP(hipMemcpyAsync(sim->vel[sim->cur ^ 1].as<double>() + n * D,
+                                  sim->vel[sim->cur].as<double>() + n * D, sizeof(double) * (nt - n) * D,
+                                  hipMemcpyDeviceToDevice, stream));
+  }
+  TSPH_CHECK_HIP(hipMemcpyAsync(sim->anchor.ptr, npos, sizeof(double) * n * D, hipMemcpyDeviceToDevice, stream));
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
+  sim->rebuilds++;
+  sim->built = true;
+  return TSPH_OK;
+}
+
+template <int D, typename S>
+WcsphArgs<S> sim_args(tsph_sim* sim) {
+  WcsphArgs<S> a = make_wcsph_args<S>(sim->prm.op, D);
+  a.pos = sim->pos[sim->cur].as<double>();
+  a.vel = sim->vel[sim->cur].as<double>();
+  a.mass = sim->mass.as<S>();
+  a.wall_accel = nullptr;
+  a.rho = sim->rho.as<S>(); a.p = sim->p.as<S>(); a.pr2 = sim->pr2.as<S>(); a.vol = sim->vol.as<S>();
+  return a;
+}
+
+// Queue one step (density [+diffusion], wall, force+integrate) reading the
+// stale slot of step `k`.
+template <int D, typename S>
+int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
+  tsph_ctx* ctx = sim->ctx;
+  const int64_t n = sim->n, nw = sim->nw;
+  const int slot = (int)(k & 1);
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  WcsphArgs<S> a = sim_args<D, S>(sim);
+  const int64_t* off = ctx->nb_off.as<int64_t>();
+  const int32_t* idx = ctx->nb_idx.as<int32_t>();
+  const unsigned gx = grid_xcd(n);
+  const int r = (int)(k % kRing);
+  if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
+  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
+  if (sim->prm.op.diffusion_scale > 0.0) {
+    WcsphArgs<S> a0 = a;
+    a0.rho = sim->rho_raw.as<S>();
+    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a0, off, idx, fl, slot, 0);
+    if (sim->prm.op.diffusion_form == 1) {
+      k_step_gradient<D, S><<<grid_for(n), kBlock, 0, stream>>>(a0, off, idx, sim->grad.as<S>(), fl, slot);
+      k_step_diffusion<D, S, 1><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx, a0.rho, sim->grad.as<S>(), fl, slot);
+    } else {
+      k_step_diffusion<D, S, 0><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx, a0.rho, nullptr, fl, slot);
+    }
+  } else {
+    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a, off, idx, fl, slot, 1);
+  }
+  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
+  if (nw > 0)
+    k_step_wall<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(),
+                                                           ctx->wnb_idx.as<int32_t>(), fl, slot);
+  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
+  StepArgs st{};
+  st.dt = sim->prm.op.dt;
+  st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
+  st.vel_out = sim->vel[sim->cur ^ 1].as<double>();
+  for (int d = 0; d < 3; ++d) {
+    st.lo[d] = sim->prm.clamp ? sim->prm.lo[d] : -1e300;
+    st.hi[d] = sim->prm.clamp ? sim->prm.hi[d] : 1e300;
+  }
+  st.tol = sim->prm.tol;
+  st.anchor = sim->anchor.as<double>();
+  st.block_sums = sim->block_sums.as<double>();
+  st.gravity = sim->prm.op.gravity;
+  if (meters)
+    k_step_force<D, S, true><<<gx, kBlock, 0, stream>>>(a, off, idx, sim->acc.as<S>(), st, fl, slot,
+                                                        0.5 * sim->prm.skin);
+  else
+    k_step_force<D, S, false><<<gx, kBlock, 0, stream>>>(a, off, idx, sim->acc.as<S>(), st, fl, slot,
+                                                         0.5 * sim->prm.skin);
+  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
+  if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
+  TSPH_CHECK_LAUNCH();
+  sim->cur ^= 1;
+  // record this step for the host
+  StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
+  k_pack_record<<<1, 64, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev);
+  TSPH_CHECK_LAUNCH();
+  TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
+  sim->ring_step[r] = k;
+  return TSPH_OK;
+}
+
+// Consume the record of step k: returns 1 if the step after it must be redone
+// (its list went stale), 0 otherwise.
+int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
+  const int r = (int)(k % kRing);
+  TSPH_CHECK_HIP(hipEventSynchronize(sim->ev[r]));
+  StepRecord* rec = &sim->ring[r];
+  *stale_after = rec->stale_out != 0;
+  if (rec->stale_in) return TSPH_OK;   // skipped: nothing to record
+  sim->steps_done = k + 1;
+  if (sim->timing) {
+    for (int q = 0; q < 3; ++q) {
+      float ms = 0.0f;
+      TSPH_CHECK_HIP(hipEventElapsedTime(&ms, sim->tev[r][q], sim->tev[r][q + 1]));
+      sim->sweep_ms[q] += ms;
+    }
+    sim->sweep_count++;
+  }
+  sim->escaped_total = (int64_t)rec->escaped;
+  const double rho_max = ord_dec_host(rec->maxes[MM_PEAK_RHO]);
+  const double spd = ord_dec_host(rec->maxes[MM_PEAK_SPEED]);
+  sim->peak_rho = rho_max > sim->peak_rho ? rho_max : sim->peak_rho;
+  sim->peak_speed = spd > sim->peak_speed ? spd : sim->peak_speed;
+  if (meters) {
+    sim->rows.push_back((double)k);
+    for (int i = 0; i < MS_NSUM; ++i) sim->rows.push_back(rec->sums[i]);
+    for (int i = 0; i < MM_NMAX; ++i) sim->rows.push_back(ord_dec_host(rec->maxes[i]));
+    for (int i = 0; i < MN_NMIN; ++i) sim->rows.push_back(ord_dec_host(rec->mins[i]));
+  }
+  return TSPH_OK;
+}
+
+template <int D, typename S>
+int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stream) {
+  const int64_t start = sim->steps_done;
+  const int64_t end = start + nsteps;
+  if (!sim->built) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
+  // steps in flight (queued but not yet consumed)
+  int64_t next = start;        // next step index to queue
+  int64_t done = start;        // next step index to consume
+  const int depth = 2;
+  auto is_meter = [&](int64_t k) { return meter_every > 0 && (k % meter_every) == 0; };
+  while (done < end) {
+    while (next < end && next - done < depth) {
+      TSPH_TRY((sim_queue_step<D, S>(sim, next, is_meter(next), stream)));
+      ++next;
+    }
+    bool stale_after = false;
+    TSPH_TRY(sim_consume(sim, done, is_meter(done), &stale_after));
+    ++done;
+    if (stale_after) {
+      // steps done..next-1 were skipped on the device; drain them, rebuild, redo
+      for (int64_t k = done; k < next; ++k) {
+        bool dummy;
+        TSPH_TRY(sim_consume(sim, k, false, &dummy));
+      }
+      TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
+      next = done;
+    }
+  }
+  return TSPH_OK;
+}
+
+}  // namespace tsph
+
+extern "C" {
+
+int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double* vel, const void* mass,
+                    hipStream_t stream, tsph_sim** out) {
+  TSPH_REQUIRE(prm && out && pos && vel && mass, TSPH_ERR_ARG, "NULL argument");
+  TSPH_REQUIRE(prm->op.dim == 2 || prm->op.dim == 3, TSPH_ERR_ARG, "dim must be 2 or 3");
+  TSPH_REQUIRE(prm->op.precision == TSPH_F64 || prm->op.precision == TSPH_F32, TSPH_ERR_ARG, "bad precision");
+  TSPH_REQUIRE(prm->op.n_fluid > 0 && prm->op.n_wall >= 0, TSPH_ERR_ARG, "need at least one fluid particle");
+  TSPH_REQUIRE(prm->skin > 0.0, TSPH_ERR_ARG, "the resident stepper needs a positive Verlet skin");
+  TSPH_REQUIRE(prm->op.h > 0.0 && prm->op.dt > 0.0, TSPH_ERR_ARG, "h and dt must be positive");
+  TSPH_REQUIRE(prm->op.reaction_style == 0 && prm->op.has_wall_accel == 0, TSPH_ERR_ARG,
+               "the resident stepper runs static walls without load output");
+  tsph_sim* sim = new tsph_sim();
+  sim->prm = *prm;
+  sim->dim = prm->op.dim;
+  sim->prec = prm->op.precision;
+  sim->n = prm->op.n_fluid; sim->nw = prm->op.n_wall; sim->nt = sim->n + sim->nw;
+  sim->elt = sim->prec == TSPH_F64 ? 8 : 4;
+  int st = tsph_ctx_create(&sim->ctx);
+  if (st != TSPH_OK) { delete sim; return st; }
+  const int D = sim->dim;
+  const int64_t nt = sim->nt, n = sim->n;
+  const size_t e = sim->elt;
+  int rc = TSPH_OK;
+#define ENS(buf, bytes) if (rc == TSPH_OK) rc = (buf).ensure(bytes)
+  ENS(sim->pos[0], 8 * nt * D); ENS(sim->pos[1], 8 * nt * D);
+  ENS(sim->vel[0], 8 * nt * D); ENS(sim->vel[1], 8 * nt * D);
+  ENS(sim->mass, e * nt); ENS(sim->mass2, e * nt);
+  ENS(sim->orig, 8 * nt); ENS(sim->orig2, 8 * nt);
+  ENS(sim->rho, e * nt); ENS(sim->p, e * nt); ENS(sim->pr2, e * nt); ENS(sim->vol, e * nt);
+  ENS(sim->rho_raw, e * nt); ENS(sim->grad, e * n * D); ENS(sim->acc, e * n * D);
+  ENS(sim->anchor, 8 * n * D);
+  ENS(sim->flags, sizeof(SimFlags));
+  ENS(sim->block_sums, 8 * MS_NSUM * (int64_t)grid_xcd(n));
+  ENS(sim->sums_out, 8 * MS_NSUM);
+  ENS(sim->rec_dev, sizeof(StepRecord) * kRing);
+  ENS(sim->old2new, 8 * nt); ENS(sim->new2old, 8 * nt); ENS(sim->fscan, 8 * nt);
+  ENS(sim->cells_new, 8 * nt * 3);
+#undef ENS
+  if (rc != TSPH_OK) { tsph_sim_destroy(sim); return rc; }
+  hipError_t he = hipHostMalloc((void**)&sim->ring, sizeof(StepRecord) * kRing, hipHostMallocDefault);
+  if (he != hipSuccess) { set_error("hipHostMalloc: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
+  memset(sim->ring, 0, sizeof(StepRecord) * kRing);
+  for (int r = 0; r < kRing; ++r) {
+    he = hipEventCreateWithFlags(&sim->ev[r], hipEventDisableTiming);
+    if (he != hipSuccess) { set_error("hipEventCreate: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
+  }
+  for (int r = 0; r < kRing; ++r)
+    for (int q = 0; q < 4; ++q) {
+      he = hipEventCreate(&sim->tev[r][q]);
+      if (he != hipSuccess) { set_error("hipEventCreate: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
+    }
+  // initial state: caller order; velocities of wall rows are zero
+  he = hipMemcpyAsync(sim->pos[0].ptr, pos, 8 * nt * D, hipMemcpyDeviceToDevice, stream);
+  if (he == hipSuccess) he = hipMemsetAsync(sim->vel[0].ptr, 0, 8 * nt * D, stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(sim->vel[0].ptr, vel, 8 * n * D, hipMemcpyDeviceToDevice, stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(sim->mass.ptr, mass, e * nt, hipMemcpyDeviceToDevice, stream);
+  if (he == hipSuccess) he = hipMemsetAsync(sim->flags.ptr, 0, sizeof(SimFlags), stream);
+  if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
+  k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
+  *out = sim;
+  return TSPH_OK;
+}
+
+int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stream) {
+  TSPH_REQUIRE(sim && nsteps >= 0, TSPH_ERR_ARG, "bad arguments");
+  if (nsteps == 0) return TSPH_OK;
+  if (sim->dim == 2)
+    return sim->prec == TSPH_F64 ? sim_run<2, double>(sim, nsteps, meter_every, stream)
+                                 : sim_run<2, float>(sim, nsteps, meter_every, stream);
+  return sim->prec == TSPH_F64 ? sim_run<3, double>(sim, nsteps, meter_every, stream)
+                               : sim_run<3, float>(sim, nsteps, meter_every, stream);
+}
+
+int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc, hipStream_t stream) {
+  TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
+  const int64_t n = sim->n;
+  const unsigned g = grid_for(n);
+  const double* P = sim->pos[sim->cur].as<double>();
+  const double* V = sim->vel[sim->cur].as<double>();
+  if (sim->dim == 2) {
+    if (sim->prec == TSPH_F64)
+      k_unpermute<2, double><<<g, kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, P, V, sim->rho.as<double>(), sim->p.as<double>(), sim->acc.as<double>(), pos, vel, (double*)rho, (double*)p, (double*)acc);
+    else
+      k_unpermute<2, float><<<g, kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, P, V, sim->rho.as<float>(), sim->p.as<float>(), sim->acc.as<float>(), pos, vel, (float*)rho, (float*)p, (float*)acc);
+  } else {
+    if (sim->prec == TSPH_F64)
+      k_unpermute<3, double><<<g, kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, P, V, sim->rho.as<double>(), sim->p.as<double>(), sim->acc.as<double>(), pos, vel, (double*)rho, (double*)p, (double*)acc);
+    else
+      k_unpermute<3, float><<<g, kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, P, V, sim->rho.as<float>(), sim->p.as<float>(), sim->acc.as<float>(), pos, vel, (float*)rho, (float*)p, (float*)acc);
+  }
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
+int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
+  TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
+  if (counters) {
+    counters[0] = sim->steps_done;
+    counters[1] = sim->rebuilds;
+    counters[2] = sim->escaped_total;
+    counters[3] = sim->ctx->nb_entries;
+    counters[4] = (int64_t)(sim->rows.size() / (1 + MS_NSUM + MM_NMAX + MN_NMIN));
+  }
+  if (peaks) { peaks[0] = sim->peak_rho; peaks[1] = sim->peak_speed; }
+  return TSPH_OK;
+}
+
+int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows) {
+  TSPH_REQUIRE(sim && nrows, TSPH_ERR_ARG, "bad arguments");
+  const int64_t width = 1 + MS_NSUM + MM_NMAX + MN_NMIN;
+  const int64_t have = (int64_t)sim->rows.size() / width;
+  *nrows = have;
+  if (rows) {
+    const int64_t m = have < max_rows ? have : max_rows;
+    memcpy(rows, sim->rows.data(), sizeof(double) * width * m);
+  }
+  return TSPH_OK;
+}
+
+int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps) {
+  TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
+  if (sweep_ms) for (int q = 0; q < 3; ++q) sweep_ms[q] = sim->sweep_ms[q];
+  if (steps) *steps = sim->sweep_count;
+  if (enable >= 0) {
+    sim->timing = enable;
+    for (int q = 0; q < 3; ++q) sim->sweep_ms[q] = 0.0;
+    sim->sweep_count = 0;
+  }
+  return TSPH_OK;
+}
+
+int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream) {
+  TSPH_REQUIRE(sim && pairs && radius > 0.0, TSPH_ERR_ARG, "bad arguments");
+  TSPH_REQUIRE(sim->built, TSPH_ERR_ARG, "no neighbour list yet: step the simulation first");
+  tsph_ctx* ctx = sim->ctx;
+  TSPH_TRY(ctx->s_red.ensure(sizeof(unsigned long long)));
+  unsigned long long* cnt = ctx->s_red.as<unsigned long long>();
+  TSPH_CHECK_HIP(hipMemsetAsync(cnt, 0, 8, stream));
+  const double r2 = radius * radius;
+  if (sim->dim == 2)
+    k_count_close<2><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
+        ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+  else
+    k_count_close<3><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
+        ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+  TSPH_CHECK_LAUNCH();
+  unsigned long long h = 0;
+  TSPH_CHECK_HIP(hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  *pairs = (int64_t)h;
+  return TSPH_OK;
+}
+
+int tsph_sim_destroy(tsph_sim* sim) {
+  if (!sim) return TSPH_OK;
+  DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
+                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw,
+                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
+                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new};
+  for (DevBuf* b : bufs) b->release();
+  for (int r = 0; r < kRing; ++r) if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
+  for (int r = 0; r < kRing; ++r)
+    for (int q = 0; q < 4; ++q) if (sim->tev[r][q]) (void)hipEventDestroy(sim->tev[r][q]);
+  if (sim->ring) (void)hipHostFree(sim->ring);
+  if (sim->ctx) tsph_ctx_destroy(sim->ctx);
+  delete sim;
+  return TSPH_OK;
+}
+
+}  // extern "C"

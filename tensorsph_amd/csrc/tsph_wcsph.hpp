@@ -1,0 +1,209 @@
+// Argument blocks and device helpers of the WCSPH sweeps (shared by the
+// operator entry points and the resident stepper).
+#pragma once
+#include <string.h>
+
+#include "tsph_common.hpp"
+#include "../../include/tsph.h"
+
+namespace tsph {
+
+template <typename S> struct WcsphArgs {
+  KernelConst<S> k;
+  S skip2;            // squared support with a relative margin: beyond it W = grad W = 0
+  S rho0, bulk, gamma, inv_gamma, ratio_floor, rho_floor;
+  S gravity, nu2, friction, eta2, dt, diff_scale, eig_floor;
+  int64_t n, nw;
+  const double* pos;  // (n + nw, D)
+  const double* vel;  // (n + nw, D)
+  const S* mass;      // (n + nw)
+  const double* wall_accel;
+  S* rho;             // (n + nw): fluid rows from sweep A, wall rows from sweep W
+  S* p;
+  S* pr2;             // p / rho^2
+  S* vol;             // m / rho
+};
+
+template <typename S>
+inline WcsphArgs<S> make_wcsph_args(const tsph_wcsph_params& p, int dim) {
+  WcsphArgs<S> a{};
+  a.k = make_cubic<S>(p.h, dim);
+  a.skip2 = S(4.0 * p.h * p.h * (1.0 + 1e-5));
+  a.rho0 = S(p.rho0);
+  a.bulk = S(p.rho0 * (p.sound_speed * p.sound_speed) / p.gamma);
+  a.gamma = S(p.gamma);
+  a.inv_gamma = S(1.0 / p.gamma);
+  a.ratio_floor = S(0.01);
+  a.rho_floor = S(0.1 * p.rho0);
+  a.gravity = S(p.gravity);
+  a.nu2 = S(2.0 * p.viscosity);
+  // with walls present and no friction, fluid-wall pairs carry no viscosity
+  a.friction = S(p.n_wall > 0 ? p.wall_friction : 1.0);
+  a.eta2 = S(0.01 * p.h * p.h);
+  a.dt = S(p.dt);
+  a.diff_scale = S(p.diffusion_scale);
+  a.eig_floor = S(p.eigenvalue_floor);
+  a.n = p.n_fluid;
+  a.nw = p.n_wall;
+  return a;
+}
+
+// Order-preserving map of a double onto uint64, so atomicMin/Max on the
+// integers orders the doubles (and so is independent of arrival order).
+TSPH_D unsigned long long ord_enc(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+inline double ord_dec_host(unsigned long long u) {
+  unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+  double x;
+  memcpy(&x, &b, sizeof(x));
+  return x;
+}
+
+// Meter slots (per-block partial sums, then atomics on ordered keys).
+enum MeterSum { MS_MASS = 0, MS_MX, MS_MY, MS_MZ, MS_KE, MS_RHO, MS_PE, MS_NSUM };
+// per-row maxima (reset at every metered step), the per-step displacement and
+// the run-long peaks of density and speed (droplet_impact.py:1697-1705)
+enum MeterMax { MM_RHO_MAX = 0, MM_P_MAX, MM_SPEED, MM_ACC, MM_Y_MAX, MM_X_MAX, MM_DISP,
+                MM_PEAK_RHO, MM_PEAK_SPEED, MM_NMAX };
+enum MeterMin { MN_RHO_MIN = 0, MN_P_MIN, MN_Y_MIN, MN_NMIN };
+
+struct StepArgs {
+  double dt;
+  double* pos_out;                 // (n + nw, D); wall rows are kept from the input
+  double* vel_out;
+  double lo[3], hi[3], tol;        // apply_boundary box (droplet_impact.py:1293-1312)
+  const double* anchor;            // positions at the last neighbour-list build
+  unsigned long long* maxes;       // MM_NMAX ordered keys (device)
+  double* block_sums;              // gridDim.x * MS_NSUM partials, or NULL
+  unsigned long long* escaped;     // boundary crossings
+  int meters;                      // 1: also fill block_sums and the meter maxima
+  double gravity;                  // for the potential energy meter
+};
+
+// v' = v + dt a, x' = x + dt v', then the safety clamp of apply_boundary:
+// positions clamped to the box, the normal velocity zeroed beyond 0.25 dx.
+template <int D, typename S>
+TSPH_D void integrate_one(int64_t i, const double* xi, const double* vi, const S* acc,
+                          const WcsphArgs<S>& a, const StepArgs& st) {
+  double xn[D], vn[D];
+  int hit_any = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    vn[d] = vi[d] + st.dt * (double)acc[d];
+    double x = xi[d] + st.dt * vn[d];
+    const int hit = (x < st.lo[d] - st.tol) || (x > st.hi[d] + st.tol);
+    hit_any |= hit;
+    x = x < st.lo[d] ? st.lo[d] : (x > st.hi[d] ? st.hi[d] : x);
+    if (hit) vn[d] = 0.0;
+    xn[d] = x;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) { st.pos_out[i * D + d] = xn[d]; st.vel_out[i * D + d] = vn[d]; }
+  double disp2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) { const double df = xn[d] - st.anchor[i * D + d]; disp2 += df * df; }
+  // wave-level reductions of the per-step maxima
+  double disp = sqrt(disp2);
+  double spd2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) spd2 += vi[d] * vi[d];
+  double rho = (double)a.rho[i];
+  unsigned long long hits = hit_any;
+  for (int off = 32; off > 0; off >>= 1) {
+    disp = fmax(disp, __shfl_xor(disp, off, 64));
+    spd2 = fmax(spd2, __shfl_xor(spd2, off, 64));
+    rho = fmax(rho, __shfl_xor(rho, off, 64));
+    hits += __shfl_xor(hits, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&st.maxes[MM_DISP], ord_enc(disp));
+    atomicMax(&st.maxes[MM_PEAK_SPEED], ord_enc(sqrt(spd2)));
+    atomicMax(&st.maxes[MM_PEAK_RHO], ord_enc(rho));
+    if (hits) atomicAdd(st.escaped, hits);
+  }
+}
+
+// Regularised inverse f(M) g, f(l) = 1 / max(l, floor_frac * max(trace/D, 1e-30)),
+// by cyclic Jacobi on the symmetric matrix built from M's lower triangle (as
+// torch.linalg.eigh(UPLO='L') reads it).
+template <int D>
+TSPH_D void sym_eig(const double* m_in, double* w, double* v) {
+  double a[D * D];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) a[r * D + c] = r >= c ? m_in[r * D + c] : m_in[c * D + r];
+#pragma unroll
+  for (int r = 0; r < D; ++r)
+#pragma unroll
+    for (int c = 0; c < D; ++c) v[r * D + c] = r == c ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 12; ++sweep) {
+    double offn = 0.0;
+#pragma unroll
+    for (int p = 0; p < D; ++p)
+#pragma unroll
+      for (int q = p + 1; q < D; ++q) offn += a[p * D + q] * a[p * D + q];
+    if (offn < 1e-300) break;
+#pragma unroll
+    for (int p = 0; p < D; ++p)
+#pragma unroll
+      for (int q = p + 1; q < D; ++q) {
+        const double apq = a[p * D + q];
+        if (apq == 0.0) continue;
+        const double theta = (a[q * D + q] - a[p * D + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const double akp = a[k * D + p], akq = a[k * D + q];
+          a[k * D + p] = c * akp - s * akq;
+          a[k * D + q] = s * akp + c * akq;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const double apk = a[p * D + k], aqk = a[q * D + k];
+          a[p * D + k] = c * apk - s * aqk;
+          a[q * D + k] = s * apk + c * aqk;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const double vkp = v[k * D + p], vkq = v[k * D + q];
+          v[k * D + p] = c * vkp - s * vkq;
+          v[k * D + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) w[k] = a[k * D + k];
+}
+
+template <int D>
+TSPH_D void regularised_solve(const double* m, const double* g, double floor_frac, double* out) {
+  double w[D], v[D * D];
+  sym_eig<D>(m, w, v);
+  double tr = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) tr += w[k];
+  double mean = tr / D;
+  mean = mean > 1e-30 ? mean : 1e-30;
+  const double fl = floor_frac * mean;
+  double proj[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < D; ++r) s += v[r * D + k] * g[r];
+    proj[k] = s / (w[k] > fl ? w[k] : fl);
+  }
+#pragma unroll
+  for (int r = 0; r < D; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s += v[r * D + k] * proj[k];
+    out[r] = s;
+  }
+}
+
+}  // namespace tsph

@@ -1,0 +1,169 @@
+"""Resident WCSPH stepper (libtsph ``tsph_sim_*``): the state lives in HBM
+between steps; Python only queues work and reads meters.
+
+``WcsphSimulation`` runs the loop of droplet_impact.run (droplet_impact.py:
+1674-1763) / dam_break.run_case (dam_break.py:374-465): Verlet-skin pair
+cache, rhs, symplectic Euler, the apply_boundary clamp, and per-step peak
+meters, without a host round trip per step.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+SimParams = _lib.SimParams
+_vp, _i64 = C.c_void_p, C.c_int64
+
+METER_KEYS = ["step", "mass", "momentum_x", "momentum_y", "momentum_z", "kinetic_energy",
+              "rho_sum", "potential_energy", "rho_max", "pressure_max", "max_speed",
+              "max_acceleration", "y_max", "x_max", "displacement", "peak_rho", "peak_speed",
+              "rho_min", "pressure_min", "y_min"]
+
+
+def boundary_clamp(pos: torch.Tensor, vel: torch.Tensor, lo, hi, tol: float) -> int:
+    """In-place clamp of FP64 (n, D) pos/vel; returns the crossing count."""
+    dev = _lib.require_gpu(pos, vel)
+    if pos.dtype != torch.float64 or not pos.is_contiguous():
+        work_p = pos.detach().to(torch.float64).contiguous()
+        work_v = vel.detach().to(torch.float64).contiguous()
+    else:
+        work_p, work_v = pos, vel
+    dim = pos.shape[1]
+    lo_a = (C.c_double * 3)(*(list(lo) + [0.0] * (3 - dim)))
+    hi_a = (C.c_double * 3)(*(list(hi) + [0.0] * (3 - dim)))
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    _lib.check(_lib.lib().tsph_apply_boundary(_lib.ptr(work_p), _lib.ptr(work_v), pos.shape[0], dim,
+                                              lo_a, hi_a, float(tol), _lib.ptr(cnt),
+                                              _lib.stream_handle(dev)))
+    if work_p is not pos:
+        pos.copy_(work_p.to(pos.dtype))
+        vel.copy_(work_v.to(vel.dtype))
+    return int(cnt.item())
+
+
+class WcsphSimulation:
+    """A WCSPH simulation resident on one GPU.
+
+    pos (n, D), vel (n, D), mass (n) are the fluid; wall (nw, D) static wall
+    particles of mass ``wall_mass``.  ``precision`` is "fp64" or "fp32"
+    (FP32 compensated).  ``lo``/``hi``/``tol`` give the apply_boundary box.
+    """
+
+    def __init__(self, pos, vel, mass, wall, *, h, rho0, sound_speed, gamma, gravity, viscosity,
+                 dt, wall_mass, skin, lo, hi, tol, wall_friction=0.0, density_diffusion=0.0,
+                 diffusion_form="molteni", eigenvalue_floor=0.1, precision="fp32", clamp=True,
+                 device="cuda"):
+        dev = torch.device(device)
+        pos = torch.as_tensor(pos, dtype=torch.float64, device=dev)
+        vel = torch.as_tensor(vel, dtype=torch.float64, device=dev)
+        dim = pos.shape[1]
+        wall = torch.zeros((0, dim), dtype=torch.float64, device=dev) if wall is None else \
+            torch.as_tensor(wall, dtype=torch.float64, device=dev)
+        self.n, self.nw, self.dim = pos.shape[0], wall.shape[0], dim
+        self.code = _lib.F64 if precision == "fp64" else _lib.F32
+        self.sdt = torch.float64 if self.code == _lib.F64 else torch.float32
+        self.device = dev
+        mass_all = torch.cat((torch.as_tensor(mass, dtype=torch.float64, device=dev),
+                              torch.full((self.nw,), float(wall_mass), dtype=torch.float64,
+                                         device=dev))).to(self.sdt).contiguous()
+        pos_all = torch.cat((pos, wall)).contiguous()
+        p = SimParams()
+        op = p.op
+        op.dim, op.precision, op.n_fluid, op.n_wall = dim, self.code, self.n, self.nw
+        op.h, op.rho0, op.sound_speed, op.gamma = h, rho0, sound_speed, gamma
+        op.gravity, op.viscosity, op.wall_friction, op.dt = gravity, viscosity, wall_friction, dt
+        if density_diffusion > 0.0:
+            op.diffusion_scale = 2.0 * density_diffusion * sound_speed * h
+            op.diffusion_form = 1 if diffusion_form == "antuono" else 0
+        op.eigenvalue_floor = eigenvalue_floor
+        p.skin = skin
+        for d in range(3):
+            p.lo[d] = lo[d] if d < dim else 0.0
+            p.hi[d] = hi[d] if d < dim else 0.0
+        p.tol = tol
+        p.clamp = 1 if clamp else 0
+        self.params = p
+        handle = _vp()
+        stream = _lib.stream_handle(dev)
+        _lib.check(_lib.lib().tsph_sim_create(C.byref(p), _lib.ptr(pos_all), _lib.ptr(vel),
+                                              _lib.ptr(mass_all), stream, C.byref(handle)))
+        self.handle = handle
+        self._keep = (pos_all, vel, mass_all)   # alive until the queued copies ran
+        torch.cuda.current_stream(dev).synchronize()
+        self._keep = None
+
+    def step(self, nsteps: int, meter_every: int = 0) -> None:
+        _lib.check(_lib.lib().tsph_sim_step(self.handle, int(nsteps), int(meter_every),
+                                            _lib.stream_handle(self.device)))
+
+    def read(self) -> dict:
+        n, d = self.n, self.dim
+        out = {
+            "position": torch.empty((n, d), dtype=torch.float64, device=self.device),
+            "velocity": torch.empty((n, d), dtype=torch.float64, device=self.device),
+            "density": torch.empty(n, dtype=self.sdt, device=self.device),
+            "pressure": torch.empty(n, dtype=self.sdt, device=self.device),
+            "acceleration": torch.empty((n, d), dtype=self.sdt, device=self.device),
+        }
+        _lib.check(_lib.lib().tsph_sim_read(self.handle, _lib.ptr(out["position"]),
+                                            _lib.ptr(out["velocity"]), _lib.ptr(out["density"]),
+                                            _lib.ptr(out["pressure"]), _lib.ptr(out["acceleration"]),
+                                            _lib.stream_handle(self.device)))
+        return out
+
+    def stats(self) -> dict:
+        c = (_i64 * 5)()
+        pk = (C.c_double * 2)()
+        _lib.check(_lib.lib().tsph_sim_stats(self.handle, c, pk))
+        return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "list_entries": c[3],
+                "meter_rows": c[4], "peak_rho": pk[0], "peak_speed": pk[1]}
+
+    def meters(self) -> np.ndarray:
+        nrows = _i64(0)
+        _lib.check(_lib.lib().tsph_sim_meters(self.handle, None, 0, C.byref(nrows)))
+        rows = np.zeros((nrows.value, len(METER_KEYS)), np.float64)
+        if nrows.value:
+            _lib.check(_lib.lib().tsph_sim_meters(
+                self.handle, rows.ctypes.data_as(C.POINTER(C.c_double)), nrows.value,
+                C.byref(nrows)))
+        return rows
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            _lib.lib().tsph_sim_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+
+def _set_timing(self, enable: bool) -> None:
+    _lib.check(_lib.lib().tsph_sim_timing(self.handle, 1 if enable else 0, None, None))
+
+
+def _sweep_times(self):
+    ms = (C.c_double * 3)()
+    steps = _i64(0)
+    _lib.check(_lib.lib().tsph_sim_timing(self.handle, -1, ms, C.byref(steps)))
+    return [ms[0], ms[1], ms[2]], steps.value
+
+
+def _count_pairs(self, radius: float) -> int:
+    out = _i64(0)
+    _lib.check(_lib.lib().tsph_sim_count_pairs(self.handle, float(radius), C.byref(out),
+                                               _lib.stream_handle(self.device)))
+    return out.value
+
+
+WcsphSimulation.set_timing = _set_timing
+WcsphSimulation.sweep_times = _sweep_times
+WcsphSimulation.count_pairs = _count_pairs
