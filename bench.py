@@ -73,7 +73,10 @@ def make_sim(cfg, pos, mass, wall, wmass, precision):
 
 def time_gpu(sim, steps, warmup):
     import torch
-    sim.step(warmup)
+    from tensorsph_amd.autotune import tune_tiles
+    sim.step(max(warmup // 2, 1))
+    tuned = tune_tiles(sim, steps=4, warm=1)     # trial steps count as warm-up
+    sim.step(max(warmup - warmup // 2, 1))
     torch.cuda.synchronize()
     sim.set_timing(True)
     t0 = time.perf_counter()
@@ -82,7 +85,7 @@ def time_gpu(sim, steps, warmup):
     wall = time.perf_counter() - t0
     sweep_ms, nsteps = sim.sweep_times()
     sim.set_timing(False)
-    return wall, sweep_ms, nsteps
+    return wall, sweep_ms, nsteps, tuned
 
 
 def cpu_baseline(seconds: float):
@@ -133,7 +136,7 @@ def main():
     h = cfg.h_factor * cfg.dx
 
     sim = make_sim(cfg, pos, mass, wall, wmass, "fp32")
-    wall_s, sweep_ms, nsteps = time_gpu(sim, args.steps, args.warmup)
+    wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup)
     pairs = sim.count_pairs(2.0 * h)
     stats = sim.stats()
     sim.close()
@@ -173,18 +176,19 @@ def main():
         "pair_interactions_per_s": pairs * args.steps / wall_s,
         "pairs_within_2h": pairs,
         "list_rebuilds_in_run": stats["rebuilds"],
+        "tile_shape": {"chosen": tuned["best"], "trial_ms_per_step": tuned["ms"]},
         "algorithmic_GBps_step": step_bytes * args.steps / wall_s / 1e9,
         "sweep_ms_per_step": {"density": sweep_ms[0] / max(nsteps, 1),
                               "wall": sweep_ms[1] / max(nsteps, 1),
                               "force_integrate": force_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_step_force (force + viscosity + integrate)",
+                     "kernel": "k_tile_force (force + viscosity + integrate)",
                      "algorithmic_bytes_per_launch": b_force},
     }
     if not args.no_fp64:
         sim = make_sim(cfg, pos, mass, wall, wmass, "fp64")
-        w64, sw64, n64 = time_gpu(sim, args.steps, args.warmup)
+        w64, sw64, n64, tuned64 = time_gpu(sim, args.steps, args.warmup)
         sim.close()
         del sim
         v64 = nf * args.steps / w64

@@ -173,6 +173,11 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
  * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, peak rho,
  * peak speed], minima[rho, p, y]: 20 doubles per row. */
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
+/* Sweep layout.  tiles = 1 (the 2-D default) runs the LDS cell-tile sweeps
+ * with tile_x x tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108,
+ * tile_x tile_y <= 64); tiles = 0 runs the CSR neighbour-list sweeps (the 3-D
+ * path).  Takes effect at the next step, which rebuilds the layout. */
+int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y);
 /* Sweep timing with HIP events on the stepping stream.  Returns the summed
  * milliseconds of the density, wall and force+integrate sweeps and the number
  * of timed steps since the last reset; enable = 1/0 resets the sums and turns

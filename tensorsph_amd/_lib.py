@@ -103,6 +103,7 @@ def _declare(L):
         "tsph_sim_stats": [_vp, _ip, _dp],
         "tsph_sim_meters": [_vp, _dp, _i64, _ip],
         "tsph_sim_timing": [_vp, C.c_int, _dp, _ip],
+        "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],
         "tsph_sim_destroy": [_vp],
     }

@@ -27,25 +27,9 @@
 #include "tsph_common.hpp"
 #include "tsph_internal.hpp"
 #include "tsph_wcsph.hpp"
+#include "tsph_engine.hpp"
 
 namespace tsph {
-
-// device flag block
-struct SimFlags {
-  unsigned long long maxes[MM_NMAX];
-  unsigned long long escaped;
-  unsigned long long stale[2];      // slot k&1 = input of step k, slot (k+1)&1 = output
-  unsigned long long mins[MN_NMIN]; // ordered keys, atomicMin
-};
-
-// one entry of the host ring, packed on the device after every step and
-// copied to pinned memory in one transfer
-struct StepRecord {
-  unsigned long long stale_in, stale_out, escaped;
-  unsigned long long maxes[MM_NMAX];
-  unsigned long long mins[MN_NMIN];
-  double sums[MS_NSUM];
-};
 
 __global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __restrict__ sums, int slot_in,
                               int with_sums, StepRecord* __restrict__ rec) {
@@ -56,7 +40,7 @@ __global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __r
   if (t < MS_NSUM) rec->sums[t] = with_sums ? sums[t] : 0.0;
 }
 
-constexpr int kRing = 4;
+
 
 // ------------------------------------------------------------------ step sweeps
 template <int D, typename S>
@@ -544,33 +528,6 @@ __global__ void k_iota(int64_t* __restrict__ v, int64_t n) {
 
 using namespace tsph;
 
-struct tsph_sim {
-  tsph_sim_params prm{};
-  int dim = 2, prec = 0;
-  int64_t n = 0, nw = 0, nt = 0;
-  tsph_ctx* ctx = nullptr;
-  DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
-  DevBuf rho, p, pr2, vol, rho_raw, grad, acc, anchor;
-  DevBuf flags, block_sums, sums_out, rec_dev;
-  DevBuf old2new, new2old, fscan, cells_new;
-  StepRecord* ring = nullptr;       // pinned
-  hipEvent_t ev[kRing] = {};
-  // per-sweep timing (HIP events around each sweep on the stepping stream)
-  int timing = 0;
-  hipEvent_t tev[kRing][4] = {};
-  double sweep_ms[3] = {0, 0, 0};
-  int64_t sweep_count = 0;
-  int64_t pairs_2h = -1;
-  int64_t ring_step[kRing];
-  int cur = 0;
-  int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
-  bool built = false;
-  // host-side meter log
-  std::vector<double> rows;         // per recorded step: step, sums..., maxes..., mins...
-  double peak_rho = 0.0, peak_speed = 0.0;
-  size_t elt = 8;
-};
-
 namespace tsph {
 
 template <int D, typename S>
@@ -658,10 +615,9 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   return a;
 }
 
-// Queue one step (density [+diffusion], wall, force+integrate) reading the
-// stale slot of step `k`.
+// The CSR-list sweeps of one step (used where no cell-tile path exists).
 template <int D, typename S>
-int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
+int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   tsph_ctx* ctx = sim->ctx;
   const int64_t n = sim->n, nw = sim->nw;
   const int slot = (int)(k & 1);
@@ -671,7 +627,6 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   const int32_t* idx = ctx->nb_idx.as<int32_t>();
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
-  if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
   if (sim->prm.op.diffusion_scale > 0.0) {
     WcsphArgs<S> a0 = a;
@@ -712,6 +667,34 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
   TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
+// Queue one step (density [+diffusion], wall, force+integrate) reading the
+// stale slot of step `k`.
+template <int D, typename S>
+int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
+  tsph_ctx* ctx = sim->ctx;
+  const int64_t n = sim->n, nw = sim->nw;
+  const int slot = (int)(k & 1);
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  WcsphArgs<S> a = sim_args<D, S>(sim);
+  const int64_t* off = ctx->nb_off.as<int64_t>();
+  const int32_t* idx = ctx->nb_idx.as<int32_t>();
+  const unsigned gx = grid_xcd(n);
+  const int r = (int)(k % kRing);
+  if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
+  bool tiled = false;
+  if constexpr (D == 2) {
+    if (sim->tiles) {
+      TSPH_TRY((sim_queue_step_tiles<D, S>(sim, k, meters, stream)));
+      const int64_t nblocks = sim->ntiles_f;   // partials are indexed by tile
+      if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), nblocks, sim->sums_out.as<double>(), fl, slot);
+      TSPH_CHECK_LAUNCH();
+      tiled = true;
+    }
+  }
+  if (!tiled) TSPH_TRY((queue_list_sweeps<D, S>(sim, k, meters, stream)));
   sim->cur ^= 1;
   // record this step for the host
   StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
@@ -758,7 +741,13 @@ template <int D, typename S>
 int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stream) {
   const int64_t start = sim->steps_done;
   const int64_t end = start + nsteps;
-  if (!sim->built) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
+  auto rebuild = [&]() -> int {
+    if constexpr (D == 2) {
+      if (sim->tiles) return sim_rebuild_tiles<D, S>(sim, stream);
+    }
+    return sim_rebuild<D, S>(sim, stream);
+  };
+  if (!sim->built) TSPH_TRY(rebuild());
   // steps in flight (queued but not yet consumed)
   int64_t next = start;        // next step index to queue
   int64_t done = start;        // next step index to consume
@@ -778,7 +767,7 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
         bool dummy;
         TSPH_TRY(sim_consume(sim, k, false, &dummy));
       }
-      TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
+      TSPH_TRY(rebuild());
       next = done;
     }
   }
@@ -847,6 +836,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemsetAsync(sim->flags.ptr, 0, sizeof(SimFlags), stream);
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
+  if (sim->dim == 2) { sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4; }
   *out = sim;
   return TSPH_OK;
 }
@@ -907,6 +897,18 @@ int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrow
   return TSPH_OK;
 }
 
+int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
+  TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
+  TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
+  TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
+                              tile_x * tile_y <= 64),
+               TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
+  sim->tiles = tiles;
+  if (tiles) { sim->tile_dim[0] = tile_x; sim->tile_dim[1] = tile_y; }
+  sim->built = false;   // next step rebuilds with the new layout
+  return TSPH_OK;
+}
+
 int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   if (sweep_ms) for (int q = 0; q < 3; ++q) sweep_ms[q] = sim->sweep_ms[q];
@@ -927,13 +929,21 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
   unsigned long long* cnt = ctx->s_red.as<unsigned long long>();
   TSPH_CHECK_HIP(hipMemsetAsync(cnt, 0, 8, stream));
   const double r2 = radius * radius;
-  if (sim->dim == 2)
-    k_count_close<2><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
-        ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
-  else
-    k_count_close<3><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
-        ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
-  TSPH_CHECK_LAUNCH();
+  TSPH_REQUIRE(radius <= 2.0 * sim->prm.op.h + sim->prm.skin, TSPH_ERR_ARG,
+               "radius exceeds the neighbour-list build radius");
+  if (sim->tiles) {
+    TSPH_TRY(tile_count_pairs(sim, radius, cnt, stream));
+  } else {
+    TSPH_REQUIRE(ctx->nb_off.ptr && ctx->nb_idx.ptr && ctx->nb_targets == sim->n, TSPH_ERR_ARG,
+                 "no neighbour lists built");
+    if (sim->dim == 2)
+      k_count_close<2><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
+          ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+    else
+      k_count_close<3><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
+          ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+    TSPH_CHECK_LAUNCH();
+  }
   unsigned long long h = 0;
   TSPH_CHECK_HIP(hipMemcpyAsync(&h, cnt, 8, hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));

@@ -1,0 +1,85 @@
+// Resident stepper state shared by the list-based (tsph_engine.hip) and the
+// cell-tile (tsph_tiles.hip) sweeps.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+
+#include "tsph_internal.hpp"
+#include "tsph_wcsph.hpp"
+
+namespace tsph {
+
+// device flag block
+struct SimFlags {
+  unsigned long long maxes[MM_NMAX];
+  unsigned long long escaped;
+  unsigned long long stale[2];      // slot k&1 = input of step k, slot (k+1)&1 = output
+  unsigned long long mins[MN_NMIN]; // ordered keys, atomicMin
+  unsigned long long lds_overflow;  // tiles that ran from HBM (block larger than the LDS budget)
+};
+
+// one entry of the host ring, packed on the device after every step and
+// copied to pinned memory in one transfer
+struct StepRecord {
+  unsigned long long stale_in, stale_out, escaped;
+  unsigned long long maxes[MM_NMAX];
+  unsigned long long mins[MN_NMIN];
+  double sums[MS_NSUM];
+};
+
+constexpr int kRing = 4;
+
+// one workgroup's tile: its cell-tile id and its fluid (or wall) rows
+struct TileRec {
+  int64_t tile;
+  int32_t start, end;
+};
+
+}  // namespace tsph
+
+struct tsph_sim {
+  tsph_sim_params prm{};
+  int dim = 2, prec = 0;
+  int64_t n = 0, nw = 0, nt = 0;
+  tsph_ctx* ctx = nullptr;
+  tsph::DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
+  tsph::DevBuf rho, p, pr2, vol, rho_raw, grad, acc, anchor;
+  tsph::DevBuf flags, block_sums, sums_out, rec_dev;
+  tsph::DevBuf old2new, new2old, fscan, cells_new;
+  tsph::StepRecord* ring = nullptr;       // pinned
+  hipEvent_t ev[tsph::kRing] = {};
+  // per-sweep timing (HIP events around each sweep on the stepping stream)
+  int timing = 0;
+  hipEvent_t tev[tsph::kRing][4] = {};
+  double sweep_ms[3] = {0, 0, 0};
+  int64_t sweep_count = 0;
+  int64_t pairs_2h = -1;
+  int64_t ring_step[tsph::kRing];
+  int cur = 0;
+  int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
+  bool built = false;
+  // cell-tile mode (2-D): rows sorted tile-major, dense per-cell row ranges
+  int tiles = 0;                 // 1: LDS cell-tile sweeps, 0: CSR lists
+  int tile_dim[3] = {4, 4, 1};
+  int64_t ext[3] = {1, 1, 1};
+  int64_t ntile[3] = {1, 1, 1};
+  int64_t ntiles_f = 0, ntiles_w = 0;
+  tsph::DevBuf ranges;           // int32 4 x ncells: fluid start/end, wall start/end
+  tsph::DevBuf tiles_f, tiles_w; // TileRec
+  tsph::DevBuf tkeys, tkeys_sorted, tidx, tflags;
+  int64_t lds_overflow = 0;
+  // host-side meter log
+  std::vector<double> rows;         // per recorded step: step, sums..., maxes..., mins...
+  double peak_rho = 0.0, peak_speed = 0.0;
+  size_t elt = 8;
+};
+
+
+namespace tsph {
+int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after);
+__global__ void k_reset_meters(SimFlags* fl, int slot_in);
+__global__ void k_reduce_sums(const double* part, int64_t nblocks, double* out, SimFlags* fl, int slot_in);
+template <int D, typename S> int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream);
+int tile_count_pairs(tsph_sim* sim, double radius, unsigned long long* cnt, hipStream_t stream);
+template <int D, typename S> int sim_queue_step_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream);
+}  // namespace tsph

@@ -157,6 +157,12 @@ def _sweep_times(self):
     return [ms[0], ms[1], ms[2]], steps.value
 
 
+def _configure(self, tiles: bool = True, tile=(8, 4)) -> None:
+    """Choose the sweep layout: LDS cell tiles (2-D) or CSR neighbour lists."""
+    _lib.check(_lib.lib().tsph_sim_configure(self.handle, 1 if tiles else 0, int(tile[0]),
+                                             int(tile[1])))
+
+
 def _count_pairs(self, radius: float) -> int:
     out = _i64(0)
     _lib.check(_lib.lib().tsph_sim_count_pairs(self.handle, float(radius), C.byref(out),
@@ -167,3 +173,4 @@ def _count_pairs(self, radius: float) -> int:
 WcsphSimulation.set_timing = _set_timing
 WcsphSimulation.sweep_times = _sweep_times
 WcsphSimulation.count_pairs = _count_pairs
+WcsphSimulation.configure = _configure

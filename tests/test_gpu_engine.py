@@ -16,7 +16,7 @@ from tests.gpu_util import dev, n, scaled_err, t
 pytestmark = pytest.mark.gpu
 
 
-def _impact_sim(precision, golden):
+def _impact_sim(precision, golden, layout="tiles"):
     from tensorsph_amd.cases import impact_state, domain_top
     from tensorsph_amd.engine import WcsphSimulation
     g = golden("run2d")
@@ -29,12 +29,17 @@ def _impact_sim(precision, golden):
                           viscosity=nu, dt=dt, wall_mass=float(mass[0]), skin=0.25 * h,
                           lo=[0.0, 0.0], hi=[cfg.pool_width, domain_top(cfg)],
                           tol=0.25 * cfg.dx, precision=precision)
+    if layout == "lists":
+        sim.configure(tiles=False)
+    elif layout != "tiles":
+        sim.configure(tiles=True, tile=layout)
     return sim, g, int(steps), mass
 
 
-def test_stepper_fp64_tracks_reference_run(golden):
+@pytest.mark.parametrize("layout", ["tiles", "lists", (4, 4), (16, 4)])
+def test_stepper_fp64_tracks_reference_run(golden, layout):
     from tensorsph_amd.engine import METER_KEYS
-    sim, g, steps, mass = _impact_sim("fp64", golden)
+    sim, g, steps, mass = _impact_sim("fp64", golden, layout)
     sim.step(steps, meter_every=1)
     rows = sim.meters()
     keys = list(g["row_keys"])
@@ -55,7 +60,8 @@ def test_stepper_fp64_tracks_reference_run(golden):
     assert stats["steps"] == steps and stats["rebuilds"] >= 1
 
 
-def test_stepper_matches_oracle_loop_on_dam_break():
+@pytest.mark.parametrize("layout", ["tiles", "lists"])
+def test_stepper_matches_oracle_loop_on_dam_break(layout):
     from tensorsph_amd.config import DamBreakConfig
     from tensorsph_amd.cases import make_column, wall_lattice, lattice_mass_unit
     from tensorsph_amd.engine import WcsphSimulation
@@ -69,6 +75,8 @@ def test_stepper_matches_oracle_loop_on_dam_break():
     sim = WcsphSimulation(pos, np.zeros_like(pos), mass, wall, sound_speed=cfg.sound_speed,
                           viscosity=cfg.viscosity, lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()],
                           tol=0.25 * cfg.dx, precision="fp64", **kw)
+    if layout == "lists":
+        sim.configure(tiles=False)
     ref = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, c0=cfg.sound_speed, nu=cfg.viscosity,
                         lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx, **kw)
     steps = 60
@@ -81,10 +89,11 @@ def test_stepper_matches_oracle_loop_on_dam_break():
     assert sim.stats()["rebuilds"] == ref.rebuilds
 
 
-def test_stepper_fp32_compensated_vs_fp64_and_determinism(golden):
-    a, g, steps, _ = _impact_sim("fp32", golden)
-    b, _, _, _ = _impact_sim("fp32", golden)
-    c, _, _, _ = _impact_sim("fp64", golden)
+@pytest.mark.parametrize("layout", ["tiles", "lists"])
+def test_stepper_fp32_compensated_vs_fp64_and_determinism(golden, layout):
+    a, g, steps, _ = _impact_sim("fp32", golden, layout)
+    b, _, _, _ = _impact_sim("fp32", golden, layout)
+    c, _, _, _ = _impact_sim("fp64", golden, layout)
     for s in (a, b, c):
         s.step(20)
     ra, rb, rc = a.read(), b.read(), c.read()
@@ -93,3 +102,47 @@ def test_stepper_fp32_compensated_vs_fp64_and_determinism(golden):
     assert scaled_err(n(ra["velocity"]), n(rc["velocity"])) < 1e-5
     assert scaled_err(n(ra["density"]), n(rc["density"])) < 1e-5
     assert np.abs(n(ra["position"]) - n(rc["position"])).max() < 1e-7
+
+
+def test_tiles_and_lists_agree_on_dam_break_fp32():
+    """Both sweep layouts visit the same pairs; in FP32 they may differ only by
+    summation order (within 1e-5 of each other after 30 steps)."""
+    from tensorsph_amd.config import DamBreakConfig
+    from tensorsph_amd.cases import make_column, wall_lattice, lattice_mass_unit
+    from tensorsph_amd.engine import WcsphSimulation
+    cfg = DamBreakConfig(dx=0.01, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.8, sound_speed=20.0, dt=1.5e-4, viscosity=1e-3)
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=cfg.tank_top())
+    h = cfg.h_factor * cfg.dx
+    out = []
+    for layout in ("tiles", "lists"):
+        sim = WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+                              sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+                              viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=lattice_mass_unit(cfg),
+                              skin=0.25 * h, lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()],
+                              tol=0.25 * cfg.dx, precision="fp32")
+        if layout == "lists":
+            sim.configure(tiles=False)
+        sim.step(30)
+        out.append(sim.read())
+    assert scaled_err(n(out[0]["velocity"]), n(out[1]["velocity"])) < 1e-5
+    assert np.abs(n(out[0]["position"]) - n(out[1]["position"])).max() < 1e-8
+
+
+@pytest.mark.parametrize("layout", ["tiles", "lists"])
+def test_pair_count_matches_oracle(golden, layout):
+    """Regression: the pair count walks the structure of the active layout."""
+    sim, g, steps, mass = _impact_sim("fp64", golden, layout)
+    sim.step(3)
+    st = sim.read()
+    from tensorsph_amd.cases import impact_state
+    dx = float(g["params"][0])
+    cfg, pos, _, wall = impact_state(dx=dx, pool_width=0.4, pool_depth=0.15, drop_radius=0.05,
+                                     drop_center_y=0.25)
+    h = cfg.h_factor * cfg.dx
+    allp = np.concatenate([n(st["position"]), wall])
+    pairs = orc.cell_linked_pairs(allp, 2 * h)
+    nf = pos.shape[0]
+    expected = int(((pairs[:, 0] < nf) | (pairs[:, 1] < nf)).sum())
+    assert sim.count_pairs(2 * h) == expected
