@@ -140,6 +140,88 @@ int tsph_apply_boundary(double* pos, double* vel, int64_t n, int dim, const doub
                         const double* hi, double tol, unsigned long long* count,
                         hipStream_t stream);
 
+/* ---------------------------------------------------------------- 3-D operators
+ * threed.operators3.accelerations (operators3.py:212-256) with
+ * summation_density (:70-79), tait_pressure (:82-86), wall_pressure_adami
+ * (:173-209) + tait_density (:89-93), pressure_acceleration (:96-113),
+ * viscous_acceleration with order 2 (:116-154) and density_diffusion_rate_3d
+ * (:259-295).  Every particle is a target: build the lists with
+ * tsph_nlist_from_pairs(..., all_targets = 1). */
+typedef struct tsph_threed_params {
+  int32_t precision;         /* TSPH_F64 or TSPH_F32 */
+  int64_t n, n_fluid;        /* all particles; fluid rows come first */
+  double h, rho0, sound_speed, gamma, viscosity;
+  double gravity[3];
+  int32_t has_gravity;       /* 0: gravity=None */
+  double box[3];
+  int32_t periodic;          /* 1: minimum-image pair deltas in box */
+  int32_t renormalise;       /* divide the viscous kernel by K_ij / 2 */
+  double diffusion_scale;    /* 2 delta c0 h; 0 disables density diffusion */
+  int32_t diffusion_form;    /* 0 Molteni-Colagrossi, 1 Antuono (renormalised) */
+  double eigenvalue_floor;
+  double dt;                 /* used by density diffusion only */
+  int32_t wall_model;        /* 0 "adami", 1 "tait" (walls keep their own Tait state) */
+} tsph_threed_params;
+
+/* pos, vel: (n, 3) FP64; mass (n) scalar.  Outputs: acc (n, 3), rho and p (n)
+ * in the scalar type (rho, p may be NULL).  Wall rows of acc are computed
+ * like fluid rows, as the reference does. */
+int tsph_threed_accelerations(tsph_ctx* ctx, const tsph_threed_params* prm, const double* pos,
+                              const double* vel, const void* mass, void* acc, void* rho, void* p,
+                              hipStream_t stream);
+
+/* One operator of threed.operators3 with the caller's rho / p (scalar type),
+ * on the same all-target lists:
+ *   op 0 summation_density (operators3.py:70-79)         -> out (n)
+ *   op 1 pressure_acceleration (:96-113), rho and p       -> out (n, 3)
+ *   op 2 viscous_acceleration, order 2 (:116-154), rho    -> out (n, 3)
+ *   op 3 wall_pressure_adami (:173-209), rho and p        -> out (n - n_fluid)
+ *   op 4 density_diffusion_rate_3d (:259-295), rho        -> out (n)
+ * gravity enters op 3 only; viscosity / renormalise enter op 2 only;
+ * diffusion_scale / diffusion_form / eigenvalue_floor enter op 4 only. */
+int tsph_threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const double* pos,
+                         const double* vel, const void* mass, const void* rho, const void* p,
+                         void* out, hipStream_t stream);
+
+/* ---------------------------------------------------------------- conservation ledger
+ * threed.conservation.diagnostics (conservation.py:106-121) and the shear-wave
+ * mode projections (operators3.py:401-408) as one FP64 row of
+ * TSPH_CONSERVATION_ROW host doubles:
+ *   [0] sum m, [1..3] momentum, [4..6] angular momentum about the origin,
+ *   [7] kinetic energy, [8] sum m e(rho) (Tait internal energy),
+ *   [9..11] sum m sin(2 pi x_d / L_d), [12..14] sum m cos(2 pi x_d / L_d)
+ *   (L = box, or 1 when box is NULL), [15] sum u_x sin(k z),
+ *   [16] sum u_x sin(k y) sin(k z), [17] sum m |v|_1, [18..20] sum m x_d.
+ * pos, vel: (n, 3) FP64; mass, rho: (n) in the scalar type of `precision`.
+ * Synchronises. */
+#define TSPH_CONSERVATION_ROW 21
+int tsph_conservation_row(tsph_ctx* ctx, const double* pos, const double* vel, const void* mass,
+                          const void* rho, int64_t n, int precision, double rho0, double sound_speed, double gamma,
+                          const double* box, double wavenumber, double* out, hipStream_t stream);
+
+/* ---------------------------------------------------------------- periodic stepper
+ * The step loop of threed.solver.run_shear_wave (solver.py:211-308) with the
+ * state resident in HBM: periodic_pairs at 2h + skin (neighbors3.py:71-142)
+ * kept until the largest minimum-image displacement exceeds skin / 2,
+ * accelerations as above, kick-drift-kick leapfrog with the positions wrapped
+ * by torch.remainder (:285-299).  prm->periodic must be 1 and n_fluid == n. */
+typedef struct tsph_psim tsph_psim;
+
+int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* pos,
+                     const double* vel, const void* mass, hipStream_t stream, tsph_psim** out);
+/* Advances nsteps leapfrog steps; synchronises once per step for the skin test. */
+int tsph_psim_step(tsph_psim* sim, int64_t nsteps, hipStream_t stream);
+/* The conservation row (TSPH_CONSERVATION_ROW doubles, see
+ * tsph_conservation_row) of the current state.  Synchronises. */
+int tsph_psim_diagnostics(tsph_psim* sim, double wavenumber, double* out, hipStream_t stream);
+/* State in the caller's order: pos, vel (n, 3) FP64; rho (n) and acc (n, 3)
+ * of the last evaluation in the scalar type.  Any pointer may be NULL. */
+int tsph_psim_read(tsph_psim* sim, double* pos, double* vel, void* rho, void* acc,
+                   hipStream_t stream);
+/* counters: steps done, list builds, list entries.  Host pointer. */
+int tsph_psim_stats(tsph_psim* sim, int64_t* counters);
+int tsph_psim_destroy(tsph_psim* sim);
+
 /* ---------------------------------------------------------------- resident stepper
  * The step loop of droplet_impact.run (droplet_impact.py:1674-1763) and
  * dam_break.run_case (dam_break.py:374-465): Verlet-skin pair cache

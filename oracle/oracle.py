@@ -214,6 +214,60 @@ def threed_accelerations(pos, vel, mass, h, pairs, *, rho0, sound_speed, gamma=7
     return acc, rho, p
 
 
+def ledger_row(pos, vel, mass, rho, *, rho0, sound_speed, gamma, box, wavenumber):
+    """Conservation row (threed/conservation.py:106-121) plus the shear-wave
+    projections (operators3.py:401-408): mass, momentum, angular momentum,
+    kinetic and Tait internal energy, sum u_x sin(kz), sum u_x sin(ky) sin(kz),
+    sum m |v|_1."""
+    m = mass[:, None]
+    ratio = rho / rho0
+    e = (sound_speed ** 2 / gamma) * ((ratio ** (gamma - 1.0) - 1.0) / (gamma - 1.0)
+                                      + 1.0 / ratio - 1.0)
+    return {"mass": float(mass.sum()), "momentum": (m * vel).sum(0),
+            "angular_momentum": (m * np.cross(pos, vel)).sum(0),
+            "kinetic_energy": float(0.5 * (m * vel ** 2).sum()),
+            "internal_energy": float((mass * e).sum()),
+            "mode1": float((vel[:, 0] * np.sin(wavenumber * pos[:, 2])).sum()),
+            "mode2": float((vel[:, 0] * np.sin(wavenumber * pos[:, 1])
+                            * np.sin(wavenumber * pos[:, 2])).sum()),
+            "absmv": float((m * np.abs(vel)).sum())}
+
+
+def shear_wave_loop(pos, vel, mass, *, h, box, rho0, sound_speed, gamma=7.0, nu, dt, steps,
+                    skin, renormalise=True, sample_every=10, wavenumber=0.0):
+    """The step loop of threed.solver.run_shear_wave (solver.py:254-308), FP64:
+    periodic pairs at 2h + skin, rebuilt when 2 max|min_image(x - x_anchor)| >
+    skin; kick-drift-kick with x wrapped by the floored remainder."""
+    pos = np.array(pos, np.float64); vel = np.array(vel, np.float64)
+    mass = np.asarray(mass, np.float64)
+    cutoff = 2.0 * h + skin
+    pairs = periodic_pairs(pos, box, cutoff)
+    anchor = pos.copy()
+    rebuilds = 1
+    kw = dict(rho0=rho0, sound_speed=sound_speed, gamma=gamma, viscosity=nu, box=box,
+              renormalise=renormalise)
+    acc, rho, _ = threed_accelerations(pos, vel, mass, h, pairs, **kw)
+    rows = []
+    for k in range(steps + 1):
+        if k % sample_every == 0:
+            rows.append(ledger_row(pos, vel, mass, rho, rho0=rho0, sound_speed=sound_speed,
+                                   gamma=gamma, box=box, wavenumber=wavenumber))
+        if k == steps:
+            break
+        vel = vel + 0.5 * dt * acc
+        pos = np.remainder(pos + dt * vel, box)
+        d = pos - anchor
+        d = d - box * np.round(d / box)
+        if 2.0 * float(np.sqrt((d * d).sum(1)).max()) > skin:
+            pairs = periodic_pairs(pos, box, cutoff)
+            anchor = pos.copy()
+            rebuilds += 1
+        acc, rho, _ = threed_accelerations(pos, vel, mass, h, pairs, **kw)
+        vel = vel + 0.5 * dt * acc
+    return {"pos": pos, "vel": vel, "rho": rho, "acc": acc, "rows": rows, "rebuilds": rebuilds,
+            "pairs_last": int(pairs.shape[0])}
+
+
 def apply_boundary(pos, vel, lo, hi, tol):
     pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
     dim = pos.shape[1]

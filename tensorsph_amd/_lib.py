@@ -106,6 +106,17 @@ def _declare(L):
         "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],
         "tsph_sim_destroy": [_vp],
+        "tsph_threed_accelerations": [_vp, C.POINTER(ThreedParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+        "tsph_threed_operator": [_vp, C.POINTER(ThreedParams), C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                 _vp],
+        "tsph_psim_create": [C.POINTER(ThreedParams), _dbl, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
+        "tsph_psim_step": [_vp, _i64, _vp],
+        "tsph_psim_diagnostics": [_vp, _dbl, _dp, _vp],
+        "tsph_psim_read": [_vp, _vp, _vp, _vp, _vp, _vp],
+        "tsph_psim_stats": [_vp, _ip],
+        "tsph_psim_destroy": [_vp],
+        "tsph_conservation_row": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dp, _dbl, _dp,
+                                  _vp],
     }
     for name, args in sigs.items():
         f = getattr(L, name)

@@ -40,6 +40,11 @@ template <> struct Sum<float> {
   TSPH_D float get() const { return s; }
 };
 
+// Value of an accumulator to FP64 accuracy.  For the Kahan sum that is s - c:
+// c holds the negative of the low-order bits the running sum s dropped.
+TSPH_D double hi(const Sum<double>& a) { return a.s; }
+TSPH_D double hi(const Sum<float>& a) { return (double)a.s - (double)a.c; }
+
 template <int D, typename S> struct VSum {
   Sum<S> v[D];
   TSPH_D void add(const S* x) {
@@ -56,6 +61,7 @@ template <int D, typename S> struct VSum {
 // radial Laplacian f'' + 2 f'/q with its q -> 0 limit -6.
 template <typename S> struct KernelConst {
   S h, inv_h, sigma, sigma_over_h, w0;
+  double w0d;   // W(0) in FP64, for the self term of the density
 };
 
 template <typename S> TSPH_D void cubic_wg(S r, const KernelConst<S>& k, S& w, S& g) {
@@ -117,6 +123,7 @@ inline KernelConst<S> make_cubic(double h, int dim) {
   double sigma = dim == 3 ? 1.0 / (kPi * h * h * h) : 10.0 / (7.0 * kPi * h * h);
   k.h = S(h); k.inv_h = S(1.0 / h); k.sigma = S(sigma); k.sigma_over_h = S(sigma / h);
   k.w0 = S(sigma);
+  k.w0d = sigma;
   return k;
 }
 
@@ -126,6 +133,7 @@ inline KernelConst<S> make_wendland2d(double h) {
   double sigma = 7.0 / (4.0 * kPi * h * h);
   k.h = S(h); k.inv_h = S(1.0 / h); k.sigma = S(sigma); k.sigma_over_h = S(sigma / h);
   k.w0 = S(sigma);
+  k.w0d = sigma;
   return k;
 }
 
@@ -146,6 +154,35 @@ TSPH_D S tait(S rho, S rho0, S bulk, S gamma, S ratio_floor) {
   if (ratio_floor > S(0) && ratio < ratio_floor) ratio = ratio_floor;
   S p = bulk * (pow_s<S>(ratio, gamma) - S(1));
   return p > S(0) ? p : S(0);
+}
+
+// Tait EOS of a density known to FP64 accuracy (rho from hi() plus the self
+// term).  FP64: the power form above.  FP32: eps = ratio - 1 is formed in FP64
+// and p = B expm1(gamma log1p(eps)) is evaluated in FP32, which is accurate
+// relative to p itself; forming ratio^gamma - 1 from an FP32-rounded density
+// instead cancels to an error of about gamma B 2^-24, far above p near rho0.
+template <typename S>
+TSPH_D S tait_hi(double rho, double rho0, S bulk, S gamma, S ratio_floor);
+template <>
+TSPH_D double tait_hi<double>(double rho, double rho0, double bulk, double gamma, double ratio_floor) {
+  return tait<double>(rho, rho0, bulk, gamma, ratio_floor);
+}
+template <>
+TSPH_D float tait_hi<float>(double rho, double rho0, float bulk, float gamma, float ratio_floor) {
+  double ratio = rho / rho0;
+  if (ratio_floor > 0.0f && ratio < (double)ratio_floor) ratio = (double)ratio_floor;
+  const float p = bulk * expm1f(gamma * log1pf((float)(ratio - 1.0)));
+  return p > 0.0f ? p : 0.0f;
+}
+
+// Stores an FP64 density as S and, when lo != NULL, the part S dropped, so a
+// later sweep can rebuild the FP64 value (lo[i] = 0 in FP64 mode).
+template <typename S>
+TSPH_D S store_rho(double rho_d, S* rho, S* lo, int64_t i) {
+  const S r = (S)rho_d;
+  rho[i] = r;
+  if (lo) lo[i] = (S)(rho_d - (double)r);
+  return r;
 }
 
 // ---------------------------------------------------------------- launch helpers

@@ -55,27 +55,26 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, const i
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
-  Sum<S> acc;
-  acc.add(a.mass[i] * a.k.w0);
+  Sum<double> acc;
   const int64_t e1 = off[i + 1];
   for (int64_t e = off[i]; e < e1; ++e) {
     const int64_t j = idx[e];
-    S r2 = S(0);
+    double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const S df = (S)(xi[d] - a.pos[j * D + d]);
+      const double df = xi[d] - a.pos[j * D + d];
       r2 += df * df;
     }
-    if (r2 > a.skip2) continue;
-    S w, g;
-    cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
-    acc.add(a.mass[j] * w);
+    if (r2 > a.skip2d) continue;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    acc.add((double)a.mass[j] * w);
   }
-  S rho = acc.get();
-  rho = rho > a.rho_floor ? rho : a.rho_floor;
-  a.rho[i] = rho;
+  double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
   if (write_eos) {
-    const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+    const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
     a.p[i] = p;
     a.pr2[i] = p / (rho * rho);
     a.vol[i] = a.mass[i] / rho;
@@ -123,9 +122,10 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const
     }
     rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * (contrast * radial / (r2 + S(1e-12))));
   }
-  S rho = rhoi + a.dt * rate.get();
-  rho = rho > a.rho_floor ? rho : a.rho_floor;
-  const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+  double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)a.dt * hi(rate);
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = (S)rho_d;
+  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
   a.rho[i] = rho;
   a.p[i] = p;
   a.pr2[i] = p / (rho * rho);
@@ -629,6 +629,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const int r = (int)(k % kRing);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
   if (sim->prm.op.diffusion_scale > 0.0) {
+    if (sizeof(S) == 4) a.rho_lo = sim->rho_lo.as<S>();
     WcsphArgs<S> a0 = a;
     a0.rho = sim->rho_raw.as<S>();
     k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a0, off, idx, fl, slot, 0);
@@ -806,7 +807,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->mass, e * nt); ENS(sim->mass2, e * nt);
   ENS(sim->orig, 8 * nt); ENS(sim->orig2, 8 * nt);
   ENS(sim->rho, e * nt); ENS(sim->p, e * nt); ENS(sim->pr2, e * nt); ENS(sim->vol, e * nt);
-  ENS(sim->rho_raw, e * nt); ENS(sim->grad, e * n * D); ENS(sim->acc, e * n * D);
+  ENS(sim->rho_raw, e * nt); ENS(sim->rho_lo, e * nt); ENS(sim->grad, e * n * D); ENS(sim->acc, e * n * D);
   ENS(sim->anchor, 8 * n * D);
   ENS(sim->flags, sizeof(SimFlags));
   ENS(sim->block_sums, 8 * MS_NSUM * (int64_t)grid_xcd(n));
@@ -954,7 +955,7 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
 int tsph_sim_destroy(tsph_sim* sim) {
   if (!sim) return TSPH_OK;
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
-                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw,
+                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new};
   for (DevBuf* b : bufs) b->release();

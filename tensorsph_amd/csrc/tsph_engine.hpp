@@ -43,7 +43,7 @@ struct tsph_sim {
   int64_t n = 0, nw = 0, nt = 0;
   tsph_ctx* ctx = nullptr;
   tsph::DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
-  tsph::DevBuf rho, p, pr2, vol, rho_raw, grad, acc, anchor;
+  tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
   tsph::StepRecord* ring = nullptr;       // pinned

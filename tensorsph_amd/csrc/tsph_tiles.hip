@@ -201,8 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_density(WcsphArgs<S> a, TileGeo
   for (int64_t r = tr.start + threadIdx.x; r < tr.end; r += blockDim.x) {
     const double xi0 = a.pos[2 * r], xi1 = a.pos[2 * r + 1];
     const int bx = (int)(cells[2 * r] - bx0), by = (int)(cells[2 * r + 1] - by0);
-    Sum<S> acc;
-    acc.add(a.mass[r] * a.k.w0);
+    Sum<double> acc;
     auto body = [&](int64_t q) {
       int64_t j;
       double xj0, xj1;
@@ -210,19 +209,19 @@ __global__ void __launch_bounds__(kBlock) k_tile_density(WcsphArgs<S> a, TileGeo
       if (lds) { j = srow[q]; xj0 = sx[q].x; xj1 = sx[q].y; mj = sm[q]; }
       else { j = q; xj0 = a.pos[2 * q]; xj1 = a.pos[2 * q + 1]; mj = a.mass[q]; }
       if (j == r) return;
-      const S d0 = (S)(xi0 - xj0), d1 = (S)(xi1 - xj1);
-      const S r2 = d0 * d0 + d1 * d1;
-      if (r2 > a.skip2) return;
-      S w, gg;
-      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, gg);
-      acc.add(mj * w);
+      const double d0 = xi0 - xj0, d1 = xi1 - xj1;
+      const double r2 = d0 * d0 + d1 * d1;
+      if (r2 > a.skip2d) return;
+      double w, gg;
+      cubic_wg<double>(sqrt(r2), a.kd, w, gg);
+      acc.add((double)mj * w);
     };
     bool dense = false;
     if (lds) {
       int cnt = 0;
       visit_stencil(bt, BW, bx, by, true, [&](int64_t q) {
-        const S d0 = (S)(xi0 - sx[q].x), d1 = (S)(xi1 - sx[q].y);
-        if (d0 * d0 + d1 * d1 > a.skip2 || srow[q] == r) return;
+        const double d0 = xi0 - sx[q].x, d1 = xi1 - sx[q].y;
+        if (d0 * d0 + d1 * d1 > a.skip2d || srow[q] == r) return;
         if (cnt < kNbMax) nb[cnt * kBlock + threadIdx.x] = (unsigned short)q;
         ++cnt;
       });
@@ -232,11 +231,11 @@ __global__ void __launch_bounds__(kBlock) k_tile_density(WcsphArgs<S> a, TileGeo
       }
     }
     if (!dense) visit_stencil(bt, BW, bx, by, lds, body);
-    S rho = acc.get();
-    rho = rho > a.rho_floor ? rho : a.rho_floor;
-    a.rho[r] = rho;
+    double rho_d = hi(acc) + (double)a.mass[r] * a.kd.w0d;
+    rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+    const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, r);
     if (write_eos) {
-      const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+      const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
       a.p[r] = p;
       a.pr2[r] = p / (rho * rho);
       a.vol[r] = a.mass[r] / rho;

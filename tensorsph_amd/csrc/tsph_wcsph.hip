@@ -31,27 +31,26 @@ __global__ void __launch_bounds__(kBlock) k_density(WcsphArgs<S> a, const int64_
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
-  Sum<S> acc;
-  acc.add(a.mass[i] * a.k.w0);
+  Sum<double> acc;
   const int64_t e0 = off[i], e1 = off[i + 1];
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t j = idx[e];
-    S r2 = S(0);
+    double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const S df = (S)(xi[d] - a.pos[j * D + d]);
+      const double df = xi[d] - a.pos[j * D + d];
       r2 += df * df;
     }
-    if (r2 > a.skip2) continue;
-    S w, g;
-    cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
-    acc.add(a.mass[j] * w);
+    if (r2 > a.skip2d) continue;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    acc.add((double)a.mass[j] * w);
   }
-  S rho = acc.get();
-  rho = rho > a.rho_floor ? rho : a.rho_floor;
-  a.rho[i] = rho;
+  double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
   if (write_eos) {
-    const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+    const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
     a.p[i] = p;
     a.pr2[i] = p / (rho * rho);
     a.vol[i] = a.mass[i] / rho;
@@ -152,9 +151,10 @@ __global__ void __launch_bounds__(kBlock) k_diffusion(WcsphArgs<S> a, const int6
     const S coupling = contrast * radial / (r2 + S(1e-12));
     rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * coupling);
   }
-  S rho = rhoi + a.dt * rate.get();
-  rho = rho > a.rho_floor ? rho : a.rho_floor;
-  const S p = tait<S>(rho, a.rho0, a.bulk, a.gamma, a.ratio_floor);
+  double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)a.dt * hi(rate);
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = (S)rho_d;
+  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
   a.rho[i] = rho;
   a.p[i] = p;
   a.pr2[i] = p / (rho * rho);
@@ -321,6 +321,11 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
     if (!diff) {
       k_density<D, S><<<grid_xcd(n), kBlock, 0, stream>>>(a, off, idx, 1);
     } else {
+      if (sizeof(S) == 4) {
+        // low part of the raw FP32 density, so diffusion + EOS see it in FP64
+        TSPH_TRY(ctx->s_aux3.ensure(sizeof(S) * (nt + 1)));
+        a.rho_lo = ctx->s_aux3.as<S>();
+      }
       WcsphArgs<S> a0 = a;
       a0.rho = ctx->s_rho_raw.as<S>();
       k_density<D, S><<<grid_xcd(n), kBlock, 0, stream>>>(a0, off, idx, 0);

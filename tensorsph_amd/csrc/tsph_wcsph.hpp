@@ -10,8 +10,11 @@ namespace tsph {
 
 template <typename S> struct WcsphArgs {
   KernelConst<S> k;
+  KernelConst<double> kd;   // the density sweep evaluates W in FP64 in both precisions
   S skip2;            // squared support with a relative margin: beyond it W = grad W = 0
+  double skip2d;
   S rho0, bulk, gamma, inv_gamma, ratio_floor, rho_floor;
+  double rho0d;       // rho0 in FP64: the EOS forms rho / rho0 - 1 there
   S gravity, nu2, friction, eta2, dt, diff_scale, eig_floor;
   int64_t n, nw;
   const double* pos;  // (n + nw, D)
@@ -22,19 +25,23 @@ template <typename S> struct WcsphArgs {
   S* p;
   S* pr2;             // p / rho^2
   S* vol;             // m / rho
+  S* rho_lo;          // optional: low part of the stored density (FP32 mode)
 };
 
 template <typename S>
 inline WcsphArgs<S> make_wcsph_args(const tsph_wcsph_params& p, int dim) {
   WcsphArgs<S> a{};
   a.k = make_cubic<S>(p.h, dim);
+  a.kd = make_cubic<double>(p.h, dim);
   a.skip2 = S(4.0 * p.h * p.h * (1.0 + 1e-5));
+  a.skip2d = 4.0 * p.h * p.h * (1.0 + 1e-5);
   a.rho0 = S(p.rho0);
   a.bulk = S(p.rho0 * (p.sound_speed * p.sound_speed) / p.gamma);
   a.gamma = S(p.gamma);
   a.inv_gamma = S(1.0 / p.gamma);
   a.ratio_floor = S(0.01);
   a.rho_floor = S(0.1 * p.rho0);
+  a.rho0d = p.rho0;
   a.gravity = S(p.gravity);
   a.nu2 = S(2.0 * p.viscosity);
   // with walls present and no friction, fluid-wall pairs carry no viscosity

@@ -331,5 +331,32 @@ def main(reference: str = "/root/reference") -> None:
     print("golden fixtures written to", HERE)
 
 
+def shear(reference: str = "/root/reference") -> None:
+    """run_shear_wave (threed/solver.py:211-308) at n = 10 with an amplitude
+    large enough that the skin test rebuilds the pair list inside the run."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    from tensorsph.threed.solver import run_shear_wave
+    res = run_shear_wave(n=10, amplitude=0.5, steps=60, sample_every=5)
+    keys = ("mass", "kinetic_energy", "internal_energy", "total_energy", "momentum_magnitude")
+    rows = np.array([[d[k] for k in keys] + list(d["momentum"]) + list(d["angular_momentum"])
+                     + list(d["centre_of_mass"]) for d in res.diagnostics])
+    np.savez_compressed(HERE / "shear.npz", times=np.array(res.times),
+                        amplitudes=np.array(res.amplitudes), rows=rows,
+                        row_keys=np.array(list(keys) + ["px", "py", "pz", "lx", "ly", "lz",
+                                                        "cx", "cy", "cz"]),
+                        pairs_last=np.int64(res.pairs_last),
+                        pair_rebuilds=np.int64(res.pair_rebuilds),
+                        momentum_scale=np.float64(res.momentum_scale),
+                        fit_rate=np.float64(res.fit_rate),
+                        params=np.array([res.n, res.dx, res.h, res.nu, res.wavenumber,
+                                         res.sound_speed, res.amplitude0, res.dt, res.steps]))
+    print("shear fixture written to", HERE)
+
+
 if __name__ == "__main__":
-    main(*sys.argv[1:])
+    args = sys.argv[1:]
+    if args and args[0] == "--only-shear":
+        shear(*args[1:])
+    else:
+        main(*args)
+        shear(*args)

@@ -373,3 +373,61 @@ def test_delta_plus_elliptical_drop_published_errors():
     err = stretch / np.array([1.464774, 0.682699]) - 1
     assert abs(err[0] - (-0.00660)) < 5e-5
     assert abs(err[1] - (-0.00037)) < 5e-5
+
+
+# ----------------------------------------------------------------- 3-D shear wave (A14, A16, A17)
+
+def _shear_state(n=10, amplitude=0.5, mode="double", jitter=0.0, seed=20261203):
+    from tensorsph_amd.cases import calibrated_particle_mass_3d, cubic_lattice
+    dx = 1.0 / n
+    h = 1.3 * dx
+    k = 2.0 * math.pi
+    pos = cubic_lattice((n, n, n), dx)
+    if jitter:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = np.remainder(pos + rng.uniform(-jitter * dx, jitter * dx, pos.shape), 1.0)
+    vel = np.zeros_like(pos)
+    vel[:, 0] = amplitude * (np.sin(k * pos[:, 1]) * np.sin(k * pos[:, 2]) if mode == "double"
+                             else np.sin(k * pos[:, 2]))
+    mass = np.full(pos.shape[0], calibrated_particle_mass_3d(dx, h, 1000.0))
+    return pos, vel, mass, h, k
+
+
+def test_shear_wave_loop_matches_reference(golden):
+    """The oracle's run_shear_wave loop against the reference's own run at
+    n = 10 (amplitude series, energies, momentum, list builds, pair count)."""
+    g = golden("shear")
+    n, dx, h, nu, k, c0, amp, dt, steps = g["params"]
+    pos, vel, mass, h2, k2 = _shear_state(int(n), amp)
+    assert h2 == h and k2 == k
+    out = orc.shear_wave_loop(pos, vel, mass, h=h, box=1.0, rho0=1000.0, sound_speed=c0, nu=nu,
+                              dt=dt, steps=int(steps), skin=2.0 * h * 1.25 - 2.0 * h,
+                              sample_every=5, wavenumber=k)
+    amps = np.array([4.0 * r["mode2"] / pos.shape[0] for r in out["rows"]])
+    assert np.abs(amps - g["amplitudes"]).max() < 1e-12 * amp
+    keys = list(g["row_keys"])
+    ke = np.array([r["kinetic_energy"] for r in out["rows"]])
+    ie = np.array([r["internal_energy"] for r in out["rows"]])
+    assert np.abs(ke - g["rows"][:, keys.index("kinetic_energy")]).max() < 1e-11 * ke[0]
+    assert np.abs(ie - g["rows"][:, keys.index("internal_energy")]).max() < 1e-9 * ke[0]
+    mom = np.array([r["momentum"] for r in out["rows"]])
+    assert np.abs(mom).max() < 1e-12 * float(g["momentum_scale"])
+    assert out["rebuilds"] == int(g["pair_rebuilds"]) and out["pairs_last"] == int(g["pairs_last"])
+
+
+def test_ledger_row_closed_forms():
+    """Internal energy vanishes at rho0 and matches a Taylor expansion
+    (B/rho0^2) drho^2 / 2 per unit mass for small drho; angular momentum of a
+    rigid rotation is I omega."""
+    rho0, c0, gamma = 1000.0, 10.0, 7.0
+    pos = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0]])
+    omega = 0.3
+    vel = np.cross(np.array([0.0, 0.0, omega]), pos)
+    mass = np.array([2.0, 3.0])
+    rho = np.array([rho0, rho0 * (1 + 1e-4)])
+    r = orc.ledger_row(pos, vel, mass, rho, rho0=rho0, sound_speed=c0, gamma=gamma, box=1.0,
+                       wavenumber=1.0)
+    inertia = 2.0 * 1.0 + 3.0 * 4.0
+    assert abs(r["angular_momentum"][2] - inertia * omega) < 1e-14
+    e_small = 3.0 * (c0 ** 2) * 0.5 * 1e-8       # m c0^2 (drho/rho0)^2 / 2
+    assert abs(r["internal_energy"] - e_small) < 1e-3 * e_small
