@@ -23,6 +23,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 
@@ -30,6 +31,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(HERE, "profiles", "k_tile_force_dambreak_1p6M_fp32c.md")
+
+
+def profiled_traffic(nf: int):
+    """HBM bytes per force-sweep launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, when it was taken on this workload; else None."""
+    try:
+        with open(PROFILE) as f:
+            text = f.read()
+    except OSError:
+        return None
+    if f"{nf:,} fluid" not in text:
+        return None
+    m = re.search(r"traffic_bytes_per_launch k_tile_force = (\d+)", text)
+    return int(m.group(1)) if m else None
 
 
 def parse():
@@ -41,6 +57,8 @@ def parse():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--tile", default=None,
+                    help="fixed tile shape TXxTY (e.g. 8x6) instead of the autotuner's choice")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -71,12 +89,17 @@ def make_sim(cfg, pos, mass, wall, wmass, precision):
                            precision=precision)
 
 
-def time_gpu(sim, steps, warmup):
+def time_gpu(sim, steps, warmup, tile=None):
     import torch
     from tensorsph_amd.autotune import tune_tiles
-    sim.step(max(warmup // 2, 1))
-    tuned = tune_tiles(sim, steps=4, warm=1)     # trial steps count as warm-up
-    sim.step(max(warmup - warmup // 2, 1))
+    if tile is not None:
+        sim.configure(tiles=True, tile=tile)
+        tuned = {"best": tuple(tile), "ms": {}}
+        sim.step(max(warmup, 1))
+    else:
+        sim.step(max(warmup // 2, 1))
+        tuned = tune_tiles(sim, steps=4, warm=1)     # trial steps count as warm-up
+        sim.step(max(warmup - warmup // 2, 1))
     torch.cuda.synchronize()
     sim.set_timing(True)
     t0 = time.perf_counter()
@@ -135,8 +158,9 @@ def main():
     nf, nw = pos.shape[0], wall.shape[0]
     h = cfg.h_factor * cfg.dx
 
+    tile = None if args.tile is None else tuple(int(v) for v in args.tile.lower().split("x"))
     sim = make_sim(cfg, pos, mass, wall, wmass, "fp32")
-    wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup)
+    wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup, tile)
     pairs = sim.count_pairs(2.0 * h)
     stats = sim.stats()
     sim.close()
@@ -182,13 +206,14 @@ def main():
                               "wall": sweep_ms[1] / max(nsteps, 1),
                               "force_integrate": force_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(nf),
+                     "traffic_source": os.path.relpath(PROFILE, HERE),
                      "kernel": "k_tile_force (force + viscosity + integrate)",
                      "algorithmic_bytes_per_launch": b_force},
     }
     if not args.no_fp64:
         sim = make_sim(cfg, pos, mass, wall, wmass, "fp64")
-        w64, sw64, n64, tuned64 = time_gpu(sim, args.steps, args.warmup)
+        w64, sw64, n64, tuned64 = time_gpu(sim, args.steps, args.warmup, tile)
         sim.close()
         del sim
         v64 = nf * args.steps / w64
