@@ -183,6 +183,28 @@ int tsph_threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, c
                          const double* vel, const void* mass, const void* rho, const void* p,
                          void* out, hipStream_t stream);
 
+/* ---------------------------------------------------------------- delta-plus step
+ * delta_plus.delta_plus_step (delta_plus.py:105-229): one 2-D δ⁺-SPH step,
+ * FP64, on the lists of tsph_nlist_from_pairs(pairs, n_fluid = n,
+ * n_total = n + n_wall, all_targets = 0) built from the pair list over
+ * cat(pos, wall) (skin pairs allowed; the strict d^2 < (2h)^2 test, h = 2 dx,
+ * is applied per pair).  pos, vel (n, 2); rho, mass (n); wall, wall_vel,
+ * wall_normal (n_wall, 2) with unit normals.  Outputs: pos_out, vel_out,
+ * acc_out, shift (n, 2); rho_out (n); reaction (n_wall, 2).  Synchronises and
+ * returns TSPH_ERR_NUMERIC for a nonpositive wall density or a nonpositive /
+ * nonfinite new density, as the reference raises FloatingPointError. */
+typedef struct tsph_dp_params {
+  int32_t precision;         /* TSPH_F64 */
+  int64_t n, n_wall;
+  double dx, sound_speed, reference_speed, dt, rho0, gravity, viscosity, diffusion, alpha;
+} tsph_dp_params;
+
+int tsph_deltaplus_step(tsph_ctx* ctx, const tsph_dp_params* prm, const double* pos,
+                        const double* vel, const double* rho, const double* mass,
+                        const double* wall, const double* wall_vel, const double* wall_normal,
+                        double* pos_out, double* vel_out, double* rho_out, double* acc_out,
+                        double* reaction, double* shift, hipStream_t stream);
+
 /* ---------------------------------------------------------------- conservation ledger
  * threed.conservation.diagnostics (conservation.py:106-121) and the shear-wave
  * mode projections (operators3.py:401-408) as one FP64 row of

@@ -115,6 +115,7 @@ def _declare(L):
         "tsph_psim_read": [_vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_psim_stats": [_vp, _ip],
         "tsph_psim_destroy": [_vp],
+        "tsph_deltaplus_step": [_vp, C.POINTER(DeltaPlusParams)] + [_vp] * 14,
         "tsph_conservation_row": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dp, _dbl, _dp,
                                   _vp],
     }

@@ -1,0 +1,35 @@
+// δ⁺-SPH step: per-simulation parameters and the argument block of the four
+// sweeps (tsph_deltaplus.hip), shared with the batch engine.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tsph {
+
+// Parameters of one simulation (delta_plus.py:105-130): h = 2 dx,
+// rv = 8 nu + alpha h c0 (radial viscosity), diff_scale = 2 delta h c0.
+struct DpSim {
+  double dx, h, c0, dt, rho0, g, rv, diff_scale, ref_speed;
+};
+
+struct DpArgs {
+  int64_t nf, nw;                  // fluid rows, wall stations (all simulations)
+  DpSim sim0;                      // used when sims == NULL (one simulation)
+  const DpSim* sims;               // per-simulation table (device) or NULL
+  const int32_t* fsim;             // simulation of each fluid row, NULL: all 0
+  const int32_t* wsim;             // simulation of each wall station, NULL: all 0
+  const unsigned char* active;     // per simulation: 0 skips it this step, NULL: all active
+  int* status;                     // per simulation: bit 0 wall density <= 0, bit 1 bad density
+  const double *x, *v, *rho, *m;   // fluid state (FP64, (nf, 2) / (nf))
+  const double *xw, *vw, *nw_;     // wall stations: position, velocity, unit normal
+  double *wall_p, *wall_rho, *reaction;
+  double *acc, *vnew, *lam, *inv, *gr, *rho_new, *normal, *strength, *shift, *x_new;
+  unsigned char* surface;
+};
+
+int dp_queue_step(const DpArgs& a, const int64_t* off, const int32_t* idx, const int64_t* woff,
+                  const int32_t* widx, hipStream_t stream);
+DpSim dp_sim_params(double dx, double c0, double ref_speed, double dt, double rho0, double g,
+                    double nu, double diffusion, double alpha);
+
+}  // namespace tsph

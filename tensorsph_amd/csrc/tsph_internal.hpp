@@ -118,4 +118,5 @@ struct tsph_ctx {
   // operator scratch
   tsph::DevBuf s_rho_raw, s_rho, s_p, s_aux, s_aux2, s_aux3, s_gr, s_flags;
   tsph::DevBuf s_red;        // reductions
+  tsph::DevBuf s_dp;         // delta-plus step intermediates
 };
