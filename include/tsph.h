@@ -205,6 +205,56 @@ int tsph_deltaplus_step(tsph_ctx* ctx, const tsph_dp_params* prm, const double* 
                         double* pos_out, double* vel_out, double* rho_out, double* acc_out,
                         double* reaction, double* shift, hipStream_t stream);
 
+/* ---------------------------------------------------------------- served batch of delta-plus runs
+ * No counterpart in the reference (which runs one simulation per Python
+ * loop): the batch engine that advances many independent δ⁺-SPH runs, each
+ * the loop of benchmarks/delta_plus/elliptical_drop.py:34-57 (adaptive
+ * dt = courant 2 dx / (c0 + max|v|), fresh cell lists every step) with the
+ * step of delta_plus.py:105-229, packed together in HBM.  Capacities are
+ * fixed at creation; one captured step graph advances every active run, so
+ * runs can be admitted and released between steps.  A run's results are
+ * bitwise the same alone and in any mix.  Calls on one batch must not
+ * overlap; the stream must not be the legacy default stream (graph capture). */
+typedef struct tsph_batch tsph_batch;
+
+typedef struct tsph_batch_caps {
+  int64_t fluid, wall, sims;       /* rows and run slots */
+  int64_t cells;                   /* cells of all per-run grids together (cell edge 4 dx) */
+  int64_t fluid_entries, wall_entries;  /* neighbour-list entries */
+  int32_t sorted_lists;            /* 1: lists in ascending row order (the order of
+                                      tsph_deltaplus_step, for bitwise checks); 0: cell order */
+} tsph_batch_caps;
+
+typedef struct tsph_dp_case {
+  int64_t n, n_wall, steps;        /* fluid rows, wall stations, steps to run */
+  double dx, sound_speed, reference_speed, rho0, gravity, viscosity, diffusion, alpha;
+  double courant;                  /* adaptive step factor, <= 0.3 */
+} tsph_dp_case;
+
+int tsph_batch_create(const tsph_batch_caps* caps, hipStream_t stream, tsph_batch** out);
+/* Copies one run's initial state (device pointers, layouts as for
+ * tsph_deltaplus_step) into free capacity.  *slot receives its id.  Returns
+ * TSPH_ERR_OOM when no slot or no contiguous room is free (a scheduler
+ * retries after a release).  Synchronises. */
+int tsph_batch_admit(tsph_batch* b, const tsph_dp_case* c, const double* pos, const double* vel,
+                     const double* rho, const double* mass, const double* wall,
+                     const double* wall_vel, const double* wall_normal, int32_t* slot,
+                     hipStream_t stream);
+/* Advances every active run by nsteps steps (runs stop at their own step
+ * count); synchronises at the end. */
+int tsph_batch_step(tsph_batch* b, int64_t nsteps, hipStream_t stream);
+/* Finished runs since their admission: slots and states (2 done, 3 failed:
+ * a nonpositive wall or fluid density, the reference's FloatingPointError). */
+int tsph_batch_poll(tsph_batch* b, int32_t* slots, int32_t* states, int32_t max, int32_t* n);
+/* A run's state: pos, vel (n, 2), rho (n) device pointers (may be NULL);
+ * *steps the steps it has taken. */
+int tsph_batch_read(tsph_batch* b, int32_t slot, double* pos, double* vel, double* rho,
+                    int64_t* steps, hipStream_t stream);
+int tsph_batch_release(tsph_batch* b, int32_t slot, hipStream_t stream);
+/* counters: batch steps, resident runs, resident fluid rows, wall rows */
+int tsph_batch_stats(tsph_batch* b, int64_t* counters);
+int tsph_batch_destroy(tsph_batch* b);
+
 /* ---------------------------------------------------------------- conservation ledger
  * threed.conservation.diagnostics (conservation.py:106-121) and the shear-wave
  * mode projections (operators3.py:401-408) as one FP64 row of

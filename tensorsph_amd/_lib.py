@@ -71,6 +71,17 @@ class DeltaPlusParams(C.Structure):
                 ("alpha", _dbl)]
 
 
+class BatchCaps(C.Structure):
+    _fields_ = [("fluid", _i64), ("wall", _i64), ("sims", _i64), ("cells", _i64),
+                ("fluid_entries", _i64), ("wall_entries", _i64), ("sorted_lists", C.c_int32)]
+
+
+class DpCase(C.Structure):
+    _fields_ = [("n", _i64), ("n_wall", _i64), ("steps", _i64), ("dx", _dbl), ("sound_speed", _dbl),
+                ("reference_speed", _dbl), ("rho0", _dbl), ("gravity", _dbl), ("viscosity", _dbl),
+                ("diffusion", _dbl), ("alpha", _dbl), ("courant", _dbl)]
+
+
 class SimParams(C.Structure):
     _fields_ = [("op", WcsphParams), ("skin", _dbl), ("lo", _dbl * 3), ("hi", _dbl * 3),
                 ("tol", _dbl), ("clamp", C.c_int32)]
@@ -116,6 +127,15 @@ def _declare(L):
         "tsph_psim_stats": [_vp, _ip],
         "tsph_psim_destroy": [_vp],
         "tsph_deltaplus_step": [_vp, C.POINTER(DeltaPlusParams)] + [_vp] * 14,
+        "tsph_batch_create": [C.POINTER(BatchCaps), _vp, C.POINTER(_vp)],
+        "tsph_batch_admit": [_vp, C.POINTER(DpCase)] + [_vp] * 7 + [C.POINTER(C.c_int32), _vp],
+        "tsph_batch_step": [_vp, _i64, _vp],
+        "tsph_batch_poll": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
+                            C.POINTER(C.c_int32)],
+        "tsph_batch_read": [_vp, C.c_int32, _vp, _vp, _vp, C.POINTER(_i64), _vp],
+        "tsph_batch_release": [_vp, C.c_int32, _vp],
+        "tsph_batch_stats": [_vp, _ip],
+        "tsph_batch_destroy": [_vp],
         "tsph_conservation_row": [_vp, _vp, _vp, _vp, _vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dp, _dbl, _dp,
                                   _vp],
     }

@@ -105,8 +105,9 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.nw) return;
   const int s = dp_sim(a.wsim, w);
-  const DpSim P = a.sims ? a.sims[s] : a.sim0;
+  if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
+  const DpSim P = a.sims ? a.sims[s] : a.sim0;
   const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xw = a.xw[2 * w], yw = a.xw[2 * w + 1];
   double psum = 0.0, wsum = 0.0;
@@ -156,8 +157,9 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
-  const DpSim P = a.sims ? a.sims[s] : a.sim0;
+  if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
+  const DpSim P = a.sims ? a.sims[s] : a.sim0;
   const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xt = a.x[2 * t], yt = a.x[2 * t + 1];
   const double vtx = a.v[2 * t], vty = a.v[2 * t + 1];
@@ -240,8 +242,9 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
-  const DpSim P = a.sims ? a.sims[s] : a.sim0;
+  if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
+  const DpSim P = a.sims ? a.sims[s] : a.sim0;
   const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xt = a.x[2 * t], yt = a.x[2 * t + 1];
   const double ut = a.vnew[2 * t], wt_ = a.vnew[2 * t + 1];
@@ -312,8 +315,9 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
-  const DpSim P = a.sims ? a.sims[s] : a.sim0;
+  if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
+  const DpSim P = a.sims ? a.sims[s] : a.sim0;
   const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
   double wref, sref;
   dp_kernel(P.dx, P.h, sigma, wref, sref);

@@ -1,0 +1,324 @@
+"""Served δ⁺-SPH simulations: continuous batching on one GPU.
+
+``BatchEngine`` wraps libtsph's batch engine (tsph_batch_*): simulations of
+uneven size are admitted by their row footprint into fixed-capacity HBM
+arrays and advanced together, one captured step graph per step; finished
+ones are read back and released, and waiting ones take their room.
+
+``serve`` runs a request stream against it: requests arrive at seeded Poisson
+times, are admitted as slots and rows free up, and every batch step's
+latency is recorded.  ``drop_case`` / ``impact_case`` build the two δ⁺ cases
+of the served workload: a free elliptical drop (benchmarks/delta_plus/
+elliptical_drop.py:34-48, generalised to radius R and strain rate A) and a
+drop falling onto a pool in a walled tank (hydrostatic pool as in
+spheric19.py:124-131, 4 wall layers with floor / side normals).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+
+STATE_ACTIVE, STATE_DONE, STATE_FAILED = 1, 2, 3
+
+
+@dataclass
+class DpCase:
+    """One δ⁺ simulation request: initial state (numpy, FP64) and parameters."""
+    pos: np.ndarray
+    vel: np.ndarray
+    rho: np.ndarray
+    mass: np.ndarray
+    wall: np.ndarray
+    wall_vel: np.ndarray
+    wall_normal: np.ndarray
+    dx: float
+    sound_speed: float
+    reference_speed: float
+    steps: int
+    rho0: float = 1000.0
+    gravity: float = 9.81
+    viscosity: float = 1e-6
+    diffusion: float = 0.1
+    alpha: float = 0.02
+    courant: float = 0.2
+    kind: str = ""
+
+    @property
+    def n(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def n_wall(self) -> int:
+        return int(self.wall.shape[0])
+
+
+def _lattice(nx: int, ny: int, dx: float, x0: float = 0.0, y0: float = 0.0) -> np.ndarray:
+    ax = (np.arange(nx) + 0.5) * dx + x0
+    ay = (np.arange(ny) + 0.5) * dx + y0
+    return np.stack(np.meshgrid(ax, ay, indexing="ij"), -1).reshape(-1, 2)
+
+
+def drop_case(n_target: int, radius: float, strain: float, viscosity: float, c_mult: float,
+              steps: int) -> DpCase:
+    """Free elliptical drop: u = A (x, -y), p = rho0 A^2 (R^2 - r^2) / 2."""
+    dx = radius * math.sqrt(math.pi / n_target)
+    cells = int(math.ceil(radius / dx))
+    pts = _lattice(2 * cells, 2 * cells, dx, -cells * dx, -cells * dx)
+    r2 = (pts ** 2).sum(1)
+    pts = pts[r2 < radius * radius]
+    r2 = (pts ** 2).sum(1)
+    speed = strain * radius
+    c0 = c_mult * speed
+    rho0 = 1000.0
+    rho = rho0 * (1.0 + 0.5 * strain * strain * (radius * radius - r2) / c0 ** 2)
+    vel = pts * np.array([strain, -strain])
+    empty = np.zeros((0, 2))
+    return DpCase(pts, vel, rho, rho * dx * dx, empty, empty.copy(), empty.copy(), dx=dx, sound_speed=c0,
+                  reference_speed=speed, steps=steps, rho0=rho0, gravity=0.0, viscosity=viscosity,
+                  kind="drop")
+
+
+def impact_case(n_target: int, radius: float, speed: float, viscosity: float, c_mult: float,
+                steps: int, width: float = 1.0, depth: float = 0.2, layers: int = 4) -> DpCase:
+    """A drop of radius R falling at speed V onto a pool of depth D in a tank."""
+    g = 9.81
+    area = width * depth + math.pi * radius * radius
+    dx = math.sqrt(area / n_target)
+    nx, ny = int(round(width / dx)), int(round(depth / dx))
+    dx = width / nx
+    pool = _lattice(nx, ny, dx)
+    cy = ny * dx + radius + 3.0 * dx
+    cx = 0.5 * width
+    cells = int(math.ceil(radius / dx))
+    blob = _lattice(2 * cells, 2 * cells, dx, cx - cells * dx, cy - cells * dx)
+    blob = blob[((blob - [cx, cy]) ** 2).sum(1) < radius * radius]
+    top = cy + radius + 2.0 * dx
+    c0 = c_mult * max(speed, math.sqrt(2.0 * g * depth))
+    rho0 = 1000.0
+    h_pool = ny * dx
+    rho_pool = rho0 * np.exp(g * (h_pool - pool[:, 1]) / c0 ** 2)
+    rho_blob = np.full(blob.shape[0], rho0)
+    pos = np.concatenate([pool, blob])
+    rho = np.concatenate([rho_pool, rho_blob])
+    vel = np.zeros_like(pos)
+    vel[pool.shape[0]:, 1] = -speed
+    # wall stations: `layers` rows under the floor and beside both sides, on the fluid lattice
+    ny_t = int(math.ceil(top / dx))
+    floor = _lattice(nx + 2 * layers, layers, dx, -layers * dx, -layers * dx)
+    left = _lattice(layers, ny_t, dx, -layers * dx, 0.0)
+    right = _lattice(layers, ny_t, dx, width, 0.0)
+    wall = np.concatenate([floor, left, right])
+    normal = np.where((wall[:, 1] < 0.0)[:, None], np.array([0.0, 1.0]), np.array([1.0, 0.0]))
+    return DpCase(pos, vel, rho, rho * dx * dx, wall, np.zeros_like(wall), normal, dx=dx,
+                  sound_speed=c0, reference_speed=max(speed, math.sqrt(2.0 * g * depth)),
+                  steps=steps, rho0=rho0, gravity=g, viscosity=viscosity, kind="impact")
+
+
+def request_stream(count: int = 512, seed: int = 512, steps: int = 500, n_min: int = 2000,
+                   n_max: int = 250000):
+    """The served workload: `count` drop / impact requests with log-uniform
+    particle counts in [n_min, n_max], nu log-uniform in [1e-6, 1e-3] and a
+    sound-speed multiplier in [10, 20], all drawn from PCG64(seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        nu = math.exp(rng.uniform(math.log(1e-6), math.log(1e-3)))
+        mult = rng.uniform(10.0, 20.0)
+        if rng.uniform() < 0.5:
+            out.append(("drop", dict(n_target=n, radius=rng.uniform(0.05, 0.2),
+                                     strain=rng.uniform(2.0, 10.0), viscosity=nu, c_mult=mult,
+                                     steps=steps)))
+        else:
+            out.append(("impact", dict(n_target=n, radius=rng.uniform(0.04, 0.1),
+                                       speed=rng.uniform(0.5, 2.0), viscosity=nu, c_mult=mult,
+                                       steps=steps)))
+    return out
+
+
+def build_case(kind: str, kw: dict) -> DpCase:
+    return drop_case(**kw) if kind == "drop" else impact_case(**kw)
+
+
+class BatchEngine:
+    """Resident batch of δ⁺ simulations on one GPU (tsph_batch_*)."""
+
+    def __init__(self, fluid: int, wall: int, sims: int = 256, cells: int | None = None,
+                 fluid_entries: int | None = None, wall_entries: int | None = None, device=None,
+                 sorted_lists: bool = False):
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        # graph capture needs a stream other than the legacy default stream
+        self.stream = torch.cuda.Stream(device=self.device)
+        caps = _lib.BatchCaps()
+        caps.fluid, caps.wall, caps.sims = int(fluid), int(wall), int(sims)
+        caps.cells = int(cells if cells is not None else fluid + wall)
+        caps.fluid_entries = int(fluid_entries if fluid_entries is not None else 64 * fluid)
+        caps.wall_entries = int(wall_entries if wall_entries is not None else 40 * max(wall, 1))
+        caps.sorted_lists = 1 if sorted_lists else 0
+        self.caps = caps
+        h = C.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(_lib.lib().tsph_batch_create(C.byref(caps), self._s(), C.byref(h)))
+        self._h = h
+        self.cases: dict[int, DpCase] = {}
+
+    def _s(self):
+        return C.c_void_p(self.stream.cuda_stream)
+
+    def admit(self, case: DpCase):
+        """Admit a simulation; returns its slot, or None when it does not fit now."""
+        dev = self.device
+        f = lambda a: torch.as_tensor(np.ascontiguousarray(a, np.float64)).to(dev)
+        tensors = [f(case.pos), f(case.vel), f(case.rho), f(case.mass), f(case.wall),
+                   f(case.wall_vel), f(case.wall_normal)]
+        torch.cuda.synchronize(dev)
+        c = _lib.DpCase()
+        c.n, c.n_wall, c.steps = case.n, case.n_wall, int(case.steps)
+        c.dx, c.sound_speed, c.reference_speed = case.dx, case.sound_speed, case.reference_speed
+        c.rho0, c.gravity, c.viscosity = case.rho0, case.gravity, case.viscosity
+        c.diffusion, c.alpha, c.courant = case.diffusion, case.alpha, case.courant
+        slot = C.c_int32(-1)
+        st = _lib.lib().tsph_batch_admit(self._h, C.byref(c), *[_lib.ptr(t) for t in tensors],
+                                         C.byref(slot), self._s())
+        if st == _lib.TSPH_ERR_OOM:
+            return None
+        _lib.check(st)
+        self.cases[slot.value] = case
+        return slot.value
+
+    def step(self, n: int = 1) -> None:
+        _lib.check(_lib.lib().tsph_batch_step(self._h, int(n), self._s()))
+
+    def poll(self):
+        buf_s = (C.c_int32 * 1024)()
+        buf_t = (C.c_int32 * 1024)()
+        k = C.c_int32(0)
+        _lib.check(_lib.lib().tsph_batch_poll(self._h, buf_s, buf_t, 1024, C.byref(k)))
+        return [(buf_s[i], buf_t[i]) for i in range(min(k.value, 1024))]
+
+    def read(self, slot: int) -> dict:
+        case = self.cases[slot]
+        pos = torch.empty((case.n, 2), dtype=torch.float64, device=self.device)
+        vel = torch.empty_like(pos)
+        rho = torch.empty(case.n, dtype=torch.float64, device=self.device)
+        steps = C.c_int64(0)
+        _lib.check(_lib.lib().tsph_batch_read(self._h, slot, _lib.ptr(pos), _lib.ptr(vel), _lib.ptr(rho),
+                                              C.byref(steps), self._s()))
+        self.stream.synchronize()
+        return {"position": pos, "velocity": vel, "density": rho, "steps": steps.value}
+
+    def release(self, slot: int) -> None:
+        _lib.check(_lib.lib().tsph_batch_release(self._h, slot, self._s()))
+        self.cases.pop(slot, None)
+
+    def stats(self) -> dict:
+        c = (C.c_int64 * 4)()
+        _lib.check(_lib.lib().tsph_batch_stats(self._h, c))
+        return {"steps": c[0], "resident": c[1], "fluid_rows": c[2], "wall_rows": c[3]}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.lib().tsph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class ServeReport:
+    sims: int
+    failed: int
+    wall_s: float
+    step_ms: list = field(default_factory=list)
+    particle_updates: int = 0
+    completion_s: list = field(default_factory=list)
+
+    def summary(self) -> dict:
+        ms = np.asarray(self.step_ms) if self.step_ms else np.zeros(1)
+        return {"sims": self.sims, "failed": self.failed, "wall_s": self.wall_s,
+                "sims_per_s": self.sims / self.wall_s if self.wall_s > 0 else 0.0,
+                "batch_steps": len(self.step_ms),
+                "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
+                "particle_updates_per_s": self.particle_updates / self.wall_s if self.wall_s > 0 else 0.0,
+                "completion_s_p50": float(np.percentile(self.completion_s, 50)) if self.completion_s else 0.0,
+                "completion_s_p99": float(np.percentile(self.completion_s, 99)) if self.completion_s else 0.0}
+
+
+def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_result=None) -> ServeReport:
+    """Continuous batching: requests (kind, kwargs) arrive at Poisson times of
+    the given rate; each is admitted when a slot and rows are free, every
+    active simulation advances one step per batch step, finished ones are read
+    back (``on_result(index, state, result)``) and released."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, size=len(requests)))
+    waiting = deque()
+    running: dict[int, tuple[int, float]] = {}
+    rep = ServeReport(sims=len(requests), failed=0, wall_s=0.0)
+    nxt = 0
+    t0 = time.perf_counter()
+    done = 0
+    while done < len(requests):
+        now = time.perf_counter() - t0
+        while nxt < len(requests) and arrivals[nxt] <= now:
+            waiting.append((nxt, arrivals[nxt], build_case(*requests[nxt])))
+            nxt += 1
+        while waiting:
+            idx, t_arr, case = waiting[0]
+            slot = engine.admit(case)
+            if slot is None:
+                break
+            waiting.popleft()
+            running[slot] = (idx, t_arr)
+        if not running:
+            if nxt < len(requests):
+                time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
+            continue
+        active_rows = sum(engine.cases[s].n for s in running)
+        ts = time.perf_counter()
+        engine.step(1)
+        rep.step_ms.append(1e3 * (time.perf_counter() - ts))
+        rep.particle_updates += active_rows
+        for slot, state in engine.poll():
+            idx, t_arr = running.pop(slot)
+            if state == STATE_FAILED:
+                rep.failed += 1
+            if on_result is not None:
+                on_result(idx, state, engine.read(slot))
+            engine.release(slot)
+            rep.completion_s.append(time.perf_counter() - t0 - t_arr)
+            done += 1
+    rep.wall_s = time.perf_counter() - t0
+    return rep
+
+
+def run_sequential(requests, fluid: int, wall: int, device=None) -> tuple[float, list]:
+    """Single-case runs one after another on a batch engine holding one
+    simulation: returns (seconds, per-case seconds)."""
+    eng = BatchEngine(fluid, wall, sims=1, device=device)
+    times = []
+    try:
+        for kind, kw in requests:
+            case = build_case(kind, kw)
+            t0 = time.perf_counter()
+            slot = eng.admit(case)
+            if slot is None:
+                raise MemoryError("case does not fit the sequential engine")
+            while not eng.poll():
+                eng.step(1)
+            eng.release(slot)
+            times.append(time.perf_counter() - t0)
+    finally:
+        eng.close()
+    return sum(times), times
