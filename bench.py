@@ -59,6 +59,11 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", default=None,
                     help="fixed tile shape TXxTY (e.g. 8x6) instead of the autotuner's choice")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served δ⁺ batch")
+    ap.add_argument("--serve-requests", type=int, default=512)
+    ap.add_argument("--serve-steps", type=int, default=500)
+    ap.add_argument("--serve-sequential", type=int, default=16,
+                    help="requests of the stream timed as sequential single-case runs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -134,6 +139,46 @@ def cpu_baseline(seconds: float):
             "sample": f"FP64 C restatement, 1 thread, dam break at {pos.shape[0]} fluid + "
                       f"{wall.shape[0]} wall particles (dx={cfg.dx:.4g}), {steps} steps in "
                       f"{dt:.1f} s, list rebuilds included"}
+
+
+def serving_bench(n_requests: int, steps: int, n_seq: int) -> dict:
+    """The served δ⁺ workload: Poisson requests of 2k-250k-particle drop / impact
+    runs, continuous batching on one engine holding ~40 % of the stream's rows,
+    against sequential single-case runs of a sample of the same requests."""
+    from tensorsph_amd.serving import BatchEngine, build_case, request_stream, run_sequential, serve
+    reqs = request_stream(count=n_requests, seed=512, steps=steps)
+    cases = [build_case(k, kw) for k, kw in reqs]
+    rows = sum(c.n for c in cases)
+    walls = sum(c.n_wall for c in cases)
+    fluid_cap = int(0.4 * rows) + max(c.n for c in cases)
+    wall_cap = int(0.4 * walls) + max(c.n_wall for c in cases)
+    eng = BatchEngine(fluid_cap, wall_cap, sims=256)
+    eng.step(0)                                   # capture the step graphs
+    footprint = eng.footprint_bytes()
+    rate = n_requests / 20.0                      # requests arrive faster than they finish
+    rep = serve(eng, cases, rate)
+    eng.close()
+    del cases
+    summary = rep.summary()
+    sample = reqs[::max(1, n_requests // max(n_seq, 1))][:n_seq]
+    seq_s, _ = run_sequential(sample)
+    seq_rate = len(sample) / seq_s
+    return {
+        "workload": f"{n_requests} δ⁺ drop / impact requests, {steps} steps each, "
+                    f"{rows} fluid + {walls} wall particles in total, Poisson arrivals at "
+                    f"{rate:.1f}/s, FP64",
+        "sims_per_s": summary["sims_per_s"],
+        "step_ms_p50": summary["step_ms_p50"], "step_ms_p99": summary["step_ms_p99"],
+        "completion_s_p50": summary["completion_s_p50"], "completion_s_p99": summary["completion_s_p99"],
+        "particle_updates_per_s": summary["particle_updates_per_s"],
+        "batch_steps": summary["batch_steps"], "failed": summary["failed"],
+        "engine_capacity": {"fluid_rows": fluid_cap, "wall_rows": wall_cap, "bytes": footprint,
+                            "fraction_of_stream_rows": fluid_cap / rows},
+        "sequential": {"sims_per_s": seq_rate, "sample": f"{len(sample)} requests of the stream "
+                       f"(every {max(1, n_requests // max(n_seq, 1))}th), each alone on an engine "
+                       f"sized for it, graphs captured before timing", "seconds": seq_s},
+        "batched_over_sequential": summary["sims_per_s"] / seq_rate,
+    }
 
 
 def main():
@@ -223,6 +268,8 @@ def main():
                           "force_sweep_ms": f64_ms,
                           "force_sweep_GBps": b64 / (f64_ms * 1e-3) / 1e9 if f64_ms > 0 else 0.0}
         result["fp32c_over_fp64"] = value / v64
+    if not args.no_serve:
+        result["serving"] = serving_bench(args.serve_requests, args.serve_steps, args.serve_sequential)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result))

@@ -288,7 +288,7 @@ __global__ void __launch_bounds__(kBlock) k_b_cell_sort(BatchDev b) {
 // wall stations: fluid rows only.  Visit order: 3 x 3 cells lexicographic.
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= b.cap_f + b.cap_w) return;
   const bool wall = c >= b.cap_f;
   if (b.cid[c] < 0) {
@@ -516,13 +516,13 @@ static int batch_queue_step(tsph_batch* B, int side, hipStream_t stream) {
   TSPH_TRY(queue_scan(pc, d.ccount, d.total_cells, d.cap_cells, d.cstart, stream));
   k_b_scatter<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   k_b_cell_sort<<<nblk(d.cap_cells), kBlock, 0, stream>>>(d);
-  k_b_lists<0><<<nblk(ncloud), kBlock, 0, stream>>>(d);
+  k_b_lists<0><<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
   TSPH_CHECK_LAUNCH();
   ScanPlan pf{d.cap_f, (d.cap_f + kScanTile) / kScanTile, B->f_tiles.as<int64_t>()};
   ScanPlan pw{d.cap_w, (d.cap_w + kScanTile) / kScanTile, B->w_tiles.as<int64_t>()};
   TSPH_TRY(queue_scan(pf, d.fcnt, nullptr, d.cap_f, d.foff, stream));
   TSPH_TRY(queue_scan(pw, d.wcnt, nullptr, d.cap_w, d.woff, stream));
-  k_b_lists<1><<<nblk(ncloud), kBlock, 0, stream>>>(d);
+  k_b_lists<1><<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
   if (B->caps.sorted_lists) k_b_sort_lists<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   TSPH_CHECK_LAUNCH();
   // the δ⁺ step lists index walls as n_fluid + w: here n_fluid = cap_f

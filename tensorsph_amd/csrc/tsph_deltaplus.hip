@@ -102,7 +102,7 @@ TSPH_D void dp_wall_force(const DpSim& P, double gx, double gy, double nx, doubl
 // ---------------------------------------------------------------- sweep A (walls)
 __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __restrict__ woff,
                                                     const int32_t* __restrict__ widx) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (w >= a.nw) return;
   const int s = dp_sim(a.wsim, w);
   if (s < 0) return;                          // a hole of the batch layout
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
 // ---------------------------------------------------------------- sweep B (forces, geometry)
 __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ idx) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
 // ---------------------------------------------------------------- sweep C (continuity, normal)
 __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_t* __restrict__ off,
                                                           const int32_t* __restrict__ idx) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -312,7 +312,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
 // ---------------------------------------------------------------- sweep D (shift, positions)
 __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ idx) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -389,11 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
 
 int dp_queue_step(const DpArgs& a, const int64_t* off, const int32_t* idx, const int64_t* woff,
                   const int32_t* widx, hipStream_t stream) {
-  if (a.nw > 0) k_dp_wall<<<grid_for(a.nw), kBlock, 0, stream>>>(a, woff, widx);
+  // chunked block -> XCD mapping: neighbouring rows are processed on one XCD
+  // and share its L2 (round-robin blocks made every L2 fetch every neighbour)
+  if (a.nw > 0) k_dp_wall<<<grid_xcd(a.nw), kBlock, 0, stream>>>(a, woff, widx);
   if (a.nf > 0) {
-    k_dp_force<<<grid_for(a.nf), kBlock, 0, stream>>>(a, off, idx);
-    k_dp_continuity<<<grid_for(a.nf), kBlock, 0, stream>>>(a, off, idx);
-    k_dp_shift<<<grid_for(a.nf), kBlock, 0, stream>>>(a, off, idx);
+    k_dp_force<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
+    k_dp_continuity<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
+    k_dp_shift<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
   }
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;

@@ -5,7 +5,9 @@ uneven size are admitted by their row footprint into fixed-capacity HBM
 arrays and advanced together, one captured step graph per step; finished
 ones are read back and released, and waiting ones take their room.
 
-``serve`` runs a request stream against it: requests arrive at seeded Poisson
+Particles of a built case are ordered along a Z-order curve of their lattice
+indices, so consecutive rows form compact patches (cache locality of the
+neighbour gathers).  ``serve`` runs a request stream against it: requests arrive at seeded Poisson
 times, are admitted as slots and rows free up, and every batch step's
 latency is recorded.  ``drop_case`` / ``impact_case`` build the two δ⁺ cases
 of the served workload: a free elliptical drop (benchmarks/delta_plus/
@@ -60,6 +62,29 @@ class DpCase:
         return int(self.wall.shape[0])
 
 
+def _morton_order(pts: np.ndarray, dx: float) -> np.ndarray:
+    """Permutation along a Z-order curve of the lattice indices: a run of
+    consecutive rows is a compact 2-D patch, so the neighbours one wave of
+    targets gathers stay in few cache lines."""
+    if pts.shape[0] == 0:
+        return np.zeros(0, dtype=np.int64)
+    ij = np.floor((pts - pts.min(0)) / dx + 0.5).astype(np.uint64)
+    key = np.zeros(pts.shape[0], dtype=np.uint64)
+    for bit in range(24):
+        key |= ((ij[:, 0] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(2 * bit + 1)
+        key |= ((ij[:, 1] >> np.uint64(bit)) & np.uint64(1)) << np.uint64(2 * bit)
+    return np.argsort(key, kind="stable")
+
+
+def _local(case: "DpCase") -> "DpCase":
+    f = _morton_order(case.pos, case.dx)
+    w = _morton_order(case.wall, case.dx)
+    case.pos, case.vel, case.rho, case.mass = case.pos[f], case.vel[f], case.rho[f], case.mass[f]
+    if w.size:
+        case.wall, case.wall_vel, case.wall_normal = case.wall[w], case.wall_vel[w], case.wall_normal[w]
+    return case
+
+
 def _lattice(nx: int, ny: int, dx: float, x0: float = 0.0, y0: float = 0.0) -> np.ndarray:
     ax = (np.arange(nx) + 0.5) * dx + x0
     ay = (np.arange(ny) + 0.5) * dx + y0
@@ -81,9 +106,9 @@ def drop_case(n_target: int, radius: float, strain: float, viscosity: float, c_m
     rho = rho0 * (1.0 + 0.5 * strain * strain * (radius * radius - r2) / c0 ** 2)
     vel = pts * np.array([strain, -strain])
     empty = np.zeros((0, 2))
-    return DpCase(pts, vel, rho, rho * dx * dx, empty, empty.copy(), empty.copy(), dx=dx, sound_speed=c0,
-                  reference_speed=speed, steps=steps, rho0=rho0, gravity=0.0, viscosity=viscosity,
-                  kind="drop")
+    return _local(DpCase(pts, vel, rho, rho * dx * dx, empty, empty.copy(), empty.copy(), dx=dx,
+                         sound_speed=c0, reference_speed=speed, steps=steps, rho0=rho0, gravity=0.0,
+                         viscosity=viscosity, kind="drop"))
 
 
 def impact_case(n_target: int, radius: float, speed: float, viscosity: float, c_mult: float,
@@ -117,9 +142,9 @@ def impact_case(n_target: int, radius: float, speed: float, viscosity: float, c_
     right = _lattice(layers, ny_t, dx, width, 0.0)
     wall = np.concatenate([floor, left, right])
     normal = np.where((wall[:, 1] < 0.0)[:, None], np.array([0.0, 1.0]), np.array([1.0, 0.0]))
-    return DpCase(pos, vel, rho, rho * dx * dx, wall, np.zeros_like(wall), normal, dx=dx,
-                  sound_speed=c0, reference_speed=max(speed, math.sqrt(2.0 * g * depth)),
-                  steps=steps, rho0=rho0, gravity=g, viscosity=viscosity, kind="impact")
+    return _local(DpCase(pos, vel, rho, rho * dx * dx, wall, np.zeros_like(wall), normal, dx=dx,
+                         sound_speed=c0, reference_speed=max(speed, math.sqrt(2.0 * g * depth)),
+                         steps=steps, rho0=rho0, gravity=g, viscosity=viscosity, kind="impact"))
 
 
 def request_stream(count: int = 512, seed: int = 512, steps: int = 500, n_min: int = 2000,
@@ -146,6 +171,13 @@ def request_stream(count: int = 512, seed: int = 512, steps: int = 500, n_min: i
 
 def build_case(kind: str, kw: dict) -> DpCase:
     return drop_case(**kw) if kind == "drop" else impact_case(**kw)
+
+
+# device bytes one fluid row / one wall station occupy in the engine (state in
+# both ping-pong sides, sweep intermediates, cell and list bookkeeping with the
+# default 64 / 40 list entries per row)
+FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 8 + 64 * 4 + 16 + 8 + 32 + 16 + 16 + 8 + 16 + 1 + 8
+WALL_ROW_BYTES = 3 * 16 + 4 + 4 + 4 + 4 + 8 + 40 * 4 + 8 + 8 + 16 + 8
 
 
 class BatchEngine:
@@ -219,6 +251,9 @@ class BatchEngine:
         _lib.check(_lib.lib().tsph_batch_release(self._h, slot, self._s()))
         self.cases.pop(slot, None)
 
+    def footprint_bytes(self) -> int:
+        return self.caps.fluid * FLUID_ROW_BYTES + self.caps.wall * WALL_ROW_BYTES
+
     def stats(self) -> dict:
         c = (C.c_int64 * 4)()
         _lib.check(_lib.lib().tsph_batch_stats(self._h, c))
@@ -257,7 +292,8 @@ class ServeReport:
 
 
 def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_result=None) -> ServeReport:
-    """Continuous batching: requests (kind, kwargs) arrive at Poisson times of
+    """Continuous batching: requests (built DpCase, or (kind, kwargs) built on
+    arrival) arrive at Poisson times of
     the given rate; each is admitted when a slot and rows are free, every
     active simulation advances one step per batch step, finished ones are read
     back (``on_result(index, state, result)``) and released."""
@@ -272,7 +308,8 @@ def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_re
     while done < len(requests):
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrivals[nxt] <= now:
-            waiting.append((nxt, arrivals[nxt], build_case(*requests[nxt])))
+            item = requests[nxt]
+            waiting.append((nxt, arrivals[nxt], item if isinstance(item, DpCase) else build_case(*item)))
             nxt += 1
         while waiting:
             idx, t_arr, case = waiting[0]
@@ -303,22 +340,24 @@ def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_re
     return rep
 
 
-def run_sequential(requests, fluid: int, wall: int, device=None) -> tuple[float, list]:
-    """Single-case runs one after another on a batch engine holding one
-    simulation: returns (seconds, per-case seconds)."""
-    eng = BatchEngine(fluid, wall, sims=1, device=device)
+def run_sequential(requests, device=None) -> tuple[float, list]:
+    """Single-case runs one after another, each on an engine sized for that
+    case alone with its step graphs captured before the clock starts: returns
+    (seconds, per-case seconds)."""
     times = []
-    try:
-        for kind, kw in requests:
-            case = build_case(kind, kw)
+    for kind, kw in requests:
+        case = build_case(kind, kw)
+        eng = BatchEngine(case.n + 8, case.n_wall + 8, sims=1, device=device)
+        try:
+            eng.step(0)                       # capture the step graphs
             t0 = time.perf_counter()
             slot = eng.admit(case)
             if slot is None:
-                raise MemoryError("case does not fit the sequential engine")
+                raise MemoryError("case does not fit its own engine")
             while not eng.poll():
                 eng.step(1)
             eng.release(slot)
             times.append(time.perf_counter() - t0)
-    finally:
-        eng.close()
+        finally:
+            eng.close()
     return sum(times), times
