@@ -307,6 +307,17 @@ typedef struct tsph_sim_params {
   double skin;               /* Verlet skin (absolute length, > 0) */
   double lo[3], hi[3], tol;  /* clamp box and crossing tolerance */
   int32_t clamp;             /* 1: apply the boundary clamp after every step */
+  /* Rigid obstacle (kleefsman_obstacle_3d.py:553-726; dim 3): wall rows
+   * [obstacle_begin, obstacle_end) of the caller's wall order.  When the range
+   * is nonempty every step records the obstacle's load F = sum of the wall
+   * reactions of rhs_3d(return_wall_force=True) (dam_break_3d.py:410-422) and
+   * its moment M = sum (x_w - centre) x F_w (:652-673).  obstacle_clamp = 1
+   * applies clamp_out_of_obstacle (:208-266) to the box [obstacle_lo,
+   * obstacle_hi] after the boundary clamp, with eps = 0.01 dx. */
+  int64_t obstacle_begin, obstacle_end;
+  double obstacle_centre[3];
+  int32_t obstacle_clamp;
+  double obstacle_lo[3], obstacle_hi[3], obstacle_eps;
 } tsph_sim_params;
 
 /* pos: (n_fluid + n_wall, dim) FP64 device, fluid then wall; vel: (n_fluid,
@@ -327,6 +338,11 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
  * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, peak rho,
  * peak speed], minima[rho, p, y]: 20 doubles per row. */
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
+/* Obstacle loads of every completed step (host), 11 doubles per row: step,
+ * F x/y/z, M x/y/z (about obstacle_centre), cumulative clamp impulse x/y/z
+ * (sum of m (v_before - v_after) of clamped particles) and the cumulative
+ * number of clamped particles. */
+int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
 /* Sweep layout.  tiles = 1 (the 2-D default) runs the LDS cell-tile sweeps
  * with tile_x x tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108,
  * tile_x tile_y <= 64); tiles = 0 runs the CSR neighbour-list sweeps (the 3-D

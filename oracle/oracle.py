@@ -268,6 +268,27 @@ def shear_wave_loop(pos, vel, mass, *, h, box, rho0, sound_speed, gamma=7.0, nu,
             "pairs_last": int(pairs.shape[0])}
 
 
+def clamp_out_of_obstacle(pos, vel, mass, lo, hi, eps):
+    """kleefsman_obstacle_3d.clamp_out_of_obstacle (kleefsman_obstacle_3d.py:208-266):
+    particles strictly inside the box leave through the nearest face (first of
+    -x, +x, -y, +y, -z, +z on a tie) to face -/+ eps, that velocity component
+    is zeroed.  Returns (pos, vel, count, impulse = sum m (v_before - v_after))."""
+    pos = np.array(pos, np.float64); vel = np.array(vel, np.float64)
+    lo = np.asarray(lo, np.float64); hi = np.asarray(hi, np.float64)
+    inside = np.all((pos > lo) & (pos < hi), axis=1)
+    impulse = np.zeros(3)
+    idx = np.nonzero(inside)[0]
+    for i in idx:
+        cand = [pos[i, 0] - lo[0], hi[0] - pos[i, 0], pos[i, 1] - lo[1], hi[1] - pos[i, 1],
+                pos[i, 2] - lo[2], hi[2] - pos[i, 2]]
+        face = int(np.argmin(cand))
+        d = face // 2
+        pos[i, d] = hi[d] + eps if face % 2 else lo[d] - eps
+        impulse[d] += mass[i] * vel[i, d]
+        vel[i, d] = 0.0
+    return pos, vel, int(idx.size), impulse
+
+
 def apply_boundary(pos, vel, lo, hi, tol):
     pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
     dim = pos.shape[1]
@@ -323,6 +344,61 @@ def lattice_kernel_sum_2d(h, dx):
     r = np.sqrt(gx ** 2 + gy ** 2).reshape(-1)
     w, _, _ = kernel_eval(0, r, h)
     return float(w.sum())
+
+
+class Stepper3D:
+    """The step loop of kleefsman_obstacle_3d.run_case (:553-726) in FP64:
+    Verlet pair cache with wall-wall pairs pruned, rhs_3d with wall reactions,
+    obstacle force / moment, symplectic Euler, apply_boundary_3d and
+    clamp_out_of_obstacle."""
+
+    def __init__(self, pos, vel, mass, wall, *, h, rho0, c0, gamma, gravity, nu, dt, wall_mass,
+                 skin, lo, hi, tol, obstacle_rows, centre, box=None, eps=0.0, wall_friction=0.0):
+        self.pos = np.array(pos, np.float64); self.vel = np.array(vel, np.float64)
+        self.mass = np.asarray(mass, np.float64); self.wall = np.asarray(wall, np.float64)
+        self.kw = dict(dim=3, h=h, rho0=rho0, c0=c0, gamma=gamma, gravity=gravity, nu=nu,
+                       wall_friction=wall_friction, dt=dt, wall_mass=wall_mass, kernel3d=True,
+                       reaction_style=2)
+        self.h, self.dt, self.skin = h, dt, skin
+        self.lo, self.hi, self.tol = lo, hi, tol
+        self.rows = slice(*obstacle_rows)
+        self.centre = np.asarray(centre, np.float64)
+        self.box, self.eps = box, eps
+        self.anchor = None
+        self.pairs = None
+        self.rebuilds = 0
+        self.clamped = 0
+        self.clamp_impulse = np.zeros(3)
+        self.loads = []
+
+    def _pairs(self):
+        n = self.pos.shape[0]
+        allp = np.concatenate([self.pos, self.wall])
+        disp = math.inf if self.anchor is None else float(np.sqrt(((allp - self.anchor) ** 2).sum(1)).max())
+        if self.pairs is None or disp > 0.5 * self.skin:
+            pairs = cell_linked_pairs(allp, 2.0 * self.h + self.skin)
+            keep = (pairs[:, 0] < n) | (pairs[:, 1] < n)
+            self.pairs = pairs[keep]
+            self.anchor = allp.copy()
+            self.rebuilds += 1
+        return self.pairs
+
+    def step(self):
+        pairs = self._pairs()
+        rho, p, acc, _, _, reac = rhs_morris(self.pos, self.vel, self.mass, self.wall, pairs, **self.kw)
+        loads = reac[self.rows]
+        force = loads.sum(0)
+        moment = np.cross(self.wall[self.rows] - self.centre, loads).sum(0)
+        self.loads.append(np.concatenate([force, moment]))
+        self.vel = self.vel + self.dt * acc
+        self.pos = self.pos + self.dt * self.vel
+        self.pos, self.vel, _ = apply_boundary(self.pos, self.vel, self.lo, self.hi, self.tol)
+        if self.box is not None:
+            self.pos, self.vel, cnt, imp = clamp_out_of_obstacle(self.pos, self.vel, self.mass,
+                                                                 self.box[0], self.box[1], self.eps)
+            self.clamped += cnt
+            self.clamp_impulse += imp
+        return rho, p, acc
 
 
 class Stepper2D:

@@ -84,7 +84,9 @@ class DpCase(C.Structure):
 
 class SimParams(C.Structure):
     _fields_ = [("op", WcsphParams), ("skin", _dbl), ("lo", _dbl * 3), ("hi", _dbl * 3),
-                ("tol", _dbl), ("clamp", C.c_int32)]
+                ("tol", _dbl), ("clamp", C.c_int32), ("obstacle_begin", _i64), ("obstacle_end", _i64),
+                ("obstacle_centre", _dbl * 3), ("obstacle_clamp", C.c_int32),
+                ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3), ("obstacle_eps", _dbl)]
 
 
 _dp = C.POINTER(_dbl)
@@ -113,6 +115,7 @@ def _declare(L):
         "tsph_sim_read": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_sim_stats": [_vp, _ip, _dp],
         "tsph_sim_meters": [_vp, _dp, _i64, _ip],
+        "tsph_sim_loads": [_vp, _dp, _i64, _ip],
         "tsph_sim_timing": [_vp, C.c_int, _dp, _ip],
         "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],

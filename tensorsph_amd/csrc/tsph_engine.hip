@@ -32,12 +32,99 @@
 namespace tsph {
 
 __global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __restrict__ sums, int slot_in,
-                              int with_sums, StepRecord* __restrict__ rec) {
+                              int with_sums, StepRecord* __restrict__ rec, const double* __restrict__ loads,
+                              const double* __restrict__ clamp) {
   const int t = threadIdx.x;
   if (t == 0) { rec->stale_in = fl->stale[slot_in]; rec->stale_out = fl->stale[slot_in ^ 1]; rec->escaped = fl->escaped; }
   if (t < MM_NMAX) rec->maxes[t] = fl->maxes[t];
   if (t < MN_NMIN) rec->mins[t] = fl->mins[t];
   if (t < MS_NSUM) rec->sums[t] = with_sums ? sums[t] : 0.0;
+  if (t < 6) rec->loads[t] = loads ? loads[t] : 0.0;
+  if (t < 3) rec->clamp_imp[t] = clamp ? clamp[t] : 0.0;
+  if (t == 0) rec->clamp_count = clamp ? ((const unsigned long long*)(clamp + 3))[0] : 0ull;
+}
+
+// a run keeps per-step load rows when it has obstacle stations or the clamp
+static inline bool load_rows(const tsph_sim* sim) { return sim->obstacle || sim->prm.obstacle_clamp; }
+
+// Obstacle loads of one step: the rhs_3d reaction of every obstacle station
+// (the k_wall_loads formula, dam_break_3d.py:410-422) and its moment about the
+// centre, reduced per block (partials in a fixed order, no float atomics).
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, const int64_t* __restrict__ woff,
+                                                       const int32_t* __restrict__ widx,
+                                                       const int64_t* __restrict__ orig, int64_t obs_lo,
+                                                       int64_t obs_hi, double c0, double c1, double c2,
+                                                       SimFlags* fl, int slot_in, double* __restrict__ part) {
+  if (fl->stale[slot_in]) return;                // uniform over the grid
+  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double out[6] = {0, 0, 0, 0, 0, 0};
+  if (w < a.nw) {
+    const int64_t iw = a.n + w;
+    const int64_t o = orig[iw];
+    if (o >= obs_lo && o < obs_hi) {
+      double xw[D], vw[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) { xw[d] = a.pos[iw * D + d]; vw[d] = a.vel[iw * D + d]; }
+      const S prw = a.pr2[iw], mw = a.mass[iw], volw = a.vol[iw];
+      Sum<S> r[D];
+      for (int64_t e = woff[w]; e < woff[w + 1]; ++e) {
+        const int64_t f = widx[e];
+        S df[D], r2 = S(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) { df[d] = (S)(a.pos[f * D + d] - xw[d]); r2 += df[d] * df[d]; }
+        if (r2 > a.skip2) continue;
+        S wgt, g;
+        cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
+        const S mf = a.mass[f];
+        const S term = a.pr2[f] + prw;
+        S dg = S(0);
+#pragma unroll
+        for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
+        const S relu = -dg > S(0) ? -dg : S(0);
+        const S lw = relu / (r2 + a.eta2);
+        const S cv = -(a.nu2 * a.friction) * (volw * lw);
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const S pp = (mf * mw) * (term * (df[d] * g));
+          const S visc_f = cv * (S)(a.vel[f * D + d] - vw[d]);
+          r[d].add(pp - mf * visc_f);
+        }
+      }
+      double F[3] = {0, 0, 0}, R[3] = {0, 0, 0};
+      const double cc[3] = {c0, c1, c2};
+#pragma unroll
+      for (int d = 0; d < D; ++d) { F[d] = (double)r[d].get(); R[d] = xw[d] - cc[d]; }
+      out[0] = F[0]; out[1] = F[1]; out[2] = F[2];
+      out[3] = R[1] * F[2] - R[2] * F[1];
+      out[4] = R[2] * F[0] - R[0] * F[2];
+      out[5] = R[0] * F[1] - R[1] * F[0];
+    }
+  }
+  __shared__ double red[6][kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double v = out[k];
+    for (int s = 32; s > 0; s >>= 1) v += __shfl_xor(v, s, 64);
+    if (lane == 0) red[k][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = 0.0;
+    for (int q = 0; q < kBlock / 64; ++q) v += red[threadIdx.x][q];
+    part[(int64_t)blockIdx.x * 6 + threadIdx.x] = v;
+  }
+}
+
+__global__ void k_reduce_loads(const double* __restrict__ part, int64_t nblocks, double* __restrict__ out,
+                               const SimFlags* fl, int slot_in) {
+  if (fl->stale[slot_in]) return;
+  if (threadIdx.x < 6) {
+    double v = 0.0;
+    for (int64_t b = 0; b < nblocks; ++b) v += part[b * 6 + threadIdx.x];
+    out[threadIdx.x] = v;
+  }
 }
 
 
@@ -305,6 +392,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, const int
       if (hit) vn[d] = 0.0;
       xn[d] = x;
     }
+    obstacle_clamp<D>(xn, vn, mi, st);
 #pragma unroll
     for (int d = 0; d < D; ++d) { st.pos_out[i * D + d] = xn[d]; st.vel_out[i * D + d] = vn[d]; }
     double d2 = 0.0;
@@ -659,6 +747,25 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   st.anchor = sim->anchor.as<double>();
   st.block_sums = sim->block_sums.as<double>();
   st.gravity = sim->prm.op.gravity;
+  if (D == 3 && sim->prm.obstacle_clamp) {
+    st.obs_clamp = 1;
+    for (int d = 0; d < 3; ++d) { st.obs_lo[d] = sim->prm.obstacle_lo[d]; st.obs_hi[d] = sim->prm.obstacle_hi[d]; }
+    st.obs_eps = sim->prm.obstacle_eps;
+    st.clamp_imp = sim->clamp_acc.as<double>();
+    st.clamp_cnt = (unsigned long long*)(sim->clamp_acc.as<double>() + 3);
+  }
+  if (sim->obstacle && nw > 0) {
+    // loads read the state before this step's integration (the same pr2 / vol
+    // the force sweep uses); the force sweep writes the other pos / vel side
+    const int64_t lo = n + sim->prm.obstacle_begin, hi = n + sim->prm.obstacle_end;
+    const double* c = sim->prm.obstacle_centre;
+    k_step_loads<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(), ctx->wnb_idx.as<int32_t>(),
+                                                             sim->orig.as<int64_t>(), lo, hi, c[0], c[1], c[2], fl,
+                                                             slot, sim->load_part.as<double>());
+    k_reduce_loads<<<1, 64, 0, stream>>>(sim->load_part.as<double>(), grid_for(nw), sim->loads_out.as<double>(), fl,
+                                         slot);
+    TSPH_CHECK_LAUNCH();
+  }
   if (meters)
     k_step_force<D, S, true><<<gx, kBlock, 0, stream>>>(a, off, idx, sim->acc.as<S>(), st, fl, slot,
                                                         0.5 * sim->prm.skin);
@@ -699,7 +806,9 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   sim->cur ^= 1;
   // record this step for the host
   StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
-  k_pack_record<<<1, 64, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev);
+  k_pack_record<<<1, 64, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
+                                      load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
+                                      load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr);
   TSPH_CHECK_LAUNCH();
   TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
@@ -729,6 +838,12 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
   const double spd = ord_dec_host(rec->maxes[MM_PEAK_SPEED]);
   sim->peak_rho = rho_max > sim->peak_rho ? rho_max : sim->peak_rho;
   sim->peak_speed = spd > sim->peak_speed ? spd : sim->peak_speed;
+  if (load_rows(sim)) {
+    sim->load_rows.push_back((double)k);
+    for (int i = 0; i < 6; ++i) sim->load_rows.push_back(rec->loads[i]);
+    for (int i = 0; i < 3; ++i) sim->load_rows.push_back(rec->clamp_imp[i]);
+    sim->load_rows.push_back((double)rec->clamp_count);
+  }
   if (meters) {
     sim->rows.push_back((double)k);
     for (int i = 0; i < MS_NSUM; ++i) sim->rows.push_back(rec->sums[i]);
@@ -788,7 +903,14 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   TSPH_REQUIRE(prm->skin > 0.0, TSPH_ERR_ARG, "the resident stepper needs a positive Verlet skin");
   TSPH_REQUIRE(prm->op.h > 0.0 && prm->op.dt > 0.0, TSPH_ERR_ARG, "h and dt must be positive");
   TSPH_REQUIRE(prm->op.reaction_style == 0 && prm->op.has_wall_accel == 0, TSPH_ERR_ARG,
-               "the resident stepper runs static walls without load output");
+               "the resident stepper runs static walls (obstacle loads come through obstacle_begin/end)");
+  TSPH_REQUIRE(prm->obstacle_begin >= 0 && prm->obstacle_begin <= prm->obstacle_end &&
+               prm->obstacle_end <= prm->op.n_wall, TSPH_ERR_ARG, "obstacle rows outside the wall rows");
+  TSPH_REQUIRE(!prm->obstacle_clamp || prm->op.dim == 3, TSPH_ERR_ARG, "the obstacle clamp is 3-D");
+  if (prm->obstacle_clamp)
+    for (int d = 0; d < 3; ++d)
+      TSPH_REQUIRE(prm->obstacle_lo[d] < prm->obstacle_hi[d] && prm->obstacle_eps >= 0.0, TSPH_ERR_ARG,
+                   "obstacle box must have lo < hi and eps >= 0");
   tsph_sim* sim = new tsph_sim();
   sim->prm = *prm;
   sim->dim = prm->op.dim;
@@ -811,6 +933,10 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->anchor, 8 * n * D);
   ENS(sim->flags, sizeof(SimFlags));
   ENS(sim->block_sums, 8 * MS_NSUM * (int64_t)grid_xcd(n));
+  sim->obstacle = prm->obstacle_end > prm->obstacle_begin ? 1 : 0;
+  ENS(sim->load_part, 8 * 6 * ((int64_t)grid_for(sim->nw) + 1));
+  ENS(sim->loads_out, 8 * 8);
+  ENS(sim->clamp_acc, 8 * 4);
   ENS(sim->sums_out, 8 * MS_NSUM);
   ENS(sim->rec_dev, sizeof(StepRecord) * kRing);
   ENS(sim->old2new, 8 * nt); ENS(sim->new2old, 8 * nt); ENS(sim->fscan, 8 * nt);
@@ -835,9 +961,12 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemcpyAsync(sim->vel[0].ptr, vel, 8 * n * D, hipMemcpyDeviceToDevice, stream);
   if (he == hipSuccess) he = hipMemcpyAsync(sim->mass.ptr, mass, e * nt, hipMemcpyDeviceToDevice, stream);
   if (he == hipSuccess) he = hipMemsetAsync(sim->flags.ptr, 0, sizeof(SimFlags), stream);
+  if (he == hipSuccess) he = hipMemsetAsync(sim->loads_out.ptr, 0, 8 * 8, stream);
+  if (he == hipSuccess) he = hipMemsetAsync(sim->clamp_acc.ptr, 0, 8 * 4, stream);
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
-  if (sim->dim == 2) { sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4; }
+  // obstacle loads run on the CSR-list sweeps
+  if (sim->dim == 2 && !sim->obstacle) { sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4; }
   *out = sim;
   return TSPH_OK;
 }
@@ -898,9 +1027,22 @@ int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrow
   return TSPH_OK;
 }
 
+int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows) {
+  TSPH_REQUIRE(sim && nrows, TSPH_ERR_ARG, "bad arguments");
+  const int64_t width = 11;
+  const int64_t have = (int64_t)sim->load_rows.size() / width;
+  *nrows = have;
+  if (rows) {
+    const int64_t m = have < max_rows ? have : max_rows;
+    memcpy(rows, sim->load_rows.data(), sizeof(double) * width * m);
+  }
+  return TSPH_OK;
+}
+
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
+  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the CSR-list sweeps");
   TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
                               tile_x * tile_y <= 64),
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
@@ -957,7 +1099,9 @@ int tsph_sim_destroy(tsph_sim* sim) {
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
-                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new};
+                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->ranges,
+                    &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
+                    &sim->load_part, &sim->loads_out, &sim->clamp_acc};
   for (DevBuf* b : bufs) b->release();
   for (int r = 0; r < kRing; ++r) if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
   for (int r = 0; r < kRing; ++r)

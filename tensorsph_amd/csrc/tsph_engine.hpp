@@ -25,6 +25,9 @@ struct StepRecord {
   unsigned long long maxes[MM_NMAX];
   unsigned long long mins[MN_NMIN];
   double sums[MS_NSUM];
+  double loads[6];                       // obstacle force and moment of this step
+  double clamp_imp[3];                   // cumulative obstacle-clamp impulse
+  unsigned long long clamp_count;        // cumulative clamped particles
 };
 
 constexpr int kRing = 4;
@@ -70,6 +73,10 @@ struct tsph_sim {
   int64_t lds_overflow = 0;
   // host-side meter log
   std::vector<double> rows;         // per recorded step: step, sums..., maxes..., mins...
+  // obstacle loads (dim 3): per-block partials, reduced loads, clamp accumulators
+  int obstacle = 0;
+  tsph::DevBuf load_part, loads_out, clamp_acc;
+  std::vector<double> load_rows;    // per completed step: step, F, M, clamp impulse, clamp count
   double peak_rho = 0.0, peak_speed = 0.0;
   size_t elt = 8;
 };

@@ -87,7 +87,40 @@ struct StepArgs {
   unsigned long long* escaped;     // boundary crossings
   int meters;                      // 1: also fill block_sums and the meter maxima
   double gravity;                  // for the potential energy meter
+  // clamp_out_of_obstacle (kleefsman_obstacle_3d.py:208-266), dim 3
+  int obs_clamp;
+  double obs_lo[3], obs_hi[3], obs_eps;
+  double* clamp_imp;               // 3 doubles, cumulative m (v_before - v_after)
+  unsigned long long* clamp_cnt;
 };
+
+// clamp_out_of_obstacle for one particle after the tank clamp: a particle
+// strictly inside the box leaves through the nearest face (first of -x, +x,
+// -y, +y, -z, +z on a tie) to that face -/+ eps and loses that velocity
+// component, whose momentum goes to the impulse accumulator.
+template <int D>
+TSPH_D void obstacle_clamp(double* xn, double* vn, double m, const StepArgs& st) {
+  if constexpr (D == 3) {
+    if (!st.obs_clamp) return;
+    bool inside = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) inside = inside && xn[d] > st.obs_lo[d] && xn[d] < st.obs_hi[d];
+    if (!inside) return;
+    int face = 0;
+    double best = xn[0] - st.obs_lo[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) {
+      const int d = k >> 1;
+      const double c = (k & 1) ? st.obs_hi[d] - xn[d] : xn[d] - st.obs_lo[d];
+      if (c < best) { best = c; face = k; }
+    }
+    const int d = face >> 1;
+    xn[d] = (face & 1) ? st.obs_hi[d] + st.obs_eps : st.obs_lo[d] - st.obs_eps;
+    atomicAdd(&st.clamp_imp[d], m * vn[d]);
+    atomicAdd(st.clamp_cnt, 1ull);
+    vn[d] = 0.0;
+  }
+}
 
 // v' = v + dt a, x' = x + dt v', then the safety clamp of apply_boundary:
 // positions clamped to the box, the normal velocity zeroed beyond 0.25 dx.
@@ -106,6 +139,7 @@ TSPH_D void integrate_one(int64_t i, const double* xi, const double* vi, const S
     if (hit) vn[d] = 0.0;
     xn[d] = x;
   }
+  obstacle_clamp<D>(xn, vn, (double)a.mass[i], st);
 #pragma unroll
   for (int d = 0; d < D; ++d) { st.pos_out[i * D + d] = xn[d]; st.vel_out[i * D + d] = vn[d]; }
   double disp2 = 0.0;

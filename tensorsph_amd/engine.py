@@ -57,7 +57,7 @@ class WcsphSimulation:
     def __init__(self, pos, vel, mass, wall, *, h, rho0, sound_speed, gamma, gravity, viscosity,
                  dt, wall_mass, skin, lo, hi, tol, wall_friction=0.0, density_diffusion=0.0,
                  diffusion_form="molteni", eigenvalue_floor=0.1, precision="fp32", clamp=True,
-                 device="cuda"):
+                 device="cuda", obstacle=None):
         dev = torch.device(device)
         pos = torch.as_tensor(pos, dtype=torch.float64, device=dev)
         vel = torch.as_tensor(vel, dtype=torch.float64, device=dev)
@@ -87,6 +87,19 @@ class WcsphSimulation:
             p.hi[d] = hi[d] if d < dim else 0.0
         p.tol = tol
         p.clamp = 1 if clamp else 0
+        if obstacle is not None:
+            # obstacle = {"rows": (begin, end) wall rows, "centre": (3,), optional
+            # "box": (lo (3,), hi (3,)) for clamp_out_of_obstacle and "eps"}
+            begin, end = obstacle["rows"]
+            p.obstacle_begin, p.obstacle_end = int(begin), int(end)
+            for d in range(3):
+                p.obstacle_centre[d] = float(obstacle["centre"][d]) if d < len(obstacle["centre"]) else 0.0
+            if obstacle.get("box") is not None:
+                lo_b, hi_b = obstacle["box"]
+                p.obstacle_clamp = 1
+                for d in range(3):
+                    p.obstacle_lo[d], p.obstacle_hi[d] = float(lo_b[d]), float(hi_b[d])
+                p.obstacle_eps = float(obstacle.get("eps", 0.0))
         self.params = p
         handle = _vp()
         stream = _lib.stream_handle(dev)
@@ -131,6 +144,17 @@ class WcsphSimulation:
             _lib.check(_lib.lib().tsph_sim_meters(
                 self.handle, rows.ctypes.data_as(C.POINTER(C.c_double)), nrows.value,
                 C.byref(nrows)))
+        return rows
+
+    def loads(self) -> np.ndarray:
+        """Obstacle loads per completed step: (rows, 11) = step, F (3), M (3),
+        cumulative clamp impulse (3), cumulative clamped particles."""
+        nrows = _i64(0)
+        _lib.check(_lib.lib().tsph_sim_loads(self.handle, None, 0, C.byref(nrows)))
+        rows = np.zeros((nrows.value, 11), np.float64)
+        if nrows.value:
+            _lib.check(_lib.lib().tsph_sim_loads(self.handle, rows.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 nrows.value, C.byref(nrows)))
         return rows
 
     def close(self) -> None:

@@ -431,3 +431,37 @@ def test_ledger_row_closed_forms():
     assert abs(r["angular_momentum"][2] - inertia * omega) < 1e-14
     e_small = 3.0 * (c0 ** 2) * 0.5 * 1e-8       # m c0^2 (drho/rho0)^2 / 2
     assert abs(r["internal_energy"] - e_small) < 1e-3 * e_small
+
+
+# ----------------------------------------------------------------- obstacle clamp (A14, config 5)
+
+def test_obstacle_clamp_closed_form():
+    """Each particle leaves through its nearest face to face -/+ eps; the
+    normal velocity is zeroed and its momentum goes into the impulse."""
+    lo, hi, eps = np.array([1.0, 0.0, 2.0]), np.array([2.0, 0.5, 3.0]), 1e-3
+    pos = np.array([[1.1, 0.25, 2.5],     # nearest -x
+                    [1.95, 0.25, 2.5],    # nearest +x
+                    [1.5, 0.02, 2.5],     # nearest -y
+                    [1.5, 0.45, 2.5],     # nearest +y
+                    [1.5, 0.25, 2.05],    # nearest -z
+                    [1.5, 0.25, 2.97],    # nearest +z
+                    [0.5, 0.25, 2.5],     # outside: untouched
+                    [1.0, 0.25, 2.5]])    # on a face: not strictly inside
+    vel = np.arange(24, dtype=np.float64).reshape(8, 3) + 1.0
+    mass = np.linspace(1.0, 2.0, 8)
+    new, nv, cnt, imp = orc.clamp_out_of_obstacle(pos, vel, mass, lo, hi, eps)
+    assert cnt == 6
+    want = [(0, 0, lo[0] - eps), (1, 0, hi[0] + eps), (2, 1, lo[1] - eps), (3, 1, hi[1] + eps),
+            (4, 2, lo[2] - eps), (5, 2, hi[2] + eps)]
+    expect_imp = np.zeros(3)
+    for i, d, x in want:
+        assert new[i, d] == x and nv[i, d] == 0.0
+        others = [k for k in range(3) if k != d]
+        assert np.array_equal(new[i, others], pos[i, others]) and np.array_equal(nv[i, others], vel[i, others])
+        expect_imp[d] += mass[i] * vel[i, d]
+    assert np.array_equal(new[6:], pos[6:]) and np.array_equal(nv[6:], vel[6:])
+    assert np.allclose(imp, expect_imp, rtol=0, atol=1e-12)
+    # the box centre ties all six faces: it leaves through -x (argmin takes the first)
+    tie, _, _, _ = orc.clamp_out_of_obstacle(np.array([[1.5, 0.5, 2.5]]), np.zeros((1, 3)), [1.0],
+                                             [1.0, 0.0, 2.0], [2.0, 1.0, 3.0], eps)
+    assert tie[0, 0] == 1.0 - eps
