@@ -63,7 +63,14 @@ def parse():
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=500)
     ap.add_argument("--serve-sequential", type=int, default=16,
-                    help="requests of the stream timed as sequential single-case runs")
+                    help="requests of the stream timed alone on the batch engine")
+    ap.add_argument("--serve-sequential-api", type=int, default=8,
+                    help="requests of the stream timed through the single-case API loop")
+    ap.add_argument("--workload", choices=("dambreak", "shear", "kleefsman"), default="dambreak",
+                    help="dambreak: the 2-D dam break line (default); shear: 3-D periodic shear "
+                         "wave; kleefsman: 3-D dam break against a box obstacle")
+    ap.add_argument("--shear-n", type=int, default=128, help="lattice edge of the shear-wave cube")
+    ap.add_argument("--fluid3d", type=float, default=5e6, help="fluid particles of the box case")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -141,11 +148,14 @@ def cpu_baseline(seconds: float):
                       f"{dt:.1f} s, list rebuilds included"}
 
 
-def serving_bench(n_requests: int, steps: int, n_seq: int) -> dict:
+def serving_bench(n_requests: int, steps: int, n_seq: int, n_api: int) -> dict:
     """The served δ⁺ workload: Poisson requests of 2k-250k-particle drop / impact
     runs, continuous batching on one engine holding ~40 % of the stream's rows,
-    against sequential single-case runs of a sample of the same requests."""
-    from tensorsph_amd.serving import BatchEngine, build_case, request_stream, run_sequential, serve
+    against sequential single-case runs of samples of the same requests: alone
+    on the batch engine, and through the single-case API loop of the
+    reference's drivers (dt, pair cache, delta_plus_step per step)."""
+    from tensorsph_amd.serving import (BatchEngine, build_case, request_stream, run_sequential,
+                                       run_sequential_api, serve)
     reqs = request_stream(count=n_requests, seed=512, steps=steps)
     cases = [build_case(k, kw) for k, kw in reqs]
     rows = sum(c.n for c in cases)
@@ -163,6 +173,17 @@ def serving_bench(n_requests: int, steps: int, n_seq: int) -> dict:
     sample = reqs[::max(1, n_requests // max(n_seq, 1))][:n_seq]
     seq_s, _ = run_sequential(sample)
     seq_rate = len(sample) / seq_s
+    api = None
+    if n_api > 0:
+        stride = max(1, n_requests // n_api)
+        api_sample = reqs[stride // 2::stride][:n_api]
+        api_s, _ = run_sequential_api(api_sample)
+        api_rate = len(api_sample) / api_s
+        api = {"sims_per_s": api_rate, "seconds": api_s,
+               "sample": f"{len(api_sample)} requests of the stream (every {stride}th from "
+                         f"{stride // 2}), each through delta_plus_step with a PairListCache(4 dx, "
+                         f"0.4 dx) and the adaptive dt per step, one after another",
+               "batched_over_api": summary["sims_per_s"] / api_rate}
     return {
         "workload": f"{n_requests} δ⁺ drop / impact requests, {steps} steps each, "
                     f"{rows} fluid + {walls} wall particles in total, Poisson arrivals at "
@@ -178,14 +199,142 @@ def serving_bench(n_requests: int, steps: int, n_seq: int) -> dict:
                        f"(every {max(1, n_requests // max(n_seq, 1))}th), each alone on an engine "
                        f"sized for it, graphs captured before timing", "seconds": seq_s},
         "batched_over_sequential": summary["sims_per_s"] / seq_rate,
+        "sequential_api": api,
     }
+
+
+def _timed(step, steps, warmup):
+    import torch
+    step(max(warmup, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step(steps)
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def _line(metric, value, steps, warmup, wall_s, dtype, config, **extra):
+    out = {"metric": metric, "value": value, "unit": "particle-updates/s", "n_gpus": 1,
+           "steps": steps, "warmup": warmup, "ms_per_step": 1000.0 * wall_s / steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic: deterministic lattice (no dataset)", "config": config}
+    out.update(extra)
+    return out
+
+
+def shear_bench(args) -> dict:
+    """3-D triply periodic shear wave on the resident leapfrog stepper: unit
+    cube, n^3 lattice jittered by U(-0.01 dx, 0.01 dx) from PCG64(20261203),
+    h = 1.3 dx, rho0 = 1, nu = 0.01, c0 = 10, u_x = 1e-3 sin(2 pi z),
+    renormalised Laplacian, dt = min(0.25 h / c0, 0.125 h^2 / nu), skin 0.5 h."""
+    import numpy as np
+    import torch
+    from tensorsph_amd.threed.kernel3 import calibrated_particle_mass
+    from tensorsph_amd.threed.solver import PeriodicSimulation, cubic_lattice, shear_wave_velocity
+    n = args.shear_n
+    dx = 1.0 / n
+    h = 1.3 * dx
+    nu, c0, rho0 = 0.01, 10.0, 1.0
+    dt = min(0.25 * h / c0, 0.125 * h * h / nu)
+    pos = cubic_lattice((n, n, n), dx)
+    rng = np.random.Generator(np.random.PCG64(20261203))
+    pos = (pos + torch.from_numpy(rng.uniform(-0.01 * dx, 0.01 * dx, size=tuple(pos.shape)))).remainder(1.0)
+    vel = shear_wave_velocity(pos, 1e-3, 2.0 * math.pi, "single")
+    mass = torch.full((pos.shape[0],), calibrated_particle_mass(dx, h, rho0), dtype=torch.float64)
+    runs = {}
+    for prec in ("fp32", "fp64"):
+        sim = PeriodicSimulation(pos, vel, mass, h=h, box=1.0, rho0=rho0, sound_speed=c0, viscosity=nu,
+                                 dt=dt, skin=0.5 * h, renormalise=True, precision=prec, device="cuda")
+        wall_s = _timed(sim.step, args.steps, args.warmup)
+        st = sim.stats()
+        sim.close()
+        runs[prec] = {"wall_s": wall_s, "value": pos.shape[0] * args.steps / wall_s, "pairs": st["pairs"],
+                      "rebuilds": st["rebuilds"]}
+    npart = pos.shape[0]
+    v32, v64 = runs["fp32"]["value"], runs["fp64"]["value"]
+    # compulsory bytes per particle-update: density sweep (x, m; rho written), the
+    # moment pre-pass (x, m, rho; K written) and the force sweep + leapfrog
+    # (x, v, m, rho, p, K; x, v written): 40 D + 4 s, plus 8 D + 2 s for K
+    b32, b64 = 40 * 3 + 4 * 4 + 24 + 2 * 4, 40 * 3 + 4 * 8 + 24 + 2 * 8
+    cfg = {"workload": f"3-D periodic shear wave, {n}^3 = {npart} particles, dx=1/{n}, h=1.3dx, "
+                       f"rho0=1, nu=0.01, c0=10, dt={dt:.4g} s, jitter 0.01 dx",
+           "precision": "pair arithmetic FP32 with Kahan sums (density FP64); positions, velocities "
+                        "and leapfrog FP64", "particles": npart, "parallelism": "single GPU"}
+    return _line("particle-updates/s (3-D periodic shear wave, FP32-compensated) + HBM GB/s", v32,
+                 args.steps, args.warmup, runs["fp32"]["wall_s"], "f32", cfg,
+                 algorithmic_GBps_step=v32 * b32 / 1e9, algorithmic_bytes_per_update=b32,
+                 pair_interactions_per_s=runs["fp32"]["pairs"] * args.steps / runs["fp32"]["wall_s"],
+                 list_pairs=runs["fp32"]["pairs"], list_rebuilds_in_run=runs["fp32"]["rebuilds"],
+                 fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
+                       "algorithmic_GBps_step": v64 * b64 / 1e9},
+                 fp32c_over_fp64=v32 / v64)
+
+
+def kleefsman_bench(args) -> dict:
+    """3-D dam break against a box obstacle (Kleefsman): 3.22 x 1 x 1 m tank,
+    1.228 x 0.55 x 1 m column, 0.161 x 0.161 x 0.403 m box at x = 2.3955 m,
+    3 wall layers on the tank, the box as a 3-layer shell, c0 = 20 m/s,
+    dt = 0.9 of the acoustic Courant limit, h = 1.3 dx; the box force and moment
+    are reduced every step."""
+    import torch
+    from tensorsph_amd.config import Kleefsman3DConfig
+    from tensorsph_amd.kleefsman import dx_for_fluid, make_simulation
+    cfg = Kleefsman3DConfig(dx=dx_for_fluid(args.fluid3d), sound_speed=20.0, wall_layers=3,
+                            obstacle_fill="shell")
+    cfg.dt = 0.9 * cfg.dt_from_courant()
+    runs = {}
+    for prec in ("fp32", "fp64"):
+        sim, nf, nw = make_simulation(cfg, prec)
+        sim.step(max(args.warmup, 1))
+        torch.cuda.synchronize()
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        sim.step(args.steps)
+        torch.cuda.synchronize()
+        wall_s = time.perf_counter() - t0
+        sweep_ms, nsteps = sim.sweep_times()
+        sim.set_timing(False)
+        loads = sim.loads()
+        stats = sim.stats()
+        sim.close()
+        runs[prec] = {"wall_s": wall_s, "value": nf * args.steps / wall_s,
+                      "sweep_ms": [x / max(nsteps, 1) for x in sweep_ms], "rebuilds": stats["rebuilds"],
+                      "force": loads[-1, 1:4].tolist(), "moment": loads[-1, 4:7].tolist()}
+    v32, v64 = runs["fp32"]["value"], runs["fp64"]["value"]
+    s = 4
+    b_force = nf * (48 * 2 + 2 * s) + nw * (24 + 3 * s)
+    force_ms = runs["fp32"]["sweep_ms"][2]
+    achieved = b_force / (force_ms * 1e-3) / 1e9 if force_ms > 0 else 0.0
+    f32, f64 = [runs[p]["force"] for p in ("fp32", "fp64")]
+    scale = max(abs(x) for x in f64) or 1.0
+    cfg_out = {"workload": f"3-D dam break with a box obstacle, {nf} fluid + {nw} wall particles, "
+                           f"dx={cfg.dx:.5g} m, h=1.3dx, c0=20 m/s, dt={cfg.dt:.4g} s",
+               "precision": "pair arithmetic FP32 with Kahan sums (density FP64); positions, "
+                            "velocities and integration FP64", "fluid": nf, "wall": nw,
+               "parallelism": "single GPU"}
+    return _line("particle-updates/s (3-D dam break with a box obstacle, FP32-compensated) + HBM GB/s",
+                 v32, args.steps, args.warmup, runs["fp32"]["wall_s"], "f32", cfg_out,
+                 sweep_ms_per_step=dict(zip(("density", "wall", "force_integrate"), runs["fp32"]["sweep_ms"])),
+                 roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "k_step_force (force + viscosity + integrate + box clamp)",
+                           "algorithmic_bytes_per_launch": b_force},
+                 list_rebuilds_in_run=runs["fp32"]["rebuilds"],
+                 box_load_last_step={"force_N_fp32c": f32, "force_N_fp64": f64,
+                                     "moment_Nm_fp32c": runs["fp32"]["moment"],
+                                     "moment_Nm_fp64": runs["fp64"]["moment"],
+                                     "force_rel_diff": max(abs(a - b) for a, b in zip(f32, f64)) / scale},
+                 fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
+                       "sweep_ms_per_step": dict(zip(("density", "wall", "force_integrate"),
+                                                     runs["fp64"]["sweep_ms"]))},
+                 fp32c_over_fp64=v32 / v64)
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU "
-                                   "slab decomposition is a design only, see DESIGN.md)"}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 (a slab "
+                                   "decomposition over several GPUs is designed, not built)"}))
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -198,6 +347,12 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": src.numel() * 8}))
+        return
+    if args.workload == "shear":
+        print(json.dumps(shear_bench(args)))
+        return
+    if args.workload == "kleefsman":
+        print(json.dumps(kleefsman_bench(args)))
         return
     cfg, pos, mass, wall, wmass = dam_break_state(args.fluid)
     nf, nw = pos.shape[0], wall.shape[0]
@@ -269,7 +424,8 @@ def main():
                           "force_sweep_GBps": b64 / (f64_ms * 1e-3) / 1e9 if f64_ms > 0 else 0.0}
         result["fp32c_over_fp64"] = value / v64
     if not args.no_serve:
-        result["serving"] = serving_bench(args.serve_requests, args.serve_steps, args.serve_sequential)
+        result["serving"] = serving_bench(args.serve_requests, args.serve_steps, args.serve_sequential,
+                                          args.serve_sequential_api)
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(result))

@@ -290,7 +290,9 @@ int tsph_psim_diagnostics(tsph_psim* sim, double wavenumber, double* out, hipStr
  * of the last evaluation in the scalar type.  Any pointer may be NULL. */
 int tsph_psim_read(tsph_psim* sim, double* pos, double* vel, void* rho, void* acc,
                    hipStream_t stream);
-/* counters: steps done, list builds, list entries.  Host pointer. */
+/* counters: steps done, list builds (cell binnings), entries the
+ * periodic_pairs lists at the build radius would hold (each pair twice).
+ * Host pointer. */
 int tsph_psim_stats(tsph_psim* sim, int64_t* counters);
 int tsph_psim_destroy(tsph_psim* sim);
 
@@ -331,8 +333,9 @@ int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_
  * and acc (n_fluid, dim) of the last step.  Any pointer may be NULL. */
 int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc,
                   hipStream_t stream);
-/* counters: steps done, list builds, boundary crossings, list entries, meter
- * rows; peaks: max density and max speed over all steps.  Host pointers. */
+/* counters: steps done, list builds (cell binnings), boundary crossings,
+ * cells of the dense cell table the sweeps walk, meter rows; peaks: max
+ * density and max speed over all steps.  Host pointers. */
 int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
 /* Meter rows (host): step, sums[mass, momentum x/y/z, kinetic energy, sum rho,
  * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, peak rho,

@@ -444,7 +444,7 @@ struct tsph_batch {
   DevBuf dps, bs, state, active, status, vmax, bbox, total_cells, flags, step_counter;
   DevBuf cid, ccount, cfill, members, cstart, cstart_tiles;
   DevBuf fcnt, wcnt, foff, woff, fidx, widx, f_tiles, w_tiles;
-  DevBuf wall_p, wall_rho, reaction, acc, lam, inv, gr, normal, strength, shift, surface;
+  DevBuf wall_p, wall_rho, reaction, acc, lam, inv, gr, normal, strength, shift, surface, vol;
   DevBuf record;
   BatchRecord* host_rec = nullptr;               // pinned
   unsigned char* host_state = nullptr;           // pinned, cap_s
@@ -497,6 +497,7 @@ static DpArgs batch_dp_args(tsph_batch* B, const BatchDev& d) {
   a.gr = B->gr.as<double>(); a.rho_new = d.rho_new; a.normal = B->normal.as<double>();
   a.strength = B->strength.as<double>(); a.shift = B->shift.as<double>(); a.x_new = d.x_new;
   a.surface = B->surface.as<unsigned char>();
+  a.vol = B->vol.as<double>();
   return a;
 }
 
@@ -607,7 +608,7 @@ int tsph_batch_destroy(tsph_batch* B) {
                     &B->members, &B->cstart, &B->cstart_tiles, &B->fcnt, &B->wcnt, &B->foff, &B->woff,
                     &B->fidx, &B->widx, &B->f_tiles, &B->w_tiles, &B->wall_p, &B->wall_rho, &B->reaction,
                     &B->acc, &B->lam, &B->inv, &B->gr, &B->normal, &B->strength, &B->shift, &B->surface,
-                    &B->record};
+                    &B->vol, &B->record};
   for (DevBuf* b : bufs) b->release();
   if (B->host_rec) (void)hipHostFree(B->host_rec);
   if (B->host_state) (void)hipHostFree(B->host_state);
@@ -643,7 +644,8 @@ int tsph_batch_create(const tsph_batch_caps* caps, hipStream_t stream, tsph_batc
       {&B->fidx, 4 * (size_t)c.fluid_entries + 4}, {&B->widx, 4 * (size_t)c.wall_entries + 4},
       {&B->wall_p, 8 * W}, {&B->wall_rho, 8 * W}, {&B->reaction, 16 * W}, {&B->acc, 16 * F},
       {&B->lam, 8 * F}, {&B->inv, 32 * F}, {&B->gr, 16 * F}, {&B->normal, 16 * F}, {&B->strength, 8 * F},
-      {&B->shift, 16 * F}, {&B->surface, F}, {&B->record, sizeof(BatchRecord)}};
+      {&B->shift, 16 * F}, {&B->surface, F}, {&B->vol, 8 * F},
+      {&B->record, sizeof(BatchRecord)}};
   int rc = TSPH_OK;
   for (auto& p : plan) if (rc == TSPH_OK) rc = p.b->ensure(p.bytes);
   if (rc == TSPH_OK && (hipHostMalloc((void**)&B->host_rec, sizeof(BatchRecord), 0) != hipSuccess ||

@@ -38,9 +38,41 @@ TSPH_D int64_t find_cell(const int64_t* __restrict__ ukeys, int64_t m, int64_t k
   return (lo < m && ukeys[lo] == key) ? lo : -1;
 }
 
+// Morton (Z-order) key of non-negative cell coordinates: the bits of the D
+// axes interleaved, axis 0 the most significant of each group.  Consecutive
+// keys form compact blocks of cells, so rows sorted by it keep every cell's
+// 3^D neighbourhood close in memory (up to 21 bits per axis in 3-D, 31 in 2-D).
+TSPH_HD uint64_t spread_bits2(uint64_t v) {
+  v &= 0x7fffffffull;
+  v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+  v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+TSPH_HD uint64_t spread_bits3(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x001f00000000ffffull;
+  v = (v | (v << 16)) & 0x001f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+template <int D>
+TSPH_HD int64_t morton_key(const int64_t* c) {
+  if constexpr (D == 1) return c[0];
+  else if constexpr (D == 2) return (int64_t)((spread_bits2((uint64_t)c[0]) << 1) | spread_bits2((uint64_t)c[1]));
+  else
+    return (int64_t)((spread_bits3((uint64_t)c[0]) << 2) | (spread_bits3((uint64_t)c[1]) << 1) |
+                     spread_bits3((uint64_t)c[2]));
+}
+
 struct GridDesc {
   int64_t ext[3];      // open: cells per axis after normalisation; periodic: ncell
   int64_t stride[3];
+  int morton = 0;      // keys are morton_key of the cell (else row-major)
 };
 
 // Visits every j in the 3^D neighbour cells of particle i (reference order:
@@ -60,7 +92,7 @@ TSPH_D void for_neighbours(const T* __restrict__ pos, int64_t i, const int64_t* 
     int64_t off[3] = {0, 0, 0};
 #pragma unroll
     for (int d = D - 1; d >= 0; --d) { off[d] = rem % 3 - 1; rem /= 3; }
-    int64_t key = 0;
+    int64_t key = 0, nbc[3] = {0, 0, 0};
     bool inside = true;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -71,9 +103,11 @@ TSPH_D void for_neighbours(const T* __restrict__ pos, int64_t i, const int64_t* 
       } else if (nb < 0 || nb >= g.ext[d]) {
         inside = false;
       }
-      key += (nb < 0 ? 0 : nb) * g.stride[d];
+      nbc[d] = nb < 0 ? 0 : nb;
+      key += nbc[d] * g.stride[d];
     }
     if (!inside) continue;
+    if (g.morton) key = morton_key<D>(nbc);
     const int64_t slot = find_cell(ukeys, ncells, key);
     if (slot < 0) continue;
     const int64_t s0 = ustart[slot], s1 = ustart[slot + 1];
@@ -83,6 +117,104 @@ TSPH_D void for_neighbours(const T* __restrict__ pos, int64_t i, const int64_t* 
       if (dist2<T, D, PER>(pos, i, j, box) <= c2) f(j);
     }
   }
+}
+
+// ---------------------------------------------------------------- neighbour sources
+// The pair sweeps take their candidates from one of two sources with the same
+// visiting order, so a sweep gives the same bits over either:
+//   NbList  per-target CSR lists (built by for_neighbours at the list radius);
+//   NbCells a walk over the 3^D cells around the target's cell (edge >= list
+//           radius) through a dense table of per-cell row ranges.
+// Both visit the neighbour cells with offsets in lexicographic order (last
+// axis fastest) and, in each cell, its fluid rows then its wall rows in
+// ascending order, skipping the target itself.  The cell walk also offers the
+// candidates beyond the list radius; every sweep re-tests the kernel support,
+// which is inside that radius, so those contribute nothing.  Lanes of a wave
+// that share a cell read the same rows, so the walk's loads are mostly
+// broadcasts where a list walk gathers one cache line per lane.
+// largest dense cell table the steppers allocate (16 B per cell)
+constexpr int64_t kMaxDenseCells = 1ll << 27;
+
+struct NbList {
+  const int64_t* off;
+  const int32_t* idx;
+  // with_walls is implied by how the lists were built
+  template <int D, typename F>
+  TSPH_D void visit(int64_t i, bool /*with_walls*/, F&& f) const {
+    const int64_t e1 = off[i + 1];
+    for (int64_t e = off[i]; e < e1; ++e) f((int64_t)idx[e]);
+  }
+};
+
+struct NbCells {
+  const int4* R;            // per row-major cell: fluid rows [x, y), wall rows [z, w)
+  const int64_t* cells;     // (rows, D) cell of every row at the last binning
+  int64_t ext[3], stride[3];
+  int periodic;
+  // f runs for every candidate of the walk and re-tests the support itself.
+  // A two-phase variant (scan, keep accepted rows, then run f over them) was
+  // measured 2.6-3.7x slower on MI355X: it turns the shared candidate reads
+  // of lanes in one cell into per-lane gathers again.
+  template <int D, typename F>
+  TSPH_D void visit(int64_t i, bool with_walls, F&& f) const {
+    int64_t ci[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < D; ++d) ci[d] = cells[i * D + d];
+    constexpr int NOFF = D == 1 ? 3 : (D == 2 ? 9 : 27);
+    for (int o = 0; o < NOFF; ++o) {
+      int rem = o;
+      int64_t L = 0;
+      bool inside = true;
+#pragma unroll
+      for (int d = D - 1; d >= 0; --d) {
+        int64_t nb = ci[d] + (rem % 3 - 1);
+        rem /= 3;
+        if (periodic) {
+          if (nb < 0) nb += ext[d];
+          else if (nb >= ext[d]) nb -= ext[d];
+        } else if (nb < 0 || nb >= ext[d]) {
+          inside = false;
+        }
+        L += nb * stride[d];
+      }
+      if (!inside) continue;
+      const int4 r = R[L];
+      for (int32_t j = r.x; j < r.y; ++j)
+        if (j != i) f((int64_t)j);
+      if (with_walls)
+        for (int32_t j = r.z; j < r.w; ++j)
+          if (j != i) f((int64_t)j);
+    }
+  }
+};
+
+// Dense per-cell row ranges of rows sorted by cell, fluid rows [0, nf) before
+// wall rows [nf, nt): every run of equal cells writes its begin / end.  The
+// table starts zeroed (k_cell_table_clear), so empty cells are empty ranges.
+static __global__ void k_cell_table_clear(int4* __restrict__ R, int64_t ncells) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) R[c] = make_int4(0, 0, 0, 0);
+}
+
+template <int D>
+__global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int64_t nf, int64_t s0, int64_t s1,
+                             int64_t s2, int4* __restrict__ R) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nt) return;
+  const int64_t st[3] = {s0, s1, s2};
+  auto lin = [&](int64_t k) {
+    int64_t L = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) L += cells[k * D + d] * st[d];
+    return L;
+  };
+  const int64_t L = lin(r);
+  const bool wall = r >= nf;
+  const bool first = r == 0 || r == nf || lin(r - 1) != L;
+  const bool last = r == nt - 1 || r + 1 == nf || lin(r + 1) != L;
+  int* cell = reinterpret_cast<int*>(R + L);
+  if (first) cell[wall ? 2 : 0] = (int)r;
+  if (last) cell[wall ? 3 : 1] = (int)(r + 1);
 }
 
 }  // namespace tsph

@@ -27,15 +27,18 @@
 
 namespace tsph {
 
-TSPH_D void dp_kernel(double r, double h, double sigma, double& w, double& s) {
-  const double q = r / h;
+// Wendland C2 value W and gradient factor s (grad_i W = s (x_i - x_j)) at
+// squared separation d2.  sigma dW/dq / (h r) with dW/dq = -5 q (1 - q/2)^3
+// reduces to gscale (1 - q/2)^3 for r > 1e-12 (no division per pair); at
+// smaller r the reference's clamp of r to 1e-12 is kept.
+TSPH_D void dp_kernel(double d2, const DpSim& P, double& w, double& s) {
+  const double r = sqrt(d2);
+  const double q = r * P.inv_h;
   double t = 1.0 - 0.5 * q;
   t = t < 0.0 ? 0.0 : t;
-  const double t2 = t * t;
-  w = sigma * (t2 * t2) * (2.0 * q + 1.0);
-  const double dshape = -5.0 * q * (t2 * t);
-  const double rc = r > 1.0e-12 ? r : 1.0e-12;
-  s = sigma * dshape / (h * rc);
+  const double t2 = t * t, t3 = t2 * t;
+  w = P.sigma * (t2 * t2) * (2.0 * q + 1.0);
+  s = r > 1.0e-12 ? P.gscale * t3 : P.sigma * (-5.0 * q * t3) / (P.h * 1.0e-12);
 }
 
 // separation x_t - x_o and the strict support test, unfused like the reference
@@ -47,8 +50,6 @@ TSPH_D bool dp_sep(double xt, double yt, double xo, double yo, double cut2, doub
   d2 = sx * sx + sy * sy;
   return d2 < cut2;
 }
-
-TSPH_D double dp_sigma(double h) { return 7.0 / (4.0 * kPi * h * h); }
 
 TSPH_D int dp_sim(const int32_t* ids, int64_t k) { return ids ? ids[k] : 0; }
 
@@ -99,6 +100,18 @@ TSPH_D void dp_wall_force(const DpSim& P, double gx, double gy, double nx, doubl
   fy = c * (gt * ty);
 }
 
+// ---------------------------------------------------------------- sweep 0 (per-particle volume)
+// m / rho of every fluid row, read by the pair loops of sweeps A-C instead of
+// a division per pair (the same IEEE quotient)
+__global__ void __launch_bounds__(kBlock) k_dp_prep(DpArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nf) return;
+  const int s = dp_sim(a.fsim, t);
+  if (s < 0) return;                          // a hole of the batch layout
+  if (a.active && !a.active[s]) return;
+  a.vol[t] = a.m[t] / a.rho[t];
+}
+
 // ---------------------------------------------------------------- sweep A (walls)
 __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __restrict__ woff,
                                                     const int32_t* __restrict__ widx) {
@@ -108,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
   if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
   const DpSim P = a.sims ? a.sims[s] : a.sim0;
-  const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
+  const double cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xw = a.xw[2 * w], yw = a.xw[2 * w + 1];
   double psum = 0.0, wsum = 0.0;
   int cnt = 0;
@@ -117,7 +130,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
     double sx, sy, d2;
     if (!dp_sep(a.x[2 * f], a.x[2 * f + 1], xw, yw, cut2, sx, sy, d2)) continue;   // x_f - x_w
     double wt, sc;
-    dp_kernel(sqrt(d2), P.h, sigma, wt, sc);
+    dp_kernel(d2, P, wt, sc);
     const double rf = a.rho[f];
     const double pf = (P.c0 * P.c0) * (rf - P.rho0);
     psum += wt * (pf + rf * P.g * sy);
@@ -138,12 +151,11 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
     double sx, sy, d2;
     if (!dp_sep(a.x[2 * f], a.x[2 * f + 1], xw, yw, cut2, sx, sy, d2)) continue;
     double wt, sc;
-    dp_kernel(sqrt(d2), P.h, sigma, wt, sc);
+    dp_kernel(d2, P, wt, sc);
     const double rf = a.rho[f];
-    const double pf = (P.c0 * P.c0) * (rf - P.rho0);
     double fx, fy;
-    dp_wall_force(P, sx * sc, sy * sc, nx, ny, a.v[2 * f], a.v[2 * f + 1], vwx, vwy, pf, pw, rf,
-                  a.m[f] / rf, fx, fy);
+    dp_wall_force(P, sx * sc, sy * sc, nx, ny, a.v[2 * f], a.v[2 * f + 1], vwx, vwy, (P.c0 * P.c0) * (rf - P.rho0),
+                  pw, rf, a.vol[f], fx, fy);
     rx += -fx;
     ry += -fy;
   }
@@ -160,11 +172,11 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
   if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
   const DpSim P = a.sims ? a.sims[s] : a.sim0;
-  const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
+  const double cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xt = a.x[2 * t], yt = a.x[2 * t + 1];
   const double vtx = a.v[2 * t], vty = a.v[2 * t + 1];
   const double rt = a.rho[t], mt = a.m[t];
-  const double volt = mt / rt;
+  const double volt = a.vol[t];
   const double pt = (P.c0 * P.c0) * (rt - P.rho0);
   const bool diff = P.diff_scale != 0.0;
   double nx_ = 0.0, ny_ = 0.0;
@@ -175,10 +187,10 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
       double sx, sy, d2;
       if (!dp_sep(xt, yt, a.x[2 * o], a.x[2 * o + 1], cut2, sx, sy, d2)) continue;
       double wt, sc;
-      dp_kernel(sqrt(d2), P.h, sigma, wt, sc);
+      dp_kernel(d2, P, wt, sc);
       const double gx = sx * sc, gy = sy * sc;
       const double ro = a.rho[o];
-      const double volo = a.m[o] / ro;
+      const double volo = a.vol[o];
       const double po = (P.c0 * P.c0) * (ro - P.rho0);
       const double c = -(volt * volo * (pt + po));
       double fx = c * gx, fy = c * gy;
@@ -204,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
       double sx, sy, d2;
       if (!dp_sep(xt, yt, a.xw[2 * w], a.xw[2 * w + 1], cut2, sx, sy, d2)) continue;
       double wt, sc;
-      dp_kernel(sqrt(d2), P.h, sigma, wt, sc);
+      dp_kernel(d2, P, wt, sc);
       double fx, fy;
       dp_wall_force(P, sx * sc, sy * sc, a.nw_[2 * w], a.nw_[2 * w + 1], vtx, vty, a.vw[2 * w],
                     a.vw[2 * w + 1], pt, a.wall_p[w], rt, volt, fx, fy);
@@ -245,7 +257,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
   if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
   const DpSim P = a.sims ? a.sims[s] : a.sim0;
-  const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
+  const double cut2 = (2.0 * P.h) * (2.0 * P.h);
   const double xt = a.x[2 * t], yt = a.x[2 * t + 1];
   const double ut = a.vnew[2 * t], wt_ = a.vnew[2 * t + 1];
   const double rt = a.rho[t], lt = a.lam[t];
@@ -258,18 +270,17 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
       double sx, sy, d2;
       if (!dp_sep(xt, yt, a.x[2 * o], a.x[2 * o + 1], cut2, sx, sy, d2)) continue;
       double wk, sc;
-      dp_kernel(sqrt(d2), P.h, sigma, wk, sc);
+      dp_kernel(d2, P, wk, sc);
       const double gx = sx * sc, gy = sy * sc;
-      const double ro = a.rho[o];
-      const double volo = a.m[o] / ro;
+      const double volo = a.vol[o];
       const double dp = (a.vnew[2 * o] - ut) * gx + (a.vnew[2 * o + 1] - wt_) * gy;
       div += volo * dp;
       if (diff) {
         const double proj = 0.5 * ((grt0 + a.gr[2 * o]) * sx + (grt1 + a.gr[2 * o + 1]) * sy);
-        const double contrast = (rt - ro) - proj;
+        const double contrast = (rt - a.rho[o]) - proj;
         const double radial = -(sx * gx + sy * gy);
         const double cp = contrast * radial / (d2 + 1.0e-12);
-        hr += -P.diff_scale * (a.m[o] / ro) * cp;
+        hr += -P.diff_scale * volo * cp;
       }
       const double dl = a.lam[o] - lt;
       gl0 += volo * (dl * gx);
@@ -279,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
       double sx, sy, d2;
       if (!dp_sep(xt, yt, a.xw[2 * w], a.xw[2 * w + 1], cut2, sx, sy, d2)) continue;
       double wk, sc;
-      dp_kernel(sqrt(d2), P.h, sigma, wk, sc);
+      dp_kernel(d2, P, wk, sc);
       const double nx = a.nw_[2 * w], ny = a.nw_[2 * w + 1];
       const double ns = (a.vw[2 * w] - ut) * nx + (a.vw[2 * w + 1] - wt_) * ny;
       div += 2.0 * (P.dx * P.dx) * ns * (nx * (sx * sc) + ny * (sy * sc));
@@ -298,13 +309,17 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
   a.normal[2 * t] = nx;
   a.normal[2 * t + 1] = ny;
   a.strength[t] = mg * P.h;
+  // the empty-cap test decides only for 0.2 <= lambda < 0.75 (below: surface,
+  // at or above: interior), so the second pass runs for those rows alone
   int covered = 0;
-  for (int64_t e = off[t]; e < off[t + 1]; ++e) {
-    const int64_t o = idx[e];
-    if (o >= a.nf) continue;
-    double sx, sy, d2;
-    if (!dp_sep(xt, yt, a.x[2 * o], a.x[2 * o + 1], cut2, sx, sy, d2)) continue;
-    covered += dp_occupied(-sx, -sy, nx, ny, P.h) ? 1 : 0;
+  if (lt >= 0.2 && lt < 0.75) {
+    for (int64_t e = off[t]; e < off[t + 1]; ++e) {
+      const int64_t o = idx[e];
+      if (o >= a.nf) continue;
+      double sx, sy, d2;
+      if (!dp_sep(xt, yt, a.x[2 * o], a.x[2 * o + 1], cut2, sx, sy, d2)) continue;
+      covered += dp_occupied(-sx, -sy, nx, ny, P.h) ? 1 : 0;
+    }
   }
   a.surface[t] = (lt < 0.2) || ((lt < 0.75) && covered == 0);
 }
@@ -318,9 +333,10 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
   if (s < 0) return;                          // a hole of the batch layout
   if (a.active && !a.active[s]) return;
   const DpSim P = a.sims ? a.sims[s] : a.sim0;
-  const double sigma = dp_sigma(P.h), cut2 = (2.0 * P.h) * (2.0 * P.h);
+  const double cut2 = (2.0 * P.h) * (2.0 * P.h);
   double wref, sref;
-  dp_kernel(P.dx, P.h, sigma, wref, sref);
+  dp_kernel(P.dx * P.dx, P, wref, sref);
+  const double inv_wref = 1.0 / wref;
   const double xt = a.x[2 * t], yt = a.x[2 * t + 1];
   const double rt = a.rho[t];
   double nbs = 0.0, rawx = 0.0, rawy = 0.0, nearest = INFINITY;
@@ -333,17 +349,17 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
     const double xo = fluid ? a.x[2 * o] : a.xw[2 * w], yo = fluid ? a.x[2 * o + 1] : a.xw[2 * w + 1];
     if (!dp_sep(xt, yt, xo, yo, cut2, sx, sy, d2)) continue;
     double wk, sc;
-    dp_kernel(sqrt(d2), P.h, sigma, wk, sc);
-    const double ratio = wk / wref;
+    dp_kernel(d2, P, wk, sc);
+    const double ratio = wk * inv_wref;
     const double r2 = ratio * ratio;
     const double f = 1.0 + 0.2 * (r2 * r2);
     const double repx = f * (sx * sc), repy = f * (sy * sc);
     if (fluid) {
       nbs += a.surface[o] ? 1.0 : 0.0;
-      const double den = rt + a.rho[o];
+      const double rden = 1.0 / (rt + a.rho[o]);
       const double mo = a.m[o];
-      rawx += mo * (repx / den);
-      rawy += mo * (repy / den);
+      rawx += mo * (repx * rden);
+      rawy += mo * (repy * rden);
     } else {
       const double rw = a.wall_rho[w];
       const double wr = rw * (P.dx * P.dx) / (rt + rw);
@@ -391,6 +407,7 @@ int dp_queue_step(const DpArgs& a, const int64_t* off, const int32_t* idx, const
                   const int32_t* widx, hipStream_t stream) {
   // chunked block -> XCD mapping: neighbouring rows are processed on one XCD
   // and share its L2 (round-robin blocks made every L2 fetch every neighbour)
+  if (a.nf > 0) k_dp_prep<<<(unsigned)((a.nf + kBlock - 1) / kBlock), kBlock, 0, stream>>>(a);
   if (a.nw > 0) k_dp_wall<<<grid_xcd(a.nw), kBlock, 0, stream>>>(a, woff, widx);
   if (a.nf > 0) {
     k_dp_force<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
@@ -413,6 +430,9 @@ DpSim dp_sim_params(double dx, double c0, double ref_speed, double dt, double rh
   P.rv = 8.0 * nu + alpha * P.h * c0;
   P.diff_scale = 2.0 * diffusion * P.h * c0;
   P.ref_speed = ref_speed;
+  P.inv_h = 1.0 / P.h;
+  P.sigma = 7.0 / (4.0 * kPi * P.h * P.h);
+  P.gscale = -5.0 * P.sigma / (P.h * P.h);
   return P;
 }
 
@@ -438,8 +458,9 @@ extern "C" int tsph_deltaplus_step(tsph_ctx* ctx, const tsph_dp_params* prm, con
   TSPH_REQUIRE(nw == 0 || (wall && wall_vel && wall_normal && reaction), TSPH_ERR_ARG, "NULL wall array");
   DpSim P = dp_sim_params(prm->dx, prm->sound_speed, prm->reference_speed, prm->dt, prm->rho0, prm->gravity,
                           prm->viscosity, prm->diffusion, prm->alpha);
-  // scratch: sims(1) | status | wall_p, wall_rho (nw) | lam, strength (n) | inv (4n) | gr, normal (2n) | surface
-  const size_t need = 256 + 256 + 8 * (2 * nw + 2) + 8 * (2 * n) + 8 * (4 * n) + 8 * (4 * n) + (size_t)n + 256;
+  // scratch: sims(1) | status | wall_p, wall_rho (nw) | lam, strength, vol (n) | inv (4n) |
+  //          gr, normal (2n) | surface
+  const size_t need = 256 + 256 + 8 * (2 * nw + 2) + 8 * (3 * n) + 8 * (4 * n) + 8 * (4 * n) + (size_t)n + 256;
   TSPH_TRY(ctx->s_dp.ensure(need));
   char* base = (char*)ctx->s_dp.ptr;
   DpArgs a{};
@@ -451,6 +472,7 @@ extern "C" int tsph_deltaplus_step(tsph_ctx* ctx, const tsph_dp_params* prm, con
   a.wall_rho = d; d += nw + 1;
   a.lam = d; d += n;
   a.strength = d; d += n;
+  a.vol = d; d += n;
   a.inv = d; d += 4 * n;
   a.gr = d; d += 2 * n;
   a.normal = d; d += 2 * n;

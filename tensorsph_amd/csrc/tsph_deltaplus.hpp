@@ -7,9 +7,12 @@
 namespace tsph {
 
 // Parameters of one simulation (delta_plus.py:105-130): h = 2 dx,
-// rv = 8 nu + alpha h c0 (radial viscosity), diff_scale = 2 delta h c0.
+// rv = 8 nu + alpha h c0 (radial viscosity), diff_scale = 2 delta h c0;
+// derived: 1 / h, the Wendland C2 normalisation sigma = 7 / (4 pi h^2) and the
+// gradient factor gscale = -5 sigma / h^2 (grad W = gscale (1 - q/2)^3 (x_i - x_j)).
 struct DpSim {
   double dx, h, c0, dt, rho0, g, rv, diff_scale, ref_speed;
+  double inv_h, sigma, gscale;
 };
 
 struct DpArgs {
@@ -21,6 +24,7 @@ struct DpArgs {
   const unsigned char* active;     // per simulation: 0 skips it this step, NULL: all active
   int* status;                     // per simulation: bit 0 wall density <= 0, bit 1 bad density
   const double *x, *v, *rho, *m;   // fluid state (FP64, (nf, 2) / (nf))
+  double* vol;                     // per fluid row: m / rho, set by sweep 0
   const double *xw, *vw, *nw_;     // wall stations: position, velocity, unit normal
   double *wall_p, *wall_rho, *reaction;
   double *acc, *vnew, *lam, *inv, *gr, *rho_new, *normal, *strength, *shift, *x_new;

@@ -5,13 +5,17 @@
 //
 // Layout.  Fluid rows [0, n) then wall rows [n, n + nw), positions and
 // velocities FP64 (nt, D), ping-ponged between two buffers.  At every
-// neighbour-list build the rows are re-ordered by cell key (fluid among
-// fluid, wall among wall), so a particle's neighbours sit close to it in
-// memory; orig[] maps every row back to the caller's index.
+// rebuild the particles are binned into cells of edge 2h + skin and the rows
+// re-ordered by the Morton key of their cell (fluid among fluid, wall among
+// wall), so a particle's neighbours sit close to it in memory; orig[] maps
+// every row back to the caller's index.  A dense table of per-cell row
+// ranges replaces per-particle lists: each sweep walks the 3^D cells around
+// its target in the order the reference-format lists would list them
+// (NbCells, tsph_cells.hpp).
 //
-// Verlet skin without a per-step host round trip.  The list is built at
-// 2h + skin and stays valid while no particle has moved more than skin/2
-// since the build (neighbor_search.py:219-227).  Step k's force sweep
+// Verlet skin without a per-step host round trip.  The binning at 2h + skin
+// stays valid while no particle has moved more than skin/2 since it was made
+// (neighbor_search.py:219-227).  Step k's force sweep
 // raises a device flag when its new positions break that bound; every sweep
 // of step k+1 reads the flag first and, when set, does nothing (the force
 // sweep copies the state through).  The host learns of the flag one step
@@ -47,12 +51,21 @@ __global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __r
 // a run keeps per-step load rows when it has obstacle stations or the clamp
 static inline bool load_rows(const tsph_sim* sim) { return sim->obstacle || sim->prm.obstacle_clamp; }
 
+// the cell-walk neighbour source over the rows binned at the last rebuild
+static inline NbCells sim_cells(const tsph_sim* sim) {
+  NbCells c{};
+  c.R = sim->cell_R.as<int4>();
+  c.cells = sim->cells_new.as<int64_t>();
+  for (int d = 0; d < 3; ++d) { c.ext[d] = sim->cell_ext[d]; c.stride[d] = sim->cell_stride[d]; }
+  c.periodic = 0;
+  return c;
+}
+
 // Obstacle loads of one step: the rhs_3d reaction of every obstacle station
 // (the k_wall_loads formula, dam_break_3d.py:410-422) and its moment about the
 // centre, reduced per block (partials in a fixed order, no float atomics).
 template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, const int64_t* __restrict__ woff,
-                                                       const int32_t* __restrict__ widx,
+__global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, NbCells nb,
                                                        const int64_t* __restrict__ orig, int64_t obs_lo,
                                                        int64_t obs_hi, double c0, double c1, double c2,
                                                        SimFlags* fl, int slot_in, double* __restrict__ part) {
@@ -68,12 +81,11 @@ __global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, const int
       for (int d = 0; d < D; ++d) { xw[d] = a.pos[iw * D + d]; vw[d] = a.vel[iw * D + d]; }
       const S prw = a.pr2[iw], mw = a.mass[iw], volw = a.vol[iw];
       Sum<S> r[D];
-      for (int64_t e = woff[w]; e < woff[w + 1]; ++e) {
-        const int64_t f = widx[e];
+      nb.visit<D>(iw, false, [&](int64_t f) {
         S df[D], r2 = S(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) { df[d] = (S)(a.pos[f * D + d] - xw[d]); r2 += df[d] * df[d]; }
-        if (r2 > a.skip2) continue;
+        if (r2 > a.skip2) return;
         S wgt, g;
         cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
         const S mf = a.mass[f];
@@ -90,7 +102,7 @@ __global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, const int
           const S visc_f = cv * (S)(a.vel[f * D + d] - vw[d]);
           r[d].add(pp - mf * visc_f);
         }
-      }
+      });
       double F[3] = {0, 0, 0}, R[3] = {0, 0, 0};
       const double cc[3] = {c0, c1, c2};
 #pragma unroll
@@ -131,8 +143,7 @@ __global__ void k_reduce_loads(const double* __restrict__ part, int64_t nblocks,
 
 // ------------------------------------------------------------------ step sweeps
 template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, const int64_t* __restrict__ off,
-                                                         const int32_t* __restrict__ idx, SimFlags* fl,
+__global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NbCells nbr, SimFlags* fl,
                                                          int slot_in, int write_eos) {
   if (fl->stale[slot_in]) return;
   const int64_t nb = gridDim.x;
@@ -143,20 +154,18 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, const i
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   Sum<double> acc;
-  const int64_t e1 = off[i + 1];
-  for (int64_t e = off[i]; e < e1; ++e) {
-    const int64_t j = idx[e];
+  nbr.visit<D>(i, true, [&](int64_t j) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const double df = xi[d] - a.pos[j * D + d];
       r2 += df * df;
     }
-    if (r2 > a.skip2d) continue;
+    if (r2 > a.skip2d) return;
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
     acc.add((double)a.mass[j] * w);
-  }
+  });
   double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
@@ -169,13 +178,12 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, const i
 }
 
 template <int D, typename S, int FORM>
-__global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const int64_t* __restrict__ off,
-                                                           const int32_t* __restrict__ idx,
+__global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NbCells nbr,
                                                            const S* __restrict__ rho_in,
                                                            const S* __restrict__ grad, SimFlags* fl,
                                                            int slot_in) {
   if (fl->stale[slot_in]) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[D];
 #pragma unroll
@@ -187,13 +195,11 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const
     for (int d = 0; d < D; ++d) gi[d] = grad[i * D + d];
   }
   Sum<S> rate;
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
-    if (j >= a.n) continue;
+  nbr.visit<D>(i, false, [&](int64_t j) {
     S df[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
     S radial = S(0);
@@ -208,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const
       contrast = contrast - S(0.5) * proj;
     }
     rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * (contrast * radial / (r2 + S(1e-12))));
-  }
+  });
   double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)a.dt * hi(rate);
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = (S)rho_d;
@@ -220,25 +226,22 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, const
 }
 
 template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, const int64_t* __restrict__ off,
-                                                          const int32_t* __restrict__ idx,
+__global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, NbCells nbr,
                                                           S* __restrict__ grad_out, SimFlags* fl,
                                                           int slot_in) {
   if (fl->stale[slot_in]) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   const S rhoi = a.rho[i];
   Sum<S> G[D], M[D * D];
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
-    if (j >= a.n) continue;
+  nbr.visit<D>(i, false, [&](int64_t j) {
     S df[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
     const S vj = a.mass[j] / a.rho[j];
@@ -249,7 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, const 
 #pragma unroll
       for (int c = 0; c < D; ++c) M[r * D + c].add(vj * (-(df[r] * (df[c] * g))));
     }
-  }
+  });
   double m[9], gv[3], out[3];
   double tr = 0.0;
 #pragma unroll
@@ -270,8 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, const 
 }
 
 template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, const int64_t* __restrict__ woff,
-                                                      const int32_t* __restrict__ widx, SimFlags* fl,
+__global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NbCells nbr, SimFlags* fl,
                                                       int slot_in) {
   if (fl->stale[slot_in]) return;
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,19 +283,18 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, const int6
 #pragma unroll
   for (int d = 0; d < D; ++d) xw[d] = a.pos[iw * D + d];
   Sum<S> num, den;
-  for (int64_t e = woff[w]; e < woff[w + 1]; ++e) {
-    const int64_t f = widx[e];
+  nbr.visit<D>(iw, false, [&](int64_t f) {
     S rw[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { rw[d] = (S)(xw[d] - a.pos[f * D + d]); r2 += rw[d] * rw[d]; }
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S wgt, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
     S src = a.p[f];
     if (a.gravity != S(0)) src = src + a.rho[f] * (-a.gravity * rw[1]);
     num.add(src * wgt);
     den.add(wgt);
-  }
+  });
   const S dd = den.get() > S(1e-30) ? den.get() : S(1e-30);
   S pw = num.get() / dd;
   pw = pw > S(0) ? pw : S(0);
@@ -319,13 +320,12 @@ TSPH_D double wave_sum(double v) {
 }
 
 template <int D, typename S, bool METERS>
-__global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, const int64_t* __restrict__ off,
-                                                       const int32_t* __restrict__ idx, S* __restrict__ acc_out,
+__global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NbCells nbr, S* __restrict__ acc_out,
                                                        StepArgs st, SimFlags* fl, int slot_in, double half_skin) {
   const int64_t nb = gridDim.x;
   const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
   if (fl->stale[slot_in]) {
-    // list no longer valid: carry the state through unchanged, keep the flag up
+    // binning no longer valid: carry the state through unchanged, keep the flag up
     if (i < a.n) {
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -345,13 +345,11 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, const int
     for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
     const S pri = a.pr2[i];
     Sum<S> ap[D], av[D];
-    const int64_t e1 = off[i + 1];
-    for (int64_t e = off[i]; e < e1; ++e) {
-      const int64_t j = idx[e];
+    nbr.visit<D>(i, true, [&](int64_t j) {
       S df[D], r2 = S(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
-      if (r2 > a.skip2) continue;
+      if (r2 > a.skip2) return;
       S w, g;
       cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
       const S cp = -a.mass[j] * (pri + a.pr2[j]);
@@ -367,7 +365,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, const int
 #pragma unroll
         for (int d = 0; d < D; ++d) av[d].add(cv * (S)(vi[d] - a.vel[j * D + d]));
       }
-    }
+    });
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
     if (acc_out) {
@@ -535,37 +533,6 @@ __global__ void k_remap_order(int64_t* __restrict__ order, const int64_t* __rest
   if (s < nt) order[s] = old2new[order[s]];
 }
 
-// MODE 0 count / 1 fill.  Fluid targets list every neighbour (fluid or wall)
-// within the build radius; wall targets list their fluid neighbours.
-template <int D, int MODE>
-__global__ void k_build_lists(const double* __restrict__ pos, int64_t n, int64_t nt,
-                              const int64_t* __restrict__ cells, GridDesc g,
-                              const int64_t* __restrict__ ukeys, const int64_t* __restrict__ ustart,
-                              const int64_t* __restrict__ order, int64_t ncells, double c2,
-                              unsigned long long* __restrict__ cnt, const int64_t* __restrict__ off,
-                              int32_t* __restrict__ idx, unsigned long long* __restrict__ wcnt,
-                              const int64_t* __restrict__ woff, int32_t* __restrict__ widx) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nt) return;
-  const double box[3] = {0, 0, 0};
-  int64_t k = 0;
-  const bool fluid = i < n;
-  int64_t base = 0;
-  if (MODE == 1) base = fluid ? off[i] : woff[i - n];
-  for_neighbours<double, D, false>(pos, i, cells, g, ukeys, ustart, order, ncells, c2, box,
-                                   [&](int64_t j) {
-                                     if (!fluid && j >= n) return;
-                                     if (MODE == 1) {
-                                       if (fluid) idx[base + k] = (int32_t)j;
-                                       else widx[base + k] = (int32_t)j;
-                                     }
-                                     ++k;
-                                   });
-  if (MODE == 0) {
-    if (fluid) cnt[i] = k; else wcnt[i - n] = k;
-  }
-}
-
 // ------------------------------------------------------------------ scatter back
 template <int D, typename S>
 __global__ void k_unpermute(const int64_t* __restrict__ orig, int64_t n, const double* __restrict__ pos,
@@ -589,19 +556,18 @@ __global__ void k_unpermute(const int64_t* __restrict__ orig, int64_t n, const d
 // unordered pairs with |x_i - x_j| <= radius: fluid-fluid pairs are listed
 // from both ends (counted at the lower row), fluid-wall pairs once
 template <int D>
-__global__ void k_count_close(const double* __restrict__ pos, int64_t n, const int64_t* __restrict__ off,
-                              const int32_t* __restrict__ idx, double r2max, unsigned long long* cnt) {
+__global__ void k_count_close(const double* __restrict__ pos, int64_t n, NbCells nbr, double r2max,
+                              unsigned long long* cnt) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
   if (i < n) {
-    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-      const int64_t j = idx[e];
-      if (j < n && j < i) continue;
+    nbr.visit<D>(i, true, [&](int64_t j) {
+      if (j < n && j < i) return;
       double r2 = 0.0;
 #pragma unroll
       for (int d = 0; d < D; ++d) { const double df = pos[i * D + d] - pos[j * D + d]; r2 += df * df; }
       if (r2 <= r2max) ++c;
-    }
+    });
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
@@ -624,7 +590,9 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   const int64_t n = sim->n, nt = sim->nt;
   const double rc = 2.0 * sim->prm.op.h + sim->prm.skin;
   const double* pos = sim->pos[sim->cur].as<double>();
-  TSPH_TRY((run_search<double, D>(ctx, pos, nt, rc, nullptr, stream)));
+  // rows in Morton order of their cells: a run of consecutive targets and
+  // all their neighbours sit in a compact block of memory (L2 reuse)
+  TSPH_TRY((run_search<double, D>(ctx, pos, nt, rc, nullptr, stream, true)));
   CellSearch& cs = ctx->search;
   const unsigned g = grid_for(nt);
   // new row of every particle: fluid by cell order, walls by cell order
@@ -651,30 +619,19 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   std::swap(sim->orig, sim->orig2);
   sim->cur = other;
   const double* npos = sim->pos[sim->cur].as<double>();
-  // neighbour lists on the re-ordered rows
-  GridDesc gd;
-  for (int d = 0; d < 3; ++d) { gd.ext[d] = cs.ext[d]; gd.stride[d] = cs.stride[d]; }
-  ctx->nb_targets = n; ctx->wnb_targets = nt - n;
-  TSPH_TRY(ctx->nb_off.ensure(sizeof(int64_t) * (n + 1)));
-  TSPH_TRY(ctx->wnb_off.ensure(sizeof(int64_t) * (nt - n + 1)));
-  TSPH_TRY(ctx->nb_cursor.ensure(sizeof(int64_t) * (n + 1)));
-  TSPH_TRY(ctx->wnb_cursor.ensure(sizeof(int64_t) * (nt - n + 1)));
-  unsigned long long* cnt = ctx->nb_cursor.as<unsigned long long>();
-  unsigned long long* wcnt = ctx->wnb_cursor.as<unsigned long long>();
-  k_build_lists<D, 0><<<g, kBlock, 0, stream>>>(npos, n, nt, sim->cells_new.as<int64_t>(), gd, cs.ukeys.as<int64_t>(),
-                                                cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells,
-                                                rc * rc, cnt, nullptr, nullptr, wcnt, nullptr, nullptr);
-  TSPH_CHECK_LAUNCH();
-  TSPH_TRY(exclusive_scan(ctx, cnt, n, ctx->nb_off.as<int64_t>(), stream, &ctx->nb_entries));
-  TSPH_TRY(exclusive_scan(ctx, wcnt, nt - n, ctx->wnb_off.as<int64_t>(), stream, &ctx->wnb_entries));
-  TSPH_REQUIRE(ctx->nb_entries < (1ll << 40), TSPH_ERR_OOM, "neighbour list too large");
-  TSPH_TRY(ctx->nb_idx.ensure(sizeof(int32_t) * (ctx->nb_entries + 1)));
-  TSPH_TRY(ctx->wnb_idx.ensure(sizeof(int32_t) * (ctx->wnb_entries + 1)));
-  k_build_lists<D, 1><<<g, kBlock, 0, stream>>>(npos, n, nt, sim->cells_new.as<int64_t>(), gd, cs.ukeys.as<int64_t>(),
-                                                cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells,
-                                                rc * rc, nullptr, ctx->nb_off.as<int64_t>(),
-                                                ctx->nb_idx.as<int32_t>(), nullptr, ctx->wnb_off.as<int64_t>(),
-                                                ctx->wnb_idx.as<int32_t>());
+  // dense per-cell row ranges of the re-ordered rows: the sweeps walk the
+  // 3^D cells around each target instead of per-target lists
+  double dense = 1.0;
+  for (int d = 0; d < D; ++d) dense *= (double)cs.ext[d];
+  TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG,
+               "cell grid of %.3g cells exceeds the dense range table (%lld cells): the particle "
+               "cloud is too sparse for its cutoff", dense, (long long)kMaxDenseCells);
+  sim->ncells_dense = (int64_t)dense;
+  for (int d = 0; d < 3; ++d) { sim->cell_ext[d] = cs.ext[d]; sim->cell_stride[d] = cs.stride[d]; }
+  TSPH_TRY(sim->cell_R.ensure(sizeof(int4) * (size_t)sim->ncells_dense));
+  k_cell_table_clear<<<grid_for(sim->ncells_dense), kBlock, 0, stream>>>(sim->cell_R.as<int4>(), sim->ncells_dense);
+  k_cell_table<D><<<g, kBlock, 0, stream>>>(sim->cells_new.as<int64_t>(), nt, n, cs.stride[0], cs.stride[1],
+                                            cs.stride[2], sim->cell_R.as<int4>());
   TSPH_CHECK_LAUNCH();
   // both ping-pong buffers carry the (static) wall rows
   if (nt > n) {
@@ -703,7 +660,7 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   return a;
 }
 
-// The CSR-list sweeps of one step (used where no cell-tile path exists).
+// The cell-walk sweeps of one step (used where no cell-tile path exists).
 template <int D, typename S>
 int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   tsph_ctx* ctx = sim->ctx;
@@ -711,8 +668,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const int slot = (int)(k & 1);
   SimFlags* fl = sim->flags.as<SimFlags>();
   WcsphArgs<S> a = sim_args<D, S>(sim);
-  const int64_t* off = ctx->nb_off.as<int64_t>();
-  const int32_t* idx = ctx->nb_idx.as<int32_t>();
+  const NbCells nbr = sim_cells(sim);
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
@@ -720,20 +676,19 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     if (sizeof(S) == 4) a.rho_lo = sim->rho_lo.as<S>();
     WcsphArgs<S> a0 = a;
     a0.rho = sim->rho_raw.as<S>();
-    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a0, off, idx, fl, slot, 0);
+    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0);
     if (sim->prm.op.diffusion_form == 1) {
-      k_step_gradient<D, S><<<grid_for(n), kBlock, 0, stream>>>(a0, off, idx, sim->grad.as<S>(), fl, slot);
-      k_step_diffusion<D, S, 1><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx, a0.rho, sim->grad.as<S>(), fl, slot);
+      k_step_gradient<D, S><<<gx, kBlock, 0, stream>>>(a0, nbr, sim->grad.as<S>(), fl, slot);
+      k_step_diffusion<D, S, 1><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, sim->grad.as<S>(), fl, slot);
     } else {
-      k_step_diffusion<D, S, 0><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx, a0.rho, nullptr, fl, slot);
+      k_step_diffusion<D, S, 0><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, nullptr, fl, slot);
     }
   } else {
-    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a, off, idx, fl, slot, 1);
+    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
   }
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   if (nw > 0)
-    k_step_wall<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(),
-                                                           ctx->wnb_idx.as<int32_t>(), fl, slot);
+    k_step_wall<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.dt = sim->prm.op.dt;
@@ -759,7 +714,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     // the force sweep uses); the force sweep writes the other pos / vel side
     const int64_t lo = n + sim->prm.obstacle_begin, hi = n + sim->prm.obstacle_end;
     const double* c = sim->prm.obstacle_centre;
-    k_step_loads<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(), ctx->wnb_idx.as<int32_t>(),
+    k_step_loads<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr,
                                                              sim->orig.as<int64_t>(), lo, hi, c[0], c[1], c[2], fl,
                                                              slot, sim->load_part.as<double>());
     k_reduce_loads<<<1, 64, 0, stream>>>(sim->load_part.as<double>(), grid_for(nw), sim->loads_out.as<double>(), fl,
@@ -767,10 +722,10 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     TSPH_CHECK_LAUNCH();
   }
   if (meters)
-    k_step_force<D, S, true><<<gx, kBlock, 0, stream>>>(a, off, idx, sim->acc.as<S>(), st, fl, slot,
+    k_step_force<D, S, true><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
                                                         0.5 * sim->prm.skin);
   else
-    k_step_force<D, S, false><<<gx, kBlock, 0, stream>>>(a, off, idx, sim->acc.as<S>(), st, fl, slot,
+    k_step_force<D, S, false><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
                                                          0.5 * sim->prm.skin);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
@@ -787,8 +742,6 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   const int slot = (int)(k & 1);
   SimFlags* fl = sim->flags.as<SimFlags>();
   WcsphArgs<S> a = sim_args<D, S>(sim);
-  const int64_t* off = ctx->nb_off.as<int64_t>();
-  const int32_t* idx = ctx->nb_idx.as<int32_t>();
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
   if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
@@ -965,7 +918,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemsetAsync(sim->clamp_acc.ptr, 0, 8 * 4, stream);
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
-  // obstacle loads run on the CSR-list sweeps
+  // obstacle loads run on the cell-walk sweeps
   if (sim->dim == 2 && !sim->obstacle) { sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4; }
   *out = sim;
   return TSPH_OK;
@@ -1008,7 +961,7 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
     counters[0] = sim->steps_done;
     counters[1] = sim->rebuilds;
     counters[2] = sim->escaped_total;
-    counters[3] = sim->ctx->nb_entries;
+    counters[3] = sim->ncells_dense;
     counters[4] = (int64_t)(sim->rows.size() / (1 + MS_NSUM + MM_NMAX + MN_NMIN));
   }
   if (peaks) { peaks[0] = sim->peak_rho; peaks[1] = sim->peak_speed; }
@@ -1042,7 +995,7 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
-  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the CSR-list sweeps");
+  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the cell-walk sweeps");
   TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
                               tile_x * tile_y <= 64),
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
@@ -1066,7 +1019,7 @@ int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps)
 
 int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream) {
   TSPH_REQUIRE(sim && pairs && radius > 0.0, TSPH_ERR_ARG, "bad arguments");
-  TSPH_REQUIRE(sim->built, TSPH_ERR_ARG, "no neighbour list yet: step the simulation first");
+  TSPH_REQUIRE(sim->built, TSPH_ERR_ARG, "no cell binning yet: step the simulation first");
   tsph_ctx* ctx = sim->ctx;
   TSPH_TRY(ctx->s_red.ensure(sizeof(unsigned long long)));
   unsigned long long* cnt = ctx->s_red.as<unsigned long long>();
@@ -1077,14 +1030,12 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
   if (sim->tiles) {
     TSPH_TRY(tile_count_pairs(sim, radius, cnt, stream));
   } else {
-    TSPH_REQUIRE(ctx->nb_off.ptr && ctx->nb_idx.ptr && ctx->nb_targets == sim->n, TSPH_ERR_ARG,
-                 "no neighbour lists built");
     if (sim->dim == 2)
       k_count_close<2><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
-          ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+                                                                sim_cells(sim), r2, cnt);
     else
       k_count_close<3><<<grid_for(sim->n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->n,
-          ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), r2, cnt);
+                                                                sim_cells(sim), r2, cnt);
     TSPH_CHECK_LAUNCH();
   }
   unsigned long long h = 0;
@@ -1099,7 +1050,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
-                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->ranges,
+                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
                     &sim->load_part, &sim->loads_out, &sim->clamp_acc};
   for (DevBuf* b : bufs) b->release();

@@ -49,6 +49,8 @@ struct tsph_sim {
   tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
+  tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell-walk sweeps)
+  int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   tsph::StepRecord* ring = nullptr;       // pinned
   hipEvent_t ev[tsph::kRing] = {};
   // per-sweep timing (HIP events around each sweep on the stepping stream)
@@ -62,7 +64,7 @@ struct tsph_sim {
   int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
   bool built = false;
   // cell-tile mode (2-D): rows sorted tile-major, dense per-cell row ranges
-  int tiles = 0;                 // 1: LDS cell-tile sweeps, 0: CSR lists
+  int tiles = 0;                 // 1: LDS cell-tile sweeps, 0: cell-walk sweeps
   int tile_dim[3] = {4, 4, 1};
   int64_t ext[3] = {1, 1, 1};
   int64_t ntile[3] = {1, 1, 1};

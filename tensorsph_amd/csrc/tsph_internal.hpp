@@ -72,6 +72,7 @@ struct CellSearch {
   int64_t n = 0, ncells = 0;           // particles, occupied cells
   int dim = 0;
   int periodic = 0;
+  int morton = 0;                      // keys are Morton codes of the cells (resident steppers)
   int64_t cmin[3] = {0, 0, 0};
   int64_t ext[3] = {1, 1, 1};
   int64_t stride[3] = {1, 1, 1};
@@ -94,7 +95,7 @@ int exclusive_scan(tsph_ctx* ctx, const unsigned long long* cnt, int64_t n, int6
                    hipStream_t stream, int64_t* total);
 template <typename T, int D>
 int run_search(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const double* box,
-               hipStream_t stream);
+               hipStream_t stream, bool morton = false);
 
 }  // namespace tsph
 

@@ -104,19 +104,20 @@ __global__ void k_init_minmax(unsigned long long* mm) {
 
 template <int D>
 __global__ void k_keys(int64_t* __restrict__ cells, int64_t n, int64_t c0, int64_t c1, int64_t c2,
-                       int64_t s0, int64_t s1, int64_t s2, int normalise,
+                       int64_t s0, int64_t s1, int64_t s2, int normalise, int morton,
                        int64_t* __restrict__ keys, int32_t* __restrict__ idx) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t cm[3] = {c0, c1, c2}, st[3] = {s0, s1, s2};
-  int64_t key = 0;
+  int64_t key = 0, cc[3] = {0, 0, 0};
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     int64_t c = cells[i * D + d];
     if (normalise) { c -= cm[d]; cells[i * D + d] = c; }
+    cc[d] = c;
     key += c * st[d];
   }
-  keys[i] = key;
+  keys[i] = morton ? morton_key<D>(cc) : key;
   idx[i] = (int32_t)i;
 }
 
@@ -248,9 +249,9 @@ int exclusive_scan(tsph_ctx* ctx, const unsigned long long* cnt, int64_t n, int6
 // ---------------------------------------------------------------- search driver
 template <typename T, int D>
 int run_search(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const double* box,
-               hipStream_t stream) {
+               hipStream_t stream, bool morton) {
   CellSearch& cs = ctx->search;
-  cs.n = n; cs.dim = D; cs.periodic = box != nullptr; cs.ncells = 0;
+  cs.n = n; cs.dim = D; cs.periodic = box != nullptr; cs.ncells = 0; cs.morton = morton ? 1 : 0;
   if (n == 0) return TSPH_OK;
   TSPH_TRY(cs.cells.ensure(sizeof(int64_t) * n * D));
   TSPH_TRY(cs.keys.ensure(sizeof(int64_t) * n));
@@ -303,15 +304,24 @@ int run_search(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const doub
   cs.stride[D - 1] = 1;
   for (int d = D - 2; d >= 0; --d) cs.stride[d] = cs.stride[d + 1] * cs.ext[d + 1];
   for (int d = D; d < 3; ++d) cs.stride[d] = 0;
+  int axis_bits = 1;
+  if (morton) {
+    int64_t emax = 1;
+    for (int d = 0; d < D; ++d) emax = cs.ext[d] > emax ? cs.ext[d] : emax;
+    while (axis_bits < 31 && (1ll << axis_bits) < emax) ++axis_bits;
+    TSPH_REQUIRE(axis_bits <= 63 / D, TSPH_ERR_ARG, "cell grid of %lld cells per axis exceeds the %d-bit "
+                 "Morton key range", (long long)emax, 63 / D);
+  }
   k_keys<D><<<grid, kBlock, 0, stream>>>(cs.cells.as<int64_t>(), n, cs.cmin[0], cs.cmin[1], cs.cmin[2],
                                          cs.stride[0], cs.stride[1], cs.stride[2], box ? 0 : 1,
-                                         cs.keys.as<int64_t>(), cs.idx_in.as<int32_t>());
+                                         morton ? 1 : 0, cs.keys.as<int64_t>(), cs.idx_in.as<int32_t>());
   TSPH_CHECK_LAUNCH();
   // stable LSD radix sort over the key bits actually used
   double maxkey = 1.0;
   for (int d = 0; d < D; ++d) maxkey *= (double)cs.ext[d];
   int end_bit = 1;
   while (end_bit < 63 && (double)(1ull << end_bit) < maxkey) ++end_bit;
+  if (morton) end_bit = D * axis_bits;
   size_t bytes = 0;
   TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortPairs(
       nullptr, bytes, (const unsigned long long*)cs.keys.ptr, (unsigned long long*)cs.skeys.ptr,
@@ -349,6 +359,7 @@ int search_pairs(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const do
   CellSearch& cs = ctx->search;
   GridDesc g;
   for (int d = 0; d < 3; ++d) { g.ext[d] = cs.ext[d]; g.stride[d] = cs.stride[d]; }
+  g.morton = cs.morton;
   const T c2 = (T)(cutoff * cutoff);
   T b[3] = {T(0), T(0), T(0)};
   if (box) for (int d = 0; d < D; ++d) b[d] = (T)box[d];
@@ -381,6 +392,7 @@ int fill_pairs(tsph_ctx* ctx, const T* pos, double cutoff, const double* box, in
   const int64_t n = cs.n;
   GridDesc g;
   for (int d = 0; d < 3; ++d) { g.ext[d] = cs.ext[d]; g.stride[d] = cs.stride[d]; }
+  g.morton = cs.morton;
   const T c2 = (T)(cutoff * cutoff);
   T b[3] = {T(0), T(0), T(0)};
   if (box) for (int d = 0; d < D; ++d) b[d] = (T)box[d];
@@ -437,8 +449,8 @@ __global__ void k_max_disp(const T* __restrict__ pos, const T* __restrict__ anch
   }
 }
 
-template int run_search<double, 2>(tsph_ctx*, const double*, int64_t, double, const double*, hipStream_t);
-template int run_search<double, 3>(tsph_ctx*, const double*, int64_t, double, const double*, hipStream_t);
+template int run_search<double, 2>(tsph_ctx*, const double*, int64_t, double, const double*, hipStream_t, bool);
+template int run_search<double, 3>(tsph_ctx*, const double*, int64_t, double, const double*, hipStream_t, bool);
 
 }  // namespace tsph
 

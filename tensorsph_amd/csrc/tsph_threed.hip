@@ -32,6 +32,7 @@ template <typename S> struct ThreedArgs {
   double eig_floor;
   double rho0d;
   double box[3];
+  double inv_box[3];   // 1 / box: the image shift rint(v / L) is exact with v * (1 / L) for |v - k L| < L / 3
   double grav[3];
   int periodic, renormalise, has_gravity, adami;
   int do_pressure;   // 0: k3_force skips the pressure term (viscous_acceleration alone)
@@ -55,7 +56,7 @@ TSPH_D void delta3(const ThreedArgs<S>& a, int64_t j, const double* xi, S* df, S
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     double v = xi[d] - a.pos[j * 3 + d];
-    if (min_image) v = v - a.box[d] * rint(v / a.box[d]);
+    if (min_image) v = v - a.box[d] * rint(v * a.inv_box[d]);
     df[d] = (S)v;
     r2 += df[d] * df[d];
   }
@@ -67,35 +68,32 @@ TSPH_D void load3(const double* p, int64_t i, double* x) {
 
 // ---------------------------------------------------------------- density
 // summation_density (operators3.py:70-79): pair sums, then the self term
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                     const int32_t* __restrict__ idx, S* __restrict__ out) {
+template <typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
   Sum<double> acc;
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
+  nb.template visit<3>(i, true, [&](int64_t j) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double v = xi[d] - a.pos[j * 3 + d];
-      if (a.periodic) v = v - a.box[d] * rint(v / a.box[d]);
+      if (a.periodic) v = v - a.box[d] * rint(v * a.inv_box[d]);
       r2 += v * v;
     }
-    if (r2 > a.skip2d) continue;
+    if (r2 > a.skip2d) return;
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
     acc.add((double)a.mass[j] * w);
-  }
+  });
   store_rho<S>(hi(acc) + (double)a.mass[i] * a.kd.w0d, out, a.rho_lo, i);
 }
 
 // renormalised density gradient over fluid-fluid pairs (density_diffusion.py:95-142)
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                      const int32_t* __restrict__ idx, const S* __restrict__ rho) {
+template <typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, NB nb, const S* __restrict__ rho) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   if (i >= a.nf) { a.gr[3 * i] = a.gr[3 * i + 1] = a.gr[3 * i + 2] = S(0); return; }
@@ -103,12 +101,11 @@ __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, const int
   load3(a.pos, i, xi);
   const S rhoi = rho[i];
   Sum<S> G[3], M[9];
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
-    if (j >= a.nf) continue;
+  nb.template visit<3>(i, true, [&](int64_t j) {
+    if (j >= a.nf) return;
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
     const S vj = a.mass[j] / rho[j];
@@ -119,7 +116,7 @@ __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, const int
 #pragma unroll
       for (int c = 0; c < 3; ++c) M[r * 3 + c].add(vj * (-(df[r] * (df[c] * g))));
     }
-  }
+  });
   double m[9], gv[3], out[3], tr = 0.0;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
@@ -137,9 +134,8 @@ __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, const int
 // Optional density diffusion over fluid-fluid pairs (density_diffusion_rate_3d,
 // operators3.py:259-295; rho += dt rate, floored at 0.1 rho0 for every row),
 // then the Tait pressure without a ratio floor (operators3.py:82-86).
-template <typename S, int FORM>
-__global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                 const int32_t* __restrict__ idx, const S* __restrict__ rho_in,
+template <typename S, int FORM, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S* __restrict__ rho_in,
                                                  int diffusion) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -151,12 +147,11 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, const int64_t*
       load3(a.pos, i, xi);
       S gi[3] = {S(0), S(0), S(0)};
       if (FORM == 1) { gi[0] = a.gr[3 * i]; gi[1] = a.gr[3 * i + 1]; gi[2] = a.gr[3 * i + 2]; }
-      for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-        const int64_t j = idx[e];
-        if (j >= a.nf) continue;
+      nb.template visit<3>(i, true, [&](int64_t j) {
+        if (j >= a.nf) return;
         S df[3], r2;
         delta3<S>(a, j, xi, df, r2, a.periodic);
-        if (r2 > a.skip2) continue;
+        if (r2 > a.skip2) return;
         S w, g;
         cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
         S radial = S(0);
@@ -171,7 +166,7 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, const int64_t*
           contrast = contrast - S(0.5) * proj;
         }
         rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * (contrast * radial / (r2 + S(1e-12))));
-      }
+      });
     }
     if (a.rate_out) a.rate_out[i] = rate.get();
     rho_d = rho_d + (double)a.dt * hi(rate);
@@ -187,20 +182,18 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, const int64_t*
 // Adami wall pressure (wall_pressure_adami, operators3.py:173-209): raw
 // separations, p = 0 without a fluid neighbour, and tait_density
 // (operators3.py:89-93) without a floor for the wall density.
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                   const int32_t* __restrict__ idx) {
+template <typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
   const int64_t w = a.nf + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.n) return;
   double xw[3];
   load3(a.pos, w, xw);
   Sum<S> num, den;
-  for (int64_t e = off[w]; e < off[w + 1]; ++e) {
-    const int64_t f = idx[e];
-    if (f >= a.nf) continue;
+  nb.template visit<3>(w, true, [&](int64_t f) {
+    if (f >= a.nf) return;
     S df[3], r2;
     delta3<S>(a, f, xw, df, r2, false);      // x_w - x_f
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S wt, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, wt, g);
     S donor = a.p[f];
@@ -210,7 +203,7 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, const int64_
     }
     num.add(donor * wt);
     den.add(wt);
-  }
+  });
   const S dn = den.get();
   const S dd = dn > S(1e-30) ? dn : S(1e-30);
   S pw = dn > S(0) ? num.get() / dd : S(0);
@@ -220,25 +213,23 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, const int64_
 }
 
 // second moment of the Laplacian (viscous_acceleration, operators3.py:137-145)
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                    const int32_t* __restrict__ idx) {
+template <typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
   const S ri = a.rho_u[i];
   Sum<S> acc;
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
+  nb.template visit<3>(i, true, [&](int64_t j) {
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S w, g, lap;
     cubic_wgl<S>(sqrt_s<S>(r2), a.k, w, g, lap);
     const S rij = S(0.5) * (ri + a.rho_u[j]);
     acc.add(r2 * lap * (a.mass[j] / rij));
-  }
+  });
   a.moment[i] = acc.get() / S(3);
 }
 
@@ -248,9 +239,8 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, const int64
 // ``orig`` maps rows to the caller's particle order (NULL: identity); the
 // reference weights the viscous pair term by the mass of the pair member with
 // the higher index, so that order matters.
-template <typename S, bool KICK>
-__global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, const int64_t* __restrict__ off,
-                                                   const int32_t* __restrict__ idx, const int64_t* __restrict__ orig,
+template <typename S, bool KICK, typename NB>
+__global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -264,11 +254,10 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, const int64_
   const S Ki = (visc && a.renormalise) ? a.moment[i] : S(0);
   const int64_t oi = orig ? orig[i] : i;
   Sum<S> ap[3], av[3];
-  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
-    const int64_t j = idx[e];
+  nb.template visit<3>(i, true, [&](int64_t j) {
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
-    if (r2 > a.skip2) continue;
+    if (r2 > a.skip2) return;
     S w, g, lap;
     cubic_wgl<S>(sqrt_s<S>(r2), a.k, w, g, lap);
     const S rj = a.rho_u[j];
@@ -291,7 +280,7 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, const int64_
 #pragma unroll
       for (int d = 0; d < 3; ++d) av[d].add((S)(a.vel[3 * j + d] - vi[d]) * wgt);
     }
-  }
+  });
   S acc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -338,7 +327,7 @@ ThreedArgs<S> make_threed_args(const ThreedPrm& q, int64_t n, int64_t nf) {
   a.eig_floor = q.eig_floor;
   a.dt = S(q.dt);
   a.periodic = q.periodic;
-  for (int d = 0; d < 3; ++d) { a.box[d] = q.box[d]; a.grav[d] = q.grav[d]; }
+  for (int d = 0; d < 3; ++d) { a.box[d] = q.box[d]; a.inv_box[d] = 1.0 / q.box[d]; a.grav[d] = q.grav[d]; }
   a.renormalise = q.renormalise;
   a.has_gravity = q.has_grav;
   a.adami = q.adami;
@@ -353,30 +342,30 @@ template <typename S> struct ThreedScratch {
   S *rho_raw, *rho, *rho_u, *p, *moment, *gr, *rho_lo;   // rho_lo: NULL in FP64 mode
 };
 
-// Queues the sweeps of one accelerations() evaluation on the CSR lists.
-template <typename S, bool KICK>
-int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b, const int64_t* off,
-                  const int32_t* idx, const int64_t* orig, S* acc_out, double half_dt, double* vel_out,
-                  hipStream_t stream) {
+// Queues the sweeps of one accelerations() evaluation over a neighbour source
+// (CSR lists or the cell walk).
+template <typename S, bool KICK, typename NB>
+int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b, NB nb, const int64_t* orig,
+                  S* acc_out, double half_dt, double* vel_out, hipStream_t stream) {
   const int64_t n = a.n;
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
   const unsigned g = grid_for(n);
   const bool diff = q.diff_scale > 0.0;
-  k3_density<S><<<g, kBlock, 0, stream>>>(a, off, idx, diff ? b.rho_raw : b.rho);
+  k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho);
   if (diff) {
     if (q.diff_form == 1) {
-      k3_gradient<S><<<g, kBlock, 0, stream>>>(a, off, idx, b.rho_raw);
-      k3_eos<S, 1><<<g, kBlock, 0, stream>>>(a, off, idx, b.rho_raw, 1);
+      k3_gradient<S, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho_raw);
+      k3_eos<S, 1, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho_raw, 1);
     } else {
-      k3_eos<S, 0><<<g, kBlock, 0, stream>>>(a, off, idx, b.rho_raw, 1);
+      k3_eos<S, 0, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho_raw, 1);
     }
   } else {
-    k3_eos<S, 0><<<g, kBlock, 0, stream>>>(a, off, idx, b.rho, 0);
+    k3_eos<S, 0, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho, 0);
   }
-  if (a.nf < n && a.adami) k3_adami<S><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, off, idx);
-  if (a.nu != S(0) && a.renormalise) k3_moment<S><<<g, kBlock, 0, stream>>>(a, off, idx);
-  k3_force<S, KICK><<<g, kBlock, 0, stream>>>(a, off, idx, orig, acc_out, half_dt, vel_out);
+  if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, nb);
+  if (a.nu != S(0) && a.renormalise) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
+  k3_force<S, KICK, NB><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }
@@ -407,8 +396,8 @@ int threed_accel(tsph_ctx* ctx, const tsph_threed_params* prm, const double* pos
   const ThreedPrm q = threed_prm(*prm);
   ThreedArgs<S> a = make_threed_args<S>(q, n, prm->n_fluid);
   a.pos = pos; a.vel = vel; a.mass = (const S*)mass;
-  TSPH_TRY((threed_sweeps<S, false>(a, q, b, ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>(), nullptr,
-                                    (S*)acc, 0.0, nullptr, stream)));
+  const NbList nb{ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>()};
+  TSPH_TRY((threed_sweeps<S, false>(a, q, b, nb, nullptr, (S*)acc, 0.0, nullptr, stream)));
   if (rho) TSPH_CHECK_HIP(hipMemcpyAsync(rho, b.rho, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
   if (p) TSPH_CHECK_HIP(hipMemcpyAsync(p, b.p, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
   return TSPH_OK;
@@ -426,39 +415,38 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
   ThreedArgs<S> a = make_threed_args<S>(q, n, nf);
   a.pos = pos; a.vel = vel; a.mass = (const S*)mass;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr;
-  const int64_t* off = ctx->nb_off.as<int64_t>();
-  const int32_t* idx = ctx->nb_idx.as<int32_t>();
+  const NbList nb{ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>()};
   const unsigned g = grid_for(n);
   S* rho_in = (S*)rho;           // read only
   S* p_in = (S*)p;               // read only
   switch (op) {
     case 0:
-      k3_density<S><<<g, kBlock, 0, stream>>>(a, off, idx, (S*)out);
+      k3_density<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, (S*)out);
       break;
     case 1:
       a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0;
-      k3_force<S, false><<<g, kBlock, 0, stream>>>(a, off, idx, nullptr, (S*)out, 0.0, nullptr);
+      k3_force<S, false, NbList><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 2:
       a.rho_u = rho_in; a.do_pressure = 0; a.has_gravity = 0;
-      if (a.nu != S(0) && a.renormalise) k3_moment<S><<<g, kBlock, 0, stream>>>(a, off, idx);
-      k3_force<S, false><<<g, kBlock, 0, stream>>>(a, off, idx, nullptr, (S*)out, 0.0, nullptr);
+      if (a.nu != S(0) && a.renormalise) k3_moment<S, NbList><<<g, kBlock, 0, stream>>>(a, nb);
+      k3_force<S, false, NbList><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 3:
       if (nf == n) return TSPH_OK;
       TSPH_CHECK_HIP(hipMemcpyAsync(b.p, p_in, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
       a.rho = rho_in;
-      k3_adami<S><<<grid_for(n - nf), kBlock, 0, stream>>>(a, off, idx);
+      k3_adami<S, NbList><<<grid_for(n - nf), kBlock, 0, stream>>>(a, nb);
       TSPH_CHECK_LAUNCH();
       TSPH_CHECK_HIP(hipMemcpyAsync(out, b.p + nf, sizeof(S) * (n - nf), hipMemcpyDeviceToDevice, stream));
       break;
     case 4:
       a.rate_out = (S*)out;
       if (q.diff_form == 1) {
-        k3_gradient<S><<<g, kBlock, 0, stream>>>(a, off, idx, rho_in);
-        k3_eos<S, 1><<<g, kBlock, 0, stream>>>(a, off, idx, rho_in, 1);
+        k3_gradient<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, rho_in);
+        k3_eos<S, 1, NbList><<<g, kBlock, 0, stream>>>(a, nb, rho_in, 1);
       } else {
-        k3_eos<S, 0><<<g, kBlock, 0, stream>>>(a, off, idx, rho_in, 1);
+        k3_eos<S, 0, NbList><<<g, kBlock, 0, stream>>>(a, nb, rho_in, 1);
       }
       break;
     default:
@@ -509,24 +497,22 @@ __global__ void __launch_bounds__(kBlock) k3_kick_drift(double* __restrict__ pos
   if ((threadIdx.x & 63) == 0) atomicMax(disp, ord_enc(v));
 }
 
-// MODE 0 counts, MODE 1 writes each particle's periodic neighbours within rc
-// (visit order: 27 cell offsets, members in cell order).
-template <int MODE>
-__global__ void __launch_bounds__(kBlock) k3_lists_periodic(
-    const double* __restrict__ pos, int64_t n, const int64_t* __restrict__ cells, GridDesc g,
-    const int64_t* __restrict__ ukeys, const int64_t* __restrict__ ustart, const int64_t* __restrict__ order,
-    int64_t ncells, double c2, double b0, double b1, double b2, unsigned long long* __restrict__ cnt,
-    const int64_t* __restrict__ off, int32_t* __restrict__ idx) {
+// Entries the periodic_pairs lists at the build radius would hold (each
+// unordered pair twice), with the list build's own distance test
+// (minimum image by division, unfused); the sweeps walk the cells instead.
+__global__ void __launch_bounds__(kBlock) k3_count_entries(const double* __restrict__ pos, int64_t n, NbCells nb,
+                                                           double c2, double b0, double b1, double b2,
+                                                           unsigned long long* __restrict__ total) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double box[3] = {b0, b1, b2};
-  int64_t k = 0;
-  const int64_t base = MODE == 1 ? off[i] : 0;
-  for_neighbours<double, 3, true>(pos, i, cells, g, ukeys, ustart, order, ncells, c2, box, [&](int64_t j) {
-    if (MODE == 1) idx[base + k] = (int32_t)j;
-    ++k;
-  });
-  if (MODE == 0) cnt[i] = k;
+  unsigned long long k = 0;
+  if (i < n) {
+    const double box[3] = {b0, b1, b2};
+    nb.visit<3>(i, true, [&](int64_t j) {
+      if (dist2<double, 3, true>(pos, i, j, box) <= c2) ++k;
+    });
+  }
+  for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
+  if ((threadIdx.x & 63) == 0 && k) atomicAdd(total, k);
 }
 
 // Re-orders the state by the cell order of the last search; afterwards the
@@ -663,6 +649,8 @@ struct tsph_psim {
   DevBuf pos, vel[2], mass, acc, orig, anchor, disp;
   DevBuf pos2, vel2, mass2, acc2, orig2, cells2, part, diag;
   DevBuf rho_raw, rho, rho_u, p, moment, gr, rho_lo;
+  DevBuf cell_R, entries;                    // dense cell table; list-equivalent entry count
+  int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
   int cur = 0;
   int64_t steps = 0, rebuilds = 0;
@@ -679,14 +667,24 @@ template <typename S> ThreedScratch<S> psim_scratch(tsph_psim* ps) {
   return b;
 }
 
-// periodic_pairs at cutoff 2h + skin (threed/solver.py:254-256, 296-298),
-// kept as per-particle CSR lists over the cell-ordered state
+static NbCells psim_cells(const tsph_psim* ps) {
+  NbCells c{};
+  c.R = ps->cell_R.as<int4>();
+  c.cells = ps->cells2.as<int64_t>();
+  for (int d = 0; d < 3; ++d) { c.ext[d] = ps->cell_ext[d]; c.stride[d] = ps->cell_stride[d]; }
+  c.periodic = 1;
+  return c;
+}
+
+// periodic_pairs at cutoff 2h + skin (threed/solver.py:254-256, 296-298):
+// the state is re-ordered by (Morton) cell and a dense table of per-cell row
+// ranges lets the sweeps walk the 27 cells around every particle
 template <typename S>
 int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   tsph_ctx* ctx = ps->ctx;
   const int64_t n = ps->n;
   const double rc = 2.0 * ps->prm.h + ps->skin;
-  TSPH_TRY((run_search<double, 3>(ctx, ps->pos.as<double>(), n, rc, ps->prm.box, stream)));
+  TSPH_TRY((run_search<double, 3>(ctx, ps->pos.as<double>(), n, rc, ps->prm.box, stream, true)));
   CellSearch& cs = ctx->search;
   const unsigned g = grid_for(n);
   k3_permute<S><<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), n, ps->pos.as<double>(),
@@ -701,23 +699,21 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   std::swap(ps->mass, ps->mass2);
   std::swap(ps->acc, ps->acc2);
   std::swap(ps->orig, ps->orig2);
-  GridDesc gd;
-  for (int d = 0; d < 3; ++d) { gd.ext[d] = cs.ext[d]; gd.stride[d] = cs.stride[d]; }
-  ctx->nb_targets = n;
-  ctx->wnb_targets = 0;
-  TSPH_TRY(ctx->nb_off.ensure(sizeof(int64_t) * (n + 1)));
-  TSPH_TRY(ctx->nb_cursor.ensure(sizeof(int64_t) * (n + 1)));
-  unsigned long long* cnt = ctx->nb_cursor.as<unsigned long long>();
+  double dense = 1.0;
+  for (int d = 0; d < 3; ++d) dense *= (double)cs.ext[d];
+  TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG, "periodic grid of %.3g cells exceeds the dense "
+               "range table (%lld cells)", dense, (long long)kMaxDenseCells);
+  ps->ncells_dense = (int64_t)dense;
+  for (int d = 0; d < 3; ++d) { ps->cell_ext[d] = cs.ext[d]; ps->cell_stride[d] = cs.stride[d]; }
+  TSPH_TRY(ps->cell_R.ensure(sizeof(int4) * (size_t)ps->ncells_dense));
+  TSPH_TRY(ps->entries.ensure(sizeof(unsigned long long)));
+  k_cell_table_clear<<<grid_for(ps->ncells_dense), kBlock, 0, stream>>>(ps->cell_R.as<int4>(), ps->ncells_dense);
+  k_cell_table<3><<<g, kBlock, 0, stream>>>(ps->cells2.as<int64_t>(), n, n, cs.stride[0], cs.stride[1],
+                                            cs.stride[2], ps->cell_R.as<int4>());
+  TSPH_CHECK_HIP(hipMemsetAsync(ps->entries.ptr, 0, sizeof(unsigned long long), stream));
   const double* b = ps->prm.box;
-  k3_lists_periodic<0><<<g, kBlock, 0, stream>>>(ps->pos.as<double>(), n, ps->cells2.as<int64_t>(), gd,
-      cs.ukeys.as<int64_t>(), cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells, rc * rc,
-      b[0], b[1], b[2], cnt, nullptr, nullptr);
-  TSPH_CHECK_LAUNCH();
-  TSPH_TRY(exclusive_scan(ctx, cnt, n, ctx->nb_off.as<int64_t>(), stream, &ctx->nb_entries));
-  TSPH_TRY(ctx->nb_idx.ensure(sizeof(int32_t) * (ctx->nb_entries + 1)));
-  k3_lists_periodic<1><<<g, kBlock, 0, stream>>>(ps->pos.as<double>(), n, ps->cells2.as<int64_t>(), gd,
-      cs.ukeys.as<int64_t>(), cs.ustart.as<int64_t>(), cs.order.as<int64_t>(), cs.ncells, rc * rc,
-      b[0], b[1], b[2], nullptr, ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>());
+  k3_count_entries<<<g, kBlock, 0, stream>>>(ps->pos.as<double>(), n, psim_cells(ps),
+                                             rc * rc, b[0], b[1], b[2], ps->entries.as<unsigned long long>());
   TSPH_CHECK_LAUNCH();
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
   ps->rebuilds++;
@@ -731,14 +727,13 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   a.pos = ps->pos.as<double>();
   a.vel = ps->vel[ps->cur].as<double>();
   a.mass = ps->mass.as<S>();
-  const int64_t* off = ps->ctx->nb_off.as<int64_t>();
-  const int32_t* idx = ps->ctx->nb_idx.as<int32_t>();
+  const NbCells nb = psim_cells(ps);
   if (kick) {
-    TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), off, idx, ps->orig.as<int64_t>(),
+    TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
                                      ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream)));
     ps->cur ^= 1;
   } else {
-    TSPH_TRY((threed_sweeps<S, false>(a, ps->q, psim_scratch<S>(ps), off, idx, ps->orig.as<int64_t>(),
+    TSPH_TRY((threed_sweeps<S, false>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
                                       ps->acc.as<S>(), 0.0, nullptr, stream)));
   }
   return TSPH_OK;
@@ -845,7 +840,8 @@ int tsph_psim_destroy(tsph_psim* ps) {
   if (!ps) return TSPH_OK;
   DevBuf* b[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->mass, &ps->acc, &ps->orig, &ps->anchor, &ps->disp,
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
-                 &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo};
+                 &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo,
+                 &ps->cell_R, &ps->entries};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);
@@ -954,7 +950,9 @@ int tsph_psim_stats(tsph_psim* ps, int64_t* counters) {
   TSPH_REQUIRE(ps && counters, TSPH_ERR_ARG, "bad arguments");
   counters[0] = ps->steps;
   counters[1] = ps->rebuilds;
-  counters[2] = ps->ctx->nb_entries;
+  unsigned long long e = 0;
+  if (ps->built) TSPH_CHECK_HIP(hipMemcpy(&e, ps->entries.ptr, sizeof(e), hipMemcpyDeviceToHost));
+  counters[2] = (int64_t)e;
   return TSPH_OK;
 }
 

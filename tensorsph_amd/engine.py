@@ -133,7 +133,7 @@ class WcsphSimulation:
         c = (_i64 * 5)()
         pk = (C.c_double * 2)()
         _lib.check(_lib.lib().tsph_sim_stats(self.handle, c, pk))
-        return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "list_entries": c[3],
+        return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "cells": c[3],
                 "meter_rows": c[4], "peak_rho": pk[0], "peak_speed": pk[1]}
 
     def meters(self) -> np.ndarray:

@@ -176,7 +176,7 @@ def build_case(kind: str, kw: dict) -> DpCase:
 # device bytes one fluid row / one wall station occupy in the engine (state in
 # both ping-pong sides, sweep intermediates, cell and list bookkeeping with the
 # default 64 / 40 list entries per row)
-FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 8 + 64 * 4 + 16 + 8 + 32 + 16 + 16 + 8 + 16 + 1 + 8
+FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 8 + 64 * 4 + 16 + 8 + 32 + 16 + 16 + 8 + 16 + 1 + 8 + 8
 WALL_ROW_BYTES = 3 * 16 + 4 + 4 + 4 + 4 + 8 + 40 * 4 + 8 + 8 + 16 + 8
 
 
@@ -360,4 +360,45 @@ def run_sequential(requests, device=None) -> tuple[float, list]:
             times.append(time.perf_counter() - t0)
         finally:
             eng.close()
+    return sum(times), times
+
+
+def run_single_case_api(case: DpCase, device=None) -> dict:
+    """One case through the single-case API in the reference's own driver loop
+    (benchmarks/delta_plus/elliptical_drop.py:44-57, spheric19.py:139-170):
+    per step the adaptive dt from max |v|, a PairListCache(2h, 0.4 dx) lookup
+    over cat(pos, wall), then delta_plus_step.  The state is on the device
+    before the clock starts.  Returns the final state and the seconds taken."""
+    from .delta_plus import delta_plus_step
+    from .neighbor_search import PairListCache
+    dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
+    f = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    pos, vel, rho, mass = f(case.pos), f(case.vel), f(case.rho), f(case.mass)
+    walls = {}
+    if case.n_wall:
+        walls = dict(wall=f(case.wall), wall_velocity=f(case.wall_vel), wall_normal=f(case.wall_normal))
+    wall = walls.get("wall", pos.new_empty((0, 2)))
+    cache = PairListCache(4.0 * case.dx, 0.4 * case.dx, fluid_count=case.n)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(case.steps):
+        vmax = float(torch.sqrt((vel * vel).sum(1)).max())
+        dt = case.courant * 2.0 * case.dx / (case.sound_speed + vmax)
+        pairs, _ = cache.get(torch.cat((pos, wall)))
+        st = delta_plus_step(pos, vel, rho, mass, dx=case.dx, sound_speed=case.sound_speed,
+                             reference_speed=case.reference_speed, dt=dt, rho0=case.rho0,
+                             gravity=case.gravity, viscosity=case.viscosity, diffusion=case.diffusion,
+                             alpha=case.alpha, pairs=pairs, **walls)
+        pos, vel, rho = st.position, st.velocity, st.density
+    torch.cuda.synchronize(dev)
+    return {"position": pos, "velocity": vel, "density": rho, "seconds": time.perf_counter() - t0,
+            "rebuilds": cache.rebuilds}
+
+
+def run_sequential_api(requests, device=None) -> tuple[float, list]:
+    """``run_single_case_api`` over the requests one after another: returns
+    (seconds, per-case seconds)."""
+    times = []
+    for kind, kw in requests:
+        times.append(run_single_case_api(build_case(kind, kw), device)["seconds"])
     return sum(times), times

@@ -73,6 +73,19 @@ def test_batch_tracks_single_step_api(kind):
     assert d.max() < 1e-6
 
 
+def test_single_case_api_loop_matches_batch():
+    """The reference-style driver loop (adaptive dt, pair cache with a skin,
+    delta_plus_step) gives the batch engine's run bit for bit: skin pairs fail
+    the strict support test and the sums keep ascending neighbour order."""
+    from tensorsph_amd.serving import run_single_case_api
+    case = _small("impact", 2000, steps=30)
+    got = run_single_case_api(case)
+    ref = _alone(case, sorted_lists=True)
+    assert ref["steps"] == case.steps and got["rebuilds"] >= 1
+    for key in ("position", "velocity", "density"):
+        assert torch.equal(got[key], ref[key]), key
+
+
 def test_alone_and_in_a_mix_are_bitwise_equal():
     from tensorsph_amd.serving import BatchEngine
     a = _small("impact", 2500, steps=25)
