@@ -1,0 +1,161 @@
+"""2-D dam break behind the reference's ``dam_break`` API (dam_break.py:45-486).
+
+``run_case`` integrates the water column on the resident stepper (Verlet
+pair cache, rhs, symplectic Euler, apply_boundary: the loop of
+dam_break.py:374-465) and returns the same ``DamBreakRun`` record: the
+sampled front position and energies, every-step extrema, the wall-impact
+time and the compression before it.  Position shifting and transport-velocity
+correction (``shift_every``, ``tvc_every``) and the adaptive step are not
+part of the GPU particle pipeline and raise NotImplementedError.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .cases import make_column, wall_lattice, lattice_mass_unit
+from .config import DamBreakConfig, check_supported
+
+__all__ = ["DamBreakConfig", "DamBreakRun", "validate_case", "front_position", "run_case"]
+
+
+@dataclass
+class DamBreakRun:
+    """The record of dam_break.DamBreakRun (dam_break.py:234-265)."""
+    times: list
+    front: list
+    kinetic_energy: list
+    potential_energy: list
+    mass: float
+    escaped: int
+    rebuilds: int
+    max_speed: float
+    max_density_compression: float
+    compression_before_wall_impact: float
+    wall_impact_time: float | None
+    initial_mass: float
+    final_mass: float
+    final_positions: torch.Tensor
+    steps_taken: int = 0
+    simulated_time: float = 0.0
+    dt_min: float = 0.0
+    dt_max: float = 0.0
+    max_acoustic_courant: float = 0.0
+    max_tvc_shift: float = 0.0
+    max_tvc_clipped: float = 0.0
+    snapshots: list = field(default_factory=list)
+
+
+def validate_case(cfg: DamBreakConfig) -> None:
+    """dam_break.validate_case (dam_break.py:123-140)."""
+    if cfg.boundary != "wall":
+        raise ValueError("the dam-break case integrates static wall particles only; "
+                         f"boundary={cfg.boundary!r} would be silently ignored")
+    if cfg.dx <= 0 or cfg.h_factor <= 0:
+        raise ValueError("dx and h_factor must be positive")
+    if cfg.column_width <= 0 or cfg.column_height <= 0:
+        raise ValueError("column dimensions must be positive")
+    if cfg.column_width > cfg.pool_width or cfg.column_height > cfg.tank_height:
+        raise ValueError("the column must fit inside the tank")
+    if cfg.sound_speed <= 0 or cfg.rho0 <= 0 or cfg.gamma <= 0:
+        raise ValueError("rho0, sound_speed, and gamma must be positive")
+    courant = cfg.dt * cfg.sound_speed / (cfg.h_factor * cfg.dx)
+    if courant > cfg.max_courant * (1.0 + 1.0e-9):
+        raise ValueError(f"dt={cfg.dt:g} gives acoustic Courant {courant:.3f} > max_courant "
+                         f"{cfg.max_courant:g}; use dt <= {cfg.dt_from_courant():.3e} s")
+
+
+def front_position(pos, cfg: DamBreakConfig) -> float:
+    """Leading-edge x of the water (dam_break.py:172-176)."""
+    if pos.numel() == 0:
+        return 0.0
+    return float(pos[:, 0].max())
+
+
+def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapshot_times=None,
+             precision: str | None = None, device=None) -> DamBreakRun:
+    """dam_break.run_case (dam_break.py:316-486) on the resident stepper.
+
+    Every step is metered on the device (its pre-integration state, as the
+    reference samples it), so the all-step extrema and the first step whose
+    front reaches pool_width - 3 dx come from the same rows; samples every
+    ``output_every`` steps and at the end.  ``snapshot_times`` captures the
+    fluid positions and velocities at those times.  ``precision`` "fp64" /
+    "fp32" (FP32-compensated) defaults to torch's default dtype."""
+    from .engine import METER_KEYS, WcsphSimulation
+
+    validate_case(cfg)
+    check_supported(cfg)
+    if cfg.adaptive_dt:
+        raise NotImplementedError("adaptive_dt decides each step on the host; the resident stepper "
+                                  "runs a fixed dt")
+    if precision is None:
+        precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
+    dev = torch.device(device) if device is not None else torch.device("cuda")
+    top = cfg.tank_top()
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=top)
+    h = cfg.h_factor * cfg.dx
+    skin = cfg.pair_skin_factor if pair_skin_factor is None else pair_skin_factor
+    initial_mass = float(mass.sum())
+    sim = WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+                          sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+                          viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=lattice_mass_unit(cfg),
+                          skin=skin * h, lo=[0.0, 0.0], hi=[cfg.pool_width, top], tol=0.25 * cfg.dx,
+                          wall_friction=cfg.wall_friction, density_diffusion=cfg.density_diffusion,
+                          diffusion_form=cfg.density_diffusion_form,
+                          eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
+                          precision=precision, device=dev)
+    targets = sorted(float(t) for t in (snapshot_times or []))
+    sample = [k for k in range(cfg.steps + 1) if k % cfg.output_every == 0 or k == cfg.steps]
+    # a snapshot is taken at the first sampled step whose time reaches it
+    # (dam_break.py:410-414); times past the end are never reached
+    due = {}
+    for t in targets:
+        k = next((k for k in sample if k * cfg.dt >= t - 1.0e-12), None)
+        if k is not None:
+            due.setdefault(k, []).append(t)
+    snapshots = []
+    try:
+        done = 0
+        for s in sorted(due) + [cfg.steps]:
+            if s > done:
+                sim.step(s - done, meter_every=1)
+                done = s
+            if s in due:
+                st = sim.read()
+                for _ in due.pop(s):
+                    snapshots.append((s * cfg.dt, st["position"].clone(), st["velocity"].clone()))
+        escaped = sim.stats()["escaped"]
+        final = sim.read()["position"]
+        sim.step(1, meter_every=1)                 # the final sample: rhs of the last state
+        rows = sim.meters()
+        stats = sim.stats()
+    finally:
+        sim.close()
+    col = {k: i for i, k in enumerate(METER_KEYS)}
+    rows = rows[np.argsort(rows[:, 0], kind="stable")][: cfg.steps + 1]
+    impact_limit = cfg.pool_width - 3.0 * cfg.dx
+    compression = np.maximum.accumulate(rows[:, col["rho_max"]] / cfg.rho0 - 1.0)
+    hit = np.nonzero(rows[:, col["x_max"]] >= impact_limit)[0]
+    wall_impact_time = float(rows[hit[0], 0] * cfg.dt) if hit.size else None
+    compression_at_impact = float(compression[hit[0]]) if hit.size else None
+    speed_peak = float(rows[:, col["max_speed"]].max())
+    sampled_speed = max(float(rows[k, col["max_speed"]]) for k in sample)
+    peak_compression = float(compression[-1])
+    return DamBreakRun(
+        times=[k * cfg.dt for k in sample],
+        front=[float(rows[k, col["x_max"]]) for k in sample],
+        kinetic_energy=[float(rows[k, col["kinetic_energy"]]) for k in sample],
+        potential_energy=[float(rows[k, col["potential_energy"]]) for k in sample],
+        mass=initial_mass, escaped=int(escaped), rebuilds=int(stats["rebuilds"]),
+        max_speed=max(sampled_speed, speed_peak),
+        max_density_compression=peak_compression,
+        compression_before_wall_impact=(peak_compression if compression_at_impact is None
+                                        else compression_at_impact),
+        wall_impact_time=wall_impact_time, initial_mass=initial_mass,
+        final_mass=float(mass.sum()), final_positions=final, steps_taken=cfg.steps,
+        simulated_time=cfg.steps * cfg.dt, dt_min=cfg.dt, dt_max=cfg.dt,
+        max_acoustic_courant=cfg.dt * (cfg.sound_speed + speed_peak) / h, snapshots=snapshots)

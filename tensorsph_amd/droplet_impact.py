@@ -32,7 +32,8 @@ __all__ = [
     "wendland_c2_gradient", "wall_mass", "drop_wall_wall_pairs", "apply_boundary",
     "boundary_box", "validate_config", "acoustic_courant", "suggested_dt", "stable_dt",
     "next_dt", "lattice_kernel_sum", "lattice_mass_unit", "cell_centres", "domain_top",
-    "hydrostatic_density", "PairListCache", "cell_linked_pairs",
+    "hydrostatic_density", "PairListCache", "cell_linked_pairs", "run", "evaluate_acceptance",
+    "ROW_KEYS",
 ]
 
 
@@ -246,3 +247,207 @@ def apply_boundary(pos, vel, cfg: Config, top: float | None = None) -> int:
     from .engine import boundary_clamp
     x_min, x_max, y_min, y_max = boundary_box(cfg, top)
     return boundary_clamp(pos, vel, [x_min, y_min], [x_max, y_max], 0.25 * cfg.dx)
+
+
+# --------------------------------------------------------------------- driver
+
+ROW_KEYS = ["step", "time", "dt", "particle_count", "mass_total", "momentum_x", "momentum_y",
+            "rho_min", "rho_max", "rho_mean", "rho_max_relative_error", "pressure_min",
+            "pressure_max", "max_speed", "max_acceleration", "mach_number", "courant_number",
+            "kinetic_energy", "y_min", "y_max"]
+
+
+def _meter_row(cfg: Config, m, step: int, n: int) -> dict:
+    """One diagnostic row of run() (droplet_impact.py:1711-1733) from a meter
+    row of the resident stepper (engine.METER_KEYS: the state of ``step``
+    before its integration, as the reference samples it)."""
+    from .engine import METER_KEYS
+    g = {k: float(m[i]) for i, k in enumerate(METER_KEYS)}
+    rho_min, rho_max = g["rho_min"], g["rho_max"]
+    speed = g["max_speed"]
+    return {
+        "step": step, "time": step * cfg.dt, "dt": cfg.dt, "particle_count": n,
+        "mass_total": g["mass"], "momentum_x": g["momentum_x"], "momentum_y": g["momentum_y"],
+        "rho_min": rho_min, "rho_max": rho_max, "rho_mean": g["rho_sum"] / n,
+        "rho_max_relative_error": max(abs(rho_min / cfg.rho0 - 1.0), abs(rho_max / cfg.rho0 - 1.0)),
+        "pressure_min": g["pressure_min"], "pressure_max": g["pressure_max"],
+        "max_speed": speed, "max_acceleration": g["max_acceleration"],
+        "mach_number": speed / cfg.sound_speed, "courant_number": acoustic_courant(cfg.dt, cfg, speed),
+        "kinetic_energy": g["kinetic_energy"], "y_min": g["y_min"], "y_max": g["y_max"],
+    }
+
+
+def evaluate_acceptance(rows, cfg: Config, boundary_events: int, extremes: dict | None = None) -> list:
+    """droplet_impact.evaluate_acceptance (droplet_impact.py:1456-1512): the
+    recorded numerical criteria of a run, from every step's extrema when
+    ``extremes`` is given, else from the sampled rows."""
+    if extremes is None:
+        population = "sampled_rows"
+        worst_compression = max(r["rho_max"] / cfg.rho0 - 1.0 for r in rows)
+        worst_speed = max(r["max_speed"] for r in rows)
+        worst_courant = max(r["courant_number"] for r in rows)
+    else:
+        population = "all_steps"
+        worst_compression = float(extremes["peak_rho_compression"])
+        worst_speed = float(extremes["peak_speed"])
+        worst_courant = float(extremes["peak_courant"])
+    finite = all(math.isfinite(v) for r in rows for v in r.values())
+
+    def criterion(ident, metric, measured, threshold, unit, comparison, scope=population):
+        passed = measured >= threshold if comparison == ">=" else measured <= threshold
+        return {"id": ident, "metric": metric, "measured": measured, "threshold": threshold,
+                "unit": unit, "comparison": comparison, "population": scope,
+                "result": "pass" if passed else "fail"}
+
+    out = [criterion("finite_state", "all diagnostics finite", 1.0 if finite else 0.0, 1.0, "bool",
+                     ">=", "sampled_rows"),
+           criterion("density_compression", "max(rho/rho0 - 1)", worst_compression,
+                     cfg.max_density_error, "1", "<="),
+           criterion("mach_number", "max|v| / c0", worst_speed / cfg.sound_speed, cfg.max_mach, "1", "<="),
+           criterion("acoustic_courant", "max dt(c0+|v|)/h", worst_courant, cfg.max_courant, "1", "<=")]
+    escape = criterion("boundary_escape", "particles clamped beyond the box", float(boundary_events), 0.0,
+                       "count", "<=", "all_steps")
+    if cfg.boundary == "clamp":
+        escape["result"] = "not_evaluated"
+    out.append(escape)
+    return out
+
+
+def _json_number(v: float):
+    return None if not math.isfinite(v) else v
+
+
+def run(cfg: Config, output=None, plot: bool = True, precision: str | None = None) -> list:
+    """droplet_impact.run (droplet_impact.py:1607-1795) on the resident stepper.
+
+    The loop (pairs with the Verlet skin, rhs, every-step extrema, sampled
+    rows, symplectic Euler, apply_boundary) runs on the GPU without a host
+    round trip per step; rows carry the reference's keys and sampling
+    (every ``output_every`` steps and the final one).  With ``output`` the
+    diagnostics CSV, the run manifest with the acceptance criteria and one
+    particles_XXXXXX.pt snapshot (position, density) per row are written;
+    ``plot`` is accepted and ignored (no figures are drawn).  ``precision``
+    "fp64" / "fp32" (FP32-compensated) defaults to torch's default dtype, the
+    dtype the reference runs in.  Fixed ``dt`` only: ``adaptive_dt`` raises
+    NotImplementedError.  ``fail_fast`` raises SimulationUnstableError after
+    the artifacts are written, as the reference does.
+    """
+    import json
+    from pathlib import Path
+
+    import numpy as np
+
+    from .cases import make_particles, wall_lattice
+    from .engine import WcsphSimulation
+
+    validate_config(cfg)
+    check_supported(cfg)
+    if cfg.adaptive_dt:
+        raise NotImplementedError("adaptive_dt decides each step on the host; the resident stepper "
+                                  "runs a fixed dt")
+    if precision is None:
+        precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
+    pos, mass, n_pool = make_particles(cfg)
+    x_min, x_max, y_min, y_max = boundary_box(cfg)
+    if (pos[:, 0].min() < x_min or pos[:, 0].max() > x_max or pos[:, 1].min() < y_min
+            or pos[:, 1].max() > y_max):
+        raise ValueError("initial particle arrangement leaves the computational box")
+    wall = wall_lattice(cfg, top=cfg.wall_top) if cfg.boundary == "wall" else np.zeros((0, 2))
+    vel = np.zeros_like(pos)
+    vel[n_pool:] = np.asarray(cfg.initial_velocity, dtype=np.float64)
+    h = cfg.h_factor * cfg.dx
+    n = pos.shape[0]
+    device = "cuda" if cfg.device in ("auto", "cpu") else cfg.device
+    sim = WcsphSimulation(pos, vel, mass, wall, h=h, rho0=cfg.rho0, sound_speed=cfg.sound_speed,
+                          gamma=cfg.gamma, gravity=cfg.gravity, viscosity=cfg.viscosity, dt=cfg.dt,
+                          wall_mass=lattice_mass_unit(cfg), skin=cfg.pair_skin_factor * h,
+                          lo=[x_min, y_min], hi=[x_max, y_max], tol=0.25 * cfg.dx,
+                          wall_friction=cfg.wall_friction, density_diffusion=cfg.density_diffusion,
+                          diffusion_form=cfg.density_diffusion_form,
+                          eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
+                          precision=precision, device=device)
+    out = Path(output) if output is not None else None
+    if out is not None:
+        out.mkdir(parents=True, exist_ok=True)
+    row_steps = list(range(0, cfg.steps, cfg.output_every)) + [cfg.steps]
+    snapshots = []
+    escaped = 0
+    try:
+        for k, step in enumerate(row_steps):
+            if out is not None:
+                snapshots.append((step, _snapshot(sim, cfg, wall, mass, out, step)))
+            if step == cfg.steps:
+                escaped = sim.stats()["escaped"]
+                sim.step(1, meter_every=1)         # the final row: rhs of the last state
+                break
+            nxt = row_steps[k + 1]
+            sim.step(nxt - step, meter_every=cfg.output_every)
+        meters = sim.meters()
+        stats = sim.stats()
+    finally:
+        sim.close()
+    by_step = {int(r[0]): r for r in meters}
+    rows = [_meter_row(cfg, by_step[s], s, n) for s in row_steps if s in by_step]
+    peak_speed = float(stats["peak_speed"])
+    extremes = {"peak_rho_compression": float(stats["peak_rho"]) / cfg.rho0 - 1.0,
+                "peak_speed": peak_speed,
+                "peak_courant": acoustic_courant(cfg.dt, cfg, peak_speed),
+                "simulated_time": cfg.steps * cfg.dt, "dt_min": cfg.dt, "dt_max": cfg.dt}
+    criteria = evaluate_acceptance(rows, cfg, escaped, extremes=extremes)
+    failure = None if all(math.isfinite(v) for r in rows for v in r.values()) else "non-finite state"
+    if out is not None:
+        import csv
+        import dataclasses
+        with (out / "diagnostics.csv").open("w", newline="") as f:
+            writer = csv.DictWriter(f, fieldnames=ROW_KEYS)
+            writer.writeheader()
+            writer.writerows(rows)
+        numerical = "fail" if any(c["result"] == "fail" for c in criteria) else "pass"
+        manifest = {
+            "case": "droplet_impact_2d_wcsph",
+            "solver": {"name": "tensorsph_amd", "device": str(torch.device(device)),
+                       "precision": precision},
+            "initial_particle_count": int(n), "wall_particle_count": int(wall.shape[0]),
+            "config": {k: (list(v) if isinstance(v, tuple) else v)
+                       for k, v in dataclasses.asdict(cfg).items()},
+            "derived": {"smoothing_length": h, "suggested_dt": suggested_dt(cfg),
+                        "viscous_dt_limit": _json_number(viscous_dt_limit(cfg)),
+                        "gravity_dt_limit": _json_number(gravity_dt_limit(cfg)),
+                        "diffusion_dt_limit": _json_number(diffusion_dt_limit(cfg)),
+                        "acoustic_courant": acoustic_courant(cfg.dt, cfg),
+                        "simulated_time": extremes["simulated_time"], "adaptive_dt": False,
+                        "dt_min": cfg.dt, "dt_max": cfg.dt},
+            "statuses": {"software_execution": "failed" if failure else "success",
+                         "numerical_verification": numerical, "physical_validation": "not_evaluated"},
+            "acceptance": {"criteria": criteria},
+            "performance": {"pair_skin": cfg.pair_skin_factor * h, "pair_rebuilds": int(stats["rebuilds"])},
+            "sampling": {"diagnostic_rows": len(rows), "output_every": cfg.output_every,
+                         "extrema_population": "all_steps"},
+            "extremes": extremes, "boundary_events": int(escaped),
+            "snapshots": [f"particles_{s:06d}.pt" for s, _ in snapshots],
+        }
+        (out / "run_manifest.json").write_text(json.dumps(manifest, indent=2))
+    if cfg.fail_fast:
+        failed = [c for c in criteria if c["result"] == "fail"]
+        if failure or failed:
+            detail = failure or ", ".join(f"{c['id']}={c['measured']:.4g} (limit {c['threshold']:g})"
+                                          for c in failed)
+            raise SimulationUnstableError(f"numerical acceptance failed: {detail}"
+                                          + (f"; artifacts in {out}" if out is not None else ""))
+    return rows
+
+
+def _snapshot(sim, cfg: Config, wall, mass, out, step: int):
+    """particles_XXXXXX.pt of the current state: positions and the rhs density
+    of that state (droplet_impact.py:1734-1739)."""
+    st = sim.read()
+    pos = st["position"]
+    dt = pos.dtype if sim.code == _lib.F64 else torch.float32
+    p = pos.to(dt)
+    w = torch.as_tensor(wall, dtype=dt, device=pos.device) if len(wall) else None
+    rho, _, _ = rhs(p, st["velocity"].to(dt), torch.as_tensor(mass, dtype=dt, device=pos.device), cfg,
+                    wall=w, dt=cfg.dt)
+    name = out / f"particles_{step:06d}.pt"
+    torch.save({"step": int(step), "time": float(step * cfg.dt), "position": pos.detach().cpu(),
+                "density": rho.detach().cpu(), "boundary": str(cfg.boundary)}, name)
+    return name
