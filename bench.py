@@ -32,19 +32,20 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(HERE, "profiles", "k_tile_force_dambreak_1p6M_fp32c.md")
+PROFILE_3D = os.path.join(HERE, "profiles", "k_step_force_kleefsman_5M_fp32c.md")
 
 
-def profiled_traffic(nf: int):
-    """HBM bytes per force-sweep launch from the committed rocprofv3 FETCH_SIZE /
+def profiled_traffic(nf: int, path: str = PROFILE, kernel: str = "k_tile_force"):
+    """HBM bytes per force-sweep launch from a committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, when it was taken on this workload; else None."""
     try:
-        with open(PROFILE) as f:
+        with open(path) as f:
             text = f.read()
     except OSError:
         return None
     if f"{nf:,} fluid" not in text:
         return None
-    m = re.search(r"traffic_bytes_per_launch k_tile_force = (\d+)", text)
+    m = re.search(rf"traffic_bytes_per_launch {kernel} = (\d+)", text)
     return int(m.group(1)) if m else None
 
 
@@ -317,7 +318,9 @@ def kleefsman_bench(args) -> dict:
                  v32, args.steps, args.warmup, runs["fp32"]["wall_s"], "f32", cfg_out,
                  sweep_ms_per_step=dict(zip(("density", "wall", "force_integrate"), runs["fp32"]["sweep_ms"])),
                  roofline={"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                           "frac": achieved / HBM_PEAK_GBS,
+                           "traffic": profiled_traffic(nf, PROFILE_3D, "k_step_force"),
+                           "traffic_source": os.path.relpath(PROFILE_3D, HERE),
                            "kernel": "k_step_force (force + viscosity + integrate + box clamp)",
                            "algorithmic_bytes_per_launch": b_force},
                  list_rebuilds_in_run=runs["fp32"]["rebuilds"],

@@ -137,6 +137,34 @@ template <typename S> TSPH_D void cubic_wgl(S r, const KernelConst<S>& k, S& w, 
   lap = q < S(2) ? k.sigma * (fpp + twice) * (k.inv_h * k.inv_h) : S(0);
 }
 
+// Gradient factor g and Laplacian lap of the cubic spline from the squared
+// separation (cubic_wgl's values).  FP64 is cubic_wgl itself; FP32 takes r
+// and 1/r from one hardware reciprocal square root, so 2 f'(q) / q becomes
+// 2 f'(q) h / r without a division.
+template <typename S> TSPH_D void cubic_gl_r2(S r2, const KernelConst<S>& k, S& g, S& lap);
+template <> TSPH_D void cubic_gl_r2<double>(double r2, const KernelConst<double>& k, double& g, double& lap) {
+  double w;
+  cubic_wgl<double>(sqrt(r2), k, w, g, lap);
+}
+template <> TSPH_D void cubic_gl_r2<float>(float r2, const KernelConst<float>& k, float& g, float& lap) {
+  const float rinv = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 1e12f;
+  const float q = (r2 * rinv) * k.inv_h;
+  float fp, fpp;
+  if (q < 1.0f) {
+    fp = -3.0f * q + 2.25f * (q * q);
+    fpp = -3.0f + 4.5f * q;
+  } else if (q < 2.0f) {
+    const float t = 2.0f - q;
+    fp = -0.75f * (t * t);
+    fpp = 1.5f * t;
+  } else {
+    fp = 0.0f; fpp = 0.0f;
+  }
+  g = k.sigma_over_h * fp * rinv;
+  const float twice = q > 1e-10f ? 2.0f * fp * (k.h * rinv) : -6.0f;
+  lap = q < 2.0f ? k.sigma * (fpp + twice) * (k.inv_h * k.inv_h) : 0.0f;
+}
+
 // Wendland C2, 2-D sigma = 7/(4 pi h^2) (droplet_impact.py:320-346).
 template <typename S> TSPH_D void wendland_wg(S r, const KernelConst<S>& k, S& w, S& g) {
   const S q = r * k.inv_h;

@@ -129,14 +129,28 @@ __global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, NbCells n
   }
 }
 
-__global__ void k_reduce_loads(const double* __restrict__ part, int64_t nblocks, double* __restrict__ out,
-                               const SimFlags* fl, int slot_in) {
+// Sums the per-block partials of k_step_loads in a fixed order (thread t takes
+// blocks t, t + 256, ..., then a fixed tree over the threads): one block of 256.
+__global__ void __launch_bounds__(256) k_reduce_loads(const double* __restrict__ part, int64_t nblocks,
+                                                      double* __restrict__ out, const SimFlags* fl, int slot_in) {
   if (fl->stale[slot_in]) return;
-  if (threadIdx.x < 6) {
-    double v = 0.0;
-    for (int64_t b = 0; b < nblocks; ++b) v += part[b * 6 + threadIdx.x];
-    out[threadIdx.x] = v;
+  __shared__ double red[6][256];
+  double v[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v[k] += part[b * 6 + k];
   }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 
@@ -716,8 +730,8 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     k_step_loads<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr,
                                                              sim->orig.as<int64_t>(), lo, hi, c[0], c[1], c[2], fl,
                                                              slot, sim->load_part.as<double>());
-    k_reduce_loads<<<1, 64, 0, stream>>>(sim->load_part.as<double>(), grid_for(nw), sim->loads_out.as<double>(), fl,
-                                         slot);
+    k_reduce_loads<<<1, 256, 0, stream>>>(sim->load_part.as<double>(), grid_for(nw), sim->loads_out.as<double>(),
+                                          fl, slot);
     TSPH_CHECK_LAUNCH();
   }
   if (meters)

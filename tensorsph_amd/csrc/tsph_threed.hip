@@ -225,10 +225,10 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
     if (r2 > a.skip2) return;
-    S w, g, lap;
-    cubic_wgl<S>(sqrt_s<S>(r2), a.k, w, g, lap);
+    S g, lap;
+    cubic_gl_r2<S>(r2, a.k, g, lap);
     const S rij = S(0.5) * (ri + a.rho_u[j]);
-    acc.add(r2 * lap * (a.mass[j] / rij));
+    acc.add(r2 * lap * quot<S>(a.mass[j], rij));
   });
   a.moment[i] = acc.get() / S(3);
 }
@@ -258,11 +258,11 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
     if (r2 > a.skip2) return;
-    S w, g, lap;
-    cubic_wgl<S>(sqrt_s<S>(r2), a.k, w, g, lap);
+    S g, lap;
+    cubic_gl_r2<S>(r2, a.k, g, lap);
     const S rj = a.rho_u[j];
     if (a.do_pressure) {
-      const S sym = pri + a.p[j] / (rj * rj);
+      const S sym = pri + quot<S>(a.p[j], rj * rj);
       const S cp = -(a.mass[j] * sym);
 #pragma unroll
       for (int d = 0; d < 3; ++d) ap[d].add((df[d] * g) * cp);
@@ -271,11 +271,11 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
       const S rij = S(0.5) * (ri + rj);
       const int64_t oj = orig ? orig[j] : j;
       const S mw = oj > oi ? a.mass[j] : a.mass[i];
-      S wgt = mw * lap / rij;
+      S wgt = quot<S>(mw * lap, rij);
       if (a.renormalise) {
         S mk = S(0.5) * (Ki + a.moment[j]);
         mk = mk > S(1e-12) ? mk : S(1e-12);
-        wgt = wgt * (S(2) / mk);
+        wgt = wgt * quot<S>(S(2), mk);
       }
 #pragma unroll
       for (int d = 0; d < 3; ++d) av[d].add((S)(a.vel[3 * j + d] - vi[d]) * wgt);
