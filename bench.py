@@ -71,6 +71,8 @@ def parse():
                     help="dambreak: the 2-D dam break line (default); shear: 3-D periodic shear "
                          "wave; kleefsman: 3-D dam break against a box obstacle")
     ap.add_argument("--shear-n", type=int, default=128, help="lattice edge of the shear-wave cube")
+    ap.add_argument("--decay-n", type=int, default=64,
+                    help="lattice edge of the full-length decay run of the shear workload (0: skip)")
     ap.add_argument("--fluid3d", type=float, default=5e6, help="fluid particles of the box case")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
@@ -262,6 +264,7 @@ def shear_bench(args) -> dict:
                        f"rho0=1, nu=0.01, c0=10, dt={dt:.4g} s, jitter 0.01 dx",
            "precision": "pair arithmetic FP32 with Kahan sums (density FP64); positions, velocities "
                         "and leapfrog FP64", "particles": npart, "parallelism": "single GPU"}
+    decay = shear_decay(args.decay_n) if args.decay_n > 0 else None
     return _line("particle-updates/s (3-D periodic shear wave, FP32-compensated) + HBM GB/s", v32,
                  args.steps, args.warmup, runs["fp32"]["wall_s"], "f32", cfg,
                  algorithmic_GBps_step=v32 * b32 / 1e9, algorithmic_bytes_per_update=b32,
@@ -269,7 +272,28 @@ def shear_bench(args) -> dict:
                  list_pairs=runs["fp32"]["pairs"], list_rebuilds_in_run=runs["fp32"]["rebuilds"],
                  fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
                        "algorithmic_GBps_step": v64 * b64 / 1e9},
-                 fp32c_over_fp64=v32 / v64)
+                 fp32c_over_fp64=v32 / v64, decay=decay)
+
+
+def shear_decay(n: int) -> dict:
+    """The decay check of the shear workload: u_x = 1e-3 sin(2 pi z) on an n^3
+    lattice (jitter 0.01 dx, PCG64(20261203)), nu = 0.01, rho0 = 1, c0 = 10,
+    integrated to t = 1 / (4 pi^2 nu) with amplitude samples every 50 steps,
+    against the analytic rate nu k^2 = 4 pi^2 nu of exp(-nu k^2 t)."""
+    from tensorsph_amd.threed.solver import run_shear_wave
+    nu = 0.01
+    t_end = 1.0 / (4.0 * math.pi ** 2 * nu)
+    out = {"lattice": f"{n}^3", "t_end_s": t_end}
+    for name, dtype in (("fp32c", "float32"), ("fp64", "float64")):
+        import torch
+        t0 = time.perf_counter()
+        res = run_shear_wave(n=n, nu=nu, amplitude=1e-3, sound_speed=10.0, rho0=1.0, t_end=t_end,
+                             sample_every=50, mode="single", jitter=0.01, device="cuda",
+                             dtype=getattr(torch, dtype))
+        out[name] = {"fit_rate": res.fit_rate, "analytic_rate": res.analytic_rate,
+                     "rel_error": res.rate_error, "steps": res.steps, "seconds": time.perf_counter() - t0,
+                     "amplitude_final_over_initial": res.amplitudes[-1] / res.amplitudes[0]}
+    return out
 
 
 def kleefsman_bench(args) -> dict:
