@@ -74,6 +74,9 @@ def parse():
     ap.add_argument("--decay-n", type=int, default=64,
                     help="lattice edge of the full-length decay run of the shear workload (0: skip)")
     ap.add_argument("--fluid3d", type=float, default=5e6, help="fluid particles of the box case")
+    ap.add_argument("--t-end", type=float, default=0.0,
+                    help="box case: also integrate both precisions to this physical time and "
+                         "compare the force / moment histories on the box")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     return ap.parse_args()
@@ -327,6 +330,7 @@ def kleefsman_bench(args) -> dict:
                       "sweep_ms": [x / max(nsteps, 1) for x in sweep_ms], "rebuilds": stats["rebuilds"],
                       "force": loads[-1, 1:4].tolist(), "moment": loads[-1, 4:7].tolist()}
     v32, v64 = runs["fp32"]["value"], runs["fp64"]["value"]
+    history = kleefsman_history(cfg, args.t_end) if args.t_end > 0 else None
     s = 4
     b_force = nf * (48 * 2 + 2 * s) + nw * (24 + 3 * s)
     force_ms = runs["fp32"]["sweep_ms"][2]
@@ -355,7 +359,48 @@ def kleefsman_bench(args) -> dict:
                  fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
                        "sweep_ms_per_step": dict(zip(("density", "wall", "force_integrate"),
                                                      runs["fp64"]["sweep_ms"]))},
-                 fp32c_over_fp64=v32 / v64)
+                 fp32c_over_fp64=v32 / v64, history=history)
+
+
+def kleefsman_history(cfg, t_end: float) -> dict:
+    """Both precisions integrated to t_end with the box force and moment of
+    every step (kleefsman_obstacle_3d.py:644-706): peak force, impulse, first
+    contact and the FP32c - FP64 difference of the histories."""
+    import numpy as np
+    import torch
+    from tensorsph_amd.kleefsman import make_simulation
+    steps = int(math.ceil(t_end / cfg.dt))
+    loads = {}
+    nf = 0
+    for prec in ("fp32", "fp64"):
+        sim, nf, _ = make_simulation(cfg, prec)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:                     # chunks, with a progress line after each
+            k = min(1000, steps - done)
+            sim.step(k)
+            done += k
+            print(f"box history {prec}: step {done}/{steps}, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        loads[prec] = (sim.loads(), secs, sim.stats())
+        sim.close()
+    out = {"t_end_s": steps * cfg.dt, "steps": steps}
+    f64 = loads["fp64"][0][:, 1:4]
+    scale = float(np.abs(f64).max()) or 1.0
+    for prec, (rows, secs, st) in loads.items():
+        f, m = rows[:, 1:4], rows[:, 4:7]
+        mag = np.sqrt((f * f).sum(1))
+        hit = np.nonzero(mag > 1e-3 * max(mag.max(), 1e-30))[0]
+        out[prec] = {"seconds": secs, "particle_updates_per_s": nf * steps / secs,
+                     "peak_force_N": f[int(np.argmax(mag))].tolist(), "peak_force_time_s": float(np.argmax(mag) * cfg.dt),
+                     "impulse_Ns": (f.sum(0) * cfg.dt).tolist(), "peak_moment_Nm": float(np.sqrt((m * m).sum(1)).max()),
+                     "first_contact_s": float(hit[0] * cfg.dt) if hit.size else None,
+                     "clamped_particles": int(rows[-1, 10]), "rebuilds": int(st["rebuilds"])}
+    out["force_history_max_diff_over_peak"] = float(np.abs(loads["fp32"][0][:, 1:4] - f64).max() / scale)
+    return out
 
 
 def main():

@@ -73,8 +73,13 @@ __global__ void k_cell_coords(const T* __restrict__ pos, int64_t n, T cutoff,
         b = smax[d][w] > b ? smax[d][w] : b;
       }
       if (a <= b) {
-        atomicMin(&minmax[d], (unsigned long long)(a + (long long)kEnc));
-        atomicMax(&minmax[3 + d], (unsigned long long)(b + (long long)kEnc));
+        // the bounds only move outwards, so a block that cannot improve on
+        // the current value (a plain, possibly stale read) skips its atomic;
+        // otherwise every block's atomics serialise on six addresses
+        const unsigned long long ea = (unsigned long long)(a + (long long)kEnc);
+        const unsigned long long eb = (unsigned long long)(b + (long long)kEnc);
+        if (ea < __atomic_load_n(&minmax[d], __ATOMIC_RELAXED)) atomicMin(&minmax[d], ea);
+        if (eb > __atomic_load_n(&minmax[3 + d], __ATOMIC_RELAXED)) atomicMax(&minmax[3 + d], eb);
       }
     }
   }
