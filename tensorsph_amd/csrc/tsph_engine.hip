@@ -931,8 +931,11 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemsetAsync(sim->clamp_acc.ptr, 0, 8 * 4, stream);
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
-  // obstacle loads run on the cell-walk sweeps
-  if (sim->dim == 2 && !sim->obstacle) { sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4; }
+  // obstacle loads and density diffusion run on the cell-walk sweeps; other
+  // 2-D runs default to the LDS cell tiles
+  if (sim->dim == 2 && !sim->obstacle && !(sim->prm.op.diffusion_scale > 0.0)) {
+    sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4;
+  }
   *out = sim;
   return TSPH_OK;
 }
@@ -1009,6 +1012,8 @@ int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
   TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the cell-walk sweeps");
+  TSPH_REQUIRE(tiles == 0 || !(sim->prm.op.diffusion_scale > 0.0), TSPH_ERR_ARG,
+               "density diffusion runs on the cell-walk sweeps");
   TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
                               tile_x * tile_y <= 64),
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
