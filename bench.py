@@ -56,6 +56,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--fluid", type=float, default=1.6e6, help="target fluid particle count")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--diffusion", type=float, default=0.0,
+                    help="dam break with Molteni δ-diffusion of this δ (cell-walk sweeps; "
+                         "the LDS tiles carry no diffusion term)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", default=None,
@@ -95,7 +98,7 @@ def dam_break_state(n_target: float):
     return cfg, pos, mass, wall, lattice_mass_unit(cfg)
 
 
-def make_sim(cfg, pos, mass, wall, wmass, precision):
+def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0):
     import numpy as np
     from tensorsph_amd.engine import WcsphSimulation
     h = cfg.h_factor * cfg.dx
@@ -104,13 +107,16 @@ def make_sim(cfg, pos, mass, wall, wmass, precision):
                            viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=wmass,
                            skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0],
                            hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx,
-                           precision=precision)
+                           precision=precision, density_diffusion=diffusion)
 
 
 def time_gpu(sim, steps, warmup, tile=None):
     import torch
     from tensorsph_amd.autotune import tune_tiles
-    if tile is not None:
+    if tile == "walk":
+        tuned = {"best": "cell walk", "ms": {}}
+        sim.step(max(warmup, 1))
+    elif tile is not None:
         sim.configure(tiles=True, tile=tile)
         tuned = {"best": tuple(tile), "ms": {}}
         sim.step(max(warmup, 1))
@@ -432,7 +438,9 @@ def main():
     h = cfg.h_factor * cfg.dx
 
     tile = None if args.tile is None else tuple(int(v) for v in args.tile.lower().split("x"))
-    sim = make_sim(cfg, pos, mass, wall, wmass, "fp32")
+    if args.diffusion > 0.0:
+        tile = "walk"
+    sim = make_sim(cfg, pos, mass, wall, wmass, "fp32", args.diffusion)
     wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup, tile)
     pairs = sim.count_pairs(2.0 * h)
     stats = sim.stats()
@@ -466,7 +474,8 @@ def main():
         "data": "synthetic: deterministic dam-break lattice (no dataset)",
         "config": {"workload": f"2-D dam break, {nf} fluid + {nw} wall particles, "
                                f"dx={cfg.dx:.5g} m, h=1.3dx, c0={cfg.sound_speed:.4g} m/s, "
-                               f"dt={cfg.dt:.4g} s, nu={cfg.viscosity}",
+                               f"dt={cfg.dt:.4g} s, nu={cfg.viscosity}"
+                               + (f", Molteni δ-diffusion δ={args.diffusion}" if args.diffusion else ""),
                    "precision": "pair arithmetic FP32 with Kahan sums; positions, velocities "
                                 "and integration FP64",
                    "fluid": nf, "wall": nw, "parallelism": "single GPU"},
@@ -479,13 +488,15 @@ def main():
                               "wall": sweep_ms[1] / max(nsteps, 1),
                               "force_integrate": force_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": profiled_traffic(nf),
-                     "traffic_source": os.path.relpath(PROFILE, HERE),
-                     "kernel": "k_tile_force (force + viscosity + integrate)",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if args.diffusion else profiled_traffic(nf),
+                     "traffic_source": None if args.diffusion else os.path.relpath(PROFILE, HERE),
+                     "kernel": ("k_step_force (cell walk: force + viscosity + integrate)" if args.diffusion
+                                else "k_tile_force (force + viscosity + integrate)"),
                      "algorithmic_bytes_per_launch": b_force},
     }
     if not args.no_fp64:
-        sim = make_sim(cfg, pos, mass, wall, wmass, "fp64")
+        sim = make_sim(cfg, pos, mass, wall, wmass, "fp64", args.diffusion)
         w64, sw64, n64, tuned64 = time_gpu(sim, args.steps, args.warmup, tile)
         sim.close()
         del sim
