@@ -348,7 +348,9 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     pos, mass, n_pool = make_particles(cfg)
-    x_min, x_max, y_min, y_max = boundary_box(cfg)
+    # the box follows the wall height (the harness option ``wall_top``; the
+    # reference's box and walls both stop at domain_top)
+    x_min, x_max, y_min, y_max = boundary_box(cfg, cfg.wall_top)
     if (pos[:, 0].min() < x_min or pos[:, 0].max() > x_max or pos[:, 1].min() < y_min
             or pos[:, 1].max() > y_max):
         raise ValueError("initial particle arrangement leaves the computational box")
