@@ -49,7 +49,8 @@ struct BSim {                       // device-side bookkeeping of one simulation
 };
 
 struct BatchDev {
-  int64_t cap_f, cap_w, cap_s, cap_cells, cap_e, cap_we;
+  int64_t cap_f, cap_w, cap_s, cap_cells;
+  int64_t kf, kw;                   // list slots per fluid row / per wall station
   const double *x, *v, *rho;        // current state (ping-pong side)
   double *x_new, *v_new, *rho_new;
   const double *m, *xw, *vw, *nw_;
@@ -61,11 +62,14 @@ struct BatchDev {
   int* status;
   unsigned long long *vmax, *bbox;  // bbox: 4 per simulation (min x, min y, max x, max y), biased
   int64_t* total_cells;
-  unsigned int* flags;              // bit 0 cell overflow, bit 1 list overflow
+  unsigned int* flags;              // bit 0 cell overflow
   int32_t *cid, *ccount, *cfill, *members;
   int64_t* cstart;
+  // Lists in fixed slots, interleaved by wave: entry k of row r sits at
+  // (r / 64) * 64 * K + 64 k + r % 64, so the 64 rows of a wave read their
+  // k-th entries as one coalesced 256-byte run and no count pass or scan is
+  // needed; fcnt / wcnt hold the counts.
   int32_t *fcnt, *wcnt, *fidx, *widx;
-  int64_t *foff, *woff;
   int64_t* step_counter;
 };
 
@@ -286,15 +290,18 @@ __global__ void __launch_bounds__(kBlock) k_b_cell_sort(BatchDev b) {
 // MODE 0 counts, MODE 1 writes.  Fluid rows: every other live row of the same
 // simulation within the list radius (fluid rows sort before walls in a cell);
 // wall stations: fluid rows only.  Visit order: 3 x 3 cells lexicographic.
-template <int MODE>
+// One pass over the 3 x 3 cells around every row: fluid rows list fluid and
+// wall neighbours of their own simulation, wall stations list fluid rows, all
+// with d^2 <= (cell edge)^2, into the row's fixed slots.  A row with more
+// neighbours than slots keeps the first ones and fails its simulation
+// (status bit 2): its step is discarded by k_b_advance.
 __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
   const int64_t c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (c >= b.cap_f + b.cap_w) return;
   const bool wall = c >= b.cap_f;
-  if (b.cid[c] < 0) {
-    if (MODE == 0) { if (wall) b.wcnt[c - b.cap_f] = 0; else b.fcnt[c] = 0; }
-    return;
-  }
+  const int64_t r = wall ? c - b.cap_f : c;
+  int32_t* cnt = wall ? b.wcnt : b.fcnt;
+  if (b.cid[c] < 0) { cnt[r] = 0; return; }
   int s;
   row_live(b, c, s);
   double x, y;
@@ -302,16 +309,9 @@ __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
   const BSim& q = b.bs[s];
   const double cut2 = q.cell * q.cell;
   const int64_t cx = cell_of(x, q.cell) - q.cmin[0], cy = cell_of(y, q.cell) - q.cmin[1];
-  int64_t base = 0, cap = 0;
-  int32_t* out = nullptr;
-  if (MODE == 1) {
-    base = wall ? b.woff[c - b.cap_f] : b.foff[c];
-    const int64_t end = wall ? b.woff[c - b.cap_f + 1] : b.foff[c + 1];
-    cap = wall ? b.cap_we : b.cap_e;
-    if (end > cap) { atomicOr(b.flags, 2u); return; }
-    out = wall ? b.widx : b.fidx;
-  }
-  int k = 0;
+  const int64_t K = wall ? b.kw : b.kf;
+  int32_t* out = (wall ? b.widx : b.fidx) + (r >> 6) * 64 * K + (r & 63);
+  int64_t k = 0;
   for (int dx = -1; dx <= 1; ++dx) {
     const int64_t nx = cx + dx;
     if (nx < 0 || nx >= q.ext[0]) continue;
@@ -333,29 +333,30 @@ __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
           d2 = sx * sx + sy * sy;
         }
         if (d2 > cut2) continue;
-        if (MODE == 1) out[base + k] = (int32_t)o;
+        if (k < K) out[64 * k] = (int32_t)o;
         ++k;
       }
     }
   }
-  if (MODE == 0) { if (wall) b.wcnt[c - b.cap_f] = k; else b.fcnt[c] = k; }
+  cnt[r] = (int32_t)(k < K ? k : K);
+  if (k > K) atomicOr(&b.status[s], 4);
 }
 
 // Optional: each list in ascending row order (the order of tsph_nlist_from_pairs),
 // so sums run in the single-step API's order; used to test the engine bitwise.
 __global__ void __launch_bounds__(kBlock) k_b_sort_lists(BatchDev b) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= b.cap_f + b.cap_w || b.cid[c] < 0 || (*b.flags & 2u)) return;
+  if (c >= b.cap_f + b.cap_w || b.cid[c] < 0) return;
   const bool wall = c >= b.cap_f;
   const int64_t r = wall ? c - b.cap_f : c;
-  const int64_t* off = wall ? b.woff : b.foff;
-  int32_t* seg = (wall ? b.widx : b.fidx) + off[r];
-  const int64_t n = off[r + 1] - off[r];
+  const int64_t K = wall ? b.kw : b.kf;
+  int32_t* seg = (wall ? b.widx : b.fidx) + (r >> 6) * 64 * K + (r & 63);   // stride 64
+  const int64_t n = (wall ? b.wcnt : b.fcnt)[r];
   for (int64_t a = 1; a < n; ++a) {
-    const int32_t v = seg[a];
+    const int32_t v = seg[64 * a];
     int64_t k = a - 1;
-    while (k >= 0 && seg[k] > v) { seg[k + 1] = seg[k]; --k; }
-    seg[k + 1] = v;
+    while (k >= 0 && seg[64 * k] > v) { seg[64 * (k + 1)] = seg[64 * k]; --k; }
+    seg[64 * (k + 1)] = v;
   }
 }
 
@@ -443,7 +444,7 @@ struct tsph_batch {
   DevBuf x[2], v[2], rho[2], m, fsim, xw, vw, nw, wsim;
   DevBuf dps, bs, state, active, status, vmax, bbox, total_cells, flags, step_counter;
   DevBuf cid, ccount, cfill, members, cstart, cstart_tiles;
-  DevBuf fcnt, wcnt, foff, woff, fidx, widx, f_tiles, w_tiles;
+  DevBuf fcnt, wcnt, fidx, widx;
   DevBuf wall_p, wall_rho, reaction, acc, lam, inv, gr, normal, strength, shift, surface, vol;
   DevBuf record;
   BatchRecord* host_rec = nullptr;               // pinned
@@ -465,7 +466,8 @@ static BatchDev batch_dev(tsph_batch* B, int side) {
   BatchDev d{};
   const tsph_batch_caps& c = B->caps;
   d.cap_f = c.fluid; d.cap_w = c.wall; d.cap_s = c.sims; d.cap_cells = c.cells;
-  d.cap_e = c.fluid_entries; d.cap_we = c.wall_entries;
+  d.kf = c.fluid_entries / c.fluid;
+  d.kw = c.wall > 0 ? c.wall_entries / c.wall : 0;
   d.x = B->x[side].as<double>(); d.v = B->v[side].as<double>(); d.rho = B->rho[side].as<double>();
   d.x_new = B->x[side ^ 1].as<double>(); d.v_new = B->v[side ^ 1].as<double>();
   d.rho_new = B->rho[side ^ 1].as<double>();
@@ -480,7 +482,6 @@ static BatchDev batch_dev(tsph_batch* B, int side) {
   d.members = B->members.as<int32_t>(); d.cstart = B->cstart.as<int64_t>();
   d.fcnt = B->fcnt.as<int32_t>(); d.wcnt = B->wcnt.as<int32_t>();
   d.fidx = B->fidx.as<int32_t>(); d.widx = B->widx.as<int32_t>();
-  d.foff = B->foff.as<int64_t>(); d.woff = B->woff.as<int64_t>();
   d.step_counter = B->step_counter.as<int64_t>();
   return d;
 }
@@ -498,6 +499,7 @@ static DpArgs batch_dp_args(tsph_batch* B, const BatchDev& d) {
   a.strength = B->strength.as<double>(); a.shift = B->shift.as<double>(); a.x_new = d.x_new;
   a.surface = B->surface.as<unsigned char>();
   a.vol = B->vol.as<double>();
+  a.fcnt = d.fcnt; a.wcnt = d.wcnt; a.kf = d.kf; a.kw = d.kw;
   return a;
 }
 
@@ -517,18 +519,12 @@ static int batch_queue_step(tsph_batch* B, int side, hipStream_t stream) {
   TSPH_TRY(queue_scan(pc, d.ccount, d.total_cells, d.cap_cells, d.cstart, stream));
   k_b_scatter<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   k_b_cell_sort<<<nblk(d.cap_cells), kBlock, 0, stream>>>(d);
-  k_b_lists<0><<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
-  TSPH_CHECK_LAUNCH();
-  ScanPlan pf{d.cap_f, (d.cap_f + kScanTile) / kScanTile, B->f_tiles.as<int64_t>()};
-  ScanPlan pw{d.cap_w, (d.cap_w + kScanTile) / kScanTile, B->w_tiles.as<int64_t>()};
-  TSPH_TRY(queue_scan(pf, d.fcnt, nullptr, d.cap_f, d.foff, stream));
-  TSPH_TRY(queue_scan(pw, d.wcnt, nullptr, d.cap_w, d.woff, stream));
-  k_b_lists<1><<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
+  k_b_lists<<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
   if (B->caps.sorted_lists) k_b_sort_lists<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   TSPH_CHECK_LAUNCH();
   // the δ⁺ step lists index walls as n_fluid + w: here n_fluid = cap_f
   DpArgs a = batch_dp_args(B, d);
-  TSPH_TRY(dp_queue_step(a, d.foff, d.fidx, d.woff, d.widx, stream));
+  TSPH_TRY(dp_queue_step(a, nullptr, d.fidx, nullptr, d.widx, stream));
   k_b_vmax<<<nblk(d.cap_f), kBlock, 0, stream>>>(d);
   k_b_advance<<<nblk(d.cap_s), kBlock, 0, stream>>>(d);
   k_b_record<<<1, 64, 0, stream>>>(d, B->record.as<BatchRecord>());
@@ -605,8 +601,8 @@ int tsph_batch_destroy(tsph_batch* B) {
   DevBuf* bufs[] = {&B->x[0], &B->x[1], &B->v[0], &B->v[1], &B->rho[0], &B->rho[1], &B->m, &B->fsim, &B->xw,
                     &B->vw, &B->nw, &B->wsim, &B->dps, &B->bs, &B->state, &B->active, &B->status, &B->vmax,
                     &B->bbox, &B->total_cells, &B->flags, &B->step_counter, &B->cid, &B->ccount, &B->cfill,
-                    &B->members, &B->cstart, &B->cstart_tiles, &B->fcnt, &B->wcnt, &B->foff, &B->woff,
-                    &B->fidx, &B->widx, &B->f_tiles, &B->w_tiles, &B->wall_p, &B->wall_rho, &B->reaction,
+                    &B->members, &B->cstart, &B->cstart_tiles, &B->fcnt, &B->wcnt,
+                    &B->fidx, &B->widx, &B->wall_p, &B->wall_rho, &B->reaction,
                     &B->acc, &B->lam, &B->inv, &B->gr, &B->normal, &B->strength, &B->shift, &B->surface,
                     &B->vol, &B->record};
   for (DevBuf* b : bufs) b->release();
@@ -625,6 +621,10 @@ int tsph_batch_create(const tsph_batch_caps* caps, hipStream_t stream, tsph_batc
                c.wall_entries >= 0, TSPH_ERR_ARG, "capacities must be positive");
   TSPH_REQUIRE(c.fluid + c.wall < (1ll << 31) - 1 && c.cells < (1ll << 31) - 1 &&
                c.fluid_entries < (1ll << 40) && c.sims <= (1 << 20), TSPH_ERR_ARG, "capacities too large");
+  const int64_t kf = c.fluid_entries / c.fluid, kw = c.wall > 0 ? c.wall_entries / c.wall : 0;
+  TSPH_REQUIRE(kf >= 32 && kf <= 256 && (c.wall == 0 || (kw >= 32 && kw <= 256)), TSPH_ERR_ARG,
+               "neighbour-list entries must give 32..256 slots per row (fluid %lld, wall %lld)",
+               (long long)kf, (long long)kw);
   tsph_batch* B = new tsph_batch();
   B->caps = c;
   const size_t F = (size_t)c.fluid, W = (size_t)c.wall + 1, S = (size_t)c.sims, C = (size_t)c.cells;
@@ -638,10 +638,8 @@ int tsph_batch_create(const tsph_batch_caps* caps, hipStream_t stream, tsph_batc
       {&B->flags, 8}, {&B->step_counter, 8},
       {&B->cid, 4 * N}, {&B->ccount, 4 * C}, {&B->cfill, 4 * C}, {&B->members, 4 * N},
       {&B->cstart, 8 * (C / kScanTile + 2) * kScanTile}, {&B->cstart_tiles, 8 * (C / kScanTile + 3)},
-      {&B->fcnt, 4 * F}, {&B->wcnt, 4 * W}, {&B->foff, 8 * (F / kScanTile + 2) * kScanTile},
-      {&B->woff, 8 * (W / kScanTile + 2) * kScanTile}, {&B->f_tiles, 8 * (F / kScanTile + 3)},
-      {&B->w_tiles, 8 * (W / kScanTile + 3)},
-      {&B->fidx, 4 * (size_t)c.fluid_entries + 4}, {&B->widx, 4 * (size_t)c.wall_entries + 4},
+      {&B->fcnt, 4 * F}, {&B->wcnt, 4 * W},
+      {&B->fidx, 4 * ((F + 63) / 64) * 64 * (size_t)kf + 4}, {&B->widx, 4 * ((W + 63) / 64) * 64 * (size_t)kw + 4},
       {&B->wall_p, 8 * W}, {&B->wall_rho, 8 * W}, {&B->reaction, 16 * W}, {&B->acc, 16 * F},
       {&B->lam, 8 * F}, {&B->inv, 32 * F}, {&B->gr, 16 * F}, {&B->normal, 16 * F}, {&B->strength, 8 * F},
       {&B->shift, 16 * F}, {&B->surface, F}, {&B->vol, 8 * F},
@@ -757,7 +755,6 @@ int tsph_batch_step(tsph_batch* B, int64_t nsteps, hipStream_t stream) {
   }
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));
   TSPH_REQUIRE(!(B->host_rec->flags & 1u), TSPH_ERR_OOM, "cell capacity exceeded (grow tsph_batch_caps.cells)");
-  TSPH_REQUIRE(!(B->host_rec->flags & 2u), TSPH_ERR_OOM, "neighbour list capacity exceeded");
   return TSPH_OK;
 }
 

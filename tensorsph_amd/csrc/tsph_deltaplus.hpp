@@ -25,6 +25,11 @@ struct DpArgs {
   int* status;                     // per simulation: bit 0 wall density <= 0, bit 1 bad density
   const double *x, *v, *rho, *m;   // fluid state (FP64, (nf, 2) / (nf))
   double* vol;                     // per fluid row: m / rho, set by sweep 0
+  // Neighbour lists: CSR (off / idx arguments of the sweeps) when fcnt is
+  // NULL; else the batch engine's fixed slots, entry k of row r at
+  // (r / 64) * 64 * K + 64 k + r % 64 with fcnt / wcnt holding the counts.
+  const int32_t *fcnt, *wcnt;
+  int64_t kf, kw;
   const double *xw, *vw, *nw_;     // wall stations: position, velocity, unit normal
   double *wall_p, *wall_rho, *reaction;
   double *acc, *vnew, *lam, *inv, *gr, *rho_new, *normal, *strength, *shift, *x_new;

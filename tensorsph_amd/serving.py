@@ -175,9 +175,9 @@ def build_case(kind: str, kw: dict) -> DpCase:
 
 # device bytes one fluid row / one wall station occupy in the engine (state in
 # both ping-pong sides, sweep intermediates, cell and list bookkeeping with the
-# default 64 / 40 list entries per row)
-FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 8 + 64 * 4 + 16 + 8 + 32 + 16 + 16 + 8 + 16 + 1 + 8 + 8
-WALL_ROW_BYTES = 3 * 16 + 4 + 4 + 4 + 4 + 8 + 40 * 4 + 8 + 8 + 16 + 8
+# default 64 / 40 list slots per row)
+FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 64 * 4 + 16 + 8 + 32 + 16 + 16 + 8 + 16 + 1 + 8 + 8
+WALL_ROW_BYTES = 3 * 16 + 4 + 4 + 4 + 4 + 40 * 4 + 8 + 8 + 16 + 8
 
 
 class BatchEngine:
