@@ -188,6 +188,29 @@ struct NbCells {
   }
 };
 
+// Per-row lists in fixed slots interleaved by wave: entry k of row r sits at
+// (r / 64) * 64 * K + 64 k + r % 64, so the 64 rows of a wave read their k-th
+// entries as one coalesced 256-byte run.  Filled from NbCells in its visiting
+// order with the build radius test (k_slot_lists), so a sweep over the slots
+// sums the same terms in the same order as over the walk.  Fluid targets list
+// fluid and wall rows, wall targets fluid rows only; with_walls == false
+// skips the wall rows (index >= nf) of a fluid target's list.
+struct NbSlots {
+  const int32_t* idx;
+  const int32_t* cnt;
+  int64_t K, nf;
+  template <int D, typename F>
+  TSPH_D void visit(int64_t i, bool with_walls, F&& f) const {
+    const int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
+    const int n = cnt[i];
+    for (int k = 0; k < n; ++k) {
+      const int64_t j = p[64 * k];
+      if (!with_walls && j >= nf) continue;
+      f(j);
+    }
+  }
+};
+
 // Dense per-cell row ranges of rows sorted by cell, fluid rows [0, nf) before
 // wall rows [nf, nt): every run of equal cells writes its begin / end.  The
 // table starts zeroed (k_cell_table_clear), so empty cells are empty ranges.
@@ -215,6 +238,43 @@ __global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int6
   int* cell = reinterpret_cast<int*>(R + L);
   if (first) cell[wall ? 2 : 0] = (int)r;
   if (last) cell[wall ? 3 : 1] = (int)(r + 1);
+}
+
+// MODE 0: entries of every row within the build radius (cell walk, the
+// unfused distance test of the pair search, minimum image when PER), their
+// maximum and, if total is given, their sum; MODE 1: the entries themselves
+// into K slots per row (NbSlots layout).
+template <int D, int MODE, bool PER>
+__global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos, int64_t nf,
+                                                       int64_t nt, double rc2, double b0, double b1, double b2,
+                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                       int64_t K, unsigned int* __restrict__ kmax,
+                                                       unsigned long long* __restrict__ total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int k = 0;
+  if (i < nt) {
+    const double box[3] = {b0, b1, b2};
+    int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
+    walk.visit<D>(i, i < nf, [&](int64_t j) {
+      if (dist2<double, D, PER>(pos, i, j, box) > rc2) return;
+      if (MODE == 1 && k < K) p[64 * k] = (int32_t)j;
+      ++k;
+    });
+    if (MODE == 0) cnt[i] = k;
+  }
+  if (MODE == 0) {
+    int m = k;
+    unsigned long long sum = (unsigned long long)k;
+    for (int o = 32; o > 0; o >>= 1) {
+      const int v = __shfl_xor(m, o, 64);
+      m = v > m ? v : m;
+      sum += __shfl_xor(sum, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      if ((unsigned int)m > __atomic_load_n(kmax, __ATOMIC_RELAXED)) atomicMax(kmax, (unsigned int)m);
+      if (total && sum) atomicAdd(total, sum);
+    }
+  }
 }
 
 }  // namespace tsph

@@ -51,6 +51,16 @@ __global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __r
 // a run keeps per-step load rows when it has obstacle stations or the clamp
 static inline bool load_rows(const tsph_sim* sim) { return sim->obstacle || sim->prm.obstacle_clamp; }
 
+// the per-row slot lists built at the last rebuild
+static inline NbSlots sim_slots(const tsph_sim* sim) {
+  NbSlots l{};
+  l.idx = sim->slot_idx.as<int32_t>();
+  l.cnt = sim->slot_cnt.as<int32_t>();
+  l.K = sim->slot_K;
+  l.nf = sim->n;
+  return l;
+}
+
 // the cell-walk neighbour source over the rows binned at the last rebuild
 static inline NbCells sim_cells(const tsph_sim* sim) {
   NbCells c{};
@@ -64,8 +74,8 @@ static inline NbCells sim_cells(const tsph_sim* sim) {
 // Obstacle loads of one step: the rhs_3d reaction of every obstacle station
 // (the k_wall_loads formula, dam_break_3d.py:410-422) and its moment about the
 // centre, reduced per block (partials in a fixed order, no float atomics).
-template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, NbCells nb,
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, NB nb,
                                                        const int64_t* __restrict__ orig, int64_t obs_lo,
                                                        int64_t obs_hi, double c0, double c1, double c2,
                                                        SimFlags* fl, int slot_in, double* __restrict__ part) {
@@ -81,7 +91,7 @@ __global__ void __launch_bounds__(kBlock) k_step_loads(WcsphArgs<S> a, NbCells n
       for (int d = 0; d < D; ++d) { xw[d] = a.pos[iw * D + d]; vw[d] = a.vel[iw * D + d]; }
       const S prw = a.pr2[iw], mw = a.mass[iw], volw = a.vol[iw];
       Sum<S> r[D];
-      nb.visit<D>(iw, false, [&](int64_t f) {
+      nb.template visit<D>(iw, false, [&](int64_t f) {
         S df[D], r2 = S(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) { df[d] = (S)(a.pos[f * D + d] - xw[d]); r2 += df[d] * df[d]; }
@@ -156,8 +166,8 @@ __global__ void __launch_bounds__(256) k_reduce_loads(const double* __restrict__
 
 
 // ------------------------------------------------------------------ step sweeps
-template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NbCells nbr, SimFlags* fl,
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr, SimFlags* fl,
                                                          int slot_in, int write_eos) {
   if (fl->stale[slot_in]) return;
   const int64_t nb = gridDim.x;
@@ -168,7 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NbCells
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   Sum<double> acc;
-  nbr.visit<D>(i, true, [&](int64_t j) {
+  nbr.template visit<D>(i, true, [&](int64_t j) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -191,8 +201,8 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NbCells
   }
 }
 
-template <int D, typename S, int FORM>
-__global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NbCells nbr,
+template <int D, typename S, int FORM, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NB nbr,
                                                            const S* __restrict__ rho_in,
                                                            const S* __restrict__ grad, SimFlags* fl,
                                                            int slot_in) {
@@ -209,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NbCel
     for (int d = 0; d < D; ++d) gi[d] = grad[i * D + d];
   }
   Sum<S> rate;
-  nbr.visit<D>(i, false, [&](int64_t j) {
+  nbr.template visit<D>(i, false, [&](int64_t j) {
     S df[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
@@ -239,8 +249,8 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NbCel
   a.vol[i] = a.mass[i] / rho;
 }
 
-template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, NbCells nbr,
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, NB nbr,
                                                           S* __restrict__ grad_out, SimFlags* fl,
                                                           int slot_in) {
   if (fl->stale[slot_in]) return;
@@ -251,7 +261,7 @@ __global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, NbCell
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   const S rhoi = a.rho[i];
   Sum<S> G[D], M[D * D];
-  nbr.visit<D>(i, false, [&](int64_t j) {
+  nbr.template visit<D>(i, false, [&](int64_t j) {
     S df[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
@@ -286,8 +296,8 @@ __global__ void __launch_bounds__(kBlock) k_step_gradient(WcsphArgs<S> a, NbCell
   for (int d = 0; d < D; ++d) grad_out[i * D + d] = (S)out[d];
 }
 
-template <int D, typename S>
-__global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NbCells nbr, SimFlags* fl,
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NB nbr, SimFlags* fl,
                                                       int slot_in) {
   if (fl->stale[slot_in]) return;
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -297,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NbCells nb
 #pragma unroll
   for (int d = 0; d < D; ++d) xw[d] = a.pos[iw * D + d];
   Sum<S> num, den;
-  nbr.visit<D>(iw, false, [&](int64_t f) {
+  nbr.template visit<D>(iw, false, [&](int64_t f) {
     S rw[D], r2 = S(0);
 #pragma unroll
     for (int d = 0; d < D; ++d) { rw[d] = (S)(xw[d] - a.pos[f * D + d]); r2 += rw[d] * rw[d]; }
@@ -333,8 +343,8 @@ TSPH_D double wave_sum(double v) {
   return v;
 }
 
-template <int D, typename S, bool METERS>
-__global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NbCells nbr, S* __restrict__ acc_out,
+template <int D, typename S, bool METERS, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S* __restrict__ acc_out,
                                                        StepArgs st, SimFlags* fl, int slot_in, double half_skin) {
   const int64_t nb = gridDim.x;
   const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
@@ -359,7 +369,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NbCells n
     for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
     const S pri = a.pr2[i];
     Sum<S> ap[D], av[D];
-    nbr.visit<D>(i, true, [&](int64_t j) {
+    nbr.template visit<D>(i, true, [&](int64_t j) {
       S df[D], r2 = S(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
@@ -574,7 +584,7 @@ __global__ void k_count_close(const double* __restrict__ pos, int64_t n, NbCells
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
   if (i < n) {
-    nbr.visit<D>(i, true, [&](int64_t j) {
+    nbr.template visit<D>(i, true, [&](int64_t j) {
       if (j < n && j < i) return;
       double r2 = 0.0;
 #pragma unroll
@@ -646,6 +656,23 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   k_cell_table<D><<<g, kBlock, 0, stream>>>(sim->cells_new.as<int64_t>(), nt, n, cs.stride[0], cs.stride[1],
                                             cs.stride[2], sim->cell_R.as<int4>());
   TSPH_CHECK_LAUNCH();
+  // per-row lists at the build radius in fixed slots: count and maximum, then fill
+  TSPH_TRY(sim->slot_cnt.ensure(sizeof(int32_t) * (size_t)nt));
+  TSPH_TRY(sim->slot_max.ensure(sizeof(unsigned int)));
+  TSPH_CHECK_HIP(hipMemsetAsync(sim->slot_max.ptr, 0, sizeof(unsigned int), stream));
+  k_slot_lists<D, 0, false><<<g, kBlock, 0, stream>>>(sim_cells(sim), npos, n, nt, rc * rc, 0.0, 0.0, 0.0,
+                                                      sim->slot_cnt.as<int32_t>(), nullptr, 0,
+                                                      sim->slot_max.as<unsigned int>(), nullptr);
+  TSPH_CHECK_LAUNCH();
+  unsigned int kmax = 0;
+  TSPH_CHECK_HIP(hipMemcpyAsync(&kmax, sim->slot_max.ptr, sizeof(kmax), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  sim->slot_K = ((int64_t)kmax + 3) / 4 * 4 + 4;
+  TSPH_TRY(sim->slot_idx.ensure(sizeof(int32_t) * (size_t)((nt + 63) / 64) * 64 * (size_t)sim->slot_K));
+  k_slot_lists<D, 1, false><<<g, kBlock, 0, stream>>>(sim_cells(sim), npos, n, nt, rc * rc, 0.0, 0.0, 0.0,
+                                                      sim->slot_cnt.as<int32_t>(), sim->slot_idx.as<int32_t>(),
+                                                      sim->slot_K, sim->slot_max.as<unsigned int>(), nullptr);
+  TSPH_CHECK_LAUNCH();
   // both ping-pong buffers carry the (static) wall rows
   if (nt > n) {
     TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[sim->cur ^ 1].as<double>() + n * D, npos + n * D,
@@ -681,7 +708,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const int slot = (int)(k & 1);
   SimFlags* fl = sim->flags.as<SimFlags>();
   WcsphArgs<S> a = sim_args<D, S>(sim);
-  const NbCells nbr = sim_cells(sim);
+  const NbSlots nbr = sim_slots(sim);
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
@@ -689,19 +716,19 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     if (sizeof(S) == 4) a.rho_lo = sim->rho_lo.as<S>();
     WcsphArgs<S> a0 = a;
     a0.rho = sim->rho_raw.as<S>();
-    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0);
+    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0);
     if (sim->prm.op.diffusion_form == 1) {
-      k_step_gradient<D, S><<<gx, kBlock, 0, stream>>>(a0, nbr, sim->grad.as<S>(), fl, slot);
-      k_step_diffusion<D, S, 1><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, sim->grad.as<S>(), fl, slot);
+      k_step_gradient<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, sim->grad.as<S>(), fl, slot);
+      k_step_diffusion<D, S, 1, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, sim->grad.as<S>(), fl, slot);
     } else {
-      k_step_diffusion<D, S, 0><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, nullptr, fl, slot);
+      k_step_diffusion<D, S, 0, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, nullptr, fl, slot);
     }
   } else {
-    k_step_density<D, S><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
+    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
   }
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   if (nw > 0)
-    k_step_wall<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
+    k_step_wall<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.dt = sim->prm.op.dt;
@@ -727,7 +754,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     // the force sweep uses); the force sweep writes the other pos / vel side
     const int64_t lo = n + sim->prm.obstacle_begin, hi = n + sim->prm.obstacle_end;
     const double* c = sim->prm.obstacle_centre;
-    k_step_loads<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr,
+    k_step_loads<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr,
                                                              sim->orig.as<int64_t>(), lo, hi, c[0], c[1], c[2], fl,
                                                              slot, sim->load_part.as<double>());
     k_reduce_loads<<<1, 256, 0, stream>>>(sim->load_part.as<double>(), grid_for(nw), sim->loads_out.as<double>(),
@@ -735,10 +762,10 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     TSPH_CHECK_LAUNCH();
   }
   if (meters)
-    k_step_force<D, S, true><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
+    k_step_force<D, S, true, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
                                                         0.5 * sim->prm.skin);
   else
-    k_step_force<D, S, false><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
+    k_step_force<D, S, false, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
                                                          0.5 * sim->prm.skin);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
@@ -1068,7 +1095,8 @@ int tsph_sim_destroy(tsph_sim* sim) {
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
-                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->ranges,
+                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
+                    &sim->slot_cnt, &sim->slot_max, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
                     &sim->load_part, &sim->loads_out, &sim->clamp_acc};
   for (DevBuf* b : bufs) b->release();

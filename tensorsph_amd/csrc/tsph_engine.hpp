@@ -49,8 +49,10 @@ struct tsph_sim {
   tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
-  tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell-walk sweeps)
+  tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
+  tsph::DevBuf slot_idx, slot_cnt, slot_max;   // per-row lists in fixed slots (NbSlots)
+  int64_t slot_K = 0;
   tsph::StepRecord* ring = nullptr;       // pinned
   hipEvent_t ev[tsph::kRing] = {};
   // per-sweep timing (HIP events around each sweep on the stepping stream)

@@ -497,24 +497,6 @@ __global__ void __launch_bounds__(kBlock) k3_kick_drift(double* __restrict__ pos
   if ((threadIdx.x & 63) == 0) atomicMax(disp, ord_enc(v));
 }
 
-// Entries the periodic_pairs lists at the build radius would hold (each
-// unordered pair twice), with the list build's own distance test
-// (minimum image by division, unfused); the sweeps walk the cells instead.
-__global__ void __launch_bounds__(kBlock) k3_count_entries(const double* __restrict__ pos, int64_t n, NbCells nb,
-                                                           double c2, double b0, double b1, double b2,
-                                                           unsigned long long* __restrict__ total) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long k = 0;
-  if (i < n) {
-    const double box[3] = {b0, b1, b2};
-    nb.visit<3>(i, true, [&](int64_t j) {
-      if (dist2<double, 3, true>(pos, i, j, box) <= c2) ++k;
-    });
-  }
-  for (int o = 32; o > 0; o >>= 1) k += __shfl_xor(k, o, 64);
-  if ((threadIdx.x & 63) == 0 && k) atomicAdd(total, k);
-}
-
 // Re-orders the state by the cell order of the last search; afterwards the
 // search's own slot order is the identity.
 template <typename S>
@@ -649,7 +631,9 @@ struct tsph_psim {
   DevBuf pos, vel[2], mass, acc, orig, anchor, disp;
   DevBuf pos2, vel2, mass2, acc2, orig2, cells2, part, diag;
   DevBuf rho_raw, rho, rho_u, p, moment, gr, rho_lo;
-  DevBuf cell_R, entries;                    // dense cell table; list-equivalent entry count
+  DevBuf cell_R, entries;                    // dense cell table; list entries at the build radius
+  DevBuf slot_idx, slot_cnt, slot_max;       // per-row lists in fixed slots (NbSlots layout)
+  int64_t slot_K = 0;
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
   int cur = 0;
@@ -667,6 +651,15 @@ template <typename S> ThreedScratch<S> psim_scratch(tsph_psim* ps) {
   return b;
 }
 
+static NbSlots psim_slots(const tsph_psim* ps) {
+  NbSlots l{};
+  l.idx = ps->slot_idx.as<int32_t>();
+  l.cnt = ps->slot_cnt.as<int32_t>();
+  l.K = ps->slot_K;
+  l.nf = ps->n;
+  return l;
+}
+
 static NbCells psim_cells(const tsph_psim* ps) {
   NbCells c{};
   c.R = ps->cell_R.as<int4>();
@@ -677,8 +670,9 @@ static NbCells psim_cells(const tsph_psim* ps) {
 }
 
 // periodic_pairs at cutoff 2h + skin (threed/solver.py:254-256, 296-298):
-// the state is re-ordered by (Morton) cell and a dense table of per-cell row
-// ranges lets the sweeps walk the 27 cells around every particle
+// the state is re-ordered by (Morton) cell; a dense table of per-cell row
+// ranges drives a walk of the 27 cells around every particle that writes each
+// row's list, in the walk's order, into fixed slots interleaved by wave
 template <typename S>
 int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   tsph_ctx* ctx = ps->ctx;
@@ -710,10 +704,28 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   k_cell_table_clear<<<grid_for(ps->ncells_dense), kBlock, 0, stream>>>(ps->cell_R.as<int4>(), ps->ncells_dense);
   k_cell_table<3><<<g, kBlock, 0, stream>>>(ps->cells2.as<int64_t>(), n, n, cs.stride[0], cs.stride[1],
                                             cs.stride[2], ps->cell_R.as<int4>());
-  TSPH_CHECK_HIP(hipMemsetAsync(ps->entries.ptr, 0, sizeof(unsigned long long), stream));
+  TSPH_CHECK_LAUNCH();
+  // per-row lists at the build radius (minimum image) in fixed slots: entry
+  // count, its maximum and sum (the periodic_pairs entry count), then fill
   const double* b = ps->prm.box;
-  k3_count_entries<<<g, kBlock, 0, stream>>>(ps->pos.as<double>(), n, psim_cells(ps),
-                                             rc * rc, b[0], b[1], b[2], ps->entries.as<unsigned long long>());
+  TSPH_TRY(ps->slot_cnt.ensure(sizeof(int32_t) * (size_t)n));
+  TSPH_TRY(ps->slot_max.ensure(sizeof(unsigned int)));
+  TSPH_CHECK_HIP(hipMemsetAsync(ps->entries.ptr, 0, sizeof(unsigned long long), stream));
+  TSPH_CHECK_HIP(hipMemsetAsync(ps->slot_max.ptr, 0, sizeof(unsigned int), stream));
+  k_slot_lists<3, 0, true><<<g, kBlock, 0, stream>>>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, b[0],
+                                                     b[1], b[2], ps->slot_cnt.as<int32_t>(), nullptr, 0,
+                                                     ps->slot_max.as<unsigned int>(),
+                                                     ps->entries.as<unsigned long long>());
+  TSPH_CHECK_LAUNCH();
+  unsigned int kmax = 0;
+  TSPH_CHECK_HIP(hipMemcpyAsync(&kmax, ps->slot_max.ptr, sizeof(kmax), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  ps->slot_K = ((int64_t)kmax + 3) / 4 * 4 + 4;
+  TSPH_TRY(ps->slot_idx.ensure(sizeof(int32_t) * (size_t)((n + 63) / 64) * 64 * (size_t)ps->slot_K));
+  k_slot_lists<3, 1, true><<<g, kBlock, 0, stream>>>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, b[0],
+                                                     b[1], b[2], ps->slot_cnt.as<int32_t>(),
+                                                     ps->slot_idx.as<int32_t>(), ps->slot_K,
+                                                     ps->slot_max.as<unsigned int>(), nullptr);
   TSPH_CHECK_LAUNCH();
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
   ps->rebuilds++;
@@ -727,7 +739,7 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   a.pos = ps->pos.as<double>();
   a.vel = ps->vel[ps->cur].as<double>();
   a.mass = ps->mass.as<S>();
-  const NbCells nb = psim_cells(ps);
+  const NbSlots nb = psim_slots(ps);
   if (kick) {
     TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
                                      ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream)));
@@ -841,7 +853,7 @@ int tsph_psim_destroy(tsph_psim* ps) {
   DevBuf* b[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->mass, &ps->acc, &ps->orig, &ps->anchor, &ps->disp,
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
                  &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo,
-                 &ps->cell_R, &ps->entries};
+                 &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);
