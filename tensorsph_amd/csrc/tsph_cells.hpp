@@ -2,6 +2,7 @@
 // neighbour visit, shared by the pair export and the resident stepper.
 #pragma once
 #include "tsph_common.hpp"
+#include "tsph_internal.hpp"
 
 namespace tsph {
 
@@ -16,17 +17,21 @@ template <> TSPH_D float rint_t<float>(float x) { return rintf(x); }
 // Unfused, in T, summed in axis order: the reference's
 // (pos[i] - pos[j]).square().sum(dim=1).
 template <typename T, int D, bool PER>
-TSPH_D T dist2(const T* __restrict__ pos, int64_t i, int64_t j, const T* box) {
+TSPH_D T dist2_pt(const T* xi, const T* xj, const T* box) {
 #pragma clang fp contract(off)
   T acc = T(0);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    T df = pos[i * D + d] - pos[j * D + d];
+    T df = xi[d] - xj[d];
     if (PER) df = df - box[d] * rint_t<T>(df / box[d]);
     const T sq = df * df;
     acc = d == 0 ? sq : acc + sq;
   }
   return acc;
+}
+template <typename T, int D, bool PER>
+TSPH_D T dist2(const T* __restrict__ pos, int64_t i, int64_t j, const T* box) {
+  return dist2_pt<T, D, PER>(pos + i * D, pos + j * D, box);
 }
 
 TSPH_D int64_t find_cell(const int64_t* __restrict__ ukeys, int64_t m, int64_t key) {
@@ -155,8 +160,9 @@ struct NbCells {
   // A two-phase variant (scan, keep accepted rows, then run f over them) was
   // measured 2.6-3.7x slower on MI355X: it turns the shared candidate reads
   // of lanes in one cell into per-lane gathers again.
-  template <int D, typename F>
-  TSPH_D void visit(int64_t i, bool with_walls, F&& f) const {
+  // g(begin, end) for every non-empty candidate row range, in the order of visit
+  template <int D, typename G>
+  TSPH_D void ranges(int64_t i, bool with_walls, G&& g) const {
     int64_t ci[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < D; ++d) ci[d] = cells[i * D + d];
@@ -179,12 +185,16 @@ struct NbCells {
       }
       if (!inside) continue;
       const int4 r = R[L];
-      for (int32_t j = r.x; j < r.y; ++j)
-        if (j != i) f((int64_t)j);
-      if (with_walls)
-        for (int32_t j = r.z; j < r.w; ++j)
-          if (j != i) f((int64_t)j);
+      if (r.y > r.x) g(r.x, r.y);
+      if (with_walls && r.w > r.z) g(r.z, r.w);
     }
+  }
+  template <int D, typename F>
+  TSPH_D void visit(int64_t i, bool with_walls, F&& f) const {
+    ranges<D>(i, with_walls, [&](int32_t b, int32_t e) {
+      for (int32_t j = b; j < e; ++j)
+        if (j != i) f((int64_t)j);
+    });
   }
 };
 
@@ -240,41 +250,94 @@ __global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int6
   if (last) cell[wall ? 3 : 1] = (int)(r + 1);
 }
 
-// MODE 0: entries of every row within the build radius (cell walk, the
-// unfused distance test of the pair search, minimum image when PER), their
-// maximum and, if total is given, their sum; MODE 1: the entries themselves
-// into K slots per row (NbSlots layout).
-template <int D, int MODE, bool PER>
+// Per-row lists at the build radius (cell walk, the unfused distance test of
+// the pair search, minimum image when PER).  Every pass writes each row's
+// entry count, their maximum (kmax) and, if total is given, their sum; with
+// FILL the entries go into K slots per row (NbSlots layout; a row with more
+// than K entries keeps its first K and its full count, so the caller widens
+// the slots and runs the pass again).  Candidates are fetched four at a time
+// ahead of their tests; launched with grid_xcd.
+template <int D, bool FILL, bool PER>
 __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos, int64_t nf,
                                                        int64_t nt, double rc2, double b0, double b1, double b2,
                                                        int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
                                                        int64_t K, unsigned int* __restrict__ kmax,
                                                        unsigned long long* __restrict__ total) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < nt) {
     const double box[3] = {b0, b1, b2};
-    int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
-    walk.visit<D>(i, i < nf, [&](int64_t j) {
-      if (dist2<double, D, PER>(pos, i, j, box) > rc2) return;
-      if (MODE == 1 && k < K) p[64 * k] = (int32_t)j;
-      ++k;
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = pos[i * D + d];
+    int32_t* p = FILL ? idx + (i >> 6) * 64 * K + (i & 63) : nullptr;
+    walk.ranges<D>(i, i < nf, [&](int32_t b, int32_t e) {
+      for (int32_t j0 = b; j0 < e; j0 += 4) {
+        double xj[4][D];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t j = j0 + u < e ? j0 + u : e - 1;   // in range: e > j0 >= b
+#pragma unroll
+          for (int d = 0; d < D; ++d) xj[u][d] = pos[j * D + d];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int32_t j = j0 + u;
+          if (j >= e || j == i) continue;
+          if (dist2_pt<double, D, PER>(xi, xj[u], box) > rc2) continue;
+          if (FILL && k < K) p[64 * k] = j;
+          ++k;
+        }
+      }
     });
-    if (MODE == 0) cnt[i] = k;
+    cnt[i] = k;
   }
-  if (MODE == 0) {
-    int m = k;
-    unsigned long long sum = (unsigned long long)k;
-    for (int o = 32; o > 0; o >>= 1) {
-      const int v = __shfl_xor(m, o, 64);
-      m = v > m ? v : m;
-      sum += __shfl_xor(sum, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      if ((unsigned int)m > __atomic_load_n(kmax, __ATOMIC_RELAXED)) atomicMax(kmax, (unsigned int)m);
-      if (total && sum) atomicAdd(total, sum);
-    }
+  int m = k;
+  unsigned long long sum = (unsigned long long)k;
+  for (int o = 32; o > 0; o >>= 1) {
+    const int v = __shfl_xor(m, o, 64);
+    m = v > m ? v : m;
+    sum += __shfl_xor(sum, o, 64);
   }
+  if ((threadIdx.x & 63) == 0) {
+    if ((unsigned int)m > __atomic_load_n(kmax, __ATOMIC_RELAXED)) atomicMax(kmax, (unsigned int)m);
+    if (total && sum) atomicAdd(total, sum);
+  }
+}
+
+// Builds the NbSlots lists of rows [0, nt) over walk: one pass at the current
+// width K (a counting pass while K is 0), and when a list did not fit, the
+// slots are widened to its length + 1/8 + 4 (a multiple of 4) and the pass
+// runs again.  total, if given, receives the entry count.  Reads the longest
+// list back, so the stream is synchronised.
+template <int D, bool PER>
+int build_slot_lists(const NbCells& walk, const double* pos, int64_t nf, int64_t nt, double rc2,
+                     const double* box, DevBuf& cnt, DevBuf& idx, DevBuf& kmax, int64_t& K,
+                     unsigned long long* total, hipStream_t stream) {
+  TSPH_TRY(cnt.ensure(sizeof(int32_t) * (size_t)(nt > 0 ? nt : 1)));
+  TSPH_TRY(kmax.ensure(sizeof(unsigned int)));
+  const unsigned g = grid_xcd(nt);
+  const double b0 = PER ? box[0] : 0.0, b1 = PER ? box[1] : 0.0, b2 = PER && D > 2 ? box[2] : 0.0;
+  for (int pass = 0; pass < 3; ++pass) {
+    TSPH_CHECK_HIP(hipMemsetAsync(kmax.ptr, 0, sizeof(unsigned int), stream));
+    if (total) TSPH_CHECK_HIP(hipMemsetAsync(total, 0, sizeof(unsigned long long), stream));
+    if (K > 0) {
+      TSPH_TRY(idx.ensure(sizeof(int32_t) * (size_t)((nt + 63) / 64) * 64 * (size_t)K));
+      k_slot_lists<D, true, PER><<<g, kBlock, 0, stream>>>(walk, pos, nf, nt, rc2, b0, b1, b2, cnt.as<int32_t>(),
+                                                            idx.as<int32_t>(), K, kmax.as<unsigned int>(), total);
+    } else {
+      k_slot_lists<D, false, PER><<<g, kBlock, 0, stream>>>(walk, pos, nf, nt, rc2, b0, b1, b2, cnt.as<int32_t>(),
+                                                             nullptr, 0, kmax.as<unsigned int>(), total);
+    }
+    TSPH_CHECK_LAUNCH();
+    unsigned int longest = 0;
+    TSPH_CHECK_HIP(hipMemcpyAsync(&longest, kmax.ptr, sizeof(longest), hipMemcpyDeviceToHost, stream));
+    TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+    if (K > 0 && (int64_t)longest <= K) return TSPH_OK;
+    K = ((int64_t)longest + longest / 8 + 4 + 3) / 4 * 4;
+  }
+  TSPH_REQUIRE(false, TSPH_ERR_HIP, "neighbour lists did not fit %lld slots per row", (long long)K);
+  return TSPH_OK;
 }
 
 }  // namespace tsph

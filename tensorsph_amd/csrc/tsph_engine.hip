@@ -607,6 +607,34 @@ using namespace tsph;
 
 namespace tsph {
 
+// Dense per-cell row ranges of the rows in their current order (fluid and
+// wall rows each grouped by cell; cells_new holds every row's cell) and the
+// per-row lists at the build radius rc in fixed slots, walked from them.
+template <int D>
+int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t* ext, const int64_t* stride,
+                    hipStream_t stream) {
+  const int64_t n = sim->n, nt = sim->nt;
+  double dense = 1.0;
+  for (int d = 0; d < D; ++d) dense *= (double)ext[d];
+  TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG,
+               "cell grid of %.3g cells exceeds the dense range table (%lld cells): the particle "
+               "cloud is too sparse for its cutoff", dense, (long long)kMaxDenseCells);
+  sim->ncells_dense = (int64_t)dense;
+  for (int d = 0; d < 3; ++d) {
+    sim->cell_ext[d] = d < D ? ext[d] : 1;
+    sim->cell_stride[d] = d < D ? stride[d] : 0;
+  }
+  TSPH_TRY(sim->cell_R.ensure(sizeof(int4) * (size_t)sim->ncells_dense));
+  k_cell_table_clear<<<grid_for(sim->ncells_dense), kBlock, 0, stream>>>(sim->cell_R.as<int4>(), sim->ncells_dense);
+  k_cell_table<D><<<grid_for(nt), kBlock, 0, stream>>>(sim->cells_new.as<int64_t>(), nt, n, sim->cell_stride[0],
+                                                       sim->cell_stride[1], sim->cell_stride[2],
+                                                       sim->cell_R.as<int4>());
+  TSPH_CHECK_LAUNCH();
+  return build_slot_lists<D, false>(sim_cells(sim), npos, n, nt, rc * rc, nullptr, sim->slot_cnt, sim->slot_idx,
+                                    sim->slot_max, sim->slot_K, nullptr, stream);
+}
+template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*, const int64_t*, hipStream_t);
+
 template <int D, typename S>
 int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   tsph_ctx* ctx = sim->ctx;
@@ -642,37 +670,7 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   std::swap(sim->orig, sim->orig2);
   sim->cur = other;
   const double* npos = sim->pos[sim->cur].as<double>();
-  // dense per-cell row ranges of the re-ordered rows: the sweeps walk the
-  // 3^D cells around each target instead of per-target lists
-  double dense = 1.0;
-  for (int d = 0; d < D; ++d) dense *= (double)cs.ext[d];
-  TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG,
-               "cell grid of %.3g cells exceeds the dense range table (%lld cells): the particle "
-               "cloud is too sparse for its cutoff", dense, (long long)kMaxDenseCells);
-  sim->ncells_dense = (int64_t)dense;
-  for (int d = 0; d < 3; ++d) { sim->cell_ext[d] = cs.ext[d]; sim->cell_stride[d] = cs.stride[d]; }
-  TSPH_TRY(sim->cell_R.ensure(sizeof(int4) * (size_t)sim->ncells_dense));
-  k_cell_table_clear<<<grid_for(sim->ncells_dense), kBlock, 0, stream>>>(sim->cell_R.as<int4>(), sim->ncells_dense);
-  k_cell_table<D><<<g, kBlock, 0, stream>>>(sim->cells_new.as<int64_t>(), nt, n, cs.stride[0], cs.stride[1],
-                                            cs.stride[2], sim->cell_R.as<int4>());
-  TSPH_CHECK_LAUNCH();
-  // per-row lists at the build radius in fixed slots: count and maximum, then fill
-  TSPH_TRY(sim->slot_cnt.ensure(sizeof(int32_t) * (size_t)nt));
-  TSPH_TRY(sim->slot_max.ensure(sizeof(unsigned int)));
-  TSPH_CHECK_HIP(hipMemsetAsync(sim->slot_max.ptr, 0, sizeof(unsigned int), stream));
-  k_slot_lists<D, 0, false><<<g, kBlock, 0, stream>>>(sim_cells(sim), npos, n, nt, rc * rc, 0.0, 0.0, 0.0,
-                                                      sim->slot_cnt.as<int32_t>(), nullptr, 0,
-                                                      sim->slot_max.as<unsigned int>(), nullptr);
-  TSPH_CHECK_LAUNCH();
-  unsigned int kmax = 0;
-  TSPH_CHECK_HIP(hipMemcpyAsync(&kmax, sim->slot_max.ptr, sizeof(kmax), hipMemcpyDeviceToHost, stream));
-  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-  sim->slot_K = ((int64_t)kmax + 3) / 4 * 4 + 4;
-  TSPH_TRY(sim->slot_idx.ensure(sizeof(int32_t) * (size_t)((nt + 63) / 64) * 64 * (size_t)sim->slot_K));
-  k_slot_lists<D, 1, false><<<g, kBlock, 0, stream>>>(sim_cells(sim), npos, n, nt, rc * rc, 0.0, 0.0, 0.0,
-                                                      sim->slot_cnt.as<int32_t>(), sim->slot_idx.as<int32_t>(),
-                                                      sim->slot_K, sim->slot_max.as<unsigned int>(), nullptr);
-  TSPH_CHECK_LAUNCH();
+  TSPH_TRY((build_sim_slots<D>(sim, npos, rc, cs.ext, cs.stride, stream)));
   // both ping-pong buffers carry the (static) wall rows
   if (nt > n) {
     TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[sim->cur ^ 1].as<double>() + n * D, npos + n * D,
@@ -699,6 +697,25 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   a.rho = sim->rho.as<S>(); a.p = sim->p.as<S>(); a.pr2 = sim->pr2.as<S>(); a.vol = sim->vol.as<S>();
   return a;
 }
+
+// Density (+ EOS) and wall sweeps over the slot lists: the first two sweeps of
+// a step on the cell-tile path, whose force sweep runs on the tiles.
+template <int D, typename S>
+int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
+  const int slot = (int)(k & 1);
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  const WcsphArgs<S> a = sim_args<D, S>(sim);
+  const NbSlots nbr = sim_slots(sim);
+  const int r = (int)(k % kRing);
+  k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
+  TSPH_CHECK_LAUNCH();
+  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
+  if (sim->nw > 0) k_step_wall<D, S, NbSlots><<<grid_for(sim->nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+template int queue_slot_density_wall<2, float>(tsph_sim*, int64_t, hipStream_t);
+template int queue_slot_density_wall<2, double>(tsph_sim*, int64_t, hipStream_t);
 
 // The cell-walk sweeps of one step (used where no cell-tile path exists).
 template <int D, typename S>

@@ -93,4 +93,8 @@ __global__ void k_reduce_sums(const double* part, int64_t nblocks, double* out, 
 template <int D, typename S> int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream);
 int tile_count_pairs(tsph_sim* sim, double radius, unsigned long long* cnt, hipStream_t stream);
 template <int D, typename S> int sim_queue_step_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream);
+template <int D>
+int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t* ext, const int64_t* stride,
+                    hipStream_t stream);
+template <int D, typename S> int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream);
 }  // namespace tsph

@@ -705,28 +705,11 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   k_cell_table<3><<<g, kBlock, 0, stream>>>(ps->cells2.as<int64_t>(), n, n, cs.stride[0], cs.stride[1],
                                             cs.stride[2], ps->cell_R.as<int4>());
   TSPH_CHECK_LAUNCH();
-  // per-row lists at the build radius (minimum image) in fixed slots: entry
-  // count, its maximum and sum (the periodic_pairs entry count), then fill
-  const double* b = ps->prm.box;
-  TSPH_TRY(ps->slot_cnt.ensure(sizeof(int32_t) * (size_t)n));
-  TSPH_TRY(ps->slot_max.ensure(sizeof(unsigned int)));
-  TSPH_CHECK_HIP(hipMemsetAsync(ps->entries.ptr, 0, sizeof(unsigned long long), stream));
-  TSPH_CHECK_HIP(hipMemsetAsync(ps->slot_max.ptr, 0, sizeof(unsigned int), stream));
-  k_slot_lists<3, 0, true><<<g, kBlock, 0, stream>>>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, b[0],
-                                                     b[1], b[2], ps->slot_cnt.as<int32_t>(), nullptr, 0,
-                                                     ps->slot_max.as<unsigned int>(),
-                                                     ps->entries.as<unsigned long long>());
-  TSPH_CHECK_LAUNCH();
-  unsigned int kmax = 0;
-  TSPH_CHECK_HIP(hipMemcpyAsync(&kmax, ps->slot_max.ptr, sizeof(kmax), hipMemcpyDeviceToHost, stream));
-  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-  ps->slot_K = ((int64_t)kmax + 3) / 4 * 4 + 4;
-  TSPH_TRY(ps->slot_idx.ensure(sizeof(int32_t) * (size_t)((n + 63) / 64) * 64 * (size_t)ps->slot_K));
-  k_slot_lists<3, 1, true><<<g, kBlock, 0, stream>>>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, b[0],
-                                                     b[1], b[2], ps->slot_cnt.as<int32_t>(),
-                                                     ps->slot_idx.as<int32_t>(), ps->slot_K,
-                                                     ps->slot_max.as<unsigned int>(), nullptr);
-  TSPH_CHECK_LAUNCH();
+  // per-row lists at the build radius (minimum image) in fixed slots; their
+  // entry count is the periodic_pairs entry count
+  TSPH_TRY((build_slot_lists<3, true>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, ps->prm.box,
+                                      ps->slot_cnt, ps->slot_idx, ps->slot_max, ps->slot_K,
+                                      ps->entries.as<unsigned long long>(), stream)));
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
   ps->rebuilds++;
   ps->built = true;

@@ -1,4 +1,6 @@
-// LDS cell-tile sweeps of the resident 2-D stepper.
+// LDS cell-tile force sweep of the resident 2-D stepper (the density and
+// wall sweeps of the same steps run over the per-row slot lists, which the
+// rebuild here also writes).
 //
 // The neighbour search's cells (edge 2h + skin) are grouped into TX x TY
 // tiles.  Rows are ordered tile-major (fluid rows first, then wall rows, each
@@ -7,8 +9,8 @@
 // of cells around the tile (the tile plus a one-cell halo) from HBM into LDS
 // with contiguous, coalesced range reads, laid out so that the three cells
 // of any stencil row are adjacent.  Each target particle of the tile then
-// scans three contiguous LDS ranges.  There is no per-particle neighbour
-// list, so the only HBM streams are the particle state itself.
+// scans three contiguous LDS ranges, so the force sweep streams only the
+// particle state itself.
 //
 // Visiting order is fixed (stencil rows bottom to top, cells left to right,
 // fluid members then wall members, each in row order), so sums are
@@ -167,120 +169,6 @@ TSPH_D void visit_stencil(const BlockTable& bt, int BW, int bx, int by, bool lds
         for (int m = 0; m < nwl; ++m) f((int64_t)bt.ws[c] + m);
       }
     }
-  }
-}
-
-// ------------------------------------------------------------------ sweep A (density)
-template <typename S, int MAXP>
-__global__ void __launch_bounds__(kBlock) k_tile_density(WcsphArgs<S> a, TileGeom g,
-    const TileRec* __restrict__ tiles, int64_t ntiles, const int32_t* __restrict__ R,
-    const int64_t* __restrict__ cells, SimFlags* fl, int slot, int write_eos) {
-  if (fl->stale[slot]) return;
-  const int64_t b = xcd_block(blockIdx.x, gridDim.x);
-  if (blockIdx.x == 0 && threadIdx.x == 0) fl->maxes[MM_DISP] = 0;
-  if (b >= ntiles) return;
-  __shared__ BlockTable bt;
-  __shared__ double2 sx[MAXP];
-  __shared__ S sm[MAXP];
-  __shared__ int srow[MAXP];
-  __shared__ unsigned short nb[kNbMax * kBlock];
-  const TileRec tr = tiles[b];
-  const int64_t bx0 = (tr.tile / g.nty) * g.TX - 1, by0 = (tr.tile % g.nty) * g.TY - 1;
-  const int BW = g.TX + 2, NC = BW * (g.TY + 2);
-  const int total = stage_table(bt, g, bx0, by0, R);
-  const bool lds = total <= MAXP;
-  if (lds) {
-    for (int k = threadIdx.x; k < total; k += blockDim.x) {
-      const int64_t row = staged_row(bt, NC, k);
-      sx[k] = make_double2(a.pos[2 * row], a.pos[2 * row + 1]);
-      sm[k] = a.mass[row];
-      srow[k] = (int)row;
-    }
-  }
-  __syncthreads();
-  for (int64_t r = tr.start + threadIdx.x; r < tr.end; r += blockDim.x) {
-    const double xi0 = a.pos[2 * r], xi1 = a.pos[2 * r + 1];
-    const int bx = (int)(cells[2 * r] - bx0), by = (int)(cells[2 * r + 1] - by0);
-    Sum<double> acc;
-    auto body = [&](int64_t q) {
-      int64_t j;
-      double xj0, xj1;
-      S mj;
-      if (lds) { j = srow[q]; xj0 = sx[q].x; xj1 = sx[q].y; mj = sm[q]; }
-      else { j = q; xj0 = a.pos[2 * q]; xj1 = a.pos[2 * q + 1]; mj = a.mass[q]; }
-      if (j == r) return;
-      const double d0 = xi0 - xj0, d1 = xi1 - xj1;
-      const double r2 = d0 * d0 + d1 * d1;
-      if (r2 > a.skip2d) return;
-      double w, gg;
-      cubic_wg<double>(sqrt(r2), a.kd, w, gg);
-      acc.add((double)mj * w);
-    };
-    bool dense = false;
-    if (lds) {
-      int cnt = 0;
-      visit_stencil(bt, BW, bx, by, true, [&](int64_t q) {
-        const double d0 = xi0 - sx[q].x, d1 = xi1 - sx[q].y;
-        if (d0 * d0 + d1 * d1 > a.skip2d || srow[q] == r) return;
-        if (cnt < kNbMax) nb[cnt * kBlock + threadIdx.x] = (unsigned short)q;
-        ++cnt;
-      });
-      if (cnt <= kNbMax) {
-        for (int e = 0; e < cnt; ++e) body((int64_t)nb[e * kBlock + threadIdx.x]);
-        dense = true;
-      }
-    }
-    if (!dense) visit_stencil(bt, BW, bx, by, lds, body);
-    double rho_d = hi(acc) + (double)a.mass[r] * a.kd.w0d;
-    rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
-    const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, r);
-    if (write_eos) {
-      const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
-      a.p[r] = p;
-      a.pr2[r] = p / (rho * rho);
-      a.vol[r] = a.mass[r] / rho;
-    }
-  }
-}
-
-// ------------------------------------------------------------------ sweep W (wall pressure)
-template <typename S>
-__global__ void __launch_bounds__(kBlock) k_tile_wall(WcsphArgs<S> a, TileGeom g,
-    const TileRec* __restrict__ tiles, int64_t ntiles, const int32_t* __restrict__ R,
-    const int64_t* __restrict__ cells, SimFlags* fl, int slot) {
-  if (fl->stale[slot]) return;
-  const int64_t b = blockIdx.x;
-  if (b >= ntiles) return;
-  __shared__ BlockTable bt;
-  const TileRec tr = tiles[b];
-  const int64_t bx0 = (tr.tile / g.nty) * g.TX - 1, by0 = (tr.tile % g.nty) * g.TY - 1;
-  const int BW = g.TX + 2;
-  stage_table(bt, g, bx0, by0, R);
-  for (int64_t r = tr.start + threadIdx.x; r < tr.end; r += blockDim.x) {
-    const double xw0 = a.pos[2 * r], xw1 = a.pos[2 * r + 1];
-    const int bx = (int)(cells[2 * r] - bx0), by = (int)(cells[2 * r + 1] - by0);
-    Sum<S> num, den;
-    visit_stencil(bt, BW, bx, by, false, [&](int64_t f) {
-      if (f >= a.n) return;                     // fluid donors only
-      const S r0 = (S)(xw0 - a.pos[2 * f]), r1 = (S)(xw1 - a.pos[2 * f + 1]);
-      const S r2 = r0 * r0 + r1 * r1;
-      if (r2 > a.skip2) return;
-      S wgt, gg;
-      cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, gg);
-      S src = a.p[f];
-      if (a.gravity != S(0)) src = src + a.rho[f] * (-a.gravity * r1);
-      num.add(src * wgt);
-      den.add(wgt);
-    });
-    const S dd = den.get() > S(1e-30) ? den.get() : S(1e-30);
-    S pw = num.get() / dd;
-    pw = pw > S(0) ? pw : S(0);
-    S rw = a.rho0 * pow_s<S>(pw / a.bulk + S(1), a.inv_gamma);
-    rw = rw > a.rho0 ? rw : a.rho0;
-    a.p[r] = pw;
-    a.rho[r] = rw;
-    a.pr2[r] = pw / (rw * rw);
-    a.vol[r] = a.mass[r] / rw;
   }
 }
 
@@ -563,6 +451,8 @@ int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream) {
   TSPH_CHECK_HIP(hipMemsetAsync(sim->ranges.ptr, 0, sizeof(int32_t) * 4 * ncells, stream));
   k_cell_ranges<<<g, kBlock, 0, stream>>>(sim->cells_new.as<int64_t>(), nt, n, cs.ext[1], ncells,
                                           sim->ranges.as<int32_t>());
+  // per-row lists for the density and wall sweeps, walked over the same cells
+  TSPH_TRY((build_sim_slots<2>(sim, sim->pos[sim->cur].as<double>(), rc, cs.ext, cs.stride, stream)));
   unsigned int* flags = sim->tflags.as<unsigned int>();
   unsigned int* ids = flags + nt;
   k_tile_flags<<<g, kBlock, 0, stream>>>(sim->tkeys_sorted.as<unsigned long long>(), nt, n, K2, TX * TY, flags);
@@ -619,12 +509,7 @@ int queue_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   const unsigned gf = grid_xcd(sim->ntiles_f * kBlock);
   const int r = (int)(k % kRing);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
-  k_tile_density<S, MAXP><<<gf, kBlock, 0, stream>>>(a, g, tf, sim->ntiles_f, R, cells, fl, slot, 1);
-  TSPH_CHECK_LAUNCH();
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
-  if (sim->ntiles_w > 0)
-    k_tile_wall<S><<<(unsigned)sim->ntiles_w, kBlock, 0, stream>>>(a, g, tw, sim->ntiles_w, R, cells, fl, slot);
-  TSPH_CHECK_LAUNCH();
+  TSPH_TRY((queue_slot_density_wall<2, S>(sim, k, stream)));
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.dt = sim->prm.op.dt;
