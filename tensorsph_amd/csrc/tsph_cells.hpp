@@ -149,6 +149,20 @@ struct NbList {
     const int64_t e1 = off[i + 1];
     for (int64_t e = off[i]; e < e1; ++e) f((int64_t)idx[e]);
   }
+  // f(j, true) for every entry, U at a time in one straight-line block
+  template <int D, int U = 4, typename F>
+  TSPH_D void visit_masked(int64_t i, bool /*with_walls*/, F&& f) const {
+    const int64_t e1 = off[i + 1];
+    int64_t e = off[i];
+    for (; e + U <= e1; e += U) {
+      int32_t j[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) j[u] = idx[e + u];
+#pragma unroll
+      for (int u = 0; u < U; ++u) f((int64_t)j[u], true);
+    }
+    for (; e < e1; ++e) f((int64_t)idx[e], true);
+  }
 };
 
 struct NbCells {
@@ -217,6 +231,27 @@ struct NbSlots {
       const int64_t j = p[64 * k];
       if (!with_walls && j >= nf) continue;
       f(j);
+    }
+  }
+  // f(j, keep) for every entry, U at a time in one straight-line block: the
+  // U index loads and the gathers f makes for them are all issued before the
+  // arithmetic that needs them.  keep is false for the wall rows that
+  // with_walls == false skips; f must not branch on it.
+  template <int D, int U = 4, typename F>
+  TSPH_D void visit_masked(int64_t i, bool with_walls, F&& f) const {
+    const int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
+    const int n = cnt[i];
+    int k = 0;
+    for (; k + U <= n; k += U) {
+      int32_t j[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) j[u] = p[64 * (k + u)];
+#pragma unroll
+      for (int u = 0; u < U; ++u) f((int64_t)j[u], with_walls || j[u] < nf);
+    }
+    for (; k < n; ++k) {
+      const int32_t j = p[64 * k];
+      f((int64_t)j, with_walls || j < nf);
     }
   }
 };

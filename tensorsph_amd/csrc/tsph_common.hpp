@@ -22,9 +22,13 @@ constexpr double kPi = 3.14159265358979323846;
 
 // ---------------------------------------------------------------- accumulators
 template <typename S> struct Sum;
+// add_if(keep, x) is add(x) when keep and leaves the sum untouched otherwise,
+// so a sweep can evaluate every candidate without branching and still sum
+// exactly the terms the branching form sums.
 template <> struct Sum<double> {
   double s = 0.0;
   TSPH_D void add(double x) { s += x; }
+  TSPH_D void add_if(bool keep, double x) { s = keep ? s + x : s; }
   TSPH_D double get() const { return s; }
 };
 // Kahan summation: the running compensation c carries the low-order bits that
@@ -36,6 +40,13 @@ template <> struct Sum<float> {
     float t = s + y;
     c = (t - s) - y;
     s = t;
+  }
+  TSPH_D void add_if(bool keep, float x) {
+    const float y = x - c;
+    const float t = s + y;
+    const float cn = (t - s) - y;
+    c = keep ? cn : c;
+    s = keep ? t : s;
   }
   TSPH_D float get() const { return s; }
 };

@@ -178,17 +178,17 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   Sum<double> acc;
-  nbr.template visit<D>(i, true, [&](int64_t j) {
+  nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const double df = xi[d] - a.pos[j * D + d];
       r2 += df * df;
     }
-    if (r2 > a.skip2d) return;
+    keep = keep && !(r2 > a.skip2d);
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
-    acc.add((double)a.mass[j] * w);
+    acc.add_if(keep, (double)a.mass[j] * w);
   });
   double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
@@ -369,25 +369,26 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
     const S pri = a.pr2[i];
     Sum<S> ap[D], av[D];
-    nbr.template visit<D>(i, true, [&](int64_t j) {
+    // every list entry is evaluated; terms outside the support (and the
+    // viscous term of frictionless wall pairs) are left out of the sums
+    nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
       S df[D], r2 = S(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
-      if (r2 > a.skip2) return;
+      keep = keep && !(r2 > a.skip2);
       const S g = cubic_g_r2<S>(r2, a.k);
       const S cp = -a.mass[j] * (pri + a.pr2[j]);
 #pragma unroll
-      for (int d = 0; d < D; ++d) ap[d].add(cp * (df[d] * g));
+      for (int d = 0; d < D; ++d) ap[d].add_if(keep, cp * (df[d] * g));
       const S blend = j < a.n ? S(1) : a.friction;
-      if (blend != S(0)) {
-        S dg = S(0);
+      S dg = S(0);
 #pragma unroll
-        for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
-        const S relu = -dg > S(0) ? -dg : S(0);
-        const S cv = -(a.nu2 * blend) * (a.vol[j] * quot<S>(relu, r2 + a.eta2));
+      for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
+      const S relu = -dg > S(0) ? -dg : S(0);
+      const S cv = -(a.nu2 * blend) * (a.vol[j] * quot<S>(relu, r2 + a.eta2));
+      const bool visc = keep && blend != S(0);
 #pragma unroll
-        for (int d = 0; d < D; ++d) av[d].add(cv * (S)(vi[d] - a.vel[j * D + d]));
-      }
+      for (int d = 0; d < D; ++d) av[d].add_if(visc, cv * (S)(vi[d] - a.vel[j * D + d]));
     });
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "tsph_cells.hpp"
 #include "tsph_common.hpp"
 #include "tsph_internal.hpp"
@@ -75,18 +77,19 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
   double xi[3];
   load3(a.pos, i, xi);
   Sum<double> acc;
-  nb.template visit<3>(i, true, [&](int64_t j) {
+  const bool per = a.periodic;
+  nb.template visit_masked<3>(i, true, [&](int64_t j, bool keep) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       double v = xi[d] - a.pos[j * 3 + d];
-      if (a.periodic) v = v - a.box[d] * rint(v * a.inv_box[d]);
+      v = per ? v - a.box[d] * rint(v * a.inv_box[d]) : v;
       r2 += v * v;
     }
-    if (r2 > a.skip2d) return;
+    keep = keep && !(r2 > a.skip2d);
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
-    acc.add((double)a.mass[j] * w);
+    acc.add_if(keep, (double)a.mass[j] * w);
   });
   store_rho<S>(hi(acc) + (double)a.mass[i] * a.kd.w0d, out, a.rho_lo, i);
 }
@@ -221,14 +224,14 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
   load3(a.pos, i, xi);
   const S ri = a.rho_u[i];
   Sum<S> acc;
-  nb.template visit<3>(i, true, [&](int64_t j) {
+  nb.template visit_masked<3>(i, true, [&](int64_t j, bool keep) {
     S df[3], r2;
     delta3<S>(a, j, xi, df, r2, a.periodic);
-    if (r2 > a.skip2) return;
+    keep = keep && !(r2 > a.skip2);
     S g, lap;
     cubic_gl_r2<S>(r2, a.k, g, lap);
     const S rij = S(0.5) * (ri + a.rho_u[j]);
-    acc.add(r2 * lap * quot<S>(a.mass[j], rij));
+    acc.add_if(keep, r2 * lap * quot<S>(a.mass[j], rij));
   });
   a.moment[i] = acc.get() / S(3);
 }
@@ -236,10 +239,11 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
 // Pressure acceleration (operators3.py:96-113) + renormalised viscous
 // acceleration (:116-154) + gravity on fluid rows.  With KICK the second
 // half-kick of the leapfrog (threed/solver.py:299) is fused: vel_out = v + dt/2 a.
-// ``orig`` maps rows to the caller's particle order (NULL: identity); the
-// reference weights the viscous pair term by the mass of the pair member with
-// the higher index, so that order matters.
-template <typename S, bool KICK, typename NB>
+// The reference weights the viscous pair term by the mass of the pair member
+// with the higher index (in the caller's order), so that order matters: MW 0
+// takes the row order, MW 1 maps rows through ``orig``, MW 2 is for equal
+// masses, where either member's mass is the same.
+template <typename S, bool KICK, typename NB, int MW>
 __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
@@ -252,35 +256,51 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
   const S ri = a.rho_u[i];
   const S pri = a.do_pressure ? a.p[i] / (ri * ri) : S(0);
   const S Ki = (visc && a.renormalise) ? a.moment[i] : S(0);
-  const int64_t oi = orig ? orig[i] : i;
+  const int64_t oi = MW == 1 ? orig[i] : i;
+  const S mi = a.mass[i];
   Sum<S> ap[3], av[3];
-  nb.template visit<3>(i, true, [&](int64_t j) {
-    S df[3], r2;
-    delta3<S>(a, j, xi, df, r2, a.periodic);
-    if (r2 > a.skip2) return;
-    S g, lap;
-    cubic_gl_r2<S>(r2, a.k, g, lap);
-    const S rj = a.rho_u[j];
-    if (a.do_pressure) {
-      const S sym = pri + quot<S>(a.p[j], rj * rj);
-      const S cp = -(a.mass[j] * sym);
+  // every list entry is evaluated; the terms of those outside the support are
+  // left out of the sums.  The pressure / viscous switches are uniform, so
+  // each combination gets its own straight-line loop body.
+  auto body = [&](auto press, auto vis) {
+    return [&](int64_t j, bool keep) {
+      S df[3], r2;
+      delta3<S>(a, j, xi, df, r2, a.periodic);
+      keep = keep && !(r2 > a.skip2);
+      S g, lap;
+      cubic_gl_r2<S>(r2, a.k, g, lap);
+      const S rj = a.rho_u[j];
+      if constexpr (decltype(press)::value) {
+        const S sym = pri + quot<S>(a.p[j], rj * rj);
+        const S cp = -(a.mass[j] * sym);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) ap[d].add((df[d] * g) * cp);
-    }
-    if (visc) {
-      const S rij = S(0.5) * (ri + rj);
-      const int64_t oj = orig ? orig[j] : j;
-      const S mw = oj > oi ? a.mass[j] : a.mass[i];
-      S wgt = quot<S>(mw * lap, rij);
-      if (a.renormalise) {
-        S mk = S(0.5) * (Ki + a.moment[j]);
-        mk = mk > S(1e-12) ? mk : S(1e-12);
-        wgt = wgt * quot<S>(S(2), mk);
+        for (int d = 0; d < 3; ++d) ap[d].add_if(keep, (df[d] * g) * cp);
       }
+      if constexpr (decltype(vis)::value) {
+        const S rij = S(0.5) * (ri + rj);
+        S mw;
+        if constexpr (MW == 2) {
+          mw = a.mass[j];
+        } else {
+          const int64_t oj = MW == 1 ? orig[j] : j;
+          mw = oj > oi ? a.mass[j] : mi;
+        }
+        S wgt = quot<S>(mw * lap, rij);
+        if (a.renormalise) {
+          S mk = S(0.5) * (Ki + a.moment[j]);
+          mk = mk > S(1e-12) ? mk : S(1e-12);
+          wgt = wgt * quot<S>(S(2), mk);
+        }
 #pragma unroll
-      for (int d = 0; d < 3; ++d) av[d].add((S)(a.vel[3 * j + d] - vi[d]) * wgt);
-    }
-  });
+        for (int d = 0; d < 3; ++d) av[d].add_if(keep, (S)(a.vel[3 * j + d] - vi[d]) * wgt);
+      }
+    };
+  };
+  using T1 = std::integral_constant<bool, true>;
+  using F0 = std::integral_constant<bool, false>;
+  if (a.do_pressure && visc) nb.template visit_masked<3>(i, true, body(T1{}, T1{}));
+  else if (a.do_pressure) nb.template visit_masked<3>(i, true, body(T1{}, F0{}));
+  else if (visc) nb.template visit_masked<3>(i, true, body(F0{}, T1{}));
   S acc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -343,10 +363,10 @@ template <typename S> struct ThreedScratch {
 };
 
 // Queues the sweeps of one accelerations() evaluation over a neighbour source
-// (CSR lists or the cell walk).
+// (CSR lists or slot lists).  uniform_mass: every particle has the same mass.
 template <typename S, bool KICK, typename NB>
 int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b, NB nb, const int64_t* orig,
-                  S* acc_out, double half_dt, double* vel_out, hipStream_t stream) {
+                  S* acc_out, double half_dt, double* vel_out, hipStream_t stream, bool uniform_mass = false) {
   const int64_t n = a.n;
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
@@ -365,7 +385,12 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   }
   if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, nb);
   if (a.nu != S(0) && a.renormalise) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
-  k3_force<S, KICK, NB><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+  if (uniform_mass)
+    k3_force<S, KICK, NB, 2><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+  else if (orig)
+    k3_force<S, KICK, NB, 1><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+  else
+    k3_force<S, KICK, NB, 0><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }
@@ -425,12 +450,12 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
       break;
     case 1:
       a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0;
-      k3_force<S, false, NbList><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
+      k3_force<S, false, NbList, 0><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 2:
       a.rho_u = rho_in; a.do_pressure = 0; a.has_gravity = 0;
       if (a.nu != S(0) && a.renormalise) k3_moment<S, NbList><<<g, kBlock, 0, stream>>>(a, nb);
-      k3_force<S, false, NbList><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
+      k3_force<S, false, NbList, 0><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 3:
       if (nf == n) return TSPH_OK;
@@ -457,6 +482,13 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
 }
 
 // ---------------------------------------------------------------- periodic stepper kernels
+// flag = 1 when some mass differs from the first one
+template <typename S>
+__global__ void k3_mass_differs(const S* __restrict__ m, int64_t n, unsigned int* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && m[i] != m[0]) atomicOr(flag, 1u);
+}
+
 __global__ void k3_iota(int64_t* __restrict__ v, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) v[i] = i;
@@ -639,6 +671,7 @@ struct tsph_psim {
   int cur = 0;
   int64_t steps = 0, rebuilds = 0;
   bool built = false, have_acc = false;
+  bool uniform_mass = false;                 // every particle has the same mass
 };
 
 namespace tsph {
@@ -725,11 +758,12 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   const NbSlots nb = psim_slots(ps);
   if (kick) {
     TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
-                                     ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream)));
+                                     ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream,
+                                     ps->uniform_mass)));
     ps->cur ^= 1;
   } else {
     TSPH_TRY((threed_sweeps<S, false>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
-                                      ps->acc.as<S>(), 0.0, nullptr, stream)));
+                                      ps->acc.as<S>(), 0.0, nullptr, stream, ps->uniform_mass)));
   }
   return TSPH_OK;
 }
@@ -889,7 +923,18 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
     return TSPH_ERR_HIP;
   }
   k3_iota<<<grid_for(ps->n), kBlock, 0, stream>>>(ps->orig.as<int64_t>(), ps->n);
-  e = hipGetLastError();
+  // disp (64 bytes) is free until the first step: it takes the equal-mass flag
+  if (e == hipSuccess) e = hipMemsetAsync(ps->disp.ptr, 0, sizeof(unsigned int), stream);
+  if (s == 8)
+    k3_mass_differs<double><<<grid_for(ps->n), kBlock, 0, stream>>>(ps->mass.as<double>(), ps->n, ps->disp.as<unsigned int>());
+  else
+    k3_mass_differs<float><<<grid_for(ps->n), kBlock, 0, stream>>>(ps->mass.as<float>(), ps->n, ps->disp.as<unsigned int>());
+  unsigned int differs = 1;
+  if (e == hipSuccess) e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(&differs, ps->disp.ptr, sizeof(differs), hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  ps->uniform_mass = differs == 0;
+  if (e == hipSuccess) e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("periodic stepper init: %s", hipGetErrorString(e));
     tsph_psim_destroy(ps);
