@@ -56,6 +56,12 @@ template <> struct Sum<float> {
 TSPH_D double hi(const Sum<double>& a) { return a.s; }
 TSPH_D double hi(const Sum<float>& a) { return (double)a.s - (double)a.c; }
 
+// Four values of one particle that a pair sweep reads together, so that a
+// neighbour costs one 16-byte (FP32) or two 16-byte (FP64) gathers.
+template <typename S> struct alignas(4 * sizeof(S)) Vec4 {
+  S x, y, z, w;
+};
+
 template <int D, typename S> struct VSum {
   Sum<S> v[D];
   TSPH_D void add(const S* x) {

@@ -195,9 +195,11 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
   const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
   if (write_eos) {
     const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
+    const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
     a.p[i] = p;
-    a.pr2[i] = p / (rho * rho);
-    a.vol[i] = a.mass[i] / rho;
+    a.pr2[i] = pr2;
+    a.vol[i] = vol;
+    if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
   }
 }
 
@@ -243,10 +245,12 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NB nb
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = (S)rho_d;
   const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
+  const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
   a.rho[i] = rho;
   a.p[i] = p;
-  a.pr2[i] = p / (rho * rho);
-  a.vol[i] = a.mass[i] / rho;
+  a.pr2[i] = pr2;
+  a.vol[i] = vol;
+  if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
 }
 
 template <int D, typename S, typename NB>
@@ -324,10 +328,12 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NB nbr, Si
   pw = pw > S(0) ? pw : S(0);
   S rwv = a.rho0 * pow_s<S>(pw / a.bulk + S(1), a.inv_gamma);
   rwv = rwv > a.rho0 ? rwv : a.rho0;
+  const S pr2 = pw / (rwv * rwv), vol = a.mass[iw] / rwv;
   a.p[iw] = pw;
   a.rho[iw] = rwv;
-  a.pr2[iw] = pw / (rwv * rwv);
-  a.vol[iw] = a.mass[iw] / rwv;
+  a.pr2[iw] = pr2;
+  a.vol[iw] = vol;
+  if (a.aux) a.aux[iw] = Vec4<S>{a.mass[iw], pr2, vol, S(0)};
 }
 
 template <typename T> TSPH_D T wave_max(T v) {
@@ -377,7 +383,10 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
       keep = keep && !(r2 > a.skip2);
       const S g = cubic_g_r2<S>(r2, a.k);
-      const S cp = -a.mass[j] * (pri + a.pr2[j]);
+      Vec4<S> q;                                  // m, p / rho^2, m / rho
+      if constexpr (sizeof(S) == 4) q = a.aux[j];
+      else q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
+      const S cp = -q.x * (pri + q.y);
 #pragma unroll
       for (int d = 0; d < D; ++d) ap[d].add_if(keep, cp * (df[d] * g));
       const S blend = j < a.n ? S(1) : a.friction;
@@ -385,7 +394,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
 #pragma unroll
       for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
       const S relu = -dg > S(0) ? -dg : S(0);
-      const S cv = -(a.nu2 * blend) * (a.vol[j] * quot<S>(relu, r2 + a.eta2));
+      const S cv = -(a.nu2 * blend) * (q.z * quot<S>(relu, r2 + a.eta2));
       const bool visc = keep && blend != S(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d].add_if(visc, cv * (S)(vi[d] - a.vel[j * D + d]));
@@ -696,6 +705,8 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   a.mass = sim->mass.as<S>();
   a.wall_accel = nullptr;
   a.rho = sim->rho.as<S>(); a.p = sim->p.as<S>(); a.pr2 = sim->pr2.as<S>(); a.vol = sim->vol.as<S>();
+  // FP32: one 16-byte record per neighbour; FP64 reads the three arrays
+  a.aux = sizeof(S) == 4 ? sim->aux.as<Vec4<S>>() : nullptr;
   return a;
 }
 
@@ -705,7 +716,8 @@ template <int D, typename S>
 int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
   const int slot = (int)(k & 1);
   SimFlags* fl = sim->flags.as<SimFlags>();
-  const WcsphArgs<S> a = sim_args<D, S>(sim);
+  WcsphArgs<S> a = sim_args<D, S>(sim);
+  a.aux = nullptr;                       // the tile force sweep reads the arrays
   const NbSlots nbr = sim_slots(sim);
   const int r = (int)(k % kRing);
   k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
@@ -940,6 +952,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->mass, e * nt); ENS(sim->mass2, e * nt);
   ENS(sim->orig, 8 * nt); ENS(sim->orig2, 8 * nt);
   ENS(sim->rho, e * nt); ENS(sim->p, e * nt); ENS(sim->pr2, e * nt); ENS(sim->vol, e * nt);
+  if (e == 4) ENS(sim->aux, 4 * e * nt);
   ENS(sim->rho_raw, e * nt); ENS(sim->rho_lo, e * nt); ENS(sim->grad, e * n * D); ENS(sim->acc, e * n * D);
   ENS(sim->anchor, 8 * n * D);
   ENS(sim->flags, sizeof(SimFlags));
@@ -1111,7 +1124,7 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
 int tsph_sim_destroy(tsph_sim* sim) {
   if (!sim) return TSPH_OK;
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
-                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->rho_raw, &sim->rho_lo,
+                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->ranges,

@@ -49,6 +49,7 @@ template <typename S> struct ThreedArgs {
   S* gr;        // renormalised density gradient (Antuono diffusion)
   S* rate_out;  // optional: the diffusion rate of k3_eos
   S* rho_lo;    // low part of the summed density (FP32 mode), NULL in FP64
+  Vec4<S>* pack;  // optional: (m, rho_u, p / rho_u^2, K / 3) per row for the force sweep
 };
 
 // x_i - x_j from FP64 coordinates, minimum image if asked, rounded to S.
@@ -177,9 +178,11 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S
     rho_d = rho_d > fl ? rho_d : fl;
   }
   const S rho = (S)rho_d;
+  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, S(-1));
   a.rho[i] = rho;
   a.rho_u[i] = rho;
-  a.p[i] = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, S(-1));
+  a.p[i] = p;
+  if (a.pack) a.pack[i] = Vec4<S>{a.mass[i], rho, quot<S>(p, rho * rho), S(0)};
 }
 
 // Adami wall pressure (wall_pressure_adami, operators3.py:173-209): raw
@@ -211,8 +214,10 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
   const S dd = dn > S(1e-30) ? dn : S(1e-30);
   S pw = dn > S(0) ? num.get() / dd : S(0);
   pw = pw > S(0) ? pw : S(0);
+  const S rw = a.rho0 * pow_s<S>(S(1) + pw / a.bulk, a.inv_gamma);
   a.p[w] = pw;
-  a.rho_u[w] = a.rho0 * pow_s<S>(S(1) + pw / a.bulk, a.inv_gamma);
+  a.rho_u[w] = rw;
+  if (a.pack) a.pack[w] = Vec4<S>{a.mass[w], rw, quot<S>(pw, rw * rw), S(0)};
 }
 
 // second moment of the Laplacian (viscous_acceleration, operators3.py:137-145)
@@ -233,7 +238,9 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
     const S rij = S(0.5) * (ri + a.rho_u[j]);
     acc.add_if(keep, r2 * lap * quot<S>(a.mass[j], rij));
   });
-  a.moment[i] = acc.get() / S(3);
+  const S K = acc.get() / S(3);
+  a.moment[i] = K;
+  if (a.pack) a.pack[i] = Vec4<S>{a.mass[i], ri, quot<S>(a.p[i], ri * ri), K};
 }
 
 // Pressure acceleration (operators3.py:96-113) + renormalised viscous
@@ -243,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
 // with the higher index (in the caller's order), so that order matters: MW 0
 // takes the row order, MW 1 maps rows through ``orig``, MW 2 is for equal
 // masses, where either member's mass is the same.
-template <typename S, bool KICK, typename NB, int MW>
+template <typename S, bool KICK, typename NB, int MW, bool PK>
 __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
@@ -269,25 +276,29 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
       keep = keep && !(r2 > a.skip2);
       S g, lap;
       cubic_gl_r2<S>(r2, a.k, g, lap);
-      const S rj = a.rho_u[j];
+      Vec4<S> q;                                  // m, rho_u, p / rho_u^2, K / 3 of j
+      if constexpr (PK) q = a.pack[j];
+      else q.y = a.rho_u[j];
+      const S rj = q.y;
       if constexpr (decltype(press)::value) {
-        const S sym = pri + quot<S>(a.p[j], rj * rj);
-        const S cp = -(a.mass[j] * sym);
+        const S sym = pri + (PK ? q.z : quot<S>(a.p[j], rj * rj));
+        const S cp = -((PK ? q.x : a.mass[j]) * sym);
 #pragma unroll
         for (int d = 0; d < 3; ++d) ap[d].add_if(keep, (df[d] * g) * cp);
       }
       if constexpr (decltype(vis)::value) {
         const S rij = S(0.5) * (ri + rj);
+        const S mj = PK ? q.x : a.mass[j];
         S mw;
         if constexpr (MW == 2) {
-          mw = a.mass[j];
+          mw = mj;
         } else {
           const int64_t oj = MW == 1 ? orig[j] : j;
-          mw = oj > oi ? a.mass[j] : mi;
+          mw = oj > oi ? mj : mi;
         }
         S wgt = quot<S>(mw * lap, rij);
         if (a.renormalise) {
-          S mk = S(0.5) * (Ki + a.moment[j]);
+          S mk = S(0.5) * (Ki + (PK ? q.w : a.moment[j]));
           mk = mk > S(1e-12) ? mk : S(1e-12);
           wgt = wgt * quot<S>(S(2), mk);
         }
@@ -360,6 +371,7 @@ ThreedArgs<S> make_threed_args(const ThreedPrm& q, int64_t n, int64_t nf) {
 // scratch of one evaluation, n + 1 elements each (gr: 3 (n + 1))
 template <typename S> struct ThreedScratch {
   S *rho_raw, *rho, *rho_u, *p, *moment, *gr, *rho_lo;   // rho_lo: NULL in FP64 mode
+  Vec4<S>* pack;
 };
 
 // Queues the sweeps of one accelerations() evaluation over a neighbour source
@@ -370,6 +382,7 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   const int64_t n = a.n;
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
+  a.pack = b.pack;
   const unsigned g = grid_for(n);
   const bool diff = q.diff_scale > 0.0;
   k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho);
@@ -386,11 +399,11 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, nb);
   if (a.nu != S(0) && a.renormalise) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
   if (uniform_mass)
-    k3_force<S, KICK, NB, 2><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    k3_force<S, KICK, NB, 2, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
   else if (orig)
-    k3_force<S, KICK, NB, 1><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    k3_force<S, KICK, NB, 1, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
   else
-    k3_force<S, KICK, NB, 0><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    k3_force<S, KICK, NB, 0, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }
@@ -405,7 +418,9 @@ int threed_ctx_scratch(tsph_ctx* ctx, int64_t n, ThreedScratch<S>& b) {
   TSPH_TRY(ctx->s_aux2.ensure(s));
   TSPH_TRY(ctx->s_gr.ensure(3 * s));
   TSPH_TRY(ctx->s_aux3.ensure(s));
+  TSPH_TRY(ctx->s_pack.ensure(4 * s));
   b.rho_lo = sizeof(S) == 4 ? ctx->s_aux3.as<S>() : nullptr;
+  b.pack = ctx->s_pack.as<Vec4<S>>();
   b.rho_raw = ctx->s_rho_raw.as<S>(); b.rho = ctx->s_rho.as<S>(); b.rho_u = ctx->s_aux.as<S>();
   b.p = ctx->s_p.as<S>(); b.moment = ctx->s_aux2.as<S>(); b.gr = ctx->s_gr.as<S>();
   return TSPH_OK;
@@ -450,12 +465,12 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
       break;
     case 1:
       a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0;
-      k3_force<S, false, NbList, 0><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
+      k3_force<S, false, NbList, 0, false><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 2:
       a.rho_u = rho_in; a.do_pressure = 0; a.has_gravity = 0;
       if (a.nu != S(0) && a.renormalise) k3_moment<S, NbList><<<g, kBlock, 0, stream>>>(a, nb);
-      k3_force<S, false, NbList, 0><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
+      k3_force<S, false, NbList, 0, false><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 3:
       if (nf == n) return TSPH_OK;
@@ -662,7 +677,7 @@ struct tsph_psim {
   tsph_ctx* ctx = nullptr;
   DevBuf pos, vel[2], mass, acc, orig, anchor, disp;
   DevBuf pos2, vel2, mass2, acc2, orig2, cells2, part, diag;
-  DevBuf rho_raw, rho, rho_u, p, moment, gr, rho_lo;
+  DevBuf rho_raw, rho, rho_u, p, moment, gr, rho_lo, pack;
   DevBuf cell_R, entries;                    // dense cell table; list entries at the build radius
   DevBuf slot_idx, slot_cnt, slot_max;       // per-row lists in fixed slots (NbSlots layout)
   int64_t slot_K = 0;
@@ -681,6 +696,7 @@ template <typename S> ThreedScratch<S> psim_scratch(tsph_psim* ps) {
   b.rho_raw = ps->rho_raw.as<S>(); b.rho = ps->rho.as<S>(); b.rho_u = ps->rho_u.as<S>();
   b.p = ps->p.as<S>(); b.moment = ps->moment.as<S>(); b.gr = ps->gr.as<S>();
   b.rho_lo = sizeof(S) == 4 ? ps->rho_lo.as<S>() : nullptr;
+  b.pack = ps->pack.as<Vec4<S>>();
   return b;
 }
 
@@ -869,7 +885,7 @@ int tsph_psim_destroy(tsph_psim* ps) {
   if (!ps) return TSPH_OK;
   DevBuf* b[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->mass, &ps->acc, &ps->orig, &ps->anchor, &ps->disp,
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
-                 &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo,
+                 &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo, &ps->pack,
                  &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
@@ -901,11 +917,11 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
   DevBuf* bufs[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->vel2, &ps->pos2, &ps->anchor, &ps->cells2,
                     &ps->mass, &ps->mass2, &ps->acc, &ps->acc2, &ps->orig, &ps->orig2, &ps->disp,
                     &ps->part, &ps->diag, &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr,
-                    &ps->rho_lo};
+                    &ps->rho_lo, &ps->pack};
   const size_t sizes[] = {24 * n, 24 * n, 24 * n, 24 * n, 24 * n, 24 * n, 24 * n,
                           s * (n + 1), s * (n + 1), 3 * s * (n + 1), 3 * s * (n + 1), 8 * n, 8 * n, 64,
                           8 * DG_N * ((size_t)grid_for(ps->n) + 1), 8 * DG_N, s * (n + 1), s * (n + 1), s * (n + 1),
-                          s * (n + 1), s * (n + 1), 3 * s * (n + 1), s * (n + 1)};
+                          s * (n + 1), s * (n + 1), 3 * s * (n + 1), s * (n + 1), 4 * s * (n + 1)};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]) && rc == TSPH_OK; ++k) rc = bufs[k]->ensure(sizes[k]);
   if (rc == TSPH_OK && hipHostMalloc((void**)&ps->disp_host, 8, 0) != hipSuccess) {
     ps->disp_host = nullptr;

@@ -26,6 +26,7 @@ template <typename S> struct WcsphArgs {
   S* pr2;             // p / rho^2
   S* vol;             // m / rho
   S* rho_lo;          // optional: low part of the stored density (FP32 mode)
+  Vec4<S>* aux;       // FP32 slot sweeps: (m, p / rho^2, m / rho, 0) per row for the force sweep
 };
 
 template <typename S>
