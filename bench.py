@@ -57,8 +57,8 @@ def parse():
     ap.add_argument("--fluid", type=float, default=1.6e6, help="target fluid particle count")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--diffusion", type=float, default=0.0,
-                    help="dam break with Molteni δ-diffusion of this δ (cell-walk sweeps; "
-                         "the LDS tiles carry no diffusion term)")
+                    help="dam break with Molteni δ-diffusion of this δ (slot-list sweeps; "
+                         "the LDS tile force sweep carries no diffusion term)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", default=None,
@@ -113,8 +113,9 @@ def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0):
 def time_gpu(sim, steps, warmup, tile=None):
     import torch
     from tensorsph_amd.autotune import tune_tiles
-    if tile == "walk":
-        tuned = {"best": "cell walk", "ms": {}}
+    if tile == "lists":
+        sim.configure(tiles=False)
+        tuned = {"best": "lists", "ms": {}}
         sim.step(max(warmup, 1))
     elif tile is not None:
         sim.configure(tiles=True, tile=tile)
@@ -439,7 +440,7 @@ def main():
 
     tile = None if args.tile is None else tuple(int(v) for v in args.tile.lower().split("x"))
     if args.diffusion > 0.0:
-        tile = "walk"
+        tile = "lists"
     sim = make_sim(cfg, pos, mass, wall, wmass, "fp32", args.diffusion)
     wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup, tile)
     pairs = sim.count_pairs(2.0 * h)
@@ -491,7 +492,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if args.diffusion else profiled_traffic(nf),
                      "traffic_source": None if args.diffusion else os.path.relpath(PROFILE, HERE),
-                     "kernel": ("k_step_force (cell walk: force + viscosity + integrate)" if args.diffusion
+                     "kernel": ("k_step_force (slot lists: force + viscosity + integrate)" if args.diffusion
                                 else "k_tile_force (force + viscosity + integrate)"),
                      "algorithmic_bytes_per_launch": b_force},
     }

@@ -336,7 +336,7 @@ int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_
 int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc,
                   hipStream_t stream);
 /* counters: steps done, list builds (cell binnings), boundary crossings,
- * cells of the dense cell table the sweeps walk, meter rows; peaks: max
+ * cells of the dense cell table the list builds walk, meter rows; peaks: max
  * density and max speed over all steps.  Host pointers. */
 int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
 /* Meter rows (host): step, sums[mass, momentum x/y/z, kinetic energy, sum rho,

@@ -10,6 +10,7 @@ part of the GPU particle pipeline and raise NotImplementedError.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -18,7 +19,8 @@ import torch
 from .cases import make_column, wall_lattice, lattice_mass_unit
 from .config import DamBreakConfig, check_supported
 
-__all__ = ["DamBreakConfig", "DamBreakRun", "validate_case", "front_position", "run_case"]
+__all__ = ["DamBreakConfig", "DamBreakRun", "validate_case", "front_position", "ritter_front",
+           "ritter_height_profile", "ritter_front_over_a", "run_case"]
 
 
 @dataclass
@@ -47,6 +49,19 @@ class DamBreakRun:
     max_tvc_clipped: float = 0.0
     snapshots: list = field(default_factory=list)
 
+    @property
+    def mass_drift(self) -> float:
+        """Final minus initial total mass (dam_break.py:271-279): particle masses
+        are constants, so this is zero, measured rather than assumed."""
+        return self.final_mass - self.initial_mass
+
+    def dimensionless(self, cfg: DamBreakConfig):
+        """(tau, Z/a) samples with tau = t sqrt(2 g / a), a the column width
+        (dam_break.py:281-284)."""
+        a = cfg.column_width
+        rate = math.sqrt(2.0 * cfg.gravity / a)
+        return [t * rate for t in self.times], [z / a for z in self.front]
+
 
 def validate_case(cfg: DamBreakConfig) -> None:
     """dam_break.validate_case (dam_break.py:123-140)."""
@@ -72,6 +87,34 @@ def front_position(pos, cfg: DamBreakConfig) -> float:
     if pos.numel() == 0:
         return 0.0
     return float(pos[:, 0].max())
+
+
+def ritter_front(time: float, cfg: DamBreakConfig) -> float:
+    """Ritter's dry-bed front distance from the column face, 2 sqrt(g H) t, with
+    H the column height (dam_break.py:179-196).  Compare it with
+    front_position - column_width."""
+    return 2.0 * math.sqrt(cfg.gravity * cfg.column_height) * time
+
+
+def ritter_height_profile(x: float, time: float, cfg: DamBreakConfig) -> float | None:
+    """Ritter's free surface h(x, t), x measured from the column face
+    (dam_break.py:199-219): H behind the rarefaction (x < -c t), the fan
+    (2c - x/t)^2 / (9 g) for -c t <= x <= 2 c t, dry beyond; c = sqrt(g H).
+    None before release (t <= 0)."""
+    if time <= 0.0:
+        return None
+    c = math.sqrt(cfg.gravity * cfg.column_height)
+    if x < -c * time:
+        return cfg.column_height
+    if x > 2.0 * c * time:
+        return 0.0
+    return (2.0 * c - x / time) ** 2 / (9.0 * cfg.gravity)
+
+
+def ritter_front_over_a(tau: float) -> float:
+    """The Ritter front in Z/a against tau = t sqrt(2 g / a) for H = 2a: 2 tau
+    (dam_break.py:222-230)."""
+    return 2.0 * tau
 
 
 def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapshot_times=None,

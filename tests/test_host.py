@@ -1,0 +1,84 @@
+"""Host-side helpers of the drivers (no GPU): the Ritter dam-break reference
+(dam_break.py:179-230) and the DamBreakRun derived values (dam_break.py:
+271-284).
+
+The Ritter checks are the solution's own landmarks, as the reference's
+tests/test_ritter_reference.py pins them: front at 2 sqrt(gH) t, depth H at
+the rarefaction tail, 4H/9 at the dam and 0 at the front, the Riemann
+invariant u + 2 sqrt(g h) = 2 sqrt(g H) across the fan, self-similarity, the
+water area ahead of the dam (4/27) H x_front, and Z/a = 2 tau for H = 2a.
+"""
+import math
+
+import pytest
+import torch
+
+from tensorsph_amd.dam_break import (DamBreakConfig, DamBreakRun, ritter_front, ritter_front_over_a,
+                                     ritter_height_profile)
+
+G, A = 9.81, 0.2
+H = 2.0 * A
+
+
+@pytest.fixture(scope="module")
+def cfg():
+    return DamBreakConfig(dx=0.02, column_width=A, column_height=H, pool_width=2.0, gravity=G)
+
+
+def test_front_travels_at_twice_the_wave_speed(cfg):
+    c = math.sqrt(G * H)
+    for t in (0.05, 0.1, 0.25, 0.5):
+        assert ritter_front(t, cfg) == pytest.approx(2.0 * c * t, rel=1e-15)
+        assert ritter_height_profile(ritter_front(t, cfg), t, cfg) == pytest.approx(0.0, abs=1e-15)
+
+
+def test_landmarks_and_riemann_invariant(cfg):
+    c = math.sqrt(G * H)
+    t = 0.3
+    assert ritter_height_profile(-c * t, t, cfg) == pytest.approx(H, rel=1e-14)
+    assert ritter_height_profile(0.0, t, cfg) == pytest.approx(4.0 * H / 9.0, rel=1e-14)
+    assert ritter_height_profile(-2.0 * c * t, t, cfg) == H          # undisturbed reservoir
+    assert ritter_height_profile(3.0 * c * t, t, cfg) == 0.0          # dry bed
+    assert ritter_height_profile(0.1, 0.0, cfg) is None               # before release
+    for k in range(11):
+        x = -c * t + k * (3.0 * c * t) / 10.0
+        h = ritter_height_profile(x, t, cfg)
+        u = (2.0 / 3.0) * (c + x / t)
+        assert u + 2.0 * math.sqrt(G * h) == pytest.approx(2.0 * c, rel=1e-12)
+
+
+def test_self_similar_and_area_ahead_of_dam(cfg):
+    c = math.sqrt(G * H)
+    for lam in (0.5, 2.0, 3.7):
+        for x, t in ((0.05, 0.1), (-0.1, 0.2), (0.3, 0.25)):
+            assert ritter_height_profile(lam * x, lam * t, cfg) == pytest.approx(
+                ritter_height_profile(x, t, cfg), rel=1e-13)
+    t = 0.2
+    front = 2.0 * c * t
+    n = 4000                                   # Simpson over [0, front]
+    hstep = front / n
+    area = ritter_height_profile(0.0, t, cfg) + ritter_height_profile(front, t, cfg)
+    for k in range(1, n):
+        area += (4 if k % 2 else 2) * ritter_height_profile(k * hstep, t, cfg)
+    area *= hstep / 3.0
+    assert area == pytest.approx(4.0 / 27.0 * H * front, rel=1e-12)
+
+
+def test_z_over_a_is_two_tau(cfg):
+    for tau in (0.5, 1.0, 2.5):
+        t = tau * math.sqrt(A / (2.0 * G))
+        assert ritter_front(t, cfg) / A == pytest.approx(ritter_front_over_a(tau), rel=1e-14)
+        assert ritter_front_over_a(tau) == 2.0 * tau
+
+
+def test_run_record_derived_values(cfg):
+    run = DamBreakRun(times=[0.0, 0.1, 0.2], front=[A, 0.3, 0.5], kinetic_energy=[0.0] * 3,
+                      potential_energy=[0.0] * 3, mass=4.0, escaped=0, rebuilds=1, max_speed=0.0,
+                      max_density_compression=0.0, compression_before_wall_impact=0.0,
+                      wall_impact_time=None, initial_mass=4.0, final_mass=4.0,
+                      final_positions=torch.zeros((0, 2)))
+    assert run.mass_drift == 0.0
+    tau, za = run.dimensionless(cfg)
+    rate = math.sqrt(2.0 * G / A)
+    assert tau == pytest.approx([0.0, 0.1 * rate, 0.2 * rate], rel=1e-15)
+    assert za == pytest.approx([1.0, 1.5, 2.5], rel=1e-15)
