@@ -212,46 +212,53 @@ struct NbCells {
   }
 };
 
-// Per-row lists in fixed slots interleaved by wave: entry k of row r sits at
-// (r / 64) * 64 * K + 64 k + r % 64, so the 64 rows of a wave read their k-th
-// entries as one coalesced 256-byte run.  Filled from NbCells in its visiting
-// order with the build radius test (k_slot_lists), so a sweep over the slots
-// sums the same terms in the same order as over the walk.  Fluid targets list
-// fluid and wall rows, wall targets fluid rows only; with_walls == false
+// Per-row lists in fixed slots interleaved by wave, four entries per lane: the
+// K slots of a row (K a multiple of 4) form K / 4 chunks of four int32, and
+// chunk c of row r sits at int4 index (r / 64) * 16 * K + 64 c + r % 64.  The
+// 64 rows of a wave read their c-th chunks (entries 4c .. 4c + 3) as one
+// coalesced 1 KB run, one 16-byte load per lane.  Filled in the cell walk's
+// visiting order with the build radius test (k_slot_lists).  Fluid targets
+// list fluid and wall rows, wall targets fluid rows only; with_walls == false
 // skips the wall rows (index >= nf) of a fluid target's list.
 struct NbSlots {
   const int32_t* idx;
   const int32_t* cnt;
   int64_t K, nf;
+  TSPH_D const int4* chunks(int64_t i) const {
+    return reinterpret_cast<const int4*>(idx) + (i >> 6) * 16 * K + (i & 63);
+  }
   template <int D, typename F>
   TSPH_D void visit(int64_t i, bool with_walls, F&& f) const {
-    const int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
+    const int4* p = chunks(i);
     const int n = cnt[i];
-    for (int k = 0; k < n; ++k) {
-      const int64_t j = p[64 * k];
-      if (!with_walls && j >= nf) continue;
-      f(j);
+    for (int k = 0; k < n; k += 4) {
+      const int4 q = p[(k >> 2) * 64];
+      const int32_t j4[4] = {q.x, q.y, q.z, q.w};
+      for (int u = 0; u < 4 && k + u < n; ++u) {
+        const int64_t j = j4[u];
+        if (!with_walls && j >= nf) continue;
+        f(j);
+      }
     }
   }
-  // f(j, keep) for every entry, U at a time in one straight-line block: the
-  // U index loads and the gathers f makes for them are all issued before the
-  // arithmetic that needs them.  keep is false for the wall rows that
-  // with_walls == false skips; f must not branch on it.
-  template <int D, int U = 4, typename F>
+  // f(j, keep) for every entry, one chunk of four at a time in one
+  // straight-line block: the chunk load and the gathers f makes for its four
+  // entries are all issued before the arithmetic that needs them.  Slots past
+  // the row's count run f on the row itself with keep false, as do the wall
+  // rows that with_walls == false skips; f must not branch on keep.
+  template <int D, typename F>
   TSPH_D void visit_masked(int64_t i, bool with_walls, F&& f) const {
-    const int32_t* p = idx + (i >> 6) * 64 * K + (i & 63);
+    const int4* p = chunks(i);
     const int n = cnt[i];
-    int k = 0;
-    for (; k + U <= n; k += U) {
-      int32_t j[U];
+    for (int k = 0; k < n; k += 4) {
+      const int4 q = p[(k >> 2) * 64];
+      const int32_t j4[4] = {q.x, q.y, q.z, q.w};
 #pragma unroll
-      for (int u = 0; u < U; ++u) j[u] = p[64 * (k + u)];
-#pragma unroll
-      for (int u = 0; u < U; ++u) f((int64_t)j[u], with_walls || j[u] < nf);
-    }
-    for (; k < n; ++k) {
-      const int32_t j = p[64 * k];
-      f((int64_t)j, with_walls || j < nf);
+      for (int u = 0; u < 4; ++u) {
+        const bool in = k + u < n;
+        const int64_t j = in ? (int64_t)j4[u] : i;
+        f(j, in && (with_walls || j < nf));
+      }
     }
   }
 };
@@ -290,8 +297,10 @@ __global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int6
 // entry count, their maximum (kmax) and, if total is given, their sum; with
 // FILL the entries go into K slots per row (NbSlots layout; a row with more
 // than K entries keeps its first K and its full count, so the caller widens
-// the slots and runs the pass again).  Candidates are fetched four at a time
-// ahead of their tests; launched with grid_xcd.
+// the slots and runs the pass again).  Each lane gathers four entries in
+// registers and stores them as one 16-byte chunk; the slots of a row's last
+// chunk past its count hold the row itself.  Candidates are fetched four at a
+// time ahead of their tests; launched with grid_xcd.
 template <int D, bool FILL, bool PER>
 __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos, int64_t nf,
                                                        int64_t nt, double rc2, double b0, double b1, double b2,
@@ -305,7 +314,9 @@ __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const doubl
     double xi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = pos[i * D + d];
-    int32_t* p = FILL ? idx + (i >> 6) * 64 * K + (i & 63) : nullptr;
+    int4* p = FILL ? reinterpret_cast<int4*>(idx) + (i >> 6) * 16 * K + (i & 63) : nullptr;
+    const int chunks = (int)(K >> 2);
+    int4 buf = make_int4((int)i, (int)i, (int)i, (int)i);
     walk.ranges<D>(i, i < nf, [&](int32_t b, int32_t e) {
       for (int32_t j0 = b; j0 < e; j0 += 4) {
         double xj[4][D];
@@ -320,11 +331,25 @@ __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const doubl
           const int32_t j = j0 + u;
           if (j >= e || j == i) continue;
           if (dist2_pt<double, D, PER>(xi, xj[u], box) > rc2) continue;
-          if (FILL && k < K) p[64 * k] = j;
+          if (FILL) {
+            const int s = k & 3;
+            buf.x = s == 0 ? j : buf.x;
+            buf.y = s == 1 ? j : buf.y;
+            buf.z = s == 2 ? j : buf.z;
+            buf.w = s == 3 ? j : buf.w;
+            if (s == 3 && (k >> 2) < chunks) p[(k >> 2) * 64] = buf;
+          }
           ++k;
         }
       }
     });
+    if (FILL && (k & 3) && (k >> 2) < chunks) {
+      const int s = k & 3;                      // the row itself in the unused slots
+      buf.y = s > 1 ? buf.y : (int)i;
+      buf.z = s > 2 ? buf.z : (int)i;
+      buf.w = (int)i;
+      p[(k >> 2) * 64] = buf;
+    }
     cnt[i] = k;
   }
   int m = k;
