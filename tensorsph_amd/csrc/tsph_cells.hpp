@@ -1,6 +1,8 @@
 // Device side of the linked-cell search: cell lookup and the reference-order
 // neighbour visit, shared by the pair export and the resident stepper.
 #pragma once
+#include <math.h>
+
 #include "tsph_common.hpp"
 #include "tsph_internal.hpp"
 
@@ -292,6 +294,29 @@ __global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int6
   if (last) cell[wall ? 3 : 1] = (int)(r + 1);
 }
 
+// FP32 copies of the build-time positions relative to row 0, for the list
+// build's prefilter: float2 rows in 2-D, float4 (w unused) in 3-D.
+template <int D> struct PosF;
+template <> struct PosF<2> {
+  using T = float2;
+  static TSPH_D float at(const T& v, int d) { return d == 0 ? v.x : v.y; }
+};
+template <> struct PosF<3> {
+  using T = float4;
+  static TSPH_D float at(const T& v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+};
+
+template <int D>
+__global__ void k_pos_f32(const double* __restrict__ pos, int64_t nt, typename PosF<D>::T* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nt) return;
+  float v[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int d = 0; d < D; ++d) v[d] = (float)(pos[i * D + d] - pos[d]);
+  if constexpr (D == 2) out[i] = make_float2(v[0], v[1]);
+  else out[i] = make_float4(v[0], v[1], v[2], 0.f);
+}
+
 // Per-row lists at the build radius (cell walk, the unfused distance test of
 // the pair search, minimum image when PER).  Every pass writes each row's
 // entry count, their maximum (kmax) and, if total is given, their sum; with
@@ -301,45 +326,86 @@ __global__ void k_cell_table(const int64_t* __restrict__ cells, int64_t nt, int6
 // registers and stores them as one 16-byte chunk; the slots of a row's last
 // chunk past its count hold the row itself.  Candidates are fetched four at a
 // time ahead of their tests; launched with grid_xcd.
-template <int D, bool FILL, bool PER>
-__global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos, int64_t nf,
-                                                       int64_t nt, double rc2, double b0, double b1, double b2,
-                                                       int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
-                                                       int64_t K, unsigned int* __restrict__ kmax,
+//
+// With PF the candidates are first tested in FP32 on posf: a squared distance
+// below lo2 accepts, above hi2 rejects, and only the band between them, which
+// holds the FP32 rounding error of the test, is decided by the FP64 test.
+template <int D, bool FILL, bool PER, bool PF>
+__global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos,
+                                                       const typename PosF<D>::T* __restrict__ posf, int64_t nf,
+                                                       int64_t nt, double rc2, float lo2, float hi2, double b0,
+                                                       double b1, double b2, int32_t* __restrict__ cnt,
+                                                       int32_t* __restrict__ idx, int64_t K,
+                                                       unsigned int* __restrict__ kmax,
                                                        unsigned long long* __restrict__ total) {
+  using VF = typename PosF<D>::T;
   const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < nt) {
     const double box[3] = {b0, b1, b2};
+    const float boxf[3] = {(float)b0, (float)b1, (float)b2};
+    const float ibf[3] = {PER ? 1.0f / (float)b0 : 0.f, PER ? 1.0f / (float)b1 : 0.f,
+                          PER && D > 2 ? 1.0f / (float)b2 : 0.f};
     double xi[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) xi[d] = pos[i * D + d];
+    VF xfi{};
+    if constexpr (PF) xfi = posf[i];
     int4* p = FILL ? reinterpret_cast<int4*>(idx) + (i >> 6) * 16 * K + (i & 63) : nullptr;
     const int chunks = (int)(K >> 2);
     int4 buf = make_int4((int)i, (int)i, (int)i, (int)i);
+    auto accept = [&](int32_t j) {
+      if (FILL) {
+        const int s = k & 3;
+        buf.x = s == 0 ? j : buf.x;
+        buf.y = s == 1 ? j : buf.y;
+        buf.z = s == 2 ? j : buf.z;
+        buf.w = s == 3 ? j : buf.w;
+        if (s == 3 && (k >> 2) < chunks) p[(k >> 2) * 64] = buf;
+      }
+      ++k;
+    };
     walk.ranges<D>(i, i < nf, [&](int32_t b, int32_t e) {
       for (int32_t j0 = b; j0 < e; j0 += 4) {
-        double xj[4][D];
+        if constexpr (PF) {
+          VF c[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int64_t j = j0 + u < e ? j0 + u : e - 1;   // in range: e > j0 >= b
+          for (int u = 0; u < 4; ++u) c[u] = posf[j0 + u < e ? j0 + u : e - 1];   // in range: e > j0 >= b
 #pragma unroll
-          for (int d = 0; d < D; ++d) xj[u][d] = pos[j * D + d];
-        }
+          for (int u = 0; u < 4; ++u) {
+            const int32_t j = j0 + u;
+            if (j >= e || j == i) continue;
+            float d2 = 0.f;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int32_t j = j0 + u;
-          if (j >= e || j == i) continue;
-          if (dist2_pt<double, D, PER>(xi, xj[u], box) > rc2) continue;
-          if (FILL) {
-            const int s = k & 3;
-            buf.x = s == 0 ? j : buf.x;
-            buf.y = s == 1 ? j : buf.y;
-            buf.z = s == 2 ? j : buf.z;
-            buf.w = s == 3 ? j : buf.w;
-            if (s == 3 && (k >> 2) < chunks) p[(k >> 2) * 64] = buf;
+            for (int d = 0; d < D; ++d) {
+              float df = PosF<D>::at(xfi, d) - PosF<D>::at(c[u], d);
+              if (PER) df = df - boxf[d] * rintf(df * ibf[d]);
+              d2 += df * df;
+            }
+            bool in = d2 < lo2;
+            if (!in && d2 <= hi2) {
+              double xj[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) xj[d] = pos[(int64_t)j * D + d];
+              in = dist2_pt<double, D, PER>(xi, xj, box) <= rc2;
+            }
+            if (in) accept(j);
           }
-          ++k;
+        } else {
+          double xj[4][D];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t j = j0 + u < e ? j0 + u : e - 1;   // in range: e > j0 >= b
+#pragma unroll
+            for (int d = 0; d < D; ++d) xj[u][d] = pos[j * D + d];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int32_t j = j0 + u;
+            if (j >= e || j == i) continue;
+            if (dist2_pt<double, D, PER>(xi, xj[u], box) > rc2) continue;
+            accept(j);
+          }
         }
       }
     });
@@ -370,25 +436,53 @@ __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const doubl
 // slots are widened to its length + 1/8 + 4 (a multiple of 4) and the pass
 // runs again.  total, if given, receives the entry count.  Reads the longest
 // list back, so the stream is synchronised.
+//
+// posf (scratch) enables the FP32 prefilter.  extent bounds |x - x_0| over the
+// rows in every axis (x_0 = row 0).  Each FP32 coordinate then carries at most
+// e0 = 2^-23 extent of rounding, a candidate difference at most
+// e = 2 e0 + 2^-24 1.01 rc (+ 2^-22 L for the periodic image), and the squared
+// distance at most 2 sqrt(D) rc e + D e^2 + 4.4 2^-24 rc^2 near rc^2.  Twice
+// that is the undecided band; wider than 1 % of rc^2, the build runs in FP64.
 template <int D, bool PER>
 int build_slot_lists(const NbCells& walk, const double* pos, int64_t nf, int64_t nt, double rc2,
                      const double* box, DevBuf& cnt, DevBuf& idx, DevBuf& kmax, int64_t& K,
-                     unsigned long long* total, hipStream_t stream) {
+                     unsigned long long* total, hipStream_t stream, DevBuf* posf = nullptr,
+                     double extent = 0.0) {
   TSPH_TRY(cnt.ensure(sizeof(int32_t) * (size_t)(nt > 0 ? nt : 1)));
   TSPH_TRY(kmax.ensure(sizeof(unsigned int)));
   const unsigned g = grid_xcd(nt);
   const double b0 = PER ? box[0] : 0.0, b1 = PER ? box[1] : 0.0, b2 = PER && D > 2 ? box[2] : 0.0;
+  const double rc = sqrt(rc2);
+  double e = 2.0 * ldexp(extent, -23) + ldexp(1.01 * rc, -24);
+  if (PER) e += ldexp(fmax(fmax(b0, b1), b2), -22);
+  const double band = 2.0 * (2.0 * sqrt((double)D) * rc * e + D * e * e + 4.4 * ldexp(rc2, -24));
+  const bool pf = posf && extent > 0.0 && band < 0.01 * rc2;
+  float lo2 = 0.f, hi2 = 0.f;
+  using VF = typename PosF<D>::T;
+  if (pf) {
+    lo2 = nextafterf((float)(rc2 - band), 0.0f);
+    hi2 = nextafterf((float)(rc2 + band), INFINITY);
+    TSPH_TRY(posf->ensure(sizeof(VF) * (size_t)(nt > 0 ? nt : 1)));
+    k_pos_f32<D><<<grid_for(nt), kBlock, 0, stream>>>(pos, nt, posf->as<VF>());
+    TSPH_CHECK_LAUNCH();
+  }
+  const VF* pf_ptr = pf ? posf->as<VF>() : nullptr;
   for (int pass = 0; pass < 3; ++pass) {
     TSPH_CHECK_HIP(hipMemsetAsync(kmax.ptr, 0, sizeof(unsigned int), stream));
     if (total) TSPH_CHECK_HIP(hipMemsetAsync(total, 0, sizeof(unsigned long long), stream));
+    int32_t* out = nullptr;
     if (K > 0) {
       TSPH_TRY(idx.ensure(sizeof(int32_t) * (size_t)((nt + 63) / 64) * 64 * (size_t)K));
-      k_slot_lists<D, true, PER><<<g, kBlock, 0, stream>>>(walk, pos, nf, nt, rc2, b0, b1, b2, cnt.as<int32_t>(),
-                                                            idx.as<int32_t>(), K, kmax.as<unsigned int>(), total);
-    } else {
-      k_slot_lists<D, false, PER><<<g, kBlock, 0, stream>>>(walk, pos, nf, nt, rc2, b0, b1, b2, cnt.as<int32_t>(),
-                                                             nullptr, 0, kmax.as<unsigned int>(), total);
+      out = idx.as<int32_t>();
     }
+#define TSPH_SLOT_PASS(FILL, PFV)                                                                     \
+  k_slot_lists<D, FILL, PER, PFV><<<g, kBlock, 0, stream>>>(walk, pos, pf_ptr, nf, nt, rc2, lo2, hi2, b0, b1, b2, \
+                                                            cnt.as<int32_t>(), out, K, kmax.as<unsigned int>(), total)
+    if (K > 0 && pf) TSPH_SLOT_PASS(true, true);
+    else if (K > 0) TSPH_SLOT_PASS(true, false);
+    else if (pf) TSPH_SLOT_PASS(false, true);
+    else TSPH_SLOT_PASS(false, false);
+#undef TSPH_SLOT_PASS
     TSPH_CHECK_LAUNCH();
     unsigned int longest = 0;
     TSPH_CHECK_HIP(hipMemcpyAsync(&longest, kmax.ptr, sizeof(longest), hipMemcpyDeviceToHost, stream));

@@ -640,8 +640,11 @@ int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t*
                                                        sim->cell_stride[1], sim->cell_stride[2],
                                                        sim->cell_R.as<int4>());
   TSPH_CHECK_LAUNCH();
+  // every row lies within ext[d] cells of edge rc of row 0
+  double extent = 0.0;
+  for (int d = 0; d < D; ++d) extent = fmax(extent, (double)sim->cell_ext[d] * rc);
   return build_slot_lists<D, false>(sim_cells(sim), npos, n, nt, rc * rc, nullptr, sim->slot_cnt, sim->slot_idx,
-                                    sim->slot_max, sim->slot_K, nullptr, stream);
+                                    sim->slot_max, sim->slot_K, nullptr, stream, &sim->slot_posf, extent);
 }
 template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*, const int64_t*, hipStream_t);
 
@@ -1127,7 +1130,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
-                    &sim->slot_cnt, &sim->slot_max, &sim->ranges,
+                    &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
                     &sim->load_part, &sim->loads_out, &sim->clamp_acc};
   for (DevBuf* b : bufs) b->release();

@@ -53,6 +53,7 @@ struct tsph_sim {
   tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   tsph::DevBuf slot_idx, slot_cnt, slot_max;   // per-row lists in fixed slots (NbSlots)
+  tsph::DevBuf slot_posf;                      // FP32 build positions (list prefilter)
   int64_t slot_K = 0;
   tsph::StepRecord* ring = nullptr;       // pinned
   hipEvent_t ev[tsph::kRing] = {};

@@ -680,6 +680,7 @@ struct tsph_psim {
   DevBuf rho_raw, rho, rho_u, p, moment, gr, rho_lo, pack;
   DevBuf cell_R, entries;                    // dense cell table; list entries at the build radius
   DevBuf slot_idx, slot_cnt, slot_max;       // per-row lists in fixed slots (NbSlots layout)
+  DevBuf slot_posf;                          // FP32 build positions (list prefilter)
   int64_t slot_K = 0;
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
@@ -758,7 +759,8 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   // entry count is the periodic_pairs entry count
   TSPH_TRY((build_slot_lists<3, true>(psim_cells(ps), ps->pos.as<double>(), n, n, rc * rc, ps->prm.box,
                                       ps->slot_cnt, ps->slot_idx, ps->slot_max, ps->slot_K,
-                                      ps->entries.as<unsigned long long>(), stream)));
+                                      ps->entries.as<unsigned long long>(), stream, &ps->slot_posf,
+                                      fmax(fmax(ps->prm.box[0], ps->prm.box[1]), ps->prm.box[2]))));
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
   ps->rebuilds++;
   ps->built = true;
@@ -886,7 +888,7 @@ int tsph_psim_destroy(tsph_psim* ps) {
   DevBuf* b[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->mass, &ps->acc, &ps->orig, &ps->anchor, &ps->disp,
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
                  &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo, &ps->pack,
-                 &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max};
+                 &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max, &ps->slot_posf};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);

@@ -82,3 +82,57 @@ def test_run_record_derived_values(cfg):
     rate = math.sqrt(2.0 * G / A)
     assert tau == pytest.approx([0.0, 0.1 * rate, 0.2 * rate], rel=1e-15)
     assert za == pytest.approx([1.0, 1.5, 2.5], rel=1e-15)
+
+
+def _prefilter_band(rc2, dim, extent, box=None):
+    """The undecided band of the FP32 list prefilter (build_slot_lists in
+    csrc/tsph_cells.hpp): twice the bound on |d2_fp32 - d2|."""
+    rc = math.sqrt(rc2)
+    e = 2.0 * math.ldexp(extent, -23) + math.ldexp(1.01 * rc, -24)
+    if box is not None:
+        e += math.ldexp(box, -22)
+    return 2.0 * (2.0 * math.sqrt(dim) * rc * e + dim * e * e + 4.4 * math.ldexp(rc2, -24))
+
+
+@pytest.mark.parametrize("dim,extent,rc,box", [(3, 3.22, 0.015, None), (2, 4.0, 0.0033, None),
+                                                (3, 1.0, 3.25 / 128, 1.0), (3, 60.0, 0.02, None),
+                                                (2, 1.0e3, 0.5, None)])
+def test_list_prefilter_band_holds_the_fp32_error(dim, extent, rc, box):
+    """Outside the band the FP32 test decides as the FP64 one does.  Pairs are
+    placed around the build radius (within a few band widths and far from it)
+    anywhere in the extent, the FP32 distance formed as the kernel forms it."""
+    import numpy as np
+    rng = np.random.default_rng(20261204)
+    rc2 = rc * rc
+    band = _prefilter_band(rc2, dim, extent, box)
+    lo2 = np.nextafter(np.float32(rc2 - band), np.float32(0))
+    hi2 = np.nextafter(np.float32(rc2 + band), np.float32(np.inf))
+    n = 200_000
+    x0 = rng.uniform(-extent, extent, dim) * (0.0 if box else 1.0)
+    lo, hi = (0.0, box) if box else (x0, x0 + extent)
+    xi = rng.uniform(lo, hi, (n, dim))
+    u = rng.normal(size=(n, dim))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    r = rc * (1.0 + rng.choice([-1, 1], n) * np.concatenate(
+        [rng.uniform(0, 8 * band / rc2, n // 2), rng.uniform(0, 0.5, n - n // 2)]))
+    xj = xi + u * r[:, None]
+    if box:
+        xj = np.mod(xj, box)
+    else:
+        xj = np.clip(xj, lo, hi)
+    fi = (xi - xi[0]).astype(np.float32)          # FP32 rows relative to row 0
+    fj = (xj - xi[0]).astype(np.float32)
+    df = fi - fj
+    d = xi - xj
+    if box:
+        bf, ibf = np.float32(box), np.float32(1.0) / np.float32(box)
+        df = (df - bf * np.rint(df * ibf)).astype(np.float32)
+        d = d - box * np.rint(d / box)
+    d2f = (df * df).sum(axis=1, dtype=np.float32)
+    d2 = (d * d).sum(axis=1)
+    assert band < 0.01 * rc2
+    assert not np.any((d2f < lo2) & (d2 > rc2))
+    assert not np.any((d2f > hi2) & (d2 <= rc2))
+    # the band is thin: of the pairs placed within 8 band widths of the build
+    # radius (half of them) about one in eight falls inside it
+    assert np.mean((d2f >= lo2) & (d2f <= hi2)) < 0.1
