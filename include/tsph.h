@@ -322,26 +322,43 @@ typedef struct tsph_sim_params {
   double obstacle_centre[3];
   int32_t obstacle_clamp;
   double obstacle_lo[3], obstacle_hi[3], obstacle_eps;
+  /* Adaptive step (adaptive_dt; droplet_impact.py:495-552, :1655-1763,
+   * dam_break.py:364-465).  adaptive = 1 integrates to target_time: after step
+   * k, dt = min(dt_safety * min(courant_num / (c0 + v_k), dt_limit),
+   * dt * dt_growth, op.dt) with v_k the largest speed at the start of step k,
+   * courant_num = max_courant * h_factor * dx and dt_limit the smallest
+   * speed-independent limit (viscous, gravity, diffusion); step k integrates
+   * with min(dt, target_time - elapsed), and the step that starts within
+   * 1e-12 max(1, target_time) of target_time is the final sample: it is
+   * evaluated and metered but not moved, and tsph_sim_step returns after it.
+   * adaptive = 0: every step integrates with op.dt. */
+  int32_t adaptive;
+  double target_time, courant_num, dt_limit, dt_safety, dt_growth;
 } tsph_sim_params;
 
 /* pos: (n_fluid + n_wall, dim) FP64 device, fluid then wall; vel: (n_fluid,
  * dim) FP64; mass: (n_fluid + n_wall) in the scalar type. */
 int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double* vel,
                     const void* mass, hipStream_t stream, tsph_sim** out);
-/* Advances nsteps steps.  meter_every > 0 records a meter row on the steps
- * whose index is a multiple of it.  Returns when the last step has finished. */
+/* Advances nsteps steps (adaptive runs: fewer once the final sample is done).
+ * meter_every > 0 records a meter row on the steps whose index is a multiple
+ * of it.  Returns when the last step has finished. */
 int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stream);
 /* State in the caller's order: pos, vel (n_fluid, dim) FP64; rho, p (n_fluid)
  * and acc (n_fluid, dim) of the last step.  Any pointer may be NULL. */
 int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc,
                   hipStream_t stream);
-/* counters: steps done, list builds (cell binnings), boundary crossings,
- * cells of the dense cell table the list builds walk, meter rows; peaks: max
- * density and max speed over all steps.  Host pointers. */
+/* counters (6): steps done, list builds (cell binnings), boundary crossings,
+ * cells of the dense cell table the list builds walk, meter rows, 1 once an
+ * adaptive run has done its final sample; peaks: max density and max speed
+ * over all steps.  Host pointers. */
 int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
 /* Meter rows (host): step, sums[mass, momentum x/y/z, kinetic energy, sum rho,
- * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, peak rho,
- * peak speed], minima[rho, p, y]: 20 doubles per row. */
+ * potential energy], maxima[rho, p, speed, |acc|, y, x, displacement, and the
+ * step's density and speed maxima tracked on every step (tsph_sim_stats keeps
+ * the run's peaks)], minima[rho, p, y], time[elapsed at the step's start, the
+ * previous step's size, this step's size, the step-size variable]: 24
+ * doubles per row.  The meters are of the state at the step's start. */
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
 /* Obstacle loads of every completed step (host), 11 doubles per row: step,
  * F x/y/z, M x/y/z (about obstacle_centre), cumulative clamp impulse x/y/z

@@ -443,3 +443,21 @@ class Stepper2D:
         self.pos, self.vel, cnt = apply_boundary(self.pos, self.vel, self.lo, self.hi, self.tol)
         self.escaped += cnt
         return rho, p, acc
+
+
+def adaptive_next_dt(dt: float, max_speed: float, *, dt_cfg: float, max_courant: float, h_factor: float,
+                     dx: float, c0: float, viscosity: float, gravity: float, diffusion: float,
+                     dt_safety: float, dt_growth: float) -> float:
+    """The adaptive step after one step (droplet_impact.py:495-552, :1389-1421):
+    the smallest of the acoustic limit at the step's starting speed
+    max_courant h / (c0 + |v|), 0.125 h^2 / nu, 0.25 sqrt(h / g) and
+    0.125 h / (delta c0), scaled by dt_safety; grown from dt by at most
+    dt_growth (or taken at once when dt_growth < 1); never above dt_cfg."""
+    h = h_factor * dx
+    acoustic = max_courant * h_factor * dx / (c0 + max(0.0, max_speed))
+    viscous = 0.125 * h * h / viscosity if viscosity > 0.0 else math.inf
+    grav = 0.25 * math.sqrt(h_factor * dx / gravity) if gravity > 0.0 else math.inf
+    diff = 0.125 * h / (diffusion * c0) if diffusion > 0.0 else math.inf
+    target = dt_safety * min(acoustic, viscous, grav, diff)
+    grown = dt * dt_growth if dt_growth >= 1.0 else target
+    return min(target, grown, dt_cfg)

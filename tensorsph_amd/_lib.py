@@ -86,7 +86,9 @@ class SimParams(C.Structure):
     _fields_ = [("op", WcsphParams), ("skin", _dbl), ("lo", _dbl * 3), ("hi", _dbl * 3),
                 ("tol", _dbl), ("clamp", C.c_int32), ("obstacle_begin", _i64), ("obstacle_end", _i64),
                 ("obstacle_centre", _dbl * 3), ("obstacle_clamp", C.c_int32),
-                ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3), ("obstacle_eps", _dbl)]
+                ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3), ("obstacle_eps", _dbl),
+                ("adaptive", C.c_int32), ("target_time", _dbl), ("courant_num", _dbl),
+                ("dt_limit", _dbl), ("dt_safety", _dbl), ("dt_growth", _dbl)]
 
 
 _dp = C.POINTER(_dbl)

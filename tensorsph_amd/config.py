@@ -235,6 +235,17 @@ def next_dt(cfg: Config, dt: float, max_speed: float = 0.0) -> float:
     return min(target, grown, cfg.dt)
 
 
+def adaptive_controls(cfg: Config) -> dict:
+    """The adaptive-step rule of next_dt (droplet_impact.py:495-552) for the
+    resident stepper: the run integrates to cfg.steps * cfg.dt; the acoustic
+    limit's numerator max_courant * h_factor * dx is formed as the reference
+    forms it, and the speed-independent limits are folded into one."""
+    return {"target_time": cfg.steps * cfg.dt,
+            "courant_num": cfg.max_courant * cfg.h_factor * cfg.dx,
+            "dt_limit": min(viscous_dt_limit(cfg), gravity_dt_limit(cfg), diffusion_dt_limit(cfg)),
+            "dt_safety": cfg.dt_safety, "dt_growth": cfg.dt_growth}
+
+
 _UNSUPPORTED = (
     ("surface_tension_coefficient", lambda c: c.surface_tension_coefficient > 0.0),
     ("surface_pressure_closure", lambda c: bool(c.surface_pressure_closure)),

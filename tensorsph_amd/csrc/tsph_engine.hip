@@ -35,17 +35,54 @@
 
 namespace tsph {
 
-__global__ void k_pack_record(const SimFlags* __restrict__ fl, const double* __restrict__ sums, int slot_in,
+TSPH_D double ord_dec(unsigned long long u) {
+  const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+  return __longlong_as_double((long long)b);
+}
+
+// Packs the record of the step just queued and, unless it was skipped,
+// advances the time state: elapsed += step_dt, and with the adaptive step
+// dt = next_dt(dt, max |v| at the step's start) (droplet_impact.py:536-552),
+// the next step_dt = min(dt, target - elapsed), and the step that starts at the
+// target (within tol) is the final sample, integrated with step_dt = 0.
+__global__ void k_pack_record(SimFlags* fl, const double* __restrict__ sums, int slot_in,
                               int with_sums, StepRecord* __restrict__ rec, const double* __restrict__ loads,
-                              const double* __restrict__ clamp) {
+                              const double* __restrict__ clamp, DtCtl ctl) {
   const int t = threadIdx.x;
   if (t == 0) { rec->stale_in = fl->stale[slot_in]; rec->stale_out = fl->stale[slot_in ^ 1]; rec->escaped = fl->escaped; }
-  if (t < MM_NMAX) rec->maxes[t] = fl->maxes[t];
+  if (t < MM_NMAX && t != MM_PEAK_SPEED && t != MM_PEAK_RHO) rec->maxes[t] = fl->maxes[t];
   if (t < MN_NMIN) rec->mins[t] = fl->mins[t];
   if (t < MS_NSUM) rec->sums[t] = with_sums ? sums[t] : 0.0;
   if (t < 6) rec->loads[t] = loads ? loads[t] : 0.0;
   if (t < 3) rec->clamp_imp[t] = clamp ? clamp[t] : 0.0;
-  if (t == 0) rec->clamp_count = clamp ? ((const unsigned long long*)(clamp + 3))[0] : 0ull;
+  if (t == 0) {
+    rec->clamp_count = clamp ? ((const unsigned long long*)(clamp + 3))[0] : 0ull;
+    rec->elapsed = fl->elapsed; rec->last_dt = fl->last_dt; rec->step_dt = fl->step_dt; rec->dt = fl->dt;
+    rec->final = fl->final;
+    // the step's largest speed and density, read and cleared in one atomic
+    // each (a plain scalar load followed by an independent store of 0 is not
+    // ordered against it); the host keeps the run's peaks
+    const unsigned long long kv = atomicExch(&fl->maxes[MM_PEAK_SPEED], 0ull);
+    rec->maxes[MM_PEAK_SPEED] = kv;
+    rec->maxes[MM_PEAK_RHO] = atomicExch(&fl->maxes[MM_PEAK_RHO], 0ull);
+    const double vmax = ord_dec(kv);
+    if (!fl->stale[slot_in] && !fl->final) {
+      fl->elapsed += fl->step_dt;
+      fl->last_dt = fl->step_dt;
+      if (ctl.adaptive) {
+        const double v = vmax > 0.0 ? vmax : 0.0;
+        const double acoustic = ctl.courant_num / (ctl.c0 + v);
+        const double target = ctl.dt_safety * (acoustic < ctl.dt_limit ? acoustic : ctl.dt_limit);
+        const double grown = ctl.dt_growth >= 1.0 ? fl->dt * ctl.dt_growth : target;
+        double dt = target < grown ? target : grown;
+        dt = dt < ctl.dt_cfg ? dt : ctl.dt_cfg;
+        fl->dt = dt;
+        const double left = ctl.target_time - fl->elapsed;
+        fl->final = fl->elapsed >= ctl.target_time - ctl.tol ? 1ull : 0ull;
+        fl->step_dt = fl->final ? 0.0 : (dt < left ? dt : left);
+      }
+    }
+  }
 }
 
 // a run keeps per-step load rows when it has obstacle stations or the clamp
@@ -241,7 +278,7 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NB nb
     }
     rate.add(-a.diff_scale * (a.mass[j] / rho_in[j]) * (contrast * radial / (r2 + S(1e-12))));
   });
-  double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)a.dt * hi(rate);
+  double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)(S)fl->step_dt * hi(rate);
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = (S)rho_d;
   const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
@@ -408,6 +445,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     mi = (double)a.mass[i]; rhoi = (double)a.rho[i]; pi_ = (double)a.p[i];
   }
   // ---- integrate + clamp (droplet_impact.py:1745-1756, :1293-1312)
+  const double dt = fl->step_dt;
   double disp = 0.0, spd2 = 0.0;
   unsigned long long hits = 0;
   if (live) {
@@ -415,8 +453,8 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     int hit_any = 0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      vn[d] = vi[d] + st.dt * (double)acc[d];
-      double x = xi[d] + st.dt * vn[d];
+      vn[d] = vi[d] + dt * (double)acc[d];
+      double x = xi[d] + dt * vn[d];
       const int hit = (x < st.lo[d] - st.tol) || (x > st.hi[d] + st.tol);
       hit_any |= hit;
       x = x < st.lo[d] ? st.lo[d] : (x > st.hi[d] ? st.hi[d] : x);
@@ -434,22 +472,35 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     for (int d = 0; d < D; ++d) spd2 += vi[d] * vi[d];
     hits = hit_any;
   }
-  // ---- per-step maxima (peak compression, peak speed, displacement)
+  // ---- per-step maxima (density, speed, displacement), one atomic per block
   const double wdisp = wave_max(disp);
-  const double wspd = sqrt(wave_max(spd2));
+  const double wspd2 = wave_max(spd2);
   const double wrho = wave_max(live ? rhoi : -1e300);
   unsigned long long whits = hits;
   for (int o = 32; o > 0; o >>= 1) whits += __shfl_xor(whits, o, 64);
+  __shared__ double bmax[3][kBlock / 64];
+  __shared__ unsigned long long bhits[kBlock / 64];
   if ((threadIdx.x & 63) == 0) {
-    atomicMax(&fl->maxes[MM_DISP], ord_enc(wdisp));
-    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(wspd));
-    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(wrho));
-    if (METERS) {
-      atomicMax(&fl->maxes[MM_SPEED], ord_enc(wspd));
-      atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(wrho));
+    const int w = threadIdx.x >> 6;
+    bmax[0][w] = wdisp; bmax[1][w] = wspd2; bmax[2][w] = wrho; bhits[w] = whits;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bd = bmax[0][0], bv = bmax[1][0], br = bmax[2][0];
+    unsigned long long bh = bhits[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      bd = fmax(bd, bmax[0][w]); bv = fmax(bv, bmax[1][w]); br = fmax(br, bmax[2][w]); bh += bhits[w];
     }
-    if (whits) atomicAdd(&fl->escaped, whits);
-    if (wdisp > half_skin) fl->stale[slot_in ^ 1] = 1;
+    const double bspd = sqrt(bv);
+    atomicMax(&fl->maxes[MM_DISP], ord_enc(bd));
+    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(bspd));
+    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(br));
+    if (METERS) {
+      atomicMax(&fl->maxes[MM_SPEED], ord_enc(bspd));
+      atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(br));
+    }
+    if (bh) atomicAdd(&fl->escaped, bh);
+    if (bd > half_skin) fl->stale[slot_in ^ 1] = 1;
   }
   if (METERS) {
     // sums of the pre-integration state (the rows of droplet_impact.py:1709-1733)
@@ -733,7 +784,7 @@ int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
 template int queue_slot_density_wall<2, float>(tsph_sim*, int64_t, hipStream_t);
 template int queue_slot_density_wall<2, double>(tsph_sim*, int64_t, hipStream_t);
 
-// The cell-walk sweeps of one step (used where no cell-tile path exists).
+// The slot-list sweeps of one step (used where no cell-tile path exists).
 template <int D, typename S>
 int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   tsph_ctx* ctx = sim->ctx;
@@ -764,7 +815,6 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     k_step_wall<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
-  st.dt = sim->prm.op.dt;
   st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
   st.vel_out = sim->vel[sim->cur ^ 1].as<double>();
   for (int d = 0; d < 3; ++d) {
@@ -806,6 +856,30 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   return TSPH_OK;
 }
 
+static DtCtl sim_dtctl(const tsph_sim* sim) {
+  const tsph_sim_params& p = sim->prm;
+  DtCtl c{};
+  c.adaptive = p.adaptive;
+  c.dt_cfg = p.op.dt;
+  c.target_time = p.target_time;
+  c.tol = 1.0e-12 * (p.target_time > 1.0 ? p.target_time : 1.0);
+  c.courant_num = p.courant_num;
+  c.c0 = p.op.sound_speed;
+  c.dt_limit = p.dt_limit;
+  c.dt_safety = p.dt_safety;
+  c.dt_growth = p.dt_growth;
+  return c;
+}
+
+// the time state before the first step
+__global__ void k_init_time(SimFlags* fl, DtCtl c) {
+  fl->dt = c.dt_cfg;
+  fl->last_dt = c.dt_cfg;
+  fl->elapsed = 0.0;
+  fl->final = c.adaptive && 0.0 >= c.target_time - c.tol ? 1ull : 0ull;
+  fl->step_dt = fl->final ? 0.0 : (c.adaptive && c.target_time < c.dt_cfg ? c.target_time : c.dt_cfg);
+}
+
 // Queue one step (density [+diffusion], wall, force+integrate) reading the
 // stale slot of step `k`.
 template <int D, typename S>
@@ -834,7 +908,7 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
   k_pack_record<<<1, 64, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
                                       load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
-                                      load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr);
+                                      load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr, sim_dtctl(sim));
   TSPH_CHECK_LAUNCH();
   TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
@@ -850,6 +924,7 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
   StepRecord* rec = &sim->ring[r];
   *stale_after = rec->stale_out != 0;
   if (rec->stale_in) return TSPH_OK;   // skipped: nothing to record
+  if (sim->finished) return TSPH_OK;   // after an adaptive run's final sample: no-op steps
   sim->steps_done = k + 1;
   if (sim->timing) {
     for (int q = 0; q < 3; ++q) {
@@ -875,7 +950,12 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
     for (int i = 0; i < MS_NSUM; ++i) sim->rows.push_back(rec->sums[i]);
     for (int i = 0; i < MM_NMAX; ++i) sim->rows.push_back(ord_dec_host(rec->maxes[i]));
     for (int i = 0; i < MN_NMIN; ++i) sim->rows.push_back(ord_dec_host(rec->mins[i]));
+    sim->rows.push_back(rec->elapsed);
+    sim->rows.push_back(rec->last_dt);
+    sim->rows.push_back(rec->step_dt);
+    sim->rows.push_back(rec->dt);
   }
+  if (rec->final) sim->finished = true;
   return TSPH_OK;
 }
 
@@ -889,6 +969,7 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
     }
     return sim_rebuild<D, S>(sim, stream);
   };
+  if (sim->finished) return TSPH_OK;
   if (!sim->built) TSPH_TRY(rebuild());
   // steps in flight (queued but not yet consumed)
   int64_t next = start;        // next step index to queue
@@ -903,6 +984,11 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
     bool stale_after = false;
     TSPH_TRY(sim_consume(sim, done, is_meter(done), &stale_after));
     ++done;
+    if (sim->finished) {
+      // the final sample is done; steps queued after it integrate with dt = 0
+      for (int64_t k = done; k < next; ++k) TSPH_CHECK_HIP(hipEventSynchronize(sim->ev[k % kRing]));
+      break;
+    }
     if (stale_after) {
       // steps done..next-1 were skipped on the device; drain them, rebuild, redo
       for (int64_t k = done; k < next; ++k) {
@@ -933,6 +1019,10 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   TSPH_REQUIRE(prm->obstacle_begin >= 0 && prm->obstacle_begin <= prm->obstacle_end &&
                prm->obstacle_end <= prm->op.n_wall, TSPH_ERR_ARG, "obstacle rows outside the wall rows");
   TSPH_REQUIRE(!prm->obstacle_clamp || prm->op.dim == 3, TSPH_ERR_ARG, "the obstacle clamp is 3-D");
+  if (prm->adaptive)
+    TSPH_REQUIRE(prm->target_time >= 0.0 && prm->courant_num > 0.0 && prm->dt_limit > 0.0 &&
+                 prm->dt_safety > 0.0 && prm->dt_growth > 0.0, TSPH_ERR_ARG,
+                 "adaptive step: target_time >= 0 and positive courant_num, dt_limit, dt_safety, dt_growth");
   if (prm->obstacle_clamp)
     for (int d = 0; d < 3; ++d)
       TSPH_REQUIRE(prm->obstacle_lo[d] < prm->obstacle_hi[d] && prm->obstacle_eps >= 0.0, TSPH_ERR_ARG,
@@ -992,7 +1082,8 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemsetAsync(sim->clamp_acc.ptr, 0, 8 * 4, stream);
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
-  // obstacle loads and density diffusion run on the cell-walk sweeps; other
+  k_init_time<<<1, 1, 0, stream>>>(sim->flags.as<SimFlags>(), sim_dtctl(sim));
+  // obstacle loads and density diffusion run on the slot-list sweeps; other
   // 2-D runs default to the LDS cell tiles
   if (sim->dim == 2 && !sim->obstacle && !(sim->prm.op.diffusion_scale > 0.0)) {
     sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4;
@@ -1039,7 +1130,8 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
     counters[1] = sim->rebuilds;
     counters[2] = sim->escaped_total;
     counters[3] = sim->ncells_dense;
-    counters[4] = (int64_t)(sim->rows.size() / (1 + MS_NSUM + MM_NMAX + MN_NMIN));
+    counters[4] = (int64_t)(sim->rows.size() / kMeterWidth);
+    counters[5] = sim->finished ? 1 : 0;
   }
   if (peaks) { peaks[0] = sim->peak_rho; peaks[1] = sim->peak_speed; }
   return TSPH_OK;
@@ -1047,7 +1139,7 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
 
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows) {
   TSPH_REQUIRE(sim && nrows, TSPH_ERR_ARG, "bad arguments");
-  const int64_t width = 1 + MS_NSUM + MM_NMAX + MN_NMIN;
+  const int64_t width = kMeterWidth;
   const int64_t have = (int64_t)sim->rows.size() / width;
   *nrows = have;
   if (rows) {
@@ -1072,9 +1164,9 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
-  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the cell-walk sweeps");
+  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the slot-list sweeps");
   TSPH_REQUIRE(tiles == 0 || !(sim->prm.op.diffusion_scale > 0.0), TSPH_ERR_ARG,
-               "density diffusion runs on the cell-walk sweeps");
+               "density diffusion runs on the slot-list sweeps");
   TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
                               tile_x * tile_y <= 64),
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);

@@ -16,6 +16,20 @@ struct SimFlags {
   unsigned long long stale[2];      // slot k&1 = input of step k, slot (k+1)&1 = output
   unsigned long long mins[MN_NMIN]; // ordered keys, atomicMin
   unsigned long long lds_overflow;  // tiles that ran from HBM (block larger than the LDS budget)
+  // time step of the coming step, decided on the device after every step
+  double dt;                        // the step size variable of next_dt
+  double step_dt;                   // what the coming step integrates with (0: final sample)
+  double elapsed;                   // simulated time at the start of the coming step
+  double last_dt;                   // the previous step's step_dt (op.dt before the first)
+  unsigned long long final;         // adaptive: the coming step is the final sample
+};
+
+// adaptive step control (droplet_impact.py:495-552, 1674-1763)
+struct DtCtl {
+  int adaptive;
+  double dt_cfg, target_time, tol;
+  double courant_num;               // max_courant * h_factor * dx, as the reference forms it
+  double c0, dt_limit, dt_safety, dt_growth;
 };
 
 // one entry of the host ring, packed on the device after every step and
@@ -28,9 +42,13 @@ struct StepRecord {
   double loads[6];                       // obstacle force and moment of this step
   double clamp_imp[3];                   // cumulative obstacle-clamp impulse
   unsigned long long clamp_count;        // cumulative clamped particles
+  double elapsed, last_dt, step_dt, dt;  // time state at the start of this step
+  unsigned long long final;              // adaptive: this step was the final sample
 };
 
 constexpr int kRing = 4;
+// doubles per meter row: step, sums, maxima, minima, time state
+constexpr int kMeterWidth = 1 + MS_NSUM + MM_NMAX + MN_NMIN + 4;
 
 // one workgroup's tile: its cell-tile id and its fluid (or wall) rows
 struct TileRec {
@@ -67,8 +85,9 @@ struct tsph_sim {
   int cur = 0;
   int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
   bool built = false;
+  bool finished = false;                  // an adaptive run has done its final sample
   // cell-tile mode (2-D): rows sorted tile-major, dense per-cell row ranges
-  int tiles = 0;                 // 1: LDS cell-tile sweeps, 0: cell-walk sweeps
+  int tiles = 0;                 // 1: LDS cell-tile force sweep, 0: slot-list sweeps
   int tile_dim[3] = {4, 4, 1};
   int64_t ext[3] = {1, 1, 1};
   int64_t ntile[3] = {1, 1, 1};

@@ -299,8 +299,9 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
     const S acc1 = (ap1.get() + av1.get()) + (-a.gravity);
     if (acc_out) { acc_out[2 * r] = acc0; acc_out[2 * r + 1] = acc1; }
     // symplectic Euler + clamp (droplet_impact.py:1745-1756, :1293-1312)
-    double vn0 = vi0 + st.dt * (double)acc0, vn1 = vi1 + st.dt * (double)acc1;
-    double xn0 = xi0 + st.dt * vn0, xn1 = xi1 + st.dt * vn1;
+    const double dt = fl->step_dt;
+    double vn0 = vi0 + dt * (double)acc0, vn1 = vi1 + dt * (double)acc1;
+    double xn0 = xi0 + dt * vn0, xn1 = xi1 + dt * vn1;
     const int h0 = (xn0 < st.lo[0] - st.tol) || (xn0 > st.hi[0] + st.tol);
     const int h1 = (xn1 < st.lo[1] - st.tol) || (xn1 > st.hi[1] + st.tol);
     xn0 = xn0 < st.lo[0] ? st.lo[0] : (xn0 > st.hi[0] ? st.hi[0] : xn0);
@@ -330,12 +331,21 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
   unsigned long long whits = hits;
   for (int o = 32; o > 0; o >>= 1) whits += __shfl_xor(whits, o, 64);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) {
-    atomicMax(&fl->maxes[MM_DISP], ord_enc(wdisp));
-    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(wspd));
-    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(wrho));
-    if (whits) atomicAdd(&fl->escaped, whits);
-    if (wdisp > half_skin) fl->stale[slot ^ 1] = 1;
+  __shared__ double bmax[3][kBlock / 64];
+  __shared__ unsigned long long bhits[kBlock / 64];
+  if (lane == 0) { bmax[0][wave] = wdisp; bmax[1][wave] = wspd; bmax[2][wave] = wrho; bhits[wave] = whits; }
+  __syncthreads();
+  if (threadIdx.x == 0) {                 // one atomic per block and quantity
+    double bd = bmax[0][0], bv = bmax[1][0], br = bmax[2][0];
+    unsigned long long bh = bhits[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      bd = fmax(bd, bmax[0][w]); bv = fmax(bv, bmax[1][w]); br = fmax(br, bmax[2][w]); bh += bhits[w];
+    }
+    atomicMax(&fl->maxes[MM_DISP], ord_enc(bd));
+    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(bv));
+    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(br));
+    if (bh) atomicAdd(&fl->escaped, bh);
+    if (bd > half_skin) fl->stale[slot ^ 1] = 1;
   }
   if (METERS) {
 #pragma unroll
@@ -512,7 +522,6 @@ int queue_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   TSPH_TRY((queue_slot_density_wall<2, S>(sim, k, stream)));
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
-  st.dt = sim->prm.op.dt;
   st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
   st.vel_out = sim->vel[sim->cur ^ 1].as<double>();
   for (int d = 0; d < 3; ++d) {

@@ -72,7 +72,8 @@ inline double ord_dec_host(unsigned long long u) {
 // Meter slots (per-block partial sums, then atomics on ordered keys).
 enum MeterSum { MS_MASS = 0, MS_MX, MS_MY, MS_MZ, MS_KE, MS_RHO, MS_PE, MS_NSUM };
 // per-row maxima (reset at every metered step), the per-step displacement and
-// the run-long peaks of density and speed (droplet_impact.py:1697-1705)
+// the density and speed maxima of every step, from which the resident stepper
+// keeps the run's peaks on the host (droplet_impact.py:1697-1705)
 enum MeterMax { MM_RHO_MAX = 0, MM_P_MAX, MM_SPEED, MM_ACC, MM_Y_MAX, MM_X_MAX, MM_DISP,
                 MM_PEAK_RHO, MM_PEAK_SPEED, MM_NMAX };
 enum MeterMin { MN_RHO_MIN = 0, MN_P_MIN, MN_Y_MIN, MN_NMIN };

@@ -5,8 +5,8 @@ pair cache, rhs, symplectic Euler, apply_boundary: the loop of
 dam_break.py:374-465) and returns the same ``DamBreakRun`` record: the
 sampled front position and energies, every-step extrema, the wall-impact
 time and the compression before it.  Position shifting and transport-velocity
-correction (``shift_every``, ``tvc_every``) and the adaptive step are not
-part of the GPU particle pipeline and raise NotImplementedError.
+correction (``shift_every``, ``tvc_every``) are not part of the GPU particle
+pipeline and raise NotImplementedError; the adaptive step runs on the device.
 """
 from __future__ import annotations
 
@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .cases import make_column, wall_lattice, lattice_mass_unit
-from .config import DamBreakConfig, check_supported
+from .config import DamBreakConfig, adaptive_controls, check_supported
 
 __all__ = ["DamBreakConfig", "DamBreakRun", "validate_case", "front_position", "ritter_front",
            "ritter_height_profile", "ritter_front_over_a", "run_case"]
@@ -124,16 +124,17 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     Every step is metered on the device (its pre-integration state, as the
     reference samples it), so the all-step extrema and the first step whose
     front reaches pool_width - 3 dx come from the same rows; samples every
-    ``output_every`` steps and at the end.  ``snapshot_times`` captures the
-    fluid positions and velocities at those times.  ``precision`` "fp64" /
-    "fp32" (FP32-compensated) defaults to torch's default dtype."""
+    ``output_every`` steps and at the end, at the accumulated simulated time.
+    ``snapshot_times`` captures the fluid positions and velocities at the first
+    sample reaching each time.  With ``adaptive_dt`` the step follows next_dt
+    on the device up to cfg.steps * cfg.dt (at most 4 cfg.steps + 1000 steps).
+    ``precision`` "fp64" / "fp32" (FP32-compensated) defaults to torch's
+    default dtype."""
     from .engine import METER_KEYS, WcsphSimulation
+    from .droplet_impact import _state_time
 
     validate_case(cfg)
     check_supported(cfg)
-    if cfg.adaptive_dt:
-        raise NotImplementedError("adaptive_dt decides each step on the host; the resident stepper "
-                                  "runs a fixed dt")
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     dev = torch.device(device) if device is not None else torch.device("cuda")
@@ -143,6 +144,7 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     h = cfg.h_factor * cfg.dx
     skin = cfg.pair_skin_factor if pair_skin_factor is None else pair_skin_factor
     initial_mass = float(mass.sum())
+    adaptive = bool(cfg.adaptive_dt)
     sim = WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
                           sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
                           viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=lattice_mass_unit(cfg),
@@ -150,46 +152,66 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
                           wall_friction=cfg.wall_friction, density_diffusion=cfg.density_diffusion,
                           diffusion_form=cfg.density_diffusion_form,
                           eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
-                          precision=precision, device=dev)
+                          precision=precision, device=dev,
+                          adaptive=adaptive_controls(cfg) if adaptive else None)
     targets = sorted(float(t) for t in (snapshot_times or []))
-    sample = [k for k in range(cfg.steps + 1) if k % cfg.output_every == 0 or k == cfg.steps]
-    # a snapshot is taken at the first sampled step whose time reaches it
-    # (dam_break.py:410-414); times past the end are never reached
-    due = {}
-    for t in targets:
-        k = next((k for k in sample if k * cfg.dt >= t - 1.0e-12), None)
-        if k is not None:
-            due.setdefault(k, []).append(t)
     snapshots = []
+    max_steps = 4 * cfg.steps + 1000 if adaptive else cfg.steps
+    ow = cfg.output_every
     try:
-        done = 0
-        for s in sorted(due) + [cfg.steps]:
-            if s > done:
-                sim.step(s - done, meter_every=1)
-                done = s
-            if s in due:
-                st = sim.read()
-                for _ in due.pop(s):
-                    snapshots.append((s * cfg.dt, st["position"].clone(), st["velocity"].clone()))
+        step = 0                                   # the step whose start state the stepper holds
+        while True:
+            final = sim.stats()["finished"] if adaptive else step == cfg.steps
+            # a snapshot is taken at the first sample reaching its time (dam_break.py:410-414)
+            if targets and (step % ow == 0 or final):
+                t_state = _state_time(sim)
+                while targets and t_state >= targets[0] - 1.0e-12:
+                    st = sim.read()
+                    snapshots.append((t_state, st["position"].clone(), st["velocity"].clone()))
+                    targets.pop(0)
+            if final or step > max_steps:
+                break
+            nxt = min((step // ow + 1) * ow, max_steps + 1)
+            sim.step(nxt - step, meter_every=1)
+            if adaptive and sim.stats()["finished"]:
+                step = sim.stats()["steps"] - 1
+            else:
+                step = nxt
         escaped = sim.stats()["escaped"]
-        final = sim.read()["position"]
-        sim.step(1, meter_every=1)                 # the final sample: rhs of the last state
+        final_pos = sim.read()["position"]
+        if not adaptive:
+            sim.step(1, meter_every=1)             # the final sample: rhs of the last state
         rows = sim.meters()
         stats = sim.stats()
     finally:
         sim.close()
+    if adaptive and not stats["finished"]:
+        raise FloatingPointError(f"the adaptive step did not reach {cfg.steps * cfg.dt:g} s within "
+                                 f"{max_steps} steps")
     col = {k: i for i, k in enumerate(METER_KEYS)}
-    rows = rows[np.argsort(rows[:, 0], kind="stable")][: cfg.steps + 1]
+    rows = rows[np.argsort(rows[:, 0], kind="stable")]
+    steps_taken = int(rows[-1, 0]) if adaptive else cfg.steps
+    rows = rows[: steps_taken + 1]
+    sample = [k for k in range(steps_taken + 1) if k % ow == 0 or k == steps_taken]
+    time = rows[:, col["time"]]
     impact_limit = cfg.pool_width - 3.0 * cfg.dx
     compression = np.maximum.accumulate(rows[:, col["rho_max"]] / cfg.rho0 - 1.0)
     hit = np.nonzero(rows[:, col["x_max"]] >= impact_limit)[0]
-    wall_impact_time = float(rows[hit[0], 0] * cfg.dt) if hit.size else None
+    wall_impact_time = float(time[hit[0]]) if hit.size else None
     compression_at_impact = float(compression[hit[0]]) if hit.size else None
-    speed_peak = float(rows[:, col["max_speed"]].max())
-    sampled_speed = max(float(rows[k, col["max_speed"]]) for k in sample)
+    speed = rows[:, col["max_speed"]]
+    speed_peak = float(speed.max())
+    sampled_speed = max(float(speed[k]) for k in sample)
     peak_compression = float(compression[-1])
+    if adaptive:
+        step_dt = rows[:, col["step_dt"]]
+        courant = float(np.max(step_dt * (cfg.sound_speed + speed) / h))
+        dt_min, dt_max = float(rows[:, col["dt"]].min()), float(rows[:, col["dt"]].max())
+    else:
+        courant = cfg.dt * (cfg.sound_speed + speed_peak) / h
+        dt_min = dt_max = cfg.dt
     return DamBreakRun(
-        times=[k * cfg.dt for k in sample],
+        times=[float(time[k]) for k in sample],
         front=[float(rows[k, col["x_max"]]) for k in sample],
         kinetic_energy=[float(rows[k, col["kinetic_energy"]]) for k in sample],
         potential_energy=[float(rows[k, col["potential_energy"]]) for k in sample],
@@ -199,6 +221,6 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
         compression_before_wall_impact=(peak_compression if compression_at_impact is None
                                         else compression_at_impact),
         wall_impact_time=wall_impact_time, initial_mass=initial_mass,
-        final_mass=float(mass.sum()), final_positions=final, steps_taken=cfg.steps,
-        simulated_time=cfg.steps * cfg.dt, dt_min=cfg.dt, dt_max=cfg.dt,
-        max_acoustic_courant=cfg.dt * (cfg.sound_speed + speed_peak) / h, snapshots=snapshots)
+        final_mass=float(mass.sum()), final_positions=final_pos, steps_taken=steps_taken,
+        simulated_time=float(time[steps_taken]), dt_min=dt_min, dt_max=dt_max,
+        max_acoustic_courant=courant, snapshots=snapshots)

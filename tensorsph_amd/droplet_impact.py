@@ -22,7 +22,7 @@ from . import _lib
 from .cases import (cell_centres, domain_top, hydrostatic_density, lattice_kernel_sum,
                     lattice_mass_unit)
 from .config import (Config, SimulationUnstableError, VISCOUS_LAPLACIAN_FACTOR, check_supported,
-                     validate_config, acoustic_courant, suggested_dt, stable_dt, next_dt,
+                     validate_config, acoustic_courant, adaptive_controls, suggested_dt, stable_dt, next_dt,
                      viscous_dt_limit, gravity_dt_limit, diffusion_dt_limit, acoustic_dt_limit)
 from .neighbor_search import PairListCache, cell_linked_pairs, drop_wall_wall_pairs
 
@@ -260,13 +260,16 @@ ROW_KEYS = ["step", "time", "dt", "particle_count", "mass_total", "momentum_x", 
 def _meter_row(cfg: Config, m, step: int, n: int) -> dict:
     """One diagnostic row of run() (droplet_impact.py:1711-1733) from a meter
     row of the resident stepper (engine.METER_KEYS: the state of ``step``
-    before its integration, as the reference samples it)."""
+    before its integration, as the reference samples it).  An adaptive run
+    reports the accumulated time and the size of the step before the row."""
     from .engine import METER_KEYS
     g = {k: float(m[i]) for i, k in enumerate(METER_KEYS)}
     rho_min, rho_max = g["rho_min"], g["rho_max"]
     speed = g["max_speed"]
+    time = g["time"] if cfg.adaptive_dt else step * cfg.dt
+    last_dt = g["last_dt"] if cfg.adaptive_dt else cfg.dt
     return {
-        "step": step, "time": step * cfg.dt, "dt": cfg.dt, "particle_count": n,
+        "step": step, "time": time, "dt": last_dt, "particle_count": n,
         "mass_total": g["mass"], "momentum_x": g["momentum_x"], "momentum_y": g["momentum_y"],
         "rho_min": rho_min, "rho_max": rho_max, "rho_mean": g["rho_sum"] / n,
         "rho_max_relative_error": max(abs(rho_min / cfg.rho0 - 1.0), abs(rho_max / cfg.rho0 - 1.0)),
@@ -328,9 +331,11 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
     particles_XXXXXX.pt snapshot (position, density) per row are written;
     ``plot`` is accepted and ignored (no figures are drawn).  ``precision``
     "fp64" / "fp32" (FP32-compensated) defaults to torch's default dtype, the
-    dtype the reference runs in.  Fixed ``dt`` only: ``adaptive_dt`` raises
-    NotImplementedError.  ``fail_fast`` raises SimulationUnstableError after
-    the artifacts are written, as the reference does.
+    dtype the reference runs in.  With ``adaptive_dt`` the step follows
+    next_dt on the device and the run integrates to cfg.steps * cfg.dt (at most
+    4 cfg.steps + 1000 steps, as the reference caps it); rows are metered every
+    step for the all-step extrema.  ``fail_fast`` raises SimulationUnstableError
+    after the artifacts are written, as the reference does.
     """
     import json
     from pathlib import Path
@@ -342,9 +347,6 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
 
     validate_config(cfg)
     check_supported(cfg)
-    if cfg.adaptive_dt:
-        raise NotImplementedError("adaptive_dt decides each step on the host; the resident stepper "
-                                  "runs a fixed dt")
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     pos, mass, n_pool = make_particles(cfg)
@@ -367,36 +369,74 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
                           wall_friction=cfg.wall_friction, density_diffusion=cfg.density_diffusion,
                           diffusion_form=cfg.density_diffusion_form,
                           eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
-                          precision=precision, device=device)
+                          precision=precision, device=device,
+                          adaptive=adaptive_controls(cfg) if cfg.adaptive_dt else None)
     out = Path(output) if output is not None else None
     if out is not None:
         out.mkdir(parents=True, exist_ok=True)
-    row_steps = list(range(0, cfg.steps, cfg.output_every)) + [cfg.steps]
     snapshots = []
     escaped = 0
+    failure = None
     try:
-        for k, step in enumerate(row_steps):
-            if out is not None:
-                snapshots.append((step, _snapshot(sim, cfg, wall, mass, out, step)))
-            if step == cfg.steps:
-                escaped = sim.stats()["escaped"]
-                sim.step(1, meter_every=1)         # the final row: rhs of the last state
-                break
-            nxt = row_steps[k + 1]
-            sim.step(nxt - step, meter_every=cfg.output_every)
-        meters = sim.meters()
-        stats = sim.stats()
+        if cfg.adaptive_dt:
+            # the step count is an outcome: advance output_every steps at a time
+            # until the final sample (the step starting at the target time)
+            max_steps = 4 * cfg.steps + 1000
+            step, shot = 0, -1
+            while True:
+                if out is not None and step != shot:
+                    snapshots.append((step, _snapshot(sim, cfg, wall, mass, out, step, time=None)))
+                    shot = step
+                if sim.stats()["finished"] or step > max_steps:
+                    break
+                todo = min(cfg.output_every, max_steps + 1 - step)
+                sim.step(todo, meter_every=1)
+                st = sim.stats()
+                step = st["steps"] - 1 if st["finished"] else step + todo
+            meters = sim.meters()
+            stats = sim.stats()
+            escaped = stats["escaped"]
+            if not stats["finished"]:
+                failure = (f"the adaptive step did not reach {cfg.steps * cfg.dt:g} s within "
+                           f"{max_steps} steps")
+            final_step = int(meters[-1, 0]) if stats["finished"] else -1
+            row_steps = [int(r[0]) for r in meters
+                         if int(r[0]) % cfg.output_every == 0 or int(r[0]) == final_step]
+        else:
+            row_steps = list(range(0, cfg.steps, cfg.output_every)) + [cfg.steps]
+            for k, step in enumerate(row_steps):
+                if out is not None:
+                    snapshots.append((step, _snapshot(sim, cfg, wall, mass, out, step)))
+                if step == cfg.steps:
+                    escaped = sim.stats()["escaped"]
+                    sim.step(1, meter_every=1)         # the final row: rhs of the last state
+                    break
+                nxt = row_steps[k + 1]
+                sim.step(nxt - step, meter_every=cfg.output_every)
+            meters = sim.meters()
+            stats = sim.stats()
     finally:
         sim.close()
     by_step = {int(r[0]): r for r in meters}
     rows = [_meter_row(cfg, by_step[s], s, n) for s in row_steps if s in by_step]
     peak_speed = float(stats["peak_speed"])
+    if cfg.adaptive_dt:
+        from .engine import METER_KEYS
+        col = {k: i for i, k in enumerate(METER_KEYS)}
+        peak_courant = max(acoustic_courant(float(r[col["dt"]]), cfg, float(r[col["max_speed"]]))
+                           for r in meters)
+        last = meters[-1]
+        simulated_time = float(last[col["time"]] + last[col["step_dt"]])
+        dt_min, dt_max = float(meters[:, col["dt"]].min()), float(meters[:, col["dt"]].max())
+    else:
+        peak_courant = acoustic_courant(cfg.dt, cfg, peak_speed)
+        simulated_time, dt_min, dt_max = cfg.steps * cfg.dt, cfg.dt, cfg.dt
     extremes = {"peak_rho_compression": float(stats["peak_rho"]) / cfg.rho0 - 1.0,
-                "peak_speed": peak_speed,
-                "peak_courant": acoustic_courant(cfg.dt, cfg, peak_speed),
-                "simulated_time": cfg.steps * cfg.dt, "dt_min": cfg.dt, "dt_max": cfg.dt}
+                "peak_speed": peak_speed, "peak_courant": peak_courant,
+                "simulated_time": simulated_time, "dt_min": dt_min, "dt_max": dt_max}
     criteria = evaluate_acceptance(rows, cfg, escaped, extremes=extremes)
-    failure = None if all(math.isfinite(v) for r in rows for v in r.values()) else "non-finite state"
+    if failure is None and not all(math.isfinite(v) for r in rows for v in r.values()):
+        failure = "non-finite state"
     if out is not None:
         import csv
         import dataclasses
@@ -417,8 +457,8 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
                         "gravity_dt_limit": _json_number(gravity_dt_limit(cfg)),
                         "diffusion_dt_limit": _json_number(diffusion_dt_limit(cfg)),
                         "acoustic_courant": acoustic_courant(cfg.dt, cfg),
-                        "simulated_time": extremes["simulated_time"], "adaptive_dt": False,
-                        "dt_min": cfg.dt, "dt_max": cfg.dt},
+                        "simulated_time": extremes["simulated_time"], "adaptive_dt": bool(cfg.adaptive_dt),
+                        "dt_min": dt_min, "dt_max": dt_max},
             "statuses": {"software_execution": "failed" if failure else "success",
                          "numerical_verification": numerical, "physical_validation": "not_evaluated"},
             "acceptance": {"criteria": criteria},
@@ -439,9 +479,15 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
     return rows
 
 
-def _snapshot(sim, cfg: Config, wall, mass, out, step: int):
+def _snapshot(sim, cfg: Config, wall, mass, out, step: int, time: float | None = None):
     """particles_XXXXXX.pt of the current state: positions and the rhs density
-    of that state (droplet_impact.py:1734-1739)."""
+    of that state (droplet_impact.py:1734-1739).  ``time`` defaults to
+    step * dt (fixed step); an adaptive run passes None for the accumulated
+    time of the state, read from the stepper."""
+    if time is None and cfg.adaptive_dt:
+        time = _state_time(sim)
+    elif time is None:
+        time = step * cfg.dt
     st = sim.read()
     pos = st["position"]
     dt = pos.dtype if sim.code == _lib.F64 else torch.float32
@@ -450,6 +496,17 @@ def _snapshot(sim, cfg: Config, wall, mass, out, step: int):
     rho, _, _ = rhs(p, st["velocity"].to(dt), torch.as_tensor(mass, dtype=dt, device=pos.device), cfg,
                     wall=w, dt=cfg.dt)
     name = out / f"particles_{step:06d}.pt"
-    torch.save({"step": int(step), "time": float(step * cfg.dt), "position": pos.detach().cpu(),
+    torch.save({"step": int(step), "time": float(time), "position": pos.detach().cpu(),
                 "density": rho.detach().cpu(), "boundary": str(cfg.boundary)}, name)
     return name
+
+
+def _state_time(sim) -> float:
+    """Simulated time of the stepper's current state: the last metered step's
+    start time plus its size (0 before the first step)."""
+    from .engine import METER_KEYS
+    m = sim.meters()
+    if m.shape[0] == 0:
+        return 0.0
+    col = {k: i for i, k in enumerate(METER_KEYS)}
+    return float(m[-1, col["time"]] + m[-1, col["step_dt"]])

@@ -19,10 +19,14 @@ from . import _lib
 SimParams = _lib.SimParams
 _vp, _i64 = C.c_void_p, C.c_int64
 
+# "time" is the simulated time at the step's start (the sum of the step sizes
+# before it), "last_dt" the previous step's size (op dt before the first),
+# "step_dt" this step's size (0 for an adaptive run's final sample), "dt" the
+# step-size variable the adaptive rule carries.
 METER_KEYS = ["step", "mass", "momentum_x", "momentum_y", "momentum_z", "kinetic_energy",
               "rho_sum", "potential_energy", "rho_max", "pressure_max", "max_speed",
               "max_acceleration", "y_max", "x_max", "displacement", "peak_rho", "peak_speed",
-              "rho_min", "pressure_min", "y_min"]
+              "rho_min", "pressure_min", "y_min", "time", "last_dt", "step_dt", "dt"]
 
 
 def boundary_clamp(pos: torch.Tensor, vel: torch.Tensor, lo, hi, tol: float) -> int:
@@ -52,12 +56,16 @@ class WcsphSimulation:
     pos (n, D), vel (n, D), mass (n) are the fluid; wall (nw, D) static wall
     particles of mass ``wall_mass``.  ``precision`` is "fp64" or "fp32"
     (FP32 compensated).  ``lo``/``hi``/``tol`` give the apply_boundary box.
+    ``adaptive`` (optional) = {"target_time", "courant_num", "dt_limit",
+    "dt_safety", "dt_growth"} runs the adaptive step of next_dt
+    (droplet_impact.py:495-552) on the device up to target_time; ``dt`` is
+    then the configured (largest) step.
     """
 
     def __init__(self, pos, vel, mass, wall, *, h, rho0, sound_speed, gamma, gravity, viscosity,
                  dt, wall_mass, skin, lo, hi, tol, wall_friction=0.0, density_diffusion=0.0,
                  diffusion_form="molteni", eigenvalue_floor=0.1, precision="fp32", clamp=True,
-                 device="cuda", obstacle=None):
+                 device="cuda", obstacle=None, adaptive=None):
         dev = torch.device(device)
         pos = torch.as_tensor(pos, dtype=torch.float64, device=dev)
         vel = torch.as_tensor(vel, dtype=torch.float64, device=dev)
@@ -100,6 +108,13 @@ class WcsphSimulation:
                 for d in range(3):
                     p.obstacle_lo[d], p.obstacle_hi[d] = float(lo_b[d]), float(hi_b[d])
                 p.obstacle_eps = float(obstacle.get("eps", 0.0))
+        if adaptive is not None:
+            p.adaptive = 1
+            p.target_time = float(adaptive["target_time"])
+            p.courant_num = float(adaptive["courant_num"])
+            p.dt_limit = float(adaptive["dt_limit"])
+            p.dt_safety = float(adaptive["dt_safety"])
+            p.dt_growth = float(adaptive["dt_growth"])
         self.params = p
         handle = _vp()
         stream = _lib.stream_handle(dev)
@@ -130,11 +145,11 @@ class WcsphSimulation:
         return out
 
     def stats(self) -> dict:
-        c = (_i64 * 5)()
+        c = (_i64 * 6)()
         pk = (C.c_double * 2)()
         _lib.check(_lib.lib().tsph_sim_stats(self.handle, c, pk))
         return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "cells": c[3],
-                "meter_rows": c[4], "peak_rho": pk[0], "peak_speed": pk[1]}
+                "meter_rows": c[4], "finished": bool(c[5]), "peak_rho": pk[0], "peak_speed": pk[1]}
 
     def meters(self) -> np.ndarray:
         nrows = _i64(0)

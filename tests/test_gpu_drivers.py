@@ -1,5 +1,6 @@
 """GPU parity of the step-path drivers: droplet_impact.run
-(droplet_impact.py:1607-1795) and dam_break.run_case (dam_break.py:316-486).
+(droplet_impact.py:1607-1795) and dam_break.run_case (dam_break.py:316-486),
+with the fixed and the adaptive step (droplet_impact.py:495-552).
 
 Bars: run() rows in FP64 against the reference's own run() rows of the
 golden case (41 rows, every key: masses and momenta 1e-12 relative, kinetic
@@ -104,7 +105,13 @@ def test_run_case_matches_oracle_loop():
             impact = k
         if k < cfg.steps:
             ref.step()
-    assert res.times == [k * cfg.dt for k in range(0, cfg.steps + 1, cfg.output_every)]
+    # samples at the accumulated simulated time (dam_break.py:395, :455)
+    elapsed, acc_t = [], 0.0
+    for k in range(cfg.steps + 1):
+        if k % cfg.output_every == 0 or k == cfg.steps:
+            elapsed.append(acc_t)
+        acc_t += cfg.dt
+    assert res.times == elapsed
     assert np.allclose(res.front, front, rtol=1e-9, atol=0)
     assert np.allclose(res.kinetic_energy, ke, rtol=1e-8, atol=1e-12 * max(ke))
     assert np.allclose(res.potential_energy, pe, rtol=1e-9, atol=0)
@@ -114,7 +121,7 @@ def test_run_case_matches_oracle_loop():
     assert res.steps_taken == cfg.steps and res.final_mass == res.initial_mass
     assert (res.wall_impact_time is None) == (impact is None)
     if impact is not None:
-        assert abs(res.wall_impact_time - impact * cfg.dt) < 1e-15
+        assert abs(res.wall_impact_time - impact * cfg.dt) < 1e-15 * max(1, impact)
     # snapshots at the first sample reaching 0.012 s (step 40) and 0.03 s (step 120)
     assert [round(t / cfg.dt) for t, _, _ in res.snapshots] == [40, 120]
     assert np.abs(n(res.snapshots[0][1]) - snaps[40]).max() < 1e-10
@@ -163,3 +170,124 @@ def test_config1_droplet_impact_tracks_oracle_loop():
     ke32 = np.array([r["kinetic_energy"] for r in rows32])
     assert np.abs(ke32 - np.array(ke_ref)).max() < 1e-4 * max(ke_ref)
     assert all(r["mass_total"] == rows32[0]["mass_total"] for r in rows32)
+
+
+def _adaptive_oracle_loop(cfg, pos, mass, wall, max_steps):
+    """The reference's adaptive loop (dam_break.py:364-465) around the oracle
+    stepper: step k integrates with min(dt, target - elapsed), dt follows the
+    step rule from the speed at the step's start, and the step that starts at
+    the target is the final sample."""
+    from tensorsph_amd.cases import lattice_mass_unit
+    h = cfg.h_factor * cfg.dx
+    ref = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0, c0=cfg.sound_speed,
+                        gamma=cfg.gamma, gravity=cfg.gravity, nu=cfg.viscosity, dt=cfg.dt,
+                        wall_mass=lattice_mass_unit(cfg), skin=cfg.pair_skin_factor * h, lo=[0, 0],
+                        hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx)
+    kw = dict(dt_cfg=cfg.dt, max_courant=cfg.max_courant, h_factor=cfg.h_factor, dx=cfg.dx,
+              c0=cfg.sound_speed, viscosity=cfg.viscosity, gravity=cfg.gravity,
+              diffusion=cfg.density_diffusion, dt_safety=cfg.dt_safety, dt_growth=cfg.dt_growth)
+    target = cfg.steps * cfg.dt
+    tol = 1.0e-12 * max(1.0, target)
+    dt, elapsed = cfg.dt, 0.0
+    out = {"times": [], "front": [], "ke": [], "dts": [cfg.dt], "courant": 0.0}
+    for step in range(max_steps + 1):
+        step_dt = min(dt, target - elapsed)
+        x, v = ref.pos.copy(), ref.vel.copy()
+        speed = float(np.sqrt((v * v).sum(1)).max())
+        out["courant"] = max(out["courant"], step_dt * (cfg.sound_speed + speed) / h)
+        final = elapsed >= target - tol
+        if step % cfg.output_every == 0 or final:
+            out["times"].append(elapsed)
+            out["front"].append(float(x[:, 0].max()))
+            out["ke"].append(0.5 * float((mass[:, None] * v ** 2).sum()))
+        if final:
+            out["steps"] = step
+            break
+        ref.dt = step_dt
+        ref.kw["dt"] = step_dt
+        ref.step()
+        elapsed += step_dt
+        dt = orc.adaptive_next_dt(dt, speed, **kw)
+        out["dts"].append(dt)
+    ref.pairs_for_step()
+    out["pos"], out["rebuilds"], out["elapsed"] = ref.pos, ref.rebuilds, elapsed
+    return out
+
+
+def test_adaptive_run_case_tracks_oracle_loop():
+    """run_case with adaptive_dt on a dam break started at the Courant limit
+    (so the step shrinks at once and follows the surge) against the oracle
+    loop: the same step count and list builds, step sizes and sample times
+    within 1e-12, front and energies within 1e-9 / 1e-8, final positions
+    within 1e-10 m."""
+    from tensorsph_amd.cases import make_column, wall_lattice
+    from tensorsph_amd.dam_break import DamBreakConfig, run_case
+    cfg = DamBreakConfig(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.5, sound_speed=20.0, viscosity=1e-3, steps=200,
+                         output_every=40, adaptive_dt=True)
+    cfg.dt = cfg.dt_from_courant()
+    res = run_case(cfg, precision="fp64")
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=cfg.tank_top())
+    ref = _adaptive_oracle_loop(cfg, pos, mass, wall, 4 * cfg.steps + 1000)
+    assert res.steps_taken == ref["steps"] > cfg.steps         # smaller steps, more of them
+    assert res.rebuilds == ref["rebuilds"]
+    assert np.allclose(res.times, ref["times"], rtol=1e-12, atol=0)
+    assert res.simulated_time == pytest.approx(cfg.steps * cfg.dt, rel=1e-12)
+    assert res.dt_min == pytest.approx(min(ref["dts"]), rel=1e-12)
+    assert res.dt_max == pytest.approx(max(ref["dts"]), rel=1e-12)
+    assert res.dt_min < res.dt_max == cfg.dt
+    assert np.allclose(res.front, ref["front"], rtol=1e-9, atol=0)
+    assert np.allclose(res.kinetic_energy, ref["ke"], rtol=1e-8, atol=1e-12 * max(ref["ke"]))
+    assert res.max_acoustic_courant == pytest.approx(ref["courant"], rel=1e-9)
+    assert np.abs(n(res.final_positions) - ref["pos"]).max() < 1e-10
+    assert res.mass_drift == 0.0
+
+
+def test_adaptive_dam_break_respects_the_courant_limit():
+    """The reference's own adaptive check (tests/test_adaptive_dt.py:114-139):
+    the fixed step overshoots its nominal Courant number by more than 5 %, the
+    adaptive run stays within 2 % of max_courant, shrinks its step, reaches the
+    same simulated time and keeps the mass exact."""
+    from tensorsph_amd.dam_break import DamBreakConfig, run_case
+
+    def build(adaptive):
+        cfg = DamBreakConfig(dx=0.04, pool_width=2.0, column_width=0.2, sound_speed=25.0,
+                             adaptive_dt=adaptive)
+        cfg.column_height = 0.4
+        cfg.dt = cfg.dt_from_courant()
+        cfg.steps = int(round(0.25 / cfg.dt))
+        cfg.output_every = max(1, int(round(0.05 / cfg.dt)))
+        return cfg
+
+    fixed, adaptive = build(False), build(True)
+    fixed_run, adaptive_run = run_case(fixed, precision="fp64"), run_case(adaptive, precision="fp64")
+    nominal = fixed.dt * fixed.sound_speed / (fixed.h_factor * fixed.dx)
+    assert nominal == pytest.approx(fixed.max_courant)
+    assert fixed_run.max_acoustic_courant > nominal * 1.05
+    assert adaptive_run.max_acoustic_courant <= adaptive.max_courant * 1.02
+    assert adaptive_run.dt_min < adaptive_run.dt_max
+    assert adaptive_run.times[-1] == pytest.approx(fixed_run.times[-1], rel=1e-3)
+    assert adaptive_run.mass_drift == 0.0
+
+
+def test_adaptive_run_reproduces_fixed_rows_below_the_cap(tmp_path):
+    """The reference's fixed-path check (tests/test_adaptive_dt.py:87-107): the
+    small impact case never leaves its configured step, so the adaptive run()
+    rows equal the fixed ones and dt_min = dt_max = dt."""
+    from tensorsph_amd import droplet_impact as di
+    base = dict(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05, drop_center_y=0.25,
+                steps=6, output_every=3, fail_fast=False)
+    fixed = di.run(di.Config(**base), tmp_path / "fixed", precision="fp64")
+    adaptive = di.run(di.Config(**base, adaptive_dt=True), tmp_path / "adaptive", precision="fp64")
+    man = json.loads((tmp_path / "adaptive" / "run_manifest.json").read_text())
+    assert man["derived"]["dt_min"] == pytest.approx(di.Config(**base).dt)
+    assert man["derived"]["dt_max"] == pytest.approx(di.Config(**base).dt)
+    assert man["derived"]["adaptive_dt"] is True
+    assert [r["step"] for r in adaptive] == [r["step"] for r in fixed] == [0, 3, 6]
+    for a, b in zip(fixed, adaptive):
+        for key in ("kinetic_energy", "rho_max", "max_speed", "momentum_x"):
+            assert a[key] == b[key]
+    assert sorted(p.name for p in (tmp_path / "adaptive").glob("particles_*.pt")) == [
+        "particles_000000.pt", "particles_000003.pt", "particles_000006.pt"]
+
