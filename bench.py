@@ -32,6 +32,7 @@ sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
 PROFILE = os.path.join(HERE, "profiles", "k_tile_force_dambreak_1p6M_fp32c.md")
+PROFILE_LISTS = os.path.join(HERE, "profiles", "k_step_force_dambreak_1p6M_fp32c.md")
 PROFILE_3D = os.path.join(HERE, "profiles", "k_step_force_kleefsman_5M_fp32c.md")
 
 
@@ -62,7 +63,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", default=None,
-                    help="fixed tile shape TXxTY (e.g. 8x6) instead of the autotuner's choice")
+                    help="fixed layout instead of the autotuner's choice: a tile shape TXxTY "
+                         "(e.g. 8x6) or 'lists'")
     ap.add_argument("--no-serve", action="store_true", help="skip the served δ⁺ batch")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=500)
@@ -438,7 +440,10 @@ def main():
     nf, nw = pos.shape[0], wall.shape[0]
     h = cfg.h_factor * cfg.dx
 
-    tile = None if args.tile is None else tuple(int(v) for v in args.tile.lower().split("x"))
+    if args.tile is None or args.tile.lower() == "lists":
+        tile = args.tile and "lists"
+    else:
+        tile = tuple(int(v) for v in args.tile.lower().split("x"))
     if args.diffusion > 0.0:
         tile = "lists"
     sim = make_sim(cfg, pos, mass, wall, wmass, "fp32", args.diffusion)
@@ -457,6 +462,8 @@ def main():
     b_force = nf * (32 * 2 + 2 * s) + nw * (16 + 3 * s)
     b_density = nf * (16 + 2 * s) + nw * (16 + s)
     force_ms = sweep_ms[2] / max(nsteps, 1)
+    on_lists = args.diffusion > 0.0 or tuned["best"] == "lists"
+    kernel, prof = ("k_step_force", PROFILE_LISTS) if on_lists else ("k_tile_force", PROFILE)
     achieved = b_force / (force_ms * 1e-3) / 1e9 if force_ms > 0 else 0.0
     step_bytes = nf * (40 * 2 + 4 * s) + nw * (24 * 2 + 2 * s)
 
@@ -490,10 +497,10 @@ def main():
                               "force_integrate": force_ms},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.diffusion else profiled_traffic(nf),
-                     "traffic_source": None if args.diffusion else os.path.relpath(PROFILE, HERE),
-                     "kernel": ("k_step_force (slot lists: force + viscosity + integrate)" if args.diffusion
-                                else "k_tile_force (force + viscosity + integrate)"),
+                     "traffic": None if args.diffusion else profiled_traffic(nf, prof, kernel),
+                     "traffic_source": None if args.diffusion else os.path.relpath(prof, HERE),
+                     "kernel": (f"{kernel} (slot lists: force + viscosity + integrate)" if on_lists
+                                else f"{kernel} (LDS tiles: force + viscosity + integrate)"),
                      "algorithmic_bytes_per_launch": b_force},
     }
     if not args.no_fp64:

@@ -541,7 +541,14 @@ __global__ void __launch_bounds__(kBlock) k3_kick_drift(double* __restrict__ pos
   }
   double v = sqrt(d2);
   for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(disp, ord_enc(v));
+  // one atomic per block: per-wave atomics on the one word serialise
+  __shared__ double wv[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, wv[w]);
+    atomicMax(disp, ord_enc(v));
+  }
 }
 
 // Re-orders the state by the cell order of the last search; afterwards the
