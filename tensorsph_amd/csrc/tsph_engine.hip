@@ -525,24 +525,29 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       if (lane == 0) red[k][wave] = v;
     }
     const double big = 1e300;
-    const double wpmax = wave_max(live ? pi_ : -big), wpmin = wave_min(live ? pi_ : big);
-    const double wrmin = wave_min(live ? rhoi : big), wacc = wave_max(sqrt(accn2));
-    const double wymin = wave_min(live ? xi[1 % D] : big), wymax = wave_max(live ? xi[1 % D] : -big);
-    const double wxmax = wave_max(live ? xi[0] : -big);
-    if (lane == 0) {
-      atomicMax(&fl->maxes[MM_P_MAX], ord_enc(wpmax));
-      atomicMax(&fl->maxes[MM_ACC], ord_enc(wacc));
-      atomicMax(&fl->maxes[MM_Y_MAX], ord_enc(wymax));
-      atomicMax(&fl->maxes[MM_X_MAX], ord_enc(wxmax));
-      atomicMin(&fl->mins[MN_RHO_MIN], ord_enc(wrmin));
-      atomicMin(&fl->mins[MN_P_MIN], ord_enc(wpmin));
-      atomicMin(&fl->mins[MN_Y_MIN], ord_enc(wymin));
-    }
+    // row extrema, ordered so that all are maxima: p, |acc|, y, x, -rho, -p, -y
+    double ext[7] = {wave_max(live ? pi_ : -big), wave_max(sqrt(accn2)), wave_max(live ? xi[1 % D] : -big),
+                     wave_max(live ? xi[0] : -big), -wave_min(live ? rhoi : big), -wave_min(live ? pi_ : big),
+                     -wave_min(live ? xi[1 % D] : big)};
+    __shared__ double bext[7][kBlock / 64];
+    if (lane == 0)
+      for (int k = 0; k < 7; ++k) bext[k][wave] = ext[k];
     __syncthreads();
     if (threadIdx.x < MS_NSUM) {
       double v = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
       st.block_sums[(int64_t)blockIdx.x * MS_NSUM + threadIdx.x] = v;
+    }
+    if (threadIdx.x == 0) {                 // one atomic per block and extremum
+      for (int k = 0; k < 7; ++k)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) ext[k] = fmax(ext[k], bext[k][w]);
+      atomicMax(&fl->maxes[MM_P_MAX], ord_enc(ext[0]));
+      atomicMax(&fl->maxes[MM_ACC], ord_enc(ext[1]));
+      atomicMax(&fl->maxes[MM_Y_MAX], ord_enc(ext[2]));
+      atomicMax(&fl->maxes[MM_X_MAX], ord_enc(ext[3]));
+      atomicMin(&fl->mins[MN_RHO_MIN], ord_enc(-ext[4]));
+      atomicMin(&fl->mins[MN_P_MIN], ord_enc(-ext[5]));
+      atomicMin(&fl->mins[MN_Y_MIN], ord_enc(-ext[6]));
     }
   }
 }

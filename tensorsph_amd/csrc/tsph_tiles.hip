@@ -353,24 +353,30 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
       const double v = wsum(ms[k]);
       if (lane == 0) red[k][wave] = v;
     }
-    const double a1 = wmax(mx_p), a2 = wmin(mn_p), a3 = wmin(mn_rho), a4 = wmax(mx_acc);
-    const double a5 = wmax(mx_y), a6 = wmin(mn_y), a7 = wmax(mx_x);
-    if (lane == 0) {
-      atomicMax(&fl->maxes[MM_SPEED], ord_enc(wspd));
-      atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(wrho));
-      atomicMax(&fl->maxes[MM_P_MAX], ord_enc(a1));
-      atomicMin(&fl->mins[MN_P_MIN], ord_enc(a2));
-      atomicMin(&fl->mins[MN_RHO_MIN], ord_enc(a3));
-      atomicMax(&fl->maxes[MM_ACC], ord_enc(a4));
-      atomicMax(&fl->maxes[MM_Y_MAX], ord_enc(a5));
-      atomicMin(&fl->mins[MN_Y_MIN], ord_enc(a6));
-      atomicMax(&fl->maxes[MM_X_MAX], ord_enc(a7));
-    }
+    // row extrema, ordered so that all are maxima: speed, rho, p, -p, -rho, |acc|, y, -y, x
+    double ext[9] = {wspd, wrho, wmax(mx_p), -wmin(mn_p), -wmin(mn_rho), wmax(mx_acc), wmax(mx_y),
+                     -wmin(mn_y), wmax(mx_x)};
+    __shared__ double bext[9][kBlock / 64];
+    if (lane == 0)
+      for (int k = 0; k < 9; ++k) bext[k][wave] = ext[k];
     __syncthreads();
     if (threadIdx.x < MS_NSUM) {
       double v = 0.0;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[threadIdx.x][w];
       st.block_sums[b * MS_NSUM + threadIdx.x] = v;
+    }
+    if (threadIdx.x == 0) {                 // one atomic per block and extremum
+      for (int k = 0; k < 9; ++k)
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) ext[k] = fmax(ext[k], bext[k][w]);
+      atomicMax(&fl->maxes[MM_SPEED], ord_enc(ext[0]));
+      atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(ext[1]));
+      atomicMax(&fl->maxes[MM_P_MAX], ord_enc(ext[2]));
+      atomicMin(&fl->mins[MN_P_MIN], ord_enc(-ext[3]));
+      atomicMin(&fl->mins[MN_RHO_MIN], ord_enc(-ext[4]));
+      atomicMax(&fl->maxes[MM_ACC], ord_enc(ext[5]));
+      atomicMax(&fl->maxes[MM_Y_MAX], ord_enc(ext[6]));
+      atomicMin(&fl->mins[MN_Y_MIN], ord_enc(-ext[7]));
+      atomicMax(&fl->maxes[MM_X_MAX], ord_enc(ext[8]));
     }
   }
 }
