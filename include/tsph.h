@@ -221,8 +221,9 @@ typedef struct tsph_batch_caps {
   int64_t fluid, wall, sims;       /* rows and run slots */
   int64_t cells;                   /* cells of all per-run grids together (cell edge 4 dx) */
   int64_t fluid_entries, wall_entries;  /* neighbour-list entries: each row gets entries / rows
-                                           slots (32..256, 64 is ample for h = 2 dx); a row
-                                           with more neighbours fails its run (state 3) */
+                                           slots rounded down to a multiple of 4 (32..256, 64
+                                           is ample for h = 2 dx); a row with more neighbours
+                                           fails its run (state 3) */
   int32_t sorted_lists;            /* 1: lists in ascending row order (the order of
                                       tsph_deltaplus_step, for bitwise checks); 0: cell order */
 } tsph_batch_caps;

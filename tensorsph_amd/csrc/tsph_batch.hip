@@ -65,10 +65,12 @@ struct BatchDev {
   unsigned int* flags;              // bit 0 cell overflow
   int32_t *cid, *ccount, *cfill, *members;
   int64_t* cstart;
-  // Lists in fixed slots, interleaved by wave: entry k of row r sits at
-  // (r / 64) * 64 * K + 64 k + r % 64, so the 64 rows of a wave read their
-  // k-th entries as one coalesced 256-byte run and no count pass or scan is
-  // needed; fcnt / wcnt hold the counts.
+  // Lists in fixed slots, interleaved by wave in chunks of four: entry k of
+  // row r sits at int32 index (r / 64) * 64 * K + 256 (k / 4) + 4 (r % 64)
+  // + k % 4 (K a multiple of 4), so a lane writes its entries four at a time
+  // as one 16-byte store, the 64 rows of a wave read their entries 4c .. 4c+3
+  // from one 1 KB run, and no count pass or scan is needed; fcnt / wcnt hold
+  // the counts.
   int32_t *fcnt, *wcnt, *fidx, *widx;
   int64_t* step_counter;
 };
@@ -310,7 +312,9 @@ __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
   const double cut2 = q.cell * q.cell;
   const int64_t cx = cell_of(x, q.cell) - q.cmin[0], cy = cell_of(y, q.cell) - q.cmin[1];
   const int64_t K = wall ? b.kw : b.kf;
-  int32_t* out = (wall ? b.widx : b.fidx) + (r >> 6) * 64 * K + (r & 63);
+  int4* out = reinterpret_cast<int4*>(wall ? b.widx : b.fidx) + (r >> 6) * 16 * K + (r & 63);
+  const int64_t chunks = K >> 2;
+  int4 buf = make_int4((int)c, (int)c, (int)c, (int)c);
   int64_t k = 0;
   for (int dx = -1; dx <= 1; ++dx) {
     const int64_t nx = cx + dx;
@@ -320,23 +324,43 @@ __global__ void __launch_bounds__(kBlock) k_b_lists(BatchDev b) {
       if (ny < 0 || ny >= q.ext[1]) continue;
       const int64_t id = q.base + nx * q.ext[1] + ny;
       const int64_t m0 = b.cstart[id], m1 = m0 + b.ccount[id];
-      for (int64_t e = m0; e < m1; ++e) {
-        const int64_t o = b.members[e];
-        if (o == c) continue;
-        if (wall && o >= b.cap_f) continue;
-        double xo, yo;
-        row_pos(b, o, xo, yo);
-        double d2;
-        {
+      // candidates four at a time: member ids, then their positions, then the tests
+      for (int64_t e0 = m0; e0 < m1; e0 += 4) {
+        int64_t o4[4];
+        double xo[4], yo[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) o4[u] = b.members[e0 + u < m1 ? e0 + u : m1 - 1];   // m1 > e0 >= m0
+#pragma unroll
+        for (int u = 0; u < 4; ++u) row_pos(b, o4[u], xo[u], yo[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int64_t o = o4[u];
+          if (e0 + u >= m1 || o == c) continue;
+          if (wall && o >= b.cap_f) continue;
+          double d2;
+          {
 #pragma clang fp contract(off)
-          const double sx = x - xo, sy = y - yo;
-          d2 = sx * sx + sy * sy;
+            const double sx = x - xo[u], sy = y - yo[u];
+            d2 = sx * sx + sy * sy;
+          }
+          if (d2 > cut2) continue;
+          const int sl = (int)(k & 3);
+          buf.x = sl == 0 ? (int)o : buf.x;
+          buf.y = sl == 1 ? (int)o : buf.y;
+          buf.z = sl == 2 ? (int)o : buf.z;
+          buf.w = sl == 3 ? (int)o : buf.w;
+          if (sl == 3 && (k >> 2) < chunks) out[(k >> 2) * 64] = buf;
+          ++k;
         }
-        if (d2 > cut2) continue;
-        if (k < K) out[64 * k] = (int32_t)o;
-        ++k;
       }
     }
+  }
+  if ((k & 3) && (k >> 2) < chunks) {        // the last partial chunk; the row itself fills it up
+    const int sl = (int)(k & 3);
+    buf.y = sl > 1 ? buf.y : (int)c;
+    buf.z = sl > 2 ? buf.z : (int)c;
+    buf.w = (int)c;
+    out[(k >> 2) * 64] = buf;
   }
   cnt[r] = (int32_t)(k < K ? k : K);
   if (k > K) atomicOr(&b.status[s], 4);
@@ -350,13 +374,14 @@ __global__ void __launch_bounds__(kBlock) k_b_sort_lists(BatchDev b) {
   const bool wall = c >= b.cap_f;
   const int64_t r = wall ? c - b.cap_f : c;
   const int64_t K = wall ? b.kw : b.kf;
-  int32_t* seg = (wall ? b.widx : b.fidx) + (r >> 6) * 64 * K + (r & 63);   // stride 64
+  int32_t* seg = (wall ? b.widx : b.fidx) + (r >> 6) * 64 * K + 4 * (r & 63);
+  auto at = [&](int64_t k) -> int32_t& { return seg[256 * (k >> 2) + (k & 3)]; };   // chunked slots
   const int64_t n = (wall ? b.wcnt : b.fcnt)[r];
   for (int64_t a = 1; a < n; ++a) {
-    const int32_t v = seg[64 * a];
+    const int32_t v = at(a);
     int64_t k = a - 1;
-    while (k >= 0 && seg[64 * k] > v) { seg[64 * (k + 1)] = seg[64 * k]; --k; }
-    seg[64 * (k + 1)] = v;
+    while (k >= 0 && at(k) > v) { at(k + 1) = at(k); --k; }
+    at(k + 1) = v;
   }
 }
 
@@ -466,8 +491,8 @@ static BatchDev batch_dev(tsph_batch* B, int side) {
   BatchDev d{};
   const tsph_batch_caps& c = B->caps;
   d.cap_f = c.fluid; d.cap_w = c.wall; d.cap_s = c.sims; d.cap_cells = c.cells;
-  d.kf = c.fluid_entries / c.fluid;
-  d.kw = c.wall > 0 ? c.wall_entries / c.wall : 0;
+  d.kf = (c.fluid_entries / c.fluid) & ~3ll;
+  d.kw = c.wall > 0 ? (c.wall_entries / c.wall) & ~3ll : 0;
   d.x = B->x[side].as<double>(); d.v = B->v[side].as<double>(); d.rho = B->rho[side].as<double>();
   d.x_new = B->x[side ^ 1].as<double>(); d.v_new = B->v[side ^ 1].as<double>();
   d.rho_new = B->rho[side ^ 1].as<double>();
@@ -621,7 +646,8 @@ int tsph_batch_create(const tsph_batch_caps* caps, hipStream_t stream, tsph_batc
                c.wall_entries >= 0, TSPH_ERR_ARG, "capacities must be positive");
   TSPH_REQUIRE(c.fluid + c.wall < (1ll << 31) - 1 && c.cells < (1ll << 31) - 1 &&
                c.fluid_entries < (1ll << 40) && c.sims <= (1 << 20), TSPH_ERR_ARG, "capacities too large");
-  const int64_t kf = c.fluid_entries / c.fluid, kw = c.wall > 0 ? c.wall_entries / c.wall : 0;
+  // slots per row, rounded down to whole chunks of four
+  const int64_t kf = (c.fluid_entries / c.fluid) & ~3ll, kw = c.wall > 0 ? (c.wall_entries / c.wall) & ~3ll : 0;
   TSPH_REQUIRE(kf >= 32 && kf <= 256 && (c.wall == 0 || (kw >= 32 && kw <= 256)), TSPH_ERR_ARG,
                "neighbour-list entries must give 32..256 slots per row (fluid %lld, wall %lld)",
                (long long)kf, (long long)kw);

@@ -53,12 +53,17 @@ TSPH_D bool dp_sep(double xt, double yt, double xo, double yo, double cut2, doub
 
 TSPH_D int dp_sim(const int32_t* ids, int64_t k) { return ids ? ids[k] : 0; }
 
-// Entries of target row t: first index, stride and count in idx (CSR, or the
-// batch engine's fixed slots interleaved by wave).
-struct DpRange { int64_t e0, step, n; };
+// Entries of target row t in idx: CSR (entry q at e0 + q), or the batch
+// engine's fixed slots interleaved by wave in chunks of four (entry q at
+// e0 + 256 (q / 4) + q % 4, e0 = (t / 64) 64 K + 4 (t % 64)).
+struct DpRange {
+  int64_t e0, n;
+  bool slots;
+  TSPH_D int64_t at(int64_t q) const { return slots ? e0 + ((q >> 2) << 8) + (q & 3) : e0 + q; }
+};
 TSPH_D DpRange dp_range(const int64_t* off, const int32_t* cnt, int64_t K, int64_t t) {
-  if (cnt) return {(t >> 6) * 64 * K + (t & 63), 64, (int64_t)cnt[t]};
-  return {off[t], 1, off[t + 1] - off[t]};
+  if (cnt) return {(t >> 6) * 64 * K + 4 * (t & 63), (int64_t)cnt[t], true};
+  return {off[t], off[t + 1] - off[t], false};
 }
 
 // Symmetric 2x2 eigen-decomposition, ascending eigenvalues (one Jacobi rotation
@@ -134,7 +139,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
   const double xw = a.xw[2 * w], yw = a.xw[2 * w + 1];
   double psum = 0.0, wsum = 0.0;
   int cnt = 0;
-  for (int64_t q_ = 0, e = wr.e0; q_ < wr.n; ++q_, e += wr.step) {
+  for (int64_t q_ = 0; q_ < wr.n; ++q_) {
+    const int64_t e = wr.at(q_);
     const int64_t f = widx[e];
     double sx, sy, d2;
     if (!dp_sep(a.x[2 * f], a.x[2 * f + 1], xw, yw, cut2, sx, sy, d2)) continue;   // x_f - x_w
@@ -155,7 +161,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
   const double nx = a.nw_[2 * w], ny = a.nw_[2 * w + 1];
   const double vwx = a.vw[2 * w], vwy = a.vw[2 * w + 1];
   double rx = 0.0, ry = 0.0;
-  for (int64_t q_ = 0, e = wr.e0; q_ < wr.n; ++q_, e += wr.step) {
+  for (int64_t q_ = 0; q_ < wr.n; ++q_) {
+    const int64_t e = wr.at(q_);
     const int64_t f = widx[e];
     double sx, sy, d2;
     if (!dp_sep(a.x[2 * f], a.x[2 * f + 1], xw, yw, cut2, sx, sy, d2)) continue;
@@ -191,7 +198,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
   const bool diff = P.diff_scale != 0.0;
   double nx_ = 0.0, ny_ = 0.0;
   double M[4] = {0.0, 0.0, 0.0, 0.0}, gs[2] = {0.0, 0.0};
-  for (int64_t q_ = 0, e = fr.e0; q_ < fr.n; ++q_, e += fr.step) {
+  for (int64_t q_ = 0; q_ < fr.n; ++q_) {
+    const int64_t e = fr.at(q_);
     const int64_t o = idx[e];
     if (o < a.nf) {
       double sx, sy, d2;
@@ -275,7 +283,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
   const bool diff = P.diff_scale != 0.0;
   const double grt0 = diff ? a.gr[2 * t] : 0.0, grt1 = diff ? a.gr[2 * t + 1] : 0.0;
   double div = 0.0, hr = 0.0, gl0 = 0.0, gl1 = 0.0;
-  for (int64_t q_ = 0, e = fr.e0; q_ < fr.n; ++q_, e += fr.step) {
+  for (int64_t q_ = 0; q_ < fr.n; ++q_) {
+    const int64_t e = fr.at(q_);
     const int64_t o = idx[e];
     if (o < a.nf) {
       double sx, sy, d2;
@@ -324,7 +333,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
   // at or above: interior), so the second pass runs for those rows alone
   int covered = 0;
   if (lt >= 0.2 && lt < 0.75) {
-    for (int64_t q_ = 0, e = fr.e0; q_ < fr.n; ++q_, e += fr.step) {
+    for (int64_t q_ = 0; q_ < fr.n; ++q_) {
+      const int64_t e = fr.at(q_);
       const int64_t o = idx[e];
       if (o >= a.nf) continue;
       double sx, sy, d2;
@@ -353,7 +363,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
   const double rt = a.rho[t];
   double nbs = 0.0, rawx = 0.0, rawy = 0.0, nearest = INFINITY;
   int64_t pick = -1;
-  for (int64_t q_ = 0, e = fr.e0; q_ < fr.n; ++q_, e += fr.step) {
+  for (int64_t q_ = 0; q_ < fr.n; ++q_) {
+    const int64_t e = fr.at(q_);
     const int64_t o = idx[e];
     const bool fluid = o < a.nf;
     const int64_t w = o - a.nf;
@@ -397,7 +408,8 @@ __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __
     const double sn = shx * wnx + shy * wny;
     const double gx = shx - sn * wnx, gy = shy - sn * wny;
     bool conflict = band && (fabs(gx * nx + gy * ny) > tol);
-    for (int64_t q_ = 0, e = fr.e0; q_ < fr.n; ++q_, e += fr.step) {
+    for (int64_t q_ = 0; q_ < fr.n; ++q_) {
+      const int64_t e = fr.at(q_);
       const int64_t o = idx[e];
       if (o < a.nf) continue;
       const int64_t w = o - a.nf;
