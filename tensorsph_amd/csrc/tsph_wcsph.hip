@@ -206,10 +206,10 @@ __global__ void __launch_bounds__(kBlock) k_wall_pressure(WcsphArgs<S> a, const 
 }
 
 // ------------------------------------------------------------------ sweep B
-template <int D, typename S, bool INTEGRATE>
+template <int D, typename S>
 __global__ void __launch_bounds__(kBlock) k_force(WcsphArgs<S> a, const int64_t* __restrict__ off,
                                                   const int32_t* __restrict__ idx,
-                                                  S* __restrict__ acc_out, StepArgs st) {
+                                                  S* __restrict__ acc_out) {
   const int64_t nb = gridDim.x;
   const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
@@ -252,7 +252,6 @@ __global__ void __launch_bounds__(kBlock) k_force(WcsphArgs<S> a, const int64_t*
 #pragma unroll
     for (int d = 0; d < D; ++d) acc_out[i * D + d] = acc[d];
   }
-  if (INTEGRATE) integrate_one<D, S>(i, xi, vi, acc, a, st);
 }
 
 // ------------------------------------------------------------------ sweep R
@@ -315,7 +314,6 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
   a.rho = ctx->s_rho.as<S>(); a.p = ctx->s_p.as<S>(); a.pr2 = ctx->s_aux.as<S>(); a.vol = ctx->s_aux2.as<S>();
   const int64_t* off = ctx->nb_off.as<int64_t>();
   const int32_t* idx = ctx->nb_idx.as<int32_t>();
-  StepArgs st{};
   if (n > 0) {
     const bool diff = prm->diffusion_scale > 0.0;
     if (!diff) {
@@ -345,7 +343,7 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
     TSPH_CHECK_LAUNCH();
   }
   if (n > 0) {
-    k_force<D, S, false><<<grid_xcd(n), kBlock, 0, stream>>>(a, off, idx, (S*)acc_out, st);
+    k_force<D, S><<<grid_xcd(n), kBlock, 0, stream>>>(a, off, idx, (S*)acc_out);
     TSPH_CHECK_LAUNCH();
     TSPH_CHECK_HIP(hipMemcpyAsync(rho_out, a.rho, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
     TSPH_CHECK_HIP(hipMemcpyAsync(p_out, a.p, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
@@ -358,25 +356,6 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
   return TSPH_OK;
 }
 
-// explicit instantiations used by the resident stepper
-#define TSPH_INST(D, S)                                                                              \
-  template __global__ void k_density<D, S>(WcsphArgs<S>, const int64_t*, const int32_t*, int);        \
-  template __global__ void k_renorm_gradient<D, S>(WcsphArgs<S>, const int64_t*, const int32_t*, S*); \
-  template __global__ void k_diffusion<D, S, 0>(WcsphArgs<S>, const int64_t*, const int32_t*,         \
-                                                const S*, const S*);                                  \
-  template __global__ void k_diffusion<D, S, 1>(WcsphArgs<S>, const int64_t*, const int32_t*,         \
-                                                const S*, const S*);                                  \
-  template __global__ void k_wall_pressure<D, S>(WcsphArgs<S>, const int64_t*, const int32_t*, S*, S*); \
-  template __global__ void k_force<D, S, true>(WcsphArgs<S>, const int64_t*, const int32_t*, S*,      \
-                                               StepArgs);                                             \
-  template __global__ void k_force<D, S, false>(WcsphArgs<S>, const int64_t*, const int32_t*, S*,     \
-                                                StepArgs);                                            \
-  template __global__ void k_wall_loads<D, S>(WcsphArgs<S>, const int64_t*, const int32_t*, S*);
-TSPH_INST(2, double)
-TSPH_INST(2, float)
-TSPH_INST(3, double)
-TSPH_INST(3, float)
-#undef TSPH_INST
 
 }  // namespace tsph
 

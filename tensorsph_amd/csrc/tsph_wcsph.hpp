@@ -124,50 +124,6 @@ TSPH_D void obstacle_clamp(double* xn, double* vn, double m, const StepArgs& st)
   }
 }
 
-// v' = v + dt a, x' = x + dt v', then the safety clamp of apply_boundary:
-// positions clamped to the box, the normal velocity zeroed beyond 0.25 dx.
-template <int D, typename S>
-TSPH_D void integrate_one(int64_t i, const double* xi, const double* vi, const S* acc,
-                          const WcsphArgs<S>& a, const StepArgs& st) {
-  double xn[D], vn[D];
-  int hit_any = 0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    vn[d] = vi[d] + st.dt * (double)acc[d];
-    double x = xi[d] + st.dt * vn[d];
-    const int hit = (x < st.lo[d] - st.tol) || (x > st.hi[d] + st.tol);
-    hit_any |= hit;
-    x = x < st.lo[d] ? st.lo[d] : (x > st.hi[d] ? st.hi[d] : x);
-    if (hit) vn[d] = 0.0;
-    xn[d] = x;
-  }
-  obstacle_clamp<D>(xn, vn, (double)a.mass[i], st);
-#pragma unroll
-  for (int d = 0; d < D; ++d) { st.pos_out[i * D + d] = xn[d]; st.vel_out[i * D + d] = vn[d]; }
-  double disp2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) { const double df = xn[d] - st.anchor[i * D + d]; disp2 += df * df; }
-  // wave-level reductions of the per-step maxima
-  double disp = sqrt(disp2);
-  double spd2 = 0.0;
-#pragma unroll
-  for (int d = 0; d < D; ++d) spd2 += vi[d] * vi[d];
-  double rho = (double)a.rho[i];
-  unsigned long long hits = hit_any;
-  for (int off = 32; off > 0; off >>= 1) {
-    disp = fmax(disp, __shfl_xor(disp, off, 64));
-    spd2 = fmax(spd2, __shfl_xor(spd2, off, 64));
-    rho = fmax(rho, __shfl_xor(rho, off, 64));
-    hits += __shfl_xor(hits, off, 64);
-  }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMax(&st.maxes[MM_DISP], ord_enc(disp));
-    atomicMax(&st.maxes[MM_PEAK_SPEED], ord_enc(sqrt(spd2)));
-    atomicMax(&st.maxes[MM_PEAK_RHO], ord_enc(rho));
-    if (hits) atomicAdd(st.escaped, hits);
-  }
-}
-
 // Regularised inverse f(M) g, f(l) = 1 / max(l, floor_frac * max(trace/D, 1e-30)),
 // by cyclic Jacobi on the symmetric matrix built from M's lower triangle (as
 // torch.linalg.eigh(UPLO='L') reads it).
