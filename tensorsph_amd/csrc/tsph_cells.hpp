@@ -265,6 +265,39 @@ struct NbSlots {
   }
 };
 
+// Appends a row's entries to its slots in the NbSlots layout, four at a time:
+// entries gather in registers and each full chunk is one 16-byte store; the
+// last partial chunk is filled up with the row itself.  push_if keeps the
+// select chain branch-free; only the store of a completed chunk branches.
+struct SlotWriter {
+  int4* p;          // chunk 0 of the row
+  int4 buf;
+  int k;
+  int32_t self;
+  TSPH_D SlotWriter(int32_t* idx, int64_t K, int64_t i)
+      : p(reinterpret_cast<int4*>(idx) + (i >> 6) * 16 * K + (i & 63)),
+        buf(make_int4((int)i, (int)i, (int)i, (int)i)), k(0), self((int32_t)i) {}
+  TSPH_D void push_if(bool keep, int32_t j) {
+    const int sl = k & 3;
+    buf.x = keep && sl == 0 ? j : buf.x;
+    buf.y = keep && sl == 1 ? j : buf.y;
+    buf.z = keep && sl == 2 ? j : buf.z;
+    buf.w = keep && sl == 3 ? j : buf.w;
+    if (keep && sl == 3) p[(k >> 2) * 64] = buf;
+    k += keep ? 1 : 0;
+  }
+  TSPH_D int finish() {
+    if (k & 3) {
+      const int sl = k & 3;
+      buf.y = sl > 1 ? buf.y : self;
+      buf.z = sl > 2 ? buf.z : self;
+      buf.w = self;
+      p[(k >> 2) * 64] = buf;
+    }
+    return k;
+  }
+};
+
 // Dense per-cell row ranges of rows sorted by cell, fluid rows [0, nf) before
 // wall rows [nf, nt): every run of equal cells writes its begin / end.  The
 // table starts zeroed (k_cell_table_clear), so empty cells are empty ranges.

@@ -71,13 +71,18 @@ TSPH_D void load3(const double* p, int64_t i, double* x) {
 
 // ---------------------------------------------------------------- density
 // summation_density (operators3.py:70-79): pair sums, then the self term
+// With sup_idx the entries within the support are also written, in list
+// order, as the support lists of the later sweeps (NbSlots layout of width K).
 template <typename S, typename NB>
-__global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* __restrict__ out,
+                                                     int32_t* __restrict__ sup_idx, int32_t* __restrict__ sup_cnt,
+                                                     int64_t K) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
   Sum<double> acc;
+  SlotWriter sup(sup_idx, K, i);
   const bool per = a.periodic;
   nb.template visit_masked<3>(i, true, [&](int64_t j, bool keep) {
     double r2 = 0.0;
@@ -91,7 +96,9 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
     acc.add_if(keep, (double)a.mass[j] * w);
+    if (sup_idx) sup.push_if(keep, (int32_t)j);
   });
+  if (sup_idx) sup_cnt[i] = sup.finish();
   store_rho<S>(hi(acc) + (double)a.mass[i] * a.kd.w0d, out, a.rho_lo, i);
 }
 
@@ -378,14 +385,27 @@ template <typename S> struct ThreedScratch {
 // (CSR lists or slot lists).  uniform_mass: every particle has the same mass.
 template <typename S, bool KICK, typename NB>
 int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b, NB nb, const int64_t* orig,
-                  S* acc_out, double half_dt, double* vel_out, hipStream_t stream, bool uniform_mass = false) {
+                  S* acc_out, double half_dt, double* vel_out, hipStream_t stream, bool uniform_mass = false,
+                  NB* support = nullptr) {
   const int64_t n = a.n;
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
   a.pack = b.pack;
   const unsigned g = grid_for(n);
   const bool diff = q.diff_scale > 0.0;
-  k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho);
+  // with support lists (slot lists only) the density sweep writes them and
+  // every later sweep reads them instead of the build-radius lists
+  if constexpr (std::is_same<NB, NbSlots>::value) {
+    if (support) {
+      k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, const_cast<int32_t*>(support->idx),
+                                                  const_cast<int32_t*>(support->cnt), nb.K);
+      nb = *support;
+    } else {
+      k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0);
+    }
+  } else {
+    k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0);
+  }
   if (diff) {
     if (q.diff_form == 1) {
       k3_gradient<S, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho_raw);
@@ -461,7 +481,7 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
   S* p_in = (S*)p;               // read only
   switch (op) {
     case 0:
-      k3_density<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, (S*)out);
+      k3_density<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, (S*)out, nullptr, nullptr, 0);
       break;
     case 1:
       a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0;
@@ -688,6 +708,7 @@ struct tsph_psim {
   DevBuf cell_R, entries;                    // dense cell table; list entries at the build radius
   DevBuf slot_idx, slot_cnt, slot_max;       // per-row lists in fixed slots (NbSlots layout)
   DevBuf slot_posf;                          // FP32 build positions (list prefilter)
+  DevBuf sup_idx, sup_cnt;                   // this evaluation's entries within the support
   int64_t slot_K = 0;
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
@@ -768,6 +789,8 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
                                       ps->slot_cnt, ps->slot_idx, ps->slot_max, ps->slot_K,
                                       ps->entries.as<unsigned long long>(), stream, &ps->slot_posf,
                                       fmax(fmax(ps->prm.box[0], ps->prm.box[1]), ps->prm.box[2]))));
+  TSPH_TRY(ps->sup_idx.ensure(sizeof(int32_t) * (size_t)((n + 63) / 64) * 64 * (size_t)ps->slot_K));
+  TSPH_TRY(ps->sup_cnt.ensure(sizeof(int32_t) * (size_t)n));
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
   ps->rebuilds++;
   ps->built = true;
@@ -781,14 +804,17 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   a.vel = ps->vel[ps->cur].as<double>();
   a.mass = ps->mass.as<S>();
   const NbSlots nb = psim_slots(ps);
+  NbSlots sup = nb;
+  sup.idx = ps->sup_idx.as<int32_t>();
+  sup.cnt = ps->sup_cnt.as<int32_t>();
   if (kick) {
     TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
                                      ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream,
-                                     ps->uniform_mass)));
+                                     ps->uniform_mass, &sup)));
     ps->cur ^= 1;
   } else {
     TSPH_TRY((threed_sweeps<S, false>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
-                                      ps->acc.as<S>(), 0.0, nullptr, stream, ps->uniform_mass)));
+                                      ps->acc.as<S>(), 0.0, nullptr, stream, ps->uniform_mass, &sup)));
   }
   return TSPH_OK;
 }
@@ -895,7 +921,8 @@ int tsph_psim_destroy(tsph_psim* ps) {
   DevBuf* b[] = {&ps->pos, &ps->vel[0], &ps->vel[1], &ps->mass, &ps->acc, &ps->orig, &ps->anchor, &ps->disp,
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
                  &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo, &ps->pack,
-                 &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max, &ps->slot_posf};
+                 &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max, &ps->slot_posf,
+                 &ps->sup_idx, &ps->sup_cnt};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);
