@@ -458,7 +458,13 @@ __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const doubl
     m = v > m ? v : m;
     sum += __shfl_xor(sum, o, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
+  // one atomic per block: per-wave atomics on one word serialise
+  __shared__ int bm[kBlock / 64];
+  __shared__ unsigned long long bs[kBlock / 64];
+  if ((threadIdx.x & 63) == 0) { bm[threadIdx.x >> 6] = m; bs[threadIdx.x >> 6] = sum; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { m = bm[w] > m ? bm[w] : m; sum += bs[w]; }
     if ((unsigned int)m > __atomic_load_n(kmax, __ATOMIC_RELAXED)) atomicMax(kmax, (unsigned int)m);
     if (total && sum) atomicAdd(total, sum);
   }

@@ -448,9 +448,16 @@ __global__ void k_max_disp(const T* __restrict__ pos, const T* __restrict__ anch
     v = o > v ? o : v;
   }
   if ((threadIdx.x & 63) == 0) {
-    // non-negative IEEE values order like their bit patterns
-    if (sizeof(T) == 4) atomicMax(out_bits_f32, __float_as_uint((float)v));
-    else atomicMax(out_bits_f64, (unsigned long long)__double_as_longlong((double)v));
+    // non-negative IEEE values order like their bit patterns; a wave whose
+    // maximum does not beat the current value (a plain, possibly stale read)
+    // skips its atomic, so the waves do not all serialise on the one word
+    if (sizeof(T) == 4) {
+      const unsigned int b = __float_as_uint((float)v);
+      if (b > __atomic_load_n(out_bits_f32, __ATOMIC_RELAXED)) atomicMax(out_bits_f32, b);
+    } else {
+      const unsigned long long b = (unsigned long long)__double_as_longlong((double)v);
+      if (b > __atomic_load_n(out_bits_f64, __ATOMIC_RELAXED)) atomicMax(out_bits_f64, b);
+    }
   }
 }
 
