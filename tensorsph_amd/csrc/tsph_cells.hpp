@@ -248,18 +248,25 @@ struct NbSlots {
   // entries are all issued before the arithmetic that needs them.  Slots past
   // the row's count run f on the row itself with keep false, as do the wall
   // rows that with_walls == false skips; f must not branch on keep.
-  template <int D, typename F>
+  // CH chunks (4 CH entries) per block.
+  template <int D, int CH = 1, typename F>
   TSPH_D void visit_masked(int64_t i, bool with_walls, F&& f) const {
     const int4* p = chunks(i);
     const int n = cnt[i];
-    for (int k = 0; k < n; k += 4) {
-      const int4 q = p[(k >> 2) * 64];
-      const int32_t j4[4] = {q.x, q.y, q.z, q.w};
+    const int4 self = make_int4((int)i, (int)i, (int)i, (int)i);
+    for (int k = 0; k < n; k += 4 * CH) {
+      int4 q[CH];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool in = k + u < n;
-        const int64_t j = in ? (int64_t)j4[u] : i;
-        f(j, in && (with_walls || j < nf));
+      for (int c = 0; c < CH; ++c) q[c] = k + 4 * c < n ? p[((k >> 2) + c) * 64] : self;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int32_t j4[4] = {q[c].x, q[c].y, q[c].z, q[c].w};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = k + 4 * c + u < n;
+          const int64_t j = in ? (int64_t)j4[u] : i;
+          f(j, in && (with_walls || j < nf));
+        }
       }
     }
   }

@@ -414,7 +414,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     Sum<S> ap[D], av[D];
     // every list entry is evaluated; terms outside the support (and the
     // viscous term of frictionless wall pairs) are left out of the sums
-    nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
+    nbr.template visit_masked<D, (D == 3 ? 2 : 1)>(i, true, [&](int64_t j, bool keep) {
       S df[D], r2 = S(0);
 #pragma unroll
       for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }

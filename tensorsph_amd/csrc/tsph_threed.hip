@@ -316,9 +316,11 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
   };
   using T1 = std::integral_constant<bool, true>;
   using F0 = std::integral_constant<bool, false>;
-  if (a.do_pressure && visc) nb.template visit_masked<3>(i, true, body(T1{}, T1{}));
-  else if (a.do_pressure) nb.template visit_masked<3>(i, true, body(T1{}, F0{}));
-  else if (visc) nb.template visit_masked<3>(i, true, body(F0{}, T1{}));
+  // slot lists: two chunks (eight entries) per block; CSR lists: four entries
+  constexpr int W = std::is_same<NB, NbSlots>::value ? 2 : 4;
+  if (a.do_pressure && visc) nb.template visit_masked<3, W>(i, true, body(T1{}, T1{}));
+  else if (a.do_pressure) nb.template visit_masked<3, W>(i, true, body(T1{}, F0{}));
+  else if (visc) nb.template visit_masked<3, W>(i, true, body(F0{}, T1{}));
   S acc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
