@@ -161,6 +161,7 @@ typedef struct tsph_threed_params {
   double eigenvalue_floor;
   double dt;                 /* used by density diffusion only */
   int32_t wall_model;        /* 0 "adami", 1 "tait" (walls keep their own Tait state) */
+  int32_t viscous_order;     /* 2 (0 reads as 2) or 4: L - L(beta L), operators3.py:146-151 */
 } tsph_threed_params;
 
 /* pos, vel: (n, 3) FP64; mass (n) scalar.  Outputs: acc (n, 3), rho and p (n)
@@ -174,7 +175,7 @@ int tsph_threed_accelerations(tsph_ctx* ctx, const tsph_threed_params* prm, cons
  * on the same all-target lists:
  *   op 0 summation_density (operators3.py:70-79)         -> out (n)
  *   op 1 pressure_acceleration (:96-113), rho and p       -> out (n, 3)
- *   op 2 viscous_acceleration, order 2 (:116-154), rho    -> out (n, 3)
+ *   op 2 viscous_acceleration, order 2 or 4 (:116-154), rho -> out (n, 3)
  *   op 3 wall_pressure_adami (:173-209), rho and p        -> out (n - n_fluid)
  *   op 4 density_diffusion_rate_3d (:259-295), rho        -> out (n)
  * gravity enters op 3 only; viscosity / renormalise enter op 2 only;

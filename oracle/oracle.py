@@ -73,7 +73,8 @@ def _declare(L):
     L.orc_threed_accelerations.argtypes = [C.c_int64, C.c_int64, _dp, _dp, _dp, C.c_double, _lp,
                                            C.c_int64, C.c_double, C.c_double, C.c_double,
                                            C.c_double, _dp, _dp, C.c_int, C.c_double,
-                                           C.c_double, C.c_int, C.c_double, _dp, _dp, _dp]
+                                           C.c_double, C.c_int, C.c_double, C.c_int,
+                                           _dp, _dp, _dp]
     L.orc_apply_boundary.restype = C.c_int64
     L.orc_apply_boundary.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, C.c_double]
     L.orc_delta_plus_step.restype = C.c_int
@@ -195,8 +196,11 @@ def threed_accelerations(pos, vel, mass, h, pairs, *, rho0, sound_speed, gamma=7
                          viscosity=0.0, gravity=None, box=None, n_fluid=None,
                          renormalise=True, density_diffusion=0.0, dt=None,
                          density_diffusion_form="antuono",
-                         density_diffusion_eigenvalue_floor=0.1):
-    """threed.operators3.accelerations (operators3.py:212-256), FP64."""
+                         density_diffusion_eigenvalue_floor=0.1, viscous_order=2):
+    """threed.operators3.accelerations (operators3.py:212-256), FP64, with the
+    viscous operator of order 2 or 4 (operators3.py:116-154)."""
+    if viscous_order not in (2, 4):
+        raise ValueError(f"viscous_order must be 2 or 4, got {viscous_order}")
     pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
     mass = np.ascontiguousarray(mass, np.float64)
     n = pos.shape[0]
@@ -210,7 +214,8 @@ def threed_accelerations(pos, vel, mass, h, pairs, *, rho0, sound_speed, gamma=7
                                    pairs.shape[0], rho0, sound_speed, gamma, viscosity,
                                    _d(g), _d(b), 1 if renormalise else 0,
                                    float(density_diffusion), float(dt or 0.0), form,
-                                   density_diffusion_eigenvalue_floor, _d(acc), _d(rho), _d(p))
+                                   density_diffusion_eigenvalue_floor, int(viscous_order),
+                                   _d(acc), _d(rho), _d(p))
     return acc, rho, p
 
 
@@ -234,7 +239,7 @@ def ledger_row(pos, vel, mass, rho, *, rho0, sound_speed, gamma, box, wavenumber
 
 
 def shear_wave_loop(pos, vel, mass, *, h, box, rho0, sound_speed, gamma=7.0, nu, dt, steps,
-                    skin, renormalise=True, sample_every=10, wavenumber=0.0):
+                    skin, renormalise=True, sample_every=10, wavenumber=0.0, viscous_order=2):
     """The step loop of threed.solver.run_shear_wave (solver.py:254-308), FP64:
     periodic pairs at 2h + skin, rebuilt when 2 max|min_image(x - x_anchor)| >
     skin; kick-drift-kick with x wrapped by the floored remainder."""
@@ -245,7 +250,7 @@ def shear_wave_loop(pos, vel, mass, *, h, box, rho0, sound_speed, gamma=7.0, nu,
     anchor = pos.copy()
     rebuilds = 1
     kw = dict(rho0=rho0, sound_speed=sound_speed, gamma=gamma, viscosity=nu, box=box,
-              renormalise=renormalise)
+              renormalise=renormalise, viscous_order=viscous_order)
     acc, rho, _ = threed_accelerations(pos, vel, mass, h, pairs, **kw)
     rows = []
     for k in range(steps + 1):

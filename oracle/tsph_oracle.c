@@ -646,14 +646,16 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
 /* ------------------------------------------------------------------------ */
 /* threed.operators3.accelerations, threed/operators3.py:212-256, with       */
 /* summation_density :70-79, tait :82-93, wall_pressure_adami :173-209,      */
-/* pressure_acceleration :96-113, viscous_acceleration :116-154 (order 2).   */
+/* pressure_acceleration :96-113, viscous_acceleration :116-154 (order 2,   */
+/* and order 4: the iterated operator L - L(beta L) of :146-151 with        */
+/* beta_ij = (I4_i + I4_j) / 240 and I4 the fourth moment of :147-149).     */
 /* ------------------------------------------------------------------------ */
 
 ORC_API void orc_threed_accelerations(int64_t n, int64_t n_fluid, const double* pos,
         const double* vel, const double* mass, double h, const int64_t* pairs, int64_t npairs,
         double rho0, double c0, double gamma, double nu, const double* gravity,
         const double* box, int renormalise, double diff_delta, double dt, int diff_form,
-        double eig_floor, double* acc, double* rho, double* pressure) {
+        double eig_floor, int viscous_order, double* acc, double* rho, double* pressure) {
     const int dim = 3;
     double* delta = (double*)malloc((size_t)(npairs > 0 ? npairs : 1) * 3 * sizeof(double));
     double* grad = (double*)malloc((size_t)(npairs > 0 ? npairs : 1) * 3 * sizeof(double));
@@ -777,6 +779,39 @@ ORC_API void orc_threed_accelerations(int64_t n, int64_t n_fluid, const double* 
         }
         for (int64_t k = 0; k < npairs; ++k) for (int d = 0; d < 3; ++d) out[pairs[2 * k] * 3 + d] += dvs[k * 3 + d];
         for (int64_t k = 0; k < npairs; ++k) for (int d = 0; d < 3; ++d) out[pairs[2 * k + 1] * 3 + d] += -dvs[k * 3 + d];
+        if (viscous_order == 4) {
+            double* fourth = (double*)calloc((size_t)n, sizeof(double));
+            for (int64_t k = 0; k < npairs; ++k) {
+                int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+                double rij = 0.5 * (rho_used[i] + rho_used[j]);
+                double r2 = 0; for (int d = 0; d < 3; ++d) r2 += delta[k * 3 + d] * delta[k * 3 + d];
+                fourth[i] += r2 * r2 * lap[k] * (mass[j] / rij);
+            }
+            for (int64_t k = 0; k < npairs; ++k) {
+                int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+                double rij = 0.5 * (rho_used[i] + rho_used[j]);
+                double r2 = 0; for (int d = 0; d < 3; ++d) r2 += delta[k * 3 + d] * delta[k * 3 + d];
+                fourth[j] += r2 * r2 * lap[k] * (mass[i] / rij);
+            }
+            /* second pass of the same pair Laplacian on the first one's output,
+             * with the pair weight scaled by beta_ij */
+            double* out2 = (double*)calloc((size_t)n * 3, sizeof(double));
+            for (int64_t k = 0; k < npairs; ++k) {
+                int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+                double rij = 0.5 * (rho_used[i] + rho_used[j]);
+                double weight = mass[j] * lap[k] / rij;
+                if (renormalise) {
+                    double mk = 0.5 * (moment[i] + moment[j]); if (mk < 1.0e-12) mk = 1.0e-12;
+                    weight = weight * (2.0 / mk);
+                }
+                double beta = 0.5 * (fourth[i] + fourth[j]) / 120.0;
+                for (int d = 0; d < 3; ++d) dvs[k * 3 + d] = (out[j * 3 + d] - out[i * 3 + d]) * (weight * beta);
+            }
+            for (int64_t k = 0; k < npairs; ++k) for (int d = 0; d < 3; ++d) out2[pairs[2 * k] * 3 + d] += dvs[k * 3 + d];
+            for (int64_t k = 0; k < npairs; ++k) for (int d = 0; d < 3; ++d) out2[pairs[2 * k + 1] * 3 + d] += -dvs[k * 3 + d];
+            for (int64_t a = 0; a < n * 3; ++a) out[a] = out[a] - out2[a];
+            free(fourth); free(out2);
+        }
         for (int64_t a = 0; a < n * 3; ++a) acc[a] = acc[a] + out[a] * nu;
         free(moment); free(out); free(dvs);
     }

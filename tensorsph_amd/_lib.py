@@ -61,7 +61,8 @@ class ThreedParams(C.Structure):
                 ("viscosity", _dbl), ("gravity", _dbl * 3), ("has_gravity", C.c_int32),
                 ("box", _dbl * 3), ("periodic", C.c_int32), ("renormalise", C.c_int32),
                 ("diffusion_scale", _dbl), ("diffusion_form", C.c_int32),
-                ("eigenvalue_floor", _dbl), ("dt", _dbl), ("wall_model", C.c_int32)]
+                ("eigenvalue_floor", _dbl), ("dt", _dbl), ("wall_model", C.c_int32),
+                ("viscous_order", C.c_int32)]
 
 
 class DeltaPlusParams(C.Structure):

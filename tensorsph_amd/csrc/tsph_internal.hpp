@@ -118,6 +118,7 @@ struct tsph_ctx {
   int64_t wnb_targets = 0, wnb_entries = 0;
   // operator scratch
   tsph::DevBuf s_rho_raw, s_rho, s_p, s_aux, s_aux2, s_aux3, s_gr, s_flags, s_pack;
+  tsph::DevBuf s_four, s_lap1;   // fourth-order viscous operator: I4 / 120 and the first Laplacian
   tsph::DevBuf s_red;        // reductions
   tsph::DevBuf s_dp;         // delta-plus step intermediates
 };

@@ -486,7 +486,8 @@ int tsph_ctx_destroy(tsph_ctx* ctx) {
                     &cs.flags, &cs.minmax, &cs.sort_tmp, &cs.idx_in, &ctx->pair_count, &ctx->pair_off,
                     &ctx->scan_tmp, &ctx->nb_off, &ctx->nb_idx, &ctx->nb_cursor, &ctx->wnb_off,
                     &ctx->wnb_idx, &ctx->wnb_cursor, &ctx->s_rho_raw, &ctx->s_rho, &ctx->s_p,
-                    &ctx->s_aux, &ctx->s_aux2, &ctx->s_aux3, &ctx->s_gr, &ctx->s_flags, &ctx->s_pack, &ctx->s_red, &ctx->s_dp};
+                    &ctx->s_aux, &ctx->s_aux2, &ctx->s_aux3, &ctx->s_gr, &ctx->s_flags, &ctx->s_pack, &ctx->s_red, &ctx->s_dp,
+                    &ctx->s_four, &ctx->s_lap1};
   for (DevBuf* b : bufs) b->release();
   delete ctx;
   return TSPH_OK;

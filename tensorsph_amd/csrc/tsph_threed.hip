@@ -50,6 +50,9 @@ template <typename S> struct ThreedArgs {
   S* rate_out;  // optional: the diffusion rate of k3_eos
   S* rho_lo;    // low part of the summed density (FP32 mode), NULL in FP64
   Vec4<S>* pack;  // optional: (m, rho_u, p / rho_u^2, K / 3) per row for the force sweep
+  int order4;     // fourth-order viscous operator (set only with nu != 0)
+  S* fourth;      // I4_i = sum_j r^4 lap_ij m_j / rho_ij (order 4)
+  S* lap1;        // (n, 3) first pair Laplacian of the velocity, before nu (order 4)
 };
 
 // x_i - x_j from FP64 coordinates, minimum image if asked, rounded to S.
@@ -228,6 +231,7 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
 }
 
 // second moment of the Laplacian (viscous_acceleration, operators3.py:137-145)
+// and, for the fourth-order operator, the fourth moment I4 (:147-149)
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -235,16 +239,23 @@ __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
   double xi[3];
   load3(a.pos, i, xi);
   const S ri = a.rho_u[i];
-  Sum<S> acc;
-  nb.template visit_masked<3>(i, true, [&](int64_t j, bool keep) {
-    S df[3], r2;
-    delta3<S>(a, j, xi, df, r2, a.periodic);
-    keep = keep && !(r2 > a.skip2);
-    S g, lap;
-    cubic_gl_r2<S>(r2, a.k, g, lap);
-    const S rij = S(0.5) * (ri + a.rho_u[j]);
-    acc.add_if(keep, r2 * lap * quot<S>(a.mass[j], rij));
-  });
+  Sum<S> acc, acc4;
+  auto body = [&](auto four) {
+    return [&](int64_t j, bool keep) {
+      S df[3], r2;
+      delta3<S>(a, j, xi, df, r2, a.periodic);
+      keep = keep && !(r2 > a.skip2);
+      S g, lap;
+      cubic_gl_r2<S>(r2, a.k, g, lap);
+      const S rij = S(0.5) * (ri + a.rho_u[j]);
+      const S t = r2 * lap * quot<S>(a.mass[j], rij);
+      acc.add_if(keep, t);
+      if constexpr (decltype(four)::value) acc4.add_if(keep, r2 * t);
+    };
+  };
+  if (a.order4) nb.template visit_masked<3>(i, true, body(std::integral_constant<bool, true>{}));
+  else nb.template visit_masked<3>(i, true, body(std::integral_constant<bool, false>{}));
+  if (a.order4) a.fourth[i] = acc4.get();
   const S K = acc.get() / S(3);
   a.moment[i] = K;
   if (a.pack) a.pack[i] = Vec4<S>{a.mass[i], ri, quot<S>(a.p[i], ri * ri), K};
@@ -321,6 +332,16 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
   if (a.do_pressure && visc) nb.template visit_masked<3, W>(i, true, body(T1{}, T1{}));
   else if (a.do_pressure) nb.template visit_masked<3, W>(i, true, body(T1{}, F0{}));
   else if (visc) nb.template visit_masked<3, W>(i, true, body(F0{}, T1{}));
+  if (a.order4) {
+    // the fourth-order operator finishes in k3_visc4, which needs every row's
+    // first Laplacian: store it with the pressure term and stop here
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      acc_out[3 * i + d] = ap[d].get();
+      a.lap1[3 * i + d] = av[d].get();
+    }
+    return;
+  }
   S acc[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -334,10 +355,86 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
   }
 }
 
+// Second pass of the fourth-order viscous operator (operators3.py:146-151):
+// L4 v_i = L v_i - sum_j w_ij beta_ij (L v_j - L v_i), with w_ij the pair
+// weight of k3_force (same member mass, same renormalisation) and beta_ij =
+// (I4_i + I4_j) / 240.  acc_out holds the pressure term of k3_force on entry;
+// the row's acceleration is that + nu L4 v (+ gravity on fluid rows), and
+// with KICK the second half-kick follows as in k3_force.
+template <typename S, bool KICK, typename NB, int MW>
+__global__ void __launch_bounds__(kBlock) k3_visc4(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
+                                                   S* __restrict__ acc_out, double half_dt,
+                                                   double* __restrict__ vel_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[3];
+  load3(a.pos, i, xi);
+  const S ri = a.rho_u[i];
+  const S Ki = a.renormalise ? a.moment[i] : S(0);
+  const S Fi = a.fourth[i];
+  const int64_t oi = MW == 1 ? orig[i] : i;
+  const S mi = a.mass[i];
+  S li[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) li[d] = a.lap1[3 * i + d];
+  Sum<S> s[3];
+  constexpr int W = std::is_same<NB, NbSlots>::value ? 2 : 4;
+  nb.template visit_masked<3, W>(i, true, [&](int64_t j, bool keep) {
+    S df[3], r2;
+    delta3<S>(a, j, xi, df, r2, a.periodic);
+    keep = keep && !(r2 > a.skip2);
+    S g, lap;
+    cubic_gl_r2<S>(r2, a.k, g, lap);
+    const S rij = S(0.5) * (ri + a.rho_u[j]);
+    const S mj = a.mass[j];
+    S mw;
+    if constexpr (MW == 2) {
+      mw = mj;
+    } else {
+      const int64_t oj = MW == 1 ? orig[j] : j;
+      mw = oj > oi ? mj : mi;
+    }
+    S wgt = quot<S>(mw * lap, rij);
+    if (a.renormalise) {
+      S mk = S(0.5) * (Ki + a.moment[j]);
+      mk = mk > S(1e-12) ? mk : S(1e-12);
+      wgt = wgt * quot<S>(S(2), mk);
+    }
+    const S wb = wgt * quot<S>(S(0.5) * (Fi + a.fourth[j]), S(120));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) s[d].add_if(keep, (a.lap1[3 * j + d] - li[d]) * wb);
+  });
+  S acc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    acc[d] = acc_out[3 * i + d] + (li[d] - s[d].get()) * a.nu;
+    if (a.has_gravity && i < a.nf) acc[d] = acc[d] + (S)a.grav[d];
+    acc_out[3 * i + d] = acc[d];
+  }
+  if (KICK) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) vel_out[3 * i + d] = a.vel[3 * i + d] + half_dt * (double)acc[d];
+  }
+}
+
+// the force sweep of one evaluation; the fourth-order operator adds a second
+// sweep, which takes over the half-kick
+template <typename S, bool KICK, typename NB, int MW, bool PK>
+void queue_force(const ThreedArgs<S>& a, NB nb, const int64_t* orig, S* acc_out, double half_dt,
+                 double* vel_out, hipStream_t stream) {
+  const unsigned g = grid_for(a.n);
+  if (a.order4) {
+    k3_force<S, false, NB, MW, PK><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, 0.0, nullptr);
+    k3_visc4<S, KICK, NB, MW><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+  } else {
+    k3_force<S, KICK, NB, MW, PK><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+  }
+}
+
 struct ThreedPrm {
   double h, rho0, c0, gamma, nu, diff_scale, eig_floor, dt;
   double box[3], grav[3];
-  int periodic, has_grav, renormalise, diff_form, adami;
+  int periodic, has_grav, renormalise, diff_form, adami, order4;
 };
 
 inline ThreedPrm threed_prm(const tsph_threed_params& q) {
@@ -346,6 +443,7 @@ inline ThreedPrm threed_prm(const tsph_threed_params& q) {
   r.diff_scale = q.diffusion_scale; r.eig_floor = q.eigenvalue_floor; r.dt = q.dt;
   r.periodic = q.periodic; r.has_grav = q.has_gravity; r.renormalise = q.renormalise;
   r.diff_form = q.diffusion_form; r.adami = q.wall_model == 0;
+  r.order4 = q.viscous_order == 4;
   for (int d = 0; d < 3; ++d) { r.box[d] = q.periodic ? q.box[d] : 1.0; r.grav[d] = q.has_gravity ? q.gravity[d] : 0.0; }
   return r;
 }
@@ -372,6 +470,7 @@ ThreedArgs<S> make_threed_args(const ThreedPrm& q, int64_t n, int64_t nf) {
   a.has_gravity = q.has_grav;
   a.adami = q.adami;
   a.do_pressure = 1;
+  a.order4 = q.order4 && q.nu != 0.0;
   a.n = n;
   a.nf = nf;
   return a;
@@ -381,6 +480,7 @@ ThreedArgs<S> make_threed_args(const ThreedPrm& q, int64_t n, int64_t nf) {
 template <typename S> struct ThreedScratch {
   S *rho_raw, *rho, *rho_u, *p, *moment, *gr, *rho_lo;   // rho_lo: NULL in FP64 mode
   Vec4<S>* pack;
+  S *fourth, *lap1;   // fourth-order viscous operator only (n + 1, 3 (n + 1)); else NULL
 };
 
 // Queues the sweeps of one accelerations() evaluation over a neighbour source
@@ -392,7 +492,7 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   const int64_t n = a.n;
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
-  a.pack = b.pack;
+  a.pack = b.pack; a.fourth = b.fourth; a.lap1 = b.lap1;
   const unsigned g = grid_for(n);
   const bool diff = q.diff_scale > 0.0;
   // with support lists (slot lists only) the density sweep writes them and
@@ -419,13 +519,13 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
     k3_eos<S, 0, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho, 0);
   }
   if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, nb);
-  if (a.nu != S(0) && a.renormalise) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
+  if (a.nu != S(0) && (a.renormalise || a.order4)) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
   if (uniform_mass)
-    k3_force<S, KICK, NB, 2, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    queue_force<S, KICK, NB, 2, true>(a, nb, orig, acc_out, half_dt, vel_out, stream);
   else if (orig)
-    k3_force<S, KICK, NB, 1, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    queue_force<S, KICK, NB, 1, true>(a, nb, orig, acc_out, half_dt, vel_out, stream);
   else
-    k3_force<S, KICK, NB, 0, true><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
+    queue_force<S, KICK, NB, 0, true>(a, nb, orig, acc_out, half_dt, vel_out, stream);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }
@@ -441,6 +541,9 @@ int threed_ctx_scratch(tsph_ctx* ctx, int64_t n, ThreedScratch<S>& b) {
   TSPH_TRY(ctx->s_gr.ensure(3 * s));
   TSPH_TRY(ctx->s_aux3.ensure(s));
   TSPH_TRY(ctx->s_pack.ensure(4 * s));
+  TSPH_TRY(ctx->s_four.ensure(s));
+  TSPH_TRY(ctx->s_lap1.ensure(3 * s));
+  b.fourth = ctx->s_four.as<S>(); b.lap1 = ctx->s_lap1.as<S>();
   b.rho_lo = sizeof(S) == 4 ? ctx->s_aux3.as<S>() : nullptr;
   b.pack = ctx->s_pack.as<Vec4<S>>();
   b.rho_raw = ctx->s_rho_raw.as<S>(); b.rho = ctx->s_rho.as<S>(); b.rho_u = ctx->s_aux.as<S>();
@@ -477,6 +580,7 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
   ThreedArgs<S> a = make_threed_args<S>(q, n, nf);
   a.pos = pos; a.vel = vel; a.mass = (const S*)mass;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr;
+  a.fourth = b.fourth; a.lap1 = b.lap1;
   const NbList nb{ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>()};
   const unsigned g = grid_for(n);
   S* rho_in = (S*)rho;           // read only
@@ -486,13 +590,13 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
       k3_density<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, (S*)out, nullptr, nullptr, 0);
       break;
     case 1:
-      a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0;
+      a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0; a.order4 = 0;
       k3_force<S, false, NbList, 0, false><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
       break;
     case 2:
       a.rho_u = rho_in; a.do_pressure = 0; a.has_gravity = 0;
-      if (a.nu != S(0) && a.renormalise) k3_moment<S, NbList><<<g, kBlock, 0, stream>>>(a, nb);
-      k3_force<S, false, NbList, 0, false><<<g, kBlock, 0, stream>>>(a, nb, nullptr, (S*)out, 0.0, nullptr);
+      if (a.nu != S(0) && (a.renormalise || a.order4)) k3_moment<S, NbList><<<g, kBlock, 0, stream>>>(a, nb);
+      queue_force<S, false, NbList, 0, false>(a, nb, nullptr, (S*)out, 0.0, nullptr, stream);
       break;
     case 3:
       if (nf == n) return TSPH_OK;
@@ -711,6 +815,7 @@ struct tsph_psim {
   DevBuf slot_idx, slot_cnt, slot_max;       // per-row lists in fixed slots (NbSlots layout)
   DevBuf slot_posf;                          // FP32 build positions (list prefilter)
   DevBuf sup_idx, sup_cnt;                   // this evaluation's entries within the support
+  DevBuf fourth, lap1;                       // fourth-order viscous operator only
   int64_t slot_K = 0;
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
@@ -728,6 +833,7 @@ template <typename S> ThreedScratch<S> psim_scratch(tsph_psim* ps) {
   b.p = ps->p.as<S>(); b.moment = ps->moment.as<S>(); b.gr = ps->gr.as<S>();
   b.rho_lo = sizeof(S) == 4 ? ps->rho_lo.as<S>() : nullptr;
   b.pack = ps->pack.as<Vec4<S>>();
+  b.fourth = ps->fourth.as<S>(); b.lap1 = ps->lap1.as<S>();
   return b;
 }
 
@@ -859,6 +965,8 @@ int tsph_threed_accelerations(tsph_ctx* ctx, const tsph_threed_params* prm, cons
   TSPH_REQUIRE(prm->h > 0.0 && prm->rho0 > 0.0 && prm->gamma > 0.0, TSPH_ERR_ARG,
                "h, rho0 and gamma must be positive");
   TSPH_REQUIRE(prm->wall_model == 0 || prm->wall_model == 1, TSPH_ERR_ARG, "wall_model must be 0 or 1");
+  TSPH_REQUIRE(prm->viscous_order == 0 || prm->viscous_order == 2 || prm->viscous_order == 4, TSPH_ERR_ARG,
+               "viscous_order must be 2 or 4");
   TSPH_REQUIRE(prm->diffusion_scale <= 0.0 || prm->dt > 0.0, TSPH_ERR_ARG,
                "density_diffusion > 0 requires a positive dt");
   if (prm->periodic)
@@ -880,6 +988,8 @@ int tsph_threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, c
                "neighbour lists hold %lld targets; the operator needs lists for all %lld particles "
                "(tsph_nlist_from_pairs with all_targets = 1)", (long long)ctx->nb_targets, (long long)prm->n);
   TSPH_REQUIRE(prm->h > 0.0, TSPH_ERR_ARG, "h must be positive");
+  TSPH_REQUIRE(prm->viscous_order == 0 || prm->viscous_order == 2 || prm->viscous_order == 4, TSPH_ERR_ARG,
+               "viscous_order must be 2 or 4");
   if (prm->periodic)
     for (int d = 0; d < 3; ++d) TSPH_REQUIRE(prm->box[d] > 0.0, TSPH_ERR_ARG, "box edges must be positive");
   if (prm->n > 0) {
@@ -924,7 +1034,7 @@ int tsph_psim_destroy(tsph_psim* ps) {
                  &ps->pos2, &ps->vel2, &ps->mass2, &ps->acc2, &ps->orig2, &ps->cells2, &ps->part, &ps->diag,
                  &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr, &ps->rho_lo, &ps->pack,
                  &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max, &ps->slot_posf,
-                 &ps->sup_idx, &ps->sup_cnt};
+                 &ps->sup_idx, &ps->sup_cnt, &ps->fourth, &ps->lap1};
   for (DevBuf* x : b) x->release();
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);
@@ -941,6 +1051,8 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
   TSPH_REQUIRE(prm->n < (1ll << 31) - 1, TSPH_ERR_ARG, "too many particles for int32 neighbour lists");
   TSPH_REQUIRE(prm->precision == TSPH_F64 || prm->precision == TSPH_F32, TSPH_ERR_ARG, "bad precision");
   TSPH_REQUIRE(skin > 0.0 && prm->dt > 0.0 && prm->h > 0.0, TSPH_ERR_ARG, "skin, dt and h must be positive");
+  TSPH_REQUIRE(prm->viscous_order == 0 || prm->viscous_order == 2 || prm->viscous_order == 4, TSPH_ERR_ARG,
+               "viscous_order must be 2 or 4");
   for (int d = 0; d < 3; ++d)
     TSPH_REQUIRE(prm->box[d] > 0.0, TSPH_ERR_ARG, "box edges must be positive");
   tsph_psim* ps = new tsph_psim();
@@ -961,6 +1073,8 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
                           8 * DG_N * ((size_t)grid_for(ps->n) + 1), 8 * DG_N, s * (n + 1), s * (n + 1), s * (n + 1),
                           s * (n + 1), s * (n + 1), 3 * s * (n + 1), s * (n + 1), 4 * s * (n + 1)};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]) && rc == TSPH_OK; ++k) rc = bufs[k]->ensure(sizes[k]);
+  if (rc == TSPH_OK && ps->q.order4) rc = ps->fourth.ensure(s * (n + 1));
+  if (rc == TSPH_OK && ps->q.order4) rc = ps->lap1.ensure(3 * s * (n + 1));
   if (rc == TSPH_OK && hipHostMalloc((void**)&ps->disp_host, 8, 0) != hipSuccess) {
     ps->disp_host = nullptr;
     set_error("hipHostMalloc of the displacement word failed");

@@ -3,7 +3,7 @@ driver of tensorsph.threed (threed/__init__.py:31-91), on libtsph.
 
 Not built here (outside the particle pipeline): the verification helpers
 laplacian_of_field / gradient_of_field / the moment diagnostics,
-viscous_order=4, run_hydrostatic, wall_shell, the surface-tension modules.
+run_hydrostatic, wall_shell, the surface-tension modules.
 """
 from .conservation import (angular_momentum, centre_of_mass, conservation_row, diagnostics,
                            internal_energy, kinetic_energy, mass_total, momentum, relative_drift,

@@ -49,7 +49,7 @@ def _box3(box):
 
 def _params(code, n, n_fluid, h, *, rho0=1.0, sound_speed=1.0, gamma=7.0, viscosity=0.0,
             gravity=None, box=None, renormalise=True, diffusion_scale=0.0, diffusion_form=1,
-            eigenvalue_floor=0.1, dt=0.0, wall_model=0) -> _lib.ThreedParams:
+            eigenvalue_floor=0.1, dt=0.0, wall_model=0, viscous_order=2) -> _lib.ThreedParams:
     p = _lib.ThreedParams()
     p.precision = code
     p.n, p.n_fluid = int(n), int(n_fluid)
@@ -73,6 +73,9 @@ def _params(code, n, n_fluid, h, *, rho0=1.0, sound_speed=1.0, gamma=7.0, viscos
     p.eigenvalue_floor = float(eigenvalue_floor)
     p.dt = float(dt)
     p.wall_model = int(wall_model)
+    if int(viscous_order) not in (2, 4):
+        raise ValueError(f"viscous_order must be 2 or 4, got {viscous_order!r}")
+    p.viscous_order = int(viscous_order)
     return p
 
 
@@ -152,14 +155,16 @@ def viscous_acceleration(pos: torch.Tensor, vel: torch.Tensor, mass: torch.Tenso
                          rho: torch.Tensor, nu: float, h: float, pairs: torch.Tensor,
                          box=None, renormalise: bool = True, order: int = 2) -> torch.Tensor:
     """nu sum_j m_j (v_j - v_i) lap W_ij / rho_ij, divided by K_ij/2 when
-    renormalise (operators3.py:116-154).  order = 4 is not built."""
-    if order != 2:
-        raise NotImplementedError("viscous_acceleration(order=4) is not available on the GPU path")
+    renormalise (operators3.py:116-154).  order = 4 applies the iterated
+    operator L v - L(beta L v), beta_ij = (I4_i + I4_j) / 240 with I4 the
+    fourth moment sum_j r^4 lap W_ij m_j / rho_ij (:146-151)."""
+    if order not in (2, 4):
+        raise ValueError(f"order must be 2 or 4, got {order!r}")
     if nu == 0.0:
         _lib.require_gpu(pos)
         return torch.zeros_like(pos)
     return _operator(OP_VISCOUS, pos, mass, h, pairs, vel=vel, rho=rho, box=box, viscosity=nu,
-                     renormalise=renormalise, out_shape=(pos.shape[0], 3))
+                     renormalise=renormalise, viscous_order=order, out_shape=(pos.shape[0], 3))
 
 
 def wall_pressure_adami(pos: torch.Tensor, mass: torch.Tensor, rho: torch.Tensor,
@@ -206,8 +211,8 @@ def accelerations(pos: torch.Tensor, vel: torch.Tensor, mass: torch.Tensor, h: f
         raise ValueError("density_diffusion > 0 requires a positive dt")
     if n_fluid < n and wall_model not in _WALL_MODELS:
         raise ValueError(f"unknown wall_model {wall_model!r}")
-    if viscous_order != 2 and viscosity != 0.0:
-        raise NotImplementedError("viscous_order=4 is not available on the GPU path")
+    if viscous_order not in (2, 4):
+        raise ValueError(f"viscous_order must be 2 or 4, got {viscous_order!r}")
     if density_diffusion > 0.0 and density_diffusion_form not in _FORMS:
         raise ValueError(f"unknown density_diffusion_form {density_diffusion_form!r}; "
                          f"expected one of {tuple(_FORMS)}")
@@ -220,7 +225,7 @@ def accelerations(pos: torch.Tensor, vel: torch.Tensor, mass: torch.Tensor, h: f
                   diffusion_form=_FORMS.get(density_diffusion_form, 1),
                   eigenvalue_floor=density_diffusion_eigenvalue_floor,
                   dt=0.0 if dt is None else dt,
-                  wall_model=_WALL_MODELS.get(wall_model, 0))
+                  wall_model=_WALL_MODELS.get(wall_model, 0), viscous_order=viscous_order)
     acc = torch.empty((n, 3), dtype=sdt, device=dev)
     rho = torch.empty(n, dtype=sdt, device=dev)
     p = torch.empty(n, dtype=sdt, device=dev)

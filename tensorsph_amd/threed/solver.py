@@ -48,7 +48,7 @@ class PeriodicSimulation:
                  gamma: float = 7.0, viscosity: float = 0.0, dt: float, skin: float,
                  renormalise: bool = True, density_diffusion: float = 0.0,
                  density_diffusion_form: str = "antuono", eigenvalue_floor: float = 0.1,
-                 precision: str = "fp64", device=None):
+                 precision: str = "fp64", device=None, viscous_order: int = 2):
         if precision not in ("fp64", "fp32"):
             raise ValueError("precision must be 'fp64' or 'fp32'")
         dev = torch.device(device) if device is not None else torch.device("cuda", torch.cuda.current_device())
@@ -78,7 +78,7 @@ class PeriodicSimulation:
                            diffusion_scale=(2.0 * density_diffusion * sound_speed * h
                                             if density_diffusion > 0.0 else 0.0),
                            diffusion_form=_FORMS.get(density_diffusion_form, 1),
-                           eigenvalue_floor=eigenvalue_floor, dt=dt)
+                           eigenvalue_floor=eigenvalue_floor, dt=dt, viscous_order=viscous_order)
         handle = C.c_void_p()
         with torch.cuda.device(dev):
             _lib.check(_lib.lib().tsph_psim_create(C.byref(self.prm), self.skin, _lib.ptr(self._x),
@@ -237,8 +237,6 @@ def run_shear_wave(*, n: int = 16, box: float = 1.0, h_factor: float = 1.3, peri
     PCG64(seed).  dtype float64 runs the FP64 kernels, float32 the
     FP32-compensated ones (positions stay FP64 on the device).
     """
-    if viscous_order != 2:
-        raise NotImplementedError("viscous_order=4 is not available on the GPU path")
     dx = box / n
     h = h_factor * dx
     wavenumber = 2.0 * math.pi * periods / box
@@ -266,7 +264,8 @@ def run_shear_wave(*, n: int = 16, box: float = 1.0, h_factor: float = 1.3, peri
     skin = cutoff - 2.0 * h
     sim = PeriodicSimulation(pos, vel, mass, h=h, box=box, rho0=rho0, sound_speed=sound_speed,
                              gamma=gamma, viscosity=nu, dt=dt, skin=skin, renormalise=renormalise,
-                             precision="fp64" if dtype == torch.float64 else "fp32", device=device)
+                             precision="fp64" if dtype == torch.float64 else "fp32", device=device,
+                             viscous_order=viscous_order)
     try:
         result = ShearWaveResult(n=n, dx=dx, h=h, nu=nu, wavenumber=wavenumber,
                                  sound_speed=sound_speed, amplitude0=amplitude, dt=dt, steps=steps,

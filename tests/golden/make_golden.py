@@ -22,6 +22,7 @@ Fixtures
                      dam_break_3d.py:185-283, kleefsman_obstacle_3d.py:187-205
   run2d.npz          droplet_impact.run rows with output_every = 1 (meters per
                      step) and the final snapshot - droplet_impact.py:1607-1795
+  threed_order4.npz  the same operators with the fourth-order viscous term
   threed.npz         threed.operators3.accelerations (periodic, renormalised,
                      walls + Adami, Antuono diffusion) - operators3.py:51-295
   rhs3d.npz          dam_break_3d.rhs_3d with wall forces - dam_break_3d.py:290-422
@@ -353,10 +354,59 @@ def shear(reference: str = "/root/reference") -> None:
     print("shear fixture written to", HERE)
 
 
+def order4(reference: str = "/root/reference") -> None:
+    """threed.operators3 with the fourth-order viscous correction
+    (operators3.py:146-151): accelerations(viscous_order=4) and
+    viscous_acceleration(order=4) on a jittered periodic lattice with
+    unequal masses, renormalised and not, and a short shear-wave run with
+    viscous_order=4."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph.threed.neighbors3 import periodic_pairs
+    from tensorsph.threed import operators3 as op3
+    from tensorsph.threed import kernel3 as k3
+    from tensorsph.threed.solver import cubic_lattice, run_shear_wave
+    torch.set_default_dtype(torch.float64)
+    n = 8
+    dx = 1.0 / n
+    h = 1.3 * dx
+    lat = cubic_lattice((n, n, n), dx) + 0.5 * dx
+    rng = np.random.Generator(np.random.PCG64(20261204))
+    lat = (lat + torch.from_numpy(rng.uniform(-0.05 * dx, 0.05 * dx, size=lat.shape))).remainder(1.0)
+    mass = torch.full((lat.shape[0],), k3.calibrated_particle_mass(dx, h, 1.0))
+    mass = mass * (1.0 + 0.01 * torch.from_numpy(rng.uniform(-1.0, 1.0, size=mass.shape)))
+    vel = torch.zeros_like(lat)
+    vel[:, 0] = 1e-3 * torch.sin(2 * math.pi * lat[:, 2])
+    vel[:, 1] = 3e-4 * torch.cos(2 * math.pi * lat[:, 0])
+    vel[:, 2] = 2e-4 * torch.sin(4 * math.pi * lat[:, 1])
+    pairs = periodic_pairs(lat, 1.0, 2.0 * h * 1.25)
+    out = dict(pos=_np(lat), vel=_np(vel), mass=_np(mass), pairs=_np(pairs), h=np.array(h))
+    for tag, ren in (("ren", True), ("raw", False)):
+        acc, rho, p = op3.accelerations(lat, vel, mass, h, pairs, rho0=1.0, sound_speed=10.0,
+                                        gamma=7.0, viscosity=0.01, box=1.0, renormalise=ren,
+                                        viscous_order=4)
+        out[f"{tag}_acc"] = _np(acc)
+        out[f"{tag}_rho"] = _np(rho)
+        out[f"{tag}_visc"] = _np(op3.viscous_acceleration(lat, vel, mass, rho, 0.01, h, pairs, box=1.0,
+                                                          renormalise=ren, order=4))
+    # run_shear_wave with the fourth-order operator (solver.py:211-308)
+    res = run_shear_wave(n=10, amplitude=0.5, steps=60, sample_every=5, viscous_order=4)
+    out["shear_times"] = np.array(res.times)
+    out["shear_amplitudes"] = np.array(res.amplitudes)
+    out["shear_ke"] = np.array([d["kinetic_energy"] for d in res.diagnostics])
+    out["shear_momentum"] = np.array([list(d["momentum"]) for d in res.diagnostics])
+    out["shear_rebuilds"] = np.int64(res.pair_rebuilds)
+    np.savez_compressed(HERE / "threed_order4.npz", **out)
+    print("order-4 fixture written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
         shear(*args[1:])
+    elif args and args[0] == "--only-order4":
+        order4(*args[1:])
     else:
         main(*args)
         shear(*args)
+        order4(*args)

@@ -6,6 +6,8 @@ own bars of the reference's golden outputs; FP32-compensated within 1e-5
 reproduces the reference's run_shear_wave series (amplitudes to 1e-10,
 identical list builds and pair count) and the oracle loop on a jittered
 single-sine field (positions to 1e-10); two FP32 runs bitwise identical.
+The fourth-order viscous operator is held to the same bars against the
+oracle and the reference's order-4 fixture.
 """
 import math
 
@@ -136,6 +138,58 @@ def test_single_operators_fp64(golden):
     assert np.abs(n(rate) - ref).max() < 1e-6 * np.abs(ref).max() + 1e-9
 
 
+@pytest.mark.parametrize("ren", [True, False])
+def test_order4_accelerations_and_viscous_operator(golden, ren):
+    """Fourth-order viscous operator (operators3.py:146-151), unequal masses:
+    FP64 against the oracle (1e-12) and the reference (1e-10); FP32c against
+    the oracle on the FP32-rounded state (1e-5 on acc, 1e-4 on the viscous
+    term alone, whose second pass differences the first pass's output)."""
+    from tensorsph_amd.threed import accelerations, viscous_acceleration
+    g = golden("threed_order4")
+    h = float(g["h"])
+    tag = "ren" if ren else "raw"
+    kw = dict(rho0=1.0, sound_speed=10.0, box=1.0, renormalise=ren, viscous_order=4)
+    acc, rho, p = accelerations(t(g["pos"]), t(g["vel"]), t(g["mass"]), h, t(g["pairs"]),
+                                viscosity=0.01, **kw)
+    oacc, orho, op = orc.threed_accelerations(g["pos"], g["vel"], g["mass"], h, g["pairs"],
+                                              viscosity=0.01, **kw)
+    a0, _, _ = orc.threed_accelerations(g["pos"], g["vel"], g["mass"], h, g["pairs"],
+                                        viscosity=0.0, **kw)
+    assert scaled_err(n(acc), oacc) < 1e-12
+    assert scaled_err(n(acc), g[f"{tag}_acc"]) < 1e-10
+    va = viscous_acceleration(t(g["pos"]), t(g["vel"]), t(g["mass"]), t(orho), 0.01, h,
+                              t(g["pairs"]), box=1.0, renormalise=ren, order=4)
+    assert scaled_err(n(va), oacc - a0) < 1e-10
+    assert scaled_err(n(va), g[f"{tag}_visc"]) < 1e-9
+    pos32, vel32, m32 = (g[k].astype(np.float32) for k in ("pos", "vel", "mass"))
+    acc32, _, _ = accelerations(t(pos32), t(vel32), t(m32), h, t(g["pairs"]), viscosity=0.01, **kw)
+    args = (pos32.astype(np.float64), vel32.astype(np.float64), m32.astype(np.float64), h, g["pairs"])
+    o1, orho32, _ = orc.threed_accelerations(*args, viscosity=0.01, **kw)
+    o0, _, _ = orc.threed_accelerations(*args, viscosity=0.0, **kw)
+    assert scaled_err(n(acc32), o1) < 1e-5
+    va32 = viscous_acceleration(t(pos32), t(vel32), t(m32), t(orho32.astype(np.float32)), 0.01, h,
+                                t(g["pairs"]), box=1.0, renormalise=ren, order=4)
+    assert scaled_err(n(va32), o1 - o0) < 1e-4
+
+
+def test_order4_walls_gravity_fp64(golden):
+    """Order 4 with Adami walls and gravity: the wall rows' first Laplacian
+    enters the fluid rows' second pass."""
+    from tensorsph_amd.threed import accelerations
+    g = golden("threed")
+    kw = dict(rho0=1000.0, sound_speed=20.0, viscosity=1e-3, gravity=[0.0, 0.0, -9.81],
+              n_fluid=int(g["wall_nf"]), renormalise=True, viscous_order=4)
+    acc, rho, p = accelerations(t(g["wall_pos"]), t(g["wall_vel"]), t(g["wall_mass"]),
+                                float(g["wall_h"]), t(g["wall_pairs"]), **kw)
+    oacc, orho, op = orc.threed_accelerations(g["wall_pos"], g["wall_vel"], g["wall_mass"],
+                                              float(g["wall_h"]), g["wall_pairs"], **kw)
+    o2, _, _ = orc.threed_accelerations(g["wall_pos"], g["wall_vel"], g["wall_mass"],
+                                        float(g["wall_h"]), g["wall_pairs"],
+                                        **dict(kw, viscous_order=2))
+    assert scaled_err(oacc, o2) > 1e-9            # the correction is seen at this bar
+    assert scaled_err(n(acc), oacc) < 1e-11
+
+
 def test_operator_errors():
     from tensorsph_amd.threed import accelerations, viscous_acceleration
     pos = torch.zeros((4, 3), dtype=torch.float64)
@@ -150,10 +204,10 @@ def test_operator_errors():
     with pytest.raises(ValueError):
         accelerations(g, g, torch.ones(4, dtype=torch.float64, device="cuda"), 0.1,
                       torch.tensor([[0, 9]], device="cuda"), rho0=1.0, sound_speed=1.0)
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError):
         viscous_acceleration(g, g, torch.ones(4, dtype=torch.float64, device="cuda"),
                              torch.ones(4, dtype=torch.float64, device="cuda"), 0.1, 0.1,
-                             pairs.cuda(), order=4)
+                             pairs.cuda(), order=3)
 
 
 def test_conservation_row_matches_numpy():
@@ -199,11 +253,46 @@ def test_shear_wave_matches_reference_run(golden):
     assert abs(res.fit_rate - float(g["fit_rate"])) < 1e-8 * float(g["fit_rate"])
 
 
-def _psim(pos, vel, mass, h, precision, nu=0.01, c0=10.0):
+def test_shear_wave_order4_matches_reference_run(golden):
+    """run_shear_wave(viscous_order=4) against the reference's run at n = 10."""
+    from tensorsph_amd.threed import run_shear_wave
+    g4 = golden("threed_order4")
+    for dtype, bar in ((torch.float64, 1e-10), (torch.float32, 1e-5)):
+        res = run_shear_wave(n=10, amplitude=0.5, steps=60, sample_every=5, viscous_order=4,
+                             dtype=dtype)
+        assert np.abs(np.array(res.amplitudes) - g4["shear_amplitudes"]).max() < bar * 0.5, dtype
+        ke = np.array([d["kinetic_energy"] for d in res.diagnostics])
+        assert np.abs(ke - g4["shear_ke"]).max() < 10 * bar * ke[0], dtype
+        assert res.pair_rebuilds == int(g4["shear_rebuilds"])
+        assert res.as_dict()["viscous_operator_order"] == 4
+
+
+def test_periodic_stepper_order4_unequal_masses_vs_oracle():
+    """The resident stepper with order 4 on a jittered lattice with unequal
+    masses (the member-mass rule maps rows through the caller's order)."""
+    pos, vel, mass, h, k = _shear_state(n=12, amplitude=0.3, mode="single", jitter=0.01)
+    mass = mass * (1.0 + 0.01 * np.sin(np.arange(len(mass))))
+    sim, dt = _psim(pos, vel, mass, h, "fp64", viscous_order=4)
+    steps = 30
+    sim.step(steps)
+    st = sim.read()
+    ref = orc.shear_wave_loop(pos, vel, mass, h=h, box=1.0, rho0=1000.0, sound_speed=10.0,
+                              nu=0.01, dt=dt, steps=steps, skin=0.5 * h, sample_every=steps,
+                              wavenumber=k, viscous_order=4)
+    d = n(st["position"]) - ref["pos"]
+    d -= np.round(d)
+    assert np.abs(d).max() < 1e-10
+    assert scaled_err(n(st["velocity"]), ref["vel"]) < 1e-9
+    assert scaled_err(n(st["acceleration"]), ref["acc"]) < 1e-8
+    sim.close()
+
+
+def _psim(pos, vel, mass, h, precision, nu=0.01, c0=10.0, viscous_order=2):
     from tensorsph_amd.threed import PeriodicSimulation
     dt = min(0.25 * h / c0, 0.125 * h * h / nu)
     return PeriodicSimulation(pos, vel, mass, h=h, box=1.0, rho0=1000.0, sound_speed=c0,
-                              viscosity=nu, dt=dt, skin=0.5 * h, precision=precision), dt
+                              viscosity=nu, dt=dt, skin=0.5 * h, precision=precision,
+                              viscous_order=viscous_order), dt
 
 
 def test_periodic_stepper_jittered_single_sine_vs_oracle():
