@@ -103,6 +103,15 @@ typedef struct tsph_wcsph_params {
   double eigenvalue_floor;   /* Antuono eigenvalue floor (fraction of trace/D) */
   int32_t reaction_style;    /* 0 none; 1 2-D split loads; 2 rhs_3d wall force */
   int32_t has_wall_accel;
+  /* opt-in branches of droplet_impact.rhs (dim 2 only; zero = off) */
+  double wrap_width;                   /* > 0: x separations modulo wrap_width (:921-927) */
+  int32_t density_renormalization;     /* Shepard: rho lattice_kernel_sum / (sum_j W_ij + W(0)) (:954-964) */
+  double lattice_kernel_sum;           /* lattice_kernel_sum(h, dx) (:577-598) */
+  int32_t boundary_consistent_density; /* wall mass scaled by rho_w / rho0 in the fluid density (:1090-1112) */
+  int32_t surface_pressure_closure;    /* free-surface pressure extrapolation (:1031-1038, :429-492) */
+  double surface_pressure_threshold;   /* receivers: rho < threshold * rho0 */
+  int32_t surface_pressure_donors_all; /* 1 "all" (one pass), 0 "interior" */
+  int32_t surface_pressure_layers;     /* passes of "interior" */
 } tsph_wcsph_params;
 
 /* Replaces droplet_impact.rhs (droplet_impact.py:810-1228; dim 2) and
@@ -116,6 +125,18 @@ int tsph_wcsph_rhs(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
                    const double* vel, const void* mass, const double* wall_accel,
                    void* rho, void* p, void* acc, void* wall_p, void* wall_rho,
                    void* reaction, hipStream_t stream);
+
+/* droplet_impact.rhs with its per-particle inputs (droplet_impact.py:810-
+ * 1228, dim 2): density_offset (n_fluid, scalar type; :936-943) is added to
+ * the clamped summation density, nu_field (n_fluid, scalar type;
+ * :1164-1178) replaces the scalar viscosity per pair.  Either may be NULL;
+ * with both NULL this is tsph_wcsph_rhs.  The caller passes no extras when
+ * the pair list is empty, as the reference returns before them. */
+int tsph_wcsph_rhs_ex(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* pos,
+                      const double* vel, const void* mass, const double* wall_accel,
+                      const void* density_offset, const void* nu_field, void* rho, void* p,
+                      void* acc, void* wall_p, void* wall_rho, void* reaction,
+                      hipStream_t stream);
 
 /* Tait EOS p = max(0, B((max(rho/rho0, ratio_floor))^gamma - 1)), B = rho0 c0^2/gamma:
  * droplet_impact.tait_pressure (droplet_impact.py:415-427; ratio_floor 0.01)

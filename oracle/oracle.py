@@ -46,7 +46,11 @@ class MorrisParams(C.Structure):
                 ("gamma", C.c_double), ("gravity", C.c_double), ("nu", C.c_double),
                 ("wall_friction", C.c_double), ("dt", C.c_double),
                 ("diff_scale", C.c_double), ("diff_form", C.c_int),
-                ("eig_floor", C.c_double), ("reaction_style", C.c_int)]
+                ("eig_floor", C.c_double), ("reaction_style", C.c_int),
+                ("wrap_width", C.c_double), ("shepard", C.c_int), ("lattice_sum", C.c_double),
+                ("bcd", C.c_int), ("closure", C.c_int), ("closure_threshold", C.c_double),
+                ("closure_literal", C.c_int), ("closure_layers", C.c_int),
+                ("density_offset", C.c_void_p), ("nu_field", C.c_void_p)]
 
 
 class DPParams(C.Structure):
@@ -166,8 +170,13 @@ def tait2d(rho, rho0, c0, gamma):
 def rhs_morris(pos, vel, mass, wall, pairs, *, dim, h, rho0, c0, gamma, gravity, nu,
                wall_friction=0.0, dt=0.0, diff_scale=0.0, diff_form=0, eig_floor=0.1,
                wall_mass=None, wall_vel=None, wall_accel=None, kernel3d=False,
-               reaction_style=0):
-    """droplet_impact.rhs (2-D) / dam_break_3d.rhs_3d (3-D) in FP64.
+               reaction_style=0, wrap_width=0.0, shepard=False, lattice_sum=0.0, bcd=False,
+               closure=False, closure_threshold=0.99, closure_literal=False, closure_layers=2,
+               density_offset=None, nu_field=None):
+    """droplet_impact.rhs (2-D) / dam_break_3d.rhs_3d (3-D) in FP64, with the
+    2-D rhs's opt-in branches: wrap_width, density_offset, Shepard density
+    renormalisation (lattice_sum = lattice_kernel_sum(h, dx)), the
+    surface-pressure closure, boundary-consistent density and nu_field.
 
     Returns (rho, p, acc, wall_p, wall_rho, reaction)."""
     pos = np.asarray(pos, np.float64); vel = np.asarray(vel, np.float64)
@@ -181,8 +190,14 @@ def rhs_morris(pos, vel, mass, wall, pairs, *, dim, h, rho0, c0, gamma, gravity,
     wm = np.full(nw, 0.0 if wall_mass is None else float(wall_mass))
     mass_all = np.ascontiguousarray(np.concatenate([mass, wm]))
     pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    off = None if density_offset is None else np.ascontiguousarray(density_offset, np.float64)
+    nuf = None if nu_field is None else np.ascontiguousarray(nu_field, np.float64)
     P = MorrisParams(dim, 1 if kernel3d else 0, h, rho0, c0, gamma, gravity, nu,
-                     wall_friction, dt, diff_scale, diff_form, eig_floor, reaction_style)
+                     wall_friction, dt, diff_scale, diff_form, eig_floor, reaction_style,
+                     float(wrap_width), int(bool(shepard)), float(lattice_sum), int(bool(bcd)),
+                     int(bool(closure)), float(closure_threshold), int(bool(closure_literal)),
+                     int(closure_layers), None if off is None else off.ctypes.data,
+                     None if nuf is None else nuf.ctypes.data)
     rho = np.zeros(n); p = np.zeros(n); acc = np.zeros((n, dim))
     wp = np.zeros(nw); wr = np.zeros(nw); reac = np.zeros((nw, dim))
     wa = None if wall_accel is None else np.ascontiguousarray(wall_accel, np.float64)

@@ -445,6 +445,17 @@ typedef struct {
     double h, rho0, c0, gamma, gravity, nu, wall_friction, dt;
     double diff_scale; int diff_form; double eig_floor;
     int reaction_style;   /* 0: none, 1: 2-D split (face pressure + reaction), 2: rhs_3d */
+    /* opt-in branches of droplet_impact.rhs (2-D) */
+    double wrap_width;    /* > 0: x separations modulo wrap_width, :921-927 */
+    int shepard;          /* density_renormalization, :954-964 */
+    double lattice_sum;   /* full-lattice kernel sum for shepard */
+    int bcd;              /* boundary_consistent_density, :1090-1112 */
+    int closure;          /* surface_pressure_closure, :1031-1038 and :429-492 */
+    double closure_threshold;
+    int closure_literal;  /* donors "all" */
+    int closure_layers;
+    const double* density_offset;   /* n or NULL, :936-943 */
+    const double* nu_field;         /* n or NULL, :1164-1178 */
 } orc_morris_params;
 
 static double tait2d(double rho, double rho0, double c0, double gamma) {
@@ -494,7 +505,11 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
     for (int64_t k = 0; k < npairs; ++k) {
         int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
         double r2 = 0;
-        for (int d = 0; d < dim; ++d) { double df = pos_all[i * dim + d] - pos_all[j * dim + d]; delta[k * dim + d] = df; r2 += df * df; }
+        for (int d = 0; d < dim; ++d) {
+            double df = pos_all[i * dim + d] - pos_all[j * dim + d];
+            if (d == 0 && P->wrap_width > 0.0) df = df - P->wrap_width * nearbyint(df / P->wrap_width);
+            delta[k * dim + d] = df; r2 += df * df;
+        }
         double r = sqrt(r2), w, s;
         if (P->kernel3d) cubic3d(r, h, &w, &s, NULL); else cubic2d(r, h, &w, &s);
         dist[k] = r; wij[k] = w;
@@ -504,6 +519,19 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
     for (int64_t k = 0; k < npairs; ++k) rho_all[pairs[2 * k + 1]] += mass_all[pairs[2 * k]] * wij[k];
     double* rho = (double*)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     for (int64_t a = 0; a < n; ++a) { double r = rho_all[a]; rho[a] = r < 0.1 * P->rho0 ? 0.1 * P->rho0 : r; }
+    if (P->density_offset)
+        for (int64_t a = 0; a < n; ++a) { double r = rho[a] + P->density_offset[a]; rho[a] = r < 0.1 * P->rho0 ? 0.1 * P->rho0 : r; }
+    if (P->shepard) {
+        /* Shepard factor: full-lattice kernel sum over this particle's sum_j W_ij + W(0) */
+        double* kt = (double*)calloc((size_t)nt, sizeof(double));
+        for (int64_t k = 0; k < npairs; ++k) kt[pairs[2 * k]] += wij[k];
+        for (int64_t k = 0; k < npairs; ++k) kt[pairs[2 * k + 1]] += wij[k];
+        for (int64_t a = 0; a < n; ++a) {
+            double t = kt[a] + w0; if (t < 1.0e-12) t = 1.0e-12;
+            double r = rho[a] * (P->lattice_sum / t); rho[a] = r < 0.1 * P->rho0 ? 0.1 * P->rho0 : r;
+        }
+        free(kt);
+    }
 
     if (P->diff_scale > 0.0) {
         int64_t nf = 0;
@@ -528,10 +556,64 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
     double* d_all = (double*)calloc((size_t)nt, sizeof(double));
     for (int64_t a = 0; a < n; ++a) { p_all[a] = tait2d(rho[a], P->rho0, P->c0, P->gamma); d_all[a] = rho[a]; }
 
+    if (P->closure && n > 0) {
+        /* free_surface_pressure over fluid-fluid pairs: receivers are rows
+         * below threshold * rho0; each pass fills receivers from donors known
+         * at the pass start ("interior") or from every neighbour ("all") */
+        const double thr = P->closure_threshold * P->rho0;
+        int* known = (int*)malloc((size_t)n * sizeof(int));
+        int* recv = (int*)malloc((size_t)n * sizeof(int));
+        double* num = (double*)malloc((size_t)n * sizeof(double));
+        double* den = (double*)malloc((size_t)n * sizeof(double));
+        int any = 0;
+        for (int64_t a = 0; a < n; ++a) { known[a] = !(rho[a] < thr); any |= !known[a]; }
+        const int layers = P->closure_literal ? 1 : (P->closure_layers > 1 ? P->closure_layers : 1);
+        for (int pass = 0; any && pass < layers; ++pass) {
+            int left = 0;
+            for (int64_t a = 0; a < n; ++a) { recv[a] = !known[a]; left |= recv[a]; num[a] = 0.0; den[a] = 0.0; }
+            if (!left) break;
+            for (int64_t k = 0; k < npairs; ++k) {
+                int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+                if (i >= n || j >= n) continue;
+                double wi = P->closure_literal ? wij[k] : wij[k] * (double)known[j];
+                double wj = P->closure_literal ? wij[k] : wij[k] * (double)known[i];
+                double grav = P->gravity * (dim > 1 ? delta[k * dim + 1] : 0.0);
+                num[i] += wi * (p_all[j] - grav * rho[j]); den[i] += wi;
+                num[j] += wj * (p_all[i] + grav * rho[i]); den[j] += wj;
+            }
+            for (int64_t a = 0; a < n; ++a)
+                if (recv[a] && den[a] > 1.0e-30) {
+                    double v = num[a] / den[a]; p_all[a] = v < 0.0 ? 0.0 : v; known[a] = 1;
+                }
+        }
+        free(known); free(recv); free(num); free(den);
+    }
+
     if (nw) {
         double bulk = P->rho0 * (P->c0 * P->c0) / P->gamma;
         double* num = (double*)calloc((size_t)nw, sizeof(double));
         double* den = (double*)calloc((size_t)nw, sizeof(double));
+        for (int bcd_pass = 0; bcd_pass < (P->bcd ? 2 : 1); ++bcd_pass) {
+        if (bcd_pass == 1) {
+            /* boundary-consistent density: wall masses scaled by rho_w / rho0
+             * enter the fluid summation density, then Tait and a second
+             * extrapolation from the corrected fluid state */
+            double* corr = (double*)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
+            for (int64_t k = 0; k < npairs; ++k) {
+                int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
+                int ifl = i < n, jfl = j < n;
+                if (ifl == jfl) continue;
+                int64_t f = ifl ? i : j, w = ifl ? j : i;
+                double mw = mass_all[w];
+                corr[f] += (mw * (d_all[w] / P->rho0) - mw) * wij[k];
+            }
+            for (int64_t a = 0; a < n; ++a) {
+                double r = rho[a] + corr[a]; rho[a] = r < 0.1 * P->rho0 ? 0.1 * P->rho0 : r;
+                p_all[a] = tait2d(rho[a], P->rho0, P->c0, P->gamma); d_all[a] = rho[a];
+            }
+            free(corr);
+            for (int64_t w = 0; w < nw; ++w) { num[w] = 0.0; den[w] = 0.0; }
+        }
         for (int64_t k = 0; k < npairs; ++k) {
             int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
             int ifl = i < n, jfl = j < n;
@@ -563,6 +645,7 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
             if (wall_p) wall_p[w] = pw;
             if (wall_rho) wall_rho[w] = rw;
         }
+        }
         free(num); free(den);
     }
 
@@ -588,10 +671,15 @@ ORC_API void orc_rhs_morris(const orc_morris_params* P, int64_t n, int64_t nw,
         int64_t i = pairs[2 * k], j = pairs[2 * k + 1];
         int fluid_pair = (i < n) && (j < n);
         int crossing = (i < n) != (j < n);
+        double nu = P->nu;
+        if (P->nu_field) {
+            double ni = i < n ? P->nu_field[i] : 0.0, nj = j < n ? P->nu_field[j] : 0.0;
+            nu = fluid_pair ? 0.5 * (ni + nj) : (i < n ? ni : nj);
+        }
         double scale;
-        if (nw && P->wall_friction > 0.0) scale = 2.0 * P->nu * ((double)fluid_pair + P->wall_friction * (double)crossing);
-        else if (nw) scale = 2.0 * P->nu * (double)fluid_pair;
-        else scale = 2.0 * P->nu;
+        if (nw && P->wall_friction > 0.0) scale = 2.0 * nu * ((double)fluid_pair + P->wall_friction * (double)crossing);
+        else if (nw) scale = 2.0 * nu * (double)fluid_pair;
+        else scale = 2.0 * nu;
         double dg = 0; for (int d = 0; d < dim; ++d) dg += delta[k * dim + d] * grad[k * dim + d];
         double relu = -dg > 0.0 ? -dg : 0.0;
         double lw = relu / (dist[k] * dist[k] + 0.01 * h * h);

@@ -52,7 +52,11 @@ class WcsphParams(C.Structure):
                 ("gamma", _dbl), ("gravity", _dbl), ("viscosity", _dbl),
                 ("wall_friction", _dbl), ("dt", _dbl), ("diffusion_scale", _dbl),
                 ("diffusion_form", C.c_int32), ("eigenvalue_floor", _dbl),
-                ("reaction_style", C.c_int32), ("has_wall_accel", C.c_int32)]
+                ("reaction_style", C.c_int32), ("has_wall_accel", C.c_int32),
+                ("wrap_width", _dbl), ("density_renormalization", C.c_int32),
+                ("lattice_kernel_sum", _dbl), ("boundary_consistent_density", C.c_int32),
+                ("surface_pressure_closure", C.c_int32), ("surface_pressure_threshold", _dbl),
+                ("surface_pressure_donors_all", C.c_int32), ("surface_pressure_layers", C.c_int32)]
 
 
 class ThreedParams(C.Structure):
@@ -110,6 +114,8 @@ def _declare(L):
         "tsph_nlist_from_pairs": [_vp, _vp, _i64, _i64, _i64, C.c_int, _vp],
         "tsph_wcsph_rhs": [_vp, C.POINTER(WcsphParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                            _vp, _vp, _vp],
+        "tsph_wcsph_rhs_ex": [_vp, C.POINTER(WcsphParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                              _vp, _vp, _vp, _vp, _vp],
         "tsph_eos_tait": [_vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dbl, _vp, _vp],
         "tsph_kernel_eval": [C.c_int, _vp, _i64, C.c_int, _dbl, _vp, _vp, _vp, _vp],
         "tsph_apply_boundary": [_vp, _vp, _i64, C.c_int, _dp, _dp, _dbl, _vp, _vp],

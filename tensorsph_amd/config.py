@@ -257,9 +257,17 @@ _UNSUPPORTED = (
 )
 
 
-def check_supported(cfg: Config) -> None:
-    """Reject opt-in branches that are not part of the particle pipeline."""
+# branches the operator rhs (tsph_wcsph_rhs_ex) implements; the resident
+# stepper does not
+_RHS_ONLY = ("surface_pressure_closure", "density_renormalization", "boundary_consistent_density")
+
+
+def check_supported(cfg: Config, rhs: bool = False) -> None:
+    """Reject opt-in branches that are not part of the particle pipeline
+    (with rhs=True, those the operator rhs does not implement)."""
     for name, on in _UNSUPPORTED:
+        if rhs and name in _RHS_ONLY:
+            continue
         if on(cfg):
             raise NotImplementedError(
                 f"Config.{name} is an opt-in rhs branch outside the GPU particle "

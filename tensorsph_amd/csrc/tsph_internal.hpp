@@ -119,6 +119,7 @@ struct tsph_ctx {
   // operator scratch
   tsph::DevBuf s_rho_raw, s_rho, s_p, s_aux, s_aux2, s_aux3, s_gr, s_flags, s_pack;
   tsph::DevBuf s_four, s_lap1;   // fourth-order viscous operator: I4 / 120 and the first Laplacian
+  tsph::DevBuf s_cl;             // free-surface pressure closure: two pressure and two flag buffers
   tsph::DevBuf s_red;        // reductions
   tsph::DevBuf s_dp;         // delta-plus step intermediates
 };

@@ -31,23 +31,35 @@ __global__ void __launch_bounds__(kBlock) k_density(WcsphArgs<S> a, const int64_
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
-  Sum<double> acc;
+  Sum<double> acc, kt;
   const int64_t e0 = off[i], e1 = off[i + 1];
   for (int64_t e = e0; e < e1; ++e) {
     const int64_t j = idx[e];
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const double df = xi[d] - a.pos[j * D + d];
+      const double df = sep(a, xi[d] - a.pos[j * D + d], d);
       r2 += df * df;
     }
     if (r2 > a.skip2d) continue;
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
     acc.add((double)a.mass[j] * w);
+    if (a.shepard) kt.add(w);
   }
+  const double floor_d = (double)a.rho_floor;
   double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
-  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  rho_d = rho_d > floor_d ? rho_d : floor_d;
+  if (a.dens_off) {                         // droplet_impact.py:936-943
+    rho_d = rho_d + (double)a.dens_off[i];
+    rho_d = rho_d > floor_d ? rho_d : floor_d;
+  }
+  if (a.shepard) {                          // Shepard renormalisation, :954-964
+    double t = hi(kt) + a.kd.w0d;
+    t = t > 1.0e-12 ? t : 1.0e-12;
+    rho_d = rho_d * (a.lattice_sum / t);
+    rho_d = rho_d > floor_d ? rho_d : floor_d;
+  }
   const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
   if (write_eos) {
     const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
@@ -77,7 +89,7 @@ __global__ void __launch_bounds__(kBlock) k_renorm_gradient(WcsphArgs<S> a, cons
     if (j >= a.n) continue;
     S df[D], r2 = S(0);
 #pragma unroll
-    for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+    for (int d = 0; d < D; ++d) { df[d] = (S)sep(a, xi[d] - a.pos[j * D + d], d); r2 += df[d] * df[d]; }
     if (r2 > a.skip2) continue;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
@@ -133,7 +145,7 @@ __global__ void __launch_bounds__(kBlock) k_diffusion(WcsphArgs<S> a, const int6
     if (j >= a.n) continue;
     S df[D], r2 = S(0);
 #pragma unroll
-    for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+    for (int d = 0; d < D; ++d) { df[d] = (S)sep(a, xi[d] - a.pos[j * D + d], d); r2 += df[d] * df[d]; }
     if (r2 > a.skip2) continue;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
@@ -181,7 +193,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_pressure(WcsphArgs<S> a, const 
     const int64_t f = widx[e];
     S rw[D], r2 = S(0);                       // r_w - r_f
 #pragma unroll
-    for (int d = 0; d < D; ++d) { rw[d] = (S)(xw[d] - a.pos[f * D + d]); r2 += rw[d] * rw[d]; }
+    for (int d = 0; d < D; ++d) { rw[d] = (S)sep(a, xw[d] - a.pos[f * D + d], d); r2 += rw[d] * rw[d]; }
     if (r2 > a.skip2) continue;
     S wgt, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
@@ -217,12 +229,13 @@ __global__ void __launch_bounds__(kBlock) k_force(WcsphArgs<S> a, const int64_t*
 #pragma unroll
   for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
   const S pri = a.pr2[i];
+  const S nui = a.nu_field ? a.nu_field[i] : S(0);
   Sum<S> ap[D], av[D];
   for (int64_t e = off[i]; e < off[i + 1]; ++e) {
     const int64_t j = idx[e];
     S df[D], r2 = S(0);
 #pragma unroll
-    for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+    for (int d = 0; d < D; ++d) { df[d] = (S)sep(a, xi[d] - a.pos[j * D + d], d); r2 += df[d] * df[d]; }
     if (r2 > a.skip2) continue;
     S w, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
@@ -240,7 +253,10 @@ __global__ void __launch_bounds__(kBlock) k_force(WcsphArgs<S> a, const int64_t*
       for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
       const S relu = -dg > S(0) ? -dg : S(0);
       const S lw = relu / (r2 + a.eta2);
-      const S cv = -(a.nu2 * blend) * (a.vol[j] * lw);
+      // a viscosity field: mean of the two members on fluid pairs, the fluid
+      // member's on fluid-wall pairs (droplet_impact.py:1168-1173)
+      const S nu2 = a.nu_field ? S(2) * (fluid_j ? S(0.5) * (nui + a.nu_field[j]) : nui) : a.nu2;
+      const S cv = -(nu2 * blend) * (a.vol[j] * lw);
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d].add(cv * (S)(vi[d] - a.vel[j * D + d]));
     }
@@ -271,7 +287,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_loads(WcsphArgs<S> a, const int
     const int64_t f = widx[e];
     S df[D], r2 = S(0);                      // delta = x_f - x_w
 #pragma unroll
-    for (int d = 0; d < D; ++d) { df[d] = (S)(a.pos[f * D + d] - xw[d]); r2 += df[d] * df[d]; }
+    for (int d = 0; d < D; ++d) { df[d] = (S)sep(a, a.pos[f * D + d] - xw[d], d); r2 += df[d] * df[d]; }
     if (r2 > a.skip2) continue;
     S wgt, g;
     cubic_wg<S>(sqrt_s<S>(r2), a.k, wgt, g);
@@ -282,7 +298,8 @@ __global__ void __launch_bounds__(kBlock) k_wall_loads(WcsphArgs<S> a, const int
     for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
     const S relu = -dg > S(0) ? -dg : S(0);
     const S lw = relu / (r2 + a.eta2);
-    const S cv = -(a.nu2 * a.friction) * (volw * lw);
+    const S nu2 = a.nu_field ? S(2) * a.nu_field[f] : a.nu2;
+    const S cv = -(nu2 * a.friction) * (volw * lw);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const S pp = (mf * mw) * (term * (df[d] * g));
@@ -294,10 +311,105 @@ __global__ void __launch_bounds__(kBlock) k_wall_loads(WcsphArgs<S> a, const int
   for (int d = 0; d < D; ++d) reaction[w * D + d] = r[d].get();
 }
 
+// ------------------------------------------------------------------ boundary-consistent density
+// droplet_impact.py:1090-1112: the wall rows' mass scaled by rho_w / rho0
+// (rho_w from the first wall pass) enters the fluid summation density; the
+// fluid EOS follows, and the wall pass runs again on the corrected state.
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_wall_mass_correction(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                                 const int32_t* __restrict__ idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  Sum<double> corr;
+  for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+    const int64_t j = idx[e];
+    if (j < a.n) continue;
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double df = sep(a, xi[d] - a.pos[j * D + d], d);
+      r2 += df * df;
+    }
+    if (r2 > a.skip2d) continue;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    const double mw = (double)a.mass[j];
+    corr.add((mw * ((double)a.rho[j] / a.rho0d) - mw) * w);
+  }
+  double rho_d = (double)a.rho[i] + hi(corr);
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = (S)rho_d;
+  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
+  a.rho[i] = rho;
+  a.p[i] = p;
+  a.pr2[i] = p / (rho * rho);
+  a.vol[i] = a.mass[i] / rho;
+}
+
+// ------------------------------------------------------------------ free-surface pressure closure
+// droplet_impact.py:429-492, fluid-fluid pairs.  Rows below threshold * rho0
+// receive p = max(0, sum_j w_j (p_j - g (y_i - y_j) rho_j) / sum_j w_j), with
+// w_j = W_ij for every neighbour ("all", one pass) or only for neighbours
+// whose pressure is known at the start of the pass ("interior", several
+// passes).  Each pass reads (p_src, known_src) and writes (p_dst, known_dst);
+// the last pass writes the state's p and p / rho^2.
+template <typename S>
+__global__ void k_closure_init(const S* __restrict__ rho, int64_t n, S thr, int* __restrict__ known) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) known[i] = rho[i] < thr ? 0 : 1;
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_closure(WcsphArgs<S> a, const int64_t* __restrict__ off,
+                                                    const int32_t* __restrict__ idx,
+                                                    const S* __restrict__ p_src, const int* __restrict__ k_src,
+                                                    S* __restrict__ p_dst, int* __restrict__ k_dst,
+                                                    int literal, int last) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  S pi = p_src[i];
+  int ki = k_src[i];
+  if (!ki) {
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+    Sum<S> num, den;
+    for (int64_t e = off[i]; e < off[i + 1]; ++e) {
+      const int64_t j = idx[e];
+      if (j >= a.n) continue;
+      S df[D], r2 = S(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { df[d] = (S)sep(a, xi[d] - a.pos[j * D + d], d); r2 += df[d] * df[d]; }
+      if (r2 > a.skip2) continue;
+      if (!literal && !k_src[j]) continue;
+      S w, g;
+      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+      const S src = p_src[j] - (a.gravity * df[1]) * a.rho[j];
+      num.add(w * src);
+      den.add(w);
+    }
+    if (den.get() > S(1e-30)) {
+      const S v = num.get() / den.get();
+      pi = v > S(0) ? v : S(0);
+      ki = 1;
+    }
+  }
+  p_dst[i] = pi;
+  k_dst[i] = ki;
+  if (last) {
+    const S r = a.rho[i];
+    a.pr2[i] = pi / (r * r);
+  }
+}
+
 // ------------------------------------------------------------------ host driver
 template <int D, typename S>
 int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* pos, const double* vel,
-                   const void* mass, const double* wall_accel, void* rho_out, void* p_out, void* acc_out,
+                   const void* mass, const double* wall_accel, const void* density_offset,
+                   const void* nu_field, void* rho_out, void* p_out, void* acc_out,
                    void* wall_p, void* wall_rho, void* reaction, hipStream_t stream) {
   const int64_t n = prm->n_fluid, nw = prm->n_wall, nt = n + nw;
   TSPH_REQUIRE(ctx->nb_targets == n && ctx->wnb_targets == nw, TSPH_ERR_ARG,
@@ -312,6 +424,8 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
   WcsphArgs<S> a = make_wcsph_args<S>(*prm, D);
   a.pos = pos; a.vel = vel; a.mass = (const S*)mass; a.wall_accel = prm->has_wall_accel ? wall_accel : nullptr;
   a.rho = ctx->s_rho.as<S>(); a.p = ctx->s_p.as<S>(); a.pr2 = ctx->s_aux.as<S>(); a.vol = ctx->s_aux2.as<S>();
+  a.dens_off = (const S*)density_offset;
+  a.nu_field = (const S*)nu_field;
   const int64_t* off = ctx->nb_off.as<int64_t>();
   const int32_t* idx = ctx->nb_idx.as<int32_t>();
   if (n > 0) {
@@ -335,12 +449,39 @@ int wcsph_rhs_impl(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* po
       }
     }
     TSPH_CHECK_LAUNCH();
+    if (prm->surface_pressure_closure) {
+      const int layers = prm->surface_pressure_donors_all ? 1
+                         : (prm->surface_pressure_layers > 1 ? prm->surface_pressure_layers : 1);
+      const size_t sb = (sizeof(S) * (size_t)(n + 1) + 255) / 256 * 256;
+      const size_t kb = (sizeof(int) * (size_t)(n + 1) + 255) / 256 * 256;
+      TSPH_TRY(ctx->s_cl.ensure(2 * sb + 2 * kb));
+      char* base = ctx->s_cl.as<char>();
+      S* pbuf[2] = {(S*)base, (S*)(base + sb)};
+      int* kbuf[2] = {(int*)(base + 2 * sb), (int*)(base + 2 * sb + kb)};
+      TSPH_CHECK_HIP(hipMemcpyAsync(pbuf[0], a.p, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
+      k_closure_init<S><<<grid_for(n), kBlock, 0, stream>>>(
+          a.rho, n, S(prm->surface_pressure_threshold * prm->rho0), kbuf[0]);
+      for (int l = 0; l < layers; ++l) {
+        const int src = l & 1, last = l == layers - 1;
+        k_closure<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx, pbuf[src], kbuf[src],
+                                                            last ? a.p : pbuf[src ^ 1], kbuf[src ^ 1],
+                                                            prm->surface_pressure_donors_all, last);
+      }
+      TSPH_CHECK_LAUNCH();
+    }
   }
   if (nw > 0) {
     k_wall_pressure<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(),
                                                                ctx->wnb_idx.as<int32_t>(),
                                                                (S*)wall_p, (S*)wall_rho);
     TSPH_CHECK_LAUNCH();
+    if (prm->boundary_consistent_density && n > 0) {
+      k_wall_mass_correction<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, off, idx);
+      k_wall_pressure<D, S><<<grid_for(nw), kBlock, 0, stream>>>(a, ctx->wnb_off.as<int64_t>(),
+                                                                 ctx->wnb_idx.as<int32_t>(),
+                                                                 (S*)wall_p, (S*)wall_rho);
+      TSPH_CHECK_LAUNCH();
+    }
   }
   if (n > 0) {
     k_force<D, S><<<grid_xcd(n), kBlock, 0, stream>>>(a, off, idx, (S*)acc_out);
@@ -365,8 +506,25 @@ extern "C" int tsph_wcsph_rhs(tsph_ctx* ctx, const tsph_wcsph_params* prm, const
                               const double* vel, const void* mass, const double* wall_accel, void* rho,
                               void* p, void* acc, void* wall_p, void* wall_rho, void* reaction,
                               hipStream_t stream) {
+  return tsph_wcsph_rhs_ex(ctx, prm, pos, vel, mass, wall_accel, nullptr, nullptr, rho, p, acc, wall_p,
+                           wall_rho, reaction, stream);
+}
+
+extern "C" int tsph_wcsph_rhs_ex(tsph_ctx* ctx, const tsph_wcsph_params* prm, const double* pos,
+                                 const double* vel, const void* mass, const double* wall_accel,
+                                 const void* density_offset, const void* nu_field, void* rho, void* p,
+                                 void* acc, void* wall_p, void* wall_rho, void* reaction, hipStream_t stream) {
   TSPH_REQUIRE(ctx && prm, TSPH_ERR_ARG, "ctx and params must not be NULL");
   TSPH_REQUIRE(prm->dim == 2 || prm->dim == 3, TSPH_ERR_ARG, "dim must be 2 or 3, got %d", prm->dim);
+  const bool opts = prm->wrap_width != 0.0 || prm->density_renormalization || prm->boundary_consistent_density ||
+                    prm->surface_pressure_closure || density_offset || nu_field;
+  TSPH_REQUIRE(!opts || prm->dim == 2, TSPH_ERR_ARG, "the opt-in rhs branches are 2-D only");
+  TSPH_REQUIRE(prm->wrap_width >= 0.0 && prm->wrap_width < INFINITY, TSPH_ERR_ARG,
+               "wrap_width must be a positive finite length (0 disables)");
+  TSPH_REQUIRE(!prm->density_renormalization || prm->lattice_kernel_sum > 0.0, TSPH_ERR_ARG,
+               "density_renormalization needs a positive lattice_kernel_sum");
+  TSPH_REQUIRE(!(density_offset && prm->density_renormalization), TSPH_ERR_ARG,
+               "density_offset requires unnormalised summation density");
   TSPH_REQUIRE(prm->precision == TSPH_F64 || prm->precision == TSPH_F32, TSPH_ERR_ARG, "bad precision");
   TSPH_REQUIRE(prm->h > 0.0, TSPH_ERR_ARG, "h must be positive");
   TSPH_REQUIRE(prm->n_fluid >= 0 && prm->n_wall >= 0, TSPH_ERR_ARG, "bad particle counts");
@@ -374,12 +532,16 @@ extern "C" int tsph_wcsph_rhs(tsph_ctx* ctx, const tsph_wcsph_params* prm, const
                "pos, vel, mass, rho, p and acc must not be NULL");
   if (prm->dim == 2) {
     if (prm->precision == TSPH_F64)
-      return wcsph_rhs_impl<2, double>(ctx, prm, pos, vel, mass, wall_accel, rho, p, acc, wall_p, wall_rho, reaction, stream);
-    return wcsph_rhs_impl<2, float>(ctx, prm, pos, vel, mass, wall_accel, rho, p, acc, wall_p, wall_rho, reaction, stream);
+      return wcsph_rhs_impl<2, double>(ctx, prm, pos, vel, mass, wall_accel, density_offset, nu_field, rho, p, acc,
+                                       wall_p, wall_rho, reaction, stream);
+    return wcsph_rhs_impl<2, float>(ctx, prm, pos, vel, mass, wall_accel, density_offset, nu_field, rho, p, acc,
+                                    wall_p, wall_rho, reaction, stream);
   }
   if (prm->precision == TSPH_F64)
-    return wcsph_rhs_impl<3, double>(ctx, prm, pos, vel, mass, wall_accel, rho, p, acc, wall_p, wall_rho, reaction, stream);
-  return wcsph_rhs_impl<3, float>(ctx, prm, pos, vel, mass, wall_accel, rho, p, acc, wall_p, wall_rho, reaction, stream);
+    return wcsph_rhs_impl<3, double>(ctx, prm, pos, vel, mass, wall_accel, nullptr, nullptr, rho, p, acc, wall_p,
+                                     wall_rho, reaction, stream);
+  return wcsph_rhs_impl<3, float>(ctx, prm, pos, vel, mass, wall_accel, nullptr, nullptr, rho, p, acc, wall_p,
+                                  wall_rho, reaction, stream);
 }
 
 // ------------------------------------------------------------------ elementwise helpers

@@ -27,7 +27,20 @@ template <typename S> struct WcsphArgs {
   S* vol;             // m / rho
   S* rho_lo;          // optional: low part of the stored density (FP32 mode)
   Vec4<S>* aux;       // FP32 slot sweeps: (m, p / rho^2, m / rho, 0) per row for the force sweep
+  // opt-in branches of the 2-D operator rhs (unused by the resident steppers)
+  double wrap;        // > 0: x separations taken modulo wrap (x-periodic strip)
+  int shepard;        // density * lattice_sum / (sum_j W_ij + W(0))
+  double lattice_sum;
+  const S* dens_off;  // optional additive density offset of the fluid rows
+  const S* nu_field;  // optional per-particle viscosity of the fluid rows
 };
+
+// Component d of a pair separation: x is reduced to the nearest image when a
+// strip width is set (round half to even, as torch.round does).
+template <typename S>
+TSPH_D double sep(const WcsphArgs<S>& a, double v, int d) {
+  return (d == 0 && a.wrap > 0.0) ? v - a.wrap * rint(v / a.wrap) : v;
+}
 
 template <typename S>
 inline WcsphArgs<S> make_wcsph_args(const tsph_wcsph_params& p, int dim) {
@@ -53,6 +66,9 @@ inline WcsphArgs<S> make_wcsph_args(const tsph_wcsph_params& p, int dim) {
   a.eig_floor = S(p.eigenvalue_floor);
   a.n = p.n_fluid;
   a.nw = p.n_wall;
+  a.wrap = p.wrap_width;
+  a.shepard = p.density_renormalization;
+  a.lattice_sum = p.lattice_kernel_sum;
   return a;
 }
 

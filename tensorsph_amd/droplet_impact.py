@@ -134,9 +134,27 @@ def _wcsph_params(cfg, n, nw, dim, code, step_dt, reaction_style, has_wall_accel
     return p
 
 
+def _rhs_options(p, cfg, wrap_width) -> None:
+    """The opt-in branches of the 2-D rhs (droplet_impact.py:921-964,
+    1031-1112) into the parameter block."""
+    p.wrap_width = 0.0 if wrap_width is None else float(wrap_width)
+    if cfg.density_renormalization:
+        p.density_renormalization = 1
+        p.lattice_kernel_sum = lattice_kernel_sum(cfg.h_factor * cfg.dx, cfg.dx)
+    p.boundary_consistent_density = 1 if cfg.boundary_consistent_density else 0
+    if cfg.surface_pressure_closure:
+        p.surface_pressure_closure = 1
+        p.surface_pressure_threshold = float(cfg.surface_pressure_threshold)
+        p.surface_pressure_donors_all = 1 if cfg.surface_pressure_donors == "all" else 0
+        p.surface_pressure_layers = int(cfg.surface_pressure_layers)
+
+
 def run_wcsph(pos, vel, mass, cfg, pairs, wall, dt, wall_vel, wall_accel, split, dim,
-              wall_mass_value, reaction_style):
-    """Shared driver of the 2-D ``rhs`` and the 3-D ``rhs_3d``."""
+              wall_mass_value, reaction_style, options=None):
+    """Shared driver of the 2-D ``rhs`` and the 3-D ``rhs_3d``.  ``options``
+    (2-D rhs only): dict of wrap_width, density_offset, nu_field; the Config
+    branches come with it.  With an empty pair list the reference returns
+    before every branch, so none is passed then."""
     dev = _lib.require_gpu(pos, vel, mass, wall, pairs)
     code = _precision(pos)
     sdt = _scalar_dtype(code)
@@ -168,16 +186,23 @@ def run_wcsph(pos, vel, mass, cfg, pairs, wall, dt, wall_vel, wall_accel, split,
     step_dt = cfg.dt if dt is None else float(dt)
     prm = _wcsph_params(cfg, n, nw, dim, code, step_dt, reaction_style if split else 0,
                         wa is not None, True)
+    off = nuf = None
+    if options is not None and pairs.numel() > 0:
+        _rhs_options(prm, cfg, options.get("wrap_width"))
+        if options.get("density_offset") is not None:
+            off = options["density_offset"].detach().to(sdt).contiguous()
+    if options is not None and options.get("nu_field") is not None:
+        nuf = options["nu_field"].detach().to(sdt).contiguous()
     rho = torch.empty(n, dtype=sdt, device=dev)
     p = torch.empty(n, dtype=sdt, device=dev)
     acc = torch.empty((n, dim), dtype=sdt, device=dev)
     wall_p = torch.empty(nw, dtype=sdt, device=dev)
     wall_rho = torch.empty(nw, dtype=sdt, device=dev)
     reaction = torch.zeros((nw, dim), dtype=sdt, device=dev) if split else None
-    _lib.check(L.tsph_wcsph_rhs(ctx.handle, C.byref(prm), _lib.ptr(all_pos), _lib.ptr(all_vel),
-                                _lib.ptr(all_mass), _lib.ptr(wa), _lib.ptr(rho), _lib.ptr(p),
-                                _lib.ptr(acc), _lib.ptr(wall_p), _lib.ptr(wall_rho),
-                                _lib.ptr(reaction), stream))
+    _lib.check(L.tsph_wcsph_rhs_ex(ctx.handle, C.byref(prm), _lib.ptr(all_pos), _lib.ptr(all_vel),
+                                   _lib.ptr(all_mass), _lib.ptr(wa), _lib.ptr(off), _lib.ptr(nuf),
+                                   _lib.ptr(rho), _lib.ptr(p), _lib.ptr(acc), _lib.ptr(wall_p),
+                                   _lib.ptr(wall_rho), _lib.ptr(reaction), stream))
     out_dtype = pos.dtype
     return (rho.to(out_dtype), p.to(out_dtype), acc.to(out_dtype), wall_p.to(out_dtype),
             None if reaction is None else reaction.to(out_dtype))
@@ -189,15 +214,14 @@ def rhs(pos, vel, mass, cfg: Config, pairs=None, wall=None, dt=None, wall_vel=No
     """WCSPH right-hand side on unordered pairs: ``(rho, pressure, acceleration)``.
 
     With ``split=True`` a fourth value ``{"reaction", "face_pressure"}`` holds
-    the force the fluid exerts on each wall particle and the wall pressure."""
+    the force the fluid exerts on each wall particle and the wall pressure.
+    The opt-in branches wrap_width, nu_field, density_offset and the Config
+    switches density_renormalization, boundary_consistent_density and
+    surface_pressure_closure run on the device (droplet_impact.py:921-964,
+    1031-1112, 1164-1178)."""
     if pos.shape[1] != 2:
         raise ValueError(f"rhs implements 2-D smoothing only, got {pos.shape[1]} dimensions")
-    for name, val in (("wrap_width", wrap_width), ("nu_field", nu_field),
-                      ("density_offset", density_offset)):
-        if val is not None:
-            raise NotImplementedError(f"rhs({name}=...) is an opt-in branch outside the GPU "
-                                      "particle pipeline")
-    check_supported(cfg)
+    check_supported(cfg, rhs=True)
     if cfg.density_diffusion > 0.0 and cfg.density_diffusion_form not in ("molteni", "antuono"):
         raise ValueError(f"unknown density_diffusion_form {cfg.density_diffusion_form!r}")
     h = cfg.h_factor * cfg.dx
@@ -221,10 +245,33 @@ def rhs(pos, vel, mass, cfg: Config, pairs=None, wall=None, dt=None, wall_vel=No
         raise ValueError("pairs must be a long tensor with shape (pair_count, 2)")
     if pairs.device != pos.device:
         raise ValueError("pairs and pos must be on the same device")
+    if nu_field is not None:
+        if not torch.is_tensor(nu_field) or nu_field.shape != (n,):
+            raise ValueError(f"nu_field must be a tensor of shape ({n},), got "
+                             f"{None if not torch.is_tensor(nu_field) else tuple(nu_field.shape)}")
+        if nu_field.device != pos.device or nu_field.dtype != pos.dtype:
+            raise ValueError("nu_field must share pos device and dtype")
+        if not bool(torch.isfinite(nu_field).all()):
+            raise ValueError("nu_field must be finite")
+        if bool((nu_field < 0).any()):
+            raise ValueError("nu_field must be non-negative")
+    if pairs.numel() > 0:
+        if wrap_width is not None:
+            width = float(wrap_width)
+            if not math.isfinite(width) or width <= 0.0:
+                raise ValueError("wrap_width must be a positive finite length")
+        if density_offset is not None:
+            if density_offset.shape != (n,) or density_offset.device != pos.device:
+                raise ValueError("density_offset must match the fluid particles")
+            if density_offset.dtype != pos.dtype:
+                raise ValueError("density_offset must have the fluid dtype")
+            if cfg.density_renormalization:
+                raise ValueError("density_offset requires unnormalised summation density")
     nw = 0 if wall is None else wall.shape[0]
     rho, p, acc, wall_p, reaction = run_wcsph(
         pos, vel, mass, cfg, pairs, wall, dt, wall_vel, wall_accel, split, 2,
-        lattice_mass_unit(cfg), 1)
+        lattice_mass_unit(cfg), 1,
+        {"wrap_width": wrap_width, "density_offset": density_offset, "nu_field": nu_field})
     if pairs.numel() == 0:
         return rho, p, acc
     if split:

@@ -23,6 +23,9 @@ Fixtures
   run2d.npz          droplet_impact.run rows with output_every = 1 (meters per
                      step) and the final snapshot - droplet_impact.py:1607-1795
   threed_order4.npz  the same operators with the fourth-order viscous term
+  rhs2d_opts.npz     droplet_impact.rhs with its opt-in branches: wrap_width,
+                     nu_field, density_offset, density renormalisation,
+                     boundary-consistent density, surface-pressure closure
   threed.npz         threed.operators3.accelerations (periodic, renormalised,
                      walls + Adami, Antuono diffusion) - operators3.py:51-295
   rhs3d.npz          dam_break_3d.rhs_3d with wall forces - dam_break_3d.py:290-422
@@ -400,13 +403,104 @@ def order4(reference: str = "/root/reference") -> None:
     print("order-4 fixture written to", HERE)
 
 
+def rhs_options(reference: str = "/root/reference") -> None:
+    """droplet_impact.rhs with its opt-in branches (droplet_impact.py:810-1228):
+    wrap_width (x-periodic strip, :921-927), nu_field (:1164-1178),
+    density_offset (:936-943), density_renormalization (:954-964),
+    boundary_consistent_density (:1090-1112) and the surface-pressure closure
+    (:1031-1038, :429-492, both donor sets), on the small impact state."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import droplet_impact as di
+    from tensorsph.neighbor_search import PairListCache
+    torch.set_default_dtype(torch.float64)
+    cpu = torch.device("cpu")
+    out = {}
+    base = dict(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05, drop_center_y=0.25,
+                device="cpu", viscosity=0.01)
+    variants = {
+        "offset": ({}, "offset"),
+        "shepard": ({"density_renormalization": True}, None),
+        "nufield": ({"wall_friction": 0.3}, "nu"),
+        "bcd": ({"boundary_consistent_density": True}, None),
+        "closure": ({"surface_pressure_closure": True}, None),
+        "closure_all": ({"surface_pressure_closure": True, "surface_pressure_donors": "all"}, None),
+        "closure3": ({"surface_pressure_closure": True, "surface_pressure_layers": 3}, None),
+        "closure_bcd": ({"surface_pressure_closure": True, "boundary_consistent_density": True,
+                         "density_diffusion": 0.1, "dt": 1e-4}, None),
+        "shepard_diff": ({"density_renormalization": True, "density_diffusion": 0.1,
+                          "density_diffusion_form": "molteni", "dt": 1e-4}, None),
+    }
+    for name, (over, extra) in variants.items():
+        cfg = di.Config(**base, **over)
+        pos, mass = di.make_particles(cfg, cpu)
+        wall = di.wall_lattice(cfg, cpu)
+        g = torch.Generator().manual_seed(11)
+        vel = torch.randn(pos.shape, generator=g) * 0.05
+        pos = pos + 0.05 * cfg.dx * torch.randn(pos.shape, generator=g)
+        if name.startswith("shepard"):
+            # with equal masses the Shepard-normalised density is rho0 to round-off
+            mass = mass * (1.0 + 0.05 * torch.rand(mass.shape, generator=g))
+        h = cfg.h_factor * cfg.dx
+        cache = PairListCache(2.0 * h, 0.25 * h, fluid_count=pos.shape[0])
+        pairs, _ = cache.get(torch.cat((pos, wall)))
+        kw = {}
+        if extra == "offset":
+            kw["density_offset"] = 5.0 * torch.randn(pos.shape[0], generator=g)
+            out[name + "_offset"] = _np(kw["density_offset"])
+        if extra == "nu":
+            kw["nu_field"] = 0.005 + 0.015 * torch.rand(pos.shape[0], generator=g)
+            out[name + "_nu"] = _np(kw["nu_field"])
+        rho, p, acc, loads = di.rhs(pos, vel, mass, cfg, pairs=pairs, wall=wall, split=True,
+                                    dt=cfg.dt, **kw)
+        out.update({name + "_pos": _np(pos), name + "_vel": _np(vel), name + "_mass": _np(mass),
+                    name + "_wall": _np(wall), name + "_pairs": _np(pairs), name + "_rho": _np(rho),
+                    name + "_p": _np(p), name + "_acc": _np(acc),
+                    name + "_reaction": _np(loads["reaction"]),
+                    name + "_face": _np(loads["face_pressure"])})
+        out[name + "_lattice_sum"] = np.array(di.lattice_kernel_sum(h, cfg.dx))
+        out[name + "_unit"] = np.array(float(di.lattice_mass_unit(cfg)))
+    # x-periodic strip: bottom wall rows only, pairs by a minimum-image brute force
+    cfg = di.Config(**base)
+    pos, mass = di.make_particles(cfg, cpu)
+    wall = di.wall_lattice(cfg, cpu)
+    width = cfg.pool_width
+    keep = (wall[:, 1] < 0.0) & (wall[:, 0] >= 0.0) & (wall[:, 0] < width)
+    wall = wall[keep]
+    g = torch.Generator().manual_seed(12)
+    pos = pos + 0.05 * cfg.dx * torch.randn(pos.shape, generator=g)
+    pos[:, 0] = pos[:, 0].remainder(width)
+    vel = torch.randn(pos.shape, generator=g) * 0.05
+    h = cfg.h_factor * cfg.dx
+    allp = _np(torch.cat((pos, wall)))
+    n = pos.shape[0]
+    d = allp[:, None, :] - allp[None, :, :]
+    d[..., 0] -= width * np.round(d[..., 0] / width)
+    close = (d ** 2).sum(-1) <= (2.0 * h * 1.1) ** 2
+    ii, jj = np.nonzero(np.triu(close, 1))
+    sel = (ii < n) | (jj < n)
+    pairs = torch.from_numpy(np.stack((ii[sel], jj[sel]), 1).astype(np.int64))
+    rho, p, acc, loads = di.rhs(pos, vel, mass, cfg, pairs=pairs, wall=wall, split=True,
+                                wrap_width=width)
+    out.update({"wrap_pos": _np(pos), "wrap_vel": _np(vel), "wrap_mass": _np(mass),
+                "wrap_wall": _np(wall), "wrap_pairs": _np(pairs), "wrap_rho": _np(rho),
+                "wrap_p": _np(p), "wrap_acc": _np(acc), "wrap_reaction": _np(loads["reaction"]),
+                "wrap_face": _np(loads["face_pressure"]), "wrap_width": np.array(width),
+                "wrap_unit": np.array(float(di.lattice_mass_unit(cfg)))})
+    np.savez_compressed(HERE / "rhs2d_opts.npz", **out)
+    print("rhs option fixture written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
         shear(*args[1:])
     elif args and args[0] == "--only-order4":
         order4(*args[1:])
+    elif args and args[0] == "--only-rhs-options":
+        rhs_options(*args[1:])
     else:
         main(*args)
         shear(*args)
         order4(*args)
+        rhs_options(*args)

@@ -196,3 +196,80 @@ def test_rhs2d_edge_cases_and_invariances(golden):
     r3 = rhs(*args, pairs=pruned, wall=W)
     for x, y, z in zip(r1, r2, r3):
         assert torch.equal(x, y) and torch.equal(y, z)
+
+
+# ----------------------------------------------------------- opt-in branches of the 2-D rhs
+
+def _opts_call(g, name, dtype):
+    import dataclasses
+    from tensorsph_amd.droplet_impact import rhs
+    from tests.test_oracle import rhs_opts_case
+    cfg, arr, extra, kw = rhs_opts_case(g, name, np.float32 if dtype == torch.float32 else np.float64)
+    cfg = dataclasses.replace(cfg, device="cuda")
+    ex = {k: (t(v, dtype) if isinstance(v, np.ndarray) else v) for k, v in extra.items()}
+    out = rhs(t(arr["pos"], dtype), t(arr["vel"], dtype), t(arr["mass"], dtype), cfg,
+              pairs=t(arr["pairs"]), wall=t(arr["wall"], dtype), split=True, dt=cfg.dt, **ex)
+    ref = orc.rhs_morris(arr["pos"].astype(np.float64), arr["vel"].astype(np.float64),
+                         arr["mass"].astype(np.float64), arr["wall"].astype(np.float64),
+                         arr["pairs"], **dict(kw, wall_mass=float(np.float32(kw["wall_mass"]))
+                                             if dtype == torch.float32 else kw["wall_mass"]))
+    return out, ref
+
+
+OPT_CASES = ["offset", "shepard", "nufield", "bcd", "closure", "closure_all", "closure3",
+             "closure_bcd", "shepard_diff", "wrap"]
+
+
+@pytest.mark.parametrize("name", OPT_CASES)
+def test_rhs2d_options_fp64(golden, name):
+    """wrap_width, density_offset, Shepard renormalisation, nu_field,
+    boundary-consistent density and the surface-pressure closure: FP64
+    against the oracle (1e-12) and the reference's own output (1e-11)."""
+    g = golden("rhs2d_opts")
+    (rho, p, acc, loads), (orho, op, oacc, owp, owr, oreac) = _opts_call(g, name, torch.float64)
+    assert scaled_err(n(rho), orho) < 1e-12
+    assert scaled_err(n(p), op) < 1e-12
+    assert scaled_err(n(acc), oacc) < 1e-12
+    assert scaled_err(n(loads["face_pressure"]), owp) < 1e-12
+    assert scaled_err(n(loads["reaction"]), oreac) < 1e-12
+    assert scaled_err(n(rho), g[name + "_rho"]) < 1e-12
+    assert scaled_err(n(p), g[name + "_p"]) < 1e-11
+    assert scaled_err(n(acc), g[name + "_acc"]) < 1e-11
+    assert scaled_err(n(loads["reaction"]), g[name + "_reaction"]) < 1e-11
+
+
+@pytest.mark.parametrize("name", OPT_CASES)
+def test_rhs2d_options_fp32_compensated(golden, name):
+    g = golden("rhs2d_opts")
+    (rho, p, acc, loads), (orho, op, oacc, owp, owr, oreac) = _opts_call(g, name, torch.float32)
+    assert rho.dtype == torch.float32
+    assert scaled_err(n(rho), orho) < 1e-6
+    assert scaled_err(n(p), op) < 1e-5
+    assert scaled_err(n(acc), oacc) < 1e-5
+    assert scaled_err(n(loads["reaction"]), oreac) < 1e-5
+
+
+def test_rhs2d_option_errors(golden):
+    import dataclasses
+    from tensorsph_amd.droplet_impact import rhs
+    from tests.test_oracle import rhs_opts_case
+    g = golden("rhs2d_opts")
+    cfg, arr, _, _ = rhs_opts_case(g, "closure")
+    args = (t(arr["pos"]), t(arr["vel"]), t(arr["mass"]), cfg)
+    kw = dict(pairs=t(arr["pairs"]), wall=t(arr["wall"]))
+    nn = arr["pos"].shape[0]
+    with pytest.raises(ValueError):
+        rhs(*args, wrap_width=-1.0, **kw)
+    with pytest.raises(ValueError):
+        rhs(*args, nu_field=t(np.full(nn, -1.0)), **kw)
+    with pytest.raises(ValueError):
+        rhs(*args, nu_field=t(np.full(nn + 1, 0.01)), **kw)
+    with pytest.raises(ValueError):
+        rhs(*args[:3], dataclasses.replace(cfg, density_renormalization=True),
+            density_offset=t(np.zeros(nn)), **kw)
+    # an empty pair list returns before every branch, as the reference does
+    empty = torch.empty((0, 2), dtype=torch.long, device=dev())
+    a = rhs(*args, pairs=empty, density_offset=t(np.full(nn, 50.0)), wrap_width=-1.0)
+    b = rhs(*args[:3], dataclasses.replace(cfg, surface_pressure_closure=False), pairs=empty)
+    for x, y in zip(a, b):
+        assert torch.equal(x, y)

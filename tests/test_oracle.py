@@ -150,6 +150,117 @@ def test_rhs2d_tracks_reference_fp32(golden, name):
     assert scaled_err(acc, g[key + "_acc"]) < 1e-3
 
 
+# opt-in branches of the 2-D rhs (droplet_impact.py:921-964, 1031-1112, 1164-1178):
+# fixture case -> the Config fields the reference run used
+RHS_OPTS = {
+    "offset": {}, "shepard": {"density_renormalization": True},
+    "nufield": {"wall_friction": 0.3}, "bcd": {"boundary_consistent_density": True},
+    "closure": {"surface_pressure_closure": True},
+    "closure_all": {"surface_pressure_closure": True, "surface_pressure_donors": "all"},
+    "closure3": {"surface_pressure_closure": True, "surface_pressure_layers": 3},
+    "closure_bcd": {"surface_pressure_closure": True, "boundary_consistent_density": True,
+                    "density_diffusion": 0.1, "dt": 1e-4},
+    "shepard_diff": {"density_renormalization": True, "density_diffusion": 0.1,
+                     "density_diffusion_form": "molteni", "dt": 1e-4},
+    "wrap": {},
+}
+RHS_OPTS_BASE = dict(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05,
+                     drop_center_y=0.25, viscosity=0.01)
+
+
+def rhs_opts_case(g, name, dtype=np.float64):
+    """(cfg, arrays, oracle keyword arguments) of one fixture case."""
+    from tensorsph_amd.config import Config
+    cfg = Config(**RHS_OPTS_BASE, **RHS_OPTS[name])
+    h = cfg.h_factor * cfg.dx
+    arr = {k: g[f"{name}_{k}"] for k in ("pos", "vel", "mass", "wall", "pairs")}
+    for k in ("pos", "vel", "mass", "wall"):
+        arr[k] = arr[k].astype(dtype)
+    extra = {}
+    if name == "offset":
+        extra["density_offset"] = g["offset_offset"].astype(dtype)
+    if name == "nufield":
+        extra["nu_field"] = g["nufield_nu"].astype(dtype)
+    if name == "wrap":
+        extra["wrap_width"] = float(g["wrap_width"])
+    kw = dict(dim=2, h=h, rho0=cfg.rho0, c0=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+              nu=cfg.viscosity, wall_friction=cfg.wall_friction, dt=cfg.dt,
+              diff_scale=(2.0 * cfg.density_diffusion * cfg.sound_speed * h
+                          if cfg.density_diffusion > 0 else 0.0),
+              diff_form=1 if cfg.density_diffusion_form == "antuono" else 0,
+              eig_floor=cfg.density_diffusion_eigenvalue_floor,
+              wall_mass=float(g[f"{name}_unit"]), reaction_style=1,
+              shepard=cfg.density_renormalization,
+              lattice_sum=float(g[f"{name}_lattice_sum"]) if name != "wrap" else 0.0,
+              bcd=cfg.boundary_consistent_density, closure=cfg.surface_pressure_closure,
+              closure_threshold=cfg.surface_pressure_threshold,
+              closure_literal=cfg.surface_pressure_donors == "all",
+              closure_layers=cfg.surface_pressure_layers,
+              **{k: (v.astype(np.float64) if hasattr(v, "astype") else v) for k, v in extra.items()})
+    return cfg, arr, extra, kw
+
+
+@pytest.mark.parametrize("name", list(RHS_OPTS))
+def test_rhs2d_options_match_reference(golden, name):
+    g = golden("rhs2d_opts")
+    _, arr, _, kw = rhs_opts_case(g, name)
+    rho, p, acc, wp, wr, reac = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"],
+                                               arr["pairs"], **kw)
+    assert scaled_err(rho, g[name + "_rho"]) < 1e-13
+    assert scaled_err(p, g[name + "_p"]) < 1e-11
+    assert scaled_err(acc, g[name + "_acc"]) < 1e-11
+    assert scaled_err(wp, g[name + "_face"]) < 1e-11
+    assert scaled_err(reac, g[name + "_reaction"]) < 1e-11
+
+
+def test_rhs2d_options_closed_forms(golden):
+    """Checks of the opt-in branches that do not lean on the fixture:
+    a shifted x-periodic strip gives the same answer; with equal masses the
+    Shepard density is m * lattice_sum on every row; an offset adds to the
+    density before the EOS; a constant nu_field is the scalar viscosity; the
+    closure with no receiver and boundary-consistent density with walls at
+    rho0 change nothing."""
+    g = golden("rhs2d_opts")
+    _, arr, _, kw = rhs_opts_case(g, "wrap")
+    base = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"], arr["pairs"], **kw)
+    width = kw["wrap_width"]
+    for shift in (0.13, 0.31):
+        sh = lambda x: np.stack((np.mod(x[:, 0] + shift, width), x[:, 1]), 1)
+        out = orc.rhs_morris(sh(arr["pos"]), arr["vel"], arr["mass"], sh(arr["wall"]), arr["pairs"],
+                             **kw)
+        for a, b in zip(out, base):
+            assert scaled_err(a, b) < 1e-12
+    _, arr, _, kw = rhs_opts_case(g, "closure")
+    plain = dict(kw, closure=False)
+    ref = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"], arr["pairs"], **plain)
+    m = np.full_like(arr["mass"], arr["mass"].mean())
+    rho, *_ = orc.rhs_morris(arr["pos"], arr["vel"], m, arr["wall"], arr["pairs"],
+                             **dict(plain, shepard=True, lattice_sum=float(g["shepard_lattice_sum"])))
+    assert np.abs(rho - m[0] * float(g["shepard_lattice_sum"])).max() < 1e-10 * rho.max()
+    off = np.linspace(-3.0, 4.0, len(m))
+    rho_o, p_o, *_ = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"], arr["pairs"],
+                                    **dict(plain, density_offset=off))
+    assert np.abs(rho_o - (ref[0] + off)).max() < 1e-12 * ref[0].max()
+    assert np.abs(p_o - orc.tait2d(rho_o, 1000.0, plain["c0"], 7.0)).max() < 1e-9 * p_o.max()
+    nu_c = np.full(len(m), plain["nu"])
+    out = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"], arr["pairs"],
+                         **dict(plain, nu_field=nu_c))
+    for a, b in zip(out, ref):
+        assert scaled_err(a, b) < 1e-14
+    out = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], arr["wall"], arr["pairs"],
+                         **dict(kw, closure_threshold=0.0))
+    for a, b in zip(out, ref):
+        assert scaled_err(a, b) < 1e-15
+    # walls far from the fluid: wall pressure 0, wall density rho0, no correction
+    far = arr["wall"] + np.array([0.0, -10.0])
+    a1 = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], far, arr["pairs"][
+        (arr["pairs"] < len(m)).all(1)], **dict(plain, bcd=True))
+    a0 = orc.rhs_morris(arr["pos"], arr["vel"], arr["mass"], far, arr["pairs"][
+        (arr["pairs"] < len(m)).all(1)], **plain)
+    for a, b in zip(a1, a0):
+        assert scaled_err(a, b) < 1e-15
+
+
 def test_rhs2d_dense_reference(golden):
     g = golden("rhs2d")
     pos, vel, mass = g["dense_pos"], g["dense_vel"], g["dense_mass"]
