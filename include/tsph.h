@@ -403,6 +403,34 @@ int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps)
 int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream);
 int tsph_sim_destroy(tsph_sim* sim);
 
+/* ---------------------------------------------------------------- gauges
+ * The water-height gauges and obstacle pressure taps of the 3-D dam break
+ * against a box (kleefsman_obstacle_3d.py:273-318, marin_probes.py:25-79).
+ * Slab: station_x - aperture/2 < x < station_x + aperture/2 and
+ * obstacle_lo[2] < z < obstacle_hi[2]; the centreline gauge is centred on
+ * (station_x, tank_depth / 2); taps: 8 sensors (4 on the upstream face, 4 on
+ * the top face) of per_tap quadrature nodes each (1: the point, 7: the disk
+ * cubature), nodes[(8 per_tap) x 3].  Output row (TSPH_GAUGE_ROW doubles):
+ * slab height, peak height in the slab (0 if empty), front x, bypass count,
+ * centreline height H2, then (pressure, support) per node (56 slots, unused
+ * slots 0).  Synchronises. */
+#define TSPH_GAUGE_ROW 117
+typedef struct tsph_gauge_spec {
+  double station_x, aperture, tank_depth, dx, h, gravity;
+  double obstacle_lo[3], obstacle_hi[3];   /* (x0, 0, z0), (x1, height, z1) */
+  int32_t per_tap;
+  double nodes[8 * 7 * 3];
+} tsph_gauge_spec;
+
+/* On caller arrays: pos (n, 3) FP64; vol = m / rho, rho, p (n) scalar type. */
+int tsph_gauges(tsph_ctx* ctx, const tsph_gauge_spec* spec, int precision, int64_t n,
+                const double* pos, const void* vol, const void* rho, const void* p, double* out,
+                hipStream_t stream);
+/* On the 3-D stepper's fluid rows at the start of its last step: the
+ * positions that step began from and the density, pressure and volume its
+ * evaluation produced (the state the reference samples at that step). */
+int tsph_sim_gauges(tsph_sim* sim, const tsph_gauge_spec* spec, double* out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -481,3 +481,45 @@ def adaptive_next_dt(dt: float, max_speed: float, *, dt_cfg: float, max_courant:
     target = dt_safety * min(acoustic, viscous, grav, diff)
     grown = dt * dt_growth if dt_growth >= 1.0 else target
     return min(target, grown, dt_cfg)
+
+
+def _probe_cubic(q):
+    q = np.asarray(q, np.float64)
+    return np.where(q < 1.0, 1.0 - 1.5 * q * q + 0.75 * q ** 3,
+                    0.25 * np.clip(2.0 - q, 0.0, None) ** 3)
+
+
+def kleefsman_gauges(pos, volume, rho, pressure, *, station_x, aperture, tank_depth, dx, h, gravity,
+                     obstacle_lo, obstacle_hi, nodes):
+    """The gauges of kleefsman_obstacle_3d.py:273-318 and marin_probes.py:23-69
+    in FP64: slab height n dx^3 / (aperture * box width) over the band
+    |x - station_x| < aperture / 2, box z-span; the highest y in the band (0
+    if empty); the front max x; the bypass count (x past the box, z outside
+    its span); the centreline height sum V cubic(r / h) * 10 / (7 pi h^2),
+    r = hypot(x - station_x, z - depth / 2); and per quadrature node (taps 0-3
+    upstream face: x <= x_node, taps 4-7 top face: y >= y_node) the pressure
+    max(0, sum w (p + rho g (y - y_node)) / max(sum w, tiny)) with
+    w = cubic(|x - node| / h) V, and the count of w > 0."""
+    pos = np.asarray(pos, np.float64)
+    vol = np.asarray(volume, np.float64)
+    x, y, z = pos[:, 0], pos[:, 1], pos[:, 2]
+    oz0, oz1 = obstacle_lo[2], obstacle_hi[2]
+    band = (x > station_x - 0.5 * aperture) & (x < station_x + 0.5 * aperture) & (z > oz0) & (z < oz1)
+    n = int(band.sum())
+    slab = n * dx ** 3 / (aperture * (oz1 - oz0))
+    peak = float(y[band].max()) if n else 0.0
+    front = float(x.max()) if len(x) else 0.0
+    bypass = int(((x > obstacle_hi[0]) & ((z < oz0) | (z > oz1))).sum())
+    r = np.hypot(x - station_x, z - 0.5 * tank_depth)
+    h2 = float((vol * _probe_cubic(r / h)).sum() * (10.0 / (7.0 * np.pi * h * h)))
+    nodes = np.asarray(nodes, np.float64)
+    delta = pos[:, None, None, :] - nodes[None]
+    w = _probe_cubic(np.linalg.norm(delta, axis=-1) / h) * vol[:, None, None]
+    w[:, :4] *= delta[:, :4, :, 0] <= 0
+    w[:, 4:] *= delta[:, 4:, :, 1] >= 0
+    corr = (np.asarray(pressure, np.float64)[:, None, None]
+            + np.asarray(rho, np.float64)[:, None, None] * gravity * delta[..., 1])
+    den = w.sum(0)
+    p = np.clip((w * corr).sum(0) / np.maximum(den, np.finfo(np.float64).tiny), 0.0, None)
+    return {"slab": slab, "peak": peak, "front": front, "bypass": bypass, "h2": h2,
+            "tap_p": p, "tap_support": (w > 0).sum(0)}

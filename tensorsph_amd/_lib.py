@@ -69,6 +69,15 @@ class ThreedParams(C.Structure):
                 ("viscous_order", C.c_int32)]
 
 
+class GaugeSpec(C.Structure):
+    _fields_ = [("station_x", _dbl), ("aperture", _dbl), ("tank_depth", _dbl), ("dx", _dbl),
+                ("h", _dbl), ("gravity", _dbl), ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3),
+                ("per_tap", C.c_int32), ("nodes", _dbl * (8 * 7 * 3))]
+
+
+GAUGE_ROW = 117
+
+
 class DeltaPlusParams(C.Structure):
     _fields_ = [("precision", C.c_int32), ("n", C.c_int64), ("n_wall", C.c_int64),
                 ("dx", _dbl), ("sound_speed", _dbl), ("reference_speed", _dbl), ("dt", _dbl),
@@ -116,6 +125,8 @@ def _declare(L):
                            _vp, _vp, _vp],
         "tsph_wcsph_rhs_ex": [_vp, C.POINTER(WcsphParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _vp, _vp, _vp, _vp, _vp],
+        "tsph_gauges": [_vp, C.POINTER(GaugeSpec), C.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+        "tsph_sim_gauges": [_vp, C.POINTER(GaugeSpec), _vp, _vp],
         "tsph_eos_tait": [_vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dbl, _vp, _vp],
         "tsph_kernel_eval": [C.c_int, _vp, _i64, C.c_int, _dbl, _vp, _vp, _vp, _vp],
         "tsph_apply_boundary": [_vp, _vp, _i64, C.c_int, _dp, _dp, _dbl, _vp, _vp],

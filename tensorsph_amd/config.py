@@ -151,6 +151,11 @@ OBSTACLE_LENGTH_M = 0.161
 OBSTACLE_HEIGHT_M = 0.161
 OBSTACLE_WIDTH_M = 0.403
 OBSTACLE_X_M = TANK_LENGTH_M - 0.744 - OBSTACLE_LENGTH_M / 2
+H2_X_M = TANK_LENGTH_M - 0.992
+# the gauge protocol of the 2-D runs, kept for the 3-D gauges
+FROZEN_APERTURE_M = 0.08
+FROZEN_STATION_OFFSET_M = OBSTACLE_X_M - H2_X_M
+FROZEN_ARRIVAL_THRESHOLD_M = 0.02
 
 
 @dataclass
@@ -167,6 +172,10 @@ class Kleefsman3DConfig(DamBreak3DConfig):
     obstacle_height: float = OBSTACLE_HEIGHT_M
     obstacle_width: float = OBSTACLE_WIDTH_M
     obstacle_z: float = (TANK_WIDTH_M - OBSTACLE_WIDTH_M) / 2
+    station_offset: float = FROZEN_STATION_OFFSET_M
+    gauge_aperture: float = FROZEN_APERTURE_M
+    arrival_threshold: float = FROZEN_ARRIVAL_THRESHOLD_M
+    pressure_disk_diameter: float = 0.0
     dt: float = 2.6e-4
     t_end: float = 2.6
     output_every: int = 25
@@ -182,6 +191,12 @@ class Kleefsman3DConfig(DamBreak3DConfig):
 
     def obstacle_z1(self) -> float:
         return self.obstacle_z + self.obstacle_width
+
+    def station_x(self) -> float:
+        return self.obstacle_x - self.station_offset
+
+    def blocked_fraction(self) -> float:
+        return self.obstacle_width / self.tank_depth
 
     def steps_for_t_end(self) -> int:
         return int(math.ceil(self.t_end / self.dt)) + 2

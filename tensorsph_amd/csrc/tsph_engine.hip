@@ -975,7 +975,10 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
     return sim_rebuild<D, S>(sim, stream);
   };
   if (sim->finished) return TSPH_OK;
-  if (!sim->built) TSPH_TRY(rebuild());
+  if (!sim->built || sim->rebuild_pending) {
+    TSPH_TRY(rebuild());
+    sim->rebuild_pending = false;
+  }
   // steps in flight (queued but not yet consumed)
   int64_t next = start;        // next step index to queue
   int64_t done = start;        // next step index to consume
@@ -999,6 +1002,13 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
       for (int64_t k = done; k < next; ++k) {
         bool dummy;
         TSPH_TRY(sim_consume(sim, k, false, &dummy));
+      }
+      if (done >= end) {
+        // nothing left to run: leave the last step's input and output sides
+        // as they are (the gauges read its input side) and re-bin first
+        // thing in the next run
+        sim->rebuild_pending = true;
+        break;
       }
       TSPH_TRY(rebuild());
       next = done;
@@ -1196,6 +1206,7 @@ int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps)
 int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream) {
   TSPH_REQUIRE(sim && pairs && radius > 0.0, TSPH_ERR_ARG, "bad arguments");
   TSPH_REQUIRE(sim->built, TSPH_ERR_ARG, "no cell binning yet: step the simulation first");
+  if (sim->rebuild_pending) TSPH_TRY(tsph_sim_step(sim, 0, 0, stream));   // re-bin the current state
   tsph_ctx* ctx = sim->ctx;
   TSPH_TRY(ctx->s_red.ensure(sizeof(unsigned long long)));
   unsigned long long* cnt = ctx->s_red.as<unsigned long long>();

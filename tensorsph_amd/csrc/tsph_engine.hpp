@@ -83,6 +83,7 @@ struct tsph_sim {
   int64_t pairs_2h = -1;
   int64_t ring_step[tsph::kRing];
   int cur = 0;
+  bool rebuild_pending = false;  // the last step of a run went stale: rebuild before the next step
   int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
   bool built = false;
   bool finished = false;                  // an adaptive run has done its final sample

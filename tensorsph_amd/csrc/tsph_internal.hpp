@@ -122,4 +122,5 @@ struct tsph_ctx {
   tsph::DevBuf s_cl;             // free-surface pressure closure: two pressure and two flag buffers
   tsph::DevBuf s_red;        // reductions
   tsph::DevBuf s_dp;         // delta-plus step intermediates
+  tsph::DevBuf g_part, g_flag, g_idx, g_tmp, g_out;   // gauges: partials, near flags / rows, select scratch
 };

@@ -487,7 +487,8 @@ int tsph_ctx_destroy(tsph_ctx* ctx) {
                     &ctx->scan_tmp, &ctx->nb_off, &ctx->nb_idx, &ctx->nb_cursor, &ctx->wnb_off,
                     &ctx->wnb_idx, &ctx->wnb_cursor, &ctx->s_rho_raw, &ctx->s_rho, &ctx->s_p,
                     &ctx->s_aux, &ctx->s_aux2, &ctx->s_aux3, &ctx->s_gr, &ctx->s_flags, &ctx->s_pack, &ctx->s_red, &ctx->s_dp,
-                    &ctx->s_four, &ctx->s_lap1, &ctx->s_cl};
+                    &ctx->s_four, &ctx->s_lap1, &ctx->s_cl, &ctx->g_part, &ctx->g_flag, &ctx->g_idx,
+                    &ctx->g_tmp, &ctx->g_out};
   for (DevBuf* b : bufs) b->release();
   delete ctx;
   return TSPH_OK;

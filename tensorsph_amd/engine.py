@@ -209,6 +209,17 @@ def _count_pairs(self, radius: float) -> int:
     return out.value
 
 
+def _gauges(self, spec) -> np.ndarray:
+    """tsph_sim_gauges: the gauge row (GAUGE_ROW doubles) of the state the
+    last step began from."""
+    out = np.zeros(_lib.GAUGE_ROW, np.float64)
+    _lib.check(_lib.lib().tsph_sim_gauges(self.handle, C.byref(spec),
+                                          out.ctypes.data_as(C.POINTER(C.c_double)),
+                                          _lib.stream_handle(self.device)))
+    return out
+
+
+WcsphSimulation.gauges = _gauges
 WcsphSimulation.set_timing = _set_timing
 WcsphSimulation.sweep_times = _sweep_times
 WcsphSimulation.count_pairs = _count_pairs

@@ -23,6 +23,9 @@ Fixtures
   run2d.npz          droplet_impact.run rows with output_every = 1 (meters per
                      step) and the final snapshot - droplet_impact.py:1607-1795
   threed_order4.npz  the same operators with the fourth-order viscous term
+  kleefsman_gauges.npz  the Kleefsman gauge and tap operators on a cloud, and a
+                     short run_case series; kleefsman_checkpoint_v3.pt a
+                     checkpoint written by save_checkpoint
   rhs2d_opts.npz     droplet_impact.rhs with its opt-in branches: wrap_width,
                      nu_field, density_offset, density renormalisation,
                      boundary-consistent density, surface-pressure closure
@@ -491,6 +494,83 @@ def rhs_options(reference: str = "/root/reference") -> None:
     print("rhs option fixture written to", HERE)
 
 
+def kleefsman_gauges(reference: str = "/root/reference") -> None:
+    """The gauge operators of kleefsman_obstacle_3d / marin_probes on a seeded
+    particle cloud around the box (slab_height, peak_height, bypass_count
+    :273-318, centreline_height and tap_node_pressures with point and disk
+    taps marin_probes.py:25-79), a run_case at dx = 0.05 to t = 0.62 s (past the impact on the box) with its
+    on_sample series (:553-726), and a v3 checkpoint written by
+    save_checkpoint (:462-517)."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import kleefsman_obstacle_3d as kl
+    from tensorsph import marin_probes as mp
+    torch.set_default_dtype(torch.float64)
+    out = {}
+    cfg = kl.Kleefsman3DConfig(dx=0.05, t_end=0.62, output_every=25, pressure_disk_diameter=0.03)
+    g = torch.Generator().manual_seed(21)
+    n = 6000
+    lo = torch.tensor([2.0, 0.0, 0.0])
+    hi = torch.tensor([2.7, 0.35, 1.0])
+    pos = lo + (hi - lo) * torch.rand((n, 3), generator=g)
+    inside = ((pos[:, 0] > cfg.obstacle_x) & (pos[:, 0] < cfg.obstacle_x1())
+              & (pos[:, 1] < cfg.obstacle_height)
+              & (pos[:, 2] > cfg.obstacle_z) & (pos[:, 2] < cfg.obstacle_z1()))
+    pos = pos[~inside]
+    m = pos.shape[0]
+    rho = 1000.0 * (1.0 + 0.02 * torch.rand(m, generator=g))
+    pressure = 2000.0 * torch.rand(m, generator=g)
+    volume = cfg.dx ** 3 * (1.0 + 0.1 * torch.rand(m, generator=g))
+    out.update(cloud_pos=pos.numpy(), cloud_rho=rho.numpy(), cloud_p=pressure.numpy(),
+               cloud_vol=volume.numpy())
+    out["cloud_slab"] = np.array(kl.slab_height(pos, cfg))
+    out["cloud_peak"] = np.array(kl.peak_height(pos, cfg))
+    out["cloud_bypass"] = np.array(kl.bypass_count(pos, cfg))
+    out["cloud_h2"] = np.array(float(mp.centreline_height(pos, volume, cfg)))
+    taps = mp.sensor_positions(cfg, pos)
+    for tag, dia in (("point", 0.0), ("disk", 0.03)):
+        nodes = mp.tap_disk_nodes(cfg, taps, dia)
+        p, support = mp.tap_node_pressures(pos, pressure, rho, volume, cfg, nodes)
+        out[f"cloud_{tag}_nodes"] = nodes.numpy()
+        out[f"cloud_{tag}_p"] = p.numpy()
+        out[f"cloud_{tag}_support"] = support.numpy()
+    out["cloud_disk_avg"] = mp.disk_average(torch.from_numpy(out["cloud_disk_p"])).numpy()
+    # a short run with the per-sample series
+    samples = []
+    run = kl.run_case(cfg, on_sample=lambda i, s: samples.append(s))
+    keys = ["t_s", "slab_height_m", "height_operator_m", "front_x_m", "n_bypass", "H2_local_m"]
+    out["run_rows"] = np.array([[float(s[k]) for k in keys] for s in samples])
+    out["run_row_keys"] = np.array(keys)
+    out["run_force"] = np.array([s["obstacle_sph_force_N"] for s in samples])
+    out["run_impulse"] = np.array([s["obstacle_sph_impulse_Ns"] for s in samples])
+    out["run_moment"] = np.array([s["obstacle_sph_moment_Nm"] for s in samples])
+    out["run_tap_p"] = np.array([s["tap_pressure_Pa"] for s in samples])
+    out["run_tap_support"] = np.array([s["tap_support"] for s in samples])
+    out["run_disk_p"] = np.array([s["disk_pressure_Pa"] for s in samples])
+    summary = run.to_dict()
+    for k in ("peak_height_m", "peak_time_s", "peak_height_operator_m", "max_speed",
+              "max_density_compression", "steps", "simulated_time_s", "local_h2_peak_m",
+              "boundary_escapes", "obstacle_clamps", "max_bypass_particles"):
+        v = summary[k]
+        out["summary_" + k] = np.array(np.nan if v is None else float(v))
+    for k in ("arrival_time_s", "local_h2_arrival_s"):
+        v = summary[k]
+        out["summary_" + k] = np.array(np.nan if v is None else float(v))
+    out["summary_sph_impulse"] = np.array(summary["obstacle_sph_impulse_Ns"])
+    out["summary_clamp_impulse"] = np.array(summary["obstacle_clamp_impulse_Ns"])
+    out["params"] = np.array([cfg.dx, cfg.t_end, cfg.dt, cfg.output_every,
+                              cfg.pressure_disk_diameter, cfg.sound_speed])
+    np.savez_compressed(HERE / "kleefsman_gauges.npz", **out)
+    # a checkpoint file as save_checkpoint writes it (tensors, lists, numbers)
+    ck = kl.Kleefsman3DRun(cfg=cfg)
+    ck.times, ck.slab, ck.height = [0.0, 0.1], [0.0, 0.05], [0.0, 0.12]
+    ck.local_h2, ck.front, ck.bypass = [0.0, 0.04], [1.2, 2.0], [0.0, 3.0]
+    kl.save_checkpoint(HERE / "kleefsman_checkpoint_v3.pt", cfg, step=40, elapsed=40 * cfg.dt,
+                       dt=cfg.dt, positions=pos[:50], velocity=0.1 * pos[:50], run=ck,
+                       speeds=1.5, compression=0.01, escapes=0, clamps=2)
+    print("kleefsman fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
@@ -499,8 +579,11 @@ if __name__ == "__main__":
         order4(*args[1:])
     elif args and args[0] == "--only-rhs-options":
         rhs_options(*args[1:])
+    elif args and args[0] == "--only-kleefsman":
+        kleefsman_gauges(*args[1:])
     else:
         main(*args)
         shear(*args)
         order4(*args)
         rhs_options(*args)
+        kleefsman_gauges(*args)

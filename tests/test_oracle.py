@@ -679,3 +679,76 @@ def test_package_step_rule_matches_the_oracle():
     for dt in (1e-7, 1e-5, cfg.dt):
         for v in (0.0, 1.0, 3.0 * cfg.sound_speed):
             assert next_dt(cfg, dt, v) == orc.adaptive_next_dt(dt, v, **kw)
+
+
+# ----------------------------------------------------------------- Kleefsman gauges (rank 4)
+
+def gauge_kw(cfg):
+    return dict(station_x=cfg.station_x(), aperture=cfg.gauge_aperture, tank_depth=cfg.tank_depth,
+                dx=cfg.dx, h=cfg.h_factor * cfg.dx, gravity=cfg.gravity,
+                obstacle_lo=[cfg.obstacle_x, 0.0, cfg.obstacle_z],
+                obstacle_hi=[cfg.obstacle_x1(), cfg.obstacle_height, cfg.obstacle_z1()])
+
+
+def gauge_cfg():
+    from tensorsph_amd.config import Kleefsman3DConfig
+    return Kleefsman3DConfig(dx=0.05, t_end=0.62, output_every=25, pressure_disk_diameter=0.03)
+
+
+@pytest.mark.parametrize("tag", ["point", "disk"])
+def test_kleefsman_gauges_match_reference(golden, tag):
+    g = golden("kleefsman_gauges")
+    o = orc.kleefsman_gauges(g["cloud_pos"], g["cloud_vol"], g["cloud_rho"], g["cloud_p"],
+                             nodes=g[f"cloud_{tag}_nodes"], **gauge_kw(gauge_cfg()))
+    assert o["slab"] == pytest.approx(float(g["cloud_slab"]), rel=1e-14)
+    assert o["peak"] == float(g["cloud_peak"]) and o["bypass"] == int(g["cloud_bypass"])
+    assert o["h2"] == pytest.approx(float(g["cloud_h2"]), rel=1e-12)
+    assert scaled_err(o["tap_p"], g[f"cloud_{tag}_p"]) < 1e-13
+    assert np.array_equal(o["tap_support"], g[f"cloud_{tag}_support"])
+
+
+def test_kleefsman_gauges_closed_forms():
+    """A lattice block filling the band gives its volume over the band area;
+    one particle at a node gives that node its own hydrostatically shifted
+    pressure; one particle on the station gives H2 = V W(0) 10 / (7 pi h^2)."""
+    cfg = gauge_cfg()
+    kw = gauge_kw(cfg)
+    dx = cfg.dx
+    nodes = np.zeros((8, 1, 3))
+    nodes[:] = [[cfg.obstacle_x, 0.05, cfg.obstacle_z + 0.2]]
+    nodes[4:] = [[cfg.obstacle_x + 0.05, cfg.obstacle_height, cfg.obstacle_z + 0.2]]
+    xs = cfg.station_x() - 0.5 * cfg.gauge_aperture + (np.arange(8) + 0.5) * 0.01
+    zs = cfg.obstacle_z + (np.arange(40) + 0.5) * cfg.obstacle_width / 40
+    ys = (np.arange(5) + 0.5) * 0.02
+    block = np.stack(np.meshgrid(xs, ys, zs, indexing="ij"), -1).reshape(-1, 3)
+    one = np.ones(len(block))
+    o = orc.kleefsman_gauges(block, one, one, one, nodes=nodes, **kw)
+    assert o["slab"] == pytest.approx(len(block) * dx ** 3 / (cfg.gauge_aperture * cfg.obstacle_width))
+    assert o["peak"] == pytest.approx(ys[-1]) and o["bypass"] == 0
+    p_one = np.array([[cfg.obstacle_x - 0.01, 0.06, cfg.obstacle_z + 0.2]])
+    o = orc.kleefsman_gauges(p_one, [2.0], [1000.0], [500.0], nodes=nodes, **kw)
+    assert o["tap_p"][0, 0] == pytest.approx(500.0 + 1000.0 * cfg.gravity * 0.01, rel=1e-14)
+    assert o["tap_support"][0, 0] == 1 and o["tap_support"][4, 0] == 0
+    h = cfg.h_factor * dx
+    st = np.array([[cfg.station_x(), 0.1, 0.5 * cfg.tank_depth]])
+    o = orc.kleefsman_gauges(st, [3.0], [1.0], [0.0], nodes=nodes, **kw)
+    assert o["h2"] == pytest.approx(3.0 * 10.0 / (7.0 * np.pi * h * h), rel=1e-14)
+
+
+def test_kleefsman_checkpoint_fixture_loads_safely(golden):
+    """The reference's v3 checkpoint loads with the tensor-only loader and
+    its fingerprint matches this configuration's."""
+    import torch
+    from pathlib import Path
+    from tensorsph_amd.kleefsman import checkpoint_fingerprint, load_checkpoint
+    path = Path(__file__).resolve().parent / "golden" / "kleefsman_checkpoint_v3.pt"
+    cfg = gauge_cfg()
+    payload = load_checkpoint(path, cfg)
+    assert payload["step"] == 40 and payload["positions"].shape == (50, 3)
+    assert payload["slab"] == [0.0, 0.05] and payload["clamps"] == 2
+    assert payload["fingerprint"] == checkpoint_fingerprint(cfg)
+    import dataclasses
+    with pytest.raises(ValueError):
+        load_checkpoint(path, dataclasses.replace(cfg, dx=0.04))
+    # t_end is a stopping rule, not part of the fingerprint
+    load_checkpoint(path, dataclasses.replace(cfg, t_end=3.0))
