@@ -426,12 +426,14 @@ class Stepper2D:
 
     def __init__(self, pos, vel, mass, wall, *, h, rho0, c0, gamma, gravity, nu, dt,
                  wall_mass, skin, lo, hi, tol, wall_friction=0.0, diff_scale=0.0,
-                 diff_form=0, eig_floor=0.1):
+                 diff_form=0, eig_floor=0.1, **options):
+        """``options``: the rhs_morris opt-in branches (shepard, lattice_sum,
+        bcd, closure, closure_threshold, closure_literal, closure_layers)."""
         self.pos = np.array(pos, np.float64); self.vel = np.array(vel, np.float64)
         self.mass = np.asarray(mass, np.float64); self.wall = np.asarray(wall, np.float64)
         self.kw = dict(dim=2, h=h, rho0=rho0, c0=c0, gamma=gamma, gravity=gravity, nu=nu,
                        wall_friction=wall_friction, dt=dt, diff_scale=diff_scale,
-                       diff_form=diff_form, eig_floor=eig_floor, wall_mass=wall_mass)
+                       diff_form=diff_form, eig_floor=eig_floor, wall_mass=wall_mass, **options)
         self.h, self.dt, self.skin = h, dt, skin
         self.lo, self.hi, self.tol = lo, hi, tol
         self.anchor = None

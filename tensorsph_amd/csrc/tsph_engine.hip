@@ -214,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
-  Sum<double> acc;
+  Sum<double> acc, kt;
   nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
     double r2 = 0.0;
 #pragma unroll
@@ -226,9 +226,16 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
     acc.add_if(keep, (double)a.mass[j] * w);
+    if (a.shepard) kt.add_if(keep, w);
   });
   double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  if (a.shepard) {                 // Shepard renormalisation, droplet_impact.py:954-964
+    double t = hi(kt) + a.kd.w0d;
+    t = t > 1.0e-12 ? t : 1.0e-12;
+    rho_d = rho_d * (a.lattice_sum / t);
+    rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  }
   const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
   if (write_eos) {
     const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
@@ -371,6 +378,99 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NB nbr, Si
   a.pr2[iw] = pr2;
   a.vol[iw] = vol;
   if (a.aux) a.aux[iw] = Vec4<S>{a.mass[iw], pr2, vol, S(0)};
+}
+
+// Boundary-consistent density (droplet_impact.py:1090-1112): the wall rows'
+// mass scaled by rho_w / rho0 enters the fluid summation density, then the
+// fluid EOS; the wall sweep runs again after it.
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_wall_mass(WcsphArgs<S> a, NB nbr, SimFlags* fl, int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  Sum<double> corr;
+  nbr.template visit<D>(i, true, [&](int64_t j) {
+    if (j < a.n) return;
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double df = xi[d] - a.pos[j * D + d];
+      r2 += df * df;
+    }
+    if (r2 > a.skip2d) return;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    const double mw = (double)a.mass[j];
+    corr.add((mw * ((double)a.rho[j] / a.rho0d) - mw) * w);
+  });
+  double rho_d = (double)a.rho[i] + hi(corr);
+  rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
+  const S rho = (S)rho_d;
+  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
+  const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
+  a.rho[i] = rho;
+  a.p[i] = p;
+  a.pr2[i] = pr2;
+  a.vol[i] = vol;
+  if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
+}
+
+// Free-surface pressure closure (droplet_impact.py:429-492) on the stepper:
+// fluid rows below threshold * rho0 take the kernel-weighted extrapolation of
+// their fluid neighbours' pressure, from donors known at the pass start
+// ("interior", several passes) or all neighbours ("all", one pass).  The
+// passes ping-pong (p, known); the last writes the state's p and p / rho^2.
+template <typename S>
+__global__ void k_step_closure_init(const S* __restrict__ rho, int64_t n, S thr, int* __restrict__ known,
+                                    SimFlags* fl, int slot_in) {
+  if (fl->stale[slot_in]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) known[i] = rho[i] < thr ? 0 : 1;
+}
+
+template <int D, typename S, typename NB>
+__global__ void __launch_bounds__(kBlock) k_step_closure(WcsphArgs<S> a, NB nbr, SimFlags* fl, int slot_in,
+                                                         const S* __restrict__ p_src, const int* __restrict__ k_src,
+                                                         S* __restrict__ p_dst, int* __restrict__ k_dst,
+                                                         int literal, int last) {
+  if (fl->stale[slot_in]) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  S pi = p_src[i];
+  int ki = k_src[i];
+  if (!ki) {
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+    Sum<S> num, den;
+    nbr.template visit<D>(i, false, [&](int64_t j) {
+      S df[D], r2 = S(0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
+      if (r2 > a.skip2) return;
+      if (!literal && !k_src[j]) return;
+      S w, g;
+      cubic_wg<S>(sqrt_s<S>(r2), a.k, w, g);
+      num.add(w * (p_src[j] - (a.gravity * df[1]) * a.rho[j]));
+      den.add(w);
+    });
+    if (den.get() > S(1e-30)) {
+      const S v = num.get() / den.get();
+      pi = v > S(0) ? v : S(0);
+      ki = 1;
+    }
+  }
+  p_dst[i] = pi;
+  k_dst[i] = ki;
+  if (last) {
+    const S r = a.rho[i];
+    const S pr2 = pi / (r * r);
+    a.pr2[i] = pr2;
+    if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, a.vol[i], S(0)};
+  }
 }
 
 template <typename T> TSPH_D T wave_max(T v) {
@@ -769,6 +869,42 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   return a;
 }
 
+// After the fluid density and EOS: the optional free-surface pressure closure,
+// the wall sweep, and the optional boundary-consistent density (fluid
+// correction, then the wall sweep again), in the reference's order.
+template <int D, typename S>
+int queue_wall_stage(tsph_sim* sim, const WcsphArgs<S>& a, const NbSlots& nbr, SimFlags* fl, int slot,
+                     hipStream_t stream) {
+  const tsph_wcsph_params& op = sim->prm.op;
+  const int64_t n = sim->n, nw = sim->nw;
+  if (op.surface_pressure_closure && n > 0) {
+    const int layers = op.surface_pressure_donors_all ? 1
+                       : (op.surface_pressure_layers > 1 ? op.surface_pressure_layers : 1);
+    const size_t sb = (sizeof(S) * (size_t)(n + 1) + 255) / 256 * 256;
+    const size_t kb = (sizeof(int) * (size_t)(n + 1) + 255) / 256 * 256;
+    char* base = sim->closure.as<char>();
+    S* pbuf[2] = {(S*)base, (S*)(base + sb)};
+    int* kbuf[2] = {(int*)(base + 2 * sb), (int*)(base + 2 * sb + kb)};
+    TSPH_CHECK_HIP(hipMemcpyAsync(pbuf[0], a.p, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
+    k_step_closure_init<S><<<grid_for(n), kBlock, 0, stream>>>(a.rho, n, S(op.surface_pressure_threshold * op.rho0),
+                                                               kbuf[0], fl, slot);
+    for (int l = 0; l < layers; ++l) {
+      const int src = l & 1, last = l == layers - 1;
+      k_step_closure<D, S, NbSlots><<<grid_for(n), kBlock, 0, stream>>>(
+          a, nbr, fl, slot, pbuf[src], kbuf[src], last ? a.p : pbuf[src ^ 1], kbuf[src ^ 1],
+          op.surface_pressure_donors_all, last);
+    }
+    TSPH_CHECK_LAUNCH();
+  }
+  if (nw > 0) k_step_wall<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
+  if (op.boundary_consistent_density && nw > 0 && n > 0) {
+    k_step_wall_mass<D, S, NbSlots><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, fl, slot);
+    k_step_wall<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
+  }
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
 // Density (+ EOS) and wall sweeps over the slot lists: the first two sweeps of
 // a step on the cell-tile path, whose force sweep runs on the tiles.
 template <int D, typename S>
@@ -782,8 +918,7 @@ int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
   k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
   TSPH_CHECK_LAUNCH();
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
-  if (sim->nw > 0) k_step_wall<D, S, NbSlots><<<grid_for(sim->nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
-  TSPH_CHECK_LAUNCH();
+  TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
   return TSPH_OK;
 }
 template int queue_slot_density_wall<2, float>(tsph_sim*, int64_t, hipStream_t);
@@ -816,8 +951,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
     k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
   }
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
-  if (nw > 0)
-    k_step_wall<D, S, NbSlots><<<grid_for(nw), kBlock, 0, stream>>>(a, nbr, fl, slot);
+  TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
@@ -1034,6 +1168,13 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   TSPH_REQUIRE(prm->obstacle_begin >= 0 && prm->obstacle_begin <= prm->obstacle_end &&
                prm->obstacle_end <= prm->op.n_wall, TSPH_ERR_ARG, "obstacle rows outside the wall rows");
   TSPH_REQUIRE(!prm->obstacle_clamp || prm->op.dim == 3, TSPH_ERR_ARG, "the obstacle clamp is 3-D");
+  TSPH_REQUIRE(prm->op.wrap_width == 0.0, TSPH_ERR_ARG, "the resident stepper has no x-periodic strip");
+  TSPH_REQUIRE(prm->op.dim == 2 || !(prm->op.density_renormalization || prm->op.boundary_consistent_density ||
+                                     prm->op.surface_pressure_closure),
+               TSPH_ERR_ARG, "density renormalisation, boundary-consistent density and the pressure closure "
+               "are 2-D rhs branches");
+  TSPH_REQUIRE(!prm->op.density_renormalization || prm->op.lattice_kernel_sum > 0.0, TSPH_ERR_ARG,
+               "density_renormalization needs a positive lattice_kernel_sum");
   if (prm->adaptive)
     TSPH_REQUIRE(prm->target_time >= 0.0 && prm->courant_num > 0.0 && prm->dt_limit > 0.0 &&
                  prm->dt_safety > 0.0 && prm->dt_growth > 0.0, TSPH_ERR_ARG,
@@ -1061,6 +1202,11 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->orig, 8 * nt); ENS(sim->orig2, 8 * nt);
   ENS(sim->rho, e * nt); ENS(sim->p, e * nt); ENS(sim->pr2, e * nt); ENS(sim->vol, e * nt);
   if (e == 4) ENS(sim->aux, 4 * e * nt);
+  if (prm->op.surface_pressure_closure) {
+    const size_t sb = (e * (size_t)(prm->op.n_fluid + 1) + 255) / 256 * 256;
+    const size_t kb = (sizeof(int) * (size_t)(prm->op.n_fluid + 1) + 255) / 256 * 256;
+    ENS(sim->closure, 2 * sb + 2 * kb);
+  }
   ENS(sim->rho_raw, e * nt); ENS(sim->rho_lo, e * nt); ENS(sim->grad, e * n * D); ENS(sim->acc, e * n * D);
   ENS(sim->anchor, 8 * n * D);
   ENS(sim->flags, sizeof(SimFlags));
@@ -1240,7 +1386,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
-                    &sim->load_part, &sim->loads_out, &sim->clamp_acc};
+                    &sim->load_part, &sim->loads_out, &sim->clamp_acc, &sim->closure};
   for (DevBuf* b : bufs) b->release();
   for (int r = 0; r < kRing; ++r) if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
   for (int r = 0; r < kRing; ++r)

@@ -66,6 +66,7 @@ struct tsph_sim {
   tsph::DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
   tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
   tsph::DevBuf aux;                 // FP32: Vec4 (m, p / rho^2, m / rho, 0) per row
+  tsph::DevBuf closure;             // free-surface pressure closure: two pressure and two flag buffers
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
   tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)

@@ -131,10 +131,10 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     ``precision`` "fp64" / "fp32" (FP32-compensated) defaults to torch's
     default dtype."""
     from .engine import METER_KEYS, WcsphSimulation
-    from .droplet_impact import _state_time
+    from .droplet_impact import _state_time, stepper_options
 
     validate_case(cfg)
-    check_supported(cfg)
+    check_supported(cfg, rhs=True)
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     dev = torch.device(device) if device is not None else torch.device("cuda")
@@ -153,7 +153,8 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
                           diffusion_form=cfg.density_diffusion_form,
                           eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
                           precision=precision, device=dev,
-                          adaptive=adaptive_controls(cfg) if adaptive else None)
+                          adaptive=adaptive_controls(cfg) if adaptive else None,
+                          rhs_options=stepper_options(cfg))
     targets = sorted(float(t) for t in (snapshot_times or []))
     snapshots = []
     max_steps = 4 * cfg.steps + 1000 if adaptive else cfg.steps

@@ -134,6 +134,17 @@ def _wcsph_params(cfg, n, nw, dim, code, step_dt, reaction_style, has_wall_accel
     return p
 
 
+def stepper_options(cfg) -> dict:
+    """The Config switches of the rhs that the resident stepper runs too."""
+    return {"density_renormalization": bool(cfg.density_renormalization),
+            "lattice_kernel_sum": lattice_kernel_sum(cfg.h_factor * cfg.dx, cfg.dx),
+            "boundary_consistent_density": bool(cfg.boundary_consistent_density),
+            "surface_pressure_closure": bool(cfg.surface_pressure_closure),
+            "surface_pressure_threshold": float(cfg.surface_pressure_threshold),
+            "surface_pressure_donors": cfg.surface_pressure_donors,
+            "surface_pressure_layers": int(cfg.surface_pressure_layers)}
+
+
 def _rhs_options(p, cfg, wrap_width) -> None:
     """The opt-in branches of the 2-D rhs (droplet_impact.py:921-964,
     1031-1112) into the parameter block."""
@@ -393,7 +404,7 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
     from .engine import WcsphSimulation
 
     validate_config(cfg)
-    check_supported(cfg)
+    check_supported(cfg, rhs=True)
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     pos, mass, n_pool = make_particles(cfg)
@@ -417,7 +428,8 @@ def run(cfg: Config, output=None, plot: bool = True, precision: str | None = Non
                           diffusion_form=cfg.density_diffusion_form,
                           eigenvalue_floor=cfg.density_diffusion_eigenvalue_floor,
                           precision=precision, device=device,
-                          adaptive=adaptive_controls(cfg) if cfg.adaptive_dt else None)
+                          adaptive=adaptive_controls(cfg) if cfg.adaptive_dt else None,
+                          rhs_options=stepper_options(cfg))
     out = Path(output) if output is not None else None
     if out is not None:
         out.mkdir(parents=True, exist_ok=True)

@@ -65,7 +65,7 @@ class WcsphSimulation:
     def __init__(self, pos, vel, mass, wall, *, h, rho0, sound_speed, gamma, gravity, viscosity,
                  dt, wall_mass, skin, lo, hi, tol, wall_friction=0.0, density_diffusion=0.0,
                  diffusion_form="molteni", eigenvalue_floor=0.1, precision="fp32", clamp=True,
-                 device="cuda", obstacle=None, adaptive=None):
+                 device="cuda", obstacle=None, adaptive=None, rhs_options=None):
         dev = torch.device(device)
         pos = torch.as_tensor(pos, dtype=torch.float64, device=dev)
         vel = torch.as_tensor(vel, dtype=torch.float64, device=dev)
@@ -89,6 +89,20 @@ class WcsphSimulation:
             op.diffusion_scale = 2.0 * density_diffusion * sound_speed * h
             op.diffusion_form = 1 if diffusion_form == "antuono" else 0
         op.eigenvalue_floor = eigenvalue_floor
+        if rhs_options:
+            # 2-D rhs switches (droplet_impact.py:954-964, 1031-1112): keys
+            # density_renormalization + lattice_kernel_sum,
+            # boundary_consistent_density, surface_pressure_closure +
+            # surface_pressure_threshold / _donors / _layers
+            if rhs_options.get("density_renormalization"):
+                op.density_renormalization = 1
+                op.lattice_kernel_sum = float(rhs_options["lattice_kernel_sum"])
+            op.boundary_consistent_density = 1 if rhs_options.get("boundary_consistent_density") else 0
+            if rhs_options.get("surface_pressure_closure"):
+                op.surface_pressure_closure = 1
+                op.surface_pressure_threshold = float(rhs_options["surface_pressure_threshold"])
+                op.surface_pressure_donors_all = 1 if rhs_options["surface_pressure_donors"] == "all" else 0
+                op.surface_pressure_layers = int(rhs_options["surface_pressure_layers"])
         p.skin = skin
         for d in range(3):
             p.lo[d] = lo[d] if d < dim else 0.0

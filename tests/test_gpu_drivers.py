@@ -291,3 +291,55 @@ def test_adaptive_run_reproduces_fixed_rows_below_the_cap(tmp_path):
     assert sorted(p.name for p in (tmp_path / "adaptive").glob("particles_*.pt")) == [
         "particles_000000.pt", "particles_000003.pt", "particles_000006.pt"]
 
+
+
+@pytest.mark.parametrize("switches", [
+    {"density_renormalization": True},
+    {"boundary_consistent_density": True},
+    {"surface_pressure_closure": True},
+    {"surface_pressure_closure": True, "surface_pressure_donors": "all",
+     "boundary_consistent_density": True},
+])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_run_case_rhs_switches_track_oracle_loop(switches, precision):
+    """dam_break.run_case with the rhs's Config switches on the resident
+    stepper (droplet_impact.py:954-964, 1031-1112) against the oracle's
+    step loop with the same branches: FP64 to 1e-9, FP32-compensated to
+    1e-5 (front, energies, final positions)."""
+    import dataclasses
+    from tensorsph_amd.cases import lattice_kernel_sum, lattice_mass_unit, make_column, wall_lattice
+    from tensorsph_amd.dam_break import DamBreakConfig, run_case
+    cfg = DamBreakConfig(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.5, sound_speed=20.0, dt=3.0e-4, viscosity=1e-3, steps=120,
+                         output_every=40)
+    cfg = dataclasses.replace(cfg, **switches)
+    res = run_case(cfg, precision=precision)
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=cfg.tank_top())
+    h = cfg.h_factor * cfg.dx
+    ref = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0, c0=cfg.sound_speed,
+                        gamma=cfg.gamma, gravity=cfg.gravity, nu=cfg.viscosity, dt=cfg.dt,
+                        wall_mass=lattice_mass_unit(cfg), skin=0.25 * h, lo=[0, 0],
+                        hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx,
+                        shepard=cfg.density_renormalization,
+                        lattice_sum=lattice_kernel_sum(h, cfg.dx),
+                        bcd=cfg.boundary_consistent_density, closure=cfg.surface_pressure_closure,
+                        closure_threshold=cfg.surface_pressure_threshold,
+                        closure_literal=cfg.surface_pressure_donors == "all",
+                        closure_layers=cfg.surface_pressure_layers)
+    front, ke = [], []
+    for k in range(cfg.steps + 1):
+        if k % cfg.output_every == 0 or k == cfg.steps:
+            front.append(ref.pos[:, 0].max())
+            ke.append(0.5 * (mass[:, None] * ref.vel ** 2).sum())
+        if k < cfg.steps:
+            ref.step()
+    bar = 1e-9 if precision == "fp64" else 1e-5
+    assert np.allclose(res.front, front, rtol=bar, atol=0)
+    assert np.allclose(res.kinetic_energy, ke, rtol=10 * bar, atol=10 * bar * max(ke))
+    assert np.abs(n(res.final_positions) - ref.pos).max() < (1e-10 if precision == "fp64" else 1e-6)
+    # and the switch changes the answer at these bars
+    plain = run_case(dataclasses.replace(cfg, density_renormalization=False,
+                                         boundary_consistent_density=False,
+                                         surface_pressure_closure=False), precision=precision)
+    assert np.abs(n(plain.final_positions) - n(res.final_positions)).max() > 1e-7
