@@ -403,6 +403,21 @@ int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps)
 int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream_t stream);
 int tsph_sim_destroy(tsph_sim* sim);
 
+/* ---------------------------------------------------------------- particle shifting
+ * particle_shifting.shift_positions (particle_shifting.py:88-224) applied to
+ * the 2-D stepper's fluid after its last step, as dam_break.run_case does
+ * every shift_every steps (dam_break.py:424-433): concentration C from the
+ * last evaluation's volumes m / rho over the fluid pairs within 2h, shift
+ * -strength h v_ref dt grad C, tangential where C < free_surface C_ref,
+ * capped at limit h.  C_ref is c_reference when > 0, else the percentile of
+ * C (linear interpolation).  v_ref <= 0 takes max(largest speed at the start
+ * of the last step, v_floor).  info (may be NULL): C_ref, largest shift,
+ * fraction of rows capped, v_ref.  Synchronises. */
+typedef struct tsph_shift_params {
+  double dt, v_ref, v_floor, strength, limit, free_surface, percentile, c_reference;
+} tsph_shift_params;
+int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream);
+
 /* ---------------------------------------------------------------- gauges
  * The water-height gauges and obstacle pressure taps of the 3-D dam break
  * against a box (kleefsman_obstacle_3d.py:273-318, marin_probes.py:25-79).

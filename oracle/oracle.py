@@ -525,3 +525,41 @@ def kleefsman_gauges(pos, volume, rho, pressure, *, station_x, aperture, tank_de
     p = np.clip((w * corr).sum(0) / np.maximum(den, np.finfo(np.float64).tiny), 0.0, None)
     return {"slab": slab, "peak": peak, "front": front, "bypass": bypass, "h2": h2,
             "tap_p": p, "tap_support": (w > 0).sum(0)}
+
+
+def lind_shift(pos, volume, h, *, dt, v_ref, strength=1.0, limit=0.25, free_surface=0.95,
+               percentile=95.0, c_reference=None):
+    """particle_shifting.shift_positions (particle_shifting.py:88-224), 2-D,
+    FP64: C_i = V_i W(0) + sum_j V_j W_ij over pairs within 2h; grad C_i =
+    sum_j V_j (C_j - C_i) grad_i W_ij; shift = -strength h v_ref dt grad C;
+    rows with C_i < free_surface * C_ref (C_ref: the percentile of C, linear
+    interpolation) keep only the part of the shift normal to n = -grad C /
+    |grad C|; |shift| is capped at limit * h.  Returns (new positions, C,
+    C_ref, free-surface mask)."""
+    pos = np.asarray(pos, np.float64)
+    vol = np.asarray(volume, np.float64)
+    pairs = cell_linked_pairs(pos, 2.0 * h)
+    i, j = pairs[:, 0], pairs[:, 1]
+    d = pos[i] - pos[j]
+    r = np.sqrt((d * d).sum(1))
+    w0, _, _ = kernel_eval(0, np.zeros(1), h)
+    w, g, _ = kernel_eval(0, r, h)          # 2-D cubic spline; g = dW/dr / r
+    c = vol * w0[0]
+    np.add.at(c, i, vol[j] * w)
+    np.add.at(c, j, vol[i] * w)
+    gw = d * g[:, None]
+    diff = (c[j] - c[i])[:, None] * gw
+    grad = np.zeros_like(pos)
+    np.add.at(grad, i, vol[j, None] * diff)
+    np.add.at(grad, j, vol[i, None] * diff)
+    norm = np.sqrt((grad * grad).sum(1))
+    normal = np.where(norm[:, None] > 1e-30, -grad / np.maximum(norm, 1e-30)[:, None], 0.0)
+    c_ref = float(np.percentile(c, percentile)) if c_reference is None else float(c_reference)
+    fs = c < free_surface * c_ref if c_ref > 0 else np.zeros(len(c), bool)
+    shift = -(strength * h * v_ref * dt) * grad
+    proj = (shift * normal).sum(1, keepdims=True) * normal
+    shift = np.where(fs[:, None], shift - proj, shift)
+    cap = limit * h
+    mag = np.sqrt((shift * shift).sum(1))
+    shift = np.where((mag > cap)[:, None], shift * (cap / np.maximum(mag, 1e-300))[:, None], shift)
+    return pos + shift, c, c_ref, fs

@@ -277,9 +277,10 @@ _UNSUPPORTED = (
 _RHS_ONLY = ("surface_pressure_closure", "density_renormalization", "boundary_consistent_density")
 
 
-def check_supported(cfg: Config, rhs: bool = False) -> None:
+def check_supported(cfg: Config, rhs: bool = False, shifting: bool = False) -> None:
     """Reject opt-in branches that are not part of the particle pipeline
-    (with rhs=True, those the operator rhs does not implement)."""
+    (with rhs=True, those the operator rhs does not implement; with
+    shifting=True the caller runs Lind shifting, shift_every)."""
     for name, on in _UNSUPPORTED:
         if rhs and name in _RHS_ONLY:
             continue
@@ -288,6 +289,8 @@ def check_supported(cfg: Config, rhs: bool = False) -> None:
                 f"Config.{name} is an opt-in rhs branch outside the GPU particle "
                 f"pipeline; switch it off to run on tensorsph_amd")
     for name in ("shift_every", "tvc_every"):
+        if name == "shift_every" and shifting:
+            continue
         if getattr(cfg, name, 0):
             raise NotImplementedError(
                 f"{type(cfg).__name__}.{name} (position shifting) is not part of the "

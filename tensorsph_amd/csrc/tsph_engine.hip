@@ -1077,6 +1077,7 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
   const double rho_max = ord_dec_host(rec->maxes[MM_PEAK_RHO]);
   const double spd = ord_dec_host(rec->maxes[MM_PEAK_SPEED]);
   sim->peak_rho = rho_max > sim->peak_rho ? rho_max : sim->peak_rho;
+  sim->last_speed = spd;
   sim->peak_speed = spd > sim->peak_speed ? spd : sim->peak_speed;
   if (load_rows(sim)) {
     sim->load_rows.push_back((double)k);
@@ -1386,7 +1387,8 @@ int tsph_sim_destroy(tsph_sim* sim) {
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
-                    &sim->load_part, &sim->loads_out, &sim->clamp_acc, &sim->closure};
+                    &sim->load_part, &sim->loads_out, &sim->clamp_acc, &sim->closure, &sim->shift_buf,
+                    &sim->shift_tmp};
   for (DevBuf* b : bufs) b->release();
   for (int r = 0; r < kRing; ++r) if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
   for (int r = 0; r < kRing; ++r)
@@ -1398,3 +1400,207 @@ int tsph_sim_destroy(tsph_sim* sim) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- particle shifting
+// particle_shifting.shift_positions (particle_shifting.py:88-224) on the
+// stepper's fluid rows, over the fluid entries of the slot lists within 2h:
+// C_i = V_i W(0) + sum_j V_j W_ij, grad C_i = sum_j V_j (C_j - C_i) grad_i W_ij,
+// shift = -A grad C (A = strength h v_ref dt), tangential at the free surface
+// (C_i < free_surface C_ref), capped at limit h.
+namespace tsph {
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_shift_conc(WcsphArgs<S> a, NbSlots nbr, double* __restrict__ c) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  Sum<double> acc;
+  nbr.template visit<D>(i, false, [&](int64_t j) {
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double df = xi[d] - a.pos[j * D + d];
+      r2 += df * df;
+    }
+    if (r2 > a.skip2d) return;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    acc.add((double)a.vol[j] * w);
+  });
+  c[i] = (double)a.vol[i] * a.kd.w0d + hi(acc);
+}
+
+struct ShiftArgs {
+  double amp, cap, fs_c;     // A, limit h, free_surface * C_ref (<= 0: no free surface)
+};
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_shift_vec(WcsphArgs<S> a, NbSlots nbr, const double* __restrict__ c,
+                                                      ShiftArgs sa, double* __restrict__ shift,
+                                                      unsigned long long* __restrict__ stats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  const double ci = c[i];
+  Sum<double> gr[D];
+  nbr.template visit<D>(i, false, [&](int64_t j) {
+    double df[D], r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) { df[d] = xi[d] - a.pos[j * D + d]; r2 += df[d] * df[d]; }
+    if (r2 > a.skip2d) return;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    const double s = (double)a.vol[j] * (c[j] - ci);
+#pragma unroll
+    for (int d = 0; d < D; ++d) gr[d].add(s * (df[d] * g));
+  });
+  double gv[D], sh[D], nn = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) { gv[d] = hi(gr[d]); nn += gv[d] * gv[d]; sh[d] = -sa.amp * gv[d]; }
+  const double norm = sqrt(nn);
+  if (sa.fs_c > 0.0 && ci < sa.fs_c) {
+    double nh[D], dot = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      nh[d] = norm > 1.0e-30 ? -gv[d] / norm : 0.0;
+      dot += sh[d] * nh[d];
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) sh[d] = sh[d] - dot * nh[d];
+  }
+  double m2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) m2 += sh[d] * sh[d];
+  double mag = sqrt(m2);
+  if (mag > sa.cap) {
+    const double sc = sa.cap / (mag > 1.0e-300 ? mag : 1.0e-300);
+#pragma unroll
+    for (int d = 0; d < D; ++d) sh[d] = sh[d] * sc;
+    atomicAdd(&stats[0], 1ull);
+    mag = sa.cap;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) shift[i * D + d] = sh[d];
+  atomicMax(&stats[1], ord_enc(mag));
+}
+
+// moves the rows and measures the displacement since the lists were built
+template <int D>
+__global__ void k_shift_move(double* __restrict__ pos, const double* __restrict__ shift,
+                             const double* __restrict__ anchor, int64_t n, unsigned long long* __restrict__ stats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double x = pos[i * D + d] + shift[i * D + d];
+    pos[i * D + d] = x;
+    const double e = x - anchor[i * D + d];
+    d2 += e * e;
+  }
+  atomicMax(&stats[2], ord_enc(sqrt(d2)));
+}
+
+// row values to the caller's order and back, through the row -> caller map
+template <typename S>
+__global__ void k_rows_to_caller(const int64_t* __restrict__ orig, int64_t n, const S* __restrict__ src,
+                                 S* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[orig[i]] = src[i];
+}
+template <typename S>
+__global__ void k_caller_to_rows(const int64_t* __restrict__ orig, int64_t n, const S* __restrict__ src,
+                                 S* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[orig[i]];
+}
+
+template <int D, typename S>
+int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream) {
+  const int64_t n = sim->n;
+  if (sim->rebuild_pending) {
+    // the lists went stale in the last step: re-bin, and carry the last
+    // evaluation's volumes into the new row order
+    TSPH_TRY(sim->shift_tmp.ensure(sizeof(S) * (size_t)(n + 1)));
+    k_rows_to_caller<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->vol.as<S>(),
+                                                            sim->shift_tmp.as<S>());
+    TSPH_CHECK_LAUNCH();
+    bool done = false;
+    if constexpr (D == 2) {
+      if (sim->tiles) { TSPH_TRY((sim_rebuild_tiles<D, S>(sim, stream))); done = true; }
+    }
+    if (!done) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
+    sim->rebuild_pending = false;
+    k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->shift_tmp.as<S>(),
+                                                            sim->vol.as<S>());
+    TSPH_CHECK_LAUNCH();
+  }
+  const size_t cb = (sizeof(double) * (size_t)(n + 1) + 255) / 256 * 256;
+  TSPH_TRY(sim->shift_buf.ensure(2 * cb + (sizeof(double) * (size_t)n * D + 255) / 256 * 256 + 256));
+  char* base = sim->shift_buf.as<char>();
+  double* c = (double*)base;
+  double* sorted = (double*)(base + cb);
+  double* shift = (double*)(base + 2 * cb);
+  unsigned long long* st = (unsigned long long*)(base + 2 * cb + (sizeof(double) * (size_t)n * D + 255) / 256 * 256);
+  WcsphArgs<S> a = sim_args<D, S>(sim);
+  const NbSlots nbr = sim_slots(sim);
+  k_shift_conc<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c);
+  TSPH_CHECK_LAUNCH();
+  double c_ref = prm->c_reference;
+  if (!(c_ref > 0.0)) {
+    size_t bytes = 0;
+    TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, c, sorted, (int)n, 0, 64, stream));
+    TSPH_TRY(sim->shift_tmp.ensure(bytes > sizeof(S) * (size_t)(n + 1) ? bytes : sizeof(S) * (size_t)(n + 1)));
+    TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(sim->shift_tmp.ptr, bytes, c, sorted, (int)n, 0, 64, stream));
+    // linear interpolation between the order statistics around q (n - 1)
+    const double q = (prm->percentile < 0.0 ? 0.0 : (prm->percentile > 100.0 ? 100.0 : prm->percentile)) / 100.0;
+    const double at = q * (double)(n - 1);
+    const int64_t lo = (int64_t)floor(at), hi_i = lo + 1 < n ? lo + 1 : lo;
+    double v[2];
+    TSPH_CHECK_HIP(hipMemcpyAsync(&v[0], sorted + lo, sizeof(double), hipMemcpyDeviceToHost, stream));
+    TSPH_CHECK_HIP(hipMemcpyAsync(&v[1], sorted + hi_i, sizeof(double), hipMemcpyDeviceToHost, stream));
+    TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+    const double frac = at - (double)lo;
+    c_ref = v[0] + frac * (v[1] - v[0]);
+  }
+  double v_ref = prm->v_ref;
+  if (!(v_ref > 0.0)) v_ref = sim->last_speed > prm->v_floor ? sim->last_speed : prm->v_floor;
+  ShiftArgs sa;
+  sa.amp = prm->strength * sim->prm.op.h * v_ref * prm->dt;
+  sa.cap = prm->limit * sim->prm.op.h;
+  sa.fs_c = c_ref > 0.0 ? prm->free_surface * c_ref : 0.0;
+  TSPH_CHECK_HIP(hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), stream));
+  k_shift_vec<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c, sa, shift, st);
+  k_shift_move<D><<<grid_for(n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), shift,
+                                                      sim->anchor.as<double>(), n, st);
+  TSPH_CHECK_LAUNCH();
+  unsigned long long h[3];
+  TSPH_CHECK_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  const double disp = n > 0 ? ord_dec_host(h[2]) : 0.0;
+  if (disp > 0.5 * sim->prm.skin) sim->rebuild_pending = true;
+  if (info) {
+    info[0] = c_ref;
+    info[1] = n > 0 ? ord_dec_host(h[1]) : 0.0;
+    info[2] = n > 0 ? (double)h[0] / (double)n : 0.0;
+    info[3] = v_ref;
+  }
+  return TSPH_OK;
+}
+
+}  // namespace tsph
+
+extern "C" int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream) {
+  TSPH_REQUIRE(sim && prm, TSPH_ERR_ARG, "bad arguments");
+  TSPH_REQUIRE(sim->dim == 2, TSPH_ERR_ARG, "particle shifting runs on the 2-D stepper");
+  TSPH_REQUIRE(sim->steps_done > 0, TSPH_ERR_ARG, "shifting needs the volumes of an evaluated step");
+  TSPH_REQUIRE(prm->dt > 0.0 && prm->limit >= 0.0 && prm->strength >= 0.0, TSPH_ERR_ARG,
+               "dt must be positive, strength and limit non-negative");
+  TSPH_REQUIRE(sim->n < (1ll << 31) - 1, TSPH_ERR_ARG, "too many rows for the concentration sort");
+  if (sim->prec == TSPH_F64) return sim_shift<2, double>(sim, prm, info, stream);
+  return sim_shift<2, float>(sim, prm, info, stream);
+}

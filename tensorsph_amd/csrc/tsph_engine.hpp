@@ -67,6 +67,8 @@ struct tsph_sim {
   tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
   tsph::DevBuf aux;                 // FP32: Vec4 (m, p / rho^2, m / rho, 0) per row
   tsph::DevBuf closure;             // free-surface pressure closure: two pressure and two flag buffers
+  tsph::DevBuf shift_buf, shift_tmp;  // particle shifting: C, sorted C, shift vectors, counters; sort scratch
+  double last_speed = 0.0;          // largest speed at the start of the last step run
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
   tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)

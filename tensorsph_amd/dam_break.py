@@ -4,9 +4,10 @@
 pair cache, rhs, symplectic Euler, apply_boundary: the loop of
 dam_break.py:374-465) and returns the same ``DamBreakRun`` record: the
 sampled front position and energies, every-step extrema, the wall-impact
-time and the compression before it.  Position shifting and transport-velocity
-correction (``shift_every``, ``tvc_every``) are not part of the GPU particle
-pipeline and raise NotImplementedError; the adaptive step runs on the device.
+time and the compression before it.  Lind shifting (``shift_every``) runs on
+the device after the steps it follows (tsph_sim_shift); the
+transport-velocity correction (``tvc_every``) is not built and raises
+NotImplementedError; the adaptive step runs on the device.
 """
 from __future__ import annotations
 
@@ -134,7 +135,10 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     from .droplet_impact import _state_time, stepper_options
 
     validate_case(cfg)
-    check_supported(cfg, rhs=True)
+    check_supported(cfg, rhs=True, shifting=True)
+    se = int(cfg.shift_every or 0)
+    if se and cfg.adaptive_dt:
+        raise NotImplementedError("Lind shifting runs with the fixed step here (adaptive_dt off)")
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     dev = torch.device(device) if device is not None else torch.device("cuda")
@@ -173,7 +177,17 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
             if final or step > max_steps:
                 break
             nxt = min((step // ow + 1) * ow, max_steps + 1)
+            if se:
+                # stop after each step s > 0 with s % shift_every == 0 (not the last):
+                # the shift moves the fluid before step s + 1 (dam_break.py:424-433)
+                s = max(step, 1)
+                s = ((s + se - 1) // se) * se
+                if s < cfg.steps:
+                    nxt = min(nxt, s + 1)
             sim.step(nxt - step, meter_every=1)
+            if se and (nxt - 1) > 0 and (nxt - 1) % se == 0 and (nxt - 1) < cfg.steps:
+                sim.shift(cfg.dt, v_ref=cfg.shift_v_ref, v_floor=0.05 * cfg.sound_speed,
+                          strength=cfg.shift_strength, limit=cfg.shift_limit)
             if adaptive and sim.stats()["finished"]:
                 step = sim.stats()["steps"] - 1
             else:

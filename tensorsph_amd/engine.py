@@ -233,6 +233,20 @@ def _gauges(self, spec) -> np.ndarray:
     return out
 
 
+def _shift(self, dt, v_ref=None, v_floor=0.0, strength=1.0, limit=0.25, free_surface=0.95,
+           percentile=95.0, c_reference=None) -> dict:
+    """tsph_sim_shift: one Lind shifting pass on the fluid after the last
+    step (v_ref None: max(speed at the start of that step, v_floor))."""
+    p = _lib.ShiftParams(float(dt), -1.0 if v_ref is None else float(v_ref), float(v_floor),
+                         float(strength), float(limit), float(free_surface), float(percentile),
+                         0.0 if c_reference is None else float(c_reference))
+    info = (C.c_double * 4)()
+    _lib.check(_lib.lib().tsph_sim_shift(self.handle, C.byref(p), info,
+                                         _lib.stream_handle(self.device)))
+    return {"c_reference": info[0], "max_shift": info[1], "clipped": info[2], "v_ref": info[3]}
+
+
+WcsphSimulation.shift = _shift
 WcsphSimulation.gauges = _gauges
 WcsphSimulation.set_timing = _set_timing
 WcsphSimulation.sweep_times = _sweep_times

@@ -26,6 +26,8 @@ Fixtures
   kleefsman_gauges.npz  the Kleefsman gauge and tap operators on a cloud, and a
                      short run_case series; kleefsman_checkpoint_v3.pt a
                      checkpoint written by save_checkpoint
+  lind_shift.npz     particle_shifting.shift_positions on a seeded column and a
+                     dam_break.run_case with shift_every
   rhs2d_opts.npz     droplet_impact.rhs with its opt-in branches: wrap_width,
                      nu_field, density_offset, density renormalisation,
                      boundary-consistent density, surface-pressure closure
@@ -571,6 +573,48 @@ def kleefsman_gauges(reference: str = "/root/reference") -> None:
     print("kleefsman fixtures written to", HERE)
 
 
+def lind_shift(reference: str = "/root/reference") -> None:
+    """particle_shifting.shift_positions (particle_shifting.py:166-224) on a
+    seeded 2-D column with a free surface, and dam_break.run_case with
+    shift_every (dam_break.py:424-433)."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import dam_break as db
+    from tensorsph.particle_shifting import shift_positions
+    torch.set_default_dtype(torch.float64)
+    out = {}
+    g = torch.Generator().manual_seed(31)
+    dx = 0.02
+    h = 1.3 * dx
+    xs = (torch.arange(20) + 0.5) * dx
+    ys = (torch.arange(12) + 0.5) * dx
+    pos = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)
+    pos = pos + 0.08 * dx * torch.randn(pos.shape, generator=g)
+    volume = dx * dx * (1.0 + 0.05 * torch.rand(pos.shape[0], generator=g))
+    for tag, kw in (("a", dict(dt=1e-3, v_ref=2.0)), ("clip", dict(dt=1e-2, v_ref=5.0, limit=0.05)),
+                    ("strong", dict(dt=2e-3, v_ref=1.0, strength=3.0, free_surface=0.9))):
+        new, st = shift_positions(pos, volume, h, return_state=True, **kw)
+        out[f"{tag}_out"] = new.numpy()
+        out[f"{tag}_c"] = st.concentration.numpy()
+        out[f"{tag}_cref"] = np.array(st.c_reference)
+        out[f"{tag}_fs"] = st.free_surface.numpy()
+        out[f"{tag}_kw"] = np.array([kw["dt"], kw["v_ref"], kw.get("strength", 1.0), kw.get("limit", 0.25),
+                                     kw.get("free_surface", 0.95)])
+    out.update(pos=pos.numpy(), volume=volume.numpy(), h=np.array(h))
+    cfg = db.DamBreakConfig(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4,
+                            pool_width=0.5, sound_speed=20.0, dt=3.0e-4, viscosity=1e-3, steps=200,
+                            output_every=20, shift_every=5, device="cpu")
+    res = db.run_case(cfg)
+    out["run_times"] = np.array(res.times)
+    out["run_front"] = np.array(res.front)
+    out["run_ke"] = np.array(res.kinetic_energy)
+    out["run_pe"] = np.array(res.potential_energy)
+    out["run_final"] = res.final_positions.numpy()
+    out["run_rebuilds"] = np.array(res.rebuilds)
+    np.savez_compressed(HERE / "lind_shift.npz", **out)
+    print("shifting fixture written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
@@ -581,9 +625,12 @@ if __name__ == "__main__":
         rhs_options(*args[1:])
     elif args and args[0] == "--only-kleefsman":
         kleefsman_gauges(*args[1:])
+    elif args and args[0] == "--only-shift":
+        lind_shift(*args[1:])
     else:
         main(*args)
         shear(*args)
         order4(*args)
         rhs_options(*args)
         kleefsman_gauges(*args)
+        lind_shift(*args)

@@ -343,3 +343,23 @@ def test_run_case_rhs_switches_track_oracle_loop(switches, precision):
                                          boundary_consistent_density=False,
                                          surface_pressure_closure=False), precision=precision)
     assert np.abs(n(plain.final_positions) - n(res.final_positions)).max() > 1e-7
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_run_case_lind_shifting_matches_reference_run(golden, precision):
+    """dam_break.run_case with shift_every = 5 (dam_break.py:424-433) against
+    the reference's own run: FP64 to 1e-9, FP32-compensated to 1e-5."""
+    from tensorsph_amd.dam_break import DamBreakConfig, run_case
+    g = golden("lind_shift")
+    cfg = DamBreakConfig(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.5, sound_speed=20.0, dt=3.0e-4, viscosity=1e-3, steps=200,
+                         output_every=20, shift_every=5)
+    res = run_case(cfg, precision=precision)
+    bar = 1e-9 if precision == "fp64" else 1e-5
+    assert res.times == pytest.approx(list(g["run_times"]), rel=1e-15)
+    assert np.allclose(res.front, g["run_front"], rtol=bar, atol=0)
+    assert np.allclose(res.kinetic_energy, g["run_ke"], rtol=10 * bar, atol=10 * bar * g["run_ke"].max())
+    assert np.allclose(res.potential_energy, g["run_pe"], rtol=bar, atol=0)
+    assert np.abs(n(res.final_positions) - g["run_final"]).max() < (1e-10 if precision == "fp64" else 1e-6)
+    if precision == "fp64":
+        assert res.rebuilds == int(g["run_rebuilds"])

@@ -752,3 +752,35 @@ def test_kleefsman_checkpoint_fixture_loads_safely(golden):
         load_checkpoint(path, dataclasses.replace(cfg, dx=0.04))
     # t_end is a stopping rule, not part of the fingerprint
     load_checkpoint(path, dataclasses.replace(cfg, t_end=3.0))
+
+
+# ----------------------------------------------------------------- Lind shifting (rank 3)
+
+@pytest.mark.parametrize("tag", ["a", "clip", "strong"])
+def test_lind_shift_matches_reference(golden, tag):
+    g = golden("lind_shift")
+    dt, v, st, lim, fs = g[f"{tag}_kw"]
+    out, c, cref, fsm = orc.lind_shift(g["pos"], g["volume"], float(g["h"]), dt=dt, v_ref=v,
+                                       strength=st, limit=lim, free_surface=fs)
+    assert np.abs(out - g[f"{tag}_out"]).max() < 1e-15
+    assert scaled_err(c, g[f"{tag}_c"]) < 1e-14
+    assert cref == pytest.approx(float(g[f"{tag}_cref"]), rel=1e-15)
+    assert np.array_equal(fsm, g[f"{tag}_fs"])
+
+
+def test_lind_shift_closed_forms():
+    """A perfect lattice is a fixed point more than 4h from its edges (every
+    neighbour within 2h then has the full-support C, so grad C = 0), the cap
+    holds, and a zero strength moves nothing."""
+    dx = 0.02
+    h = 1.3 * dx
+    ax = (np.arange(24) + 0.5) * dx
+    pos = np.stack(np.meshgrid(ax, ax, indexing="ij"), -1).reshape(-1, 2)
+    vol = np.full(len(pos), dx * dx)
+    out, c, cref, fs = orc.lind_shift(pos, vol, h, dt=1e-2, v_ref=5.0)
+    interior = np.all((pos > 4.1 * h) & (pos < 24 * dx - 4.1 * h), axis=1)
+    assert interior.sum() > 100
+    assert np.abs(out - pos)[interior].max() < 1e-15
+    assert np.sqrt(((out - pos) ** 2).sum(1)).max() <= 0.25 * h * (1 + 1e-12)
+    out0, *_ = orc.lind_shift(pos, vol, h, dt=1e-2, v_ref=5.0, strength=0.0)
+    assert np.array_equal(out0, pos)
