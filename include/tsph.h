@@ -418,6 +418,24 @@ typedef struct tsph_shift_params {
 } tsph_shift_params;
 int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream);
 
+/* transport_velocity_correction (transport_velocity_correction.py:285-513)
+ * applied to the 2-D stepper's fluid after its last step, as
+ * dam_break.run_case does every tvc_every steps (dam_break.py:434-460):
+ * g_i = -2 sum_j V_j grad_i W_ij over the fluid and wall neighbours within
+ * 2h (fluid V = m / rho of the last evaluation, wall V = wall_volume),
+ * shift = alpha h^2 L(h^2 |g|^2) g with L = min(limiter_slope x, 1) when
+ * limiter_slope > 0 (TruncatedLinear) else 1; scope 1 keeps rows with
+ * C >= bulk_c C_ref (C, C_ref as for tsph_sim_shift), scope 2 rows with
+ * |g| h <= bulk_g; max_shift > 0 caps |shift|; then the boundary clamp.
+ * info (may be NULL): largest |shift|, fraction capped, boundary crossings
+ * (counted into the stepper's escapes).  Synchronises. */
+typedef struct tsph_tvc_params {
+  double alpha, limiter_slope, max_shift, wall_volume;
+  int32_t scope;
+  double bulk_c, bulk_g, percentile;
+} tsph_tvc_params;
+int tsph_sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* info, hipStream_t stream);
+
 /* ---------------------------------------------------------------- gauges
  * The water-height gauges and obstacle pressure taps of the 3-D dam break
  * against a box (kleefsman_obstacle_3d.py:273-318, marin_probes.py:25-79).

@@ -563,3 +563,43 @@ def lind_shift(pos, volume, h, *, dt, v_ref, strength=1.0, limit=0.25, free_surf
     mag = np.sqrt((shift * shift).sum(1))
     shift = np.where((mag > cap)[:, None], shift * (cap / np.maximum(mag, 1e-300))[:, None], shift)
     return pos + shift, c, c_ref, fs
+
+
+def tvc_shift(pos, volume, h, n_fluid, pairs, *, alpha=0.05, slope=None, max_shift=None, active=None,
+              scope_tol=None):
+    """transport_velocity_correction (transport_velocity_correction.py:285-513)
+    in 2-D FP64 with unit kernel correction: g_i = -sum_j f_ij V_j grad_i W_ij
+    over the pairs (f = 2 for fluid-fluid, the wall factor 2 for fluid-wall),
+    shift = alpha h^2 L(h^2 |g|^2) g with L = 1 or min(slope x, 1), zeroed
+    outside ``active`` and where |g| h > scope_tol, capped at max_shift.
+    Returns (g, shift) over the fluid rows."""
+    pos = np.asarray(pos, np.float64)
+    vol = np.asarray(volume, np.float64)
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    i, j = pairs[:, 0], pairs[:, 1]
+    d = pos[i] - pos[j]
+    r = np.sqrt((d * d).sum(1))
+    _, gs, _ = kernel_eval(0, r, h)
+    gw = d * gs[:, None]                       # grad_i W_ij; grad_j W_ij = -gw
+    g = np.zeros((n_fluid, 2))
+    fi, fj = i < n_fluid, j < n_fluid
+    ff = fi & fj
+    np.add.at(g, i[ff], (2.0 * vol[j[ff]])[:, None] * gw[ff])
+    np.add.at(g, j[ff], (2.0 * vol[i[ff]])[:, None] * -gw[ff])
+    fw = fi & ~fj
+    np.add.at(g, i[fw], (2.0 * vol[j[fw]])[:, None] * gw[fw])
+    wf = ~fi & fj
+    np.add.at(g, j[wf], (2.0 * vol[i[wf]])[:, None] * -gw[wf])
+    g = -g
+    sq = (g * g).sum(1)
+    fac = np.ones_like(sq) if slope is None else np.minimum(slope * h * h * sq, 1.0)
+    shift = (alpha * h * h) * fac[:, None] * g
+    if active is not None:
+        shift = shift * np.asarray(active, np.float64)[:, None]
+    if scope_tol is not None:
+        shift = shift * (np.sqrt(sq) * h <= scope_tol)[:, None]
+    if max_shift is not None:
+        nrm = np.sqrt((shift * shift).sum(1))
+        over = nrm > max_shift
+        shift = np.where(over[:, None], shift * (max_shift / np.maximum(nrm, 1e-30))[:, None], shift)
+    return g, shift

@@ -78,6 +78,11 @@ class GaugeSpec(C.Structure):
 GAUGE_ROW = 117
 
 
+class TvcParams(C.Structure):
+    _fields_ = [("alpha", _dbl), ("limiter_slope", _dbl), ("max_shift", _dbl), ("wall_volume", _dbl),
+                ("scope", C.c_int32), ("bulk_c", _dbl), ("bulk_g", _dbl), ("percentile", _dbl)]
+
+
 class ShiftParams(C.Structure):
     _fields_ = [(k, _dbl) for k in ("dt", "v_ref", "v_floor", "strength", "limit", "free_surface",
                                     "percentile", "c_reference")]
@@ -133,6 +138,7 @@ def _declare(L):
         "tsph_gauges": [_vp, C.POINTER(GaugeSpec), C.c_int, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_sim_gauges": [_vp, C.POINTER(GaugeSpec), _vp, _vp],
         "tsph_sim_shift": [_vp, C.POINTER(ShiftParams), _vp, _vp],
+        "tsph_sim_tvc": [_vp, C.POINTER(TvcParams), _vp, _vp],
         "tsph_eos_tait": [_vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dbl, _vp, _vp],
         "tsph_kernel_eval": [C.c_int, _vp, _i64, C.c_int, _dbl, _vp, _vp, _vp, _vp],
         "tsph_apply_boundary": [_vp, _vp, _i64, C.c_int, _dp, _dp, _dbl, _vp, _vp],

@@ -289,7 +289,7 @@ def check_supported(cfg: Config, rhs: bool = False, shifting: bool = False) -> N
                 f"Config.{name} is an opt-in rhs branch outside the GPU particle "
                 f"pipeline; switch it off to run on tensorsph_amd")
     for name in ("shift_every", "tvc_every"):
-        if name == "shift_every" and shifting:
+        if shifting:
             continue
         if getattr(cfg, name, 0):
             raise NotImplementedError(

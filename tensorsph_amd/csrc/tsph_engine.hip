@@ -1290,7 +1290,7 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
   if (counters) {
     counters[0] = sim->steps_done;
     counters[1] = sim->rebuilds;
-    counters[2] = sim->escaped_total;
+    counters[2] = sim->escaped_total + sim->escaped_extra;
     counters[3] = sim->ncells_dense;
     counters[4] = (int64_t)(sim->rows.size() / kMeterWidth);
     counters[5] = sim->finished ? 1 : 0;
@@ -1519,9 +1519,17 @@ __global__ void k_caller_to_rows(const int64_t* __restrict__ orig, int64_t n, co
   if (i < n) dst[i] = src[orig[i]];
 }
 
+// A correction that needs the pairs within 2h of the current positions re-bins
+// first when the last step left the lists stale, and carries the last
+// evaluation's volumes into the new row order through the caller's order.
+// That re-binning only completes the lists for the correction: it is not
+// counted, and *rebuilt tells the caller to leave a rebuild pending, so the
+// next step re-bins the corrected positions as the reference's pair cache
+// does (same list builds, same anchors).
 template <int D, typename S>
-int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream) {
+int rebuild_keep_volumes(tsph_sim* sim, hipStream_t stream, bool* rebuilt) {
   const int64_t n = sim->n;
+  *rebuilt = false;
   if (sim->rebuild_pending) {
     // the lists went stale in the last step: re-bin, and carry the last
     // evaluation's volumes into the new row order
@@ -1535,10 +1543,39 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
     }
     if (!done) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
     sim->rebuild_pending = false;
+    sim->rebuilds--;
+    *rebuilt = true;
     k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->shift_tmp.as<S>(),
                                                             sim->vol.as<S>());
     TSPH_CHECK_LAUNCH();
   }
+  return TSPH_OK;
+}
+
+// the percentile of c[0, n) with linear interpolation between order statistics
+template <typename S>
+int percentile_of(tsph_sim* sim, const double* c, double* sorted, int64_t n, double percentile, double* out,
+                  hipStream_t stream) {
+  size_t bytes = 0;
+  TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, c, sorted, (int)n, 0, 64, stream));
+  TSPH_TRY(sim->shift_tmp.ensure(bytes > sizeof(S) * (size_t)(n + 1) ? bytes : sizeof(S) * (size_t)(n + 1)));
+  TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(sim->shift_tmp.ptr, bytes, c, sorted, (int)n, 0, 64, stream));
+  const double q = (percentile < 0.0 ? 0.0 : (percentile > 100.0 ? 100.0 : percentile)) / 100.0;
+  const double at = q * (double)(n - 1);
+  const int64_t lo = (int64_t)floor(at), hi_i = lo + 1 < n ? lo + 1 : lo;
+  double v[2];
+  TSPH_CHECK_HIP(hipMemcpyAsync(&v[0], sorted + lo, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipMemcpyAsync(&v[1], sorted + hi_i, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  *out = v[0] + (at - (double)lo) * (v[1] - v[0]);
+  return TSPH_OK;
+}
+
+template <int D, typename S>
+int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream) {
+  const int64_t n = sim->n;
+  bool rebuilt = false;
+  TSPH_TRY((rebuild_keep_volumes<D, S>(sim, stream, &rebuilt)));
   const size_t cb = (sizeof(double) * (size_t)(n + 1) + 255) / 256 * 256;
   TSPH_TRY(sim->shift_buf.ensure(2 * cb + (sizeof(double) * (size_t)n * D + 255) / 256 * 256 + 256));
   char* base = sim->shift_buf.as<char>();
@@ -1551,22 +1588,7 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   k_shift_conc<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c);
   TSPH_CHECK_LAUNCH();
   double c_ref = prm->c_reference;
-  if (!(c_ref > 0.0)) {
-    size_t bytes = 0;
-    TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, c, sorted, (int)n, 0, 64, stream));
-    TSPH_TRY(sim->shift_tmp.ensure(bytes > sizeof(S) * (size_t)(n + 1) ? bytes : sizeof(S) * (size_t)(n + 1)));
-    TSPH_CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(sim->shift_tmp.ptr, bytes, c, sorted, (int)n, 0, 64, stream));
-    // linear interpolation between the order statistics around q (n - 1)
-    const double q = (prm->percentile < 0.0 ? 0.0 : (prm->percentile > 100.0 ? 100.0 : prm->percentile)) / 100.0;
-    const double at = q * (double)(n - 1);
-    const int64_t lo = (int64_t)floor(at), hi_i = lo + 1 < n ? lo + 1 : lo;
-    double v[2];
-    TSPH_CHECK_HIP(hipMemcpyAsync(&v[0], sorted + lo, sizeof(double), hipMemcpyDeviceToHost, stream));
-    TSPH_CHECK_HIP(hipMemcpyAsync(&v[1], sorted + hi_i, sizeof(double), hipMemcpyDeviceToHost, stream));
-    TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-    const double frac = at - (double)lo;
-    c_ref = v[0] + frac * (v[1] - v[0]);
-  }
+  if (!(c_ref > 0.0)) TSPH_TRY((percentile_of<S>(sim, c, sorted, n, prm->percentile, &c_ref, stream)));
   double v_ref = prm->v_ref;
   if (!(v_ref > 0.0)) v_ref = sim->last_speed > prm->v_floor ? sim->last_speed : prm->v_floor;
   ShiftArgs sa;
@@ -1582,7 +1604,9 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   TSPH_CHECK_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));
   const double disp = n > 0 ? ord_dec_host(h[2]) : 0.0;
-  if (disp > 0.5 * sim->prm.skin) sim->rebuild_pending = true;
+  // the next step re-bins when the corrected positions are more than half the
+  // skin from the anchors, the pair cache's own test
+  sim->rebuild_pending = rebuilt || disp > 0.5 * sim->prm.skin;
   if (info) {
     info[0] = c_ref;
     info[1] = n > 0 ? ord_dec_host(h[1]) : 0.0;
@@ -1592,7 +1616,152 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   return TSPH_OK;
 }
 
+// Transport-velocity correction (transport_velocity_correction.py:285-513),
+// unit kernel correction: g_i = -2 sum_j V_j grad_i W_ij over fluid and wall
+// neighbours within 2h (wall rows: V = wall_volume), shift = alpha h^2
+// L(h^2 |g|^2) g, masked by the scope, capped at max_shift.
+struct TvcArgs {
+  double coef, slope, cap, wall_volume, h;
+  int scope;                 // 0 all, 1 C >= c_thr, 2 |g| h <= g_tol
+  double c_thr, g_tol;
+};
+
+template <int D, typename S>
+__global__ void __launch_bounds__(kBlock) k_tvc_vec(WcsphArgs<S> a, NbSlots nbr, const double* __restrict__ c,
+                                                    TvcArgs ta, double* __restrict__ shift,
+                                                    unsigned long long* __restrict__ stats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
+  Sum<double> gr[D];
+  nbr.template visit<D>(i, true, [&](int64_t j) {
+    double df[D], r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) { df[d] = xi[d] - a.pos[j * D + d]; r2 += df[d] * df[d]; }
+    if (r2 > a.skip2d) return;
+    double w, g;
+    cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    const double v = j < a.n ? (double)a.vol[j] : ta.wall_volume;
+#pragma unroll
+    for (int d = 0; d < D; ++d) gr[d].add((2.0 * v) * (df[d] * g));
+  });
+  double gv[D], sq = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) { gv[d] = -hi(gr[d]); sq += gv[d] * gv[d]; }
+  double fac = 1.0;
+  if (ta.slope > 0.0) {
+    const double x = ta.slope * (ta.h * ta.h * sq);
+    fac = x <= 1.0 ? x : 1.0;
+  }
+  double sh[D], m2 = 0.0;
+  bool keep = true;
+  if (ta.scope == 1) keep = c[i] >= ta.c_thr;
+  if (ta.scope == 2) keep = sqrt(sq) * ta.h <= ta.g_tol;
+#pragma unroll
+  for (int d = 0; d < D; ++d) { sh[d] = keep ? ta.coef * fac * gv[d] : 0.0; m2 += sh[d] * sh[d]; }
+  double mag = sqrt(m2);
+  if (ta.cap > 0.0 && mag > ta.cap) {
+    const double sc = ta.cap / (mag > 1.0e-30 ? mag : 1.0e-30);
+#pragma unroll
+    for (int d = 0; d < D; ++d) sh[d] = sh[d] * sc;
+    atomicAdd(&stats[0], 1ull);
+    m2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) m2 += sh[d] * sh[d];
+    mag = sqrt(m2);
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) shift[i * D + d] = sh[d];
+  atomicMax(&stats[1], ord_enc(mag));
+}
+
+template <int D>
+__global__ void k_disp(const double* __restrict__ pos, const double* __restrict__ anchor, int64_t n,
+                       unsigned long long* __restrict__ stats) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double e = pos[i * D + d] - anchor[i * D + d];
+    d2 += e * e;
+  }
+  atomicMax(&stats[2], ord_enc(sqrt(d2)));
+}
+
+template <int D, typename S>
+int sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* info, hipStream_t stream) {
+  const int64_t n = sim->n;
+  // the gradient runs over the step's own lists, as the reference passes the
+  // step's pairs; only the bulk scope's concentration needs the pairs within
+  // 2h of the current positions
+  bool rebuilt = false;
+  if (prm->scope == 1) TSPH_TRY((rebuild_keep_volumes<D, S>(sim, stream, &rebuilt)));
+  const size_t cb = (sizeof(double) * (size_t)(n + 1) + 255) / 256 * 256;
+  const size_t sb = (sizeof(double) * (size_t)n * D + 255) / 256 * 256;
+  TSPH_TRY(sim->shift_buf.ensure(2 * cb + sb + 256 + 256));
+  char* base = sim->shift_buf.as<char>();
+  double* c = (double*)base;
+  double* sorted = (double*)(base + cb);
+  double* shift = (double*)(base + 2 * cb);
+  unsigned long long* st = (unsigned long long*)(base + 2 * cb + sb);
+  unsigned long long* cross = st + 4;
+  WcsphArgs<S> a = sim_args<D, S>(sim);
+  const NbSlots nbr = sim_slots(sim);
+  TvcArgs ta{};
+  ta.h = sim->prm.op.h;
+  ta.coef = prm->alpha * ta.h * ta.h;
+  ta.slope = prm->limiter_slope;
+  ta.cap = prm->max_shift;
+  ta.wall_volume = prm->wall_volume;
+  ta.scope = prm->scope;
+  ta.g_tol = prm->bulk_g;
+  if (prm->scope == 1) {
+    k_shift_conc<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c);
+    TSPH_CHECK_LAUNCH();
+    double c_ref = 0.0;
+    TSPH_TRY((percentile_of<S>(sim, c, sorted, n, prm->percentile, &c_ref, stream)));
+    ta.c_thr = prm->bulk_c * c_ref;
+  }
+  TSPH_CHECK_HIP(hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), stream));
+  k_tvc_vec<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c, ta, shift, st);
+  k_shift_move<D><<<grid_for(n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), shift,
+                                                      sim->anchor.as<double>(), n, st);
+  TSPH_CHECK_LAUNCH();
+  TSPH_TRY(tsph_apply_boundary(sim->pos[sim->cur].as<double>(), sim->vel[sim->cur].as<double>(), n, D,
+                               sim->prm.lo, sim->prm.hi, sim->prm.tol, cross, stream));
+  TSPH_CHECK_HIP(hipMemsetAsync(st + 2, 0, sizeof(unsigned long long), stream));
+  k_disp<D><<<grid_for(n), kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), sim->anchor.as<double>(), n, st);
+  TSPH_CHECK_LAUNCH();
+  unsigned long long h[5];
+  TSPH_CHECK_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  sim->rebuild_pending = rebuilt || (n > 0 && ord_dec_host(h[2]) > 0.5 * sim->prm.skin);
+  sim->escaped_extra += (int64_t)h[4];
+  if (info) {
+    info[0] = n > 0 ? ord_dec_host(h[1]) : 0.0;
+    info[1] = n > 0 ? (double)h[0] / (double)n : 0.0;
+    info[2] = (double)h[4];
+  }
+  return TSPH_OK;
+}
+
 }  // namespace tsph
+
+extern "C" int tsph_sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* info, hipStream_t stream) {
+  TSPH_REQUIRE(sim && prm, TSPH_ERR_ARG, "bad arguments");
+  TSPH_REQUIRE(sim->dim == 2, TSPH_ERR_ARG, "the transport-velocity correction runs on the 2-D stepper");
+  TSPH_REQUIRE(sim->steps_done > 0, TSPH_ERR_ARG, "the correction needs the volumes of an evaluated step");
+  TSPH_REQUIRE(prm->scope >= 0 && prm->scope <= 2, TSPH_ERR_ARG, "scope must be 0 (all), 1 (bulk) or 2 (bulk_g)");
+  TSPH_REQUIRE(prm->limiter_slope >= 0.0 && prm->max_shift >= 0.0 && prm->wall_volume >= 0.0, TSPH_ERR_ARG,
+               "limiter slope, max shift and wall volume must be non-negative");
+  TSPH_REQUIRE(prm->scope != 2 || prm->bulk_g >= 0.0, TSPH_ERR_ARG, "scope_tol must be non-negative");
+  TSPH_REQUIRE(sim->n < (1ll << 31) - 1, TSPH_ERR_ARG, "too many rows for the concentration sort");
+  if (sim->prec == TSPH_F64) return sim_tvc<2, double>(sim, prm, info, stream);
+  return sim_tvc<2, float>(sim, prm, info, stream);
+}
 
 extern "C" int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStream_t stream) {
   TSPH_REQUIRE(sim && prm, TSPH_ERR_ARG, "bad arguments");

@@ -137,8 +137,27 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     validate_case(cfg)
     check_supported(cfg, rhs=True, shifting=True)
     se = int(cfg.shift_every or 0)
-    if se and cfg.adaptive_dt:
-        raise NotImplementedError("Lind shifting runs with the fixed step here (adaptive_dt off)")
+    te = int(cfg.tvc_every or 0)
+    if (se or te) and cfg.adaptive_dt:
+        raise NotImplementedError("the position corrections run with the fixed step here (adaptive_dt off)")
+    tvc_kw = None
+    if te:
+        name = cfg.tvc_limiter
+        if name is None or str(name).lower() in ("none", "no_limiter", "nolimiter"):
+            slope = 0.0
+        elif str(name).lower() in ("truncated_linear", "truncatedlinear"):
+            slope = float(cfg.tvc_limiter_slope)
+            if not math.isfinite(slope) or slope <= 0.0:
+                raise ValueError("TruncatedLinear slope must be finite and positive")
+        else:
+            raise ValueError(f"unknown tvc_limiter {name!r}")
+        if cfg.tvc_scope == "bulk_g" and (not math.isfinite(cfg.tvc_bulk_g) or cfg.tvc_bulk_g < 0.0):
+            raise ValueError("scope_tol must be finite and non-negative")
+        tvc_kw = dict(alpha=0.05 if cfg.tvc_alpha is None else float(cfg.tvc_alpha), limiter_slope=slope,
+                      max_shift=0.0 if cfg.tvc_max_shift is None else float(cfg.tvc_max_shift) * cfg.h_factor * cfg.dx,
+                      wall_volume=cfg.dx ** 2, scope=cfg.tvc_scope, bulk_c=float(cfg.tvc_bulk_c),
+                      bulk_g=float(cfg.tvc_bulk_g))
+    tvc_peak = tvc_clip_peak = 0.0
     if precision is None:
         precision = "fp64" if torch.get_default_dtype() == torch.float64 else "fp32"
     dev = torch.device(device) if device is not None else torch.device("cuda")
@@ -177,17 +196,23 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
             if final or step > max_steps:
                 break
             nxt = min((step // ow + 1) * ow, max_steps + 1)
-            if se:
-                # stop after each step s > 0 with s % shift_every == 0 (not the last):
-                # the shift moves the fluid before step s + 1 (dam_break.py:424-433)
-                s = max(step, 1)
-                s = ((s + se - 1) // se) * se
-                if s < cfg.steps:
-                    nxt = min(nxt, s + 1)
+            for every in (se, te):
+                if every:
+                    # stop after each step s > 0 with s % every == 0 (not the last): the
+                    # corrections move the fluid before step s + 1 (dam_break.py:424-460)
+                    s = ((max(step, 1) + every - 1) // every) * every
+                    if s < cfg.steps:
+                        nxt = min(nxt, s + 1)
             sim.step(nxt - step, meter_every=1)
-            if se and (nxt - 1) > 0 and (nxt - 1) % se == 0 and (nxt - 1) < cfg.steps:
-                sim.shift(cfg.dt, v_ref=cfg.shift_v_ref, v_floor=0.05 * cfg.sound_speed,
-                          strength=cfg.shift_strength, limit=cfg.shift_limit)
+            last = nxt - 1
+            if 0 < last < cfg.steps:
+                if se and last % se == 0:
+                    sim.shift(cfg.dt, v_ref=cfg.shift_v_ref, v_floor=0.05 * cfg.sound_speed,
+                              strength=cfg.shift_strength, limit=cfg.shift_limit)
+                if te and last % te == 0:
+                    info = sim.tvc(**tvc_kw)
+                    tvc_peak = max(tvc_peak, info["max_shift"])
+                    tvc_clip_peak = max(tvc_clip_peak, info["clipped"])
             if adaptive and sim.stats()["finished"]:
                 step = sim.stats()["steps"] - 1
             else:
@@ -238,4 +263,5 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
         wall_impact_time=wall_impact_time, initial_mass=initial_mass,
         final_mass=float(mass.sum()), final_positions=final_pos, steps_taken=steps_taken,
         simulated_time=float(time[steps_taken]), dt_min=dt_min, dt_max=dt_max,
-        max_acoustic_courant=courant, snapshots=snapshots)
+        max_acoustic_courant=courant, snapshots=snapshots, max_tvc_shift=float(tvc_peak),
+        max_tvc_clipped=float(tvc_clip_peak))

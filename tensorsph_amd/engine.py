@@ -246,6 +246,19 @@ def _shift(self, dt, v_ref=None, v_floor=0.0, strength=1.0, limit=0.25, free_sur
     return {"c_reference": info[0], "max_shift": info[1], "clipped": info[2], "v_ref": info[3]}
 
 
+def _tvc(self, alpha=0.05, limiter_slope=0.0, max_shift=0.0, wall_volume=0.0, scope="all",
+         bulk_c=0.95, bulk_g=0.02, percentile=95.0) -> dict:
+    """tsph_sim_tvc: one transport-velocity correction of the fluid after the
+    last step (scope "all", "bulk" or "bulk_g")."""
+    sc = {"bulk": 1, "bulk_g": 2}.get(scope, 0)
+    p = _lib.TvcParams(float(alpha), float(limiter_slope), float(max_shift), float(wall_volume), sc,
+                       float(bulk_c), float(bulk_g), float(percentile))
+    info = (C.c_double * 3)()
+    _lib.check(_lib.lib().tsph_sim_tvc(self.handle, C.byref(p), info, _lib.stream_handle(self.device)))
+    return {"max_shift": info[0], "clipped": info[1], "crossings": int(info[2])}
+
+
+WcsphSimulation.tvc = _tvc
 WcsphSimulation.shift = _shift
 WcsphSimulation.gauges = _gauges
 WcsphSimulation.set_timing = _set_timing

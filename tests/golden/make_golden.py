@@ -26,6 +26,8 @@ Fixtures
   kleefsman_gauges.npz  the Kleefsman gauge and tap operators on a cloud, and a
                      short run_case series; kleefsman_checkpoint_v3.pt a
                      checkpoint written by save_checkpoint
+  tvc.npz            transport_velocity_correction on a block over a wall and
+                     dam_break.run_case with tvc_every (four variants)
   lind_shift.npz     particle_shifting.shift_positions on a seeded column and a
                      dam_break.run_case with shift_every
   rhs2d_opts.npz     droplet_impact.rhs with its opt-in branches: wrap_width,
@@ -615,6 +617,54 @@ def lind_shift(reference: str = "/root/reference") -> None:
     print("shifting fixture written to", HERE)
 
 
+def tvc(reference: str = "/root/reference") -> None:
+    """transport_velocity_correction (transport_velocity_correction.py:285-513)
+    on a seeded fluid block with a wall layer, and dam_break.run_case with
+    tvc_every (dam_break.py:434-460) in three variants."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import dam_break as db
+    from tensorsph.neighbor_search import cell_linked_pairs
+    from tensorsph.transport_velocity_correction import (transport_velocity_correction,
+                                                         truncated_linear)
+    torch.set_default_dtype(torch.float64)
+    out = {}
+    g = torch.Generator().manual_seed(41)
+    dx = 0.02
+    h = 1.3 * dx
+    xs = (torch.arange(16) + 0.5) * dx
+    ys = (torch.arange(10) + 0.5) * dx
+    fluid = torch.stack(torch.meshgrid(xs, ys, indexing="ij"), -1).reshape(-1, 2)
+    fluid = fluid + 0.1 * dx * torch.randn(fluid.shape, generator=g)
+    wx = (torch.arange(-3, 19) + 0.5) * dx
+    wall = torch.stack(torch.meshgrid(wx, torch.tensor([-0.5 * dx, -1.5 * dx]), indexing="ij"), -1).reshape(-1, 2)
+    allp = torch.cat((fluid, wall))
+    vol = torch.cat((dx * dx * (1.0 + 0.05 * torch.rand(fluid.shape[0], generator=g)),
+                     torch.full((wall.shape[0],), dx * dx)))
+    pairs = cell_linked_pairs(allp, 2.0 * h)
+    for tag, kw in (("plain", {}), ("lim", dict(alpha=0.1, limiter=truncated_linear(100.0), max_shift=0.002)),
+                    ("scope", dict(alpha=0.05, scope_tol=0.5))):
+        r = transport_velocity_correction(allp, vol, h, pairs=pairs, wall_count=wall.shape[0], **kw)
+        out[f"{tag}_g"] = r.gradient_integral.numpy()
+        out[f"{tag}_shift"] = r.shift.numpy()
+    out.update(pos=allp.numpy(), vol=vol.numpy(), nf=np.array(fluid.shape[0]), h=np.array(h),
+               pairs=pairs.numpy())
+    base = dict(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4, pool_width=0.5,
+                sound_speed=20.0, dt=3.0e-4, viscosity=1e-3, steps=200, output_every=20, device="cpu")
+    for tag, kw in (("run_all", dict(tvc_every=5)),
+                    ("run_lim", dict(tvc_every=4, tvc_limiter="truncated_linear", tvc_max_shift=0.01,
+                                     tvc_alpha=0.1)),
+                    ("run_bulkg", dict(tvc_every=5, tvc_scope="bulk_g", tvc_bulk_g=0.05)),
+                    ("run_bulk", dict(tvc_every=5, tvc_scope="bulk", shift_every=10))):
+        res = db.run_case(db.DamBreakConfig(**base, **kw))
+        out[f"{tag}_front"] = np.array(res.front)
+        out[f"{tag}_ke"] = np.array(res.kinetic_energy)
+        out[f"{tag}_final"] = res.final_positions.numpy()
+        out[f"{tag}_tvc"] = np.array([res.max_tvc_shift, res.max_tvc_clipped, res.escaped, res.rebuilds])
+    np.savez_compressed(HERE / "tvc.npz", **out)
+    print("tvc fixture written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
@@ -627,6 +677,8 @@ if __name__ == "__main__":
         kleefsman_gauges(*args[1:])
     elif args and args[0] == "--only-shift":
         lind_shift(*args[1:])
+    elif args and args[0] == "--only-tvc":
+        tvc(*args[1:])
     else:
         main(*args)
         shear(*args)
@@ -634,3 +686,4 @@ if __name__ == "__main__":
         rhs_options(*args)
         kleefsman_gauges(*args)
         lind_shift(*args)
+        tvc(*args)

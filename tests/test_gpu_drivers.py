@@ -363,3 +363,29 @@ def test_run_case_lind_shifting_matches_reference_run(golden, precision):
     assert np.abs(n(res.final_positions) - g["run_final"]).max() < (1e-10 if precision == "fp64" else 1e-6)
     if precision == "fp64":
         assert res.rebuilds == int(g["run_rebuilds"])
+
+
+@pytest.mark.parametrize("tag,kw", [
+    ("run_all", dict(tvc_every=5)),
+    ("run_lim", dict(tvc_every=4, tvc_limiter="truncated_linear", tvc_max_shift=0.01, tvc_alpha=0.1)),
+    ("run_bulkg", dict(tvc_every=5, tvc_scope="bulk_g", tvc_bulk_g=0.05)),
+    ("run_bulk", dict(tvc_every=5, tvc_scope="bulk", shift_every=10)),
+])
+def test_run_case_tvc_matches_reference_run(golden, tag, kw):
+    """dam_break.run_case with the transport-velocity correction
+    (dam_break.py:434-460), alone and with Lind shifting, against the
+    reference's own runs in FP64 (front and energies to 1e-9, final positions
+    to 1e-10, peak shift and capped fraction, escapes, list builds)."""
+    from tensorsph_amd.dam_break import DamBreakConfig, run_case
+    g = golden("tvc")
+    cfg = DamBreakConfig(dx=0.02, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.5, sound_speed=20.0, dt=3.0e-4, viscosity=1e-3, steps=200,
+                         output_every=20, **kw)
+    res = run_case(cfg, precision="fp64")
+    assert np.allclose(res.front, g[f"{tag}_front"], rtol=1e-9, atol=0)
+    assert np.allclose(res.kinetic_energy, g[f"{tag}_ke"], rtol=1e-8, atol=1e-8 * g[f"{tag}_ke"].max())
+    assert np.abs(n(res.final_positions) - g[f"{tag}_final"]).max() < 1e-10
+    peak, clipped, escaped, rebuilds = g[f"{tag}_tvc"]
+    assert res.max_tvc_shift == pytest.approx(peak, rel=1e-8)
+    assert res.max_tvc_clipped == pytest.approx(clipped, abs=1e-12)
+    assert res.escaped == int(escaped) and res.rebuilds == int(rebuilds)

@@ -784,3 +784,35 @@ def test_lind_shift_closed_forms():
     assert np.sqrt(((out - pos) ** 2).sum(1)).max() <= 0.25 * h * (1 + 1e-12)
     out0, *_ = orc.lind_shift(pos, vol, h, dt=1e-2, v_ref=5.0, strength=0.0)
     assert np.array_equal(out0, pos)
+
+
+# ----------------------------------------------------------------- transport-velocity correction
+
+@pytest.mark.parametrize("tag,kw", [("plain", {}), ("lim", dict(alpha=0.1, slope=100.0, max_shift=0.002)),
+                                    ("scope", dict(alpha=0.05, scope_tol=0.5))])
+def test_tvc_matches_reference(golden, tag, kw):
+    g = golden("tvc")
+    gi, sh = orc.tvc_shift(g["pos"], g["vol"], float(g["h"]), int(g["nf"]), g["pairs"], **kw)
+    assert scaled_err(gi, g[f"{tag}_g"]) < 1e-14
+    assert np.abs(sh - g[f"{tag}_shift"]).max() < 1e-17
+
+
+def test_tvc_closed_forms():
+    """On a perfect lattice g vanishes more than 2h from its edges, and a
+    wall row counts as a fluid row of the same volume (both factors are 2)."""
+    dx = 0.02
+    h = 1.3 * dx
+    ax = (np.arange(20) + 0.5) * dx
+    pos = np.stack(np.meshgrid(ax, ax, indexing="ij"), -1).reshape(-1, 2)
+    vol = np.full(len(pos), dx * dx)
+    pairs = orc.cell_linked_pairs(pos, 2.0 * h)
+    g, _ = orc.tvc_shift(pos, vol, h, len(pos), pairs)
+    interior = np.all((pos > 2.1 * h) & (pos < 20 * dx - 2.1 * h), axis=1)
+    assert np.abs(g[interior]).max() < 1e-12 * np.abs(g).max()
+    bottom = pos[:, 1] < dx
+    order = np.concatenate([np.nonzero(~bottom)[0], np.nonzero(bottom)[0]])
+    p2 = pos[order]
+    pairs2 = orc.cell_linked_pairs(p2, 2.0 * h)
+    nf = int((~bottom).sum())
+    g2, _ = orc.tvc_shift(p2, vol[order], h, nf, pairs2)
+    assert np.abs(g2 - g[order][:nf]).max() < 1e-12 * np.abs(g).max()
