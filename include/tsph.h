@@ -409,8 +409,9 @@ int tsph_sim_destroy(tsph_sim* sim);
  * every shift_every steps (dam_break.py:424-433): concentration C from the
  * last evaluation's volumes m / rho over the fluid pairs within 2h, shift
  * -strength h v_ref dt grad C, tangential where C < free_surface C_ref,
- * capped at limit h.  C_ref is c_reference when > 0, else the percentile of
- * C (linear interpolation).  v_ref <= 0 takes max(largest speed at the start
+ * capped at limit h; dt = 0 takes the last step's size (adaptive runs).
+ * C_ref is c_reference when > 0, else the percentile of C (linear
+ * interpolation).  v_ref <= 0 takes max(largest speed at the start
  * of the last step, v_floor).  info (may be NULL): C_ref, largest shift,
  * fraction of rows capped, v_ref.  Synchronises. */
 typedef struct tsph_shift_params {

@@ -1078,6 +1078,7 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
   const double spd = ord_dec_host(rec->maxes[MM_PEAK_SPEED]);
   sim->peak_rho = rho_max > sim->peak_rho ? rho_max : sim->peak_rho;
   sim->last_speed = spd;
+  sim->last_step_dt = rec->step_dt;
   sim->peak_speed = spd > sim->peak_speed ? spd : sim->peak_speed;
   if (load_rows(sim)) {
     sim->load_rows.push_back((double)k);
@@ -1104,6 +1105,7 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
   const int64_t start = sim->steps_done;
   const int64_t end = start + nsteps;
   auto rebuild = [&]() -> int {
+    sim->rebuild_forced = false;       // a counted build re-anchors every row
     if constexpr (D == 2) {
       if (sim->tiles) return sim_rebuild_tiles<D, S>(sim, stream);
     }
@@ -1544,6 +1546,7 @@ int rebuild_keep_volumes(tsph_sim* sim, hipStream_t stream, bool* rebuilt) {
     if (!done) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
     sim->rebuild_pending = false;
     sim->rebuilds--;
+    sim->rebuild_forced = true;
     *rebuilt = true;
     k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->shift_tmp.as<S>(),
                                                             sim->vol.as<S>());
@@ -1592,7 +1595,8 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   double v_ref = prm->v_ref;
   if (!(v_ref > 0.0)) v_ref = sim->last_speed > prm->v_floor ? sim->last_speed : prm->v_floor;
   ShiftArgs sa;
-  sa.amp = prm->strength * sim->prm.op.h * v_ref * prm->dt;
+  const double dt = prm->dt > 0.0 ? prm->dt : sim->last_step_dt;
+  sa.amp = prm->strength * sim->prm.op.h * v_ref * dt;
   sa.cap = prm->limit * sim->prm.op.h;
   sa.fs_c = c_ref > 0.0 ? prm->free_surface * c_ref : 0.0;
   TSPH_CHECK_HIP(hipMemsetAsync(st, 0, 3 * sizeof(unsigned long long), stream));
@@ -1606,7 +1610,7 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   const double disp = n > 0 ? ord_dec_host(h[2]) : 0.0;
   // the next step re-bins when the corrected positions are more than half the
   // skin from the anchors, the pair cache's own test
-  sim->rebuild_pending = rebuilt || disp > 0.5 * sim->prm.skin;
+  sim->rebuild_pending = sim->rebuild_forced || disp > 0.5 * sim->prm.skin;
   if (info) {
     info[0] = c_ref;
     info[1] = n > 0 ? ord_dec_host(h[1]) : 0.0;
@@ -1738,7 +1742,7 @@ int sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* info, hipStream_t
   unsigned long long h[5];
   TSPH_CHECK_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-  sim->rebuild_pending = rebuilt || (n > 0 && ord_dec_host(h[2]) > 0.5 * sim->prm.skin);
+  sim->rebuild_pending = sim->rebuild_forced || (n > 0 && ord_dec_host(h[2]) > 0.5 * sim->prm.skin);
   sim->escaped_extra += (int64_t)h[4];
   if (info) {
     info[0] = n > 0 ? ord_dec_host(h[1]) : 0.0;
@@ -1767,8 +1771,8 @@ extern "C" int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, doubl
   TSPH_REQUIRE(sim && prm, TSPH_ERR_ARG, "bad arguments");
   TSPH_REQUIRE(sim->dim == 2, TSPH_ERR_ARG, "particle shifting runs on the 2-D stepper");
   TSPH_REQUIRE(sim->steps_done > 0, TSPH_ERR_ARG, "shifting needs the volumes of an evaluated step");
-  TSPH_REQUIRE(prm->dt > 0.0 && prm->limit >= 0.0 && prm->strength >= 0.0, TSPH_ERR_ARG,
-               "dt must be positive, strength and limit non-negative");
+  TSPH_REQUIRE(prm->dt >= 0.0 && prm->limit >= 0.0 && prm->strength >= 0.0, TSPH_ERR_ARG,
+               "dt, strength and limit must be non-negative");
   TSPH_REQUIRE(sim->n < (1ll << 31) - 1, TSPH_ERR_ARG, "too many rows for the concentration sort");
   if (sim->prec == TSPH_F64) return sim_shift<2, double>(sim, prm, info, stream);
   return sim_shift<2, float>(sim, prm, info, stream);

@@ -69,6 +69,8 @@ struct tsph_sim {
   tsph::DevBuf closure;             // free-surface pressure closure: two pressure and two flag buffers
   tsph::DevBuf shift_buf, shift_tmp;  // particle shifting: C, sorted C, shift vectors, counters; sort scratch
   double last_speed = 0.0;          // largest speed at the start of the last step run
+  double last_step_dt = 0.0;        // the size of the last step run
+  bool rebuild_forced = false;      // a correction re-binned uncounted: the next step re-bins
   int64_t escaped_extra = 0;        // boundary crossings of the position corrections
   tsph::DevBuf flags, block_sums, sums_out, rec_dev;
   tsph::DevBuf old2new, new2old, fscan, cells_new;

@@ -138,8 +138,6 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
     check_supported(cfg, rhs=True, shifting=True)
     se = int(cfg.shift_every or 0)
     te = int(cfg.tvc_every or 0)
-    if (se or te) and cfg.adaptive_dt:
-        raise NotImplementedError("the position corrections run with the fixed step here (adaptive_dt off)")
     tvc_kw = None
     if te:
         name = cfg.tvc_limiter
@@ -201,13 +199,16 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
                     # stop after each step s > 0 with s % every == 0 (not the last): the
                     # corrections move the fluid before step s + 1 (dam_break.py:424-460)
                     s = ((max(step, 1) + every - 1) // every) * every
-                    if s < cfg.steps:
+                    if s < (max_steps if adaptive else cfg.steps):
                         nxt = min(nxt, s + 1)
             sim.step(nxt - step, meter_every=1)
             last = nxt - 1
-            if 0 < last < cfg.steps:
+            moved = (not sim.stats()["finished"]) if adaptive else last < cfg.steps
+            if (se or te) and last > 0 and moved:
+                # the step's own dt: 0 asks the stepper for its last step's size
+                step_dt = 0.0 if adaptive else cfg.dt
                 if se and last % se == 0:
-                    sim.shift(cfg.dt, v_ref=cfg.shift_v_ref, v_floor=0.05 * cfg.sound_speed,
+                    sim.shift(step_dt, v_ref=cfg.shift_v_ref, v_floor=0.05 * cfg.sound_speed,
                               strength=cfg.shift_strength, limit=cfg.shift_limit)
                 if te and last % te == 0:
                     info = sim.tvc(**tvc_kw)

@@ -236,7 +236,8 @@ def _gauges(self, spec) -> np.ndarray:
 def _shift(self, dt, v_ref=None, v_floor=0.0, strength=1.0, limit=0.25, free_surface=0.95,
            percentile=95.0, c_reference=None) -> dict:
     """tsph_sim_shift: one Lind shifting pass on the fluid after the last
-    step (v_ref None: max(speed at the start of that step, v_floor))."""
+    step (v_ref None: max(speed at the start of that step, v_floor); dt 0:
+    that step's size)."""
     p = _lib.ShiftParams(float(dt), -1.0 if v_ref is None else float(v_ref), float(v_floor),
                          float(strength), float(limit), float(free_surface), float(percentile),
                          0.0 if c_reference is None else float(c_reference))

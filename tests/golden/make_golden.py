@@ -27,7 +27,8 @@ Fixtures
                      short run_case series; kleefsman_checkpoint_v3.pt a
                      checkpoint written by save_checkpoint
   tvc.npz            transport_velocity_correction on a block over a wall and
-                     dam_break.run_case with tvc_every (four variants)
+                     dam_break.run_case with tvc_every (four variants, and one
+                     with shifting and the adaptive step)
   lind_shift.npz     particle_shifting.shift_positions on a seeded column and a
                      dam_break.run_case with shift_every
   rhs2d_opts.npz     droplet_impact.rhs with its opt-in branches: wrap_width,
@@ -655,12 +656,15 @@ def tvc(reference: str = "/root/reference") -> None:
                     ("run_lim", dict(tvc_every=4, tvc_limiter="truncated_linear", tvc_max_shift=0.01,
                                      tvc_alpha=0.1)),
                     ("run_bulkg", dict(tvc_every=5, tvc_scope="bulk_g", tvc_bulk_g=0.05)),
-                    ("run_bulk", dict(tvc_every=5, tvc_scope="bulk", shift_every=10))):
+                    ("run_bulk", dict(tvc_every=5, tvc_scope="bulk", shift_every=10)),
+                    ("run_adapt", dict(tvc_every=5, shift_every=7, adaptive_dt=True, max_courant=0.3))):
         res = db.run_case(db.DamBreakConfig(**base, **kw))
         out[f"{tag}_front"] = np.array(res.front)
         out[f"{tag}_ke"] = np.array(res.kinetic_energy)
         out[f"{tag}_final"] = res.final_positions.numpy()
         out[f"{tag}_tvc"] = np.array([res.max_tvc_shift, res.max_tvc_clipped, res.escaped, res.rebuilds])
+        out[f"{tag}_times"] = np.array(res.times)
+        out[f"{tag}_steps"] = np.array(res.steps_taken)
     np.savez_compressed(HERE / "tvc.npz", **out)
     print("tvc fixture written to", HERE)
 

@@ -370,6 +370,7 @@ def test_run_case_lind_shifting_matches_reference_run(golden, precision):
     ("run_lim", dict(tvc_every=4, tvc_limiter="truncated_linear", tvc_max_shift=0.01, tvc_alpha=0.1)),
     ("run_bulkg", dict(tvc_every=5, tvc_scope="bulk_g", tvc_bulk_g=0.05)),
     ("run_bulk", dict(tvc_every=5, tvc_scope="bulk", shift_every=10)),
+    ("run_adapt", dict(tvc_every=5, shift_every=7, adaptive_dt=True, max_courant=0.3)),
 ])
 def test_run_case_tvc_matches_reference_run(golden, tag, kw):
     """dam_break.run_case with the transport-velocity correction
@@ -389,3 +390,6 @@ def test_run_case_tvc_matches_reference_run(golden, tag, kw):
     assert res.max_tvc_shift == pytest.approx(peak, rel=1e-8)
     assert res.max_tvc_clipped == pytest.approx(clipped, abs=1e-12)
     assert res.escaped == int(escaped) and res.rebuilds == int(rebuilds)
+    if f"{tag}_times" in g:
+        assert res.steps_taken == int(g[f"{tag}_steps"])
+        assert res.times == pytest.approx(list(g[f"{tag}_times"]), rel=1e-14)
