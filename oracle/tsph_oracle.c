@@ -358,19 +358,6 @@ static void sym_eig(int n, const double* a_in, double* w, double* v) {
             }
 }
 
-/* f(M) g with f(lambda) = 1 / max(lambda, floor): the regularised inverse. */
-static void reg_solve(int n, const double* m, const double* g, double floor_v, double* out) {
-    double w[3], v[9];
-    sym_eig(n, m, w, v);
-    double proj[3];
-    for (int k = 0; k < n; ++k) {
-        double s = 0; for (int r = 0; r < n; ++r) s += v[r * n + k] * g[r];
-        double lam = w[k] > floor_v ? w[k] : floor_v;
-        proj[k] = s / lam;
-    }
-    for (int r = 0; r < n; ++r) { double s = 0; for (int k = 0; k < n; ++k) s += v[r * n + k] * proj[k]; out[r] = s; }
-}
-
 ORC_API void orc_sym_eig(int n, const double* a, double* w, double* v) { sym_eig(n, a, w, v); }
 
 /* ------------------------------------------------------------------------ */

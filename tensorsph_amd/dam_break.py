@@ -4,10 +4,10 @@
 pair cache, rhs, symplectic Euler, apply_boundary: the loop of
 dam_break.py:374-465) and returns the same ``DamBreakRun`` record: the
 sampled front position and energies, every-step extrema, the wall-impact
-time and the compression before it.  Lind shifting (``shift_every``) runs on
-the device after the steps it follows (tsph_sim_shift); the
-transport-velocity correction (``tvc_every``) is not built and raises
-NotImplementedError; the adaptive step runs on the device.
+time and the compression before it.  Lind shifting (``shift_every``) and the
+transport-velocity correction (``tvc_every``) run on the device after the
+steps they follow (tsph_sim_shift, tsph_sim_tvc); the adaptive step runs on
+the device.
 """
 from __future__ import annotations
 
