@@ -357,6 +357,12 @@ typedef struct tsph_sim_params {
    * adaptive = 0: every step integrates with op.dt. */
   int32_t adaptive;
   double target_time, courant_num, dt_limit, dt_safety, dt_growth;
+  /* adaptive_free_last = 1: steps are never shortened onto target_time and
+   * the final sample is the first step starting at elapsed >= target_time -
+   * tol (kleefsman_obstacle_3d.py:644-713); adaptive_tol > 0 sets tol, 0
+   * keeps 1e-12 max(1, target_time). */
+  int32_t adaptive_free_last;
+  double adaptive_tol;
 } tsph_sim_params;
 
 /* pos: (n_fluid + n_wall, dim) FP64 device, fluid then wall; vel: (n_fluid,
@@ -383,10 +389,10 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
  * previous step's size, this step's size, the step-size variable]: 24
  * doubles per row.  The meters are of the state at the step's start. */
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
-/* Obstacle loads of every completed step (host), 11 doubles per row: step,
+/* Obstacle loads of every completed step (host), 12 doubles per row: step,
  * F x/y/z, M x/y/z (about obstacle_centre), cumulative clamp impulse x/y/z
- * (sum of m (v_before - v_after) of clamped particles) and the cumulative
- * number of clamped particles. */
+ * (sum of m (v_before - v_after) of clamped particles), the cumulative
+ * number of clamped particles and the step's size. */
 int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
 /* Sweep layout.  tiles = 1 (the 2-D default) runs the LDS cell-tile sweeps
  * with tile_x x tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108,

@@ -112,7 +112,8 @@ class SimParams(C.Structure):
                 ("obstacle_centre", _dbl * 3), ("obstacle_clamp", C.c_int32),
                 ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3), ("obstacle_eps", _dbl),
                 ("adaptive", C.c_int32), ("target_time", _dbl), ("courant_num", _dbl),
-                ("dt_limit", _dbl), ("dt_safety", _dbl), ("dt_growth", _dbl)]
+                ("dt_limit", _dbl), ("dt_safety", _dbl), ("dt_growth", _dbl),
+                ("adaptive_free_last", C.c_int32), ("adaptive_tol", _dbl)]
 
 
 _dp = C.POINTER(_dbl)

@@ -79,7 +79,7 @@ __global__ void k_pack_record(SimFlags* fl, const double* __restrict__ sums, int
         fl->dt = dt;
         const double left = ctl.target_time - fl->elapsed;
         fl->final = fl->elapsed >= ctl.target_time - ctl.tol ? 1ull : 0ull;
-        fl->step_dt = fl->final ? 0.0 : (dt < left ? dt : left);
+        fl->step_dt = fl->final ? 0.0 : (ctl.free_last || dt < left ? dt : left);
       }
     }
   }
@@ -1001,7 +1001,8 @@ static DtCtl sim_dtctl(const tsph_sim* sim) {
   c.adaptive = p.adaptive;
   c.dt_cfg = p.op.dt;
   c.target_time = p.target_time;
-  c.tol = 1.0e-12 * (p.target_time > 1.0 ? p.target_time : 1.0);
+  c.tol = p.adaptive_tol > 0.0 ? p.adaptive_tol : 1.0e-12 * (p.target_time > 1.0 ? p.target_time : 1.0);
+  c.free_last = p.adaptive_free_last;
   c.courant_num = p.courant_num;
   c.c0 = p.op.sound_speed;
   c.dt_limit = p.dt_limit;
@@ -1016,7 +1017,7 @@ __global__ void k_init_time(SimFlags* fl, DtCtl c) {
   fl->last_dt = c.dt_cfg;
   fl->elapsed = 0.0;
   fl->final = c.adaptive && 0.0 >= c.target_time - c.tol ? 1ull : 0ull;
-  fl->step_dt = fl->final ? 0.0 : (c.adaptive && c.target_time < c.dt_cfg ? c.target_time : c.dt_cfg);
+  fl->step_dt = fl->final ? 0.0 : (c.adaptive && !c.free_last && c.target_time < c.dt_cfg ? c.target_time : c.dt_cfg);
 }
 
 // Queue one step (density [+diffusion], wall, force+integrate) reading the
@@ -1085,6 +1086,7 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
     for (int i = 0; i < 6; ++i) sim->load_rows.push_back(rec->loads[i]);
     for (int i = 0; i < 3; ++i) sim->load_rows.push_back(rec->clamp_imp[i]);
     sim->load_rows.push_back((double)rec->clamp_count);
+    sim->load_rows.push_back(rec->step_dt);
   }
   if (meters) {
     sim->rows.push_back((double)k);
@@ -1315,7 +1317,7 @@ int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrow
 
 int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows) {
   TSPH_REQUIRE(sim && nrows, TSPH_ERR_ARG, "bad arguments");
-  const int64_t width = 11;
+  const int64_t width = 12;
   const int64_t have = (int64_t)sim->load_rows.size() / width;
   *nrows = have;
   if (rows) {

@@ -27,6 +27,7 @@ struct SimFlags {
 // adaptive step control (droplet_impact.py:495-552, 1674-1763)
 struct DtCtl {
   int adaptive;
+  int free_last;                    // 1: the last step is not shortened onto target_time
   double dt_cfg, target_time, tol;
   double courant_num;               // max_courant * h_factor * dx, as the reference forms it
   double c0, dt_limit, dt_safety, dt_growth;

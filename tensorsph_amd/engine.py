@@ -129,6 +129,8 @@ class WcsphSimulation:
             p.dt_limit = float(adaptive["dt_limit"])
             p.dt_safety = float(adaptive["dt_safety"])
             p.dt_growth = float(adaptive["dt_growth"])
+            p.adaptive_free_last = 1 if adaptive.get("free_last") else 0
+            p.adaptive_tol = float(adaptive.get("tol", 0.0))
         self.params = p
         handle = _vp()
         stream = _lib.stream_handle(dev)
@@ -176,11 +178,11 @@ class WcsphSimulation:
         return rows
 
     def loads(self) -> np.ndarray:
-        """Obstacle loads per completed step: (rows, 11) = step, F (3), M (3),
-        cumulative clamp impulse (3), cumulative clamped particles."""
+        """Obstacle loads per completed step: (rows, 12) = step, F (3), M (3),
+        cumulative clamp impulse (3), cumulative clamped particles, step size."""
         nrows = _i64(0)
         _lib.check(_lib.lib().tsph_sim_loads(self.handle, None, 0, C.byref(nrows)))
-        rows = np.zeros((nrows.value, 11), np.float64)
+        rows = np.zeros((nrows.value, 12), np.float64)
         if nrows.value:
             _lib.check(_lib.lib().tsph_sim_loads(self.handle, rows.ctypes.data_as(C.POINTER(C.c_double)),
                                                  nrows.value, C.byref(nrows)))

@@ -27,7 +27,7 @@ import torch
 from . import _lib
 from .cases import make_column_3d, mass_3d, obstacle_lattice_3d, wall_lattice_3d
 from .config import Kleefsman3DConfig
-from .engine import WcsphSimulation
+from .engine import METER_KEYS, WcsphSimulation
 
 # SPHERIC Test 2 workbook, H2 (marin_probes.py:7-12)
 H2_PEAK_M = .3408
@@ -58,7 +58,7 @@ def kleefsman_state(cfg: Kleefsman3DConfig):
 
 
 def make_simulation(cfg: Kleefsman3DConfig, precision: str = "fp32", device="cuda", pos=None,
-                    vel=None):
+                    vel=None, adaptive=None):
     """The resident stepper of the case; ``pos`` / ``vel`` (fluid, caller
     order) replace the initial column, as a resumed run needs."""
     pos0, mass, wall, rows, centre, box = kleefsman_state(cfg)
@@ -74,7 +74,8 @@ def make_simulation(cfg: Kleefsman3DConfig, precision: str = "fp32", device="cud
                           skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0, 0.0],
                           hi=[cfg.pool_width, cfg.tank_top(), cfg.tank_depth], tol=0.25 * cfg.dx,
                           wall_friction=cfg.wall_friction, precision=precision, device=device,
-                          obstacle={"rows": rows, "centre": centre, "box": box, "eps": 0.01 * cfg.dx})
+                          obstacle={"rows": rows, "centre": centre, "box": box, "eps": 0.01 * cfg.dx},
+                          adaptive=adaptive)
     return sim, pos.shape[0], wall.shape[0]
 
 
@@ -395,11 +396,12 @@ def run_case(cfg: Kleefsman3DConfig, device="cuda", progress=None, on_sample=Non
     the loads, clamps and escapes are those of the steps the reference
     integrates.  ``on_sample(i, sample)`` gets the reference's sample dict.
     ``precision``: "fp64" or "fp32" (FP32-compensated sweeps)."""
-    if getattr(cfg, "adaptive_dt", False):
-        raise NotImplementedError("the Kleefsman driver runs the fixed step of the reference's "
-                                  "default configuration; adaptive_dt is not built here")
     if str(device) == "cpu":
         device = "cuda"
+    if getattr(cfg, "adaptive_dt", False):
+        if checkpoint_path is not None:
+            raise NotImplementedError("checkpoints of adaptive-step runs are not built")
+        return _run_case_adaptive(cfg, device, progress, on_sample, precision)
     t_start = time.time()
     pos0, mass, wall, rows, centre, box = kleefsman_state(cfg)
     nf, nw = pos0.shape[0], wall.shape[0]
@@ -537,4 +539,93 @@ def run_case(cfg: Kleefsman3DConfig, device="cuda", progress=None, on_sample=Non
             os.unlink(checkpoint_path)
         except FileNotFoundError:
             pass
+    return run
+
+
+def _run_case_adaptive(cfg: Kleefsman3DConfig, device, progress, on_sample, precision):
+    """run_case with adaptive_dt (kleefsman_obstacle_3d.py:644-713): after each
+    step dt = next_dt(dt, largest speed at its start), the step is never
+    shortened onto t_end, and the first step starting at elapsed >= t_end -
+    1e-12 is the final sample; the loop also ends after max_steps."""
+    from .config import adaptive_controls
+    t_start = time.time()
+    pos0, mass, wall, rows_, centre, box = kleefsman_state(cfg)
+    run = Kleefsman3DRun(cfg=cfg, fluid_particles=pos0.shape[0], wall_particles=wall.shape[0])
+    max_steps = cfg.max_steps or cfg.steps_for_t_end()
+    ctl = dict(adaptive_controls(cfg), target_time=float(cfg.t_end), free_last=True, tol=1.0e-12)
+    nodes = tap_disk_nodes(cfg, sensor_positions(cfg), cfg.pressure_disk_diameter)
+    spec = gauge_spec(cfg, nodes)
+    kn = nodes.shape[1]
+    ow = cfg.output_every
+    sim, _, _ = make_simulation(cfg, precision, device, adaptive=ctl)
+    done = 0
+    tcol = METER_KEYS.index("time")
+
+    def time_at(step, m, lr):
+        """Simulated time at the start of ``step``: the last metered step's
+        time plus the sizes of the steps after it, summed in step order as
+        the stepper accumulates them."""
+        r = m[m[:, 0] <= step]
+        k0 = int(r[-1, 0])
+        t = float(r[-1, tcol])
+        for k in range(k0, step):
+            t += float(lr[k, 11])
+        return t
+
+    try:
+        while True:
+            st = sim.stats()
+            if st["finished"] or done > max_steps:
+                break
+            target = min((done // ow + (0 if done % ow == 0 else 1)) * ow, max_steps)
+            sim.step(target + 1 - done, meter_every=ow)
+            st = sim.stats()
+            done = st["steps"]
+            step = done - 1                     # the step whose start state is sampled
+            if not (st["finished"] or step % ow == 0):
+                continue
+            g = sim.gauges(spec)
+            lr = sim.loads()
+            t_s = time_at(step, sim.meters(), lr)
+            run.times.append(t_s)
+            run.slab.append(float(g[0]))
+            run.height.append(float(g[1]))
+            run.front.append(float(g[2]))
+            run.bypass.append(float(g[3]))
+            run.local_h2.append(float(g[4]))
+            if on_sample is not None:
+                prior = lr[:step]
+                pv = g[5:5 + 16 * kn:2].reshape(8, kn)
+                sv = g[6:6 + 16 * kn:2].reshape(8, kn).astype(np.int64)
+                on_sample(len(run.times) - 1, {
+                    "obstacle_sph_force_N": lr[step, 1:4].tolist(),
+                    "obstacle_sph_impulse_Ns": (prior[:, 1:4] * prior[:, 11:12]).sum(0).tolist(),
+                    "obstacle_clamp_impulse_Ns": (prior[-1, 7:10] if len(prior) else np.zeros(3)).tolist(),
+                    "H2_local_m": run.local_h2[-1],
+                    "tap_pressure_Pa": pv[:, 0].tolist(), "tap_support": sv[:, 0].tolist(),
+                    "disk_pressure_Pa": disk_average(pv).tolist() if cfg.pressure_disk_diameter else [],
+                    "disk_support": sv.min(1).tolist() if cfg.pressure_disk_diameter else [],
+                    "obstacle_sph_moment_Nm": lr[step, 4:7].tolist(),
+                    "t_s": t_s, "slab_height_m": run.slab[-1], "height_operator_m": run.height[-1],
+                    "front_x_m": run.front[-1], "n_bypass": run.bypass[-1],
+                })
+            if progress is not None and step % (ow * 20) == 0:
+                progress(t_s, run.slab[-1])
+        st = sim.stats()
+        lr = sim.loads()
+        m = sim.meters()
+    finally:
+        sim.close()
+    final = bool(st["finished"])
+    last_eval = st["steps"] - 1
+    integ = lr[:last_eval] if final else lr          # the final sample is evaluated, not moved
+    run.sph_impulse = (integ[:, 1:4] * integ[:, 11:12]).sum(0).tolist()
+    run.clamp_impulse = (integ[-1, 7:10] if len(integ) else np.zeros(3)).tolist()
+    run.obstacle_clamps = int(integ[-1, 10]) if len(integ) else 0
+    run.max_speed = st["peak_speed"]
+    run.max_density_compression = st["peak_rho"] / cfg.rho0 - 1.0
+    run.boundary_escapes = st["escaped"]
+    run.steps = last_eval
+    run.simulated_time = time_at(last_eval, m, lr) + (0.0 if final else float(lr[-1, 11]))
+    run.wall_clock_s = time.time() - t_start
     return run

@@ -26,6 +26,7 @@ Fixtures
   kleefsman_gauges.npz  the Kleefsman gauge and tap operators on a cloud, and a
                      short run_case series; kleefsman_checkpoint_v3.pt a
                      checkpoint written by save_checkpoint
+  kleefsman_adaptive.npz  Kleefsman run_case with adaptive_dt
   tvc.npz            transport_velocity_correction on a block over a wall and
                      dam_break.run_case with tvc_every (four variants, and one
                      with shifting and the adaptive step)
@@ -669,6 +670,29 @@ def tvc(reference: str = "/root/reference") -> None:
     print("tvc fixture written to", HERE)
 
 
+def kleefsman_adaptive(reference: str = "/root/reference") -> None:
+    """kleefsman_obstacle_3d.run_case with adaptive_dt (:644-713) at dx = 0.05,
+    a Courant number that lets the step shrink with the flow speed."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import kleefsman_obstacle_3d as kl
+    torch.set_default_dtype(torch.float64)
+    cfg = kl.Kleefsman3DConfig(dx=0.05, t_end=0.45, output_every=25, adaptive_dt=True,
+                               max_courant=0.085, max_steps=4000)
+    samples = []
+    run = kl.run_case(cfg, on_sample=lambda i, s: samples.append(s))
+    d = run.to_dict()
+    keys = ["t_s", "slab_height_m", "height_operator_m", "front_x_m", "n_bypass", "H2_local_m"]
+    out = {"rows": np.array([[float(s[k]) for k in keys] for s in samples]), "row_keys": np.array(keys),
+           "force": np.array([s["obstacle_sph_force_N"] for s in samples]),
+           "impulse": np.array([s["obstacle_sph_impulse_Ns"] for s in samples]),
+           "steps": np.array(d["steps"]), "simulated_time": np.array(d["simulated_time_s"]),
+           "sph_impulse": np.array(d["obstacle_sph_impulse_Ns"]), "max_speed": np.array(d["max_speed"]),
+           "compression": np.array(d["max_density_compression"])}
+    np.savez_compressed(HERE / "kleefsman_adaptive.npz", **out)
+    print("adaptive kleefsman fixture written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
     if args and args[0] == "--only-shear":
@@ -683,6 +707,8 @@ if __name__ == "__main__":
         lind_shift(*args[1:])
     elif args and args[0] == "--only-tvc":
         tvc(*args[1:])
+    elif args and args[0] == "--only-kleefsman-adaptive":
+        kleefsman_adaptive(*args[1:])
     else:
         main(*args)
         shear(*args)
@@ -691,3 +717,4 @@ if __name__ == "__main__":
         kleefsman_gauges(*args)
         lind_shift(*args)
         tvc(*args)
+        kleefsman_adaptive(*args)

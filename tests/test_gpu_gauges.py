@@ -169,3 +169,32 @@ def test_gauges_need_a_step_and_valid_nodes():
         kl.gauge_spec(cfg, np.zeros((8, 3, 3)))
     with pytest.raises(ValueError):
         kl.tap_disk_nodes(cfg, kl.sensor_positions(cfg), 0.2)   # beyond the face
+
+
+def test_run_case_adaptive_step_matches_reference_run(golden):
+    """run_case with adaptive_dt (kleefsman_obstacle_3d.py:644-713): the step
+    shrinks with the flow speed and is not shortened onto t_end; sample times,
+    gauges, loads and the run summary against the reference's own run."""
+    import dataclasses
+    from tensorsph_amd import kleefsman as kl
+    g = golden("kleefsman_adaptive")
+    cfg = dataclasses.replace(gauge_cfg(), t_end=0.45, pressure_disk_diameter=0.0, adaptive_dt=True,
+                              max_courant=0.085, max_steps=4000)
+    samples = []
+    res = kl.run_case(cfg, on_sample=lambda i, s: samples.append(s))
+    keys = list(g["row_keys"])
+    rows = np.array([[float(s[k]) for k in keys] for s in samples])
+    assert rows.shape == g["rows"].shape
+    assert np.allclose(rows[:, 0], g["rows"][:, 0], rtol=1e-12, atol=0)      # sample times
+    for k in ("front_x_m", "H2_local_m"):
+        c = keys.index(k)
+        assert np.abs(rows[:, c] - g["rows"][:, c]).max() <= 1e-7 * max(np.abs(g["rows"][:, c]).max(), 1e-3)
+    for name, key in (("obstacle_sph_force_N", "force"), ("obstacle_sph_impulse_Ns", "impulse")):
+        got = np.array([s[name] for s in samples])
+        assert np.abs(got - g[key]).max() <= 1e-7 * max(np.abs(g[key]).max(), 1.0), name
+    d = res.to_dict()
+    assert d["steps"] == int(g["steps"])
+    assert d["simulated_time_s"] == pytest.approx(float(g["simulated_time"]), rel=1e-12)
+    assert np.allclose(d["obstacle_sph_impulse_Ns"], g["sph_impulse"], rtol=1e-7, atol=1e-9)
+    assert d["max_speed"] == pytest.approx(float(g["max_speed"]), rel=1e-7)
+    assert d["max_density_compression"] == pytest.approx(float(g["compression"]), rel=1e-7)

@@ -50,7 +50,8 @@ def test_obstacle_loads_track_oracle():
     stats = sim.stats()
     sim.close()
     ref = _oracle(cfg, steps)
-    assert loads.shape == (steps, 11)
+    assert loads.shape == (steps, 12)
+    assert np.all(loads[:, 11] == cfg.dt)          # the step size column
     assert np.array_equal(loads[:, 0], np.arange(steps))
     ref_loads = np.array(ref.loads)
     scale = np.abs(ref_loads[:, :3]).max()
@@ -94,7 +95,7 @@ def test_obstacle_clamp_matches_oracle():
     assert np.abs(got - st.pos).max() < 1e-10
     assert not np.all((got > lo) & (got < hi), axis=1).any()
     # cumulative clamp bookkeeping after the last step
-    assert loads.shape == (3, 11)
+    assert loads.shape == (3, 12)
     assert int(loads[-1, 10]) == st.clamped
     imp = np.asarray(st.clamp_impulse)
     assert np.abs(loads[-1, 7:10] - imp).max() <= 1e-9 * max(np.abs(imp).max(), 1e-30)
