@@ -50,6 +50,14 @@ typedef struct tsph_ctx tsph_ctx;
 const char* tsph_last_error(void);
 int tsph_abi_version(void);
 
+/* Layout of the parameter structs below as this library was compiled: entry
+ * `index` names a struct and one of its fields with the field's offset and
+ * size and the struct's size (bytes).  Indices run from 0 until the call
+ * returns TSPH_ERR_ARG.  No counterpart in the reference (which has no C
+ * ABI): it lets a binding that restates the structs check its copy. */
+int tsph_abi_field(int index, const char** struct_name, const char** field_name, int64_t* offset,
+                   int64_t* size, int64_t* struct_size);
+
 /* Context: owns device scratch (cell grid, neighbour lists) that grows on
  * demand; one per simulation / stream.  Not thread-safe across threads. */
 int tsph_ctx_create(tsph_ctx** out);
@@ -66,6 +74,15 @@ int tsph_ctx_destroy(tsph_ctx* ctx);
 int tsph_pairs_search(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int dim,
                       double cutoff, const double* box, int64_t fluid_count,
                       hipStream_t stream, int64_t* npairs);
+
+/* The O(N^2) search of neighbor_search.brute_force_pairs (neighbor_search.py:
+ * 29-41) and, with box != NULL, threed.neighbors3.brute_force_pairs_min_image
+ * (neighbors3.py:145-170): every pair i < j is tested with the same squared
+ * distance as the linked-cell search, and nothing else of that search is
+ * used, so the two can check each other.  At most 2^20 particles.
+ * Synchronises once; tsph_pairs_export writes the pairs. */
+int tsph_pairs_brute(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int dim, double cutoff,
+                     const double* box, hipStream_t stream, int64_t* npairs);
 
 /* Writes the pairs of the last search as (npairs, 2) int64, i < j, sorted
  * lexicographically: the reference's output format. */
@@ -103,7 +120,8 @@ typedef struct tsph_wcsph_params {
   double eigenvalue_floor;   /* Antuono eigenvalue floor (fraction of trace/D) */
   int32_t reaction_style;    /* 0 none; 1 2-D split loads; 2 rhs_3d wall force */
   int32_t has_wall_accel;
-  /* opt-in branches of droplet_impact.rhs (dim 2 only; zero = off) */
+  /* opt-in branches of droplet_impact.rhs (zero = off; dim 2 only, except the
+   * surface-pressure closure, which rhs_3d also runs: dam_break_3d.py:331-335) */
   double wrap_width;                   /* > 0: x separations modulo wrap_width (:921-927) */
   int32_t density_renormalization;     /* Shepard: rho lattice_kernel_sum / (sum_j W_ij + W(0)) (:954-964) */
   double lattice_kernel_sum;           /* lattice_kernel_sum(h, dx) (:577-598) */
@@ -160,6 +178,17 @@ int tsph_kernel_eval(int kind, const void* r, int64_t n, int precision, double h
 int tsph_apply_boundary(double* pos, double* vel, int64_t n, int dim, const double* lo,
                         const double* hi, double tol, unsigned long long* count,
                         hipStream_t stream);
+
+/* clamp_out_of_obstacle (kleefsman_obstacle_3d.py:208-266), in place on FP64
+ * (n, 3) pos/vel: a particle strictly inside the box (lo, hi) leaves through
+ * the nearest face (first of -x, +x, -y, +y, -z, +z on a tie) to that face
+ * -/+ eps, and that velocity component is zeroed.  *count (device uint64)
+ * receives the number moved.  mass (n, scalar type of `precision`) may be
+ * NULL; otherwise the momentum removed, sum m (v_before - v_after), is added
+ * to impulse (3 device doubles, accumulated across calls). */
+int tsph_obstacle_clamp(double* pos, double* vel, const void* mass, int precision, int64_t n,
+                        const double* lo, const double* hi, double eps, unsigned long long* count,
+                        double* impulse, hipStream_t stream);
 
 /* ---------------------------------------------------------------- 3-D operators
  * threed.operators3.accelerations (operators3.py:212-256) with
@@ -363,6 +392,10 @@ typedef struct tsph_sim_params {
    * keeps 1e-12 max(1, target_time). */
   int32_t adaptive_free_last;
   double adaptive_tol;
+  /* Time state a resumed run starts from (kleefsman_obstacle_3d.py:597-628):
+   * the simulated time at the start of the first step and, with adaptive =
+   * 1, the step-size variable (<= 0: op.dt).  Zero for a fresh run. */
+  double initial_time, initial_dt;
 } tsph_sim_params;
 
 /* pos: (n_fluid + n_wall, dim) FP64 device, fluid then wall; vel: (n_fluid,
@@ -389,6 +422,11 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks);
  * previous step's size, this step's size, the step-size variable]: 24
  * doubles per row.  The meters are of the state at the step's start. */
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
+/* Time state of the coming step (host, 4 doubles): simulated time at its
+ * start, the step-size variable, the size it integrates with (0 for an
+ * adaptive run's final sample) and the previous step's size.  What a
+ * checkpoint of an adaptive run records.  Synchronises. */
+int tsph_sim_time(tsph_sim* sim, double* out, hipStream_t stream);
 /* Obstacle loads of every completed step (host), 12 doubles per row: step,
  * F x/y/z, M x/y/z (about obstacle_centre), cumulative clamp impulse x/y/z
  * (sum of m (v_before - v_after) of clamped particles), the cumulative
@@ -416,9 +454,9 @@ int tsph_sim_destroy(tsph_sim* sim);
  * last evaluation's volumes m / rho over the fluid pairs within 2h, shift
  * -strength h v_ref dt grad C, tangential where C < free_surface C_ref,
  * capped at limit h; dt = 0 takes the last step's size (adaptive runs).
- * C_ref is c_reference when > 0, else the percentile of C (linear
- * interpolation).  v_ref <= 0 takes max(largest speed at the start
- * of the last step, v_floor).  info (may be NULL): C_ref, largest shift,
+ * C_ref is c_reference when >= 0 (0: no free surface), else the percentile
+ * of C (linear interpolation).  v_ref < 0 takes max(largest speed at the
+ * start of the last step, v_floor); v_ref = 0 moves nothing.  info (may be NULL): C_ref, largest shift,
  * fraction of rows capped, v_ref.  Synchronises. */
 typedef struct tsph_shift_params {
   double dt, v_ref, v_floor, strength, limit, free_surface, percentile, c_reference;
@@ -433,7 +471,8 @@ int tsph_sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hi
  * shift = alpha h^2 L(h^2 |g|^2) g with L = min(limiter_slope x, 1) when
  * limiter_slope > 0 (TruncatedLinear) else 1; scope 1 keeps rows with
  * C >= bulk_c C_ref (C, C_ref as for tsph_sim_shift), scope 2 rows with
- * |g| h <= bulk_g; max_shift > 0 caps |shift|; then the boundary clamp.
+ * |g| h <= bulk_g; max_shift >= 0 caps |shift| (0 clips every shift to
+ * zero), max_shift < 0 leaves it uncapped; then the boundary clamp.
  * info (may be NULL): largest |shift|, fraction capped, boundary crossings
  * (counted into the stepper's escapes).  Synchronises. */
 typedef struct tsph_tvc_params {

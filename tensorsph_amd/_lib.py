@@ -113,7 +113,8 @@ class SimParams(C.Structure):
                 ("obstacle_lo", _dbl * 3), ("obstacle_hi", _dbl * 3), ("obstacle_eps", _dbl),
                 ("adaptive", C.c_int32), ("target_time", _dbl), ("courant_num", _dbl),
                 ("dt_limit", _dbl), ("dt_safety", _dbl), ("dt_growth", _dbl),
-                ("adaptive_free_last", C.c_int32), ("adaptive_tol", _dbl)]
+                ("adaptive_free_last", C.c_int32), ("adaptive_tol", _dbl),
+                ("initial_time", _dbl), ("initial_dt", _dbl)]
 
 
 _dp = C.POINTER(_dbl)
@@ -128,6 +129,8 @@ def _declare(L):
         "tsph_ctx_destroy": [_vp],
         "tsph_pairs_search": [_vp, _vp, C.c_int, _i64, C.c_int, _dbl, C.POINTER(_dbl), _i64, _vp,
                               C.POINTER(_i64)],
+        "tsph_pairs_brute": [_vp, _vp, C.c_int, _i64, C.c_int, _dbl, C.POINTER(_dbl), _vp,
+                             C.POINTER(_i64)],
         "tsph_pairs_export": [_vp, _vp, _vp],
         "tsph_cells_export": [_vp, _vp, _vp, _vp, C.POINTER(_i64), _vp],
         "tsph_max_displacement": [_vp, _vp, C.c_int, _i64, C.c_int, _vp, _vp],
@@ -143,12 +146,14 @@ def _declare(L):
         "tsph_eos_tait": [_vp, _i64, C.c_int, _dbl, _dbl, _dbl, _dbl, _vp, _vp],
         "tsph_kernel_eval": [C.c_int, _vp, _i64, C.c_int, _dbl, _vp, _vp, _vp, _vp],
         "tsph_apply_boundary": [_vp, _vp, _i64, C.c_int, _dp, _dp, _dbl, _vp, _vp],
+        "tsph_obstacle_clamp": [_vp, _vp, _vp, C.c_int, _i64, _dp, _dp, _dbl, _vp, _vp, _vp],
         "tsph_sim_create": [C.POINTER(SimParams), _vp, _vp, _vp, _vp, C.POINTER(_vp)],
         "tsph_sim_step": [_vp, _i64, _i64, _vp],
         "tsph_sim_read": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_sim_stats": [_vp, _ip, _dp],
         "tsph_sim_meters": [_vp, _dp, _i64, _ip],
         "tsph_sim_loads": [_vp, _dp, _i64, _ip],
+        "tsph_sim_time": [_vp, _dp, _vp],
         "tsph_sim_timing": [_vp, C.c_int, _dp, _ip],
         "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],

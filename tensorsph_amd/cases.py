@@ -1,9 +1,11 @@
 """Deterministic initial states of the benchmark cases (host-side set-up).
 
-One-time lattice builders, written in numpy float64 so the particle
-coordinates are the reference's to the last bit (the reference builds them
-with ``torch.arange`` / ``meshgrid``; ``_arange`` below reproduces
-``torch.arange``'s element count and ``start + i * step`` values).
+One-time lattice builders in float64, so the particle coordinates are the
+reference's to the last bit.  The reference builds them with ``torch.arange``
+and ``meshgrid``; ``_arange`` below calls ``torch.arange`` on the host, whose
+values are not ``start + i * step`` rounded once (its vectorised fill rounds
+some elements one ulp away), so the lattices match the reference's CPU runs
+exactly.
 
 * ``cell_centres``, ``make_particles``, ``wall_lattice``,
   ``lattice_kernel_sum``, ``lattice_mass_unit``, ``hydrostatic_density``:
@@ -20,13 +22,13 @@ import math
 from functools import lru_cache
 
 import numpy as np
+import torch
 
 from .config import Config, DamBreakConfig, DamBreak3DConfig, Kleefsman3DConfig
 
 
 def _arange(start: float, end: float, step: float) -> np.ndarray:
-    count = int(math.ceil((end - start) / step))
-    return start + np.arange(max(count, 0), dtype=np.float64) * step
+    return torch.arange(float(start), float(end), float(step), dtype=torch.float64).numpy()
 
 
 # ---------------------------------------------------------------- kernels (host)

@@ -1003,6 +1003,8 @@ static DtCtl sim_dtctl(const tsph_sim* sim) {
   c.target_time = p.target_time;
   c.tol = p.adaptive_tol > 0.0 ? p.adaptive_tol : 1.0e-12 * (p.target_time > 1.0 ? p.target_time : 1.0);
   c.free_last = p.adaptive_free_last;
+  c.t0 = p.initial_time;
+  c.dt0 = p.adaptive && p.initial_dt > 0.0 ? p.initial_dt : p.op.dt;
   c.courant_num = p.courant_num;
   c.c0 = p.op.sound_speed;
   c.dt_limit = p.dt_limit;
@@ -1011,13 +1013,15 @@ static DtCtl sim_dtctl(const tsph_sim* sim) {
   return c;
 }
 
-// the time state before the first step
+// the time state before the first step (a resumed run starts from the
+// elapsed time and step-size variable it was saved with)
 __global__ void k_init_time(SimFlags* fl, DtCtl c) {
-  fl->dt = c.dt_cfg;
-  fl->last_dt = c.dt_cfg;
-  fl->elapsed = 0.0;
-  fl->final = c.adaptive && 0.0 >= c.target_time - c.tol ? 1ull : 0ull;
-  fl->step_dt = fl->final ? 0.0 : (c.adaptive && !c.free_last && c.target_time < c.dt_cfg ? c.target_time : c.dt_cfg);
+  fl->dt = c.dt0;
+  fl->last_dt = c.dt0;
+  fl->elapsed = c.t0;
+  fl->final = c.adaptive && c.t0 >= c.target_time - c.tol ? 1ull : 0ull;
+  const double left = c.target_time - c.t0;
+  fl->step_dt = fl->final ? 0.0 : (c.adaptive && !c.free_last && left < c.dt0 ? left : c.dt0);
 }
 
 // Queue one step (density [+diffusion], wall, force+integrate) reading the
@@ -1289,6 +1293,19 @@ int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, v
   return TSPH_OK;
 }
 
+int tsph_sim_time(tsph_sim* sim, double* out, hipStream_t stream) {
+  TSPH_REQUIRE(sim && out, TSPH_ERR_ARG, "bad arguments");
+  const SimFlags* fl = sim->flags.as<SimFlags>();
+  double t[4];
+  TSPH_CHECK_HIP(hipMemcpyAsync(&t[0], &fl->elapsed, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipMemcpyAsync(&t[1], &fl->dt, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipMemcpyAsync(&t[2], &fl->step_dt, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipMemcpyAsync(&t[3], &fl->last_dt, sizeof(double), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  for (int k = 0; k < 4; ++k) out[k] = t[k];
+  return TSPH_OK;
+}
+
 int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   if (counters) {
@@ -1509,23 +1526,39 @@ __global__ void k_shift_move(double* __restrict__ pos, const double* __restrict_
   atomicMax(&stats[2], ord_enc(sqrt(d2)));
 }
 
-// row values to the caller's order and back, through the row -> caller map
+// row values (w per row) to the caller's order and back, through the row ->
+// caller map
 template <typename S>
 __global__ void k_rows_to_caller(const int64_t* __restrict__ orig, int64_t n, const S* __restrict__ src,
-                                 S* __restrict__ dst) {
+                                 S* __restrict__ dst, int w = 1) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[orig[i]] = src[i];
+  if (i < n)
+    for (int d = 0; d < w; ++d) dst[orig[i] * w + d] = src[i * w + d];
 }
 template <typename S>
 __global__ void k_caller_to_rows(const int64_t* __restrict__ orig, int64_t n, const S* __restrict__ src,
-                                 S* __restrict__ dst) {
+                                 S* __restrict__ dst, int w = 1) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[orig[i]];
+  if (i < n)
+    for (int d = 0; d < w; ++d) dst[i * w + d] = src[orig[i] * w + d];
+}
+
+// After a correction the host decides the Verlet test on the corrected
+// positions.  When they are back within half the skin of the anchors, the
+// stale flag the last step may have left on the device is cleared too, so the
+// next step runs on the current lists, as the reference's pair cache would.
+inline int clear_stale_unless_pending(tsph_sim* sim, hipStream_t stream) {
+  if (sim->rebuild_pending) return TSPH_OK;
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
+  return TSPH_OK;
 }
 
 // A correction that needs the pairs within 2h of the current positions re-bins
 // first when the last step left the lists stale, and carries the last
-// evaluation's volumes into the new row order through the caller's order.
+// evaluation's volumes, densities, pressures and accelerations into the new
+// row order through the caller's order (tsph_sim_read reports them after
+// the correction).
 // That re-binning only completes the lists for the correction: it is not
 // counted, and *rebuilt tells the caller to leave a rebuild pending, so the
 // next step re-bins the corrected positions as the reference's pair cache
@@ -1537,9 +1570,13 @@ int rebuild_keep_volumes(tsph_sim* sim, hipStream_t stream, bool* rebuilt) {
   if (sim->rebuild_pending) {
     // the lists went stale in the last step: re-bin, and carry the last
     // evaluation's volumes into the new row order
-    TSPH_TRY(sim->shift_tmp.ensure(sizeof(S) * (size_t)(n + 1)));
-    k_rows_to_caller<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->vol.as<S>(),
-                                                            sim->shift_tmp.as<S>());
+    TSPH_TRY(sim->shift_tmp.ensure(sizeof(S) * (size_t)(n * (3 + D) + 1)));
+    S* carry = sim->shift_tmp.as<S>();
+    S* const fields[3] = {sim->vol.as<S>(), sim->rho.as<S>(), sim->p.as<S>()};
+    for (int f = 0; f < 3; ++f)
+      k_rows_to_caller<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, fields[f], carry + f * n);
+    k_rows_to_caller<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->acc.as<S>(),
+                                                            carry + 3 * n, D);
     TSPH_CHECK_LAUNCH();
     bool done = false;
     if constexpr (D == 2) {
@@ -1550,8 +1587,10 @@ int rebuild_keep_volumes(tsph_sim* sim, hipStream_t stream, bool* rebuilt) {
     sim->rebuilds--;
     sim->rebuild_forced = true;
     *rebuilt = true;
-    k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, sim->shift_tmp.as<S>(),
-                                                            sim->vol.as<S>());
+    for (int f = 0; f < 3; ++f)
+      k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, carry + f * n, fields[f]);
+    k_caller_to_rows<S><<<grid_for(n), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), n, carry + 3 * n,
+                                                            sim->acc.as<S>(), D);
     TSPH_CHECK_LAUNCH();
   }
   return TSPH_OK;
@@ -1592,10 +1631,11 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   const NbSlots nbr = sim_slots(sim);
   k_shift_conc<D, S><<<grid_for(n), kBlock, 0, stream>>>(a, nbr, c);
   TSPH_CHECK_LAUNCH();
+  // negative c_reference / v_ref: not given (the reference's None); zero is a value
   double c_ref = prm->c_reference;
-  if (!(c_ref > 0.0)) TSPH_TRY((percentile_of<S>(sim, c, sorted, n, prm->percentile, &c_ref, stream)));
+  if (c_ref < 0.0) TSPH_TRY((percentile_of<S>(sim, c, sorted, n, prm->percentile, &c_ref, stream)));
   double v_ref = prm->v_ref;
-  if (!(v_ref > 0.0)) v_ref = sim->last_speed > prm->v_floor ? sim->last_speed : prm->v_floor;
+  if (v_ref < 0.0) v_ref = sim->last_speed > prm->v_floor ? sim->last_speed : prm->v_floor;
   ShiftArgs sa;
   const double dt = prm->dt > 0.0 ? prm->dt : sim->last_step_dt;
   sa.amp = prm->strength * sim->prm.op.h * v_ref * dt;
@@ -1613,6 +1653,7 @@ int sim_shift(tsph_sim* sim, const tsph_shift_params* prm, double* info, hipStre
   // the next step re-bins when the corrected positions are more than half the
   // skin from the anchors, the pair cache's own test
   sim->rebuild_pending = sim->rebuild_forced || disp > 0.5 * sim->prm.skin;
+  TSPH_TRY(clear_stale_unless_pending(sim, stream));
   if (info) {
     info[0] = c_ref;
     info[1] = n > 0 ? ord_dec_host(h[1]) : 0.0;
@@ -1668,7 +1709,7 @@ __global__ void __launch_bounds__(kBlock) k_tvc_vec(WcsphArgs<S> a, NbSlots nbr,
 #pragma unroll
   for (int d = 0; d < D; ++d) { sh[d] = keep ? ta.coef * fac * gv[d] : 0.0; m2 += sh[d] * sh[d]; }
   double mag = sqrt(m2);
-  if (ta.cap > 0.0 && mag > ta.cap) {
+  if (ta.cap >= 0.0 && mag > ta.cap) {         // cap < 0: none; 0 clips every shift to zero
     const double sc = ta.cap / (mag > 1.0e-30 ? mag : 1.0e-30);
 #pragma unroll
     for (int d = 0; d < D; ++d) sh[d] = sh[d] * sc;
@@ -1745,6 +1786,7 @@ int sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* info, hipStream_t
   TSPH_CHECK_HIP(hipMemcpyAsync(h, st, sizeof(h), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));
   sim->rebuild_pending = sim->rebuild_forced || (n > 0 && ord_dec_host(h[2]) > 0.5 * sim->prm.skin);
+  TSPH_TRY(clear_stale_unless_pending(sim, stream));
   sim->escaped_extra += (int64_t)h[4];
   if (info) {
     info[0] = n > 0 ? ord_dec_host(h[1]) : 0.0;
@@ -1761,8 +1803,8 @@ extern "C" int tsph_sim_tvc(tsph_sim* sim, const tsph_tvc_params* prm, double* i
   TSPH_REQUIRE(sim->dim == 2, TSPH_ERR_ARG, "the transport-velocity correction runs on the 2-D stepper");
   TSPH_REQUIRE(sim->steps_done > 0, TSPH_ERR_ARG, "the correction needs the volumes of an evaluated step");
   TSPH_REQUIRE(prm->scope >= 0 && prm->scope <= 2, TSPH_ERR_ARG, "scope must be 0 (all), 1 (bulk) or 2 (bulk_g)");
-  TSPH_REQUIRE(prm->limiter_slope >= 0.0 && prm->max_shift >= 0.0 && prm->wall_volume >= 0.0, TSPH_ERR_ARG,
-               "limiter slope, max shift and wall volume must be non-negative");
+  TSPH_REQUIRE(prm->limiter_slope >= 0.0 && prm->wall_volume >= 0.0, TSPH_ERR_ARG,
+               "limiter slope and wall volume must be non-negative");
   TSPH_REQUIRE(prm->scope != 2 || prm->bulk_g >= 0.0, TSPH_ERR_ARG, "scope_tol must be non-negative");
   TSPH_REQUIRE(sim->n < (1ll << 31) - 1, TSPH_ERR_ARG, "too many rows for the concentration sort");
   if (sim->prec == TSPH_F64) return sim_tvc<2, double>(sim, prm, info, stream);

@@ -31,6 +31,7 @@ struct DtCtl {
   double dt_cfg, target_time, tol;
   double courant_num;               // max_courant * h_factor * dx, as the reference forms it
   double c0, dt_limit, dt_safety, dt_growth;
+  double t0, dt0;                   // time state of a resumed run: elapsed, step-size variable
 };
 
 // one entry of the host ring, packed on the device after every step and

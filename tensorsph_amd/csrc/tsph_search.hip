@@ -416,6 +416,87 @@ int fill_pairs(tsph_ctx* ctx, const T* pos, double cutoff, const double* box, in
   return TSPH_OK;
 }
 
+// ---------------------------------------------------------------- brute force
+// The O(N^2) search of brute_force_pairs (neighbor_search.py:29-41) and
+// brute_force_pairs_min_image (neighbors3.py:145-170): every row tests every
+// later row, with no cell machinery shared with the linked-cell search.  Rows
+// are staged through LDS a block at a time; a block's threads start at the
+// block's own first row, since earlier rows only pair with them as i < j.
+// MODE 0 counts each row's partners j > i, MODE 1 writes them in ascending j.
+template <typename T, int D, bool PER, int MODE>
+__global__ void __launch_bounds__(kBlock) k_brute_pairs(const T* __restrict__ pos, int64_t n, T c2, T b0, T b1,
+                                                         T b2, int64_t* __restrict__ count,
+                                                         const int64_t* __restrict__ off,
+                                                         int64_t* __restrict__ out) {
+  __shared__ T tile[kBlock * D];
+  const int64_t first = (int64_t)blockIdx.x * kBlock;
+  const int64_t i = first + threadIdx.x;
+  const T box[3] = {b0, b1, b2};
+  T xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = i < n ? pos[i * D + d] : T(0);
+  const int64_t base = (MODE == 1 && i < n) ? off[i] : 0;
+  int64_t k = 0;
+  for (int64_t j0 = first; j0 < n; j0 += kBlock) {
+    __syncthreads();
+    const int64_t jl = j0 + threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < D; ++d) tile[threadIdx.x * D + d] = jl < n ? pos[jl * D + d] : T(0);
+    __syncthreads();
+    const int64_t m = n - j0 < kBlock ? n - j0 : kBlock;
+    if (i >= n) continue;
+    for (int64_t t = 0; t < m; ++t) {
+      const int64_t j = j0 + t;
+      if (j <= i) continue;
+      if (dist2_pt<T, D, PER>(xi, tile + t * D, box) <= c2) {
+        if (MODE == 1) {
+          out[2 * (base + k)] = i;
+          out[2 * (base + k) + 1] = j;
+        }
+        ++k;
+      }
+    }
+  }
+  if (MODE == 0 && i < n) count[i] = k;
+}
+
+template <typename T, int D>
+int brute_pairs(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const double* box, int mode,
+                int64_t* out, hipStream_t stream, int64_t* npairs) {
+  const T c2 = (T)(cutoff * cutoff);
+  T b[3] = {T(0), T(0), T(0)};
+  if (box) for (int d = 0; d < D; ++d) b[d] = (T)box[d];
+  const unsigned grid = (unsigned)((n + kBlock - 1) / kBlock);
+  if (mode == 0) {
+    ctx->npairs = 0;
+    *npairs = 0;
+    if (n < 2) return TSPH_OK;
+    TSPH_TRY(ctx->pair_count.ensure(sizeof(int64_t) * (n + 1)));
+    TSPH_TRY(ctx->pair_off.ensure(sizeof(int64_t) * (n + 1)));
+    if (box)
+      k_brute_pairs<T, D, true, 0><<<grid, kBlock, 0, stream>>>(pos, n, c2, b[0], b[1], b[2],
+                                                                 ctx->pair_count.as<int64_t>(), nullptr, nullptr);
+    else
+      k_brute_pairs<T, D, false, 0><<<grid, kBlock, 0, stream>>>(pos, n, c2, b[0], b[1], b[2],
+                                                                  ctx->pair_count.as<int64_t>(), nullptr, nullptr);
+    TSPH_CHECK_LAUNCH();
+    int64_t total = 0;
+    TSPH_TRY(exclusive_scan(ctx, ctx->pair_count.as<unsigned long long>(), n, ctx->pair_off.as<int64_t>(),
+                            stream, &total));
+    ctx->npairs = total;
+    *npairs = total;
+    return TSPH_OK;
+  }
+  if (box)
+    k_brute_pairs<T, D, true, 1><<<grid, kBlock, 0, stream>>>(pos, n, c2, b[0], b[1], b[2], nullptr,
+                                                               ctx->pair_off.as<int64_t>(), out);
+  else
+    k_brute_pairs<T, D, false, 1><<<grid, kBlock, 0, stream>>>(pos, n, c2, b[0], b[1], b[2], nullptr,
+                                                                ctx->pair_off.as<int64_t>(), out);
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
 // remembered between tsph_pairs_search and tsph_pairs_export
 struct LastSearch {
   const void* pos = nullptr;
@@ -424,6 +505,8 @@ struct LastSearch {
   double box[3] = {0, 0, 0};
   int has_box = 0;
   int64_t fluid_count = -1;
+  int brute = 0;
+  int64_t n = 0;
 };
 static thread_local LastSearch g_last;
 
@@ -504,7 +587,7 @@ int tsph_pairs_search(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int 
   if (box) for (int d = 0; d < dim; ++d)
     TSPH_REQUIRE(box[d] > 0.0, TSPH_ERR_ARG, "box edges must be positive");
   g_last.pos = pos; g_last.dtype = dtype; g_last.dim = dim; g_last.cutoff = cutoff;
-  g_last.has_box = box != nullptr; g_last.fluid_count = fluid_count;
+  g_last.has_box = box != nullptr; g_last.fluid_count = fluid_count; g_last.brute = 0; g_last.n = n;
   if (box) for (int d = 0; d < dim; ++d) g_last.box[d] = box[d];
 #define TSPH_SEARCH(T, D) return search_pairs<T, D>(ctx, (const T*)pos, n, cutoff, box, fluid_count, stream, npairs)
   if (dtype == TSPH_F64) {
@@ -519,11 +602,51 @@ int tsph_pairs_search(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int 
 #undef TSPH_SEARCH
 }
 
+int tsph_pairs_brute(tsph_ctx* ctx, const void* pos, int dtype, int64_t n, int dim, double cutoff,
+                     const double* box, hipStream_t stream, int64_t* npairs) {
+  TSPH_REQUIRE(ctx && npairs, TSPH_ERR_ARG, "ctx and npairs must not be NULL");
+  TSPH_REQUIRE(dim >= 1 && dim <= 3, TSPH_ERR_ARG, "positions must have 1 to 3 dimensions, got %d", dim);
+  TSPH_REQUIRE(dtype == TSPH_F64 || dtype == TSPH_F32, TSPH_ERR_TYPE, "positions must be float32 or float64");
+  TSPH_REQUIRE(cutoff > 0.0 && cutoff < 1e300, TSPH_ERR_ARG, "cutoff must be finite and positive");
+  TSPH_REQUIRE(n >= 0 && n <= (1ll << 20), TSPH_ERR_ARG,
+               "the O(N^2) search takes at most 2^20 particles, got %lld", (long long)n);
+  if (box) for (int d = 0; d < dim; ++d)
+    TSPH_REQUIRE(box[d] > 0.0, TSPH_ERR_ARG, "box edges must be positive");
+  g_last.pos = pos; g_last.dtype = dtype; g_last.dim = dim; g_last.cutoff = cutoff;
+  g_last.has_box = box != nullptr; g_last.fluid_count = -1; g_last.brute = 1; g_last.n = n;
+  if (box) for (int d = 0; d < dim; ++d) g_last.box[d] = box[d];
+#define TSPH_BRUTE(T, D) return brute_pairs<T, D>(ctx, (const T*)pos, n, cutoff, box, 0, nullptr, stream, npairs)
+  if (dtype == TSPH_F64) {
+    if (dim == 1) TSPH_BRUTE(double, 1);
+    if (dim == 2) TSPH_BRUTE(double, 2);
+    TSPH_BRUTE(double, 3);
+  } else {
+    if (dim == 1) TSPH_BRUTE(float, 1);
+    if (dim == 2) TSPH_BRUTE(float, 2);
+    TSPH_BRUTE(float, 3);
+  }
+#undef TSPH_BRUTE
+}
+
 int tsph_pairs_export(tsph_ctx* ctx, int64_t* pairs, hipStream_t stream) {
   TSPH_REQUIRE(ctx, TSPH_ERR_ARG, "ctx must not be NULL");
   if (ctx->npairs == 0) return TSPH_OK;
   TSPH_REQUIRE(pairs, TSPH_ERR_ARG, "pairs must not be NULL");
   const double* box = g_last.has_box ? g_last.box : nullptr;
+  if (g_last.brute) {
+    int64_t unused = 0;
+#define TSPH_BRUTE(T, D) return brute_pairs<T, D>(ctx, (const T*)g_last.pos, g_last.n, g_last.cutoff, box, 1, pairs, stream, &unused)
+    if (g_last.dtype == TSPH_F64) {
+      if (g_last.dim == 1) TSPH_BRUTE(double, 1);
+      if (g_last.dim == 2) TSPH_BRUTE(double, 2);
+      TSPH_BRUTE(double, 3);
+    } else {
+      if (g_last.dim == 1) TSPH_BRUTE(float, 1);
+      if (g_last.dim == 2) TSPH_BRUTE(float, 2);
+      TSPH_BRUTE(float, 3);
+    }
+#undef TSPH_BRUTE
+  }
 #define TSPH_FILL(T, D) return fill_pairs<T, D>(ctx, (const T*)g_last.pos, g_last.cutoff, box, g_last.fluid_count, pairs, stream)
   if (g_last.dtype == TSPH_F64) {
     if (g_last.dim == 1) TSPH_FILL(double, 1);

@@ -516,9 +516,11 @@ extern "C" int tsph_wcsph_rhs_ex(tsph_ctx* ctx, const tsph_wcsph_params* prm, co
                                  void* acc, void* wall_p, void* wall_rho, void* reaction, hipStream_t stream) {
   TSPH_REQUIRE(ctx && prm, TSPH_ERR_ARG, "ctx and params must not be NULL");
   TSPH_REQUIRE(prm->dim == 2 || prm->dim == 3, TSPH_ERR_ARG, "dim must be 2 or 3, got %d", prm->dim);
+  // rhs_3d reads the surface-pressure closure (dam_break_3d.py:331-335) but none of the other
+  // opt-in branches of the 2-D rhs
   const bool opts = prm->wrap_width != 0.0 || prm->density_renormalization || prm->boundary_consistent_density ||
-                    prm->surface_pressure_closure || density_offset || nu_field;
-  TSPH_REQUIRE(!opts || prm->dim == 2, TSPH_ERR_ARG, "the opt-in rhs branches are 2-D only");
+                    density_offset || nu_field;
+  TSPH_REQUIRE(!opts || prm->dim == 2, TSPH_ERR_ARG, "the opt-in rhs branches other than the closure are 2-D only");
   TSPH_REQUIRE(prm->wrap_width >= 0.0 && prm->wrap_width < INFINITY, TSPH_ERR_ARG,
                "wrap_width must be a positive finite length (0 disables)");
   TSPH_REQUIRE(!prm->density_renormalization || prm->lattice_kernel_sum > 0.0, TSPH_ERR_ARG,
@@ -639,6 +641,52 @@ extern "C" int tsph_apply_boundary(double* pos, double* vel, int64_t n, int dim,
   if (dim == 1) k_apply_boundary<1><<<g, kBlock, 0, stream>>>(pos, vel, n, l[0], l[1], l[2], h[0], h[1], h[2], tol, count);
   else if (dim == 2) k_apply_boundary<2><<<g, kBlock, 0, stream>>>(pos, vel, n, l[0], l[1], l[2], h[0], h[1], h[2], tol, count);
   else k_apply_boundary<3><<<g, kBlock, 0, stream>>>(pos, vel, n, l[0], l[1], l[2], h[0], h[1], h[2], tol, count);
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
+namespace tsph {
+template <typename S>
+__global__ void k_obstacle_clamp(double* __restrict__ pos, double* __restrict__ vel, const S* __restrict__ mass,
+                                 int64_t n, double lo0, double lo1, double lo2, double hi0, double hi1,
+                                 double hi2, double eps, unsigned long long* __restrict__ count,
+                                 double* __restrict__ impulse) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int moved = 0;
+  if (i < n) {
+    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2};
+    double x[3], v[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { x[d] = pos[i * 3 + d]; v[d] = vel[i * 3 + d]; }
+    const double m = mass ? (double)mass[i] : 0.0;
+    moved = box_clamp(x, v, m, lo, hi, eps, mass ? impulse : nullptr);
+    if (moved) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) { pos[i * 3 + d] = x[d]; vel[i * 3 + d] = v[d]; }
+    }
+  }
+  unsigned long long c = moved;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+  if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, c);
+}
+}  // namespace tsph
+
+extern "C" int tsph_obstacle_clamp(double* pos, double* vel, const void* mass, int precision, int64_t n,
+                                   const double* lo, const double* hi, double eps, unsigned long long* count,
+                                   double* impulse, hipStream_t stream) {
+  TSPH_REQUIRE(lo && hi && count && n >= 0, TSPH_ERR_ARG, "bad obstacle clamp arguments");
+  TSPH_REQUIRE(!mass || impulse, TSPH_ERR_ARG, "mass needs an impulse accumulator");
+  TSPH_REQUIRE(precision == TSPH_F64 || precision == TSPH_F32, TSPH_ERR_ARG, "bad precision");
+  TSPH_CHECK_HIP(hipMemsetAsync(count, 0, sizeof(unsigned long long), stream));
+  if (n == 0) return TSPH_OK;
+  TSPH_REQUIRE(pos && vel, TSPH_ERR_ARG, "pos and vel must not be NULL");
+  const unsigned g = grid_for(n);
+  if (precision == TSPH_F64)
+    k_obstacle_clamp<double><<<g, kBlock, 0, stream>>>(pos, vel, (const double*)mass, n, lo[0], lo[1], lo[2],
+                                                       hi[0], hi[1], hi[2], eps, count, impulse);
+  else
+    k_obstacle_clamp<float><<<g, kBlock, 0, stream>>>(pos, vel, (const float*)mass, n, lo[0], lo[1], lo[2],
+                                                      hi[0], hi[1], hi[2], eps, count, impulse);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }

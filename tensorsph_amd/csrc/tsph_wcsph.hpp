@@ -116,27 +116,34 @@ struct StepArgs {
 // strictly inside the box leaves through the nearest face (first of -x, +x,
 // -y, +y, -z, +z on a tie) to that face -/+ eps and loses that velocity
 // component, whose momentum goes to the impulse accumulator.
+// Returns 1 when the particle was moved; its impulse component goes to imp[d]
+// (imp may be NULL).
+TSPH_D int box_clamp(double* xn, double* vn, double m, const double* lo, const double* hi, double eps,
+                     double* imp) {
+  bool inside = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) inside = inside && xn[d] > lo[d] && xn[d] < hi[d];
+  if (!inside) return 0;
+  int face = 0;
+  double best = xn[0] - lo[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) {
+    const int d = k >> 1;
+    const double c = (k & 1) ? hi[d] - xn[d] : xn[d] - lo[d];
+    if (c < best) { best = c; face = k; }
+  }
+  const int d = face >> 1;
+  xn[d] = (face & 1) ? hi[d] + eps : lo[d] - eps;
+  if (imp) atomicAdd(&imp[d], m * vn[d]);
+  vn[d] = 0.0;
+  return 1;
+}
+
 template <int D>
 TSPH_D void obstacle_clamp(double* xn, double* vn, double m, const StepArgs& st) {
   if constexpr (D == 3) {
     if (!st.obs_clamp) return;
-    bool inside = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) inside = inside && xn[d] > st.obs_lo[d] && xn[d] < st.obs_hi[d];
-    if (!inside) return;
-    int face = 0;
-    double best = xn[0] - st.obs_lo[0];
-#pragma unroll
-    for (int k = 1; k < 6; ++k) {
-      const int d = k >> 1;
-      const double c = (k & 1) ? st.obs_hi[d] - xn[d] : xn[d] - st.obs_lo[d];
-      if (c < best) { best = c; face = k; }
-    }
-    const int d = face >> 1;
-    xn[d] = (face & 1) ? st.obs_hi[d] + st.obs_eps : st.obs_lo[d] - st.obs_eps;
-    atomicAdd(&st.clamp_imp[d], m * vn[d]);
-    atomicAdd(st.clamp_cnt, 1ull);
-    vn[d] = 0.0;
+    if (box_clamp(xn, vn, m, st.obs_lo, st.obs_hi, st.obs_eps, st.clamp_imp)) atomicAdd(st.clamp_cnt, 1ull);
   }
 }
 

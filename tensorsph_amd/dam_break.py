@@ -152,7 +152,7 @@ def run_case(cfg: DamBreakConfig, pair_skin_factor: float | None = None, snapsho
         if cfg.tvc_scope == "bulk_g" and (not math.isfinite(cfg.tvc_bulk_g) or cfg.tvc_bulk_g < 0.0):
             raise ValueError("scope_tol must be finite and non-negative")
         tvc_kw = dict(alpha=0.05 if cfg.tvc_alpha is None else float(cfg.tvc_alpha), limiter_slope=slope,
-                      max_shift=0.0 if cfg.tvc_max_shift is None else float(cfg.tvc_max_shift) * cfg.h_factor * cfg.dx,
+                      max_shift=None if cfg.tvc_max_shift is None else float(cfg.tvc_max_shift) * cfg.h_factor * cfg.dx,
                       wall_volume=cfg.dx ** 2, scope=cfg.tvc_scope, bulk_c=float(cfg.tvc_bulk_c),
                       bulk_g=float(cfg.tvc_bulk_g))
     tvc_peak = tvc_clip_peak = 0.0

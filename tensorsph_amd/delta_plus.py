@@ -37,7 +37,9 @@ def delta_plus_step(pos, velocity, density, mass, *, dx, sound_speed, reference_
     Smoothing length 2 dx; positions advance with the new velocity plus one
     positional shift; wall stations of volume dx^2 with slip ghosts.  Pairs
     follow the package convention (i < j over cat(pos, wall)); cached lists
-    may include skin pairs.  float64 state (the step runs in FP64)."""
+    may include skin pairs.  The step runs in FP64: float32 state is widened
+    for it (the pair search, when pairs is None, runs in float32 as the
+    reference's does) and the result is returned as float32."""
     if (pos.ndim != 2 or pos.shape[1] != 2 or velocity.shape != pos.shape
             or density.shape != pos.shape[:1] or mass.shape != density.shape
             or not len(pos) or pos.dtype not in (torch.float32, torch.float64)):
@@ -66,8 +68,6 @@ def delta_plus_step(pos, velocity, density, mass, *, dx, sound_speed, reference_
            for x in (velocity, density, mass, wall, wall_velocity, wall_normal)):
         raise ValueError("states must share a floating dtype and device")
     dev = _lib.require_gpu(pos)
-    if pos.dtype != torch.float64:
-        raise TypeError("the GPU delta-plus step runs in float64; pass float64 state")
     if not bool(((density > 0) & (mass > 0) & density.isfinite() & mass.isfinite()).all()):
         raise ValueError("density and mass must be positive and finite")
     norms = wall_normal.norm(dim=1)
@@ -87,6 +87,11 @@ def delta_plus_step(pos, velocity, density, mass, *, dx, sound_speed, reference_
         i, j = pairs.unbind(1)
         if bool(((i < 0) | (i >= j) | (j >= n + nw)).any()):
             raise ValueError("each pair must have 0 <= i < j < number of particles")
+    out_dtype = pos.dtype
+    if out_dtype != torch.float64:
+        pos, velocity, density, mass, wall, wall_velocity, wall_normal = (
+            x.to(torch.float64) for x in (pos, velocity, density, mass, wall, wall_velocity,
+                                          wall_normal))
     ctx = _lib.default_context(dev)
     stream = _lib.stream_handle(dev)
     L = _lib.lib()
@@ -105,4 +110,8 @@ def delta_plus_step(pos, velocity, density, mass, *, dx, sound_speed, reference_
                                      _lib.ptr(out.density), _lib.ptr(out.acceleration),
                                      _lib.ptr(out.wall_reaction if nw else None),
                                      _lib.ptr(out.shift), stream))
+    if out_dtype != torch.float64:
+        out = DeltaPlusStep(*(x.to(out_dtype) for x in (out.position, out.velocity, out.density,
+                                                          out.acceleration, out.wall_reaction,
+                                                          out.shift)))
     return out

@@ -145,9 +145,17 @@ def stepper_options(cfg) -> dict:
             "surface_pressure_layers": int(cfg.surface_pressure_layers)}
 
 
-def _rhs_options(p, cfg, wrap_width) -> None:
+def _rhs_options(p, cfg, wrap_width, dim=2) -> None:
     """The opt-in branches of the 2-D rhs (droplet_impact.py:921-964,
-    1031-1112) into the parameter block."""
+    1031-1112) into the parameter block.  rhs_3d reads only the
+    surface-pressure closure of them (dam_break_3d.py:331-335)."""
+    if dim == 3:
+        if cfg.surface_pressure_closure:
+            p.surface_pressure_closure = 1
+            p.surface_pressure_threshold = float(cfg.surface_pressure_threshold)
+            p.surface_pressure_donors_all = 1 if cfg.surface_pressure_donors == "all" else 0
+            p.surface_pressure_layers = int(cfg.surface_pressure_layers)
+        return
     p.wrap_width = 0.0 if wrap_width is None else float(wrap_width)
     if cfg.density_renormalization:
         p.density_renormalization = 1
@@ -162,9 +170,9 @@ def _rhs_options(p, cfg, wrap_width) -> None:
 
 def run_wcsph(pos, vel, mass, cfg, pairs, wall, dt, wall_vel, wall_accel, split, dim,
               wall_mass_value, reaction_style, options=None):
-    """Shared driver of the 2-D ``rhs`` and the 3-D ``rhs_3d``.  ``options``
-    (2-D rhs only): dict of wrap_width, density_offset, nu_field; the Config
-    branches come with it.  With an empty pair list the reference returns
+    """Shared driver of the 2-D ``rhs`` and the 3-D ``rhs_3d``.  ``options``:
+    dict of wrap_width, density_offset, nu_field (2-D rhs); the Config
+    branches the rhs of that dimension reads come with it.  With an empty pair list the reference returns
     before every branch, so none is passed then."""
     dev = _lib.require_gpu(pos, vel, mass, wall, pairs)
     code = _precision(pos)
@@ -199,7 +207,7 @@ def run_wcsph(pos, vel, mass, cfg, pairs, wall, dt, wall_vel, wall_accel, split,
                         wa is not None, True)
     off = nuf = None
     if options is not None and pairs.numel() > 0:
-        _rhs_options(prm, cfg, options.get("wrap_width"))
+        _rhs_options(prm, cfg, options.get("wrap_width"), dim)
         if options.get("density_offset") is not None:
             off = options["density_offset"].detach().to(sdt).contiguous()
     if options is not None and options.get("nu_field") is not None:

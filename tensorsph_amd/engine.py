@@ -57,7 +57,8 @@ class WcsphSimulation:
     particles of mass ``wall_mass``.  ``precision`` is "fp64" or "fp32"
     (FP32 compensated).  ``lo``/``hi``/``tol`` give the apply_boundary box.
     ``adaptive`` (optional) = {"target_time", "courant_num", "dt_limit",
-    "dt_safety", "dt_growth"} runs the adaptive step of next_dt
+    "dt_safety", "dt_growth"[, "free_last", "tol", "initial_time",
+    "initial_dt"]} runs the adaptive step of next_dt
     (droplet_impact.py:495-552) on the device up to target_time; ``dt`` is
     then the configured (largest) step.
     """
@@ -131,6 +132,9 @@ class WcsphSimulation:
             p.dt_growth = float(adaptive["dt_growth"])
             p.adaptive_free_last = 1 if adaptive.get("free_last") else 0
             p.adaptive_tol = float(adaptive.get("tol", 0.0))
+            # a resumed run: the time and step-size variable it was saved with
+            p.initial_time = float(adaptive.get("initial_time", 0.0))
+            p.initial_dt = float(adaptive.get("initial_dt", 0.0))
         self.params = p
         handle = _vp()
         stream = _lib.stream_handle(dev)
@@ -166,6 +170,13 @@ class WcsphSimulation:
         _lib.check(_lib.lib().tsph_sim_stats(self.handle, c, pk))
         return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "cells": c[3],
                 "meter_rows": c[4], "finished": bool(c[5]), "peak_rho": pk[0], "peak_speed": pk[1]}
+
+    def time_state(self) -> dict:
+        """The coming step's time state: simulated time at its start, the
+        step-size variable, its own size, the previous step's size."""
+        out = (C.c_double * 4)()
+        _lib.check(_lib.lib().tsph_sim_time(self.handle, out, _lib.stream_handle(self.device)))
+        return {"elapsed": out[0], "dt": out[1], "step_dt": out[2], "last_dt": out[3]}
 
     def meters(self) -> np.ndarray:
         nrows = _i64(0)
@@ -239,22 +250,32 @@ def _shift(self, dt, v_ref=None, v_floor=0.0, strength=1.0, limit=0.25, free_sur
            percentile=95.0, c_reference=None) -> dict:
     """tsph_sim_shift: one Lind shifting pass on the fluid after the last
     step (v_ref None: max(speed at the start of that step, v_floor); dt 0:
-    that step's size)."""
+    that step's size; c_reference None: the percentile of the concentration).
+    An explicit 0 is a value, as in the reference: v_ref = 0 moves nothing,
+    c_reference = 0 marks no free surface."""
+    if v_ref is not None and float(v_ref) < 0.0:
+        raise ValueError("v_ref must be non-negative")
+    if c_reference is not None and float(c_reference) < 0.0:
+        raise ValueError("c_reference must be non-negative")
     p = _lib.ShiftParams(float(dt), -1.0 if v_ref is None else float(v_ref), float(v_floor),
                          float(strength), float(limit), float(free_surface), float(percentile),
-                         0.0 if c_reference is None else float(c_reference))
+                         -1.0 if c_reference is None else float(c_reference))
     info = (C.c_double * 4)()
     _lib.check(_lib.lib().tsph_sim_shift(self.handle, C.byref(p), info,
                                          _lib.stream_handle(self.device)))
     return {"c_reference": info[0], "max_shift": info[1], "clipped": info[2], "v_ref": info[3]}
 
 
-def _tvc(self, alpha=0.05, limiter_slope=0.0, max_shift=0.0, wall_volume=0.0, scope="all",
+def _tvc(self, alpha=0.05, limiter_slope=0.0, max_shift=None, wall_volume=0.0, scope="all",
          bulk_c=0.95, bulk_g=0.02, percentile=95.0) -> dict:
     """tsph_sim_tvc: one transport-velocity correction of the fluid after the
-    last step (scope "all", "bulk" or "bulk_g")."""
+    last step (scope "all", "bulk" or "bulk_g"; max_shift None: no cap, 0:
+    every shift clipped to zero, as in the reference)."""
+    if max_shift is not None and float(max_shift) < 0.0:
+        raise ValueError("max_shift must be non-negative")
     sc = {"bulk": 1, "bulk_g": 2}.get(scope, 0)
-    p = _lib.TvcParams(float(alpha), float(limiter_slope), float(max_shift), float(wall_volume), sc,
+    p = _lib.TvcParams(float(alpha), float(limiter_slope),
+                       -1.0 if max_shift is None else float(max_shift), float(wall_volume), sc,
                        float(bulk_c), float(bulk_g), float(percentile))
     info = (C.c_double * 3)()
     _lib.check(_lib.lib().tsph_sim_tvc(self.handle, C.byref(p), info, _lib.stream_handle(self.device)))

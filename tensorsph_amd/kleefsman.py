@@ -399,9 +399,8 @@ def run_case(cfg: Kleefsman3DConfig, device="cuda", progress=None, on_sample=Non
     if str(device) == "cpu":
         device = "cuda"
     if getattr(cfg, "adaptive_dt", False):
-        if checkpoint_path is not None:
-            raise NotImplementedError("checkpoints of adaptive-step runs are not built")
-        return _run_case_adaptive(cfg, device, progress, on_sample, precision)
+        return _run_case_adaptive(cfg, device, progress, on_sample, precision, checkpoint_path,
+                                  checkpoint_every, resume)
     t_start = time.time()
     pos0, mass, wall, rows, centre, box = kleefsman_state(cfg)
     nf, nw = pos0.shape[0], wall.shape[0]
@@ -542,90 +541,160 @@ def run_case(cfg: Kleefsman3DConfig, device="cuda", progress=None, on_sample=Non
     return run
 
 
-def _run_case_adaptive(cfg: Kleefsman3DConfig, device, progress, on_sample, precision):
+def _sequential_impulse(base, loads) -> np.ndarray:
+    """base + sum_k F_k dt_k over load rows, added one step at a time in step
+    order, as the reference accumulates its impulse tensor."""
+    acc = np.array(base, np.float64)
+    for row in loads:
+        acc = acc + row[1:4] * row[11]
+    return acc
+
+
+def _run_case_adaptive(cfg: Kleefsman3DConfig, device, progress, on_sample, precision,
+                       checkpoint_path=None, checkpoint_every: int = 0, resume: bool = True):
     """run_case with adaptive_dt (kleefsman_obstacle_3d.py:644-713): after each
     step dt = next_dt(dt, largest speed at its start), the step is never
     shortened onto t_end, and the first step starting at elapsed >= t_end -
-    1e-12 is the final sample; the loop also ends after max_steps."""
+    1e-12 is the final sample; the loop also ends after max_steps.
+
+    Checkpoints (:597-628, :701-706) hold the state after each step s with
+    s % checkpoint_every == 0, the simulated time at the start of step s + 1
+    and the step-size variable the rule carries into it; a resumed run starts
+    the stepper from that time state (tsph_sim_params.initial_time /
+    initial_dt) and numbers its steps from s + 1."""
     from .config import adaptive_controls
     t_start = time.time()
     pos0, mass, wall, rows_, centre, box = kleefsman_state(cfg)
     run = Kleefsman3DRun(cfg=cfg, fluid_particles=pos0.shape[0], wall_particles=wall.shape[0])
     max_steps = cfg.max_steps or cfg.steps_for_t_end()
     ctl = dict(adaptive_controls(cfg), target_time=float(cfg.t_end), free_last=True, tol=1.0e-12)
+    positions, velocity = pos0, np.zeros_like(pos0)
+    start_step, t0 = 0, 0.0
+    speeds = compression = 0.0
+    base_escapes = base_clamps = 0
+    base_sph, base_clamp = np.zeros(3), np.zeros(3)
+    resumed = False
+    if checkpoint_path is not None and resume and Path(checkpoint_path).exists():
+        state = load_checkpoint(checkpoint_path, cfg)
+        if tuple(state["positions"].shape) != pos0.shape:
+            raise ValueError(f"checkpoint holds {tuple(state['positions'].shape)} particles, "
+                             f"this configuration builds {pos0.shape}")
+        positions = state["positions"].double().numpy()
+        velocity = state["velocity"].double().numpy()
+        t0 = float(state["elapsed"])
+        ctl["initial_time"], ctl["initial_dt"] = t0, float(state["dt"])
+        start_step = int(state["step"]) + 1
+        run.times, run.height = list(state["times"]), list(state["height"])
+        run.local_h2, run.front = list(state["local_h2"]), list(state["front"])
+        run.bypass, run.slab = list(state["bypass"]), list(state.get("slab", []))
+        speeds, compression = float(state["speeds"]), float(state["compression"])
+        base_escapes, base_clamps = int(state["escapes"]), int(state["clamps"])
+        base_sph = np.array(state["sph_impulse"], np.float64)
+        base_clamp = np.array(state["clamp_impulse"], np.float64)
+        resumed = True
+        if progress is not None:
+            progress(t0, run.slab[-1] if run.slab else 0.0)
     nodes = tap_disk_nodes(cfg, sensor_positions(cfg), cfg.pressure_disk_diameter)
     spec = gauge_spec(cfg, nodes)
     kn = nodes.shape[1]
     ow = cfg.output_every
-    sim, _, _ = make_simulation(cfg, precision, device, adaptive=ctl)
-    done = 0
-    tcol = METER_KEYS.index("time")
+    ce = int(checkpoint_every) if checkpoint_path is not None else 0
+    sim, _, _ = make_simulation(cfg, precision, device, pos=positions, vel=velocity, adaptive=ctl)
+    clock = [0, t0]                       # [local step, simulated time at its start]
 
-    def time_at(step, m, lr):
-        """Simulated time at the start of ``step``: the last metered step's
-        time plus the sizes of the steps after it, summed in step order as
-        the stepper accumulates them."""
-        r = m[m[:, 0] <= step]
-        k0 = int(r[-1, 0])
-        t = float(r[-1, tcol])
-        for k in range(k0, step):
-            t += float(lr[k, 11])
+    def time_at(local, lr):
+        """Simulated time at the start of local step ``local``: the stepper's
+        sum of step sizes, in the same order."""
+        k, t = clock
+        if local < k:
+            k, t = 0, t0
+        for j in range(k, local):
+            t += float(lr[j, 11])
+        clock[0], clock[1] = local, t
         return t
+
+    def totals(lr):
+        """(sph impulse, clamp impulse, clamp count) after the load rows lr."""
+        if not len(lr):
+            return base_sph.copy(), base_clamp.copy(), base_clamps
+        return (_sequential_impulse(base_sph, lr), base_clamp + lr[-1, 7:10],
+                base_clamps + int(lr[-1, 10]))
 
     try:
         while True:
             st = sim.stats()
-            if st["finished"] or done > max_steps:
+            nxt = start_step + st["steps"]          # the next step to run (global)
+            if st["finished"] or nxt > max_steps:
                 break
-            target = min((done // ow + (0 if done % ow == 0 else 1)) * ow, max_steps)
-            sim.step(target + 1 - done, meter_every=ow)
+            through = ((nxt + ow - 1) // ow) * ow
+            if ce > 0:
+                through = min(through, ((nxt + ce - 1) // ce) * ce)
+            through = min(through, max_steps)
+            sim.step(through + 1 - nxt, meter_every=0)
             st = sim.stats()
-            done = st["steps"]
-            step = done - 1                     # the step whose start state is sampled
-            if not (st["finished"] or step % ow == 0):
-                continue
-            g = sim.gauges(spec)
-            lr = sim.loads()
-            t_s = time_at(step, sim.meters(), lr)
-            run.times.append(t_s)
-            run.slab.append(float(g[0]))
-            run.height.append(float(g[1]))
-            run.front.append(float(g[2]))
-            run.bypass.append(float(g[3]))
-            run.local_h2.append(float(g[4]))
-            if on_sample is not None:
-                prior = lr[:step]
-                pv = g[5:5 + 16 * kn:2].reshape(8, kn)
-                sv = g[6:6 + 16 * kn:2].reshape(8, kn).astype(np.int64)
-                on_sample(len(run.times) - 1, {
-                    "obstacle_sph_force_N": lr[step, 1:4].tolist(),
-                    "obstacle_sph_impulse_Ns": (prior[:, 1:4] * prior[:, 11:12]).sum(0).tolist(),
-                    "obstacle_clamp_impulse_Ns": (prior[-1, 7:10] if len(prior) else np.zeros(3)).tolist(),
-                    "H2_local_m": run.local_h2[-1],
-                    "tap_pressure_Pa": pv[:, 0].tolist(), "tap_support": sv[:, 0].tolist(),
-                    "disk_pressure_Pa": disk_average(pv).tolist() if cfg.pressure_disk_diameter else [],
-                    "disk_support": sv.min(1).tolist() if cfg.pressure_disk_diameter else [],
-                    "obstacle_sph_moment_Nm": lr[step, 4:7].tolist(),
-                    "t_s": t_s, "slab_height_m": run.slab[-1], "height_operator_m": run.height[-1],
-                    "front_x_m": run.front[-1], "n_bypass": run.bypass[-1],
-                })
-            if progress is not None and step % (ow * 20) == 0:
-                progress(t_s, run.slab[-1])
+            local = st["steps"] - 1               # the step just run
+            step = start_step + local
+            if st["finished"] or step % ow == 0:
+                g = sim.gauges(spec)
+                lr = sim.loads()
+                t_s = time_at(local, lr)
+                run.times.append(t_s)
+                run.slab.append(float(g[0]))
+                run.height.append(float(g[1]))
+                run.front.append(float(g[2]))
+                run.bypass.append(float(g[3]))
+                run.local_h2.append(float(g[4]))
+                if on_sample is not None:
+                    sph, clamp_imp, _ = totals(lr[:local])
+                    pv = g[5:5 + 16 * kn:2].reshape(8, kn)
+                    sv = g[6:6 + 16 * kn:2].reshape(8, kn).astype(np.int64)
+                    on_sample(len(run.times) - 1, {
+                        "obstacle_sph_force_N": lr[local, 1:4].tolist(),
+                        "obstacle_sph_impulse_Ns": sph.tolist(),
+                        "obstacle_clamp_impulse_Ns": clamp_imp.tolist(),
+                        "H2_local_m": run.local_h2[-1],
+                        "tap_pressure_Pa": pv[:, 0].tolist(), "tap_support": sv[:, 0].tolist(),
+                        "disk_pressure_Pa": disk_average(pv).tolist() if cfg.pressure_disk_diameter else [],
+                        "disk_support": sv.min(1).tolist() if cfg.pressure_disk_diameter else [],
+                        "obstacle_sph_moment_Nm": lr[local, 4:7].tolist(),
+                        "t_s": t_s, "slab_height_m": run.slab[-1], "height_operator_m": run.height[-1],
+                        "front_x_m": run.front[-1], "n_bypass": run.bypass[-1],
+                    })
+                if progress is not None and step % (ow * 20) == 0:
+                    progress(t_s, run.slab[-1])
+            if ce > 0 and step % ce == 0 and not st["finished"]:
+                ts = sim.time_state()
+                cur = sim.read()
+                sph, clamp_imp, clamps = totals(sim.loads())
+                ck = Kleefsman3DRun(cfg=cfg)
+                ck.times, ck.slab, ck.height = run.times, run.slab, run.height
+                ck.local_h2, ck.front, ck.bypass = run.local_h2, run.front, run.bypass
+                ck.sph_impulse, ck.clamp_impulse = sph.tolist(), clamp_imp.tolist()
+                save_checkpoint(checkpoint_path, cfg, step=step, elapsed=ts["elapsed"], dt=ts["dt"],
+                                positions=cur["position"], velocity=cur["velocity"], run=ck,
+                                speeds=max(speeds, st["peak_speed"]),
+                                compression=max(compression, st["peak_rho"] / cfg.rho0 - 1.0),
+                                escapes=base_escapes + st["escaped"], clamps=clamps)
         st = sim.stats()
         lr = sim.loads()
-        m = sim.meters()
     finally:
         sim.close()
     final = bool(st["finished"])
     last_eval = st["steps"] - 1
     integ = lr[:last_eval] if final else lr          # the final sample is evaluated, not moved
-    run.sph_impulse = (integ[:, 1:4] * integ[:, 11:12]).sum(0).tolist()
-    run.clamp_impulse = (integ[-1, 7:10] if len(integ) else np.zeros(3)).tolist()
-    run.obstacle_clamps = int(integ[-1, 10]) if len(integ) else 0
-    run.max_speed = st["peak_speed"]
-    run.max_density_compression = st["peak_rho"] / cfg.rho0 - 1.0
-    run.boundary_escapes = st["escaped"]
-    run.steps = last_eval
-    run.simulated_time = time_at(last_eval, m, lr) + (0.0 if final else float(lr[-1, 11]))
+    sph, clamp_imp, clamps = totals(integ)
+    run.sph_impulse, run.clamp_impulse = sph.tolist(), clamp_imp.tolist()
+    run.obstacle_clamps = clamps
+    run.max_speed = max(speeds, st["peak_speed"])
+    run.max_density_compression = max(compression, st["peak_rho"] / cfg.rho0 - 1.0)
+    run.boundary_escapes = base_escapes + st["escaped"]
+    run.steps = start_step + last_eval
+    run.simulated_time = time_at(last_eval, lr) + (0.0 if final else float(lr[-1, 11]))
     run.wall_clock_s = time.time() - t_start
+    run.resumed = resumed
+    if checkpoint_path is not None:
+        try:
+            os.unlink(checkpoint_path)
+        except FileNotFoundError:
+            pass
     return run

@@ -5,7 +5,10 @@
 ``PairListCache`` (:166-227): same signatures, same validation and
 exceptions, same output -- unordered pairs ``(i, j)``, ``i < j``, sorted
 lexicographically, int64, on the input device.  The search itself is the
-HIP cell sort of libtsph (tsph_pairs_search / tsph_pairs_export).
+HIP cell sort of libtsph (tsph_pairs_search / tsph_pairs_export); the
+brute-force searches are libtsph's separate O(N^2) kernel
+(tsph_pairs_brute), so the two can check each other as the reference's
+tests do.
 """
 from __future__ import annotations
 
@@ -47,7 +50,7 @@ def _dtype_code(t: torch.Tensor) -> tuple[torch.Tensor, int]:
     return t.to(torch.float32), _lib.F32
 
 
-def _search(positions, cutoff, box=None, fluid_count=None, with_cells=False):
+def _search(positions, cutoff, box=None, fluid_count=None, with_cells=False, brute=False):
     n, dim = _validate(positions, cutoff)
     if dim > 3:
         raise ValueError("positions must have at most 3 dimensions on the GPU search")
@@ -70,8 +73,12 @@ def _search(positions, cutoff, box=None, fluid_count=None, with_cells=False):
         box_arr = (C.c_double * 3)(*(edges + [0.0] * (3 - dim)))
     fc = -1 if fluid_count is None else int(fluid_count)
     L = _lib.lib()
-    _lib.check(L.tsph_pairs_search(ctx.handle, _lib.ptr(pos), code, n, dim, float(cutoff),
-                                   box_arr, fc, stream, C.byref(count)))
+    if brute:
+        _lib.check(L.tsph_pairs_brute(ctx.handle, _lib.ptr(pos), code, n, dim, float(cutoff),
+                                      box_arr, stream, C.byref(count)))
+    else:
+        _lib.check(L.tsph_pairs_search(ctx.handle, _lib.ptr(pos), code, n, dim, float(cutoff),
+                                       box_arr, fc, stream, C.byref(count)))
     out = torch.empty((count.value, 2), dtype=torch.long, device=dev)
     if count.value:
         _lib.check(L.tsph_pairs_export(ctx.handle, _lib.ptr(out), stream))
@@ -92,10 +99,9 @@ def cell_linked_pairs(positions: torch.Tensor, cutoff: float) -> torch.Tensor:
 
 
 def brute_force_pairs(positions: torch.Tensor, cutoff: float) -> torch.Tensor:
-    """Same contract as the reference's O(N^2) search; the linked-cell search
-    returns the identical set (both test ``d^2 <= cutoff^2`` on every pair in
-    adjacent cells, and no pair within the cutoff lies further apart)."""
-    return _search(positions, cutoff)
+    """The reference's O(N^2) search (neighbor_search.py:29-41): every pair
+    i < j tested on the GPU, with no cell machinery (at most 2^20 particles)."""
+    return _search(positions, cutoff, brute=True)
 
 
 def cell_linked_pairs_with_cells(positions: torch.Tensor, cutoff: float):

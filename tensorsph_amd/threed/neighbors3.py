@@ -1,10 +1,11 @@
 """Periodic and open 3-D pair search of tensorsph.threed.neighbors3
-(neighbors3.py:36-170), on the HIP cell sort of libtsph."""
+(neighbors3.py:36-170): the HIP cell sort of libtsph, and its separate
+O(N^2) kernel for the brute-force check."""
 from __future__ import annotations
 
 import torch
 
-from ..neighbor_search import cell_linked_pairs, periodic_pairs as _periodic_pairs
+from ..neighbor_search import _search, cell_linked_pairs, periodic_pairs as _periodic_pairs
 
 
 def box_pairs(positions: torch.Tensor, cutoff: float) -> torch.Tensor:
@@ -35,6 +36,9 @@ def periodic_pairs(positions: torch.Tensor, box, cutoff: float) -> torch.Tensor:
 
 def brute_force_pairs_min_image(positions: torch.Tensor, box, cutoff: float,
                                 chunk: int = 512) -> torch.Tensor:
-    """Same pair set as the reference's O(N^2) search (neighbors3.py:145-170);
-    computed by the cell search, whose output is identical."""
-    return _periodic_pairs(positions, box, cutoff)
+    """The reference's O(N^2) minimum-image search (neighbors3.py:145-170):
+    every pair i < j tested on the GPU, no cell machinery.  ``chunk`` is the
+    reference's memory knob and has no effect here."""
+    if positions.dim() != 2:
+        raise ValueError("positions must be (N, D)")
+    return _search(positions, cutoff, box=box, brute=True)

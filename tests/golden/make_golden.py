@@ -39,6 +39,9 @@ Fixtures
                      walls + Adami, Antuono diffusion) - operators3.py:51-295
   rhs3d.npz          dam_break_3d.rhs_3d with wall forces - dam_break_3d.py:290-422
   deltaplus.npz      delta_plus_step with and without walls - delta_plus.py:105-229
+  rhs2d_walls.npz    droplet_impact.rhs with wall_vel / wall_accel (FP64, FP32)
+  rhs3d_opts.npz     dam_break_3d.rhs_3d: FP32, Molteni, closure, no walls, pairs=None
+  dam3d_run.npz      dam_break_3d.run_case, fixed and adaptive step
 """
 from __future__ import annotations
 
@@ -688,14 +691,147 @@ def kleefsman_adaptive(reference: str = "/root/reference") -> None:
            "impulse": np.array([s["obstacle_sph_impulse_Ns"] for s in samples]),
            "steps": np.array(d["steps"]), "simulated_time": np.array(d["simulated_time_s"]),
            "sph_impulse": np.array(d["obstacle_sph_impulse_Ns"]), "max_speed": np.array(d["max_speed"]),
-           "compression": np.array(d["max_density_compression"])}
+           "compression": np.array(d["max_density_compression"]),
+           "clamps": np.array(d["obstacle_clamps"]),
+           "clamp_impulse": np.array(d["obstacle_clamp_impulse_Ns"]),
+           "escapes": np.array(d["boundary_escapes"])}
     np.savez_compressed(HERE / "kleefsman_adaptive.npz", **out)
     print("adaptive kleefsman fixture written to", HERE)
 
 
+def moving_walls(reference: str = "/root/reference") -> None:
+    """droplet_impact.rhs with moving walls (wall_vel :1119-1131 enters the
+    viscous and friction terms; wall_accel :1063-1072 enters the Adami body
+    term, which the reference adds only when gravity != 0), FP64 and FP32,
+    with split loads; and dam_break_3d.rhs_3d variants (FP32, Molteni
+    diffusion, the surface-pressure closure, no walls, pairs=None)."""
+    sys.path.insert(0, str(Path(reference) / "src"))
+    import torch
+    from tensorsph import droplet_impact as di
+    from tensorsph import dam_break_3d as db3
+    from tensorsph.neighbor_search import PairListCache
+    cpu = torch.device("cpu")
+    out = {}
+    base = dict(dx=0.05, pool_width=0.4, pool_depth=0.15, drop_radius=0.05, drop_center_y=0.25,
+                device="cpu", viscosity=0.01)
+    variants = {
+        "vel": ({"wall_friction": 0.4}, True, False),
+        "acc": ({}, False, True),
+        "both": ({"wall_friction": 0.25}, True, True),
+        "acc_nograv": ({"gravity": 0.0}, False, True),
+    }
+    for tag, dtype in (("f64", torch.float64), ("f32", torch.float32)):
+        torch.set_default_dtype(dtype)
+        for name, (over, with_vel, with_acc) in variants.items():
+            cfg = di.Config(**base, **over)
+            pos, mass = di.make_particles(cfg, cpu)
+            wall = di.wall_lattice(cfg, cpu)
+            g = torch.Generator().manual_seed(21)
+            vel = torch.randn(pos.shape, generator=g, dtype=dtype) * 0.05
+            pos = pos * 0.97
+            nw = wall.shape[0]
+            kw = {}
+            if with_vel:
+                kw["wall_vel"] = torch.randn((nw, 2), generator=g, dtype=dtype) * 0.3
+            if with_acc:
+                kw["wall_accel"] = torch.randn((nw, 2), generator=g, dtype=dtype) * 4.0
+            h = cfg.h_factor * cfg.dx
+            cache = PairListCache(2.0 * h, 0.25 * h, fluid_count=pos.shape[0])
+            pairs, _ = cache.get(torch.cat((pos, wall)))
+            rho, p, acc, loads = di.rhs(pos, vel, mass, cfg, pairs=pairs, wall=wall, split=True,
+                                        **kw)
+            key = f"{tag}_{name}"
+            out.update({key + "_pos": _np(pos), key + "_vel": _np(vel), key + "_mass": _np(mass),
+                        key + "_wall": _np(wall), key + "_pairs": _np(pairs),
+                        key + "_rho": _np(rho), key + "_p": _np(p), key + "_acc": _np(acc),
+                        key + "_reaction": _np(loads["reaction"]),
+                        key + "_face": _np(loads["face_pressure"])})
+            for k, v in kw.items():
+                out[key + "_" + k] = _np(v)
+            out[key + "_params"] = np.array([cfg.dx, cfg.h_factor, cfg.rho0, cfg.sound_speed,
+                                             cfg.gamma, cfg.gravity, cfg.viscosity,
+                                             cfg.wall_friction, cfg.dt,
+                                             float(di.lattice_mass_unit(cfg))])
+    torch.set_default_dtype(torch.float64)
+    np.savez_compressed(HERE / "rhs2d_walls.npz", **out)
+
+    out = {}
+    geo = dict(dx=0.05, tank_height=0.4, tank_depth=0.2, column_width=0.2, column_depth=0.2,
+               column_height=0.25, pool_width=0.4, viscosity=1e-3)
+    variants3 = {
+        "f32": ({"wall_friction": 0.3}, torch.float32, True, True),
+        "molt": ({"density_diffusion": 0.1, "dt": 1e-4}, torch.float64, True, True),
+        "closure": ({"surface_pressure_closure": True, "wall_friction": 0.2}, torch.float64,
+                    True, True),
+        "nowall": ({}, torch.float64, False, True),
+        "nopairs": ({"wall_friction": 0.3}, torch.float64, True, False),
+    }
+    for name, (over, dtype, with_wall, with_pairs) in variants3.items():
+        torch.set_default_dtype(dtype)
+        cfg3 = db3.DamBreak3DConfig(**geo, **over)
+        pos, mass = db3.make_column_3d(cfg3, cpu)
+        wall = db3.wall_lattice_3d(cfg3, cpu) if with_wall else None
+        g3 = torch.Generator().manual_seed(31)
+        vel = 0.1 * torch.randn(pos.shape, generator=g3, dtype=dtype)
+        pos = pos + 0.02 * cfg3.dx * torch.randn(pos.shape, generator=g3, dtype=dtype)
+        if name == "closure":
+            # lift the top layers so their summation density falls below threshold
+            top = pos[:, 1] > cfg3.column_height - 1.5 * cfg3.dx
+            pos[top, 1] += 0.3 * cfg3.dx
+        h = cfg3.h_factor * cfg3.dx
+        allp = pos if wall is None else torch.cat((pos, wall))
+        pairs = None
+        if with_pairs:
+            cache = PairListCache(2.0 * h, 0.25 * h, fluid_count=pos.shape[0])
+            pairs, _ = cache.get(allp)
+        rho, p, acc, reaction = db3.rhs_3d(pos, vel, mass, cfg3, pairs=pairs, wall=wall,
+                                           return_wall_force=True)
+        out.update({name + "_pos": _np(pos), name + "_vel": _np(vel), name + "_mass": _np(mass),
+                    name + "_rho": _np(rho), name + "_p": _np(p), name + "_acc": _np(acc),
+                    name + "_reaction": _np(reaction)})
+        if wall is not None:
+            out[name + "_wall"] = _np(wall)
+        if pairs is not None:
+            out[name + "_pairs"] = _np(pairs)
+        out[name + "_params"] = np.array([cfg3.dx, cfg3.h_factor, cfg3.rho0, cfg3.sound_speed,
+                                          cfg3.gamma, cfg3.gravity, cfg3.viscosity,
+                                          cfg3.wall_friction, cfg3.dt, cfg3.density_diffusion,
+                                          db3._mass_3d(cfg3.dx, cfg3.h_factor, cfg3.rho0, True)])
+    torch.set_default_dtype(torch.float64)
+    np.savez_compressed(HERE / "rhs3d_opts.npz", **out)
+
+    # dam_break_3d.run_case on a small tank, fixed and adaptive step
+    out = {}
+    for name, over in (("fixed", {"dt": 1e-3}), ("adaptive", {"adaptive_dt": True, "dt": 1.25e-3})):
+        cfg3 = db3.DamBreak3DConfig(dx=0.05, tank_height=0.5, tank_depth=0.25,
+                                    column_width=0.2, column_depth=0.25, column_height=0.3,
+                                    pool_width=0.6, viscosity=1e-3, sound_speed=15.0,
+                                    steps=80, output_every=10, wall_friction=0.1, **over)
+        run = db3.run_case(cfg3, snapshot_times=[0.03])
+        out.update({name + "_times": np.array(run.times), name + "_front": np.array(run.front),
+                    name + "_ke": np.array(run.kinetic_energy),
+                    name + "_pe": np.array(run.potential_energy),
+                    name + "_scalars": np.array([run.escaped, run.rebuilds, run.max_speed,
+                                                 run.max_density_compression,
+                                                 run.compression_before_wall_impact,
+                                                 -1.0 if run.wall_impact_time is None
+                                                 else run.wall_impact_time,
+                                                 run.initial_energy, run.final_energy,
+                                                 run.steps_taken, run.simulated_time, run.dt_min,
+                                                 run.dt_max, run.max_acoustic_courant,
+                                                 run.initial_front]),
+                    name + "_final": _np(run.final_positions),
+                    name + "_snap_t": np.array([s[0] for s in run.snapshots]),
+                    name + "_snap_pos": np.stack([_np(s[1]) for s in run.snapshots])})
+    np.savez_compressed(HERE / "dam3d_run.npz", **out)
+    print("moving-wall and rhs_3d fixtures written to", HERE)
+
+
 if __name__ == "__main__":
     args = sys.argv[1:]
-    if args and args[0] == "--only-shear":
+    if args and args[0] == "--only-moving-walls":
+        moving_walls(*args[1:])
+    elif args and args[0] == "--only-shear":
         shear(*args[1:])
     elif args and args[0] == "--only-order4":
         order4(*args[1:])
@@ -718,3 +854,4 @@ if __name__ == "__main__":
         lind_shift(*args)
         tvc(*args)
         kleefsman_adaptive(*args)
+        moving_walls(*args)

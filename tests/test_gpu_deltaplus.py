@@ -43,6 +43,30 @@ def test_delta_plus_with_walls_matches_reference_and_oracle(golden):
         assert scaled_err(n(got), ref[key]) < 1e-12, key
 
 
+def test_delta_plus_float32_state(golden):
+    """float32 state (the reference accepts it, delta_plus.py:121): the step
+    runs on the widened state and returns float32, within 1e-5 (scaled) of
+    the FP64 oracle on the FP32-rounded inputs (FP32 output rounding only)."""
+    from tensorsph_amd.delta_plus import delta_plus_step
+    g = golden("deltaplus")
+    f32 = torch.float32
+    out = delta_plus_step(t(g["pos"], f32), t(g["vel"], f32), t(g["rho"], f32), t(g["mass"], f32),
+                          wall=t(g["wall"], f32), wall_velocity=t(g["wall_vel"], f32),
+                          wall_normal=t(g["wall_normal"], f32), pairs=t(g["pairs"]),
+                          **_dp_kw(g["params"]))
+    assert out.velocity.dtype == f32 and out.wall_reaction.dtype == f32
+    f = lambda a: np.asarray(a).astype(np.float32).astype(np.float64)
+    wn = g["wall_normal"].astype(np.float32)
+    wn = (wn / np.linalg.norm(wn, axis=1)[:, None].astype(np.float32)).astype(np.float64)
+    ref = orc.delta_plus_step(f(g["pos"]), f(g["vel"]), f(g["rho"]), f(g["mass"]), wall=f(g["wall"]),
+                              wall_velocity=f(g["wall_vel"]), wall_normal=wn, pairs=g["pairs"],
+                              **_dp_kw(g["params"]))
+    for key, got in (("velocity", out.velocity), ("density", out.density),
+                     ("acceleration", out.acceleration), ("wall_reaction", out.wall_reaction),
+                     ("shift", out.shift), ("position", out.position)):
+        assert scaled_err(n(got), ref[key]) < 1e-5, key
+
+
 def test_delta_plus_free_patch_matches_reference(golden):
     g = golden("deltaplus")
     out = _gpu_step(g, "free_", dx=1 / 8, dt=1e-3, sound_speed=20.0, reference_speed=1.0,

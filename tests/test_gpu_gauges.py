@@ -171,30 +171,92 @@ def test_gauges_need_a_step_and_valid_nodes():
         kl.tap_disk_nodes(cfg, kl.sensor_positions(cfg), 0.2)   # beyond the face
 
 
-def test_run_case_adaptive_step_matches_reference_run(golden):
-    """run_case with adaptive_dt (kleefsman_obstacle_3d.py:644-713): the step
-    shrinks with the flow speed and is not shortened onto t_end; sample times,
-    gauges, loads and the run summary against the reference's own run."""
+def _adaptive_cfg():
     import dataclasses
+    return dataclasses.replace(gauge_cfg(), t_end=0.45, pressure_disk_diameter=0.0, adaptive_dt=True,
+                               max_courant=0.085, max_steps=4000)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_run_case_adaptive_step_matches_reference_run(golden, precision):
+    """run_case with adaptive_dt (kleefsman_obstacle_3d.py:644-713): the step
+    shrinks with the flow speed and is not shortened onto t_end; every gauge
+    column, the loads, the clamp record and the run summary against the
+    reference's own run (FP64: 1e-7 of the scale; FP32-compensated: 1e-5, the
+    run's integrated impulse 2e-5; band counts within one particle)."""
     from tensorsph_amd import kleefsman as kl
     g = golden("kleefsman_adaptive")
-    cfg = dataclasses.replace(gauge_cfg(), t_end=0.45, pressure_disk_diameter=0.0, adaptive_dt=True,
-                              max_courant=0.085, max_steps=4000)
+    cfg = _adaptive_cfg()
     samples = []
-    res = kl.run_case(cfg, on_sample=lambda i, s: samples.append(s))
+    res = kl.run_case(cfg, on_sample=lambda i, s: samples.append(s), precision=precision)
+    bar = 1e-7 if precision == "fp64" else 1e-5
     keys = list(g["row_keys"])
     rows = np.array([[float(s[k]) for k in keys] for s in samples])
     assert rows.shape == g["rows"].shape
-    assert np.allclose(rows[:, 0], g["rows"][:, 0], rtol=1e-12, atol=0)      # sample times
-    for k in ("front_x_m", "H2_local_m"):
+    assert np.allclose(rows[:, 0], g["rows"][:, 0], rtol=1e-12 if precision == "fp64" else 1e-6,
+                       atol=0)                                               # sample times
+    for k in ("front_x_m", "H2_local_m", "slab_height_m", "height_operator_m"):
         c = keys.index(k)
-        assert np.abs(rows[:, c] - g["rows"][:, c]).max() <= 1e-7 * max(np.abs(g["rows"][:, c]).max(), 1e-3)
+        err = np.abs(rows[:, c] - g["rows"][:, c])
+        scale = max(np.abs(g["rows"][:, c]).max(), 1e-3)
+        if k in ("front_x_m", "H2_local_m"):
+            assert err.max() <= bar * scale, k
+        else:
+            # band gauges count particles: a particle at a band edge may fall
+            # either side, so allow a lattice spacing at no more than 3 samples
+            assert (err > bar * scale).sum() <= 3 and err.max() <= cfg.dx + bar * scale, k
+    c = keys.index("n_bypass")
+    assert np.abs(rows[:, c] - g["rows"][:, c]).max() <= 1
     for name, key in (("obstacle_sph_force_N", "force"), ("obstacle_sph_impulse_Ns", "impulse")):
         got = np.array([s[name] for s in samples])
-        assert np.abs(got - g[key]).max() <= 1e-7 * max(np.abs(g[key]).max(), 1.0), name
+        assert np.abs(got - g[key]).max() <= bar * max(np.abs(g[key]).max(), 1.0), name
     d = res.to_dict()
     assert d["steps"] == int(g["steps"])
-    assert d["simulated_time_s"] == pytest.approx(float(g["simulated_time"]), rel=1e-12)
-    assert np.allclose(d["obstacle_sph_impulse_Ns"], g["sph_impulse"], rtol=1e-7, atol=1e-9)
-    assert d["max_speed"] == pytest.approx(float(g["max_speed"]), rel=1e-7)
-    assert d["max_density_compression"] == pytest.approx(float(g["compression"]), rel=1e-7)
+    assert d["simulated_time_s"] == pytest.approx(float(g["simulated_time"]), rel=1e-12 if bar < 1e-6 else 1e-6)
+    # the impulse integrates 2,059 steps of force: FP32-compensated rounding of
+    # the forces (each within 1e-5 above) and of the speeds that set each step
+    # size adds up to 1.5e-5 of it
+    assert scaled_err(d["obstacle_sph_impulse_Ns"], g["sph_impulse"]) <= (bar if precision == "fp64"
+                                                                          else 2e-5)
+    assert d["max_speed"] == pytest.approx(float(g["max_speed"]), rel=bar)
+    assert d["max_density_compression"] == pytest.approx(float(g["compression"]), rel=10 * bar)
+    assert d["obstacle_clamps"] == int(g["clamps"])
+    assert np.abs(np.array(d["obstacle_clamp_impulse_Ns"]) - g["clamp_impulse"]).max() <= \
+        bar * max(np.abs(g["clamp_impulse"]).max(), 1.0)
+    assert d["boundary_escapes"] == int(g["escapes"])
+
+
+def test_run_case_adaptive_resumes_from_its_checkpoint(tmp_path):
+    """An adaptive run that dies after a checkpoint resumes from the saved
+    positions, velocities, time and step-size variable and ends with the
+    uninterrupted run's record: the same steps and sample count, times to
+    1e-12, gauges, impulses and peaks to 1e-9."""
+    from tensorsph_amd import kleefsman as kl
+    cfg = _adaptive_cfg()
+    full = kl.run_case(cfg)
+    ck = tmp_path / "adaptive.pt"
+
+    class Crash(Exception):
+        pass
+
+    def die(i, s):
+        if s["t_s"] > 0.3:
+            raise Crash()
+
+    with pytest.raises(Crash):
+        kl.run_case(cfg, on_sample=die, checkpoint_path=ck, checkpoint_every=150)
+    assert ck.exists()
+    part = kl.load_checkpoint(ck, cfg)
+    assert part["step"] % 150 == 0 and part["step"] > 0
+    assert 0.0 < part["dt"] <= cfg.dt
+    res = kl.run_case(cfg, checkpoint_path=ck, checkpoint_every=150)
+    assert res.resumed and not ck.exists()
+    assert res.steps == full.steps and len(res.times) == len(full.times)
+    assert np.allclose(res.times, full.times, rtol=1e-12, atol=0)
+    for a, b in ((res.front, full.front), (res.local_h2, full.local_h2), (res.slab, full.slab)):
+        assert np.abs(np.array(a) - np.array(b)).max() <= 1e-9
+    assert res.obstacle_clamps == full.obstacle_clamps
+    assert res.boundary_escapes == full.boundary_escapes
+    assert np.allclose(res.sph_impulse, full.sph_impulse, rtol=1e-9, atol=1e-12)
+    assert res.max_speed == pytest.approx(full.max_speed, rel=1e-9)
+    assert res.simulated_time == pytest.approx(full.simulated_time, rel=1e-12)

@@ -24,6 +24,26 @@ def test_pairs_match_reference_golden(golden):
         assert np.array_equal(n(brute_force_pairs(pos, cut)), g[f"c{c}_brute"]), c
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_brute_force_kernel_checks_the_cell_search(dtype):
+    # the reference's own oracle pattern (tests/test_neighbor_search.py:7-34):
+    # the O(N^2) search against the linked cells, here two separate GPU kernels
+    from tensorsph_amd.neighbor_search import brute_force_pairs, cell_linked_pairs
+    from tensorsph_amd.threed.neighbors3 import brute_force_pairs_min_image, periodic_pairs
+    rng = np.random.default_rng(8)
+    for dim, n_pts, cut in ((1, 3000, 1e-3), (2, 12000, 0.012), (3, 9000, 0.06)):
+        pos = rng.uniform(0, 1, size=(n_pts, dim)).astype(dtype)
+        pos[1] = pos[0]                                   # a duplicate
+        brute = n(brute_force_pairs(t(pos), cut))
+        assert brute.shape[0] > n_pts
+        assert np.array_equal(brute, n(cell_linked_pairs(t(pos), cut)))
+        assert np.array_equal(brute, orc.brute_force_pairs(pos, cut))
+    pos = rng.uniform(0, 1, size=(6000, 3)).astype(dtype)
+    per = n(brute_force_pairs_min_image(t(pos), 1.0, 0.07))
+    assert np.array_equal(per, n(periodic_pairs(t(pos), 1.0, 0.07)))
+    assert brute_force_pairs(t(np.zeros((1, 2), dtype)), 1.0).shape == (0, 2)
+
+
 def test_periodic_pairs_match_reference_golden(golden):
     from tensorsph_amd.neighbor_search import periodic_pairs
     g = golden("pairs")

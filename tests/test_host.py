@@ -136,3 +136,39 @@ def test_list_prefilter_band_holds_the_fp32_error(dim, extent, rc, box):
     # the band is thin: of the pairs placed within 8 band widths of the build
     # radius (half of them) about one in eight falls inside it
     assert np.mean((d2f >= lo2) & (d2f <= hi2)) < 0.1
+
+
+def test_case_builders_match_reference_lattices(golden):
+    """cases.py against the reference's own lattices (lattices.npz: make_particles,
+    wall_lattice with and without top=, make_column, make_column_3d,
+    wall_lattice_3d, obstacle_lattice_3d, lattice_mass_unit and the 3-D
+    lattice sum), element for element."""
+    import numpy as np
+    from tensorsph_amd import cases
+    from tensorsph_amd.config import Config, DamBreakConfig, DamBreak3DConfig, Kleefsman3DConfig
+    g = golden("lattices")
+    cfg1 = Config(dx=0.005, pool_width=1.0, pool_depth=0.2, drop_radius=0.05, drop_center_y=0.35,
+                  viscosity=1e-6, sound_speed=10.0 * math.sqrt(2 * 9.81 * 0.2), dt=9.84e-5)
+    pos, mass, n_pool = cases.make_particles(cfg1)
+    assert np.array_equal(pos, g["impact_pos"])
+    assert np.abs(mass - g["impact_mass"]).max() <= 1e-15 * g["impact_mass"].max()
+    assert np.array_equal(cases.wall_lattice(cfg1, top=0.6), g["impact_wall_top06"])
+    assert np.array_equal(cases.wall_lattice(cfg1), g["impact_wall_default"])
+    assert cases.lattice_mass_unit(cfg1) == pytest.approx(float(g["impact_mass_unit"]), rel=1e-15)
+    assert cases.lattice_kernel_sum(1.3 * 0.005, 0.005) == pytest.approx(float(g["lattice_sum_13"]),
+                                                                          rel=1e-14)
+    cfg2 = DamBreakConfig(dx=0.05, tank_height=3.0, column_width=1.0, column_height=2.0,
+                          pool_width=4.0, sound_speed=10.0 * math.sqrt(2 * 9.81 * 2.0))
+    pos, mass = cases.make_column(cfg2)
+    assert np.array_equal(pos, g["dam_pos"])
+    assert np.abs(mass - g["dam_mass"]).max() <= 1e-15 * g["dam_mass"].max()
+    assert np.array_equal(cases.wall_lattice(cfg2, top=cfg2.tank_top()), g["dam_wall"])
+    cfg3 = DamBreak3DConfig(dx=0.05, tank_height=0.6, tank_depth=0.3, column_width=0.15,
+                            column_depth=0.15, column_height=0.3, pool_width=1.0)
+    pos, mass = cases.make_column_3d(cfg3)
+    assert np.array_equal(pos, g["dam3_pos"])
+    assert np.abs(mass - g["dam3_mass"]).max() <= 1e-15 * g["dam3_mass"].max()
+    assert np.array_equal(cases.wall_lattice_3d(cfg3), g["dam3_wall"])
+    assert cases.lattice_kernel_sum_3d(0.1, 0.13) == pytest.approx(float(g["k3_lattice_sum"]),
+                                                                    rel=1e-14)
+    assert np.array_equal(cases.obstacle_lattice_3d(Kleefsman3DConfig(dx=0.04)), g["kl_obstacle"])

@@ -399,6 +399,104 @@ def test_rhs3d_matches_reference(golden):
     assert scaled_err(reac, g["reaction"]) < 1e-10
 
 
+def rhs3d_opts_oracle(g, name, closure=None):
+    """The oracle's rhs_3d on one case of rhs3d_opts.npz (pairs=None cases use
+    the oracle's own search at 2h, as the reference does)."""
+    dx, hf, rho0, c0, gamma, grav, nu, fric, dt, diff, wmass = g[name + "_params"]
+    h = hf * dx
+    wall = g.get(name + "_wall")
+    pos = g[name + "_pos"].astype(np.float64)
+    pairs = g.get(name + "_pairs")
+    if pairs is None:
+        allp = pos if wall is None else np.concatenate([pos, wall.astype(np.float64)])
+        pairs = orc.cell_linked_pairs(allp, 2.0 * h)
+    return orc.rhs_morris(pos, g[name + "_vel"], g[name + "_mass"], wall, pairs, dim=3, h=h,
+                          rho0=rho0, c0=c0, gamma=gamma, gravity=grav, nu=nu, wall_friction=fric,
+                          dt=dt, diff_scale=2 * diff * c0 * h, diff_form=0, wall_mass=wmass,
+                          kernel3d=True, reaction_style=2,
+                          closure=(name == "closure") if closure is None else closure)
+
+
+@pytest.mark.parametrize("name", ["molt", "closure", "nowall", "nopairs"])
+def test_rhs3d_options_match_reference(golden, name):
+    g = golden("rhs3d_opts")
+    rho, p, acc, wp, wr, reac = rhs3d_opts_oracle(g, name)
+    assert scaled_err(rho, g[name + "_rho"]) < 1e-13
+    assert scaled_err(p, g[name + "_p"]) < 1e-11
+    assert scaled_err(acc, g[name + "_acc"]) < 1e-10
+    if name != "nowall":
+        assert scaled_err(reac, g[name + "_reaction"]) < 1e-10
+    else:
+        assert g[name + "_reaction"].shape == (0, 3)
+
+
+def test_rhs3d_closure_and_diffusion_change_the_result(golden):
+    # the fixture cases exercise their branches: switching the branch off
+    # moves the result by far more than the parity bar
+    g = golden("rhs3d_opts")
+    rho, p, acc, *_ = rhs3d_opts_oracle(g, "closure", closure=False)
+    assert scaled_err(p, g["closure_p"]) > 1e-3
+    dx, hf, rho0, c0, gamma, grav, nu, fric, dt, diff, wmass = g["molt_params"]
+    rho, *_ = orc.rhs_morris(g["molt_pos"], g["molt_vel"], g["molt_mass"], g["molt_wall"],
+                             g["molt_pairs"], dim=3, h=hf * dx, rho0=rho0, c0=c0, gamma=gamma,
+                             gravity=grav, nu=nu, wall_friction=fric, dt=dt, wall_mass=wmass,
+                             kernel3d=True)
+    assert scaled_err(rho, g["molt_rho"]) > 1e-9
+
+
+def test_rhs3d_fp32_reference_within_fp32_rounding_of_the_fp64_oracle(golden):
+    # The reference's own FP32 rhs_3d sits 3e-5..5e-5 (scaled) from the FP64
+    # oracle on the same FP32 state: its plain FP32 sums and Tait law lose
+    # that much.  The FP32-compensated kernels are held to 1e-5 of the FP64
+    # oracle instead (GPU tests).
+    g = golden("rhs3d_opts")
+    rho, p, acc, wp, wr, reac = rhs3d_opts_oracle(g, "f32")
+    assert scaled_err(rho, g["f32_rho"]) < 1e-6
+    assert scaled_err(p, g["f32_p"]) < 1e-4
+    assert scaled_err(acc, g["f32_acc"]) < 1e-4
+
+
+def walls_oracle(g, key, cast=None):
+    """The oracle's 2-D rhs on one moving-wall case of rhs2d_walls.npz."""
+    dx, hf, rho0, c0, gamma, grav, nu, fric, dt, unit = g[key + "_params"]
+    f = (lambda a: a) if cast is None else (lambda a: np.asarray(a).astype(cast).astype(np.float64))
+    wv = g.get(key + "_wall_vel")
+    wa = g.get(key + "_wall_accel")
+    return orc.rhs_morris(f(g[key + "_pos"]), f(g[key + "_vel"]), f(g[key + "_mass"]),
+                          f(g[key + "_wall"]), g[key + "_pairs"], dim=2, h=hf * dx, rho0=rho0,
+                          c0=c0, gamma=gamma, gravity=grav, nu=nu, wall_friction=fric, dt=dt,
+                          wall_mass=float(unit if cast is None else np.asarray(unit).astype(cast)),
+                          wall_vel=None if wv is None else f(wv),
+                          wall_accel=None if wa is None else f(wa), reaction_style=1)
+
+
+@pytest.mark.parametrize("name", ["vel", "acc", "both", "acc_nograv"])
+def test_moving_walls_match_reference(golden, name):
+    g = golden("rhs2d_walls")
+    key = "f64_" + name
+    rho, p, acc, wp, wr, reac = walls_oracle(g, key)
+    assert scaled_err(rho, g[key + "_rho"]) < 1e-13
+    assert scaled_err(p, g[key + "_p"]) < 1e-11
+    assert scaled_err(acc, g[key + "_acc"]) < 1e-11
+    assert scaled_err(wp, g[key + "_face"]) < 1e-11
+    assert scaled_err(reac, g[key + "_reaction"]) < 1e-11
+
+
+def test_moving_walls_enter_the_result(golden):
+    # wall_vel moves the viscous term, wall_accel the wall pressure; without
+    # gravity the reference drops the body term, wall_accel included
+    g = golden("rhs2d_walls")
+    stat = lambda key: orc.rhs_morris(
+        g[key + "_pos"], g[key + "_vel"], g[key + "_mass"], g[key + "_wall"], g[key + "_pairs"],
+        dim=2, h=g[key + "_params"][1] * g[key + "_params"][0], rho0=g[key + "_params"][2],
+        c0=g[key + "_params"][3], gamma=g[key + "_params"][4], gravity=g[key + "_params"][5],
+        nu=g[key + "_params"][6], wall_friction=g[key + "_params"][7],
+        wall_mass=float(g[key + "_params"][9]), reaction_style=1)
+    assert scaled_err(stat("f64_vel")[2], g["f64_vel_acc"]) > 1e-6
+    assert scaled_err(stat("f64_acc")[3], g["f64_acc_face"]) > 1e-4
+    assert scaled_err(stat("f64_acc_nograv")[3], g["f64_acc_nograv_face"]) < 1e-13
+
+
 # ----------------------------------------------------------------- delta-plus (A13)
 
 def _dp_kw(prm):
