@@ -325,6 +325,18 @@ int tsph_conservation_row(tsph_ctx* ctx, const double* pos, const double* vel, c
                           const void* rho, int64_t n, int precision, double rho0, double sound_speed, double gamma,
                           const double* box, double wavenumber, double* out, hipStream_t stream);
 
+/* tsph_conservation_row without a context and without allocating: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops).  Its per-block partials
+ * live in a fixed 8 MiB device array of the library, so n is at most
+ * 12,782,336 rows (larger n: TSPH_ERR_ARG).  Same kernels and grid as
+ * tsph_conservation_row, so the same row bit for bit.  Calls that overlap on
+ * one GPU share that array and race; each device has its own.
+ * Synchronises. */
+int tsph_conservation_row_noalloc(const double* pos, const double* vel, const void* mass, const void* rho,
+                                  int64_t n, int precision, double rho0, double sound_speed, double gamma,
+                                  const double* box, double wavenumber, double* out, hipStream_t stream);
+
 /* ---------------------------------------------------------------- periodic stepper
  * The step loop of threed.solver.run_shear_wave (solver.py:211-308) with the
  * state resident in HBM: periodic_pairs at 2h + skin (neighbors3.py:71-142)

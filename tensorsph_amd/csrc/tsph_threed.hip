@@ -14,6 +14,8 @@
 // neighbour list, so no two threads write one address and no float atomics
 // are used.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <math.h>
 
 #include <type_traits>
@@ -949,6 +951,39 @@ int psim_run(tsph_psim* ps, int64_t nsteps, hipStream_t stream) {
 
 }  // namespace tsph
 
+// ---------------------------------------------------------------- allocation-free conservation row
+// Per-block partials of tsph_conservation_row_noalloc live in this file-scope
+// array: DG_N doubles per block of kBlock rows plus the DG_N of the reduced
+// row, 8 MiB in all (up to kNoallocMaxRows rows).  Host code reaches it only
+// through the device address hipGetSymbolAddress returns, once per device.
+constexpr int64_t kNoallocDoubles = 1 << 20;
+constexpr int64_t kNoallocMaxBlocks = kNoallocDoubles / DG_N - 1;
+constexpr int64_t kNoallocMaxRows = kNoallocMaxBlocks * kBlock;
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on the current device, looked up once
+int noalloc_scratch(double** out) {
+  int dev = -1;
+  TSPH_CHECK_HIP(hipGetDevice(&dev));
+  TSPH_REQUIRE(dev >= 0 && dev < kMaxDevices, TSPH_ERR_ARG, "device %d outside the %d this library tracks", dev,
+               kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    TSPH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TSPH_REQUIRE(p != nullptr, TSPH_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return TSPH_OK;
+}
+}  // namespace
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -1015,6 +1050,36 @@ int tsph_conservation_row(tsph_ctx* ctx, const double* pos, const double* vel, c
   const unsigned g = n > 0 ? grid_for(n) : 1;
   TSPH_TRY(ctx->s_red.ensure(sizeof(double) * DG_N * (g + 1)));
   double* red = ctx->s_red.as<double>();
+  if (precision == TSPH_F64)
+    k3_diag<double><<<g, kBlock, 0, stream>>>(pos, vel, (const double*)mass, (const double*)rho, n, rho0,
+                                              sound_speed, gamma, b[0], b[1], b[2], wavenumber, red);
+  else
+    k3_diag<float><<<g, kBlock, 0, stream>>>(pos, vel, (const float*)mass, (const float*)rho, n, rho0,
+                                             sound_speed, gamma, b[0], b[1], b[2], wavenumber, red);
+  k3_diag_reduce<<<1, 256, 0, stream>>>(red, g, red + (size_t)DG_N * g);
+  TSPH_CHECK_LAUNCH();
+  TSPH_CHECK_HIP(hipMemcpyAsync(out, red + (size_t)DG_N * g, sizeof(double) * DG_N, hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  return TSPH_OK;
+}
+
+int tsph_conservation_row_noalloc(const double* pos, const double* vel, const void* mass, const void* rho,
+                                  int64_t n, int precision, double rho0, double sound_speed, double gamma,
+                                  const double* box, double wavenumber, double* out, hipStream_t stream) {
+  TSPH_REQUIRE(n >= 0 && n <= kNoallocMaxRows, TSPH_ERR_ARG,
+               "%lld rows exceed the %lld the allocation-free conservation row holds", (long long)n,
+               (long long)kNoallocMaxRows);
+  TSPH_REQUIRE(out, TSPH_ERR_ARG, "out must not be NULL");
+  TSPH_REQUIRE(precision == TSPH_F64 || precision == TSPH_F32, TSPH_ERR_ARG, "bad precision");
+  TSPH_REQUIRE(n == 0 || (pos && vel && mass && rho), TSPH_ERR_ARG, "NULL array");
+  TSPH_REQUIRE(rho0 > 0.0 && gamma > 0.0 && gamma != 1.0, TSPH_ERR_ARG, "rho0 > 0 and gamma > 0, != 1 required");
+  double b[3] = {1.0, 1.0, 1.0};
+  if (box) for (int d = 0; d < 3; ++d) { TSPH_REQUIRE(box[d] > 0.0, TSPH_ERR_ARG, "box edges must be positive"); b[d] = box[d]; }
+  double* red = nullptr;
+  TSPH_TRY(noalloc_scratch(&red));
+  // the same kernels and grid as tsph_conservation_row: every block writes its
+  // partials before the reduction reads them, so nothing needs clearing
+  const unsigned g = n > 0 ? grid_for(n) : 1;
   if (precision == TSPH_F64)
     k3_diag<double><<<g, kBlock, 0, stream>>>(pos, vel, (const double*)mass, (const double*)rho, n, rho0,
                                               sound_speed, gamma, b[0], b[1], b[2], wavenumber, red);
