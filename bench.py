@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--tile", default=None,
                     help="fixed layout instead of the autotuner's choice: a tile shape TXxTY "
-                         "(e.g. 8x6) or 'lists'")
+                         "(e.g. 8x6), 'lists' or 'halo'")
     ap.add_argument("--no-serve", action="store_true", help="skip the served δ⁺ batch")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--serve-steps", type=int, default=500)
@@ -115,9 +115,9 @@ def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0):
 def time_gpu(sim, steps, warmup, tile=None):
     import torch
     from tensorsph_amd.autotune import tune_tiles
-    if tile == "lists":
-        sim.configure(tiles=False)
-        tuned = {"best": "lists", "ms": {}}
+    if tile in ("lists", "halo"):
+        sim.configure(tiles=False, halo=tile == "halo")
+        tuned = {"best": tile, "ms": {}}
         sim.step(max(warmup, 1))
     elif tile is not None:
         sim.configure(tiles=True, tile=tile)
@@ -440,11 +440,11 @@ def main():
     nf, nw = pos.shape[0], wall.shape[0]
     h = cfg.h_factor * cfg.dx
 
-    if args.tile is None or args.tile.lower() == "lists":
-        tile = args.tile and "lists"
+    if args.tile is None or args.tile.lower() in ("lists", "halo"):
+        tile = args.tile and args.tile.lower()
     else:
         tile = tuple(int(v) for v in args.tile.lower().split("x"))
-    if args.diffusion > 0.0:
+    if args.diffusion > 0.0 and tile != "halo":
         tile = "lists"
     sim = make_sim(cfg, pos, mass, wall, wmass, "fp32", args.diffusion)
     wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup, tile)
@@ -462,7 +462,7 @@ def main():
     b_force = nf * (32 * 2 + 2 * s) + nw * (16 + 3 * s)
     b_density = nf * (16 + 2 * s) + nw * (16 + s)
     force_ms = sweep_ms[2] / max(nsteps, 1)
-    on_lists = args.diffusion > 0.0 or tuned["best"] == "lists"
+    on_lists = args.diffusion > 0.0 or tuned["best"] in ("lists", "halo")
     kernel, prof = ("k_step_force", PROFILE_LISTS) if on_lists else ("k_tile_force", PROFILE)
     achieved = b_force / (force_ms * 1e-3) / 1e9 if force_ms > 0 else 0.0
     step_bytes = nf * (40 * 2 + 4 * s) + nw * (24 * 2 + 2 * s)

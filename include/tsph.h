@@ -444,11 +444,18 @@ int tsph_sim_time(tsph_sim* sim, double* out, hipStream_t stream);
  * (sum of m (v_before - v_after) of clamped particles), the cumulative
  * number of clamped particles and the step's size. */
 int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
-/* Sweep layout.  tiles = 1 (the 2-D default) runs the LDS cell-tile sweeps
- * with tile_x x tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108,
- * tile_x tile_y <= 64); tiles = 0 runs the CSR neighbour-list sweeps (the 3-D
- * path).  Takes effect at the next step, which rebuilds the layout. */
+/* Sweep layout.  tiles = 1 runs the LDS cell-tile sweeps with tile_x x
+ * tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108, tile_x tile_y <=
+ * 64); tiles = 0 runs the neighbour-list sweeps (the 3-D path); tiles = 2
+ * (the 2-D default) runs the neighbour-list sweeps with each workgroup's
+ * neighbourhood rows staged in LDS (the same bits as tiles = 0).
+ * Takes effect at the next step, which rebuilds the layout.  No counterpart
+ * in the reference, which has one layout (its neighbour pairs). */
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y);
+/* Layout of the last build (host, 4 int64): the layout, the 256-row tiles
+ * on the LDS halo path, all tiles (both 0 outside layout 2) and the largest
+ * halo in rows.  Diagnostic; no counterpart in the reference. */
+int tsph_sim_layout(tsph_sim* sim, int64_t* out);
 /* Sweep timing with HIP events on the stepping stream.  Returns the summed
  * milliseconds of the density, wall and force+integrate sweeps and the number
  * of timed steps since the last reset; enable = 1/0 resets the sums and turns

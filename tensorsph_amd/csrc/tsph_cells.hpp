@@ -243,6 +243,34 @@ struct NbSlots {
       }
     }
   }
+  // f(j4, keep4) once per chunk of four entries (CH chunks per block): the
+  // chunk loads come first, then f evaluates the four entries together.
+  // Slots past the row's count, and the wall rows with_walls == false skips,
+  // hold the row itself with keep false.
+  template <int D, int CH = 1, typename F>
+  TSPH_D void visit_chunks(int64_t i, bool with_walls, F&& f) const {
+    const int4* p = chunks(i);
+    const int n = cnt[i];
+    const int4 self = make_int4((int)i, (int)i, (int)i, (int)i);
+    for (int k = 0; k < n; k += 4 * CH) {
+      int4 q[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) q[c] = k + 4 * c < n ? p[((k >> 2) + c) * 64] : self;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int32_t raw[4] = {q[c].x, q[c].y, q[c].z, q[c].w};
+        int32_t j4[4];
+        bool keep4[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = k + 4 * c + u < n;
+          j4[u] = in ? raw[u] : (int32_t)i;
+          keep4[u] = in && (with_walls || j4[u] < nf);
+        }
+        f(j4, keep4);
+      }
+    }
+  }
   // f(j, keep) for every entry, one chunk of four at a time in one
   // straight-line block: the chunk load and the gathers f makes for its four
   // entries are all issued before the arithmetic that needs them.  Slots past

@@ -29,6 +29,8 @@ template <> struct Sum<double> {
   double s = 0.0;
   TSPH_D void add(double x) { s += x; }
   TSPH_D void add_if(bool keep, double x) { s = keep ? s + x : s; }
+  // four terms in order, one after the other (the order of add)
+  TSPH_D void add4(double x0, double x1, double x2, double x3) { s = (((s + x0) + x1) + x2) + x3; }
   TSPH_D double get() const { return s; }
 };
 // Kahan summation: the running compensation c carries the low-order bits that
@@ -48,6 +50,10 @@ template <> struct Sum<float> {
     c = keep ? cn : c;
     s = keep ? t : s;
   }
+  // the four terms of one neighbour chunk summed pairwise, then one
+  // compensated addition: the rounding of a four-term partial is that of its
+  // terms, and the compensation carries every partial the running sum drops
+  TSPH_D void add4(float x0, float x1, float x2, float x3) { add((x0 + x1) + (x2 + x3)); }
   TSPH_D float get() const { return s; }
 };
 
@@ -124,6 +130,35 @@ template <> TSPH_D float cubic_g_r2<float>(float r2, const KernelConst<float>& k
     dshape = 0.0f;
   }
   return k.sigma_over_h * dshape * rinv;
+}
+
+// sqrt(r2) for a squared separation, to about 5e-14 relative: the FP32
+// hardware reciprocal square root (v_rsq_f32, relative error below 2e-7 after
+// rounding r2 to FP32) refined by one Newton step in FP64, which squares the
+// error.  Separations below 1e-15 return 0.  The FP32-compensated density
+// sweeps use it in place of the correctly rounded FP64 square root.
+TSPH_D double sqrt_r2_newton(double r2) {
+  const double y0 = (double)__builtin_amdgcn_rsqf((float)r2);
+  const double y1 = y0 * (1.5 - (0.5 * r2) * (y0 * y0));
+  return r2 > 1.0e-30 ? r2 * y1 : 0.0;
+}
+
+// W of the cubic spline for r known to FP64 accuracy, branch-free: the two
+// pieces in Horner form, selected by q (both are positive inside their
+// ranges).  Agrees with cubic_wg's W to FP64 rounding; the FP32-compensated
+// density sweeps use it.
+TSPH_D double cubic_w_select(double r, const KernelConst<double>& k) {
+  const double q = r * k.inv_h;
+  const double s1 = fma(q * q, fma(0.75, q, -1.5), 1.0);
+  const double t = 2.0 - q;
+  const double s2 = (0.25 * t) * (t * t);
+  return k.sigma * (q < 1.0 ? s1 : (q < 2.0 ? s2 : 0.0));
+}
+
+// Byte offset into an array with a 32-bit index, so the address is the
+// array's (uniform) base plus a zero-extended 32-bit offset.
+template <typename T> TSPH_D const T* at_bytes(const T* base, uint32_t bytes) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bytes);
 }
 
 // a / b; FP32 as a times the hardware reciprocal of b (v_rcp_f32, 1 ulp).

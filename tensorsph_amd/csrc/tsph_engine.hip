@@ -25,6 +25,8 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 #include <string.h>
+
+#include <algorithm>
 #include <vector>
 
 #include "tsph_cells.hpp"
@@ -32,6 +34,7 @@
 #include "tsph_internal.hpp"
 #include "tsph_wcsph.hpp"
 #include "tsph_engine.hpp"
+#include "tsph_halo.hpp"
 
 namespace tsph {
 
@@ -203,31 +206,83 @@ __global__ void __launch_bounds__(256) k_reduce_loads(const double* __restrict__
 
 
 // ------------------------------------------------------------------ step sweeps
-template <int D, typename S, typename NB>
+// HALO: the tile's halo rows (positions, masses) are staged in LDS and the
+// lists of halo positions are evaluated from there (2-D slot lists); a tile
+// without a halo reads the global arrays.  Same terms in the same order
+// either way.
+template <int D, typename S, typename NB, bool HALO = false>
 __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr, SimFlags* fl,
-                                                         int slot_in, int write_eos) {
+                                                         int slot_in, int write_eos, HaloView hv = {},
+                                                         NbHalo hl = {}) {
   if (fl->stale[slot_in]) return;
   const int64_t nb = gridDim.x;
-  const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+  const int64_t tile = xcd_block(blockIdx.x, nb);
+  const int64_t i = tile * blockDim.x + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) fl->maxes[MM_DISP] = 0;
+  int H = -1;
+  extern __shared__ __align__(16) unsigned char halo_lds[];
+  double* hx = reinterpret_cast<double*>(halo_lds);
+  S* hm = nullptr;
+  if constexpr (HALO) {
+    H = tile < hv.ntiles ? hv.cnt[tile].x : -1;
+    if (H >= 0) {
+      hm = reinterpret_cast<S*>(hx + (size_t)H * D);
+      const uint32_t* hr = hv.rows + tile * kHaloCap;
+      for (int s = threadIdx.x; s < H; s += blockDim.x) {
+        const uint32_t j = hr[s];
+#pragma unroll
+        for (int d = 0; d < D; ++d) hx[s * D + d] = a.pos[(int64_t)j * D + d];
+        hm[s] = a.mass[j];
+      }
+      __syncthreads();
+    }
+  }
   if (i >= a.n) return;
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   Sum<double> acc, kt;
-  nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
+  // one entry: its position and mass, fetched from the halo or the arrays
+  auto entry = [&](const double (&pj)[D], double mj, bool keep) {
     double r2 = 0.0;
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const double df = xi[d] - a.pos[j * D + d];
+      const double df = xi[d] - pj[d];
       r2 += df * df;
     }
     keep = keep && !(r2 > a.skip2d);
+    // FP32-compensated runs take r from the refined reciprocal square root
+    // (5e-14 relative) and W from the branch-free form; FP64 runs keep the
+    // correctly rounded square root and cubic_wg
     double w, g;
-    cubic_wg<double>(sqrt(r2), a.kd, w, g);
-    acc.add_if(keep, (double)a.mass[j] * w);
+    if constexpr (sizeof(S) == 4) {
+      w = cubic_w_select(sqrt_r2_newton(r2), a.kd);
+    } else {
+      cubic_wg<double>(sqrt(r2), a.kd, w, g);
+    }
+    acc.add_if(keep, mj * w);
     if (a.shepard) kt.add_if(keep, w);
-  });
+  };
+  if (HALO && H >= 0) {
+    hl.template visit_chunks<D>(i, [&](const int32_t* l4, const bool* keep4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double pj[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) pj[d] = hx[l4[u] * D + d];
+        entry(pj, (double)hm[l4[u]], keep4[u]);
+      }
+    });
+  } else {
+    nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
+      const uint32_t jb = (uint32_t)j;
+      const double* p = at_bytes(a.pos, jb * (uint32_t)(D * sizeof(double)));
+      double pj[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) pj[d] = p[d];
+      entry(pj, (double)*at_bytes(a.mass, jb * (uint32_t)sizeof(S)), keep);
+    });
+  }
   double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   if (a.shepard) {                 // Shepard renormalisation, droplet_impact.py:954-964
@@ -486,11 +541,21 @@ TSPH_D double wave_sum(double v) {
   return v;
 }
 
-template <int D, typename S, bool METERS, typename NB>
+// One halo row as the force sweep stages it: position, velocity and the
+// (m, p / rho^2, m / rho) record.
+template <int D, typename S> struct HaloRec {
+  double x[D];
+  double v[D];
+  Vec4<S> q;
+};
+
+template <int D, typename S, bool METERS, typename NB, bool HALO = false>
 __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S* __restrict__ acc_out,
-                                                       StepArgs st, SimFlags* fl, int slot_in, double half_skin) {
+                                                       StepArgs st, SimFlags* fl, int slot_in, double half_skin,
+                                                       HaloView hv = {}, NbHalo hl = {}) {
   const int64_t nb = gridDim.x;
-  const int64_t i = xcd_block(blockIdx.x, nb) * blockDim.x + threadIdx.x;
+  const int64_t tile = xcd_block(blockIdx.x, nb);
+  const int64_t i = tile * blockDim.x + threadIdx.x;
   if (fl->stale[slot_in]) {
     // binning no longer valid: carry the state through unchanged, keep the flag up
     if (i < a.n) {
@@ -503,6 +568,24 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     if (blockIdx.x == 0 && threadIdx.x == 0) fl->stale[slot_in ^ 1] = 1;
     return;
   }
+  int H = -1, Fh = 0;
+  extern __shared__ __align__(16) unsigned char halo_lds[];
+  HaloRec<D, S>* hrec = reinterpret_cast<HaloRec<D, S>*>(halo_lds);
+  if constexpr (HALO) {
+    if (tile < hv.ntiles) { const int2 c = hv.cnt[tile]; H = c.x; Fh = c.y; }
+    if (H >= 0) {
+      const uint32_t* hr = hv.rows + tile * kHaloCap;
+      for (int s = threadIdx.x; s < H; s += blockDim.x) {
+        const uint32_t j = hr[s];
+        HaloRec<D, S>& rr = hrec[s];
+#pragma unroll
+        for (int d = 0; d < D; ++d) { rr.x[d] = a.pos[(int64_t)j * D + d]; rr.v[d] = a.vel[(int64_t)j * D + d]; }
+        if constexpr (sizeof(S) == 4) rr.q = a.aux[j];
+        else rr.q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
+      }
+      __syncthreads();
+    }
+  }
   double xi[D] = {}, vi[D] = {};
   S acc[D];
   double mi = 0.0, rhoi = 0.0, pi_ = 0.0;
@@ -512,30 +595,68 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
     const S pri = a.pr2[i];
     Sum<S> ap[D], av[D];
-    // every list entry is evaluated; terms outside the support (and the
-    // viscous term of frictionless wall pairs) are left out of the sums
-    nbr.template visit_masked<D, (D == 3 ? 2 : 1)>(i, true, [&](int64_t j, bool keep) {
-      S df[D], r2 = S(0);
+    // every list entry is evaluated, four at a time; an entry outside the
+    // support has its gradient factor zeroed, which zeroes its pressure and
+    // viscous terms (a frictionless wall pair's viscous term is zero through
+    // its blend), so the sums hold the same terms the branching form adds.
+    // fetch(u, xj, vj, q, wall) gives entry u of the chunk, from the halo or
+    // from the arrays (32-bit byte offsets).
+    auto chunk = [&](const bool* keep4, auto&& fetch) {
+      S tp[D][4], tv[D][4];
 #pragma unroll
-      for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - a.pos[j * D + d]); r2 += df[d] * df[d]; }
-      keep = keep && !(r2 > a.skip2);
-      const S g = cubic_g_r2<S>(r2, a.k);
-      Vec4<S> q;                                  // m, p / rho^2, m / rho
-      if constexpr (sizeof(S) == 4) q = a.aux[j];
-      else q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
-      const S cp = -q.x * (pri + q.y);
+      for (int u = 0; u < 4; ++u) {
+        double pj[D], vj[D];
+        Vec4<S> q;                                  // m, p / rho^2, m / rho
+        bool wall;
+        fetch(u, pj, vj, q, wall);
+        S df[D], r2 = S(0);
 #pragma unroll
-      for (int d = 0; d < D; ++d) ap[d].add_if(keep, cp * (df[d] * g));
-      const S blend = j < a.n ? S(1) : a.friction;
-      S dg = S(0);
+        for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - pj[d]); r2 += df[d] * df[d]; }
+        const bool keep = keep4[u] && !(r2 > a.skip2);
+        S g = cubic_g_r2<S>(r2, a.k);
+        g = keep ? g : S(0);
+        const S cp = -q.x * (pri + q.y);
 #pragma unroll
-      for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
-      const S relu = -dg > S(0) ? -dg : S(0);
-      const S cv = -(a.nu2 * blend) * (q.z * quot<S>(relu, r2 + a.eta2));
-      const bool visc = keep && blend != S(0);
+        for (int d = 0; d < D; ++d) tp[d][u] = cp * (df[d] * g);
+        const S blend = wall ? a.friction : S(1);
+        S dg = S(0);
 #pragma unroll
-      for (int d = 0; d < D; ++d) av[d].add_if(visc, cv * (S)(vi[d] - a.vel[j * D + d]));
-    });
+        for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
+        const S relu = -dg > S(0) ? -dg : S(0);
+        const S cv = -(a.nu2 * blend) * (q.z * quot<S>(relu, r2 + a.eta2));
+#pragma unroll
+        for (int d = 0; d < D; ++d) tv[d][u] = cv * (S)(vi[d] - vj[d]);
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        ap[d].add4(tp[d][0], tp[d][1], tp[d][2], tp[d][3]);
+        av[d].add4(tv[d][0], tv[d][1], tv[d][2], tv[d][3]);
+      }
+    };
+    if (HALO && H >= 0) {
+      hl.template visit_chunks<D>(i, [&](const int32_t* l4, const bool* keep4) {
+        chunk(keep4, [&](int u, double (&pj)[D], double (&vj)[D], Vec4<S>& q, bool& wall) {
+          const HaloRec<D, S>& rr = hrec[l4[u]];
+#pragma unroll
+          for (int d = 0; d < D; ++d) { pj[d] = rr.x[d]; vj[d] = rr.v[d]; }
+          q = rr.q;
+          wall = l4[u] >= Fh;
+        });
+      });
+    } else {
+      nbr.template visit_chunks<D, (D == 3 ? 2 : 1)>(i, true, [&](const int32_t* j4, const bool* keep4) {
+        chunk(keep4, [&](int u, double (&pj)[D], double (&vj)[D], Vec4<S>& q, bool& wall) {
+          const uint32_t j = (uint32_t)j4[u];
+          const double* xp = at_bytes(a.pos, j * (uint32_t)(D * sizeof(double)));
+          const double* vp = at_bytes(a.vel, j * (uint32_t)(D * sizeof(double)));
+#pragma unroll
+          for (int d = 0; d < D; ++d) { pj[d] = xp[d]; vj[d] = vp[d]; }
+          if constexpr (sizeof(S) == 4) q = *at_bytes(a.aux, j * (uint32_t)sizeof(Vec4<S>));
+          else q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
+          wall = j >= (uint32_t)a.n;
+        });
+      });
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
     if (acc_out) {
@@ -780,6 +901,7 @@ template <int D>
 int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t* ext, const int64_t* stride,
                     hipStream_t stream) {
   const int64_t n = sim->n, nt = sim->nt;
+  sim->halo_ok = false;                   // halos name the rows of the lists they were built from
   double dense = 1.0;
   for (int d = 0; d < D; ++d) dense *= (double)ext[d];
   TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG,
@@ -803,6 +925,54 @@ int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t*
                                     sim->slot_max, sim->slot_K, nullptr, stream, &sim->slot_posf, extent);
 }
 template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*, const int64_t*, hipStream_t);
+
+// The halos of the 2-D slot lists just built (tiles == 2): one workgroup per
+// tile of 256 fluid rows; the largest halo sizes the sweeps' LDS.  A halo
+// holds at most kHaloLds bytes of staged rows (rec_bytes each), which keeps
+// the sweeps within the default dynamic LDS limit of a launch.
+constexpr size_t kHaloLds = 60 * 1024;
+int build_halos(tsph_sim* sim, size_t rec_bytes, hipStream_t stream) {
+  const int64_t n = sim->n;
+  sim->halo_ok = false;
+  const int64_t ntiles = (n + kHaloRows - 1) / kHaloRows;
+  sim->halo_tiles = ntiles;
+  if (ntiles == 0) return TSPH_OK;
+  const int64_t groups = (n + 63) / 64;
+  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)ntiles * kHaloCap));
+  TSPH_TRY(sim->halo_cnt.ensure(sizeof(int2) * (size_t)ntiles));
+  TSPH_TRY(sim->halo_idx.ensure(sizeof(uint16_t) * (size_t)groups * 64 * (size_t)sim->slot_K + 64));
+  TSPH_TRY(sim->halo_hmax.ensure(2 * sizeof(unsigned int)));
+  TSPH_CHECK_HIP(hipMemsetAsync(sim->halo_hmax.ptr, 0, 2 * sizeof(unsigned int), stream));
+  const int cap = (int)std::min<size_t>((size_t)kHaloCap, kHaloLds / rec_bytes);
+  k_halo_build2<<<(unsigned)ntiles, kHaloRows, 0, stream>>>(sim_cells(sim), sim_slots(sim), n, cap,
+                                                            sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(),
+                                                            sim->halo_idx.as<uint16_t>(),
+                                                            sim->halo_hmax.as<unsigned int>());
+  TSPH_CHECK_LAUNCH();
+  unsigned int hmax[2] = {0, 0};
+  TSPH_CHECK_HIP(hipMemcpyAsync(hmax, sim->halo_hmax.ptr, sizeof(hmax), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  sim->halo_max = (int)hmax[0];
+  sim->halo_used = (int64_t)hmax[1];
+  sim->halo_ok = true;
+  return TSPH_OK;
+}
+
+static inline HaloView sim_halo(const tsph_sim* sim) {
+  HaloView h{};
+  h.rows = sim->halo_rows.as<uint32_t>();
+  h.cnt = sim->halo_cnt.as<int2>();
+  h.idx = sim->halo_idx.as<uint16_t>();
+  h.ntiles = sim->halo_tiles;
+  return h;
+}
+static inline NbHalo sim_halo_lists(const tsph_sim* sim) {
+  NbHalo l{};
+  l.idx = sim->halo_idx.as<uint16_t>();
+  l.cnt = sim->slot_cnt.as<int32_t>();
+  l.K = sim->slot_K;
+  return l;
+}
 
 template <int D, typename S>
 int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
@@ -840,6 +1010,9 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
   sim->cur = other;
   const double* npos = sim->pos[sim->cur].as<double>();
   TSPH_TRY((build_sim_slots<D>(sim, npos, rc, cs.ext, cs.stride, stream)));
+  if constexpr (D == 2) {
+    if (sim->tiles == 2) TSPH_TRY(build_halos(sim, sizeof(HaloRec<D, S>), stream));
+  }
   // both ping-pong buffers carry the (static) wall rows
   if (nt > n) {
     TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[sim->cur ^ 1].as<double>() + n * D, npos + n * D,
@@ -915,7 +1088,7 @@ int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
   a.aux = nullptr;                       // the tile force sweep reads the arrays
   const NbSlots nbr = sim_slots(sim);
   const int r = (int)(k % kRing);
-  k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
+  k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1, HaloView{}, NbHalo{});
   TSPH_CHECK_LAUNCH();
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
@@ -936,19 +1109,31 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
+  // halo mode (2-D): the density and force sweeps stage their tiles' halos
+  constexpr bool kHalo2 = D == 2;
+  const bool halo = kHalo2 && sim->tiles == 2 && sim->halo_ok;
+  const HaloView hv = halo ? sim_halo(sim) : HaloView{};
+  const NbHalo hl = halo ? sim_halo_lists(sim) : NbHalo{};
+  const size_t halo_density_bytes = halo ? (size_t)sim->halo_max * (D * sizeof(double) + sizeof(S)) : 0;
+  const size_t halo_force_bytes = halo ? (size_t)sim->halo_max * sizeof(HaloRec<D, S>) : 0;
   if (sim->prm.op.diffusion_scale > 0.0) {
     if (sizeof(S) == 4) a.rho_lo = sim->rho_lo.as<S>();
     WcsphArgs<S> a0 = a;
     a0.rho = sim->rho_raw.as<S>();
-    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0);
+    if (halo)
+      k_step_density<D, S, NbSlots, kHalo2><<<gx, kBlock, halo_density_bytes, stream>>>(a0, nbr, fl, slot, 0, hv, hl);
+    else
+      k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0, hv, hl);
     if (sim->prm.op.diffusion_form == 1) {
       k_step_gradient<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, sim->grad.as<S>(), fl, slot);
       k_step_diffusion<D, S, 1, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, sim->grad.as<S>(), fl, slot);
     } else {
       k_step_diffusion<D, S, 0, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, nullptr, fl, slot);
     }
+  } else if (halo) {
+    k_step_density<D, S, NbSlots, kHalo2><<<gx, kBlock, halo_density_bytes, stream>>>(a, nbr, fl, slot, 1, hv, hl);
   } else {
-    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1);
+    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1, hv, hl);
   }
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
@@ -983,12 +1168,18 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
                                           fl, slot);
     TSPH_CHECK_LAUNCH();
   }
-  if (meters)
+  if (halo && meters)
+    k_step_force<D, S, true, NbSlots, kHalo2><<<gx, kBlock, halo_force_bytes, stream>>>(
+        a, nbr, sim->acc.as<S>(), st, fl, slot, 0.5 * sim->prm.skin, hv, hl);
+  else if (halo)
+    k_step_force<D, S, false, NbSlots, kHalo2><<<gx, kBlock, halo_force_bytes, stream>>>(
+        a, nbr, sim->acc.as<S>(), st, fl, slot, 0.5 * sim->prm.skin, hv, hl);
+  else if (meters)
     k_step_force<D, S, true, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
-                                                        0.5 * sim->prm.skin);
+                                                                0.5 * sim->prm.skin, hv, hl);
   else
     k_step_force<D, S, false, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
-                                                         0.5 * sim->prm.skin);
+                                                                 0.5 * sim->prm.skin, hv, hl);
   if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
   TSPH_CHECK_LAUNCH();
@@ -1038,7 +1229,7 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
   bool tiled = false;
   if constexpr (D == 2) {
-    if (sim->tiles) {
+    if (sim->tiles == 1) {
       TSPH_TRY((sim_queue_step_tiles<D, S>(sim, k, meters, stream)));
       const int64_t nblocks = sim->ntiles_f;   // partials are indexed by tile
       if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), nblocks, sim->sums_out.as<double>(), fl, slot);
@@ -1113,7 +1304,7 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
   auto rebuild = [&]() -> int {
     sim->rebuild_forced = false;       // a counted build re-anchors every row
     if constexpr (D == 2) {
-      if (sim->tiles) return sim_rebuild_tiles<D, S>(sim, stream);
+      if (sim->tiles == 1) return sim_rebuild_tiles<D, S>(sim, stream);
     }
     return sim_rebuild<D, S>(sim, stream);
   };
@@ -1253,11 +1444,10 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
   k_init_time<<<1, 1, 0, stream>>>(sim->flags.as<SimFlags>(), sim_dtctl(sim));
-  // obstacle loads and density diffusion run on the slot-list sweeps; other
-  // 2-D runs default to the LDS cell tiles
-  if (sim->dim == 2 && !sim->obstacle && !(sim->prm.op.diffusion_scale > 0.0)) {
-    sim->tiles = 1; sim->tile_dim[0] = 8; sim->tile_dim[1] = 4;
-  }
+  // 2-D runs default to the slot lists with LDS halos (measured fastest on
+  // MI355X at 1.6M fluid rows in FP32 and FP64); the cell tiles stay selectable
+  sim->tile_dim[0] = 8; sim->tile_dim[1] = 4;
+  if (sim->dim == 2) sim->tiles = 2;
   *out = sim;
   return TSPH_OK;
 }
@@ -1320,6 +1510,15 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
   return TSPH_OK;
 }
 
+int tsph_sim_layout(tsph_sim* sim, int64_t* out) {
+  TSPH_REQUIRE(sim && out, TSPH_ERR_ARG, "bad arguments");
+  out[0] = sim->tiles;
+  out[1] = sim->halo_ok ? sim->halo_used : 0;
+  out[2] = sim->halo_ok ? sim->halo_tiles : 0;
+  out[3] = sim->halo_ok ? sim->halo_max : 0;
+  return TSPH_OK;
+}
+
 int tsph_sim_meters(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows) {
   TSPH_REQUIRE(sim && nrows, TSPH_ERR_ARG, "bad arguments");
   const int64_t width = kMeterWidth;
@@ -1346,15 +1545,17 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
 
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
-  TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
-  TSPH_REQUIRE(tiles == 0 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the slot-list sweeps");
-  TSPH_REQUIRE(tiles == 0 || !(sim->prm.op.diffusion_scale > 0.0), TSPH_ERR_ARG,
+  TSPH_REQUIRE(tiles >= 0 && tiles <= 2, TSPH_ERR_ARG, "layout must be 0 (lists), 1 (cell tiles) or 2 (halos)");
+  TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile and halo sweeps are 2-D");
+  TSPH_REQUIRE(tiles != 1 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the slot-list sweeps");
+  TSPH_REQUIRE(tiles != 1 || !(sim->prm.op.diffusion_scale > 0.0), TSPH_ERR_ARG,
                "density diffusion runs on the slot-list sweeps");
-  TSPH_REQUIRE(tiles == 0 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
+  TSPH_REQUIRE(tiles != 1 || (tile_x >= 1 && tile_y >= 1 && (tile_x + 2) * (tile_y + 2) <= 18 * 6 &&
                               tile_x * tile_y <= 64),
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
   sim->tiles = tiles;
-  if (tiles) { sim->tile_dim[0] = tile_x; sim->tile_dim[1] = tile_y; }
+  sim->halo_ok = false;
+  if (tiles == 1) { sim->tile_dim[0] = tile_x; sim->tile_dim[1] = tile_y; }
   sim->built = false;   // next step rebuilds with the new layout
   return TSPH_OK;
 }
@@ -1382,7 +1583,7 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
   const double r2 = radius * radius;
   TSPH_REQUIRE(radius <= 2.0 * sim->prm.op.h + sim->prm.skin, TSPH_ERR_ARG,
                "radius exceeds the neighbour-list build radius");
-  if (sim->tiles) {
+  if (sim->tiles == 1) {
     TSPH_TRY(tile_count_pairs(sim, radius, cnt, stream));
   } else {
     if (sim->dim == 2)
@@ -1580,7 +1781,7 @@ int rebuild_keep_volumes(tsph_sim* sim, hipStream_t stream, bool* rebuilt) {
     TSPH_CHECK_LAUNCH();
     bool done = false;
     if constexpr (D == 2) {
-      if (sim->tiles) { TSPH_TRY((sim_rebuild_tiles<D, S>(sim, stream))); done = true; }
+      if (sim->tiles == 1) { TSPH_TRY((sim_rebuild_tiles<D, S>(sim, stream))); done = true; }
     }
     if (!done) TSPH_TRY((sim_rebuild<D, S>(sim, stream)));
     sim->rebuild_pending = false;

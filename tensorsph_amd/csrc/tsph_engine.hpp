@@ -95,8 +95,16 @@ struct tsph_sim {
   int64_t steps_done = 0, rebuilds = 0, escaped_total = 0;
   bool built = false;
   bool finished = false;                  // an adaptive run has done its final sample
+  // halo mode (2-D slot lists, tiles == 2): per tile of 256 fluid rows, the
+  // rows its lists name (staged in LDS by the density and force sweeps) and
+  // the lists as 16-bit halo positions
+  tsph::DevBuf halo_rows, halo_cnt, halo_idx, halo_hmax;
+  int64_t halo_tiles = 0;        // tiles of the build
+  int64_t halo_used = 0;         // tiles of the build on the halo path
+  int halo_max = 0;              // the largest halo of the build (LDS rows to reserve)
+  bool halo_ok = false;          // the halo of the current build is in place
   // cell-tile mode (2-D): rows sorted tile-major, dense per-cell row ranges
-  int tiles = 0;                 // 1: LDS cell-tile force sweep, 0: slot-list sweeps
+  int tiles = 0;                 // 1: LDS cell-tile force sweep, 0: slot-list sweeps, 2: slot lists + halos
   int tile_dim[3] = {4, 4, 1};
   int64_t ext[3] = {1, 1, 1};
   int64_t ntile[3] = {1, 1, 1};

@@ -171,6 +171,14 @@ class WcsphSimulation:
         return {"steps": c[0], "rebuilds": c[1], "escaped": c[2], "cells": c[3],
                 "meter_rows": c[4], "finished": bool(c[5]), "peak_rho": pk[0], "peak_speed": pk[1]}
 
+    def layout(self) -> dict:
+        """Layout of the last build: "layout" (0 lists, 1 cell tiles, 2 lists
+        with LDS halos), and for layout 2 the tiles of 256 rows on the halo
+        path, all tiles and the largest halo in rows."""
+        c = (_i64 * 4)()
+        _lib.check(_lib.lib().tsph_sim_layout(self.handle, c))
+        return {"layout": c[0], "halo_tiles": c[1], "tiles": c[2], "halo_max": c[3]}
+
     def time_state(self) -> dict:
         """The coming step's time state: simulated time at its start, the
         step-size variable, its own size, the previous step's size."""
@@ -223,10 +231,15 @@ def _sweep_times(self):
     return [ms[0], ms[1], ms[2]], steps.value
 
 
-def _configure(self, tiles: bool = True, tile=(8, 4)) -> None:
-    """Choose the sweep layout: LDS cell tiles (2-D) or CSR neighbour lists."""
-    _lib.check(_lib.lib().tsph_sim_configure(self.handle, 1 if tiles else 0, int(tile[0]),
-                                             int(tile[1])))
+def _configure(self, tiles: bool = True, tile=(8, 4), halo: bool = False) -> None:
+    """Choose the sweep layout (2-D): LDS cell tiles, or the per-row slot
+    lists (tiles False), evaluated from the global arrays or, with halo True,
+    from each workgroup's neighbourhood rows staged in LDS.  Every layout
+    gives the same bits for the same lists."""
+    if halo and tiles:
+        raise ValueError("the halo layout runs on the slot lists: pass tiles=False")
+    mode = 2 if halo else (1 if tiles else 0)
+    _lib.check(_lib.lib().tsph_sim_configure(self.handle, mode, int(tile[0]), int(tile[1])))
 
 
 def _count_pairs(self, radius: float) -> int:

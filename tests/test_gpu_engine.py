@@ -16,7 +16,18 @@ from tests.gpu_util import dev, n, scaled_err, t
 pytestmark = pytest.mark.gpu
 
 
-def _impact_sim(precision, golden, layout="tiles"):
+def _use(sim, layout):
+    """"default": the library's choice; "tiles": 8 x 4 cell tiles; "lists",
+    "halo"; or a tile shape."""
+    if layout in ("lists", "halo"):
+        sim.configure(tiles=False, halo=layout == "halo")
+    elif layout == "tiles":
+        sim.configure(tiles=True, tile=(8, 4))
+    elif layout != "default":
+        sim.configure(tiles=True, tile=layout)
+
+
+def _impact_sim(precision, golden, layout="default"):
     from tensorsph_amd.cases import impact_state, domain_top
     from tensorsph_amd.engine import WcsphSimulation
     g = golden("run2d")
@@ -29,14 +40,11 @@ def _impact_sim(precision, golden, layout="tiles"):
                           viscosity=nu, dt=dt, wall_mass=float(mass[0]), skin=0.25 * h,
                           lo=[0.0, 0.0], hi=[cfg.pool_width, domain_top(cfg)],
                           tol=0.25 * cfg.dx, precision=precision)
-    if layout == "lists":
-        sim.configure(tiles=False)
-    elif layout != "tiles":
-        sim.configure(tiles=True, tile=layout)
+    _use(sim, layout)
     return sim, g, int(steps), mass
 
 
-@pytest.mark.parametrize("layout", ["tiles", "lists", (4, 4), (16, 4)])
+@pytest.mark.parametrize("layout", ["default", "tiles", "lists", "halo", (4, 4), (16, 4)])
 def test_stepper_fp64_tracks_reference_run(golden, layout):
     from tensorsph_amd.engine import METER_KEYS
     sim, g, steps, mass = _impact_sim("fp64", golden, layout)
@@ -60,7 +68,7 @@ def test_stepper_fp64_tracks_reference_run(golden, layout):
     assert stats["steps"] == steps and stats["rebuilds"] >= 1
 
 
-@pytest.mark.parametrize("layout", ["tiles", "lists"])
+@pytest.mark.parametrize("layout", ["tiles", "lists", "halo"])
 def test_stepper_matches_oracle_loop_on_dam_break(layout):
     from tensorsph_amd.config import DamBreakConfig
     from tensorsph_amd.cases import make_column, wall_lattice, lattice_mass_unit
@@ -75,8 +83,7 @@ def test_stepper_matches_oracle_loop_on_dam_break(layout):
     sim = WcsphSimulation(pos, np.zeros_like(pos), mass, wall, sound_speed=cfg.sound_speed,
                           viscosity=cfg.viscosity, lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()],
                           tol=0.25 * cfg.dx, precision="fp64", **kw)
-    if layout == "lists":
-        sim.configure(tiles=False)
+    _use(sim, layout)
     ref = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, c0=cfg.sound_speed, nu=cfg.viscosity,
                         lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx, **kw)
     steps = 60
@@ -89,7 +96,7 @@ def test_stepper_matches_oracle_loop_on_dam_break(layout):
     assert sim.stats()["rebuilds"] == ref.rebuilds
 
 
-@pytest.mark.parametrize("layout", ["tiles", "lists"])
+@pytest.mark.parametrize("layout", ["tiles", "lists", "halo"])
 def test_stepper_fp32_compensated_vs_fp64_and_determinism(golden, layout):
     a, g, steps, _ = _impact_sim("fp32", golden, layout)
     b, _, _, _ = _impact_sim("fp32", golden, layout)
@@ -122,15 +129,14 @@ def test_tiles_and_lists_agree_on_dam_break_fp32():
                               viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=lattice_mass_unit(cfg),
                               skin=0.25 * h, lo=[0, 0], hi=[cfg.pool_width, cfg.tank_top()],
                               tol=0.25 * cfg.dx, precision="fp32")
-        if layout == "lists":
-            sim.configure(tiles=False)
+        _use(sim, layout)
         sim.step(30)
         out.append(sim.read())
     assert scaled_err(n(out[0]["velocity"]), n(out[1]["velocity"])) < 1e-5
     assert np.abs(n(out[0]["position"]) - n(out[1]["position"])).max() < 1e-8
 
 
-@pytest.mark.parametrize("layout", ["tiles", "lists"])
+@pytest.mark.parametrize("layout", ["tiles", "lists", "halo"])
 def test_pair_count_matches_oracle(golden, layout):
     """Regression: the pair count walks the structure of the active layout."""
     sim, g, steps, mass = _impact_sim("fp64", golden, layout)
@@ -151,8 +157,9 @@ def test_pair_count_matches_oracle(golden, layout):
 @pytest.mark.parametrize("form", ["molteni", "antuono"])
 def test_stepper_with_density_diffusion_matches_oracle_loop(form):
     """δ-diffusion (density_diffusion.py:78-197) in the resident 2-D loop runs on
-    the cell-walk sweeps (the LDS tiles carry no diffusion term and refuse it):
-    FP64 positions within 1e-10 m of the oracle loop after 40 steps."""
+    the slot-list sweeps, by default with LDS halos (the LDS cell tiles carry
+    no diffusion term and refuse it): FP64 positions within 1e-10 m of the
+    oracle loop after 40 steps."""
     from tensorsph_amd.config import DamBreakConfig
     from tensorsph_amd.cases import make_column, wall_lattice, lattice_mass_unit
     from tensorsph_amd.engine import WcsphSimulation

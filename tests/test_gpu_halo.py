@@ -1,0 +1,134 @@
+"""GPU checks of the 2-D halo layout (tsph_sim_configure layout 2): the
+slot-list density and force sweeps evaluated from each 256-row workgroup's
+neighbourhood rows staged in LDS.  The lists keep their order and name the
+same rows, so every run must give the same bits as the plain slot-list
+layout: positions, velocities, densities, pressures, accelerations and the
+meter rows, across list rebuilds, with density diffusion, with the position
+corrections, and on a spray case whose sparse tiles fall back to the global
+arrays.
+"""
+import numpy as np
+import pytest
+
+from tests.gpu_util import n
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("position", "velocity", "density", "pressure", "acceleration")
+
+
+def _dam(dx, precision, diffusion=0.0, skin_factor=0.25):
+    from tensorsph_amd.cases import lattice_mass_unit, make_column, wall_lattice
+    from tensorsph_amd.dam_break import DamBreakConfig
+    from tensorsph_amd.engine import WcsphSimulation
+    cfg = DamBreakConfig(dx=dx, tank_height=0.6, column_width=0.3, column_height=0.4,
+                         pool_width=0.8, sound_speed=20.0, dt=0.3 * 1.3 * dx / 20.0, viscosity=1e-3)
+    pos, mass = make_column(cfg)
+    wall = wall_lattice(cfg, top=cfg.tank_top())
+    h = cfg.h_factor * cfg.dx
+
+    def make():
+        return WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+                               sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+                               viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=lattice_mass_unit(cfg),
+                               skin=skin_factor * h, lo=[0.0, 0.0], hi=[cfg.pool_width, cfg.tank_top()],
+                               tol=0.25 * cfg.dx, precision=precision, density_diffusion=diffusion)
+    return cfg, make
+
+
+def _same(a, b):
+    for k in KEYS:
+        assert np.array_equal(n(a[k]), n(b[k])), k
+
+
+@pytest.mark.parametrize("precision,diffusion", [("fp64", 0.0), ("fp32", 0.0), ("fp32", 0.1),
+                                                 ("fp64", 0.1)])
+def test_halo_gives_the_bits_of_the_lists_on_dam_break(precision, diffusion):
+    cfg, make = _dam(0.003, precision, diffusion, skin_factor=0.1)
+    out, meters = [], []
+    for halo in (False, True):
+        sim = make()
+        try:
+            sim.configure(tiles=False, halo=halo)
+            sim.step(120, meter_every=10)
+            out.append(sim.read())
+            meters.append(sim.meters())
+            lay = sim.layout()
+            stats = sim.stats()
+        finally:
+            sim.close()
+        if halo:
+            assert lay["layout"] == 2 and lay["tiles"] > 40
+            assert lay["halo_tiles"] == lay["tiles"]          # a dense column: every tile stages
+            assert 0 < lay["halo_max"] <= 1024
+            assert stats["rebuilds"] >= 3
+        else:
+            assert lay == {"layout": 0, "halo_tiles": 0, "tiles": 0, "halo_max": 0}
+    _same(out[0], out[1])
+    assert np.array_equal(meters[0], meters[1])
+
+
+def test_halo_on_spray_falls_back_per_tile():
+    """Droplet impact at late time: splash tiles whose neighbourhoods are wide
+    take the global path, the rest stage halos; bits as the lists."""
+    from tensorsph_amd.cases import domain_top, impact_state
+    from tensorsph_amd.engine import WcsphSimulation
+    cfg, pos, mass, wall = impact_state(dx=0.004, pool_width=0.4, pool_depth=0.1, drop_radius=0.04,
+                                        drop_center_y=0.16, viscosity=1e-3, dt=1.2e-4)
+    vel = np.zeros_like(pos)
+    drop = pos[:, 1] > cfg.pool_depth + 0.5 * cfg.dx
+    vel[drop, 1] = -3.0
+    h = cfg.h_factor * cfg.dx
+    out = []
+    for halo in (False, True):
+        sim = WcsphSimulation(pos, vel, mass, wall, h=h, rho0=cfg.rho0, sound_speed=cfg.sound_speed,
+                              gamma=cfg.gamma, gravity=cfg.gravity, viscosity=cfg.viscosity,
+                              dt=cfg.dt, wall_mass=float(mass[0]), skin=0.25 * h, lo=[0.0, 0.0],
+                              hi=[cfg.pool_width, domain_top(cfg)], tol=0.25 * cfg.dx,
+                              precision="fp32")
+        try:
+            sim.configure(tiles=False, halo=halo)
+            sim.step(300)
+            out.append(sim.read())
+            lay = sim.layout()
+        finally:
+            sim.close()
+        if halo:
+            assert 0 < lay["halo_tiles"] <= lay["tiles"]
+    _same(out[0], out[1])
+
+
+def test_halo_with_position_corrections():
+    cfg, make = _dam(0.004, "fp64", skin_factor=0.05)
+    out = []
+    for halo in (False, True):
+        sim = make()
+        try:
+            sim.configure(tiles=False, halo=halo)
+            for k in range(12):
+                sim.step(5)
+                if k % 2:
+                    sim.shift(cfg.dt, v_ref=0.5)
+                else:
+                    sim.tvc(max_shift=0.1 * cfg.dx, wall_volume=cfg.dx ** 2)
+            sim.step(5)
+            out.append(sim.read())
+        finally:
+            sim.close()
+    _same(out[0], out[1])
+
+
+def test_halo_layout_arguments():
+    cfg, make = _dam(0.02, "fp64")
+    sim = make()
+    try:
+        with pytest.raises(ValueError):
+            sim.configure(tiles=True, halo=True)
+        sim.configure(tiles=False, halo=True)
+        sim.step(2)
+        assert sim.layout()["layout"] == 2
+        sim.configure(tiles=True)
+        sim.step(2)
+        assert sim.layout()["layout"] == 1 and sim.layout()["tiles"] == 0
+    finally:
+        sim.close()
