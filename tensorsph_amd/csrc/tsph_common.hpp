@@ -106,6 +106,24 @@ template <typename S> TSPH_D void cubic_wg(S r, const KernelConst<S>& k, S& w, S
   g = k.sigma_over_h * dshape / rc;
 }
 
+// cubic_wg's W and g, the same expressions, with the pieces selected instead
+// of branched on: a pair sweep's entries then form straight-line code whose
+// loads can all be issued ahead of the arithmetic.
+template <typename S> TSPH_D void cubic_wg_select(S r, const KernelConst<S>& k, S& w, S& g) {
+  const S q = r * k.inv_h;
+  const S shape1 = S(1) - S(1.5) * (q * q) + S(0.75) * (q * q * q);
+  const S dshape1 = S(-3) * q + S(2.25) * (q * q);
+  const S t = S(2) - q;
+  const S shape2 = S(0.25) * (t * t * t);
+  const S dshape2 = S(-0.75) * (t * t);
+  const bool in1 = q < S(1), in2 = q < S(2);
+  const S shape = in1 ? shape1 : (in2 ? shape2 : S(0));
+  const S dshape = in1 ? dshape1 : (in2 ? dshape2 : S(0));
+  w = k.sigma * (shape > S(0) ? shape : S(0));
+  const S rc = r > S(1e-12) ? r : S(1e-12);
+  g = k.sigma_over_h * dshape / rc;
+}
+
 // Gradient factor g of the cubic spline (grad_i W = g (x_i - x_j)) from the
 // squared separation.  FP64 is cubic_wg's exact square root and quotient.
 // FP32 takes r and 1/r from one hardware reciprocal square root (v_rsq_f32,
@@ -114,21 +132,16 @@ template <typename S> TSPH_D void cubic_wg(S r, const KernelConst<S>& k, S& w, S
 template <typename S> TSPH_D S cubic_g_r2(S r2, const KernelConst<S>& k);
 template <> TSPH_D double cubic_g_r2<double>(double r2, const KernelConst<double>& k) {
   double w, g;
-  cubic_wg<double>(sqrt(r2), k, w, g);
+  cubic_wg_select<double>(sqrt(r2), k, w, g);
   return g;
 }
 template <> TSPH_D float cubic_g_r2<float>(float r2, const KernelConst<float>& k) {
   const float rinv = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 1e12f;
   const float q = (r2 * rinv) * k.inv_h;
-  float dshape;
-  if (q < 1.0f) {
-    dshape = -3.0f * q + 2.25f * (q * q);
-  } else if (q < 2.0f) {
-    const float t = 2.0f - q;
-    dshape = -0.75f * (t * t);
-  } else {
-    dshape = 0.0f;
-  }
+  const float d1 = -3.0f * q + 2.25f * (q * q);
+  const float t = 2.0f - q;
+  const float d2 = -0.75f * (t * t);
+  const float dshape = q < 1.0f ? d1 : (q < 2.0f ? d2 : 0.0f);
   return k.sigma_over_h * dshape * rinv;
 }
 

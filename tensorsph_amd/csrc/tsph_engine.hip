@@ -36,6 +36,10 @@
 #include "tsph_engine.hpp"
 #include "tsph_halo.hpp"
 
+#ifndef TSPH_HALO_CH
+#define TSPH_HALO_CH 1
+#endif
+
 namespace tsph {
 
 TSPH_D double ord_dec(unsigned long long u) {
@@ -48,10 +52,53 @@ TSPH_D double ord_dec(unsigned long long u) {
 // dt = next_dt(dt, max |v| at the step's start) (droplet_impact.py:536-552),
 // the next step_dt = min(dt, target - elapsed), and the step that starts at the
 // target (within tol) is the final sample, integrated with step_dt = 0.
-__global__ void k_pack_record(SimFlags* fl, const double* __restrict__ sums, int slot_in,
-                              int with_sums, StepRecord* __restrict__ rec, const double* __restrict__ loads,
-                              const double* __restrict__ clamp, DtCtl ctl) {
+// The force sweep's per-block records (blk, nblk of them; NULL when it used
+// atomics) are reduced here first: the displacement, speed and density
+// maxima and the boundary hits of the step.  A skipped (stale) step wrote
+// none.  kPackThreads threads.
+constexpr int kPackThreads = 256;
+__global__ void __launch_bounds__(kPackThreads) k_pack_record(
+    SimFlags* fl, const double* __restrict__ sums, int slot_in, int with_sums, StepRecord* __restrict__ rec,
+    const double* __restrict__ loads, const double* __restrict__ clamp,
+    const unsigned long long* __restrict__ blk, int64_t nblk, DtCtl ctl) {
   const int t = threadIdx.x;
+  const bool fresh = blk && !fl->stale[slot_in];
+  unsigned long long key[3] = {0ull, 0ull, 0ull}, hits = 0ull;
+  if (fresh) {
+    for (int64_t b = t; b < nblk; b += kPackThreads) {
+      const ulonglong2 a = reinterpret_cast<const ulonglong2*>(blk)[2 * b];
+      const ulonglong2 c = reinterpret_cast<const ulonglong2*>(blk)[2 * b + 1];
+      key[0] = a.x > key[0] ? a.x : key[0];
+      key[1] = a.y > key[1] ? a.y : key[1];
+      key[2] = c.x > key[2] ? c.x : key[2];
+      hits += c.y;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const unsigned long long y = __shfl_xor(key[q], o, 64);
+      key[q] = y > key[q] ? y : key[q];
+    }
+    hits += __shfl_xor(hits, o, 64);
+  }
+  __shared__ unsigned long long part[4][kPackThreads / 64];
+  if ((t & 63) == 0) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) part[q][t >> 6] = key[q];
+    part[3][t >> 6] = hits;
+  }
+  __syncthreads();
+  if (t == 0 && fresh) {
+    for (int w = 1; w < kPackThreads / 64; ++w) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) part[q][0] = part[q][w] > part[q][0] ? part[q][w] : part[q][0];
+      part[3][0] += part[3][w];
+    }
+    if (part[0][0] > fl->maxes[MM_DISP]) fl->maxes[MM_DISP] = part[0][0];
+    fl->escaped += part[3][0];
+  }
+  __syncthreads();
   if (t == 0) { rec->stale_in = fl->stale[slot_in]; rec->stale_out = fl->stale[slot_in ^ 1]; rec->escaped = fl->escaped; }
   if (t < MM_NMAX && t != MM_PEAK_SPEED && t != MM_PEAK_RHO) rec->maxes[t] = fl->maxes[t];
   if (t < MN_NMIN) rec->mins[t] = fl->mins[t];
@@ -65,9 +112,14 @@ __global__ void k_pack_record(SimFlags* fl, const double* __restrict__ sums, int
     // the step's largest speed and density, read and cleared in one atomic
     // each (a plain scalar load followed by an independent store of 0 is not
     // ordered against it); the host keeps the run's peaks
-    const unsigned long long kv = atomicExch(&fl->maxes[MM_PEAK_SPEED], 0ull);
+    unsigned long long kv = atomicExch(&fl->maxes[MM_PEAK_SPEED], 0ull);
+    unsigned long long kr = atomicExch(&fl->maxes[MM_PEAK_RHO], 0ull);
+    if (fresh) {
+      kv = part[1][0] > kv ? part[1][0] : kv;
+      kr = part[2][0] > kr ? part[2][0] : kr;
+    }
     rec->maxes[MM_PEAK_SPEED] = kv;
-    rec->maxes[MM_PEAK_RHO] = atomicExch(&fl->maxes[MM_PEAK_RHO], 0ull);
+    rec->maxes[MM_PEAK_RHO] = kr;
     const double vmax = ord_dec(kv);
     if (!fl->stale[slot_in] && !fl->final) {
       fl->elapsed += fl->step_dt;
@@ -227,12 +279,28 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
     H = tile < hv.ntiles ? hv.cnt[tile].x : -1;
     if (H >= 0) {
       hm = reinterpret_cast<S*>(hx + (size_t)H * D);
+      // every load of the staging is issued before the first store (up to
+      // kHaloCap / kBlock rows per thread)
       const uint32_t* hr = hv.rows + tile * kHaloCap;
-      for (int s = threadIdx.x; s < H; s += blockDim.x) {
-        const uint32_t j = hr[s];
+      constexpr int QS = kHaloCap / kBlock;
+      double px[QS][D];
+      S pm[QS];
 #pragma unroll
-        for (int d = 0; d < D; ++d) hx[s * D + d] = a.pos[(int64_t)j * D + d];
-        hm[s] = a.mass[j];
+      for (int q = 0; q < QS; ++q) {
+        const int s = threadIdx.x + q * kBlock;
+        const uint32_t j = hr[s < H ? s : 0];
+#pragma unroll
+        for (int d = 0; d < D; ++d) px[q][d] = a.pos[(int64_t)j * D + d];
+        pm[q] = a.mass[j];
+      }
+#pragma unroll
+      for (int q = 0; q < QS; ++q) {
+        const int s = threadIdx.x + q * kBlock;
+        if (s < H) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) hx[s * D + d] = px[q][d];
+          hm[s] = pm[q];
+        }
       }
       __syncthreads();
     }
@@ -258,13 +326,13 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
     if constexpr (sizeof(S) == 4) {
       w = cubic_w_select(sqrt_r2_newton(r2), a.kd);
     } else {
-      cubic_wg<double>(sqrt(r2), a.kd, w, g);
+      cubic_wg_select<double>(sqrt(r2), a.kd, w, g);
     }
     acc.add_if(keep, mj * w);
     if (a.shepard) kt.add_if(keep, w);
   };
   if (HALO && H >= 0) {
-    hl.template visit_chunks<D>(i, [&](const int32_t* l4, const bool* keep4) {
+    hl.template visit_chunks<D, TSPH_HALO_CH>(i, [&](const int32_t* l4, const bool* keep4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         double pj[D];
@@ -574,14 +642,34 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
   if constexpr (HALO) {
     if (tile < hv.ntiles) { const int2 c = hv.cnt[tile]; H = c.x; Fh = c.y; }
     if (H >= 0) {
+      // every load of the staging is issued before the first store (up to
+      // kHaloCap / kBlock rows per thread)
       const uint32_t* hr = hv.rows + tile * kHaloCap;
-      for (int s = threadIdx.x; s < H; s += blockDim.x) {
-        const uint32_t j = hr[s];
-        HaloRec<D, S>& rr = hrec[s];
+      constexpr int QS = kHaloCap / kBlock;
+      double rx[QS][D], rv[QS][D];
+      S r0[QS], r1[QS], r2[QS];
 #pragma unroll
-        for (int d = 0; d < D; ++d) { rr.x[d] = a.pos[(int64_t)j * D + d]; rr.v[d] = a.vel[(int64_t)j * D + d]; }
-        if constexpr (sizeof(S) == 4) rr.q = a.aux[j];
-        else rr.q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
+      for (int q = 0; q < QS; ++q) {
+        const int s = threadIdx.x + q * kBlock;
+        const uint32_t j = hr[s < H ? s : 0];
+#pragma unroll
+        for (int d = 0; d < D; ++d) { rx[q][d] = a.pos[(int64_t)j * D + d]; rv[q][d] = a.vel[(int64_t)j * D + d]; }
+        if constexpr (sizeof(S) == 4) {
+          const Vec4<S> w = a.aux[j];
+          r0[q] = w.x; r1[q] = w.y; r2[q] = w.z;
+        } else {
+          r0[q] = a.mass[j]; r1[q] = a.pr2[j]; r2[q] = a.vol[j];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < QS; ++q) {
+        const int s = threadIdx.x + q * kBlock;
+        if (s < H) {
+          HaloRec<D, S>& rr = hrec[s];
+#pragma unroll
+          for (int d = 0; d < D; ++d) { rr.x[d] = rx[q][d]; rr.v[d] = rv[q][d]; }
+          rr.q = Vec4<S>{r0[q], r1[q], r2[q], S(0)};
+        }
       }
       __syncthreads();
     }
@@ -603,12 +691,18 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     // from the arrays (32-bit byte offsets).
     auto chunk = [&](const bool* keep4, auto&& fetch) {
       S tp[D][4], tv[D][4];
+      // the four entries' loads first, then their arithmetic
+      double pj4[4][D], vj4[4][D];
+      Vec4<S> q4[4];                                // m, p / rho^2, m / rho
+      bool wall4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) fetch(u, pj4[u], vj4[u], q4[u], wall4[u]);
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        double pj[D], vj[D];
-        Vec4<S> q;                                  // m, p / rho^2, m / rho
-        bool wall;
-        fetch(u, pj, vj, q, wall);
+        const double* pj = pj4[u];
+        const double* vj = vj4[u];
+        const Vec4<S>& q = q4[u];
+        const bool wall = wall4[u];
         S df[D], r2 = S(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - pj[d]); r2 += df[d] * df[d]; }
@@ -634,7 +728,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       }
     };
     if (HALO && H >= 0) {
-      hl.template visit_chunks<D>(i, [&](const int32_t* l4, const bool* keep4) {
+      hl.template visit_chunks<D, TSPH_HALO_CH>(i, [&](const int32_t* l4, const bool* keep4) {
         chunk(keep4, [&](int u, double (&pj)[D], double (&vj)[D], Vec4<S>& q, bool& wall) {
           const HaloRec<D, S>& rr = hrec[l4[u]];
 #pragma unroll
@@ -713,14 +807,21 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       bd = fmax(bd, bmax[0][w]); bv = fmax(bv, bmax[1][w]); br = fmax(br, bmax[2][w]); bh += bhits[w];
     }
     const double bspd = sqrt(bv);
-    atomicMax(&fl->maxes[MM_DISP], ord_enc(bd));
-    atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(bspd));
-    atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(br));
+    if (st.block_max) {
+      // plain stores: thousands of blocks contending for one cache line
+      // with atomics cost more than the sweep itself
+      unsigned long long* o = st.block_max + 4 * (int64_t)blockIdx.x;
+      o[0] = ord_enc(bd); o[1] = ord_enc(bspd); o[2] = ord_enc(br); o[3] = bh;
+    } else {
+      atomicMax(&fl->maxes[MM_DISP], ord_enc(bd));
+      atomicMax(&fl->maxes[MM_PEAK_SPEED], ord_enc(bspd));
+      atomicMax(&fl->maxes[MM_PEAK_RHO], ord_enc(br));
+      if (bh) atomicAdd(&fl->escaped, bh);
+    }
     if (METERS) {
       atomicMax(&fl->maxes[MM_SPEED], ord_enc(bspd));
       atomicMax(&fl->maxes[MM_RHO_MAX], ord_enc(br));
     }
-    if (bh) atomicAdd(&fl->escaped, bh);
     if (bd > half_skin) fl->stale[slot_in ^ 1] = 1;
   }
   if (METERS) {
@@ -1148,6 +1249,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   st.tol = sim->prm.tol;
   st.anchor = sim->anchor.as<double>();
   st.block_sums = sim->block_sums.as<double>();
+  st.block_max = sim->block_max.as<unsigned long long>();
   st.gravity = sim->prm.op.gravity;
   if (D == 3 && sim->prm.obstacle_clamp) {
     st.obs_clamp = 1;
@@ -1241,9 +1343,11 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   sim->cur ^= 1;
   // record this step for the host
   StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
-  k_pack_record<<<1, 64, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
-                                      load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
-                                      load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr, sim_dtctl(sim));
+  k_pack_record<<<1, kPackThreads, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
+                                                load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
+                                                load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr,
+                                                tiled ? nullptr : sim->block_max.as<unsigned long long>(),
+                                                tiled ? 0 : (int64_t)gx, sim_dtctl(sim));
   TSPH_CHECK_LAUNCH();
   TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
@@ -1411,6 +1515,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->anchor, 8 * n * D);
   ENS(sim->flags, sizeof(SimFlags));
   ENS(sim->block_sums, 8 * MS_NSUM * (int64_t)grid_xcd(n));
+  ENS(sim->block_max, 8 * 4 * (int64_t)grid_xcd(n));
   sim->obstacle = prm->obstacle_end > prm->obstacle_begin ? 1 : 0;
   ENS(sim->load_part, 8 * 6 * ((int64_t)grid_for(sim->nw) + 1));
   ENS(sim->loads_out, 8 * 8);
@@ -1605,7 +1710,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
   if (!sim) return TSPH_OK;
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
-                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->sums_out, &sim->rec_dev,
+                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->sums_out, &sim->rec_dev,
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,

@@ -36,7 +36,7 @@ struct HaloView {
 
 // f(l4, keep4) once per chunk of four entries of row i's halo-position list
 // (the NbSlots layout with 16-bit entries): padded slots hold position 0
-// with keep false.
+// with keep false.  CH chunks per batch: the batch's loads come first.
 struct NbHalo {
   const uint16_t* idx;
   const int32_t* cnt;

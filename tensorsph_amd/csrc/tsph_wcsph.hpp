@@ -102,6 +102,10 @@ struct StepArgs {
   const double* anchor;            // positions at the last neighbour-list build
   unsigned long long* maxes;       // MM_NMAX ordered keys (device)
   double* block_sums;              // gridDim.x * MS_NSUM partials, or NULL
+  // gridDim.x records of 4 keys (displacement, speed, density maxima as
+  // ordered keys, boundary hits), or NULL: the sweep adds them into maxes and
+  // escaped with atomics
+  unsigned long long* block_max;
   unsigned long long* escaped;     // boundary crossings
   int meters;                      // 1: also fill block_sums and the meter maxima
   double gravity;                  // for the potential energy meter
