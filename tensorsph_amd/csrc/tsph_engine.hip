@@ -52,8 +52,8 @@ TSPH_D double ord_dec(unsigned long long u) {
 // dt = next_dt(dt, max |v| at the step's start) (droplet_impact.py:536-552),
 // the next step_dt = min(dt, target - elapsed), and the step that starts at the
 // target (within tol) is the final sample, integrated with step_dt = 0.
-// The force sweep's per-block records (blk, nblk of them; NULL when it used
-// atomics) are reduced here first: the displacement, speed and density
+// rec: the step's slot of the pinned host ring.  The force sweep's per-block
+// records (blk, nblk of them; NULL when it used atomics) are reduced here first: the displacement, speed and density
 // maxima and the boundary hits of the step.  A skipped (stale) step wrote
 // none.  kPackThreads threads.
 constexpr int kPackThreads = 256;
@@ -138,6 +138,8 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_record(
       }
     }
   }
+  // rec is pinned host memory: the host reads it after the step's event
+  __threadfence_system();
 }
 
 // a run keeps per-step load rows when it has obstacle stations or the clamp
@@ -1341,15 +1343,15 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   }
   if (!tiled) TSPH_TRY((queue_list_sweeps<D, S>(sim, k, meters, stream)));
   sim->cur ^= 1;
-  // record this step for the host
-  StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
-  k_pack_record<<<1, kPackThreads, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
+  // record this step for the host: the record kernel writes the pinned host
+  // ring directly (no copy on the stream)
+  StepRecord* rec_host = &sim->ring[r];
+  k_pack_record<<<1, kPackThreads, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_host,
                                                 load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
                                                 load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr,
                                                 tiled ? nullptr : sim->block_max.as<unsigned long long>(),
                                                 tiled ? 0 : (int64_t)gx, sim_dtctl(sim));
   TSPH_CHECK_LAUNCH();
-  TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, stream));
   TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
   sim->ring_step[r] = k;
   return TSPH_OK;
@@ -1521,7 +1523,6 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->loads_out, 8 * 8);
   ENS(sim->clamp_acc, 8 * 4);
   ENS(sim->sums_out, 8 * MS_NSUM);
-  ENS(sim->rec_dev, sizeof(StepRecord) * kRing);
   ENS(sim->old2new, 8 * nt); ENS(sim->new2old, 8 * nt); ENS(sim->fscan, 8 * nt);
   ENS(sim->cells_new, 8 * nt * 3);
 #undef ENS
@@ -1710,7 +1711,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
   if (!sim) return TSPH_OK;
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
-                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->sums_out, &sim->rec_dev,
+                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->sums_out,
                     &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,

@@ -74,7 +74,7 @@ struct tsph_sim {
   double last_step_dt = 0.0;        // the size of the last step run
   bool rebuild_forced = false;      // a correction re-binned uncounted: the next step re-bins
   int64_t escaped_extra = 0;        // boundary crossings of the position corrections
-  tsph::DevBuf flags, block_sums, block_max, sums_out, rec_dev;
+  tsph::DevBuf flags, block_sums, block_max, sums_out;
   tsph::DevBuf old2new, new2old, fscan, cells_new;
   tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
