@@ -452,6 +452,14 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
  * Takes effect at the next step, which rebuilds the layout.  No counterpart
  * in the reference, which has one layout (its neighbour pairs). */
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y);
+/* Slot width of the per-row neighbour lists (host): *current receives the
+ * width of the last build; width >= 0 sets the width the next build starts
+ * from (rounded up to a multiple of 4; 0: count first).  A build widens the
+ * slots by itself when a list does not fit: in the stepping loop the steps
+ * queued on such a build change nothing and are run again on a wider build,
+ * which is not counted as a list rebuild.  No counterpart in the reference;
+ * for callers that plan memory ahead. */
+int tsph_sim_slot_width(tsph_sim* sim, int64_t width, int64_t* current);
 /* Layout of the last build (host, 4 int64): the layout, the 256-row tiles
  * on the LDS halo path, all tiles (both 0 outside layout 2) and the largest
  * halo in rows.  Diagnostic; no counterpart in the reference. */

@@ -157,6 +157,7 @@ def _declare(L):
         "tsph_sim_timing": [_vp, C.c_int, _dp, _ip],
         "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
         "tsph_sim_layout": [_vp, _ip],
+        "tsph_sim_slot_width": [_vp, _i64, _ip],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],
         "tsph_sim_destroy": [_vp],
         "tsph_threed_accelerations": [_vp, C.POINTER(ThreedParams), _vp, _vp, _vp, _vp, _vp, _vp, _vp],

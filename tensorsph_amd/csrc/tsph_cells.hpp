@@ -386,7 +386,7 @@ __global__ void k_pos_f32(const double* __restrict__ pos, int64_t nt, typename P
 }
 
 // Per-row lists at the build radius (cell walk, the unfused distance test of
-// the pair search, minimum image when PER).  Every pass writes each row's
+// the pair search, minimum image when PER) of rows [i0, nt).  Every pass writes each row's
 // entry count, their maximum (kmax) and, if total is given, their sum; with
 // FILL the entries go into K slots per row (NbSlots layout; a row with more
 // than K entries keeps its first K and its full count, so the caller widens
@@ -401,13 +401,13 @@ __global__ void k_pos_f32(const double* __restrict__ pos, int64_t nt, typename P
 template <int D, bool FILL, bool PER, bool PF>
 __global__ void __launch_bounds__(kBlock) k_slot_lists(NbCells walk, const double* __restrict__ pos,
                                                        const typename PosF<D>::T* __restrict__ posf, int64_t nf,
-                                                       int64_t nt, double rc2, float lo2, float hi2, double b0,
-                                                       double b1, double b2, int32_t* __restrict__ cnt,
+                                                       int64_t i0, int64_t nt, double rc2, float lo2, float hi2,
+                                                       double b0, double b1, double b2, int32_t* __restrict__ cnt,
                                                        int32_t* __restrict__ idx, int64_t K,
                                                        unsigned int* __restrict__ kmax,
                                                        unsigned long long* __restrict__ total) {
   using VF = typename PosF<D>::T;
-  const int64_t i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t i = i0 + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   int k = 0;
   if (i < nt) {
     const double box[3] = {b0, b1, b2};
@@ -550,7 +550,7 @@ int build_slot_lists(const NbCells& walk, const double* pos, int64_t nf, int64_t
       out = idx.as<int32_t>();
     }
 #define TSPH_SLOT_PASS(FILL, PFV)                                                                     \
-  k_slot_lists<D, FILL, PER, PFV><<<g, kBlock, 0, stream>>>(walk, pos, pf_ptr, nf, nt, rc2, lo2, hi2, b0, b1, b2, \
+  k_slot_lists<D, FILL, PER, PFV><<<g, kBlock, 0, stream>>>(walk, pos, pf_ptr, nf, 0, nt, rc2, lo2, hi2, b0, b1, b2, \
                                                             cnt.as<int32_t>(), out, K, kmax.as<unsigned int>(), total)
     if (K > 0 && pf) TSPH_SLOT_PASS(true, true);
     else if (K > 0) TSPH_SLOT_PASS(true, false);

@@ -279,6 +279,7 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
   S* hm = nullptr;
   if constexpr (HALO) {
     H = tile < hv.ntiles ? hv.cnt[tile].x : -1;
+    if (H > hv.max_rows) H = -1;                     // more rows than this launch staged: global path
     if (H >= 0) {
       hm = reinterpret_cast<S*>(hx + (size_t)H * D);
       // every load of the staging is issued before the first store (up to
@@ -643,6 +644,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
   HaloRec<D, S>* hrec = reinterpret_cast<HaloRec<D, S>*>(halo_lds);
   if constexpr (HALO) {
     if (tile < hv.ntiles) { const int2 c = hv.cnt[tile]; H = c.x; Fh = c.y; }
+    if (H > hv.max_rows) H = -1;                     // more rows than this launch staged: global path
     if (H >= 0) {
       // every load of the staging is issued before the first store (up to
       // kHaloCap / kBlock rows per thread)
@@ -903,25 +905,6 @@ __global__ void k_reset_meters(SimFlags* fl, int slot_in) {
 }
 
 // ------------------------------------------------------------------ rebuild helpers
-__global__ void k_rank_flags(const int64_t* __restrict__ order, int64_t nt, int64_t n,
-                             unsigned long long* __restrict__ fflag) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nt) fflag[s] = order[s] < n ? 1ull : 0ull;
-}
-
-// new row of sorted slot s: fluid rank, or n + wall rank
-__global__ void k_new_index(const int64_t* __restrict__ order, const int64_t* __restrict__ fscan,
-                            int64_t nt, int64_t n, int64_t* __restrict__ old2new,
-                            int64_t* __restrict__ new2old) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nt) return;
-  const int64_t old = order[s];
-  const int64_t fr = fscan[s];                 // fluid rows before slot s
-  const int64_t nw_new = old < n ? fr : n + (s - fr);
-  old2new[old] = nw_new;
-  new2old[nw_new] = old;
-}
-
 template <int D, typename S>
 __global__ void k_permute(const int64_t* __restrict__ new2old, int64_t nt, const double* __restrict__ pos,
                           const double* __restrict__ vel, const S* __restrict__ mass,
@@ -939,11 +922,6 @@ __global__ void k_permute(const int64_t* __restrict__ new2old, int64_t nt, const
   }
   mass2[k] = mass[o];
   orig2[k] = orig[o];
-}
-
-__global__ void k_remap_order(int64_t* __restrict__ order, const int64_t* __restrict__ old2new, int64_t nt) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < nt) order[s] = old2new[order[s]];
 }
 
 // ------------------------------------------------------------------ scatter back
@@ -1000,20 +978,23 @@ namespace tsph {
 // Dense per-cell row ranges of the rows in their current order (fluid and
 // wall rows each grouped by cell; cells_new holds every row's cell) and the
 // per-row lists at the build radius rc in fixed slots, walked from them.
+// The dense cell table of rows sorted by cell (fluid rows, then walls):
+// row-major strides (last axis fastest) unless given.
 template <int D>
-int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t* ext, const int64_t* stride,
-                    hipStream_t stream) {
+int build_cell_table(tsph_sim* sim, const int64_t* ext, const int64_t* stride, hipStream_t stream) {
   const int64_t n = sim->n, nt = sim->nt;
-  sim->halo_ok = false;                   // halos name the rows of the lists they were built from
   double dense = 1.0;
   for (int d = 0; d < D; ++d) dense *= (double)ext[d];
   TSPH_REQUIRE(dense <= (double)kMaxDenseCells, TSPH_ERR_ARG,
                "cell grid of %.3g cells exceeds the dense range table (%lld cells): the particle "
                "cloud is too sparse for its cutoff", dense, (long long)kMaxDenseCells);
   sim->ncells_dense = (int64_t)dense;
+  int64_t rm[3] = {0, 0, 0};
+  rm[D - 1] = 1;
+  for (int d = D - 2; d >= 0; --d) rm[d] = rm[d + 1] * ext[d + 1];
   for (int d = 0; d < 3; ++d) {
     sim->cell_ext[d] = d < D ? ext[d] : 1;
-    sim->cell_stride[d] = d < D ? stride[d] : 0;
+    sim->cell_stride[d] = d < D ? (stride ? stride[d] : rm[d]) : 0;
   }
   TSPH_TRY(sim->cell_R.ensure(sizeof(int4) * (size_t)sim->ncells_dense));
   k_cell_table_clear<<<grid_for(sim->ncells_dense), kBlock, 0, stream>>>(sim->cell_R.as<int4>(), sim->ncells_dense);
@@ -1021,6 +1002,15 @@ int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t*
                                                        sim->cell_stride[1], sim->cell_stride[2],
                                                        sim->cell_R.as<int4>());
   TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
+template <int D>
+int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t* ext, const int64_t* stride,
+                    hipStream_t stream) {
+  const int64_t n = sim->n, nt = sim->nt;
+  sim->halo_ok = false;                   // halos name the rows of the lists they were built from
+  TSPH_TRY(build_cell_table<D>(sim, ext, stride, stream));
   // every row lies within ext[d] cells of edge rc of row 0
   double extent = 0.0;
   for (int d = 0; d < D; ++d) extent = fmax(extent, (double)sim->cell_ext[d] * rc);
@@ -1029,35 +1019,95 @@ int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t*
 }
 template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*, const int64_t*, hipStream_t);
 
-// The halos of the 2-D slot lists just built (tiles == 2): one workgroup per
-// tile of 256 fluid rows; the largest halo sizes the sweeps' LDS.  A halo
-// holds at most kHaloLds bytes of staged rows (rec_bytes each), which keeps
-// the sweeps within the default dynamic LDS limit of a launch.
+// The 2-D halo layout's lists (tiles == 2): per tile of kHaloRows fluid rows
+// one workgroup builds the tile's global and halo lists and its halo
+// (k_halo_lists2); the wall rows' lists come from the cell walk; the reduce
+// kernel writes the build's report into pinned memory and marks the step
+// flags stale when a list did not fit its slots or a row was binned from
+// outside the grid.  lagged: one pass at the current width, no wait; the
+// host reads the report when the steps on this build come back (a stale
+// report makes it build again, wider).  Otherwise passes are repeated, wider,
+// until every list fits (the first build counts first), and the stream is
+// synchronised.  The sweeps stage halos of up to halo_max rows (a tile with a
+// larger halo reads the global arrays): the exact largest halo after a
+// synchronised build, the last report's plus a margin after a lagged one,
+// within kHaloLds bytes of staged rows (rec_bytes each), which keeps the
+// sweeps within the default dynamic LDS limit of a launch.
 constexpr size_t kHaloLds = 60 * 1024;
-int build_halos(tsph_sim* sim, size_t rec_bytes, hipStream_t stream) {
-  const int64_t n = sim->n;
+constexpr int kHaloMargin = 64;
+int build_halo_lists(tsph_sim* sim, const double* npos, double rc, size_t rec_bytes, bool lagged,
+                     hipStream_t stream) {
+  const int64_t n = sim->n, nt = sim->nt;
   sim->halo_ok = false;
+  TSPH_TRY(build_cell_table<2>(sim, sim->bgrid.ext, nullptr, stream));
   const int64_t ntiles = (n + kHaloRows - 1) / kHaloRows;
   sim->halo_tiles = ntiles;
-  if (ntiles == 0) return TSPH_OK;
-  const int64_t groups = (n + 63) / 64;
-  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)ntiles * kHaloCap));
-  TSPH_TRY(sim->halo_cnt.ensure(sizeof(int2) * (size_t)ntiles));
-  TSPH_TRY(sim->halo_idx.ensure(sizeof(uint16_t) * (size_t)groups * 64 * (size_t)sim->slot_K + 64));
-  TSPH_TRY(sim->halo_hmax.ensure(2 * sizeof(unsigned int)));
-  TSPH_CHECK_HIP(hipMemsetAsync(sim->halo_hmax.ptr, 0, 2 * sizeof(unsigned int), stream));
+  const int64_t groups = (nt + 63) / 64;
   const int cap = (int)std::min<size_t>((size_t)kHaloCap, kHaloLds / rec_bytes);
-  k_halo_build2<<<(unsigned)ntiles, kHaloRows, 0, stream>>>(sim_cells(sim), sim_slots(sim), n, cap,
-                                                            sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(),
-                                                            sim->halo_idx.as<uint16_t>(),
-                                                            sim->halo_hmax.as<unsigned int>());
-  TSPH_CHECK_LAUNCH();
-  unsigned int hmax[2] = {0, 0};
-  TSPH_CHECK_HIP(hipMemcpyAsync(hmax, sim->halo_hmax.ptr, sizeof(hmax), hipMemcpyDeviceToHost, stream));
-  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-  sim->halo_max = (int)hmax[0];
-  sim->halo_used = (int64_t)hmax[1];
-  sim->halo_ok = true;
+  const double rc2 = rc * rc;
+  TSPH_TRY(sim->slot_cnt.ensure(sizeof(int32_t) * (size_t)(nt > 0 ? nt : 1)));
+  TSPH_TRY(sim->slot_max.ensure(sizeof(unsigned int)));
+  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)(ntiles > 0 ? ntiles : 1) * kHaloCap));
+  TSPH_TRY(sim->halo_cnt.ensure(sizeof(int2) * (size_t)(ntiles > 0 ? ntiles : 1)));
+  TSPH_TRY(sim->halo_hmax.ensure(sizeof(unsigned int) * (2 * (size_t)ntiles + 2)));
+  unsigned int* blk = sim->halo_hmax.as<unsigned int>();
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  int64_t& K = sim->slot_K;
+  const bool wait = !lagged || K == 0;
+  const unsigned int prev_hmax = sim->build_stats->hmax;
+  for (int pass = 0; pass < 4; ++pass) {
+    int32_t* gidx = nullptr;
+    uint16_t* hidx = nullptr;
+    if (K > 0) {
+      TSPH_TRY(sim->slot_idx.ensure(sizeof(int32_t) * (size_t)groups * 64 * (size_t)K));
+      TSPH_TRY(sim->halo_idx.ensure(sizeof(uint16_t) * (size_t)groups * 64 * (size_t)K + 64));
+      gidx = sim->slot_idx.as<int32_t>();
+      hidx = sim->halo_idx.as<uint16_t>();
+    }
+    TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
+    TSPH_CHECK_HIP(hipMemsetAsync(sim->slot_max.ptr, 0, sizeof(unsigned int), stream));
+    if (ntiles > 0) {
+      if (K > 0)
+        k_halo_lists2<true><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
+            sim_cells(sim), sim->bgrid, npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
+            sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
+      else
+        k_halo_lists2<false><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
+            sim_cells(sim), sim->bgrid, npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
+            sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
+    }
+    if (nt > n) {
+      const unsigned g = grid_xcd(nt - n);
+      if (K > 0)
+        k_slot_lists<2, true, false, false><<<g, kBlock, 0, stream>>>(
+            sim_cells(sim), npos, nullptr, n, n, nt, rc2, 0.f, 0.f, 0.0, 0.0, 0.0, sim->slot_cnt.as<int32_t>(),
+            gidx, K, sim->slot_max.as<unsigned int>(), nullptr);
+      else
+        k_slot_lists<2, false, false, false><<<g, kBlock, 0, stream>>>(
+            sim_cells(sim), npos, nullptr, n, n, nt, rc2, 0.f, 0.f, 0.0, 0.0, 0.0, sim->slot_cnt.as<int32_t>(),
+            gidx, K, sim->slot_max.as<unsigned int>(), nullptr);
+    }
+    k_halo_lists_reduce<<<1, kHaloRows, 0, stream>>>(blk, ntiles, sim->slot_max.as<unsigned int>(),
+                                                     sim->bin_flag.as<unsigned int>(), K, &fl->stale[0],
+                                                     sim->build_stats);
+    TSPH_CHECK_LAUNCH();
+    if (!wait) {
+      // prev_hmax: the largest halo of the last build that reported
+      sim->halo_max = std::min(cap, (int)prev_hmax + kHaloMargin);
+      sim->halo_ok = true;
+      return TSPH_OK;
+    }
+    TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+    const BuildStats st = *sim->build_stats;
+    if (st.outside) return TSPH_OK;               // the caller regrids (halo not in place)
+    if (K > 0 && (int64_t)st.kmax <= K) {
+      sim->halo_max = (int)st.hmax;
+      sim->halo_ok = true;
+      return TSPH_OK;
+    }
+    K = ((int64_t)st.kmax + st.kmax / 8 + 4 + 3) / 4 * 4;
+  }
+  TSPH_REQUIRE(false, TSPH_ERR_HIP, "neighbour lists did not fit %lld slots per row", (long long)K);
   return TSPH_OK;
 }
 
@@ -1067,6 +1117,7 @@ static inline HaloView sim_halo(const tsph_sim* sim) {
   h.cnt = sim->halo_cnt.as<int2>();
   h.idx = sim->halo_idx.as<uint16_t>();
   h.ntiles = sim->halo_tiles;
+  h.max_rows = sim->halo_max;
   return h;
 }
 static inline NbHalo sim_halo_lists(const tsph_sim* sim) {
@@ -1077,57 +1128,159 @@ static inline NbHalo sim_halo_lists(const tsph_sim* sim) {
   return l;
 }
 
+// Stable counting sort of rows [r0, r1) by their bin keys (k_bin has counted
+// them into bin_cnt): new2old[r0, r1) lists the rows in key order, ties in
+// row order.  bin_cnt is all zero again afterwards.
+int bin_sort(tsph_sim* sim, int64_t r0, int64_t r1, hipStream_t stream) {
+  if (r1 <= r0) return TSPH_OK;
+  const int64_t M = sim->bgrid.M;
+  const int64_t nparts = (M + kScanTile - 1) / kScanTile;
+  int32_t* cnt = sim->bin_cnt.as<int32_t>();
+  int32_t* start = sim->bin_start.as<int32_t>();
+  int32_t* part = sim->bin_part.as<int32_t>();
+  k_scan_sums<<<(unsigned)nparts, kBlock, 0, stream>>>(cnt, M, part);
+  k_scan_parts<<<1, kBlock, 0, stream>>>(part, nparts);
+  k_scan_apply<<<(unsigned)nparts, kBlock, 0, stream>>>(cnt, M, part, (int32_t)r0, start);
+  const unsigned g = grid_for(r1 - r0);
+  k_bin_scatter<<<g, kBlock, 0, stream>>>(sim->bin_keys.as<int32_t>(), r0, r1, start, cnt, sim->bin_tmp.as<int32_t>());
+  k_bin_rank<<<g, kBlock, 0, stream>>>(sim->bin_tmp.as<int32_t>(), sim->bin_keys.as<int32_t>(), r0, r1, start,
+                                       sim->new2old.as<int64_t>());
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
+}
+
+// The binning grid: cells of edge rc over the clamp box and every row now
+// (synchronises).  Sizes the sort's buffers.
+template <int D>
+int bin_regrid(tsph_sim* sim, double rc, hipStream_t stream) {
+  const int64_t nt = sim->nt;
+  const unsigned g = grid_for(nt);
+  TSPH_TRY(sim->bin_bounds.ensure(sizeof(long long) * (6 * (size_t)g + 6)));
+  long long* blk = sim->bin_bounds.as<long long>();
+  k_bin_bounds<D><<<g, kBlock, 0, stream>>>(sim->pos[sim->cur].as<double>(), nt, rc, blk);
+  k_bin_bounds_reduce<<<1, kBlock, 0, stream>>>(blk, (int64_t)g, blk + 6 * (size_t)g);
+  TSPH_CHECK_LAUNCH();
+  long long mm[6];
+  TSPH_CHECK_HIP(hipMemcpyAsync(mm, blk + 6 * (size_t)g, sizeof(mm), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  BinGrid& bg = sim->bgrid;
+  bg = BinGrid{};
+  bg.rc = rc;
+  double cells = 1.0;
+  int64_t blocks = 1;
+  for (int d = 0; d < 3; ++d) { bg.cmin[d] = 0; bg.ext[d] = 1; bg.nb[d] = 1; }
+  for (int d = 0; d < D; ++d) {
+    const long long blo = (long long)floor(sim->prm.lo[d] / rc), bhi = (long long)floor(sim->prm.hi[d] / rc);
+    const long long lo = mm[d] < blo ? mm[d] : blo, hi = mm[3 + d] > bhi ? mm[3 + d] : bhi;
+    bg.cmin[d] = lo;
+    bg.ext[d] = hi - lo + 1;
+    bg.nb[d] = (bg.ext[d] + BinOrder<D>::B - 1) / BinOrder<D>::B;
+    cells *= (double)bg.ext[d];
+    blocks *= bg.nb[d];
+  }
+  TSPH_REQUIRE(cells <= (double)kMaxDenseCells && (double)blocks * BinOrder<D>::PER_BLOCK < 2.0e9, TSPH_ERR_ARG,
+               "cell grid of %.3g cells exceeds the dense range table (%lld cells): the particle "
+               "cloud or the clamp box is too sparse for its cutoff", cells, (long long)kMaxDenseCells);
+  bg.M = blocks * BinOrder<D>::PER_BLOCK;
+  // the counts return to zero after every sort; a regrid starts them there
+  TSPH_TRY(sim->bin_cnt.ensure(sizeof(int32_t) * (size_t)(bg.M + 1)));
+  TSPH_CHECK_HIP(hipMemsetAsync(sim->bin_cnt.ptr, 0, sizeof(int32_t) * (size_t)(bg.M + 1), stream));
+  TSPH_TRY(sim->bin_start.ensure(sizeof(int32_t) * (size_t)(bg.M + 1)));
+  TSPH_TRY(sim->bin_part.ensure(sizeof(int32_t) * (size_t)((bg.M + kScanTile - 1) / kScanTile)));
+  TSPH_TRY(sim->bin_keys.ensure(sizeof(int32_t) * (size_t)nt));
+  TSPH_TRY(sim->bin_tmp.ensure(sizeof(int32_t) * (size_t)nt));
+  TSPH_TRY(sim->bin_cells.ensure(sizeof(int64_t) * (size_t)nt * D));
+  TSPH_TRY(sim->bin_flag.ensure(sizeof(unsigned int)));
+  sim->bgrid_ok = true;
+  return TSPH_OK;
+}
+
+// New row order: the fluid rows by key (a full build also regrids and sorts
+// the walls, which keep their rows until the next full build).  Fills
+// new2old and bin_cells (cell of every row in the old order).
+template <int D>
+int bin_rows(tsph_sim* sim, bool full, double rc, hipStream_t stream) {
+  const int64_t n = sim->n, nt = sim->nt;
+  if (full) TSPH_TRY(bin_regrid<D>(sim, rc, stream));
+  const double* pos = sim->pos[sim->cur].as<double>();
+  TSPH_CHECK_HIP(hipMemsetAsync(sim->bin_flag.ptr, 0, sizeof(unsigned int), stream));
+  auto part = [&](int64_t r0, int64_t r1) -> int {
+    if (r1 <= r0) return TSPH_OK;
+    k_bin<D><<<grid_for(r1 - r0), kBlock, 0, stream>>>(pos, r0, r1, sim->bgrid, sim->bin_cells.as<int64_t>(),
+                                                       sim->bin_keys.as<int32_t>(), sim->bin_cnt.as<int32_t>(),
+                                                       sim->bin_flag.as<unsigned int>());
+    TSPH_CHECK_LAUNCH();
+    return bin_sort(sim, r0, r1, stream);
+  };
+  TSPH_TRY(part(0, n));
+  if (full) TSPH_TRY(part(n, nt));
+  TSPH_CHECK_HIP(hipMemcpyAsync(sim->bin_outside, sim->bin_flag.ptr, sizeof(unsigned int),
+                                hipMemcpyDeviceToHost, stream));
+  return TSPH_OK;
+}
+
+// Re-bins the rows and builds the lists.  lagged (the stepping loop): the
+// 2-D halo build does not wait for its report; the report of the previous
+// build, complete by now, widens the slots or regrids when that build's lists
+// did not fit or a row was outside the grid, and such a repeat of a build on
+// unchanged positions is not counted.
 template <int D, typename S>
-int sim_rebuild(tsph_sim* sim, hipStream_t stream) {
-  tsph_ctx* ctx = sim->ctx;
+int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
   const int64_t n = sim->n, nt = sim->nt;
   const double rc = 2.0 * sim->prm.op.h + sim->prm.skin;
-  const double* pos = sim->pos[sim->cur].as<double>();
-  // rows in Morton order of their cells: a run of consecutive targets and
-  // all their neighbours sit in a compact block of memory (L2 reuse)
-  TSPH_TRY((run_search<double, D>(ctx, pos, nt, rc, nullptr, stream, true)));
-  CellSearch& cs = ctx->search;
-  const unsigned g = grid_for(nt);
-  // new row of every particle: fluid by cell order, walls by cell order
-  k_rank_flags<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), nt, n, sim->fscan.as<unsigned long long>());
-  TSPH_CHECK_LAUNCH();
-  {
-    size_t bytes = 0;
-    unsigned long long* f = sim->fscan.as<unsigned long long>();
-    TSPH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, f, f, (int)nt, stream));
-    TSPH_TRY(ctx->scan_tmp.ensure(bytes));
-    TSPH_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(ctx->scan_tmp.ptr, bytes, f, f, (int)nt, stream));
+  bool halo_lists = false;
+  if constexpr (D == 2) halo_lists = sim->tiles == 2;
+  if (sim->slot_K_next >= 0) { sim->slot_K = sim->slot_K_next; sim->slot_K_next = -1; }
+  const BuildStats prev = *sim->build_stats;
+  const bool redo = lagged && halo_lists && sim->built && prev.K > 0 && (prev.kmax > prev.K || prev.outside);
+  if (redo && (int64_t)prev.kmax > sim->slot_K) sim->slot_K = ((int64_t)prev.kmax + prev.kmax / 8 + 4 + 3) / 4 * 4;
+  // rows in blocked Morton order of their cells: a run of consecutive
+  // targets and all their neighbours sit in a compact block of memory
+  bool full = !sim->bgrid_ok || sim->bgrid.rc != rc || (redo && prev.outside);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    TSPH_TRY(bin_rows<D>(sim, full, rc, stream));
+    const int64_t rows = full ? nt : n;        // a full build moves the walls too
+    const int other = sim->cur ^ 1;
+    k_permute<D, S><<<grid_for(rows), kBlock, 0, stream>>>(
+        sim->new2old.as<int64_t>(), rows, sim->pos[sim->cur].as<double>(), sim->vel[sim->cur].as<double>(),
+        sim->mass.as<S>(), sim->orig.as<int64_t>(), sim->bin_cells.as<int64_t>(), sim->pos[other].as<double>(),
+        sim->vel[other].as<double>(), sim->mass2.as<S>(), sim->orig2.as<int64_t>(), sim->cells_new.as<int64_t>());
+    TSPH_CHECK_LAUNCH();
+    std::swap(sim->mass, sim->mass2);
+    std::swap(sim->orig, sim->orig2);
+    sim->cur = other;
+    if (full && nt > n) {
+      // both ping-pong sets carry the (static) wall rows
+      const int o = sim->cur ^ 1;
+      TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[o].as<double>() + n * D, sim->pos[sim->cur].as<double>() + n * D,
+                                    sizeof(double) * (nt - n) * D, hipMemcpyDeviceToDevice, stream));
+      TSPH_CHECK_HIP(hipMemcpyAsync(sim->vel[o].as<double>() + n * D, sim->vel[sim->cur].as<double>() + n * D,
+                                    sizeof(double) * (nt - n) * D, hipMemcpyDeviceToDevice, stream));
+      TSPH_CHECK_HIP(hipMemcpyAsync(sim->mass2.as<S>() + n, sim->mass.as<S>() + n, sizeof(S) * (nt - n),
+                                    hipMemcpyDeviceToDevice, stream));
+      TSPH_CHECK_HIP(hipMemcpyAsync(sim->orig2.as<int64_t>() + n, sim->orig.as<int64_t>() + n,
+                                    sizeof(int64_t) * (nt - n), hipMemcpyDeviceToDevice, stream));
+    }
+    const double* npos = sim->pos[sim->cur].as<double>();
+    if (halo_lists) {
+      // clears the step flags; a lagged build may mark them stale again
+      TSPH_TRY(build_halo_lists(sim, npos, rc, sizeof(HaloRec<D, S>), lagged, stream));
+      if (lagged && sim->slot_K > 0 && sim->halo_ok) break;
+      if (!sim->build_stats->outside) break;
+    } else {
+      TSPH_TRY((build_sim_slots<D>(sim, npos, rc, sim->bgrid.ext, nullptr, stream)));  // synchronises
+      SimFlags* fl = sim->flags.as<SimFlags>();
+      TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
+      if (!*sim->bin_outside) break;
+    }
+    // a fluid row left the grid (a correction moved it past the clamp box):
+    // regrid and bin every row again
+    TSPH_REQUIRE(!full, TSPH_ERR_HIP, "binning grid does not cover the rows after a full build");
+    full = true;
   }
-  k_new_index<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), sim->fscan.as<int64_t>(), nt, n,
-                                        sim->old2new.as<int64_t>(), sim->new2old.as<int64_t>());
-  const int other = sim->cur ^ 1;
-  k_permute<D, S><<<g, kBlock, 0, stream>>>(sim->new2old.as<int64_t>(), nt, pos, sim->vel[sim->cur].as<double>(),
-                                            sim->mass.as<S>(), sim->orig.as<int64_t>(), cs.cells.as<int64_t>(),
-                                            sim->pos[other].as<double>(), sim->vel[other].as<double>(),
-                                            sim->mass2.as<S>(), sim->orig2.as<int64_t>(),
-                                            sim->cells_new.as<int64_t>());
-  k_remap_order<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), sim->old2new.as<int64_t>(), nt);
-  TSPH_CHECK_LAUNCH();
-  std::swap(sim->mass, sim->mass2);
-  std::swap(sim->orig, sim->orig2);
-  sim->cur = other;
   const double* npos = sim->pos[sim->cur].as<double>();
-  TSPH_TRY((build_sim_slots<D>(sim, npos, rc, cs.ext, cs.stride, stream)));
-  if constexpr (D == 2) {
-    if (sim->tiles == 2) TSPH_TRY(build_halos(sim, sizeof(HaloRec<D, S>), stream));
-  }
-  // both ping-pong buffers carry the (static) wall rows
-  if (nt > n) {
-    TSPH_CHECK_HIP(hipMemcpyAsync(sim->pos[sim->cur ^ 1].as<double>() + n * D, npos + n * D,
-                                  sizeof(double) * (nt - n) * D, hipMemcpyDeviceToDevice, stream));
-    TSPH_CHECK_HIP(hipMemcpyAsync(sim->vel[sim->cur ^ 1].as<double>() + n * D,
-                                  sim->vel[sim->cur].as<double>() + n * D, sizeof(double) * (nt - n) * D,
-                                  hipMemcpyDeviceToDevice, stream));
-  }
   TSPH_CHECK_HIP(hipMemcpyAsync(sim->anchor.ptr, npos, sizeof(double) * n * D, hipMemcpyDeviceToDevice, stream));
-  SimFlags* fl = sim->flags.as<SimFlags>();
-  TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
-  sim->rebuilds++;
+  if (!redo) sim->rebuilds++;
   sim->built = true;
   return TSPH_OK;
 }
@@ -1359,11 +1512,12 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
 
 // Consume the record of step k: returns 1 if the step after it must be redone
 // (its list went stale), 0 otherwise.
-int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after) {
+int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after, bool* skipped) {
   const int r = (int)(k % kRing);
   TSPH_CHECK_HIP(hipEventSynchronize(sim->ev[r]));
   StepRecord* rec = &sim->ring[r];
   *stale_after = rec->stale_out != 0;
+  if (skipped) *skipped = rec->stale_in != 0;
   if (rec->stale_in) return TSPH_OK;   // skipped: nothing to record
   if (sim->finished) return TSPH_OK;   // after an adaptive run's final sample: no-op steps
   sim->steps_done = k + 1;
@@ -1412,7 +1566,7 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
     if constexpr (D == 2) {
       if (sim->tiles == 1) return sim_rebuild_tiles<D, S>(sim, stream);
     }
-    return sim_rebuild<D, S>(sim, stream);
+    return sim_rebuild<D, S>(sim, stream, true);
   };
   if (sim->finished) return TSPH_OK;
   if (!sim->built || sim->rebuild_pending) {
@@ -1429,17 +1583,19 @@ int sim_run(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_t stre
       TSPH_TRY((sim_queue_step<D, S>(sim, next, is_meter(next), stream)));
       ++next;
     }
-    bool stale_after = false;
-    TSPH_TRY(sim_consume(sim, done, is_meter(done), &stale_after));
-    ++done;
+    bool stale_after = false, skipped = false;
+    TSPH_TRY(sim_consume(sim, done, is_meter(done), &stale_after, &skipped));
+    // a step skipped on the device (its build's lists did not fit) is run
+    // again after the next build
+    if (!skipped) ++done;
     if (sim->finished) {
       // the final sample is done; steps queued after it integrate with dt = 0
       for (int64_t k = done; k < next; ++k) TSPH_CHECK_HIP(hipEventSynchronize(sim->ev[k % kRing]));
       break;
     }
-    if (stale_after) {
+    if (stale_after || skipped) {
       // steps done..next-1 were skipped on the device; drain them, rebuild, redo
-      for (int64_t k = done; k < next; ++k) {
+      for (int64_t k = skipped ? done + 1 : done; k < next; ++k) {
         bool dummy;
         TSPH_TRY(sim_consume(sim, k, false, &dummy));
       }
@@ -1523,12 +1679,15 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->loads_out, 8 * 8);
   ENS(sim->clamp_acc, 8 * 4);
   ENS(sim->sums_out, 8 * MS_NSUM);
-  ENS(sim->old2new, 8 * nt); ENS(sim->new2old, 8 * nt); ENS(sim->fscan, 8 * nt);
+  ENS(sim->new2old, 8 * nt);
   ENS(sim->cells_new, 8 * nt * 3);
 #undef ENS
   if (rc != TSPH_OK) { tsph_sim_destroy(sim); return rc; }
   hipError_t he = hipHostMalloc((void**)&sim->ring, sizeof(StepRecord) * kRing, hipHostMallocDefault);
+  if (he == hipSuccess) he = hipHostMalloc((void**)&sim->bin_outside, 64, hipHostMallocDefault);
   if (he != hipSuccess) { set_error("hipHostMalloc: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
+  memset(sim->bin_outside, 0, 64);
+  sim->build_stats = reinterpret_cast<BuildStats*>(sim->bin_outside + 4);
   memset(sim->ring, 0, sizeof(StepRecord) * kRing);
   for (int r = 0; r < kRing; ++r) {
     he = hipEventCreateWithFlags(&sim->ev[r], hipEventDisableTiming);
@@ -1616,12 +1775,19 @@ int tsph_sim_stats(tsph_sim* sim, int64_t* counters, double* peaks) {
   return TSPH_OK;
 }
 
+int tsph_sim_slot_width(tsph_sim* sim, int64_t width, int64_t* current) {
+  TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
+  if (current) *current = sim->slot_K;
+  if (width >= 0) sim->slot_K_next = (width + 3) / 4 * 4;   // the lists in place keep their width
+  return TSPH_OK;
+}
+
 int tsph_sim_layout(tsph_sim* sim, int64_t* out) {
   TSPH_REQUIRE(sim && out, TSPH_ERR_ARG, "bad arguments");
   out[0] = sim->tiles;
-  out[1] = sim->halo_ok ? sim->halo_used : 0;
+  out[1] = sim->halo_ok ? (int64_t)sim->build_stats->used : 0;
   out[2] = sim->halo_ok ? sim->halo_tiles : 0;
-  out[3] = sim->halo_ok ? sim->halo_max : 0;
+  out[3] = sim->halo_ok ? (int64_t)sim->build_stats->hmax : 0;
   return TSPH_OK;
 }
 
@@ -1661,6 +1827,7 @@ int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
                TSPH_ERR_ARG, "tile %d x %d out of range ((tx+2)(ty+2) <= 108, tx*ty <= 64)", tile_x, tile_y);
   sim->tiles = tiles;
   sim->halo_ok = false;
+  sim->bgrid_ok = false;
   if (tiles == 1) { sim->tile_dim[0] = tile_x; sim->tile_dim[1] = tile_y; }
   sim->built = false;   // next step rebuilds with the new layout
   return TSPH_OK;
@@ -1712,7 +1879,9 @@ int tsph_sim_destroy(tsph_sim* sim) {
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->sums_out,
-                    &sim->old2new, &sim->new2old, &sim->fscan, &sim->cells_new, &sim->cell_R, &sim->slot_idx,
+                    &sim->new2old, &sim->cells_new, &sim->bin_cnt, &sim->bin_start,
+                    &sim->bin_part, &sim->bin_keys, &sim->bin_tmp, &sim->bin_cells, &sim->bin_flag,
+                    &sim->bin_bounds, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
                     &sim->load_part, &sim->loads_out, &sim->clamp_acc, &sim->closure, &sim->shift_buf,
@@ -1722,6 +1891,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
   for (int r = 0; r < kRing; ++r)
     for (int q = 0; q < 4; ++q) if (sim->tev[r][q]) (void)hipEventDestroy(sim->tev[r][q]);
   if (sim->ring) (void)hipHostFree(sim->ring);
+  if (sim->bin_outside) (void)hipHostFree(sim->bin_outside);
   if (sim->ctx) tsph_ctx_destroy(sim->ctx);
   delete sim;
   return TSPH_OK;

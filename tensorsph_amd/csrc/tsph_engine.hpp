@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <vector>
 
+#include "tsph_bin.hpp"
 #include "tsph_internal.hpp"
 #include "tsph_wcsph.hpp"
 
@@ -75,12 +76,20 @@ struct tsph_sim {
   bool rebuild_forced = false;      // a correction re-binned uncounted: the next step re-bins
   int64_t escaped_extra = 0;        // boundary crossings of the position corrections
   tsph::DevBuf flags, block_sums, block_max, sums_out;
-  tsph::DevBuf old2new, new2old, fscan, cells_new;
+  tsph::DevBuf new2old, cells_new;
+  // binning of the slot-list layouts: the fixed grid (set at a full build,
+  // which also bins the walls) and the counting sort's buffers
+  tsph::BinGrid bgrid{};
+  bool bgrid_ok = false;
+  tsph::DevBuf bin_cnt, bin_start, bin_part, bin_keys, bin_tmp, bin_cells, bin_flag, bin_bounds;
+  unsigned int* bin_outside = nullptr;   // pinned: a row left the grid at the last binning
+  tsph::BuildStats* build_stats = nullptr;  // pinned: the last halo-list build's report
   tsph::DevBuf cell_R;                    // int4 per cell: fluid / wall row ranges (cell walk)
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   tsph::DevBuf slot_idx, slot_cnt, slot_max;   // per-row lists in fixed slots (NbSlots)
   tsph::DevBuf slot_posf;                      // FP32 build positions (list prefilter)
   int64_t slot_K = 0;
+  int64_t slot_K_next = -1;                    // >= 0: the width the next build starts from
   tsph::StepRecord* ring = nullptr;       // pinned
   hipEvent_t ev[tsph::kRing] = {};
   // per-sweep timing (HIP events around each sweep on the stepping stream)
@@ -100,8 +109,7 @@ struct tsph_sim {
   // the lists as 16-bit halo positions
   tsph::DevBuf halo_rows, halo_cnt, halo_idx, halo_hmax;
   int64_t halo_tiles = 0;        // tiles of the build
-  int64_t halo_used = 0;         // tiles of the build on the halo path
-  int halo_max = 0;              // the largest halo of the build (LDS rows to reserve)
+  int halo_max = 0;              // halo rows the sweeps stage (LDS rows to reserve)
   bool halo_ok = false;          // the halo of the current build is in place
   // cell-tile mode (2-D): rows sorted tile-major, dense per-cell row ranges
   int tiles = 0;                 // 1: LDS cell-tile force sweep, 0: slot-list sweeps, 2: slot lists + halos
@@ -125,7 +133,7 @@ struct tsph_sim {
 
 
 namespace tsph {
-int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after);
+int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after, bool* skipped = nullptr);
 __global__ void k_reset_meters(SimFlags* fl, int slot_in);
 __global__ void k_reduce_sums(const double* part, int64_t nblocks, double* out, SimFlags* fl, int slot_in);
 template <int D, typename S> int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream);

@@ -1,10 +1,10 @@
 // Neighbourhood halos of the 2-D resident stepper's slot-list sweeps.
 //
 // The rows of a workgroup (kHaloRows consecutive fluid rows, a tile) are
-// Morton-adjacent, so their neighbour lists name a small set of rows: those
-// of the 3^D cells around the tile's cells.  At each list rebuild a tile's
-// halo is built once: the union of those cells in Morton order (which is the
-// order of the rows), as the explicit list of its fluid rows followed by its
+// adjacent in the binning's blocked Morton order, so their neighbour lists
+// name a small set of rows: those of the 3^D cells around the tile's cells.
+// At each list rebuild a tile's halo is built once: the union of those cells
+// in key order (which is the order of the rows), as the explicit list of its fluid rows followed by its
 // wall rows, and every list entry of the tile's rows re-expressed as a 16-bit
 // position in that halo.  A sweep then stages the halo's rows in LDS with
 // mostly contiguous loads and evaluates its lists from LDS.  The lists keep
@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "tsph_bin.hpp"
 #include "tsph_cells.hpp"
 #include "tsph_common.hpp"
 
@@ -32,6 +33,7 @@ struct HaloView {
   const int2* cnt;
   const uint16_t* idx;             // lists of halo positions, NbSlots layout
   int64_t ntiles;
+  int max_rows;                    // rows the launch stages (larger halos: global path)
 };
 
 // f(l4, keep4) once per chunk of four entries of row i's halo-position list
@@ -108,16 +110,6 @@ TSPH_D int halo_array_scan(int* arr, int m, int* wtot) {
   return total;
 }
 
-TSPH_D uint64_t compact_bits2(uint64_t v) {
-  v &= 0x5555555555555555ull;
-  v = (v | (v >> 1)) & 0x3333333333333333ull;
-  v = (v | (v >> 2)) & 0x0f0f0f0f0f0f0f0full;
-  v = (v | (v >> 4)) & 0x00ff00ff00ff00ffull;
-  v = (v | (v >> 8)) & 0x0000ffff0000ffffull;
-  v = (v | (v >> 16)) & 0x00000000ffffffffull;
-  return v;
-}
-
 // the largest q in [0, m) with a[q] <= x (a ascending, a[0] <= x)
 TSPH_D int halo_floor_search(const int* a, int m, int x) {
   int lo = 0, hi = m;
@@ -128,22 +120,42 @@ TSPH_D int halo_floor_search(const int* a, int m, int x) {
   return lo;
 }
 
-// One workgroup per tile (2-D).  cg: the cell table of the build (cells of
-// every row, per-cell fluid / wall row ranges); nb: the tile rows' global
-// lists; cap (<= kHaloCap): the most rows a halo may have.  Writes the halo
-// rows, the counts and the halo-position lists; hmax[0] gets the largest
-// halo and hmax[1] counts the tiles on the halo path.
-__global__ void __launch_bounds__(kHaloRows) k_halo_build2(NbCells cg, NbSlots nb, int64_t n, int cap,
+constexpr int kHaloUniq = 512;     // distinct neighbourhood cells a tile may hold
+
+// One workgroup per tile of kHaloRows fluid rows (2-D): the tile rows' slot
+// lists at the build radius (rc2) in both forms, global rows (idx, the
+// NbSlots layout) and halo positions (lidx), and the tile's halo.
+//
+// The tile's distinct cells (runs of equal cells among its rows, which are
+// sorted by binning key) and the 3 x 3 cells around each are candidates; a
+// bitonic sort of their keys (candidate number in the low 10 bits) gives the
+// distinct neighbourhood cells in key order and, for every candidate, its
+// cell's number u.  The halo holds the fluid rows of those cells in key order,
+// then their wall rows, and their positions are staged in LDS.  A row then
+// walks the cells of its own cell's nine candidates, in the cell walk's order
+// (offsets x-major, fluid rows before wall rows), tests each row by the FP64
+// squared distance (the test the other list builds decide by) and appends the
+// accepted rows to both lists.  A tile with too many cells or rows for its
+// LDS walks the global cell table instead and gets no halo (count -1).
+//
+// cnt, idx: per-row count and global lists (K slots); lidx: halo lists, the
+// same layout with 16-bit entries; blk[2 * tile]: the tile's longest list and
+// (halo rows + 1, or 0 without a halo).
+template <bool FILL>
+__global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid bg, const double* __restrict__ pos,
+                                                           int64_t n, double rc2, int cap,
+                                                           int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                           uint16_t* __restrict__ lidx, int64_t K,
                                                            uint32_t* __restrict__ halo_rows,
                                                            int2* __restrict__ halo_cnt,
-                                                           uint16_t* __restrict__ lidx,
-                                                           unsigned int* __restrict__ hmax) {
+                                                           unsigned int* __restrict__ blk) {
   constexpr int D = 2, NOFF = 9;
   __shared__ unsigned long long key[kHaloCand];
-  __shared__ int fstart[kHaloCand], foff[kHaloCand], wstart[kHaloCand], woff[kHaloCand];
-  __shared__ int flen[kHaloCand], wlen[kHaloCand];
+  __shared__ int16_t cand_u[kHaloCand];
+  __shared__ int ufs[kHaloUniq], ufl[kHaloUniq], ufo[kHaloUniq], uws[kHaloUniq], uwl[kHaloUniq], uwo[kHaloUniq];
+  __shared__ double2 hx[kHaloCap];
   __shared__ int wtot[4];
-  __shared__ int s_bad;
+  __shared__ int s_kmax[4];
   const int t = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const int64_t r = tile * kHaloRows + t;
@@ -155,155 +167,225 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_build2(NbCells cg, NbSlots n
     c[1] = cg.cells[r * D + 1];
     head = t == 0 || cg.cells[(r - 1) * D] != c[0] || cg.cells[(r - 1) * D + 1] != c[1];
   }
-  if (t == 0) s_bad = 0;
   int nheads;
   const int hpos = halo_block_scan(head ? 1 : 0, wtot, &nheads);
+  const int hidx = hpos + (head ? 1 : 0) - 1;          // this row's cell among the tile's
   const int ncand = nheads * NOFF;
-  if (ncand > kHaloCand) {                  // too many distinct cells: global path
-    if (t == 0) halo_cnt[tile] = make_int2(-1, -1);
-    return;
-  }
-  int np = 1;
-  while (np < ncand) np <<= 1;
-  // the 3^D cells around every distinct cell of the tile, as Morton keys
-  if (head) {
-    for (int o = 0; o < NOFF; ++o) {
-      const int64_t nc[2] = {c[0] + (o / 3 - 1), c[1] + (o % 3 - 1)};
-      const bool in = nc[0] >= 0 && nc[0] < cg.ext[0] && nc[1] >= 0 && nc[1] < cg.ext[1];
-      key[hpos * NOFF + o] = in ? (unsigned long long)morton_key<2>(nc) : ~0ull;
+  bool ok = ncand <= kHaloCand;                        // block-uniform
+  int F = 0, W = 0;
+  if (ok) {
+    int np = 1;
+    while (np < ncand) np <<= 1;
+    if (head) {
+      for (int o = 0; o < NOFF; ++o) {
+        const int64_t nc[2] = {c[0] + (o / 3 - 1), c[1] + (o % 3 - 1)};
+        const bool in = nc[0] >= 0 && nc[0] < cg.ext[0] && nc[1] >= 0 && nc[1] < cg.ext[1];
+        const int id = hpos * NOFF + o;
+        key[id] = in ? (((unsigned long long)bin_key<2>(nc, bg) << 10) | (unsigned long long)id) : ~0ull;
+        cand_u[id] = -1;
+      }
     }
-  }
-  for (int s = ncand + t; s < np; s += kHaloRows) key[s] = ~0ull;
-  __syncthreads();
-  // bitonic sort of key[0, np)
-  for (int size = 2; size <= np; size <<= 1) {
-    for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int p = t; p < (np >> 1); p += kHaloRows) {
-        const int lo = 2 * p - (p & (stride - 1));
-        const int hi = lo + stride;
-        const bool up = (lo & size) == 0;
-        const unsigned long long a = key[lo], b = key[hi];
-        if ((a > b) == up) { key[lo] = b; key[hi] = a; }
+    for (int s2 = ncand + t; s2 < np; s2 += kHaloRows) key[s2] = ~0ull;
+    __syncthreads();
+    for (int size = 2; size <= np; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int p = t; p < (np >> 1); p += kHaloRows) {
+          const int lo = 2 * p - (p & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const unsigned long long x = key[lo], y = key[hi];
+          if ((x > y) == up) { key[lo] = y; key[hi] = x; }
+        }
+        __syncthreads();
+      }
+    }
+    // distinct cells: number them, and give every candidate its cell's number
+    int flag[4], fsum = 0, nu = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = 4 * t + q;
+      flag[q] = e < np && key[e] != ~0ull && (e == 0 || (key[e] >> 10) != (key[e - 1] >> 10)) ? 1 : 0;
+      fsum += flag[q];
+    }
+    const int ubase = halo_block_scan(fsum, wtot, &nu);
+    ok = nu <= kHaloUniq;
+    if (ok) {
+      int u = ubase - 1;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int e = 4 * t + q;
+        if (e < np && key[e] != ~0ull) {
+          u += flag[q];
+          cand_u[(int)(key[e] & 1023ull)] = (int16_t)u;
+          if (flag[q]) {
+            int64_t cc[2];
+            bin_cell<2>((int64_t)(key[e] >> 10), bg, cc);
+            const int4 R = cg.R[cc[0] * cg.stride[0] + cc[1] * cg.stride[1]];
+            ufs[u] = R.x; ufl[u] = R.y - R.x; ufo[u] = R.y - R.x;
+            uws[u] = R.z; uwl[u] = R.w - R.z; uwo[u] = R.w - R.z;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (ok) {
+      F = halo_array_scan(ufo, nu, wtot);               // lengths -> halo offsets
+      W = halo_array_scan(uwo, nu, wtot);
+      ok = F + W <= cap;
+    }
+    if (ok) {
+      // the halo's rows and their positions: fluid rows by cell, then wall
+      // rows; slot s of the halo finds its cell by the offsets, and every
+      // thread issues its (up to kHaloCap / kHaloRows) loads together
+      uint32_t* hr = halo_rows + tile * kHaloCap;
+      const int H = F + W;
+      constexpr int QS = kHaloCap / kHaloRows;
+      int32_t jq[QS];
+#pragma unroll
+      for (int q = 0; q < QS; ++q) {
+        const int s2 = t + q * kHaloRows;
+        int32_t j = 0;
+        if (s2 < H) {
+          const bool wl = s2 >= F;
+          const int x = wl ? s2 - F : s2;
+          const int* off = wl ? uwo : ufo;
+          const int* len = wl ? uwl : ufl;
+          const int* st = wl ? uws : ufs;
+          // the last cell whose offset is <= x and whose range is not empty
+          int lo = 0, hi = nu;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= x) lo = mid; else hi = mid;
+          }
+          while (len[lo] == 0) --lo;          // empty ranges share the next range's offset
+          j = st[lo] + (x - off[lo]);
+          hr[s2] = (uint32_t)j;
+        }
+        jq[q] = j;
+      }
+      double2 xq[QS];
+#pragma unroll
+      for (int q = 0; q < QS; ++q) xq[q] = make_double2(pos[(int64_t)jq[q] * 2], pos[(int64_t)jq[q] * 2 + 1]);
+#pragma unroll
+      for (int q = 0; q < QS; ++q) {
+        const int s2 = t + q * kHaloRows;
+        if (s2 < H) hx[s2] = xq[q];
       }
       __syncthreads();
     }
   }
-  // distinct cells, in Morton order: their fluid and wall row ranges
-  int nu = 0;
-  {
-    int flag[4], s = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = 4 * t + q;
-      flag[q] = e < np && key[e] != ~0ull && (e == 0 || key[e] != key[e - 1]) ? 1 : 0;
-      s += flag[q];
-    }
-    int base = halo_block_scan(s, wtot, &nu);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = 4 * t + q;
-      if (flag[q]) {
-        const unsigned long long k = key[e];
-        const int64_t cx = (int64_t)compact_bits2(k >> 1), cy = (int64_t)compact_bits2(k);
-        const int4 R = cg.R[cx * cg.stride[0] + cy * cg.stride[1]];
-        fstart[base] = R.x; flen[base] = R.y - R.x;
-        wstart[base] = R.z; wlen[base] = R.w - R.z;
-        ++base;
-      }
-    }
-  }
-  __syncthreads();
-  // keep the non-empty ranges (starts ascending with the Morton order)
-  int nf = 0, nw = 0;
-  {
-    int ff[4], fw[4], sf = 0, sw = 0;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = 4 * t + q;
-      ff[q] = e < nu && flen[e] > 0 ? 1 : 0;
-      fw[q] = e < nu && wlen[e] > 0 ? 1 : 0;
-      sf += ff[q]; sw += fw[q];
-    }
-    int bf = halo_block_scan(sf, wtot, &nf);
-    int bw = halo_block_scan(sw, wtot, &nw);
-    int fs[4], fl[4], ws[4], wl[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int e = 4 * t + q;
-      fs[q] = e < nu ? fstart[e] : 0; fl[q] = e < nu ? flen[e] : 0;
-      ws[q] = e < nu ? wstart[e] : 0; wl[q] = e < nu ? wlen[e] : 0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (ff[q]) { fstart[bf] = fs[q]; foff[bf] = fl[q]; ++bf; }
-      if (fw[q]) { wstart[bw] = ws[q]; woff[bw] = wl[q]; ++bw; }
-    }
-    __syncthreads();
-  }
-  const int F = halo_array_scan(foff, nf, wtot);     // lengths -> halo offsets
-  const int W = halo_array_scan(woff, nw, wtot);
-  const int H = F + W;
-  if (H > cap) {
-    if (t == 0) halo_cnt[tile] = make_int2(-1, -1);
-    return;
-  }
-  // the halo's rows: fluid ranges, then wall ranges
-  uint32_t* hr = halo_rows + tile * kHaloCap;
-  for (int q = t; q < nf; q += kHaloRows) {
-    const int len = (q + 1 < nf ? foff[q + 1] : F) - foff[q];
-    for (int k = 0; k < len; ++k) hr[foff[q] + k] = (uint32_t)(fstart[q] + k);
-  }
-  for (int q = t; q < nw; q += kHaloRows) {
-    const int len = (q + 1 < nw ? woff[q + 1] : W) - woff[q];
-    for (int k = 0; k < len; ++k) hr[F + woff[q] + k] = (uint32_t)(wstart[q] + k);
-  }
-  // each row's entries as halo positions, chunk by chunk; an entry outside
-  // the halo (a list built with a cutoff above the cell edge) sends the tile
-  // to the global path
-  bool bad = false;
+  // the lists
+  int k = 0;
   if (live) {
-    const int4* src = nb.chunks(r);
-    ushort4* dst = reinterpret_cast<ushort4*>(lidx) + (r >> 6) * 16 * nb.K + (r & 63);
-    const int cntr = nb.cnt[r];
-    for (int k = 0; k < cntr; k += 4) {
-      const int4 qj = src[(k >> 2) * 64];
-      const int32_t j4[4] = {qj.x, qj.y, qj.z, qj.w};
-      uint16_t l4[4];
+    const double xi[2] = {pos[r * 2], pos[r * 2 + 1]};
+    int4* gp = FILL ? reinterpret_cast<int4*>(idx) + (r >> 6) * 16 * K + (r & 63) : nullptr;
+    ushort4* hp = FILL ? reinterpret_cast<ushort4*>(lidx) + (r >> 6) * 16 * K + (r & 63) : nullptr;
+    const int chunks = (int)(K >> 2);
+    int4 gb = make_int4((int)r, (int)r, (int)r, (int)r);
+    ushort4 hb = make_ushort4(0, 0, 0, 0);
+    auto accept = [&](int32_t j, int l) {
+      if (FILL) {
+        const int sl = k & 3;
+        gb.x = sl == 0 ? j : gb.x; gb.y = sl == 1 ? j : gb.y; gb.z = sl == 2 ? j : gb.z; gb.w = sl == 3 ? j : gb.w;
+        const uint16_t l16 = (uint16_t)l;
+        hb.x = sl == 0 ? l16 : hb.x; hb.y = sl == 1 ? l16 : hb.y; hb.z = sl == 2 ? l16 : hb.z; hb.w = sl == 3 ? l16 : hb.w;
+        if (sl == 3 && (k >> 2) < chunks) {
+          gp[(k >> 2) * 64] = gb;
+          if (ok) hp[(k >> 2) * 64] = hb;
+        }
+      }
+      ++k;
+    };
+    if (ok) {
+      for (int o = 0; o < NOFF; ++o) {
+        const int u = cand_u[hidx * NOFF + o];
+        if (u < 0) continue;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        int l = 0;
-        if (k + u < cntr) {
-          const int j = j4[u];
-          const bool fluid = j < n;
-          const int* st = fluid ? fstart : wstart;
-          const int* of = fluid ? foff : woff;
-          const int m = fluid ? nf : nw;
-          const int end = fluid ? F : W;
-          if (m == 0 || j < st[0]) {
-            bad = true;
-          } else {
-            const int q = halo_floor_search(st, m, j);
-            const int len = (q + 1 < m ? of[q + 1] : end) - of[q];
-            if (j - st[q] >= len) bad = true;
-            l = (fluid ? 0 : F) + of[q] + (j - st[q]);
+        for (int wl = 0; wl < 2; ++wl) {
+          const int b0 = wl ? uws[u] : ufs[u], len = wl ? uwl[u] : ufl[u], o0 = wl ? F + uwo[u] : ufo[u];
+          for (int q = 0; q < len; ++q) {
+            const int32_t j = b0 + q;
+            if (j == (int32_t)r) continue;
+            const double2 xj2 = hx[o0 + q];
+            const double xj[2] = {xj2.x, xj2.y};
+            if (dist2_pt<double, 2, false>(xi, xj, nullptr) <= rc2) accept(j, o0 + q);
           }
         }
-        l4[u] = (uint16_t)(bad ? 0 : l);
       }
-      dst[(k >> 2) * 64] = make_ushort4(l4[0], l4[1], l4[2], l4[3]);
+    } else {
+      cg.ranges<2>(r, true, [&](int32_t b0, int32_t e0) {
+        for (int32_t j = b0; j < e0; ++j) {
+          if (j == (int32_t)r) continue;
+          if (dist2_pt<double, 2, false>(xi, pos + (int64_t)j * 2, nullptr) <= rc2) accept(j, 0);
+        }
+      });
     }
+    if (FILL && (k & 3) && (k >> 2) < chunks) {
+      const int sl = k & 3;                       // the row itself (halo: position 0) in the unused slots
+      gb.y = sl > 1 ? gb.y : (int)r; gb.z = sl > 2 ? gb.z : (int)r; gb.w = (int)r;
+      hb.y = sl > 1 ? hb.y : 0; hb.z = sl > 2 ? hb.z : 0; hb.w = 0;
+      gp[(k >> 2) * 64] = gb;
+      if (ok) hp[(k >> 2) * 64] = hb;
+    }
+    cnt[r] = k;
   }
-  if (bad) s_bad = 1;
+  int m = k;
+  for (int o = 32; o > 0; o >>= 1) {
+    const int v = __shfl_xor(m, o, 64);
+    m = v > m ? v : m;
+  }
+  if ((t & 63) == 0) s_kmax[t >> 6] = m;
   __syncthreads();
   if (t == 0) {
-    if (s_bad) {
-      halo_cnt[tile] = make_int2(-1, -1);
-    } else {
-      halo_cnt[tile] = make_int2(H, F);
-      atomicMax(hmax, (unsigned int)H);
-      atomicAdd(hmax + 1, 1u);
+    for (int w = 1; w < kHaloRows / 64; ++w) m = s_kmax[w] > m ? s_kmax[w] : m;
+    blk[2 * tile] = (unsigned int)m;
+    blk[2 * tile + 1] = ok ? (unsigned int)(F + W + 1) : 0u;
+    halo_cnt[tile] = ok ? make_int2(F + W, F) : make_int2(-1, -1);
+  }
+}
+
+// blk (ntiles pairs) and the wall rows' longest list (wall_kmax) -> *out; a
+// list that did not fit its K slots, or a row outside the grid, marks both
+// stale flags so that the steps queued on this build change nothing and the
+// host builds again.  One block of kHaloRows threads.
+static __global__ void __launch_bounds__(kHaloRows) k_halo_lists_reduce(
+    const unsigned int* __restrict__ blk, int64_t ntiles, const unsigned int* __restrict__ wall_kmax,
+    const unsigned int* __restrict__ outside, int64_t K, unsigned long long* __restrict__ stale,
+    BuildStats* __restrict__ out) {
+  unsigned int km = 0, hm = 0, used = 0;
+  for (int64_t b = threadIdx.x; b < ntiles; b += blockDim.x) {
+    const unsigned int a = blk[2 * b], h = blk[2 * b + 1];
+    km = a > km ? a : km;
+    hm = h > hm ? h : hm;
+    used += h ? 1u : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned int a = __shfl_xor(km, o, 64), h = __shfl_xor(hm, o, 64);
+    km = a > km ? a : km;
+    hm = h > hm ? h : hm;
+    used += __shfl_xor(used, o, 64);
+  }
+  __shared__ unsigned int part[3][kHaloRows / 64];
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = km; part[1][threadIdx.x >> 6] = hm; part[2][threadIdx.x >> 6] = used;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kHaloRows / 64; ++w) {
+      km = part[0][w] > km ? part[0][w] : km;
+      hm = part[1][w] > hm ? part[1][w] : hm;
+      used += part[2][w];
     }
+    const unsigned int wk = *wall_kmax;
+    km = wk > km ? wk : km;
+    const unsigned int outv = *outside;
+    out->kmax = km;
+    out->hmax = hm ? hm - 1 : 0;
+    out->used = used;
+    out->outside = outv;
+    out->K = (unsigned int)K;
+    if ((int64_t)km > K || outv) { stale[0] = 1ull; stale[1] = 1ull; }
+    __threadfence_system();
   }
 }
 

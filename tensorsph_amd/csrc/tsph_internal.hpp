@@ -87,6 +87,7 @@ struct CellSearch {
   DevBuf ustart;     // int64 (ncells + 1) first sorted slot of each cell
   DevBuf flags;      // int32 scratch
   DevBuf minmax;     // uint64 2*3 encoded min/max
+  DevBuf mm_blk;     // uint64 6 per block of the cell-coordinate pass
   DevBuf sort_tmp;   // rocprim temporary storage
   DevBuf idx_in;     // int32 0..n-1
 };

@@ -32,9 +32,13 @@ void set_error(const char* fmt, ...) {
 constexpr unsigned long long kEnc = 1ull << 62;
 
 // ---------------------------------------------------------------- cell coordinates
+// Cell of every row, and per block the smallest and largest cell per axis as
+// keys (c + kEnc) in blk[6 * block + d] / blk[6 * block + 3 + d] (~0 / 0 for
+// a block without rows), stored plainly: thousands of blocks meeting on six
+// words with atomics serialise.  k_minmax_reduce combines the records.
 template <typename T, int D>
 __global__ void k_cell_coords(const T* __restrict__ pos, int64_t n, T cutoff,
-                              int64_t* __restrict__ cells, unsigned long long* __restrict__ minmax) {
+                              int64_t* __restrict__ cells, unsigned long long* __restrict__ blk) {
   __shared__ long long smin[D][kBlock / 64], smax[D][kBlock / 64];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   long long lo[D], hi[D];
@@ -66,6 +70,8 @@ __global__ void k_cell_coords(const T* __restrict__ pos, int64_t n, T cutoff,
   }
   __syncthreads();
   if (threadIdx.x == 0) {
+    unsigned long long* o = blk + 6 * (int64_t)blockIdx.x;
+    for (int d = 0; d < 3; ++d) { o[d] = ~0ull; o[3 + d] = 0ull; }
     for (int d = 0; d < D; ++d) {
       long long a = LLONG_MAX, b = LLONG_MIN;
       for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
@@ -73,15 +79,44 @@ __global__ void k_cell_coords(const T* __restrict__ pos, int64_t n, T cutoff,
         b = smax[d][w] > b ? smax[d][w] : b;
       }
       if (a <= b) {
-        // the bounds only move outwards, so a block that cannot improve on
-        // the current value (a plain, possibly stale read) skips its atomic;
-        // otherwise every block's atomics serialise on six addresses
-        const unsigned long long ea = (unsigned long long)(a + (long long)kEnc);
-        const unsigned long long eb = (unsigned long long)(b + (long long)kEnc);
-        if (ea < __atomic_load_n(&minmax[d], __ATOMIC_RELAXED)) atomicMin(&minmax[d], ea);
-        if (eb > __atomic_load_n(&minmax[3 + d], __ATOMIC_RELAXED)) atomicMax(&minmax[3 + d], eb);
+        o[d] = (unsigned long long)(a + (long long)kEnc);
+        o[3 + d] = (unsigned long long)(b + (long long)kEnc);
       }
     }
+  }
+}
+
+// minmax[0..2] = smallest, minmax[3..5] = largest key over nblk records; one
+// block of kBlock threads.
+__global__ void __launch_bounds__(kBlock) k_minmax_reduce(const unsigned long long* __restrict__ blk,
+                                                          int64_t nblk, unsigned long long* __restrict__ minmax) {
+  unsigned long long v[6] = {~0ull, ~0ull, ~0ull, 0ull, 0ull, 0ull};
+  for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const unsigned long long lo = blk[6 * b + q], hi = blk[6 * b + 3 + q];
+      v[q] = lo < v[q] ? lo : v[q];
+      v[3 + q] = hi > v[3 + q] ? hi : v[3 + q];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const unsigned long long lo = __shfl_xor(v[q], o, 64), hi = __shfl_xor(v[3 + q], o, 64);
+      v[q] = lo < v[q] ? lo : v[q];
+      v[3 + q] = hi > v[3 + q] ? hi : v[3 + q];
+    }
+  }
+  __shared__ unsigned long long part[6][kBlock / 64];
+  if ((threadIdx.x & 63) == 0)
+    for (int q = 0; q < 6; ++q) part[q][threadIdx.x >> 6] = v[q];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    unsigned long long r = part[q][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      r = q < 3 ? (part[q][w] < r ? part[q][w] : r) : (part[q][w] > r ? part[q][w] : r);
+    minmax[q] = r;
   }
 }
 
@@ -101,10 +136,6 @@ __global__ void k_cell_coords_periodic(const T* __restrict__ pos, int64_t n, T w
     if (c < 0) c += m;
     cells[i * D + d] = c;
   }
-}
-
-__global__ void k_init_minmax(unsigned long long* mm) {
-  if (threadIdx.x < 3) { mm[threadIdx.x] = ~0ull; mm[3 + threadIdx.x] = 0ull; }
 }
 
 template <int D>
@@ -270,9 +301,11 @@ int run_search(tsph_ctx* ctx, const T* pos, int64_t n, double cutoff, const doub
   TSPH_TRY(cs.minmax.ensure(sizeof(unsigned long long) * 6 + sizeof(T) * 3 + sizeof(int64_t) * 3));
   const unsigned grid = grid_for(n);
   if (!box) {
-    k_init_minmax<<<1, 64, 0, stream>>>(cs.minmax.as<unsigned long long>());
+    TSPH_TRY(cs.mm_blk.ensure(sizeof(unsigned long long) * 6 * (size_t)grid));
     k_cell_coords<T, D><<<grid, kBlock, 0, stream>>>(pos, n, (T)cutoff, cs.cells.as<int64_t>(),
-                                                     cs.minmax.as<unsigned long long>());
+                                                     cs.mm_blk.as<unsigned long long>());
+    k_minmax_reduce<<<1, kBlock, 0, stream>>>(cs.mm_blk.as<unsigned long long>(), (int64_t)grid,
+                                              cs.minmax.as<unsigned long long>());
     TSPH_CHECK_LAUNCH();
     unsigned long long mm[6];
     TSPH_CHECK_HIP(hipMemcpyAsync(mm, cs.minmax.ptr, sizeof(mm), hipMemcpyDeviceToHost, stream));
@@ -566,7 +599,7 @@ int tsph_ctx_destroy(tsph_ctx* ctx) {
   if (!ctx) return TSPH_OK;
   CellSearch& cs = ctx->search;
   DevBuf* bufs[] = {&cs.cells, &cs.keys, &cs.skeys, &cs.order, &cs.order32, &cs.ukeys, &cs.ustart,
-                    &cs.flags, &cs.minmax, &cs.sort_tmp, &cs.idx_in, &ctx->pair_count, &ctx->pair_off,
+                    &cs.flags, &cs.minmax, &cs.mm_blk, &cs.sort_tmp, &cs.idx_in, &ctx->pair_count, &ctx->pair_off,
                     &ctx->scan_tmp, &ctx->nb_off, &ctx->nb_idx, &ctx->nb_cursor, &ctx->wnb_off,
                     &ctx->wnb_idx, &ctx->wnb_cursor, &ctx->s_rho_raw, &ctx->s_rho, &ctx->s_p,
                     &ctx->s_aux, &ctx->s_aux2, &ctx->s_aux3, &ctx->s_gr, &ctx->s_flags, &ctx->s_pack, &ctx->s_red, &ctx->s_dp,

@@ -421,6 +421,7 @@ int tile_count_pairs(tsph_sim* sim, double radius, unsigned long long* cnt, hipS
 // ------------------------------------------------------------------ host side
 template <int D, typename S>
 int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream) {
+  sim->bgrid_ok = false;   // the tile order moves the wall rows: the next slot-list build bins every row
   static_assert(D == 2, "cell tiles are 2-D");
   tsph_ctx* ctx = sim->ctx;
   const int64_t n = sim->n, nt = sim->nt;

@@ -179,6 +179,15 @@ class WcsphSimulation:
         _lib.check(_lib.lib().tsph_sim_layout(self.handle, c))
         return {"layout": c[0], "halo_tiles": c[1], "tiles": c[2], "halo_max": c[3]}
 
+    def slot_width(self, width: int | None = None) -> int:
+        """The neighbour lists' slot width of the last build; a width sets the
+        one the next build starts from (it widens by itself when a list does
+        not fit)."""
+        cur = _i64(0)
+        _lib.check(_lib.lib().tsph_sim_slot_width(self.handle, -1 if width is None else int(width),
+                                                  C.byref(cur)))
+        return cur.value
+
     def time_state(self) -> dict:
         """The coming step's time state: simulated time at its start, the
         step-size variable, its own size, the previous step's size."""

@@ -132,3 +132,29 @@ def test_halo_layout_arguments():
         assert sim.layout()["layout"] == 1 and sim.layout()["tiles"] == 0
     finally:
         sim.close()
+
+
+@pytest.mark.parametrize("halo", [True, False])
+def test_lists_that_outgrow_their_slots_are_rebuilt_wider(halo):
+    """A build whose lists do not fit the slot width widens and builds again:
+    with the width cut to 8 after the first build, the run gives the bits and
+    the list-rebuild count of an unconstrained run (the stepping loop's repeat
+    of a build on unchanged positions is not counted)."""
+    cfg, make = _dam(0.004, "fp64", skin_factor=0.1)
+    out, builds, widths = [], [], []
+    for cut in (False, True):
+        sim = make()
+        try:
+            sim.configure(tiles=False, halo=halo)
+            sim.step(1)
+            if cut:
+                assert sim.slot_width(8) > 8
+            sim.step(80)
+            out.append(sim.read())
+            builds.append(sim.stats()["rebuilds"])
+            widths.append(sim.slot_width())
+        finally:
+            sim.close()
+    _same(out[0], out[1])
+    assert builds[0] == builds[1] and builds[0] >= 3
+    assert widths[1] > 8
