@@ -153,9 +153,10 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid b
   __shared__ unsigned long long key[kHaloCand];
   __shared__ int16_t cand_u[kHaloCand];
   __shared__ int ufs[kHaloUniq], ufl[kHaloUniq], ufo[kHaloUniq], uws[kHaloUniq], uwl[kHaloUniq], uwo[kHaloUniq];
-  __shared__ double2 hx[kHaloCap];
+  __shared__ float2 hx[kHaloCap];                      // halo positions relative to the tile's first row
   __shared__ int wtot[4];
   __shared__ int s_kmax[4];
+  __shared__ float s_ext[4];
   const int t = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const int64_t r = tile * kHaloRows + t;
@@ -263,14 +264,22 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid b
         }
         jq[q] = j;
       }
+      const double ox = pos[tile * kHaloRows * 2], oy = pos[tile * kHaloRows * 2 + 1];
       double2 xq[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) xq[q] = make_double2(pos[(int64_t)jq[q] * 2], pos[(int64_t)jq[q] * 2 + 1]);
+      float ext = 0.f;
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
         const int s2 = t + q * kHaloRows;
-        if (s2 < H) hx[s2] = xq[q];
+        if (s2 < H) {
+          const float2 v = make_float2((float)(xq[q].x - ox), (float)(xq[q].y - oy));
+          hx[s2] = v;
+          ext = fmaxf(ext, fmaxf(fabsf(v.x), fabsf(v.y)));
+        }
       }
+      for (int o = 32; o > 0; o >>= 1) ext = fmaxf(ext, __shfl_xor(ext, o, 64));
+      if ((t & 63) == 0) s_ext[t >> 6] = ext;
       __syncthreads();
     }
   }
@@ -297,18 +306,41 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid b
       ++k;
     };
     if (ok) {
+      // FP32 test on the tile-relative positions: below lo2 accepts, above
+      // hi2 rejects, the band between (which holds the FP32 rounding error of
+      // the test) is decided by the FP64 test on the global positions.  The
+      // relative coordinates carry at most e0 = 2^-24 ext of rounding, a
+      // difference at most e = 2 e0 + 2^-24 1.01 rc, and the squared distance
+      // near rc^2 at most 2 sqrt(2) rc e + 2 e^2 + 4.4 2^-24 rc^2; the band
+      // is twice that.
+      const float ext = fmaxf(fmaxf(s_ext[0], s_ext[1]), fmaxf(s_ext[2], s_ext[3]));
+      const double rc = sqrt(rc2);
+      const double e = 2.0 * ldexp((double)ext, -24) + ldexp(1.01 * rc, -24);
+      const double band = 2.0 * (2.0 * 1.4142135623730951 * rc * e + 2.0 * e * e + 4.4 * ldexp(rc2, -24));
+      const float lo2 = (float)(rc2 - band) * (1.0f - 1.2e-7f), hi2 = (float)(rc2 + band) * (1.0f + 1.2e-7f);
+      const float2 xf = make_float2((float)(xi[0] - pos[tile * kHaloRows * 2]),
+                                    (float)(xi[1] - pos[tile * kHaloRows * 2 + 1]));
       for (int o = 0; o < NOFF; ++o) {
         const int u = cand_u[hidx * NOFF + o];
         if (u < 0) continue;
 #pragma unroll
         for (int wl = 0; wl < 2; ++wl) {
           const int b0 = wl ? uws[u] : ufs[u], len = wl ? uwl[u] : ufl[u], o0 = wl ? F + uwo[u] : ufo[u];
-          for (int q = 0; q < len; ++q) {
-            const int32_t j = b0 + q;
-            if (j == (int32_t)r) continue;
-            const double2 xj2 = hx[o0 + q];
-            const double xj[2] = {xj2.x, xj2.y};
-            if (dist2_pt<double, 2, false>(xi, xj, nullptr) <= rc2) accept(j, o0 + q);
+          for (int q0 = 0; q0 < len; q0 += 4) {
+            float2 cq[4];
+#pragma unroll
+            for (int uu = 0; uu < 4; ++uu) cq[uu] = hx[o0 + (q0 + uu < len ? q0 + uu : len - 1)];
+#pragma unroll
+            for (int uu = 0; uu < 4; ++uu) {
+              const int q = q0 + uu;
+              const int32_t j = b0 + q;
+              if (q >= len || j == (int32_t)r) continue;
+              const float dx = xf.x - cq[uu].x, dy = xf.y - cq[uu].y;
+              const float d2 = dx * dx + dy * dy;
+              bool in = d2 < lo2;
+              if (!in && d2 <= hi2) in = dist2_pt<double, 2, false>(xi, pos + (int64_t)j * 2, nullptr) <= rc2;
+              if (in) accept(j, o0 + q);
+            }
           }
         }
       }
