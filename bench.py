@@ -7,15 +7,22 @@ c0 = 10 sqrt(2 g H) with H = 2 m, dt at acoustic Courant 0.3, nu = 0.01,
 no density diffusion, Verlet skin 0.25 h.  dx = sqrt(2 / N) with N = 1.6 M
 gives 1,598,472 fluid + 26,847 wall particles.
 
-One step = one full pass of the particle pipeline (density + Tait, Adami wall
-pressure, pressure + viscous force, symplectic Euler + clamp, and the
-amortised neighbour-list rebuilds) over all fluid particles.  The state is
+The flow is first developed untimed (the collapse runs to t = --develop-time,
+about 0.25 s, where the front moves at several m/s and the lists go stale
+every few steps); the timed window of --steps steps then includes every list
+rebuild that falls in it.  One step = one full pass of the particle pipeline
+(density + Tait, Adami wall pressure, pressure + viscous force, symplectic
+Euler + clamp) over all fluid particles, plus the rebuilds.  The state is
 resident in HBM when the timed region starts.
 
 Prints one JSON line; ``value`` is fluid particle-updates/s of the
-FP32-compensated pipeline.  ``roofline`` prices the force sweep (the
-dominant kernel) at its algorithmic bytes over its HIP-event time;
-``cpu_baseline`` times the FP64 CPU restatement on a bounded sample.
+FP32-compensated pipeline in the window.  ``roofline`` prices the force
+sweep (the dominant kernel) at its algorithmic bytes over its mean HIP-event
+duration, sampled on every 10th step of a second window; ``cpu_baseline``
+times the multi-threaded CPU port (oracle/tsph_cpu_omp.c) from the same
+developed state on the host's cores and on one thread.  Blocks for the 3-D
+shear wave (192^3) and the 3-D box case (20 M), the FP32c-vs-FP64 error on a
+common state and the served delta-plus batch ride on the same line.
 """
 from __future__ import annotations
 
@@ -31,9 +38,16 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0
+# δ⁺ step (FP64), compulsory bytes per particle-update: the state x, v (2 x 16 B),
+# rho, m (2 x 8 B) read and x, v, rho (40 B) written, plus the step's
+# intermediates written once and read once (new velocity 16 B, lambda 8 B,
+# the renormalisation matrix 32 B, normal 16 B, band flags 8 B): 248 B, taken
+# as 250 B
+DP_BYTES_PER_UPDATE = 250
 PROFILE = os.path.join(HERE, "profiles", "k_tile_force_dambreak_1p6M_fp32c.md")
 PROFILE_LISTS = os.path.join(HERE, "profiles", "k_step_force_dambreak_1p6M_fp32c.md")
 PROFILE_3D = os.path.join(HERE, "profiles", "k_step_force_kleefsman_5M_fp32c.md")
+PROFILE_HALO = os.path.join(HERE, "profiles", "k_step_force_halo_dambreak_1p6M_fp32c.md")
 
 
 def profiled_traffic(nf: int, path: str = PROFILE, kernel: str = "k_tile_force"):
@@ -53,8 +67,11 @@ def profiled_traffic(nf: int, path: str = PROFILE, kernel: str = "k_tile_force")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20,
+                    help="untimed steps after the flow is developed (and the layout chosen)")
+    ap.add_argument("--develop-time", type=float, default=0.25,
+                    help="dam break: physical time run untimed before the window")
     ap.add_argument("--fluid", type=float, default=1.6e6, help="target fluid particle count")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--diffusion", type=float, default=0.0,
@@ -75,10 +92,13 @@ def parse():
     ap.add_argument("--workload", choices=("dambreak", "shear", "kleefsman"), default="dambreak",
                     help="dambreak: the 2-D dam break line (default); shear: 3-D periodic shear "
                          "wave; kleefsman: 3-D dam break against a box obstacle")
-    ap.add_argument("--shear-n", type=int, default=128, help="lattice edge of the shear-wave cube")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3-D blocks of the dam-break line")
+    ap.add_argument("--block-steps", type=int, default=40, help="timed steps of each 3-D block")
+    ap.add_argument("--block-warmup", type=int, default=10)
+    ap.add_argument("--shear-n", type=int, default=192, help="lattice edge of the shear-wave cube")
     ap.add_argument("--decay-n", type=int, default=64,
                     help="lattice edge of the full-length decay run of the shear workload (0: skip)")
-    ap.add_argument("--fluid3d", type=float, default=5e6, help="fluid particles of the box case")
+    ap.add_argument("--fluid3d", type=float, default=20e6, help="fluid particles of the box case")
     ap.add_argument("--t-end", type=float, default=0.0,
                     help="box case: also integrate both precisions to this physical time and "
                          "compare the force / moment histories on the box")
@@ -100,11 +120,11 @@ def dam_break_state(n_target: float):
     return cfg, pos, mass, wall, lattice_mass_unit(cfg)
 
 
-def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0):
+def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0, vel=None):
     import numpy as np
     from tensorsph_amd.engine import WcsphSimulation
     h = cfg.h_factor * cfg.dx
-    return WcsphSimulation(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
+    return WcsphSimulation(pos, np.zeros_like(pos) if vel is None else vel, mass, wall, h=h, rho0=cfg.rho0,
                            sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
                            viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=wmass,
                            skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0],
@@ -112,55 +132,129 @@ def make_sim(cfg, pos, mass, wall, wmass, precision, diffusion=0.0):
                            precision=precision, density_diffusion=diffusion)
 
 
-def time_gpu(sim, steps, warmup, tile=None):
-    import torch
-    from tensorsph_amd.autotune import tune_tiles
-    if tile in ("lists", "halo"):
-        sim.configure(tiles=False, halo=tile == "halo")
-        tuned = {"best": tile, "ms": {}}
-        sim.step(max(warmup, 1))
-    elif tile is not None:
-        sim.configure(tiles=True, tile=tile)
-        tuned = {"best": tuple(tile), "ms": {}}
-        sim.step(max(warmup, 1))
+def use_layout(sim, layout):
+    if layout in ("lists", "halo"):
+        sim.configure(tiles=False, halo=layout == "halo")
     else:
-        sim.step(max(warmup // 2, 1))
-        tuned = tune_tiles(sim, steps=4, warm=1)     # trial steps count as warm-up
-        sim.step(max(warmup - warmup // 2, 1))
+        sim.configure(tiles=True, tile=layout)
+
+
+def develop(sim, steps: int, label: str) -> None:
+    """Untimed steps, in chunks with a progress line after each."""
+    import torch
+    done, t0 = 0, time.perf_counter()
+    while done < steps:
+        k = min(5000, steps - done)
+        sim.step(k)
+        done += k
+        torch.cuda.synchronize()
+        print(f"{label}: developed {done}/{steps} steps, {time.perf_counter() - t0:.1f} s",
+              file=sys.stderr, flush=True)
+
+
+def timed_window(sim, steps: int):
+    """Wall time of ``steps`` steps (list rebuilds included) and the rebuilds in them."""
+    import torch
     torch.cuda.synchronize()
-    sim.set_timing(True)
+    b0 = sim.stats()["rebuilds"]
     t0 = time.perf_counter()
     sim.step(steps)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    sweep_ms, nsteps = sim.sweep_times()
+    return time.perf_counter() - t0, sim.stats()["rebuilds"] - b0
+
+
+def kernel_window(sim, steps: int, every: int = 10):
+    """Mean HIP-event durations (ms) of the density, wall and force sweeps over
+    every ``every``-th step of a window of ``steps`` steps, and the count."""
+    import torch
+    sim.set_timing(True, every)
+    sim.step(steps)
+    torch.cuda.synchronize()
+    ms, n = sim.sweep_times()
     sim.set_timing(False)
-    return wall, sweep_ms, nsteps, tuned
+    return [x / max(n, 1) for x in ms], n
 
 
-def cpu_baseline(seconds: float):
-    """FP64 oracle loop (C, one thread) on the same dam break at a 25k-particle sample."""
+def precision_error(cfg, mass, wall, wmass, state, diffusion=0.0) -> dict:
+    """FP32-compensated against FP64 on one common state (the developed FP64
+    state): one step from it in each precision; the density and acceleration
+    of that step's evaluation, largest difference over the FP64 field's
+    largest magnitude."""
+    import numpy as np
+    pos, vel = state
+    out = {}
+    for prec in ("fp32", "fp64"):
+        sim = make_sim(cfg, pos, mass, wall, wmass, prec, diffusion, vel=vel)
+        try:
+            sim.step(1)
+            r = sim.read()
+            out[prec] = {k: r[k].double().cpu().numpy() for k in ("density", "acceleration")}
+        finally:
+            sim.close()
+
+    def scaled(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    return {"density_max_rel": scaled(out["fp32"]["density"], out["fp64"]["density"]),
+            "acceleration_max_over_max": scaled(out["fp32"]["acceleration"], out["fp64"]["acceleration"]),
+            "state": "the FP64 run's state after its timed window; one step in each precision"}
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(cap)) if cap and cap.isdigit() else n
+
+
+def cpu_baseline(cfg, wmass, wall, state, seconds: float) -> dict:
+    """The multi-threaded CPU port of the same FP64 step (oracle/tsph_cpu_omp.c,
+    pinned against the oracle's Stepper2D), started from the GPU run's
+    developed state: all host threads for ``seconds``, then one thread for
+    half as long.  The first list build is outside the timing; later ones
+    are inside."""
     import numpy as np
     from oracle import oracle as orc
-    cfg, pos, mass, wall, wmass = dam_break_state(2.5e4)
     h = cfg.h_factor * cfg.dx
-    st = orc.Stepper2D(pos, np.zeros_like(pos), mass, wall, h=h, rho0=cfg.rho0,
-                       c0=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
-                       nu=cfg.viscosity, dt=cfg.dt, wall_mass=wmass,
-                       skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0],
-                       hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx)
-    st.step()                      # first list build outside the timing
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < seconds:
-        st.step()
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": pos.shape[0] * steps / dt, "unit": "particle-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"FP64 C restatement, 1 thread, dam break at {pos.shape[0]} fluid + "
-                      f"{wall.shape[0]} wall particles (dx={cfg.dx:.4g}), {steps} steps in "
-                      f"{dt:.1f} s, list rebuilds included"}
+    pos, vel, mass = state
+    kw = dict(h=h, rho0=cfg.rho0, c0=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
+              nu=cfg.viscosity, dt=cfg.dt, wall_mass=wmass, skin=cfg.pair_skin_factor * h,
+              lo=[0.0, 0.0], hi=[cfg.pool_width, cfg.tank_top()], tol=0.25 * cfg.dx)
+    out = {}
+    for label, threads, budget in (("all", host_threads(), seconds), ("one", 1, 0.5 * seconds)):
+        st = orc.OmpStepper2D(pos, vel, mass, wall, threads=threads, **kw)
+        try:
+            st.step(1)                 # first list build
+            t0 = time.perf_counter()
+            steps = 0
+            while time.perf_counter() - t0 < budget:
+                st.step(1)
+                steps += 1
+            dt = time.perf_counter() - t0
+            rebuilds = st.read()["rebuilds"] - 1
+        finally:
+            st.close()
+        out[label] = {"value": pos.shape[0] * steps / dt, "threads": st.threads, "steps": steps,
+                      "seconds": dt, "list_rebuilds": rebuilds}
+    return {"value": out["all"]["value"], "unit": "particle-updates/s", "cores": out["all"]["threads"],
+            "kind": "port", "cpu_model": cpu_model(),
+            "one_thread": {"value": out["one"]["value"], "steps": out["one"]["steps"],
+                           "seconds": out["one"]["seconds"]},
+            "sample": f"FP64 OpenMP port of the step (oracle/tsph_cpu_omp.c), the same dam break at "
+                      f"{pos.shape[0]} fluid + {wall.shape[0]} wall particles from the GPU run's "
+                      f"developed state: {out['all']['steps']} steps in {out['all']['seconds']:.1f} s on "
+                      f"{out['all']['threads']} threads ({out['all']['list_rebuilds']} list rebuilds), "
+                      f"{out['one']['steps']} steps in {out['one']['seconds']:.1f} s on 1 thread"}
 
 
 def serving_bench(n_requests: int, steps: int, n_seq: int, n_api: int) -> dict:
@@ -214,6 +308,11 @@ def serving_bench(n_requests: int, steps: int, n_seq: int, n_api: int) -> dict:
                                             "adaptive dt from max|v|, PairListCache(4 dx, 0.4 dx), "
                                             "delta_plus_step every step"),
         },
+        "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                     "algorithmic_bytes_per_update": DP_BYTES_PER_UPDATE,
+                     "achieved": summary["particle_updates_per_s"] * DP_BYTES_PER_UPDATE / 1e9,
+                     "frac": summary["particle_updates_per_s"] * DP_BYTES_PER_UPDATE / 1e9 / HBM_PEAK_GBS,
+                     "how": "served particle-updates/s times the frozen FP64 δ⁺ bytes per update"},
         "batched_over_engine_alone": summary["sims_per_s"] / seq_rate,
         "batched_over_single_case_api": summary["sims_per_s"] / api["sims_per_s"] if api else None,
     }
@@ -436,44 +535,73 @@ def main():
     if args.workload == "kleefsman":
         print(json.dumps(kleefsman_bench(args)))
         return
+    print(json.dumps(dam_break_bench(args)))
+
+
+def dam_break_bench(args) -> dict:
+    import numpy as np
+    from tensorsph_amd.autotune import tune_tiles
     cfg, pos, mass, wall, wmass = dam_break_state(args.fluid)
     nf, nw = pos.shape[0], wall.shape[0]
     h = cfg.h_factor * cfg.dx
-
+    dev_steps = int(round(args.develop_time / cfg.dt))
     if args.tile is None or args.tile.lower() in ("lists", "halo"):
-        tile = args.tile and args.tile.lower()
+        layout = args.tile and args.tile.lower()
     else:
-        tile = tuple(int(v) for v in args.tile.lower().split("x"))
-    if args.diffusion > 0.0 and tile != "halo":
-        tile = "lists"
-    sim = make_sim(cfg, pos, mass, wall, wmass, "fp32", args.diffusion)
-    wall_s, sweep_ms, nsteps, tuned = time_gpu(sim, args.steps, args.warmup, tile)
-    pairs = sim.count_pairs(2.0 * h)
-    stats = sim.stats()
-    sim.close()
-    del sim
-    value = nf * args.steps / wall_s
-    ms_step = 1000.0 * wall_s / args.steps
-
-    # algorithmic bytes per launch of the force sweep (sweep B): reads x, v (FP64,
-    # 2 x 16 B), m, rho (4 B each), writes x, v (2 x 16 B) per fluid particle,
-    # plus each wall particle's x, m, rho, p read once (16 + 3 x 4 B)
+        layout = tuple(int(v) for v in args.tile.lower().split("x"))
+    if args.diffusion > 0.0 and layout not in (None, "lists", "halo"):
+        layout = "halo"
+    runs = {}
+    tuned = {"best": layout, "ms": {}}
+    for prec in ("fp32",) + (() if args.no_fp64 else ("fp64",)):
+        sim = make_sim(cfg, pos, mass, wall, wmass, prec, args.diffusion)
+        try:
+            develop(sim, dev_steps, f"dam break {prec}")
+            warm = dev_steps
+            if prec == "fp32" and layout is None:
+                # the layout tried on the developed flow (trial steps advance it)
+                tuned = tune_tiles(sim, candidates=("halo", "lists"), steps=20, warm=2)
+                warm += 2 * 22
+            else:
+                use_layout(sim, tuned["best"])
+            sim.step(max(args.warmup, 1))
+            warm += max(args.warmup, 1)
+            t_start = float(sim.time_state()["elapsed"])
+            wall_s, rebuilds = timed_window(sim, args.steps)
+            sweep, ntimed = kernel_window(sim, min(args.steps, 1000))
+            pairs = sim.count_pairs(2.0 * h) if prec == "fp32" else None
+            st = sim.read()
+            state = (st["position"].double().cpu().numpy(), st["velocity"].double().cpu().numpy())
+        finally:
+            sim.close()
+        runs[prec] = {"wall_s": wall_s, "rebuilds": rebuilds, "sweep": sweep, "timed_launches": ntimed,
+                      "pairs": pairs, "state": state, "warm": warm, "t_start": t_start}
+        print(f"dam break {prec}: {1000 * wall_s / args.steps:.4f} ms/step, {rebuilds} rebuilds in "
+              f"{args.steps} steps", file=sys.stderr, flush=True)
+    r32 = runs["fp32"]
+    value = nf * args.steps / r32["wall_s"]
+    ms_step = 1000.0 * r32["wall_s"] / args.steps
+    # algorithmic bytes per launch of the force sweep: reads x, v (FP64, 2 x 16 B),
+    # m, rho (4 B each), writes x, v (2 x 16 B) per fluid particle, plus each wall
+    # particle's x, m, rho, p read once (16 + 3 x 4 B)
     s = 4
     b_force = nf * (32 * 2 + 2 * s) + nw * (16 + 3 * s)
-    b_density = nf * (16 + 2 * s) + nw * (16 + s)
-    force_ms = sweep_ms[2] / max(nsteps, 1)
-    on_lists = args.diffusion > 0.0 or tuned["best"] in ("lists", "halo")
-    kernel, prof = ("k_step_force", PROFILE_LISTS) if on_lists else ("k_tile_force", PROFILE)
+    force_ms = r32["sweep"][2]
     achieved = b_force / (force_ms * 1e-3) / 1e9 if force_ms > 0 else 0.0
+    # step bytes of the compulsory two-sweep model: 40 D + 4 s per fluid particle,
+    # 24 D + 2 s per wall particle
     step_bytes = nf * (40 * 2 + 4 * s) + nw * (24 * 2 + 2 * s)
-
+    on_halo = tuned["best"] == "halo"
+    kernel = "k_step_force" if tuned["best"] in ("lists", "halo") else "k_tile_force"
+    prof = PROFILE_HALO if on_halo else (PROFILE_LISTS if kernel == "k_step_force" else PROFILE)
+    traffic = None if args.diffusion else profiled_traffic(nf, prof, kernel)
     result = {
         "metric": "particle-updates/s (2-D WCSPH dam break, FP32-compensated) + HBM GB/s",
         "value": value,
         "unit": "particle-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
-        "warmup": args.warmup,
+        "warmup": r32["warm"],
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
@@ -482,45 +610,57 @@ def main():
         "data": "synthetic: deterministic dam-break lattice (no dataset)",
         "config": {"workload": f"2-D dam break, {nf} fluid + {nw} wall particles, "
                                f"dx={cfg.dx:.5g} m, h=1.3dx, c0={cfg.sound_speed:.4g} m/s, "
-                               f"dt={cfg.dt:.4g} s, nu={cfg.viscosity}"
+                               f"dt={cfg.dt:.4g} s, nu={cfg.viscosity}, skin 0.25h"
                                + (f", Molteni δ-diffusion δ={args.diffusion}" if args.diffusion else ""),
                    "precision": "pair arithmetic FP32 with Kahan sums; positions, velocities "
                                 "and integration FP64",
                    "fluid": nf, "wall": nw, "parallelism": "single GPU"},
-        "pair_interactions_per_s": pairs * args.steps / wall_s,
-        "pairs_within_2h": pairs,
-        "list_rebuilds_in_run": stats["rebuilds"],
-        "tile_shape": {"chosen": tuned["best"], "trial_ms_per_step": tuned["ms"]},
-        "algorithmic_GBps_step": step_bytes * args.steps / wall_s / 1e9,
-        "sweep_ms_per_step": {"density": sweep_ms[0] / max(nsteps, 1),
-                              "wall": sweep_ms[1] / max(nsteps, 1),
-                              "force_integrate": force_ms},
+        "window": {"start_time_s": r32["t_start"], "untimed_steps_before": r32["warm"],
+                   "list_rebuilds": r32["rebuilds"],
+                   "steps_per_rebuild": args.steps / r32["rebuilds"] if r32["rebuilds"] else None,
+                   "how": "the collapse developed untimed to start_time_s; the window's wall clock "
+                          "includes every list rebuild in it"},
+        "pair_interactions_per_s": r32["pairs"] * args.steps / r32["wall_s"],
+        "pairs_within_2h": r32["pairs"],
+        "layout": {"chosen": tuned["best"], "trial_ms_per_step": tuned["ms"]},
+        "algorithmic_GBps_step": step_bytes * args.steps / r32["wall_s"] / 1e9,
+        "algorithmic_bytes_per_step": step_bytes,
+        "step_roofline_frac": step_bytes * args.steps / r32["wall_s"] / 1e9 / HBM_PEAK_GBS,
+        "sweep_ms_per_launch": {"density": r32["sweep"][0], "wall": r32["sweep"][1],
+                                "force_integrate": force_ms, "timed_launches": r32["timed_launches"],
+                                "how": "HIP events around each sweep on every 10th step of a "
+                                       "second window on the stepping stream"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.diffusion else profiled_traffic(nf, prof, kernel),
-                     "traffic_source": None if args.diffusion else os.path.relpath(prof, HERE),
-                     "kernel": (f"{kernel} (slot lists: force + viscosity + integrate)" if on_lists
-                                else f"{kernel} (LDS tiles: force + viscosity + integrate)"),
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": None if traffic is None else os.path.relpath(prof, HERE),
+                     "kernel": f"{kernel} ({tuned['best']} layout: force + viscosity + integrate)",
                      "algorithmic_bytes_per_launch": b_force},
     }
-    if not args.no_fp64:
-        sim = make_sim(cfg, pos, mass, wall, wmass, "fp64", args.diffusion)
-        w64, sw64, n64, tuned64 = time_gpu(sim, args.steps, args.warmup, tile)
-        sim.close()
-        del sim
-        v64 = nf * args.steps / w64
-        f64_ms = sw64[2] / max(n64, 1)
+    if "fp64" in runs:
+        r64 = runs["fp64"]
+        v64 = nf * args.steps / r64["wall_s"]
         b64 = nf * (32 * 2 + 2 * 8) + nw * (16 + 3 * 8)
-        result["fp64"] = {"value": v64, "ms_per_step": 1000.0 * w64 / args.steps,
-                          "force_sweep_ms": f64_ms,
+        f64_ms = r64["sweep"][2]
+        result["fp64"] = {"value": v64, "ms_per_step": 1000.0 * r64["wall_s"] / args.steps,
+                          "list_rebuilds": r64["rebuilds"], "start_time_s": r64["t_start"],
+                          "sweep_ms_per_launch": {"density": r64["sweep"][0], "wall": r64["sweep"][1],
+                                                  "force_integrate": f64_ms},
                           "force_sweep_GBps": b64 / (f64_ms * 1e-3) / 1e9 if f64_ms > 0 else 0.0}
         result["fp32c_over_fp64"] = value / v64
+        result["fp32c_vs_fp64_error"] = precision_error(cfg, mass, wall, wmass, r64["state"], args.diffusion)
     if not args.no_serve:
         result["serving"] = serving_bench(args.serve_requests, args.serve_steps, args.serve_sequential,
                                           args.serve_sequential_api)
+    if not args.no_3d:
+        blk = argparse.Namespace(**vars(args))
+        blk.steps, blk.warmup = args.block_steps, args.block_warmup
+        blk.decay_n, blk.t_end = 0, 0.0
+        result["shear_3d"] = shear_bench(blk)
+        result["box_3d"] = kleefsman_bench(blk)
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
-    print(json.dumps(result))
+        st = runs["fp64"]["state"] if "fp64" in runs else runs["fp32"]["state"]
+        result["cpu_baseline"] = cpu_baseline(cfg, wmass, wall, (st[0], st[1], mass), args.cpu_seconds)
+    return result
 
 
 if __name__ == "__main__":

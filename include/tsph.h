@@ -466,8 +466,9 @@ int tsph_sim_slot_width(tsph_sim* sim, int64_t width, int64_t* current);
 int tsph_sim_layout(tsph_sim* sim, int64_t* out);
 /* Sweep timing with HIP events on the stepping stream.  Returns the summed
  * milliseconds of the density, wall and force+integrate sweeps and the number
- * of timed steps since the last reset; enable = 1/0 resets the sums and turns
- * timing on/off, enable = -1 only reads. */
+ * of timed steps since the last reset; enable = k > 0 resets the sums and
+ * times every k-th step (1: every step), 0 resets and turns timing off,
+ * -1 only reads. */
 int tsph_sim_timing(tsph_sim* sim, int enable, double* sweep_ms, int64_t* steps);
 /* Number of unordered pairs with separation <= radius in the current state
  * (radius <= the list's build radius); synchronises. */

@@ -603,3 +603,65 @@ def tvc_shift(pos, volume, h, n_fluid, pairs, *, alpha=0.05, slope=None, max_shi
         over = nrm > max_shift
         shift = np.where(over[:, None], shift * (max_shift / np.maximum(nrm, 1e-30))[:, None], shift)
     return g, shift
+
+
+# ---------------------------------------------------------------------------
+# Multi-threaded CPU port of the 2-D dam-break step (tsph_cpu_omp.c): the
+# benchmark's CPU baseline, the same step as Stepper2D (default rhs branch)
+# formulated per row so that it runs on every host core.
+
+OMP_LIB_PATH = HERE / "libtsph_cpu_omp.so"
+_omp = None
+
+
+class OmpParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("rho0", C.c_double), ("c0", C.c_double), ("gamma", C.c_double),
+                ("gravity", C.c_double), ("nu", C.c_double), ("dt", C.c_double), ("skin", C.c_double),
+                ("wall_mass", C.c_double), ("lo", C.c_double * 2), ("hi", C.c_double * 2),
+                ("tol", C.c_double)]
+
+
+def omp_lib():
+    global _omp
+    if _omp is None:
+        if not OMP_LIB_PATH.exists():
+            build()
+        _omp = C.CDLL(str(OMP_LIB_PATH))
+        _omp.omp_sim_create.restype = C.c_void_p
+        _omp.omp_sim_create.argtypes = [C.POINTER(OmpParams), C.c_int64, C.c_int64, _dp, _dp, _dp, _dp]
+        _omp.omp_sim_step.argtypes = [C.c_void_p, C.c_int64]
+        _omp.omp_sim_read.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _lp]
+        _omp.omp_sim_destroy.argtypes = [C.c_void_p]
+        _omp.omp_threads.argtypes = [C.c_int]
+        _omp.omp_threads.restype = C.c_int
+    return _omp
+
+
+class OmpStepper2D:
+    """Stepper2D's step (no opt-in branches) on ``threads`` host threads."""
+
+    def __init__(self, pos, vel, mass, wall, *, h, rho0, c0, gamma, gravity, nu, dt, wall_mass, skin,
+                 lo, hi, tol, threads=0):
+        self.threads = omp_lib().omp_threads(int(threads))
+        pos = np.ascontiguousarray(pos, np.float64); vel = np.ascontiguousarray(vel, np.float64)
+        mass = np.ascontiguousarray(mass, np.float64); wall = np.ascontiguousarray(wall, np.float64)
+        self.n = pos.shape[0]
+        P = OmpParams(h, rho0, c0, gamma, gravity, nu, dt, skin, wall_mass,
+                      (C.c_double * 2)(*lo), (C.c_double * 2)(*hi), tol)
+        self._h = omp_lib().omp_sim_create(C.byref(P), self.n, wall.shape[0], _d(pos), _d(vel), _d(mass),
+                                           _d(wall))
+
+    def step(self, k=1):
+        omp_lib().omp_sim_step(self._h, int(k))
+
+    def read(self):
+        pos = np.zeros((self.n, 2)); vel = np.zeros((self.n, 2)); rho = np.zeros(self.n)
+        acc = np.zeros((self.n, 2)); rb = C.c_int64(0)
+        omp_lib().omp_sim_read(self._h, _d(pos), _d(vel), _d(rho), _d(acc), C.byref(rb))
+        return {"position": pos, "velocity": vel, "density": rho, "acceleration": acc,
+                "rebuilds": rb.value}
+
+    def close(self):
+        if self._h:
+            omp_lib().omp_sim_destroy(self._h)
+            self._h = None

@@ -1069,11 +1069,11 @@ int build_halo_lists(tsph_sim* sim, const double* npos, double rc, size_t rec_by
     if (ntiles > 0) {
       if (K > 0)
         k_halo_lists2<true><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
-            sim_cells(sim), sim->bgrid, npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
+            sim_cells(sim), npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
             sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
       else
         k_halo_lists2<false><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
-            sim_cells(sim), sim->bgrid, npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
+            sim_cells(sim), npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
             sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
     }
     if (nt > n) {
@@ -1346,7 +1346,7 @@ int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
   const int r = (int)(k % kRing);
   k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1, HaloView{}, NbHalo{});
   TSPH_CHECK_LAUNCH();
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
   return TSPH_OK;
 }
@@ -1364,7 +1364,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const NbSlots nbr = sim_slots(sim);
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
   // halo mode (2-D): the density and force sweeps stage their tiles' halos
   constexpr bool kHalo2 = D == 2;
   const bool halo = kHalo2 && sim->tiles == 2 && sim->halo_ok;
@@ -1391,9 +1391,9 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   } else {
     k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1, hv, hl);
   }
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
   st.vel_out = sim->vel[sim->cur ^ 1].as<double>();
@@ -1437,7 +1437,7 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   else
     k_step_force<D, S, false, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, sim->acc.as<S>(), st, fl, slot,
                                                                  0.5 * sim->prm.skin, hv, hl);
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   if (meters) k_reduce_sums<<<1, 256, 0, stream>>>(sim->block_sums.as<double>(), gx, sim->sums_out.as<double>(), fl, slot);
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;
@@ -1483,6 +1483,9 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   WcsphArgs<S> a = sim_args<D, S>(sim);
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
+  // sweep timing on every timing-th step (events cost a little on each step they time)
+  sim->tstep = sim->timing > 0 && k % sim->timing == 0;
+  sim->timed[r] = sim->tstep;
   if (meters) k_reset_meters<<<1, 64, 0, stream>>>(fl, slot);
   bool tiled = false;
   if constexpr (D == 2) {
@@ -1521,7 +1524,7 @@ int sim_consume(tsph_sim* sim, int64_t k, bool meters, bool* stale_after, bool* 
   if (rec->stale_in) return TSPH_OK;   // skipped: nothing to record
   if (sim->finished) return TSPH_OK;   // after an adaptive run's final sample: no-op steps
   sim->steps_done = k + 1;
-  if (sim->timing) {
+  if (sim->timed[r]) {
     for (int q = 0; q < 3; ++q) {
       float ms = 0.0f;
       TSPH_CHECK_HIP(hipEventElapsedTime(&ms, sim->tev[r][q], sim->tev[r][q + 1]));

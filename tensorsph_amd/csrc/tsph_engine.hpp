@@ -93,7 +93,9 @@ struct tsph_sim {
   tsph::StepRecord* ring = nullptr;       // pinned
   hipEvent_t ev[tsph::kRing] = {};
   // per-sweep timing (HIP events around each sweep on the stepping stream)
-  int timing = 0;
+  int timing = 0;                   // > 0: time the sweeps of every timing-th step
+  bool tstep = false;               // the step being queued is timed
+  bool timed[tsph::kRing] = {};
   hipEvent_t tev[tsph::kRing][4] = {};
   double sweep_ms[3] = {0, 0, 0};
   int64_t sweep_count = 0;

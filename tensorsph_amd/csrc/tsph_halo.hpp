@@ -3,17 +3,20 @@
 // The rows of a workgroup (kHaloRows consecutive fluid rows, a tile) are
 // adjacent in the binning's blocked Morton order, so their neighbour lists
 // name a small set of rows: those of the 3^D cells around the tile's cells.
-// At each list rebuild a tile's halo is built once: the union of those cells
-// in key order (which is the order of the rows), as the explicit list of its fluid rows followed by its
-// wall rows, and every list entry of the tile's rows re-expressed as a 16-bit
+// At each list rebuild a tile's halo is built once: the union of those
+// cells, as the explicit list of its fluid rows followed by its wall rows,
+// and every list entry of the tile's rows re-expressed as a 16-bit
 // position in that halo.  A sweep then stages the halo's rows in LDS with
 // mostly contiguous loads and evaluates its lists from LDS.  The lists keep
 // their order and their entries name the same rows, so a sweep gives the same
 // bits from the halo as from the global arrays.  A tile whose neighbourhood
-// is too wide (sparse spray, more than kHaloCand candidate cells or more rows
-// than the sweeps' LDS holds) keeps the global path (count -1).
+// is too wide (sparse spray or a tile split across two regions: more than
+// kHaloBox cells in the box around its cells, more than kHaloUniq
+// neighbourhood cells, or more rows than the sweeps' LDS holds) keeps the
+// global path (count -1).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <limits.h>
 #include <stdint.h>
 
 #include "tsph_bin.hpp"
@@ -23,7 +26,6 @@
 namespace tsph {
 
 constexpr int kHaloRows = 256;     // rows per tile: the sweeps' workgroup
-constexpr int kHaloCand = 1024;    // neighbourhood cells a tile may name
 constexpr int kHaloCap = 1024;     // halo rows a tile may stage
 
 // Per tile: halo rows, fluid halo rows first (x = all, y = fluid; x = -1: the
@@ -121,28 +123,31 @@ TSPH_D int halo_floor_search(const int* a, int m, int x) {
 }
 
 constexpr int kHaloUniq = 512;     // distinct neighbourhood cells a tile may hold
+constexpr int kHaloBox = 1024;     // cells of the box around a tile's cells
 
 // One workgroup per tile of kHaloRows fluid rows (2-D): the tile rows' slot
 // lists at the build radius (rc2) in both forms, global rows (idx, the
 // NbSlots layout) and halo positions (lidx), and the tile's halo.
 //
-// The tile's distinct cells (runs of equal cells among its rows, which are
-// sorted by binning key) and the 3 x 3 cells around each are candidates; a
-// bitonic sort of their keys (candidate number in the low 10 bits) gives the
-// distinct neighbourhood cells in key order and, for every candidate, its
-// cell's number u.  The halo holds the fluid rows of those cells in key order,
-// then their wall rows, and their positions are staged in LDS.  A row then
-// walks the cells of its own cell's nine candidates, in the cell walk's order
-// (offsets x-major, fluid rows before wall rows), tests each row by the FP64
-// squared distance (the test the other list builds decide by) and appends the
-// accepted rows to both lists.  A tile with too many cells or rows for its
-// LDS walks the global cell table instead and gets no halo (count -1).
+// The tile's distinct cells are the runs of equal cells among its rows
+// (sorted by binning key).  Each marks the 3 x 3 cells around it in a table
+// over the box of the tile's cells widened by one cell; the marked cells,
+// numbered in box order, are the neighbourhood, and every (cell, offset)
+// candidate reads its cell's number u from the table.  The halo holds the
+// fluid rows of those cells in that order, then their wall rows, and their
+// positions are staged in LDS.  A row then walks the cells of its own
+// cell's nine candidates, in the cell walk's order (offsets x-major, fluid
+// rows before wall rows), tests each row by the squared distance (an FP32
+// test on tile-relative positions, the FP64 test within its rounding band:
+// the test the other list builds decide by) and appends the accepted rows to
+// both lists.  A tile with too many cells or rows for its LDS walks the
+// global cell table instead and gets no halo (count -1).
 //
 // cnt, idx: per-row count and global lists (K slots); lidx: halo lists, the
 // same layout with 16-bit entries; blk[2 * tile]: the tile's longest list and
 // (halo rows + 1, or 0 without a halo).
 template <bool FILL>
-__global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid bg, const double* __restrict__ pos,
+__global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const double* __restrict__ pos,
                                                            int64_t n, double rc2, int cap,
                                                            int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
                                                            uint16_t* __restrict__ lidx, int64_t K,
@@ -150,85 +155,94 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid b
                                                            int2* __restrict__ halo_cnt,
                                                            unsigned int* __restrict__ blk) {
   constexpr int D = 2, NOFF = 9;
-  __shared__ unsigned long long key[kHaloCand];
-  __shared__ int16_t cand_u[kHaloCand];
+  __shared__ int16_t tbl[kHaloBox];                    // the tile's box of cells: -1, 0 (marked), or the cell's number
+  __shared__ int16_t cand_u[kHaloRows * NOFF];
   __shared__ int ufs[kHaloUniq], ufl[kHaloUniq], ufo[kHaloUniq], uws[kHaloUniq], uwl[kHaloUniq], uwo[kHaloUniq];
   __shared__ float2 hx[kHaloCap];                      // halo positions relative to the tile's first row
   __shared__ int wtot[4];
+  __shared__ int s_box[4][kHaloRows / 64];
   __shared__ int s_kmax[4];
   __shared__ float s_ext[4];
   const int t = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const int64_t r = tile * kHaloRows + t;
   const bool live = r < n;
-  int64_t c[2] = {0, 0};
+  int c[2] = {0, 0};
   bool head = false;
   if (live) {
-    c[0] = cg.cells[r * D];
-    c[1] = cg.cells[r * D + 1];
+    c[0] = (int)cg.cells[r * D];
+    c[1] = (int)cg.cells[r * D + 1];
     head = t == 0 || cg.cells[(r - 1) * D] != c[0] || cg.cells[(r - 1) * D + 1] != c[1];
   }
   int nheads;
   const int hpos = halo_block_scan(head ? 1 : 0, wtot, &nheads);
   const int hidx = hpos + (head ? 1 : 0) - 1;          // this row's cell among the tile's
-  const int ncand = nheads * NOFF;
-  bool ok = ncand <= kHaloCand;                        // block-uniform
+  // the box of the tile's cells widened by one cell, cut to the grid
+  int bb[4] = {live ? c[0] : INT_MAX, live ? c[1] : INT_MAX, live ? c[0] : INT_MIN, live ? c[1] : INT_MIN};
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int y = __shfl_xor(bb[q], o, 64);
+      bb[q] = q < 2 ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
+    }
+  }
+  if ((t & 63) == 0)
+    for (int q = 0; q < 4; ++q) s_box[q][t >> 6] = bb[q];
+  __syncthreads();
+  for (int w = 0; w < kHaloRows / 64; ++w)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int y = s_box[q][w];
+      bb[q] = q < 2 ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
+    }
+  const int x0 = bb[0] - 1 > 0 ? bb[0] - 1 : 0, y0 = bb[1] - 1 > 0 ? bb[1] - 1 : 0;
+  const int x1 = bb[2] + 1 < (int)cg.ext[0] - 1 ? bb[2] + 1 : (int)cg.ext[0] - 1;
+  const int y1 = bb[3] + 1 < (int)cg.ext[1] - 1 ? bb[3] + 1 : (int)cg.ext[1] - 1;
+  const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
+  bool ok = bw * bh <= kHaloBox;                       // block-uniform
   int F = 0, W = 0;
   if (ok) {
-    int np = 1;
-    while (np < ncand) np <<= 1;
+    for (int e = t; e < kHaloBox; e += kHaloRows) tbl[e] = -1;
+    __syncthreads();
     if (head) {
       for (int o = 0; o < NOFF; ++o) {
-        const int64_t nc[2] = {c[0] + (o / 3 - 1), c[1] + (o % 3 - 1)};
-        const bool in = nc[0] >= 0 && nc[0] < cg.ext[0] && nc[1] >= 0 && nc[1] < cg.ext[1];
-        const int id = hpos * NOFF + o;
-        key[id] = in ? (((unsigned long long)bin_key<2>(nc, bg) << 10) | (unsigned long long)id) : ~0ull;
-        cand_u[id] = -1;
+        const int nx = c[0] + (o / 3 - 1), ny = c[1] + (o % 3 - 1);
+        if (nx >= x0 && nx <= x1 && ny >= y0 && ny <= y1) tbl[(nx - x0) * bh + (ny - y0)] = 0;
       }
     }
-    for (int s2 = ncand + t; s2 < np; s2 += kHaloRows) key[s2] = ~0ull;
     __syncthreads();
-    for (int size = 2; size <= np; size <<= 1) {
-      for (int stride = size >> 1; stride > 0; stride >>= 1) {
-        for (int p = t; p < (np >> 1); p += kHaloRows) {
-          const int lo = 2 * p - (p & (stride - 1));
-          const int hi = lo + stride;
-          const bool up = (lo & size) == 0;
-          const unsigned long long x = key[lo], y = key[hi];
-          if ((x > y) == up) { key[lo] = y; key[hi] = x; }
-        }
-        __syncthreads();
-      }
-    }
-    // distinct cells: number them, and give every candidate its cell's number
+    // number the marked cells (box order) and fetch their row ranges
     int flag[4], fsum = 0, nu = 0;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = 4 * t + q;
-      flag[q] = e < np && key[e] != ~0ull && (e == 0 || (key[e] >> 10) != (key[e - 1] >> 10)) ? 1 : 0;
+      flag[q] = e < bw * bh && tbl[e] == 0 ? 1 : 0;
       fsum += flag[q];
     }
     const int ubase = halo_block_scan(fsum, wtot, &nu);
     ok = nu <= kHaloUniq;
     if (ok) {
-      int u = ubase - 1;
+      int u = ubase;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
+        if (!flag[q]) continue;
         const int e = 4 * t + q;
-        if (e < np && key[e] != ~0ull) {
-          u += flag[q];
-          cand_u[(int)(key[e] & 1023ull)] = (int16_t)u;
-          if (flag[q]) {
-            int64_t cc[2];
-            bin_cell<2>((int64_t)(key[e] >> 10), bg, cc);
-            const int4 R = cg.R[cc[0] * cg.stride[0] + cc[1] * cg.stride[1]];
-            ufs[u] = R.x; ufl[u] = R.y - R.x; ufo[u] = R.y - R.x;
-            uws[u] = R.z; uwl[u] = R.w - R.z; uwo[u] = R.w - R.z;
-          }
-        }
+        tbl[e] = (int16_t)u;
+        const int64_t cx = x0 + e / bh, cy = y0 + e % bh;
+        const int4 R = cg.R[cx * cg.stride[0] + cy * cg.stride[1]];
+        ufs[u] = R.x; ufl[u] = R.y - R.x; ufo[u] = R.y - R.x;
+        uws[u] = R.z; uwl[u] = R.w - R.z; uwo[u] = R.w - R.z;
+        ++u;
       }
     }
     __syncthreads();
+    if (ok && head) {
+      for (int o = 0; o < NOFF; ++o) {
+        const int nx = c[0] + (o / 3 - 1), ny = c[1] + (o % 3 - 1);
+        const bool in = nx >= x0 && nx <= x1 && ny >= y0 && ny <= y1;
+        cand_u[hpos * NOFF + o] = in ? tbl[(nx - x0) * bh + (ny - y0)] : (int16_t)-1;
+      }
+    }
     if (ok) {
       F = halo_array_scan(ufo, nu, wtot);               // lengths -> halo offsets
       W = halo_array_scan(uwo, nu, wtot);
@@ -252,13 +266,13 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, BinGrid b
           const int* off = wl ? uwo : ufo;
           const int* len = wl ? uwl : ufl;
           const int* st = wl ? uws : ufs;
-          // the last cell whose offset is <= x and whose range is not empty
+          // the last cell whose offset is <= x (the one holding slot x)
           int lo = 0, hi = nu;
           while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (off[mid] <= x) lo = mid; else hi = mid;
           }
-          while (len[lo] == 0) --lo;          // empty ranges share the next range's offset
+          while (len[lo] == 0) --lo;
           j = st[lo] + (x - off[lo]);
           hr[s2] = (uint32_t)j;
         }

@@ -525,9 +525,9 @@ int queue_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   unsigned long long* ovf = &fl->lds_overflow;
   const unsigned gf = grid_xcd(sim->ntiles_f * kBlock);
   const int r = (int)(k % kRing);
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
   TSPH_TRY((queue_slot_density_wall<2, S>(sim, k, stream)));
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][2], stream));
   StepArgs st{};
   st.pos_out = sim->pos[sim->cur ^ 1].as<double>();
   st.vel_out = sim->vel[sim->cur ^ 1].as<double>();
@@ -546,7 +546,7 @@ int queue_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
     k_tile_force<S, MAXP, false><<<gf, kBlock, 0, stream>>>(a, g, tf, sim->ntiles_f, R, cells, sim->acc.as<S>(),
                                                             st, fl, slot, 0.5 * sim->prm.skin, ovf);
   TSPH_CHECK_LAUNCH();
-  if (sim->timing) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
+  if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][3], stream));
   return TSPH_OK;
 }
 

@@ -229,8 +229,10 @@ class WcsphSimulation:
 
 
 
-def _set_timing(self, enable: bool) -> None:
-    _lib.check(_lib.lib().tsph_sim_timing(self.handle, 1 if enable else 0, None, None))
+def _set_timing(self, enable, every: int = 1) -> None:
+    """HIP-event sweep timing: on (of every ``every``-th step) or off; resets
+    the sums."""
+    _lib.check(_lib.lib().tsph_sim_timing(self.handle, max(int(every), 1) if enable else 0, None, None))
 
 
 def _sweep_times(self):
