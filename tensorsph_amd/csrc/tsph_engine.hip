@@ -52,11 +52,11 @@ TSPH_D double ord_dec(unsigned long long u) {
 // dt = next_dt(dt, max |v| at the step's start) (droplet_impact.py:536-552),
 // the next step_dt = min(dt, target - elapsed), and the step that starts at the
 // target (within tol) is the final sample, integrated with step_dt = 0.
-// rec: the step's slot of the pinned host ring.  The force sweep's per-block
+// rec: the step's record slot (device).  The force sweep's per-block
 // records (blk, nblk of them; NULL when it used atomics) are reduced here first: the displacement, speed and density
 // maxima and the boundary hits of the step.  A skipped (stale) step wrote
 // none.  kPackThreads threads.
-constexpr int kPackThreads = 256;
+constexpr int kPackThreads = 1024;
 __global__ void __launch_bounds__(kPackThreads) k_pack_record(
     SimFlags* fl, const double* __restrict__ sums, int slot_in, int with_sums, StepRecord* __restrict__ rec,
     const double* __restrict__ loads, const double* __restrict__ clamp,
@@ -65,13 +65,23 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_record(
   const bool fresh = blk && !fl->stale[slot_in];
   unsigned long long key[3] = {0ull, 0ull, 0ull}, hits = 0ull;
   if (fresh) {
-    for (int64_t b = t; b < nblk; b += kPackThreads) {
-      const ulonglong2 a = reinterpret_cast<const ulonglong2*>(blk)[2 * b];
-      const ulonglong2 c = reinterpret_cast<const ulonglong2*>(blk)[2 * b + 1];
-      key[0] = a.x > key[0] ? a.x : key[0];
-      key[1] = a.y > key[1] ? a.y : key[1];
-      key[2] = c.x > key[2] ? c.x : key[2];
-      hits += c.y;
+    // four records per thread and round, their loads issued together
+    for (int64_t b0 = t; b0 < nblk; b0 += 4 * kPackThreads) {
+      ulonglong2 a[4], c[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t b = b0 + q * kPackThreads;
+        const int64_t bb = b < nblk ? b : b0;
+        a[q] = reinterpret_cast<const ulonglong2*>(blk)[2 * bb];
+        c[q] = reinterpret_cast<const ulonglong2*>(blk)[2 * bb + 1];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        key[0] = a[q].x > key[0] ? a[q].x : key[0];
+        key[1] = a[q].y > key[1] ? a[q].y : key[1];
+        key[2] = c[q].x > key[2] ? c[q].x : key[2];
+        hits += b0 + q * kPackThreads < nblk ? c[q].y : 0ull;
+      }
     }
   }
   for (int o = 32; o > 0; o >>= 1) {
@@ -112,11 +122,15 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_record(
     // the step's largest speed and density, read and cleared in one atomic
     // each (a plain scalar load followed by an independent store of 0 is not
     // ordered against it); the host keeps the run's peaks
-    unsigned long long kv = atomicExch(&fl->maxes[MM_PEAK_SPEED], 0ull);
-    unsigned long long kr = atomicExch(&fl->maxes[MM_PEAK_RHO], 0ull);
-    if (fresh) {
-      kv = part[1][0] > kv ? part[1][0] : kv;
-      kr = part[2][0] > kr ? part[2][0] : kr;
+    // (the record path brings them in its per-block records, the cell-tile
+    // path leaves them in the flag block)
+    unsigned long long kv = 0ull, kr = 0ull;
+    if (blk) {
+      kv = fresh ? part[1][0] : 0ull;
+      kr = fresh ? part[2][0] : 0ull;
+    } else {
+      kv = atomicExch(&fl->maxes[MM_PEAK_SPEED], 0ull);
+      kr = atomicExch(&fl->maxes[MM_PEAK_RHO], 0ull);
     }
     rec->maxes[MM_PEAK_SPEED] = kv;
     rec->maxes[MM_PEAK_RHO] = kr;
@@ -138,8 +152,6 @@ __global__ void __launch_bounds__(kPackThreads) k_pack_record(
       }
     }
   }
-  // rec is pinned host memory: the host reads it after the step's event
-  __threadfence_system();
 }
 
 // a run keeps per-step load rows when it has obstacle stations or the clamp
@@ -1499,16 +1511,19 @@ int sim_queue_step(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   }
   if (!tiled) TSPH_TRY((queue_list_sweeps<D, S>(sim, k, meters, stream)));
   sim->cur ^= 1;
-  // record this step for the host: the record kernel writes the pinned host
-  // ring directly (no copy on the stream)
-  StepRecord* rec_host = &sim->ring[r];
-  k_pack_record<<<1, kPackThreads, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_host,
+  // record this step for the host: packed on the device, copied to the
+  // pinned ring on a side stream so that the copy overlaps the next step
+  StepRecord* rec_dev = sim->rec_dev.as<StepRecord>() + r;
+  k_pack_record<<<1, kPackThreads, 0, stream>>>(fl, sim->sums_out.as<double>(), slot, meters ? 1 : 0, rec_dev,
                                                 load_rows(sim) ? sim->loads_out.as<double>() : nullptr,
                                                 load_rows(sim) ? sim->clamp_acc.as<double>() : nullptr,
                                                 tiled ? nullptr : sim->block_max.as<unsigned long long>(),
                                                 tiled ? 0 : (int64_t)gx, sim_dtctl(sim));
   TSPH_CHECK_LAUNCH();
-  TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], stream));
+  TSPH_CHECK_HIP(hipEventRecord(sim->pev[r], stream));
+  TSPH_CHECK_HIP(hipStreamWaitEvent(sim->rec_stream, sim->pev[r], 0));
+  TSPH_CHECK_HIP(hipMemcpyAsync(&sim->ring[r], rec_dev, sizeof(StepRecord), hipMemcpyDeviceToHost, sim->rec_stream));
+  TSPH_CHECK_HIP(hipEventRecord(sim->ev[r], sim->rec_stream));
   sim->ring_step[r] = k;
   return TSPH_OK;
 }
@@ -1677,6 +1692,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   ENS(sim->flags, sizeof(SimFlags));
   ENS(sim->block_sums, 8 * MS_NSUM * (int64_t)grid_xcd(n));
   ENS(sim->block_max, 8 * 4 * (int64_t)grid_xcd(n));
+  ENS(sim->rec_dev, sizeof(StepRecord) * kRing);
   sim->obstacle = prm->obstacle_end > prm->obstacle_begin ? 1 : 0;
   ENS(sim->load_part, 8 * 6 * ((int64_t)grid_for(sim->nw) + 1));
   ENS(sim->loads_out, 8 * 8);
@@ -1687,6 +1703,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
 #undef ENS
   if (rc != TSPH_OK) { tsph_sim_destroy(sim); return rc; }
   hipError_t he = hipHostMalloc((void**)&sim->ring, sizeof(StepRecord) * kRing, hipHostMallocDefault);
+  if (he == hipSuccess) he = hipStreamCreateWithFlags(&sim->rec_stream, hipStreamNonBlocking);
   if (he == hipSuccess) he = hipHostMalloc((void**)&sim->bin_outside, 64, hipHostMallocDefault);
   if (he != hipSuccess) { set_error("hipHostMalloc: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   memset(sim->bin_outside, 0, 64);
@@ -1694,6 +1711,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   memset(sim->ring, 0, sizeof(StepRecord) * kRing);
   for (int r = 0; r < kRing; ++r) {
     he = hipEventCreateWithFlags(&sim->ev[r], hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&sim->pev[r], hipEventDisableTiming);
     if (he != hipSuccess) { set_error("hipEventCreate: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   }
   for (int r = 0; r < kRing; ++r)
@@ -1879,18 +1897,25 @@ int tsph_sim_count_pairs(tsph_sim* sim, double radius, int64_t* pairs, hipStream
 
 int tsph_sim_destroy(tsph_sim* sim) {
   if (!sim) return TSPH_OK;
+  if (sim->rec_stream) {                  // record copies still in flight read rec_dev
+    (void)hipStreamSynchronize(sim->rec_stream);
+    (void)hipStreamDestroy(sim->rec_stream);
+  }
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
                     &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
-                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->sums_out,
+                    &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->rec_dev, &sim->sums_out,
                     &sim->new2old, &sim->cells_new, &sim->bin_cnt, &sim->bin_start,
                     &sim->bin_part, &sim->bin_keys, &sim->bin_tmp, &sim->bin_cells, &sim->bin_flag,
                     &sim->bin_bounds, &sim->cell_R, &sim->slot_idx,
                     &sim->slot_cnt, &sim->slot_max, &sim->slot_posf, &sim->ranges,
                     &sim->tiles_f, &sim->tiles_w, &sim->tkeys, &sim->tkeys_sorted, &sim->tidx, &sim->tflags,
                     &sim->load_part, &sim->loads_out, &sim->clamp_acc, &sim->closure, &sim->shift_buf,
-                    &sim->shift_tmp};
+                    &sim->shift_tmp, &sim->halo_rows, &sim->halo_cnt, &sim->halo_idx, &sim->halo_hmax};
   for (DevBuf* b : bufs) b->release();
-  for (int r = 0; r < kRing; ++r) if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
+  for (int r = 0; r < kRing; ++r) {
+    if (sim->ev[r]) (void)hipEventDestroy(sim->ev[r]);
+    if (sim->pev[r]) (void)hipEventDestroy(sim->pev[r]);
+  }
   for (int r = 0; r < kRing; ++r)
     for (int q = 0; q < 4; ++q) if (sim->tev[r][q]) (void)hipEventDestroy(sim->tev[r][q]);
   if (sim->ring) (void)hipHostFree(sim->ring);

@@ -75,7 +75,7 @@ struct tsph_sim {
   double last_step_dt = 0.0;        // the size of the last step run
   bool rebuild_forced = false;      // a correction re-binned uncounted: the next step re-bins
   int64_t escaped_extra = 0;        // boundary crossings of the position corrections
-  tsph::DevBuf flags, block_sums, block_max, sums_out;
+  tsph::DevBuf flags, block_sums, block_max, sums_out, rec_dev;
   tsph::DevBuf new2old, cells_new;
   // binning of the slot-list layouts: the fixed grid (set at a full build,
   // which also bins the walls) and the counting sort's buffers
@@ -91,7 +91,9 @@ struct tsph_sim {
   int64_t slot_K = 0;
   int64_t slot_K_next = -1;                    // >= 0: the width the next build starts from
   tsph::StepRecord* ring = nullptr;       // pinned
-  hipEvent_t ev[tsph::kRing] = {};
+  hipEvent_t ev[tsph::kRing] = {};        // the step's record is in the ring
+  hipEvent_t pev[tsph::kRing] = {};       // the step's record is packed (stepping stream)
+  hipStream_t rec_stream = nullptr;       // copies the records to the ring
   // per-sweep timing (HIP events around each sweep on the stepping stream)
   int timing = 0;                   // > 0: time the sweeps of every timing-th step
   bool tstep = false;               // the step being queued is timed
