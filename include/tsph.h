@@ -452,6 +452,16 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
  * Takes effect at the next step, which rebuilds the layout.  No counterpart
  * in the reference, which has one layout (its neighbour pairs). */
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y);
+/* Masses of all n_fluid + n_wall rows in the caller's order, FP64 (device).
+ * An FP32-compensated stepper keeps each as two floats (the FP32 value and
+ * the part it dropped) and sums densities with the FP64 value, so its
+ * density matches an FP64 run's to FP64 rounding: at depth H the pressure
+ * contrast between neighbours is about h / H of the pressure, so the 2^-24
+ * rounding of a mass would reach the force about H / h times amplified.
+ * An FP64 stepper stores them as given.  Any time; takes effect at the next
+ * step.  The reference holds masses in the run's dtype (droplet_impact.py:
+ * 618-700); its FP64 runs see the FP64 masses this keeps. */
+int tsph_sim_set_masses_f64(tsph_sim* sim, const double* mass, hipStream_t stream);
 /* Slot width of the per-row neighbour lists (host): *current receives the
  * width of the last build; width >= 0 sets the width the next build starts
  * from (rounded up to a multiple of 4; 0: count first).  A build widens the

@@ -156,6 +156,7 @@ def _declare(L):
         "tsph_sim_time": [_vp, _dp, _vp],
         "tsph_sim_timing": [_vp, C.c_int, _dp, _ip],
         "tsph_sim_configure": [_vp, C.c_int, C.c_int, C.c_int],
+        "tsph_sim_set_masses_f64": [_vp, _vp, _vp],
         "tsph_sim_layout": [_vp, _ip],
         "tsph_sim_slot_width": [_vp, _i64, _ip],
         "tsph_sim_count_pairs": [_vp, _dbl, _ip, _vp],

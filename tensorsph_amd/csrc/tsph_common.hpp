@@ -138,10 +138,12 @@ template <> TSPH_D double cubic_g_r2<double>(double r2, const KernelConst<double
 template <> TSPH_D float cubic_g_r2<float>(float r2, const KernelConst<float>& k) {
   const float rinv = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 1e12f;
   const float q = (r2 * rinv) * k.inv_h;
-  const float d1 = -3.0f * q + 2.25f * (q * q);
-  const float t = 2.0f - q;
+  // both pieces, then one select: 2 - q clamped at 0 makes the outer piece
+  // vanish beyond the support (a nested select here compiles to a branch)
+  const float d1 = q * fmaf(2.25f, q, -3.0f);
+  const float t = fmaxf(2.0f - q, 0.0f);
   const float d2 = -0.75f * (t * t);
-  const float dshape = q < 1.0f ? d1 : (q < 2.0f ? d2 : 0.0f);
+  const float dshape = q < 1.0f ? d1 : d2;
   return k.sigma_over_h * dshape * rinv;
 }
 
@@ -166,6 +168,24 @@ TSPH_D double cubic_w_select(double r, const KernelConst<double>& k) {
   const double t = 2.0 - q;
   const double s2 = (0.25 * t) * (t * t);
   return k.sigma * (q < 1.0 ? s1 : (q < 2.0 ? s2 : 0.0));
+}
+
+// W and dW/dr of the cubic spline for r known to FP64 accuracy, branch-free:
+// cubic_w_select's W and the derivative cubic_wg_select forms before its
+// division by r, from one q and one 2 - q.
+TSPH_D void cubic_w_dw_select(double r, const KernelConst<double>& k, double& w, double& dw) {
+  const double q = r * k.inv_h;
+  const double s1 = fma(q * q, fma(0.75, q, -1.5), 1.0);
+  const double d1 = q * fma(2.25, q, -3.0);
+  // 2 - q clamped at 0: the outer piece vanishes beyond the support, so one
+  // select per value
+  const double t = fmax(2.0 - q, 0.0);
+  const double t2 = t * t;
+  const double s2 = (0.25 * t) * t2;
+  const double d2 = -0.75 * t2;
+  const bool in1 = q < 1.0;
+  w = k.sigma * (in1 ? s1 : s2);
+  dw = k.sigma_over_h * (in1 ? d1 : d2);
 }
 
 // Byte offset into an array with a 32-bit index, so the address is the

@@ -272,14 +272,57 @@ __global__ void __launch_bounds__(256) k_reduce_loads(const double* __restrict__
 
 
 // ------------------------------------------------------------------ step sweeps
+// Row i's fluid state from its density, known to FP64 accuracy.  FP64 runs:
+// the power-form Tait EOS and p / rho^2 of the stored values.  FP32 runs
+// evaluate the EOS and q = p / rho^2 in FP64 too and keep q as q_hi + q_lo
+// (about 2^-48 relative) and p as p + p_lo.  The force sweep forms the
+// pressure contrast q_j - q_i from both parts: at depth H a row's q is about
+// H / h times the contrast it carries to a neighbour, so a q rounded to FP32
+// would put that factor times 2^-24 into the force.  The wall sweep
+// extrapolates p to FP64 accuracy from p + p_lo.
+template <typename S>
+TSPH_D void store_fluid_state(const WcsphArgs<S>& a, int64_t i, S rho, double rho_d) {
+  if constexpr (sizeof(S) == 4) {
+    const double p_d = tait<double>(rho_d, a.rho0d, a.bulkd, a.gammad, a.ratio_floord);
+    const double q_d = p_d / (rho_d * rho_d);
+    const S p = (S)p_d, q = (S)q_d, ql = (S)(q_d - (double)q);
+    const S vol = a.mass[i] / rho;
+    a.p[i] = p;
+    if (a.p_lo) a.p_lo[i] = (S)(p_d - (double)p);
+    a.pr2[i] = q;
+    a.vol[i] = vol;
+    if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], q, vol, ql};
+  } else {
+    const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
+    const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
+    a.p[i] = p;
+    a.pr2[i] = pr2;
+    a.vol[i] = vol;
+  }
+}
+
+// r = sqrt(r2) and 1 / r to about 5e-14 relative from the FP32 hardware
+// reciprocal square root and one Newton step in FP64.  r2 below 1e-30 is
+// taken as 1e-30 for 1 / r (finite, 1e15), so r = r2 / sqrt(1e-30) <= 1e-15
+// there and r = 0 at r2 = 0, without selects.
+TSPH_D void sqrt_rinv_newton(double r2, double& r, double& rinv) {
+  const double rc = fmax(r2, 1.0e-30);
+  const double y0 = (double)__builtin_amdgcn_rsqf((float)rc);
+  rinv = y0 * (1.5 - (0.5 * rc) * (y0 * y0));
+  r = r2 * rinv;
+}
+
 // HALO: the tile's halo rows (positions, masses) are staged in LDS and the
 // lists of halo positions are evaluated from there (2-D slot lists); a tile
 // without a halo reads the global arrays.  Same terms in the same order
 // either way.
-template <int D, typename S, typename NB, bool HALO = false>
+template <int D, typename S, typename NB, bool HALO = false, bool SHEP = false>
 __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr, SimFlags* fl,
                                                          int slot_in, int write_eos, HaloView hv = {},
                                                          NbHalo hl = {}) {
+  // FP32 runs: masses as hi + lo (a.mass_lo) and the pressure split's G_i
+  // (a.gsum) are always present
+  constexpr bool kSplit = sizeof(S) == 4;
   if (fl->stale[slot_in]) return;
   const int64_t nb = gridDim.x;
   const int64_t tile = xcd_block(blockIdx.x, nb);
@@ -288,25 +331,25 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
   int H = -1;
   extern __shared__ __align__(16) unsigned char halo_lds[];
   double* hx = reinterpret_cast<double*>(halo_lds);
-  S* hm = nullptr;
+  double* hm = nullptr;
   if constexpr (HALO) {
     H = tile < hv.ntiles ? hv.cnt[tile].x : -1;
     if (H > hv.max_rows) H = -1;                     // more rows than this launch staged: global path
     if (H >= 0) {
-      hm = reinterpret_cast<S*>(hx + (size_t)H * D);
+      hm = hx + (size_t)H * D;
       // every load of the staging is issued before the first store (up to
       // kHaloCap / kBlock rows per thread)
       const uint32_t* hr = hv.rows + tile * kHaloCap;
       constexpr int QS = kHaloCap / kBlock;
-      double px[QS][D];
-      S pm[QS];
+      double px[QS][D], pm[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
         const int s = threadIdx.x + q * kBlock;
         const uint32_t j = hr[s < H ? s : 0];
 #pragma unroll
         for (int d = 0; d < D; ++d) px[q][d] = a.pos[(int64_t)j * D + d];
-        pm[q] = a.mass[j];
+        pm[q] = (double)a.mass[j];
+        if constexpr (kSplit) pm[q] += (double)a.mass_lo[j];
       }
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
@@ -325,36 +368,51 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = a.pos[i * D + d];
   Sum<double> acc, kt;
+  // FP32 runs also sum G_i = sum_j m_j grad W_ij over the same entries, in
+  // FP64: the force sweep adds the pressure terms as -2 q_i G_i plus the
+  // contrast terms -m_j (q_j - q_i) grad W_ij (the same sum, regrouped)
+  double gs[D] = {};
   // one entry: its position and mass, fetched from the halo or the arrays
   auto entry = [&](const double (&pj)[D], double mj, bool keep) {
-    double r2 = 0.0;
+    double r2 = 0.0, df[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const double df = xi[d] - pj[d];
-      r2 += df * df;
+      df[d] = xi[d] - pj[d];
+      r2 += df[d] * df[d];
     }
     keep = keep && !(r2 > a.skip2d);
     // FP32-compensated runs take r from the refined reciprocal square root
     // (5e-14 relative) and W from the branch-free form; FP64 runs keep the
     // correctly rounded square root and cubic_wg
     double w, g;
-    if constexpr (sizeof(S) == 4) {
-      w = cubic_w_select(sqrt_r2_newton(r2), a.kd);
+    if constexpr (kSplit) {
+      double r, rinv, dw;
+      sqrt_rinv_newton(r2, r, rinv);
+      cubic_w_dw_select(r, a.kd, w, dw);
+      // an entry outside the support adds +0 to every sum, as add_if leaves it
+      const double mk = keep ? mj : 0.0;
+      const double gm = mk * (dw * rinv);
+#pragma unroll
+      for (int d = 0; d < D; ++d) gs[d] = fma(gm, df[d], gs[d]);
+      acc.add(mk * w);
     } else {
       cubic_wg_select<double>(sqrt(r2), a.kd, w, g);
+      acc.add_if(keep, mj * w);
     }
-    acc.add_if(keep, mj * w);
-    if (a.shepard) kt.add_if(keep, w);
+    if constexpr (SHEP) kt.add_if(keep, w);
   };
   if (HALO && H >= 0) {
     hl.template visit_chunks<D, TSPH_HALO_CH>(i, [&](const int32_t* l4, const bool* keep4) {
+      // the chunk's LDS reads first, then its arithmetic
+      double pj4[4][D], mj4[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        double pj[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) pj[d] = hx[l4[u] * D + d];
-        entry(pj, (double)hm[l4[u]], keep4[u]);
+        for (int d = 0; d < D; ++d) pj4[u][d] = hx[l4[u] * D + d];
+        mj4[u] = hm[l4[u]];
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) entry(pj4[u], mj4[u], keep4[u]);
     });
   } else {
     nbr.template visit_masked<D>(i, true, [&](int64_t j, bool keep) {
@@ -363,26 +421,39 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
       double pj[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) pj[d] = p[d];
-      entry(pj, (double)*at_bytes(a.mass, jb * (uint32_t)sizeof(S)), keep);
+      double mj = (double)*at_bytes(a.mass, jb * (uint32_t)sizeof(S));
+      if constexpr (kSplit) mj += (double)*at_bytes(a.mass_lo, jb * (uint32_t)sizeof(S));
+      entry(pj, mj, keep);
     });
   }
-  double rho_d = hi(acc) + (double)a.mass[i] * a.kd.w0d;
+  double mi = (double)a.mass[i];
+  if constexpr (kSplit) mi += (double)a.mass_lo[i];
+  double rho_d = hi(acc) + mi * a.kd.w0d;
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
-  if (a.shepard) {                 // Shepard renormalisation, droplet_impact.py:954-964
+  if constexpr (SHEP) {                 // Shepard renormalisation, droplet_impact.py:954-964
     double t = hi(kt) + a.kd.w0d;
     t = t > 1.0e-12 ? t : 1.0e-12;
     rho_d = rho_d * (a.lattice_sum / t);
     rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   }
   const S rho = store_rho<S>(rho_d, a.rho, a.rho_lo, i);
-  if (write_eos) {
-    const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
-    const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
-    a.p[i] = p;
-    a.pr2[i] = pr2;
-    a.vol[i] = vol;
-    if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
+  if constexpr (kSplit) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) a.gsum[i * D + d] = (S)gs[d];
   }
+  if (write_eos) store_fluid_state<S>(a, i, rho, rho_d);
+}
+
+// the density sweep with or without the Shepard factor (a compile-time
+// switch: the sum of W it needs costs the default sweep nothing)
+template <int D, typename S, bool HALO>
+void launch_density(int shepard, unsigned grid, size_t lds, hipStream_t stream, const WcsphArgs<S>& a,
+                    const NbSlots& nbr, SimFlags* fl, int slot, int write_eos, const HaloView& hv,
+                    const NbHalo& hl) {
+  if (shepard)
+    k_step_density<D, S, NbSlots, HALO, true><<<grid, kBlock, lds, stream>>>(a, nbr, fl, slot, write_eos, hv, hl);
+  else
+    k_step_density<D, S, NbSlots, HALO, false><<<grid, kBlock, lds, stream>>>(a, nbr, fl, slot, write_eos, hv, hl);
 }
 
 template <int D, typename S, int FORM, typename NB>
@@ -426,13 +497,8 @@ __global__ void __launch_bounds__(kBlock) k_step_diffusion(WcsphArgs<S> a, NB nb
   double rho_d = (double)rhoi + (a.rho_lo ? (double)a.rho_lo[i] : 0.0) + (double)(S)fl->step_dt * hi(rate);
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = (S)rho_d;
-  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
-  const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
   a.rho[i] = rho;
-  a.p[i] = p;
-  a.pr2[i] = pr2;
-  a.vol[i] = vol;
-  if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
+  store_fluid_state<S>(a, i, rho, rho_d);
 }
 
 template <int D, typename S, typename NB>
@@ -492,6 +558,39 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NB nbr, Si
   double xw[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xw[d] = a.pos[iw * D + d];
+  if constexpr (sizeof(S) == 4) {
+    // FP32 runs: the extrapolation and the wall EOS in FP64 from p + p_lo, so
+    // that the wall rows' q = q_hi + q_lo carries the pressure contrast to
+    // their fluid neighbours as the fluid rows' q does (store_fluid_state)
+    Sum<double> num, den;
+    nbr.template visit<D>(iw, false, [&](int64_t f) {
+      double rw[D], r2 = 0.0;
+#pragma unroll
+      for (int d = 0; d < D; ++d) { rw[d] = xw[d] - a.pos[f * D + d]; r2 += rw[d] * rw[d]; }
+      if (r2 > a.skip2d) return;
+      double r, rinv;
+      sqrt_rinv_newton(r2, r, rinv);
+      const double wgt = cubic_w_select(r, a.kd);
+      double src = (double)a.p[f] + (a.p_lo ? (double)a.p_lo[f] : 0.0);
+      if (a.gravity != S(0)) src = src + (double)a.rho[f] * (-(double)a.gravity * rw[1]);
+      num.add(src * wgt);
+      den.add(wgt);
+    });
+    const double dd = den.get() > 1e-30 ? den.get() : 1e-30;
+    double pw = num.get() / dd;
+    pw = pw > 0.0 ? pw : 0.0;
+    double rwv = a.rho0d * pow(pw / a.bulkd + 1.0, a.inv_gammad);
+    rwv = rwv > a.rho0d ? rwv : a.rho0d;
+    const double q_d = pw / (rwv * rwv);
+    const S rs = (S)rwv, ps = (S)pw, q = (S)q_d, vol = a.mass[iw] / rs;
+    a.p[iw] = ps;
+    if (a.p_lo) a.p_lo[iw] = (S)(pw - (double)ps);
+    a.rho[iw] = rs;
+    a.pr2[iw] = q;
+    a.vol[iw] = vol;
+    if (a.aux) a.aux[iw] = Vec4<S>{a.mass[iw], q, vol, (S)(q_d - (double)q)};
+    return;
+  }
   Sum<S> num, den;
   nbr.template visit<D>(iw, false, [&](int64_t f) {
     S rw[D], r2 = S(0);
@@ -547,13 +646,8 @@ __global__ void __launch_bounds__(kBlock) k_step_wall_mass(WcsphArgs<S> a, NB nb
   double rho_d = (double)a.rho[i] + hi(corr);
   rho_d = rho_d > (double)a.rho_floor ? rho_d : (double)a.rho_floor;
   const S rho = (S)rho_d;
-  const S p = tait_hi<S>(rho_d, a.rho0d, a.bulk, a.gamma, a.ratio_floor);
-  const S pr2 = p / (rho * rho), vol = a.mass[i] / rho;
   a.rho[i] = rho;
-  a.p[i] = p;
-  a.pr2[i] = pr2;
-  a.vol[i] = vol;
-  if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, vol, S(0)};
+  store_fluid_state<S>(a, i, rho, rho_d);
 }
 
 // Free-surface pressure closure (droplet_impact.py:429-492) on the stepper:
@@ -605,9 +699,16 @@ __global__ void __launch_bounds__(kBlock) k_step_closure(WcsphArgs<S> a, NB nbr,
   k_dst[i] = ki;
   if (last) {
     const S r = a.rho[i];
-    const S pr2 = pi / (r * r);
-    a.pr2[i] = pr2;
-    if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], pr2, a.vol[i], S(0)};
+    if constexpr (sizeof(S) == 4) {
+      // the closed pressure as stored (FP32 sums), its q in FP64
+      const double q_d = (double)pi / ((double)r * (double)r);
+      const S q = (S)q_d;
+      a.pr2[i] = q;
+      if (a.p_lo) a.p_lo[i] = S(0);
+      if (a.aux) a.aux[i] = Vec4<S>{a.mass[i], q, a.vol[i], (S)(q_d - (double)q)};
+    } else {
+      a.pr2[i] = pi / (r * r);
+    }
   }
 }
 
@@ -663,7 +764,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       const uint32_t* hr = hv.rows + tile * kHaloCap;
       constexpr int QS = kHaloCap / kBlock;
       double rx[QS][D], rv[QS][D];
-      S r0[QS], r1[QS], r2[QS];
+      S r0[QS], r1[QS], r2[QS], r3[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
         const int s = threadIdx.x + q * kBlock;
@@ -672,9 +773,9 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
         for (int d = 0; d < D; ++d) { rx[q][d] = a.pos[(int64_t)j * D + d]; rv[q][d] = a.vel[(int64_t)j * D + d]; }
         if constexpr (sizeof(S) == 4) {
           const Vec4<S> w = a.aux[j];
-          r0[q] = w.x; r1[q] = w.y; r2[q] = w.z;
+          r0[q] = w.x; r1[q] = w.y; r2[q] = w.z; r3[q] = w.w;
         } else {
-          r0[q] = a.mass[j]; r1[q] = a.pr2[j]; r2[q] = a.vol[j];
+          r0[q] = a.mass[j]; r1[q] = a.pr2[j]; r2[q] = a.vol[j]; r3[q] = S(0);
         }
       }
 #pragma unroll
@@ -684,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
           HaloRec<D, S>& rr = hrec[s];
 #pragma unroll
           for (int d = 0; d < D; ++d) { rr.x[d] = rx[q][d]; rr.v[d] = rv[q][d]; }
-          rr.q = Vec4<S>{r0[q], r1[q], r2[q], S(0)};
+          rr.q = Vec4<S>{r0[q], r1[q], r2[q], r3[q]};
         }
       }
       __syncthreads();
@@ -697,7 +798,18 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
   if (live) {
 #pragma unroll
     for (int d = 0; d < D; ++d) { xi[d] = a.pos[i * D + d]; vi[d] = a.vel[i * D + d]; }
-    const S pri = a.pr2[i];
+    // FP32: the pressure terms as -2 q_i G_i (G_i from the density sweep, in
+    // FP64) plus the contrast terms -m_j (q_j - q_i) grad W_ij, with q_j - q_i
+    // formed from the two-part q of both rows (store_fluid_state); FP64: the
+    // symmetric form -m_j (q_i + q_j) grad W_ij as it stands
+    constexpr bool kSplit = sizeof(S) == 4;
+    S pri, prl = S(0);
+    if constexpr (kSplit) {
+      const Vec4<S> ai = a.aux[i];
+      pri = ai.y; prl = ai.w;
+    } else {
+      pri = a.pr2[i];
+    }
     Sum<S> ap[D], av[D];
     // every list entry is evaluated, four at a time; an entry outside the
     // support has its gradient factor zeroed, which zeroes its pressure and
@@ -725,7 +837,9 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
         const bool keep = keep4[u] && !(r2 > a.skip2);
         S g = cubic_g_r2<S>(r2, a.k);
         g = keep ? g : S(0);
-        const S cp = -q.x * (pri + q.y);
+        S cp;
+        if constexpr (kSplit) cp = -q.x * ((q.y - pri) + (q.w - prl));
+        else cp = -q.x * (pri + q.y);
 #pragma unroll
         for (int d = 0; d < D; ++d) tp[d][u] = cp * (df[d] * g);
         const S blend = wall ? a.friction : S(1);
@@ -767,8 +881,16 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
         });
       });
     }
+    if constexpr (kSplit) {
+      const double qi = (double)pri + (double)prl;
 #pragma unroll
-    for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
+      for (int d = 0; d < D; ++d)
+        acc[d] = (S)(((hi(ap[d]) + hi(av[d])) - 2.0 * qi * (double)a.gsum[i * D + d]) +
+                     (d == 1 ? -(double)a.gravity : 0.0));
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
+    }
     if (acc_out) {
 #pragma unroll
       for (int d = 0; d < D; ++d) acc_out[i * D + d] = acc[d];
@@ -917,12 +1039,15 @@ __global__ void k_reset_meters(SimFlags* fl, int slot_in) {
 }
 
 // ------------------------------------------------------------------ rebuild helpers
+// Offset of the low parts of the masses in the mass buffer (FP32 runs), else 0.
+inline int64_t mass_lo_offset(const tsph_sim* sim) { return sim->prec == TSPH_F64 ? 0 : sim->nt; }
+
 template <int D, typename S>
 __global__ void k_permute(const int64_t* __restrict__ new2old, int64_t nt, const double* __restrict__ pos,
                           const double* __restrict__ vel, const S* __restrict__ mass,
                           const int64_t* __restrict__ orig, const int64_t* __restrict__ cells_old,
                           double* __restrict__ pos2, double* __restrict__ vel2, S* __restrict__ mass2,
-                          int64_t* __restrict__ orig2, int64_t* __restrict__ cells_new) {
+                          int64_t* __restrict__ orig2, int64_t* __restrict__ cells_new, int64_t lo_off) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nt) return;
   const int64_t o = new2old[k];
@@ -933,6 +1058,7 @@ __global__ void k_permute(const int64_t* __restrict__ new2old, int64_t nt, const
     cells_new[k * D + d] = cells_old[o * D + d];
   }
   mass2[k] = mass[o];
+  if (lo_off) mass2[lo_off + k] = mass[lo_off + o];    // the low parts (FP32 runs)
   orig2[k] = orig[o];
 }
 
@@ -974,6 +1100,19 @@ __global__ void k_count_close(const double* __restrict__ pos, int64_t n, NbCells
   }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
   if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
+}
+
+// row k's mass from the caller's FP64 masses: FP32 runs as hi + lo (lo at
+// lo_off), FP64 runs as given
+template <typename S>
+__global__ void k_set_masses(const double* __restrict__ m, const int64_t* __restrict__ orig, int64_t nt,
+                             S* __restrict__ mass, int64_t lo_off) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nt) return;
+  const double v = m[orig[k]];
+  const S h = (S)v;
+  mass[k] = h;
+  if (lo_off) mass[lo_off + k] = (S)(v - (double)h);
 }
 
 __global__ void k_iota(int64_t* __restrict__ v, int64_t n) {
@@ -1256,7 +1395,8 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
     k_permute<D, S><<<grid_for(rows), kBlock, 0, stream>>>(
         sim->new2old.as<int64_t>(), rows, sim->pos[sim->cur].as<double>(), sim->vel[sim->cur].as<double>(),
         sim->mass.as<S>(), sim->orig.as<int64_t>(), sim->bin_cells.as<int64_t>(), sim->pos[other].as<double>(),
-        sim->vel[other].as<double>(), sim->mass2.as<S>(), sim->orig2.as<int64_t>(), sim->cells_new.as<int64_t>());
+        sim->vel[other].as<double>(), sim->mass2.as<S>(), sim->orig2.as<int64_t>(), sim->cells_new.as<int64_t>(),
+        mass_lo_offset(sim));
     TSPH_CHECK_LAUNCH();
     std::swap(sim->mass, sim->mass2);
     std::swap(sim->orig, sim->orig2);
@@ -1270,6 +1410,9 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
                                     sizeof(double) * (nt - n) * D, hipMemcpyDeviceToDevice, stream));
       TSPH_CHECK_HIP(hipMemcpyAsync(sim->mass2.as<S>() + n, sim->mass.as<S>() + n, sizeof(S) * (nt - n),
                                     hipMemcpyDeviceToDevice, stream));
+      if (const int64_t lo = mass_lo_offset(sim))
+        TSPH_CHECK_HIP(hipMemcpyAsync(sim->mass2.as<S>() + lo + n, sim->mass.as<S>() + lo + n,
+                                      sizeof(S) * (nt - n), hipMemcpyDeviceToDevice, stream));
       TSPH_CHECK_HIP(hipMemcpyAsync(sim->orig2.as<int64_t>() + n, sim->orig.as<int64_t>() + n,
                                     sizeof(int64_t) * (nt - n), hipMemcpyDeviceToDevice, stream));
     }
@@ -1303,10 +1446,13 @@ WcsphArgs<S> sim_args(tsph_sim* sim) {
   a.pos = sim->pos[sim->cur].as<double>();
   a.vel = sim->vel[sim->cur].as<double>();
   a.mass = sim->mass.as<S>();
+  a.mass_lo = mass_lo_offset(sim) ? sim->mass.as<S>() + mass_lo_offset(sim) : nullptr;
   a.wall_accel = nullptr;
   a.rho = sim->rho.as<S>(); a.p = sim->p.as<S>(); a.pr2 = sim->pr2.as<S>(); a.vol = sim->vol.as<S>();
   // FP32: one 16-byte record per neighbour; FP64 reads the three arrays
   a.aux = sizeof(S) == 4 ? sim->aux.as<Vec4<S>>() : nullptr;
+  a.p_lo = sizeof(S) == 4 ? sim->p_lo.as<S>() : nullptr;
+  a.gsum = sizeof(S) == 4 ? sim->gsum.as<S>() : nullptr;
   return a;
 }
 
@@ -1353,10 +1499,9 @@ int queue_slot_density_wall(tsph_sim* sim, int64_t k, hipStream_t stream) {
   const int slot = (int)(k & 1);
   SimFlags* fl = sim->flags.as<SimFlags>();
   WcsphArgs<S> a = sim_args<D, S>(sim);
-  a.aux = nullptr;                       // the tile force sweep reads the arrays
   const NbSlots nbr = sim_slots(sim);
   const int r = (int)(k % kRing);
-  k_step_density<D, S, NbSlots><<<grid_xcd(sim->n), kBlock, 0, stream>>>(a, nbr, fl, slot, 1, HaloView{}, NbHalo{});
+  launch_density<D, S, false>(a.shepard, grid_xcd(sim->n), 0, stream, a, nbr, fl, slot, 1, HaloView{}, NbHalo{});
   TSPH_CHECK_LAUNCH();
   if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
@@ -1382,16 +1527,16 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const bool halo = kHalo2 && sim->tiles == 2 && sim->halo_ok;
   const HaloView hv = halo ? sim_halo(sim) : HaloView{};
   const NbHalo hl = halo ? sim_halo_lists(sim) : NbHalo{};
-  const size_t halo_density_bytes = halo ? (size_t)sim->halo_max * (D * sizeof(double) + sizeof(S)) : 0;
+  const size_t halo_density_bytes = halo ? (size_t)sim->halo_max * (D + 1) * sizeof(double) : 0;
   const size_t halo_force_bytes = halo ? (size_t)sim->halo_max * sizeof(HaloRec<D, S>) : 0;
   if (sim->prm.op.diffusion_scale > 0.0) {
     if (sizeof(S) == 4) a.rho_lo = sim->rho_lo.as<S>();
     WcsphArgs<S> a0 = a;
     a0.rho = sim->rho_raw.as<S>();
     if (halo)
-      k_step_density<D, S, NbSlots, kHalo2><<<gx, kBlock, halo_density_bytes, stream>>>(a0, nbr, fl, slot, 0, hv, hl);
+      launch_density<D, S, kHalo2>(a0.shepard, gx, halo_density_bytes, stream, a0, nbr, fl, slot, 0, hv, hl);
     else
-      k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, fl, slot, 0, hv, hl);
+      launch_density<D, S, false>(a0.shepard, gx, 0, stream, a0, nbr, fl, slot, 0, hv, hl);
     if (sim->prm.op.diffusion_form == 1) {
       k_step_gradient<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a0, nbr, sim->grad.as<S>(), fl, slot);
       k_step_diffusion<D, S, 1, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, sim->grad.as<S>(), fl, slot);
@@ -1399,9 +1544,9 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
       k_step_diffusion<D, S, 0, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, a0.rho, nullptr, fl, slot);
     }
   } else if (halo) {
-    k_step_density<D, S, NbSlots, kHalo2><<<gx, kBlock, halo_density_bytes, stream>>>(a, nbr, fl, slot, 1, hv, hl);
+    launch_density<D, S, kHalo2>(a.shepard, gx, halo_density_bytes, stream, a, nbr, fl, slot, 1, hv, hl);
   } else {
-    k_step_density<D, S, NbSlots><<<gx, kBlock, 0, stream>>>(a, nbr, fl, slot, 1, hv, hl);
+    launch_density<D, S, false>(a.shepard, gx, 0, stream, a, nbr, fl, slot, 1, hv, hl);
   }
   if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][1], stream));
   TSPH_TRY((queue_wall_stage<D, S>(sim, a, nbr, fl, slot, stream)));
@@ -1678,10 +1823,11 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
 #define ENS(buf, bytes) if (rc == TSPH_OK) rc = (buf).ensure(bytes)
   ENS(sim->pos[0], 8 * nt * D); ENS(sim->pos[1], 8 * nt * D);
   ENS(sim->vel[0], 8 * nt * D); ENS(sim->vel[1], 8 * nt * D);
-  ENS(sim->mass, e * nt); ENS(sim->mass2, e * nt);
+  // FP32 runs keep each mass as two floats: hi in [0, nt), lo in [nt, 2 nt)
+  ENS(sim->mass, e * nt * (e == 4 ? 2 : 1)); ENS(sim->mass2, e * nt * (e == 4 ? 2 : 1));
   ENS(sim->orig, 8 * nt); ENS(sim->orig2, 8 * nt);
   ENS(sim->rho, e * nt); ENS(sim->p, e * nt); ENS(sim->pr2, e * nt); ENS(sim->vol, e * nt);
-  if (e == 4) ENS(sim->aux, 4 * e * nt);
+  if (e == 4) { ENS(sim->aux, 4 * e * nt); ENS(sim->p_lo, e * nt); ENS(sim->gsum, e * n * D + 4); }
   if (prm->op.surface_pressure_closure) {
     const size_t sb = (e * (size_t)(prm->op.n_fluid + 1) + 255) / 256 * 256;
     const size_t kb = (sizeof(int) * (size_t)(prm->op.n_fluid + 1) + 255) / 256 * 256;
@@ -1724,6 +1870,7 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he == hipSuccess) he = hipMemsetAsync(sim->vel[0].ptr, 0, 8 * nt * D, stream);
   if (he == hipSuccess) he = hipMemcpyAsync(sim->vel[0].ptr, vel, 8 * n * D, hipMemcpyDeviceToDevice, stream);
   if (he == hipSuccess) he = hipMemcpyAsync(sim->mass.ptr, mass, e * nt, hipMemcpyDeviceToDevice, stream);
+  if (he == hipSuccess && e == 4) he = hipMemsetAsync(sim->mass.as<float>() + nt, 0, e * nt, stream);
   if (he == hipSuccess) he = hipMemsetAsync(sim->flags.ptr, 0, sizeof(SimFlags), stream);
   if (he == hipSuccess) he = hipMemsetAsync(sim->loads_out.ptr, 0, 8 * 8, stream);
   if (he == hipSuccess) he = hipMemsetAsync(sim->clamp_acc.ptr, 0, 8 * 4, stream);
@@ -1746,6 +1893,19 @@ int tsph_sim_step(tsph_sim* sim, int64_t nsteps, int64_t meter_every, hipStream_
                                  : sim_run<2, float>(sim, nsteps, meter_every, stream);
   return sim->prec == TSPH_F64 ? sim_run<3, double>(sim, nsteps, meter_every, stream)
                                : sim_run<3, float>(sim, nsteps, meter_every, stream);
+}
+
+int tsph_sim_set_masses_f64(tsph_sim* sim, const double* mass, hipStream_t stream) {
+  TSPH_REQUIRE(sim && mass, TSPH_ERR_ARG, "sim and mass must not be NULL");
+  const int64_t nt = sim->nt;
+  if (sim->prec == TSPH_F64)
+    k_set_masses<double><<<grid_for(nt), kBlock, 0, stream>>>(mass, sim->orig.as<int64_t>(), nt,
+                                                              sim->mass.as<double>(), 0);
+  else
+    k_set_masses<float><<<grid_for(nt), kBlock, 0, stream>>>(mass, sim->orig.as<int64_t>(), nt,
+                                                             sim->mass.as<float>(), mass_lo_offset(sim));
+  TSPH_CHECK_LAUNCH();
+  return TSPH_OK;
 }
 
 int tsph_sim_read(tsph_sim* sim, double* pos, double* vel, void* rho, void* p, void* acc, hipStream_t stream) {
@@ -1902,7 +2062,7 @@ int tsph_sim_destroy(tsph_sim* sim) {
     (void)hipStreamDestroy(sim->rec_stream);
   }
   DevBuf* bufs[] = {&sim->pos[0], &sim->pos[1], &sim->vel[0], &sim->vel[1], &sim->mass, &sim->mass2,
-                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->rho_raw, &sim->rho_lo,
+                    &sim->orig, &sim->orig2, &sim->rho, &sim->p, &sim->pr2, &sim->vol, &sim->aux, &sim->p_lo, &sim->gsum, &sim->rho_raw, &sim->rho_lo,
                     &sim->grad, &sim->acc, &sim->anchor, &sim->flags, &sim->block_sums, &sim->block_max, &sim->rec_dev, &sim->sums_out,
                     &sim->new2old, &sim->cells_new, &sim->bin_cnt, &sim->bin_start,
                     &sim->bin_part, &sim->bin_keys, &sim->bin_tmp, &sim->bin_cells, &sim->bin_flag,

@@ -68,7 +68,8 @@ struct tsph_sim {
   tsph_ctx* ctx = nullptr;
   tsph::DevBuf pos[2], vel[2], mass, mass2, orig, orig2;
   tsph::DevBuf rho, p, pr2, vol, rho_raw, rho_lo, grad, acc, anchor;
-  tsph::DevBuf aux;                 // FP32: Vec4 (m, p / rho^2, m / rho, 0) per row
+  tsph::DevBuf aux;                 // FP32: Vec4 (m, q_hi, m / rho, q_lo) per row, q = p / rho^2
+  tsph::DevBuf p_lo, gsum;          // FP32: p's dropped part per row; sum_j m_j grad W_ij per fluid row
   tsph::DevBuf closure;             // free-surface pressure closure: two pressure and two flag buffers
   tsph::DevBuf shift_buf, shift_tmp;  // particle shifting: C, sorted C, shift vectors, counters; sort scratch
   double last_speed = 0.0;          // largest speed at the start of the last step run

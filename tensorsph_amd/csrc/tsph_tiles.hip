@@ -58,7 +58,7 @@ __global__ void k_tile_permute(const int32_t* __restrict__ new2old, int64_t nt, 
                                const double* __restrict__ vel, const S* __restrict__ mass,
                                const int64_t* __restrict__ orig, const int64_t* __restrict__ cells_old,
                                double* __restrict__ pos2, double* __restrict__ vel2, S* __restrict__ mass2,
-                               int64_t* __restrict__ orig2, int64_t* __restrict__ cells_new) {
+                               int64_t* __restrict__ orig2, int64_t* __restrict__ cells_new, int64_t lo_off) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nt) return;
   const int64_t o = new2old[k];
@@ -69,6 +69,7 @@ __global__ void k_tile_permute(const int32_t* __restrict__ new2old, int64_t nt, 
     cells_new[k * 2 + d] = cells_old[o * 2 + d];
   }
   mass2[k] = mass[o];
+  if (lo_off) mass2[lo_off + k] = mass[lo_off + o];    // the low parts (FP32 runs)
   orig2[k] = orig[o];
 }
 
@@ -192,11 +193,12 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
     const int64_t* __restrict__ cells, S* __restrict__ acc_out, StepArgs st, SimFlags* fl, int slot,
     double half_skin, unsigned long long* overflow) {
   const int64_t b = xcd_block(blockIdx.x, gridDim.x);
+  constexpr bool kSplit = sizeof(S) == 4;
   __shared__ BlockTable bt;
   __shared__ double2 sx[MAXP];
   __shared__ float2 sxf[MAXP];      // FP32 positions relative to the tile origin (scan prefilter)
   __shared__ double2 sv[MAXP];
-  __shared__ S sm[MAXP], spr[MAXP], svol[MAXP];
+  __shared__ S sm[MAXP], spr[MAXP], sprl[MAXP], svol[MAXP];
   __shared__ int srow[MAXP];
   __shared__ double red[MS_NSUM][kBlock / 64];
   __shared__ unsigned short nb[kNbMax * kBlock];
@@ -226,6 +228,7 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
       sv[k] = make_double2(a.vel[2 * row], a.vel[2 * row + 1]);
       sm[k] = a.mass[row];
       spr[k] = a.pr2[row];
+      sprl[k] = kSplit ? a.aux[row].w : S(0);
       svol[k] = a.vol[row];
       srow[k] = (int)row;
     }
@@ -243,24 +246,26 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
     const double vi0 = a.vel[2 * r], vi1 = a.vel[2 * r + 1];
     const int bx = (int)(cells[2 * r] - bx0), by = (int)(cells[2 * r + 1] - by0);
     const S pri = a.pr2[r];
+    const S prli = kSplit ? a.aux[r].w : S(0);
     Sum<S> ap0, ap1, av0, av1;
     auto body = [&](int64_t q) {
       int64_t j;
       double xj0, xj1, vj0, vj1;
-      S mj, prj, volj;
+      S mj, prj, prlj, volj;
       if (lds) {
         j = srow[q]; xj0 = sx[q].x; xj1 = sx[q].y; vj0 = sv[q].x; vj1 = sv[q].y;
-        mj = sm[q]; prj = spr[q]; volj = svol[q];
+        mj = sm[q]; prj = spr[q]; prlj = sprl[q]; volj = svol[q];
       } else {
         j = q; xj0 = a.pos[2 * q]; xj1 = a.pos[2 * q + 1]; vj0 = a.vel[2 * q]; vj1 = a.vel[2 * q + 1];
-        mj = a.mass[q]; prj = a.pr2[q]; volj = a.vol[q];
+        mj = a.mass[q]; prj = a.pr2[q]; prlj = kSplit ? a.aux[q].w : S(0); volj = a.vol[q];
       }
       if (j == r) return;
       const S d0 = (S)(xi0 - xj0), d1 = (S)(xi1 - xj1);
       const S r2 = d0 * d0 + d1 * d1;
       if (r2 > a.skip2) return;
       const S gg = cubic_g_r2<S>(r2, a.k);
-      const S cp = -mj * (pri + prj);
+      // FP32: the contrast form of the resident force sweep (k_step_force)
+      const S cp = kSplit ? -mj * ((prj - pri) + (prlj - prli)) : -mj * (pri + prj);
       ap0.add(cp * (d0 * gg));
       ap1.add(cp * (d1 * gg));
       const S blend = j < a.n ? S(1) : a.friction;
@@ -295,8 +300,15 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
       }
     }
     if (!dense) visit_stencil(bt, BW, bx, by, lds, body);
-    const S acc0 = ap0.get() + av0.get();
-    const S acc1 = (ap1.get() + av1.get()) + (-a.gravity);
+    S acc0, acc1;
+    if (kSplit) {
+      const double qi = (double)pri + (double)prli;
+      acc0 = (S)((hi(ap0) + hi(av0)) - 2.0 * qi * (double)a.gsum[2 * r]);
+      acc1 = (S)(((hi(ap1) + hi(av1)) - 2.0 * qi * (double)a.gsum[2 * r + 1]) - (double)a.gravity);
+    } else {
+      acc0 = ap0.get() + av0.get();
+      acc1 = (ap1.get() + av1.get()) + (-a.gravity);
+    }
     if (acc_out) { acc_out[2 * r] = acc0; acc_out[2 * r + 1] = acc1; }
     // symplectic Euler + clamp (droplet_impact.py:1745-1756, :1293-1312)
     const double dt = fl->step_dt;
@@ -460,7 +472,7 @@ int sim_rebuild_tiles(tsph_sim* sim, hipStream_t stream) {
                                               sim->orig.as<int64_t>(), cs.cells.as<int64_t>(),
                                               sim->pos[other].as<double>(), sim->vel[other].as<double>(),
                                               sim->mass2.as<S>(), sim->orig2.as<int64_t>(),
-                                              sim->cells_new.as<int64_t>());
+                                              sim->cells_new.as<int64_t>(), sizeof(S) == 4 ? nt : 0);
   TSPH_CHECK_LAUNCH();
   std::swap(sim->mass, sim->mass2);
   std::swap(sim->orig, sim->orig2);
@@ -515,7 +527,10 @@ int queue_tiles(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream) {
   a.pos = sim->pos[sim->cur].as<double>();
   a.vel = sim->vel[sim->cur].as<double>();
   a.mass = sim->mass.as<S>();
+  a.mass_lo = sizeof(S) == 4 ? sim->mass.as<S>() + sim->nt : nullptr;
   a.rho = sim->rho.as<S>(); a.p = sim->p.as<S>(); a.pr2 = sim->pr2.as<S>(); a.vol = sim->vol.as<S>();
+  a.aux = sizeof(S) == 4 ? sim->aux.as<Vec4<S>>() : nullptr;     // q_lo of every row
+  a.gsum = sizeof(S) == 4 ? sim->gsum.as<S>() : nullptr;
   TileGeom g{sim->ext[0], sim->ext[1], sim->ntile[1], sim->tile_dim[0], sim->tile_dim[1],
              sim->ext[0] * sim->ext[1]};
   const TileRec* tf = sim->tiles_f.as<TileRec>();

@@ -15,18 +15,23 @@ template <typename S> struct WcsphArgs {
   double skip2d;
   S rho0, bulk, gamma, inv_gamma, ratio_floor, rho_floor;
   double rho0d;       // rho0 in FP64: the EOS forms rho / rho0 - 1 there
+  double bulkd, gammad, inv_gammad, ratio_floord;   // the EOS constants in FP64
   S gravity, nu2, friction, eta2, dt, diff_scale, eig_floor;
   int64_t n, nw;
   const double* pos;  // (n + nw, D)
   const double* vel;  // (n + nw, D)
   const S* mass;      // (n + nw)
+  const S* mass_lo;   // FP32 resident steps: the part of each mass that FP32 dropped (or NULL)
   const double* wall_accel;
   S* rho;             // (n + nw): fluid rows from sweep A, wall rows from sweep W
   S* p;
   S* pr2;             // p / rho^2
   S* vol;             // m / rho
   S* rho_lo;          // optional: low part of the stored density (FP32 mode)
-  Vec4<S>* aux;       // FP32 slot sweeps: (m, p / rho^2, m / rho, 0) per row for the force sweep
+  Vec4<S>* aux;       // FP32 slot sweeps: (m, q_hi, m / rho, q_lo) per row for the force sweep,
+                      // q = p / rho^2 = q_hi + q_lo to about 2^-48
+  S* p_lo;            // FP32 resident steps: the part of p that the stored FP32 p dropped
+  S* gsum;            // FP32 resident steps: (n, D) sum_j m_j grad W_ij of the density sweep (FP64 sum)
   // opt-in branches of the 2-D operator rhs (unused by the resident steppers)
   double wrap;        // > 0: x separations taken modulo wrap (x-periodic strip)
   int shepard;        // density * lattice_sum / (sum_j W_ij + W(0))
@@ -56,6 +61,10 @@ inline WcsphArgs<S> make_wcsph_args(const tsph_wcsph_params& p, int dim) {
   a.ratio_floor = S(0.01);
   a.rho_floor = S(0.1 * p.rho0);
   a.rho0d = p.rho0;
+  a.bulkd = p.rho0 * (p.sound_speed * p.sound_speed) / p.gamma;
+  a.gammad = p.gamma;
+  a.inv_gammad = 1.0 / p.gamma;
+  a.ratio_floord = 0.01;
   a.gravity = S(p.gravity);
   a.nu2 = S(2.0 * p.viscosity);
   // with walls present and no friction, fluid-wall pairs carry no viscosity

@@ -77,9 +77,10 @@ class WcsphSimulation:
         self.code = _lib.F64 if precision == "fp64" else _lib.F32
         self.sdt = torch.float64 if self.code == _lib.F64 else torch.float32
         self.device = dev
-        mass_all = torch.cat((torch.as_tensor(mass, dtype=torch.float64, device=dev),
-                              torch.full((self.nw,), float(wall_mass), dtype=torch.float64,
-                                         device=dev))).to(self.sdt).contiguous()
+        mass64 = torch.cat((torch.as_tensor(mass, dtype=torch.float64, device=dev),
+                            torch.full((self.nw,), float(wall_mass), dtype=torch.float64,
+                                       device=dev))).contiguous()
+        mass_all = mass64.to(self.sdt).contiguous()
         pos_all = torch.cat((pos, wall)).contiguous()
         p = SimParams()
         op = p.op
@@ -141,7 +142,11 @@ class WcsphSimulation:
         _lib.check(_lib.lib().tsph_sim_create(C.byref(p), _lib.ptr(pos_all), _lib.ptr(vel),
                                               _lib.ptr(mass_all), stream, C.byref(handle)))
         self.handle = handle
-        self._keep = (pos_all, vel, mass_all)   # alive until the queued copies ran
+        if self.code == _lib.F32:
+            # FP32 steps keep each mass as two floats and sum densities with
+            # the FP64 value (the pressure contrast at depth H is ~h / H of p)
+            _lib.check(_lib.lib().tsph_sim_set_masses_f64(handle, _lib.ptr(mass64), stream))
+        self._keep = (pos_all, vel, mass_all, mass64)   # alive until the queued copies ran
         torch.cuda.current_stream(dev).synchronize()
         self._keep = None
 
