@@ -570,12 +570,13 @@ def dam_break_bench(args) -> dict:
             wall_s, rebuilds = timed_window(sim, args.steps)
             sweep, ntimed = kernel_window(sim, min(args.steps, 1000))
             pairs = sim.count_pairs(2.0 * h) if prec == "fp32" else None
+            lay = sim.layout()
             st = sim.read()
             state = (st["position"].double().cpu().numpy(), st["velocity"].double().cpu().numpy())
         finally:
             sim.close()
         runs[prec] = {"wall_s": wall_s, "rebuilds": rebuilds, "sweep": sweep, "timed_launches": ntimed,
-                      "pairs": pairs, "state": state, "warm": warm, "t_start": t_start}
+                      "pairs": pairs, "state": state, "warm": warm, "t_start": t_start, "layout": lay}
         print(f"dam break {prec}: {1000 * wall_s / args.steps:.4f} ms/step, {rebuilds} rebuilds in "
               f"{args.steps} steps", file=sys.stderr, flush=True)
     r32 = runs["fp32"]
@@ -622,7 +623,9 @@ def dam_break_bench(args) -> dict:
                           "includes every list rebuild in it"},
         "pair_interactions_per_s": r32["pairs"] * args.steps / r32["wall_s"],
         "pairs_within_2h": r32["pairs"],
-        "layout": {"chosen": tuned["best"], "trial_ms_per_step": tuned["ms"]},
+        "layout": {"chosen": tuned["best"], "trial_ms_per_step": tuned["ms"],
+                   "halo_tiles": r32["layout"]["halo_tiles"], "tiles": r32["layout"]["tiles"],
+                   "largest_halo_rows": r32["layout"]["halo_max"]},
         "algorithmic_GBps_step": step_bytes * args.steps / r32["wall_s"] / 1e9,
         "algorithmic_bytes_per_step": step_bytes,
         "step_roofline_frac": step_bytes * args.steps / r32["wall_s"] / 1e9 / HBM_PEAK_GBS,

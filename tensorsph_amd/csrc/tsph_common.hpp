@@ -138,13 +138,14 @@ template <> TSPH_D double cubic_g_r2<double>(double r2, const KernelConst<double
 template <> TSPH_D float cubic_g_r2<float>(float r2, const KernelConst<float>& k) {
   const float rinv = r2 > 1e-24f ? __builtin_amdgcn_rsqf(r2) : 1e12f;
   const float q = (r2 * rinv) * k.inv_h;
-  // both pieces, then one select: 2 - q clamped at 0 makes the outer piece
-  // vanish beyond the support (a nested select here compiles to a branch)
-  const float d1 = q * fmaf(2.25f, q, -3.0f);
+  // both pieces with sigma / h folded in, then one select: 2 - q clamped at
+  // 0 makes the outer piece vanish beyond the support (a nested select here
+  // compiles to a branch)
+  const float s = k.sigma_over_h;
+  const float d1 = q * fmaf(2.25f * s, q, -3.0f * s);
   const float t = fmaxf(2.0f - q, 0.0f);
-  const float d2 = -0.75f * (t * t);
-  const float dshape = q < 1.0f ? d1 : d2;
-  return k.sigma_over_h * dshape * rinv;
+  const float d2 = (-0.75f * s) * (t * t);
+  return (q < 1.0f ? d1 : d2) * rinv;
 }
 
 // sqrt(r2) for a squared separation, to about 5e-14 relative: the FP32

@@ -563,18 +563,31 @@ __global__ void __launch_bounds__(kBlock) k_step_wall(WcsphArgs<S> a, NB nbr, Si
     // that the wall rows' q = q_hi + q_lo carries the pressure contrast to
     // their fluid neighbours as the fluid rows' q does (store_fluid_state)
     Sum<double> num, den;
-    nbr.template visit<D>(iw, false, [&](int64_t f) {
-      double rw[D], r2 = 0.0;
+    const double gy = -(double)a.gravity;
+    // four entries at a time: their loads first, then the terms in list order
+    nbr.template visit_chunks<D, 1>(iw, false, [&](const int32_t* j4, const bool* keep4) {
+      double pf[4][D], ps[4], rf[4];
 #pragma unroll
-      for (int d = 0; d < D; ++d) { rw[d] = xw[d] - a.pos[f * D + d]; r2 += rw[d] * rw[d]; }
-      if (r2 > a.skip2d) return;
-      double r, rinv;
-      sqrt_rinv_newton(r2, r, rinv);
-      const double wgt = cubic_w_select(r, a.kd);
-      double src = (double)a.p[f] + (a.p_lo ? (double)a.p_lo[f] : 0.0);
-      if (a.gravity != S(0)) src = src + (double)a.rho[f] * (-(double)a.gravity * rw[1]);
-      num.add(src * wgt);
-      den.add(wgt);
+      for (int u = 0; u < 4; ++u) {
+        const int64_t f = j4[u];
+#pragma unroll
+        for (int d = 0; d < D; ++d) pf[u][d] = a.pos[f * D + d];
+        ps[u] = (double)a.p[f] + (double)a.p_lo[f];
+        rf[u] = (double)a.rho[f];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double rw[D], r2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) { rw[d] = xw[d] - pf[u][d]; r2 += rw[d] * rw[d]; }
+        const bool keep = keep4[u] && !(r2 > a.skip2d);
+        double r, rinv;
+        sqrt_rinv_newton(r2, r, rinv);
+        const double wgt = cubic_w_select(r, a.kd);
+        const double src = gy != 0.0 ? ps[u] + rf[u] * (gy * rw[1]) : ps[u];
+        num.add_if(keep, src * wgt);
+        den.add_if(keep, wgt);
+      }
     });
     const double dd = den.get() > 1e-30 ? den.get() : 1e-30;
     double pw = num.get() / dd;
@@ -810,15 +823,18 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     } else {
       pri = a.pr2[i];
     }
-    Sum<S> ap[D], av[D];
+    Sum<S> at[D];                                   // pressure + viscous terms
     // every list entry is evaluated, four at a time; an entry outside the
     // support has its gradient factor zeroed, which zeroes its pressure and
     // viscous terms (a frictionless wall pair's viscous term is zero through
-    // its blend), so the sums hold the same terms the branching form adds.
+    // its zero viscosity factor), so the sums hold the terms the branching
+    // form adds.
     // fetch(u, xj, vj, q, wall) gives entry u of the chunk, from the halo or
     // from the arrays (32-bit byte offsets).
+    // 2 nu for a fluid pair, 2 nu x friction for a wall pair (0 without friction)
+    const S nu2w = a.nu2 * a.friction;
     auto chunk = [&](const bool* keep4, auto&& fetch) {
-      S tp[D][4], tv[D][4];
+      S tt[D][4];
       // the four entries' loads first, then their arithmetic
       double pj4[4][D], vj4[4][D];
       Vec4<S> q4[4];                                // m, p / rho^2, m / rho
@@ -830,32 +846,25 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
         const double* pj = pj4[u];
         const double* vj = vj4[u];
         const Vec4<S>& q = q4[u];
-        const bool wall = wall4[u];
         S df[D], r2 = S(0);
 #pragma unroll
         for (int d = 0; d < D; ++d) { df[d] = (S)(xi[d] - pj[d]); r2 += df[d] * df[d]; }
         const bool keep = keep4[u] && !(r2 > a.skip2);
-        S g = cubic_g_r2<S>(r2, a.k);
+        S g = cubic_g_r2<S>(r2, a.k);                // <= 0 inside the support
         g = keep ? g : S(0);
         S cp;
         if constexpr (kSplit) cp = -q.x * ((q.y - pri) + (q.w - prl));
         else cp = -q.x * (pri + q.y);
+        const S cpg = cp * g;
+        // Morris: -(2 nu) (m_j / rho_j) relu(-(x_ij . grad W)) / (r^2 + eta^2) v_ij,
+        // with x_ij . grad W = g r^2 <= 0, so relu(-g r^2) = -g r^2
+        const S nub = wall4[u] ? nu2w : a.nu2;
+        const S cv = (nub * q.z) * quot<S>(g * r2, r2 + a.eta2);
 #pragma unroll
-        for (int d = 0; d < D; ++d) tp[d][u] = cp * (df[d] * g);
-        const S blend = wall ? a.friction : S(1);
-        S dg = S(0);
-#pragma unroll
-        for (int d = 0; d < D; ++d) dg += df[d] * (df[d] * g);
-        const S relu = -dg > S(0) ? -dg : S(0);
-        const S cv = -(a.nu2 * blend) * (q.z * quot<S>(relu, r2 + a.eta2));
-#pragma unroll
-        for (int d = 0; d < D; ++d) tv[d][u] = cv * (S)(vi[d] - vj[d]);
+        for (int d = 0; d < D; ++d) tt[d][u] = cpg * df[d] + cv * (S)(vi[d] - vj[d]);
       }
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        ap[d].add4(tp[d][0], tp[d][1], tp[d][2], tp[d][3]);
-        av[d].add4(tv[d][0], tv[d][1], tv[d][2], tv[d][3]);
-      }
+      for (int d = 0; d < D; ++d) at[d].add4(tt[d][0], tt[d][1], tt[d][2], tt[d][3]);
     };
     if (HALO && H >= 0) {
       hl.template visit_chunks<D, TSPH_HALO_CH>(i, [&](const int32_t* l4, const bool* keep4) {
@@ -885,11 +894,10 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
       const double qi = (double)pri + (double)prl;
 #pragma unroll
       for (int d = 0; d < D; ++d)
-        acc[d] = (S)(((hi(ap[d]) + hi(av[d])) - 2.0 * qi * (double)a.gsum[i * D + d]) +
-                     (d == 1 ? -(double)a.gravity : 0.0));
+        acc[d] = (S)((hi(at[d]) - 2.0 * qi * (double)a.gsum[i * D + d]) + (d == 1 ? -(double)a.gravity : 0.0));
     } else {
 #pragma unroll
-      for (int d = 0; d < D; ++d) acc[d] = (ap[d].get() + av[d].get()) + (d == 1 ? -a.gravity : S(0));
+      for (int d = 0; d < D; ++d) acc[d] = at[d].get() + (d == 1 ? -a.gravity : S(0));
     }
     if (acc_out) {
 #pragma unroll
