@@ -5,10 +5,13 @@
 dam_break_3d.py:103-143; ``Kleefsman3DConfig`` kleefsman_obstacle_3d.py:80-126.
 Same names, same defaults, so a caller's existing ``Config(...)`` works
 unchanged.  Options whose operators are not on the particle pipeline
-(surface tension, compression regularisation, surface-pressure closure,
-Shepard renormalisation, boundary-consistent density, transport velocity,
-shifting/TVC) are accepted as fields but rejected by ``validate_config`` when
-switched on, with a message naming the option.
+(surface tension, compression regularisation, the transport-velocity
+formulation, initial-mass calibration) are accepted as fields but rejected
+by ``validate_config`` when switched on, with a message naming the option.
+The surface-pressure closure, Shepard renormalisation and boundary-consistent
+density run in the operator rhs and the resident stepper; Lind shifting and
+the transport-velocity correction (``shift_every`` / ``tvc_every``) run in
+``dam_break.run_case``.
 """
 from __future__ import annotations
 
@@ -272,15 +275,17 @@ _UNSUPPORTED = (
 )
 
 
-# branches the operator rhs (tsph_wcsph_rhs_ex) implements; the resident
-# stepper does not
+# branches the operator rhs (tsph_wcsph_rhs_ex) and the resident stepper
+# (queue_wall_stage, the Shepard density sweep) both implement
 _RHS_ONLY = ("surface_pressure_closure", "density_renormalization", "boundary_consistent_density")
 
 
 def check_supported(cfg: Config, rhs: bool = False, shifting: bool = False) -> None:
     """Reject opt-in branches that are not part of the particle pipeline
-    (with rhs=True, those the operator rhs does not implement; with
-    shifting=True the caller runs Lind shifting, shift_every)."""
+    (with rhs=True the closure, Shepard and boundary-consistent branches pass,
+    as the rhs and the stepper run them; with shifting=True the caller runs
+    Lind shifting and the transport-velocity correction, shift_every /
+    tvc_every)."""
     for name, on in _UNSUPPORTED:
         if rhs and name in _RHS_ONLY:
             continue
@@ -344,4 +349,4 @@ def validate_config(cfg: Config) -> None:
     if cfg.dt > lim * (1.0 + 1.0e-9):
         raise ValueError(f"dt={cfg.dt:g} exceeds the density-diffusion limit {lim:.3e} s "
                          f"(delta={cfg.density_diffusion:g})")
-    check_supported(cfg)
+    check_supported(cfg, rhs=True, shifting=True)

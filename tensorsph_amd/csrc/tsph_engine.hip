@@ -1865,12 +1865,18 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   memset(sim->ring, 0, sizeof(StepRecord) * kRing);
   for (int r = 0; r < kRing; ++r) {
     he = hipEventCreateWithFlags(&sim->ev[r], hipEventDisableTiming);
-    if (he == hipSuccess) he = hipEventCreateWithFlags(&sim->pev[r], hipEventDisableTiming);
+    // pev only orders the record copy (a kernel on this device) after the
+    // packing kernel: a device-scope release is enough, and a system-scope
+    // one would write back the L2 after every step
+    if (he == hipSuccess)
+      he = hipEventCreateWithFlags(&sim->pev[r], hipEventDisableTiming | hipEventDisableSystemFence);
     if (he != hipSuccess) { set_error("hipEventCreate: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   }
   for (int r = 0; r < kRing; ++r)
     for (int q = 0; q < 4; ++q) {
-      he = hipEventCreate(&sim->tev[r][q]);
+      // timing only: no system-scope fence, whose L2 writeback would be
+      // counted in the sweep it brackets
+      he = hipEventCreateWithFlags(&sim->tev[r][q], hipEventDisableSystemFence);
       if (he != hipSuccess) { set_error("hipEventCreate: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
     }
   // initial state: caller order; velocities of wall rows are zero
