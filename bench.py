@@ -275,13 +275,14 @@ def serving_bench(n_requests: int, steps: int, n_seq: int, n_api: int) -> dict:
     eng.step(0)                                   # capture the step graphs
     footprint = eng.footprint_bytes()
     rate = n_requests / 20.0                      # requests arrive faster than they finish
-    rep = serve(eng, cases, rate)
+    rep = serve(eng, cases, rate, tune=True)
+    launch = eng.launch
     eng.close()
     del cases
     summary = rep.summary()
     stride = max(1, n_requests // max(n_seq, 1))
     sample = reqs[::stride][:n_seq]
-    seq_s, _ = run_sequential(sample)
+    seq_s, _ = run_sequential(sample, launch=launch)      # the tuned launch shape too
     seq_rate = len(sample) / seq_s
     api = None
     if n_api:
@@ -297,12 +298,14 @@ def serving_bench(n_requests: int, steps: int, n_seq: int, n_api: int) -> dict:
         "completion_s_p50": summary["completion_s_p50"], "completion_s_p99": summary["completion_s_p99"],
         "particle_updates_per_s": summary["particle_updates_per_s"],
         "batch_steps": summary["batch_steps"], "failed": summary["failed"],
+        "autotune": summary["autotune"],
         "engine_capacity": {"fluid_rows": fluid_cap, "wall_rows": wall_cap, "bytes": footprint,
                             "fraction_of_stream_rows": fluid_cap / rows},
         "sequential": {
             "sample": f"{len(sample)} requests of the stream (every {stride}th), one after another",
             "engine_alone": {"sims_per_s": seq_rate, "seconds": seq_s,
-                             "how": "each case alone on a batch engine sized for it, step graphs "
+                             "how": "each case alone on a batch engine sized for it, with the "
+                                    "launch shape the served engine's autotuner chose, step graphs "
                                     "captured before timing"},
             "single_case_api": api and dict(api, how="the reference drivers' loop per case: "
                                             "adaptive dt from max|v|, PairListCache(4 dx, 0.4 dx), "

@@ -297,6 +297,14 @@ int tsph_batch_admit(tsph_batch* b, const tsph_dp_case* c, const double* pos, co
 /* Advances every active run by nsteps steps (runs stop at their own step
  * count); synchronises at the end. */
 int tsph_batch_step(tsph_batch* b, int64_t nsteps, hipStream_t stream);
+/* Launch shape of the captured step: rows per workgroup of the four delta-plus
+ * pair sweeps (sweep_block) and of the list build (list_block), each 64, 128,
+ * 192 or 256, and xcd_chunks 1 (consecutive workgroups of one XCD take
+ * neighbouring rows, which share that XCD's L2) or 0 (round-robin).  Results
+ * do not depend on it; the step graph is captured again at the next step.
+ * For the autotuner (serving.BatchEngine.autotune); no counterpart in the
+ * reference, which runs one case per Python loop. */
+int tsph_batch_launch(tsph_batch* b, int32_t sweep_block, int32_t list_block, int32_t xcd_chunks);
 /* Finished runs since their admission: slots and states (2 done, 3 failed:
  * a nonpositive wall or fluid density, the reference's FloatingPointError). */
 int tsph_batch_poll(tsph_batch* b, int32_t* slots, int32_t* states, int32_t max, int32_t* n);

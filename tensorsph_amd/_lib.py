@@ -174,6 +174,7 @@ def _declare(L):
         "tsph_batch_create": [C.POINTER(BatchCaps), _vp, C.POINTER(_vp)],
         "tsph_batch_admit": [_vp, C.POINTER(DpCase)] + [_vp] * 7 + [C.POINTER(C.c_int32), _vp],
         "tsph_batch_step": [_vp, _i64, _vp],
+        "tsph_batch_launch": [_vp, C.c_int32, C.c_int32, C.c_int32],
         "tsph_batch_poll": [_vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int32,
                             C.POINTER(C.c_int32)],
         "tsph_batch_read": [_vp, C.c_int32, _vp, _vp, _vp, C.POINTER(_i64), _vp],

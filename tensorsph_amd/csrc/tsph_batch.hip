@@ -479,6 +479,9 @@ struct tsph_batch {
   int64_t steps_done = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   hipStream_t graph_stream = nullptr;
+  // launch shape (tsph_batch_launch): rows per workgroup of the pair sweeps
+  // and of the list build, XCD-chunked row blocks of the pair sweeps
+  int blk_sweep = 256, blk_lists = 256, xcd = 1;
   std::vector<Range> fslot, wslot;               // per simulation slot
   std::vector<unsigned char> used;               // slot in use
   std::vector<int64_t> done_at;                  // host copy of the finishing step
@@ -544,11 +547,13 @@ static int batch_queue_step(tsph_batch* B, int side, hipStream_t stream) {
   TSPH_TRY(queue_scan(pc, d.ccount, d.total_cells, d.cap_cells, d.cstart, stream));
   k_b_scatter<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   k_b_cell_sort<<<nblk(d.cap_cells), kBlock, 0, stream>>>(d);
-  k_b_lists<<<grid_xcd(ncloud), kBlock, 0, stream>>>(d);
+  k_b_lists<<<grid_xcd(ncloud, B->blk_lists), B->blk_lists, 0, stream>>>(d);
   if (B->caps.sorted_lists) k_b_sort_lists<<<nblk(ncloud), kBlock, 0, stream>>>(d);
   TSPH_CHECK_LAUNCH();
   // the δ⁺ step lists index walls as n_fluid + w: here n_fluid = cap_f
   DpArgs a = batch_dp_args(B, d);
+  a.block = B->blk_sweep;
+  a.xcd = B->xcd;
   TSPH_TRY(dp_queue_step(a, nullptr, d.fidx, nullptr, d.widx, stream));
   k_b_vmax<<<nblk(d.cap_f), kBlock, 0, stream>>>(d);
   k_b_advance<<<nblk(d.cap_s), kBlock, 0, stream>>>(d);
@@ -781,6 +786,22 @@ int tsph_batch_step(tsph_batch* B, int64_t nsteps, hipStream_t stream) {
   }
   TSPH_CHECK_HIP(hipStreamSynchronize(stream));
   TSPH_REQUIRE(!(B->host_rec->flags & 1u), TSPH_ERR_OOM, "cell capacity exceeded (grow tsph_batch_caps.cells)");
+  return TSPH_OK;
+}
+
+int tsph_batch_launch(tsph_batch* B, int32_t sweep_block, int32_t list_block, int32_t xcd_chunks) {
+  TSPH_REQUIRE(B, TSPH_ERR_ARG, "batch must not be NULL");
+  auto ok = [](int32_t b) { return b == 64 || b == 128 || b == 192 || b == 256; };
+  TSPH_REQUIRE(ok(sweep_block) && ok(list_block), TSPH_ERR_ARG,
+               "workgroup sizes must be 64, 128, 192 or 256 rows (got %d, %d)", (int)sweep_block,
+               (int)list_block);
+  TSPH_REQUIRE(xcd_chunks == 0 || xcd_chunks == 1, TSPH_ERR_ARG, "xcd_chunks must be 0 or 1");
+  if (B->blk_sweep == sweep_block && B->blk_lists == list_block && B->xcd == xcd_chunks) return TSPH_OK;
+  B->blk_sweep = sweep_block;
+  B->blk_lists = list_block;
+  B->xcd = xcd_chunks;
+  for (int side = 0; side < 2; ++side)          // captured again at the next step
+    if (B->graph[side]) { (void)hipGraphExecDestroy(B->graph[side]); B->graph[side] = nullptr; }
   return TSPH_OK;
 }
 

@@ -116,6 +116,12 @@ TSPH_D void dp_wall_force(const DpSim& P, double gx, double gy, double nx, doubl
 // ---------------------------------------------------------------- sweep 0 (per-particle volume)
 // m / rho of every fluid row, read by the pair loops of sweeps A-C instead of
 // a division per pair (the same IEEE quotient)
+// the row block of this workgroup: XCD-chunked (consecutive blocks of one
+// XCD take neighbouring rows, which then share its L2) or round-robin
+TSPH_D int64_t dp_block(const DpArgs& a) {
+  return a.xcd ? xcd_block(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+}
+
 __global__ void __launch_bounds__(kBlock) k_dp_prep(DpArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
@@ -128,7 +134,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_prep(DpArgs a) {
 // ---------------------------------------------------------------- sweep A (walls)
 __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __restrict__ woff,
                                                     const int32_t* __restrict__ widx) {
-  const int64_t w = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t w = dp_block(a) * blockDim.x + threadIdx.x;
   if (w >= a.nw) return;
   const int s = dp_sim(a.wsim, w);
   if (s < 0) return;                          // a hole of the batch layout
@@ -182,7 +188,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_wall(DpArgs a, const int64_t* __r
 // ---------------------------------------------------------------- sweep B (forces, geometry)
 __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ idx) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t t = dp_block(a) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -269,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_force(DpArgs a, const int64_t* __
 // ---------------------------------------------------------------- sweep C (continuity, normal)
 __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_t* __restrict__ off,
                                                           const int32_t* __restrict__ idx) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t t = dp_block(a) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -348,7 +354,7 @@ __global__ void __launch_bounds__(kBlock) k_dp_continuity(DpArgs a, const int64_
 // ---------------------------------------------------------------- sweep D (shift, positions)
 __global__ void __launch_bounds__(kBlock) k_dp_shift(DpArgs a, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ idx) {
-  const int64_t t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int64_t t = dp_block(a) * blockDim.x + threadIdx.x;
   if (t >= a.nf) return;
   const int s = dp_sim(a.fsim, t);
   if (s < 0) return;                          // a hole of the batch layout
@@ -432,11 +438,12 @@ int dp_queue_step(const DpArgs& a, const int64_t* off, const int32_t* idx, const
   // chunked block -> XCD mapping: neighbouring rows are processed on one XCD
   // and share its L2 (round-robin blocks made every L2 fetch every neighbour)
   if (a.nf > 0) k_dp_prep<<<(unsigned)((a.nf + kBlock - 1) / kBlock), kBlock, 0, stream>>>(a);
-  if (a.nw > 0) k_dp_wall<<<grid_xcd(a.nw), kBlock, 0, stream>>>(a, woff, widx);
+  const int bs = a.block;
+  if (a.nw > 0) k_dp_wall<<<grid_xcd(a.nw, bs), bs, 0, stream>>>(a, woff, widx);
   if (a.nf > 0) {
-    k_dp_force<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
-    k_dp_continuity<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
-    k_dp_shift<<<grid_xcd(a.nf), kBlock, 0, stream>>>(a, off, idx);
+    k_dp_force<<<grid_xcd(a.nf, bs), bs, 0, stream>>>(a, off, idx);
+    k_dp_continuity<<<grid_xcd(a.nf, bs), bs, 0, stream>>>(a, off, idx);
+    k_dp_shift<<<grid_xcd(a.nf, bs), bs, 0, stream>>>(a, off, idx);
   }
   TSPH_CHECK_LAUNCH();
   return TSPH_OK;

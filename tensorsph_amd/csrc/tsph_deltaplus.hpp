@@ -34,6 +34,11 @@ struct DpArgs {
   double *wall_p, *wall_rho, *reaction;
   double *acc, *vnew, *lam, *inv, *gr, *rho_new, *normal, *strength, *shift, *x_new;
   unsigned char* surface;
+  // launch shape of the four pair sweeps: rows per workgroup, and whether
+  // consecutive workgroups take contiguous chunks of rows per XCD (1) or
+  // round-robin (0); results do not depend on either
+  int block = 256;
+  int xcd = 1;
 };
 
 int dp_queue_step(const DpArgs& a, const int64_t* off, const int32_t* idx, const int64_t* woff,

@@ -180,6 +180,12 @@ FLUID_ROW_BYTES = 2 * 16 * 2 + 2 * 8 + 8 + 4 + 4 + 4 + 4 + 64 * 4 + 16 + 8 + 32 
 WALL_ROW_BYTES = 3 * 16 + 4 + 4 + 4 + 4 + 40 * 4 + 8 + 8 + 16 + 8
 
 
+# launch shapes the autotuner tries: (sweep rows per workgroup, list-build
+# rows per workgroup, XCD-chunked sweep blocks)
+AUTOTUNE_CANDIDATES = ((256, 256, True), (128, 256, True), (64, 256, True), (192, 256, True),
+                       (256, 128, True), (128, 128, True), (256, 256, False), (128, 256, False))
+
+
 class BatchEngine:
     """Resident batch of δ⁺ simulations on one GPU (tsph_batch_*)."""
 
@@ -201,6 +207,7 @@ class BatchEngine:
             _lib.check(_lib.lib().tsph_batch_create(C.byref(caps), self._s(), C.byref(h)))
         self._h = h
         self.cases: dict[int, DpCase] = {}
+        self.launch = (256, 256, True)
 
     def _s(self):
         return C.c_void_p(self.stream.cuda_stream)
@@ -228,6 +235,35 @@ class BatchEngine:
 
     def step(self, n: int = 1) -> None:
         _lib.check(_lib.lib().tsph_batch_step(self._h, int(n), self._s()))
+
+    def set_launch(self, sweep_block: int = 256, list_block: int = 256, xcd_chunks: bool = True) -> None:
+        """Launch shape of the step: rows per workgroup of the four δ⁺ pair
+        sweeps and of the list build (64, 128, 192 or 256), and XCD-chunked
+        or round-robin row blocks for the sweeps.  Results are the same for
+        every shape; the step graph is captured again at the next step."""
+        _lib.check(_lib.lib().tsph_batch_launch(self._h, int(sweep_block), int(list_block),
+                                                1 if xcd_chunks else 0))
+        self.launch = (int(sweep_block), int(list_block), bool(xcd_chunks))
+
+    def autotune(self, candidates=None, steps: int = 3) -> dict:
+        """Times ``steps`` batch steps of the resident mix for each launch
+        shape (after one untimed step that captures its graph), keeps the
+        fastest and returns {"best": shape, "ms": {shape: ms per step},
+        "steps": batch steps taken}.  The trial steps advance the resident
+        simulations like any other step."""
+        cands = list(candidates) if candidates is not None else list(AUTOTUNE_CANDIDATES)
+        ms, taken = {}, 0
+        for shape in cands:
+            self.set_launch(*shape)
+            self.step(1)                          # captures this shape's graph
+            t0 = time.perf_counter()
+            self.step(steps)
+            ms[shape] = 1e3 * (time.perf_counter() - t0) / steps
+            taken += 1 + steps
+        best = min(ms, key=ms.get)
+        self.set_launch(*best)
+        return {"best": best, "ms": {f"{a}/{b}/{'xcd' if c else 'rr'}": v for (a, b, c), v in ms.items()},
+                "steps": taken}
 
     def poll(self):
         buf_s = (C.c_int32 * 1024)()
@@ -279,6 +315,7 @@ class ServeReport:
     step_ms: list = field(default_factory=list)
     particle_updates: int = 0
     completion_s: list = field(default_factory=list)
+    tuning: dict | None = None       # the autotuner's result, when serve ran it
 
     def summary(self) -> dict:
         ms = np.asarray(self.step_ms) if self.step_ms else np.zeros(1)
@@ -288,15 +325,20 @@ class ServeReport:
                 "step_ms_p50": float(np.percentile(ms, 50)), "step_ms_p99": float(np.percentile(ms, 99)),
                 "particle_updates_per_s": self.particle_updates / self.wall_s if self.wall_s > 0 else 0.0,
                 "completion_s_p50": float(np.percentile(self.completion_s, 50)) if self.completion_s else 0.0,
-                "completion_s_p99": float(np.percentile(self.completion_s, 99)) if self.completion_s else 0.0}
+                "completion_s_p99": float(np.percentile(self.completion_s, 99)) if self.completion_s else 0.0,
+                "autotune": self.tuning}
 
 
-def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_result=None) -> ServeReport:
+def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_result=None,
+          tune: bool = False, tune_steps: int = 3) -> ServeReport:
     """Continuous batching: requests (built DpCase, or (kind, kwargs) built on
     arrival) arrive at Poisson times of
     the given rate; each is admitted when a slot and rows are free, every
     active simulation advances one step per batch step, finished ones are read
-    back (``on_result(index, state, result)``) and released."""
+    back (``on_result(index, state, result)``) and released.  With ``tune``
+    the engine's launch shape is autotuned once, on the resident mix of the
+    first time the engine is full (its trial steps are batch steps of the
+    stream and count as such)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, size=len(requests)))
     waiting = deque()
@@ -311,13 +353,22 @@ def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_re
             item = requests[nxt]
             waiting.append((nxt, arrivals[nxt], item if isinstance(item, DpCase) else build_case(*item)))
             nxt += 1
+        full = False
         while waiting:
             idx, t_arr, case = waiting[0]
             slot = engine.admit(case)
             if slot is None:
+                full = True
                 break
             waiting.popleft()
             running[slot] = (idx, t_arr)
+        if tune and full and rep.tuning is None and running:
+            active_rows = sum(engine.cases[s].n for s in running)
+            ts = time.perf_counter()
+            rep.tuning = engine.autotune(steps=tune_steps)
+            per = 1e3 * (time.perf_counter() - ts) / rep.tuning["steps"]
+            rep.step_ms.extend([per] * rep.tuning["steps"])
+            rep.particle_updates += active_rows * rep.tuning["steps"]
         if not running:
             if nxt < len(requests):
                 time.sleep(max(0.0, min(arrivals[nxt] - (time.perf_counter() - t0), 0.01)))
@@ -340,15 +391,18 @@ def serve(engine: BatchEngine, requests, rate_per_s: float, seed: int = 1, on_re
     return rep
 
 
-def run_sequential(requests, device=None) -> tuple[float, list]:
+def run_sequential(requests, device=None, launch=None) -> tuple[float, list]:
     """Single-case runs one after another, each on an engine sized for that
-    case alone with its step graphs captured before the clock starts: returns
-    (seconds, per-case seconds)."""
+    case alone (with the launch shape ``launch``, if given) and its step
+    graphs captured before the clock starts: returns (seconds, per-case
+    seconds)."""
     times = []
     for kind, kw in requests:
         case = build_case(kind, kw)
         eng = BatchEngine(case.n + 8, case.n_wall + 8, sims=1, device=device)
         try:
+            if launch is not None:
+                eng.set_launch(*launch)
             eng.step(0)                       # capture the step graphs
             t0 = time.perf_counter()
             slot = eng.admit(case)

@@ -150,3 +150,68 @@ def test_admission_limits():
         eng2.admit(case)
     eng.close()
     eng2.close()
+
+
+def _mix_final(launch):
+    """Three runs admitted together, stepped to completion on one engine with
+    the given launch shape; final states by admission order."""
+    from tensorsph_amd.serving import BatchEngine
+    cases = [_small("drop", 1200, steps=12), _small("impact", 2500, steps=10),
+             _small("drop", 800, steps=15, radius=0.08)]
+    eng = BatchEngine(sum(c.n for c in cases) + 64, sum(c.n_wall for c in cases) + 64, sims=4)
+    try:
+        if launch is not None:
+            eng.set_launch(*launch)
+        slots = [eng.admit(c) for c in cases]
+        out, left = {}, set(slots)
+        while left:
+            eng.step(1)
+            for s, _ in eng.poll():
+                if s in left:
+                    out[s] = eng.read(s)
+                    eng.release(s)
+                    left.discard(s)
+        return [out[s] for s in slots]
+    finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("launch", [(128, 256, True), (64, 128, False), (192, 64, True)])
+def test_launch_shapes_give_the_same_bits(launch):
+    """tsph_batch_launch changes workgroup sizes and the block-to-XCD mapping
+    only: every run's final state is bitwise the default shape's."""
+    print(f"launch shape {launch}", flush=True)
+    ref = _mix_final(None)
+    got = _mix_final(launch)
+    for r, g in zip(ref, got):
+        assert r["steps"] == g["steps"]
+        for k in ("position", "velocity", "density"):
+            assert torch.equal(r[k], g[k]), k
+
+
+def test_autotune_keeps_the_fastest_candidate():
+    from tensorsph_amd.serving import BatchEngine
+    cases = [_small("drop", 1500, steps=200), _small("impact", 3000, steps=200)]
+    eng = BatchEngine(sum(c.n for c in cases) + 64, sum(c.n_wall for c in cases) + 64, sims=4)
+    try:
+        for c in cases:
+            assert eng.admit(c) is not None
+        cands = [(256, 256, True), (128, 128, False)]
+        res = eng.autotune(candidates=cands, steps=2)
+        assert res["best"] in cands and eng.launch == res["best"]
+        assert len(res["ms"]) == 2 and all(v > 0 for v in res["ms"].values())
+        assert res["steps"] == 2 * 3 and eng.stats()["steps"] == 6
+    finally:
+        eng.close()
+
+
+def test_launch_shape_rejects_other_sizes():
+    from tensorsph_amd.serving import BatchEngine
+    eng = BatchEngine(4096, 512, sims=2)
+    try:
+        with pytest.raises(ValueError):
+            eng.set_launch(96, 256, True)
+        with pytest.raises(ValueError):
+            eng.set_launch(256, 512, True)
+    finally:
+        eng.close()
