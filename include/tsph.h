@@ -355,7 +355,13 @@ typedef struct tsph_psim tsph_psim;
 
 int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* pos,
                      const double* vel, const void* mass, hipStream_t stream, tsph_psim** out);
-/* Advances nsteps leapfrog steps; synchronises once per step for the skin test. */
+/* Advances nsteps leapfrog steps (threed/solver.py:285-299).  The skin test
+ * runs on the device and the host reads it two steps later: a step whose
+ * drift went past half the skin has its evaluation (and the queued steps
+ * after it) skipped on the device, and the host rebuilds the lists from that
+ * step's drifted positions, evaluates it and queues the rest again, so the
+ * rebuild schedule is the sequential loop's.  Returns when the last step has
+ * finished. */
 int tsph_psim_step(tsph_psim* sim, int64_t nsteps, hipStream_t stream);
 /* The conservation row (TSPH_CONSERVATION_ROW doubles, see
  * tsph_conservation_row) of the current state.  Synchronises. */

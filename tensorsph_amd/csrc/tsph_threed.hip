@@ -55,7 +55,13 @@ template <typename S> struct ThreedArgs {
   int order4;     // fourth-order viscous operator (set only with nu != 0)
   S* fourth;      // I4_i = sum_j r^4 lap_ij m_j / rho_ij (order 4)
   S* lap1;        // (n, 3) first pair Laplacian of the velocity, before nu (order 4)
+  // resident periodic stepper: a step whose drift went past half the skin
+  // sets this word, and the sweeps of that step (and the queued steps after
+  // it) do nothing until the host has rebuilt the lists; NULL elsewhere
+  const unsigned int* skip;
 };
+
+template <typename S> TSPH_D bool skipped(const ThreedArgs<S>& a) { return a.skip && *a.skip; }
 
 // x_i - x_j from FP64 coordinates, minimum image if asked, rounded to S.
 template <typename S>
@@ -82,6 +88,7 @@ template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* __restrict__ out,
                                                      int32_t* __restrict__ sup_idx, int32_t* __restrict__ sup_cnt,
                                                      int64_t K) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
@@ -110,6 +117,7 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
 // renormalised density gradient over fluid-fluid pairs (density_diffusion.py:95-142)
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, NB nb, const S* __restrict__ rho) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   if (i >= a.nf) { a.gr[3 * i] = a.gr[3 * i + 1] = a.gr[3 * i + 2] = S(0); return; }
@@ -153,6 +161,7 @@ __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, NB nb, co
 template <typename S, int FORM, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S* __restrict__ rho_in,
                                                  int diffusion) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double rho_d = (double)rho_in[i] + (a.rho_lo ? (double)a.rho_lo[i] : 0.0);
@@ -202,6 +211,7 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S
 // (operators3.py:89-93) without a floor for the wall density.
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
+  if (skipped(a)) return;
   const int64_t w = a.nf + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.n) return;
   double xw[3];
@@ -236,6 +246,7 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
 // and, for the fourth-order operator, the fourth moment I4 (:147-149)
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
@@ -274,6 +285,7 @@ template <typename S, bool KICK, typename NB, int MW, bool PK>
 __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const bool visc = a.nu != S(0);
@@ -367,6 +379,7 @@ template <typename S, bool KICK, typename NB, int MW>
 __global__ void __launch_bounds__(kBlock) k3_visc4(ThreedArgs<S> a, NB nb, const int64_t* __restrict__ orig,
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
+  if (skipped(a)) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
@@ -651,7 +664,9 @@ __global__ void __launch_bounds__(kBlock) k3_kick_drift(double* __restrict__ pos
                                                         const S* __restrict__ acc, int64_t n, double dt,
                                                         double b0, double b1, double b2,
                                                         const double* __restrict__ anchor,
-                                                        unsigned long long* __restrict__ disp) {
+                                                        unsigned long long* __restrict__ disp,
+                                                        const unsigned int* __restrict__ stale) {
+  if (*stale) return;                      // an earlier step awaits its list rebuild
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double d2 = 0.0;
   if (i < n) {
@@ -677,6 +692,30 @@ __global__ void __launch_bounds__(kBlock) k3_kick_drift(double* __restrict__ pos
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w) v = fmax(v, wv[w]);
     atomicMax(disp, ord_enc(v));
   }
+}
+
+// The skin test of the step just drifted (threed/solver.py:293-299): a
+// displacement past half the skin marks the lists stale (the step's sweeps
+// and every queued step then do nothing); the flag also goes to the host's
+// ring entry of the step.  Resets the displacement maximum for the next step.
+__global__ void k3_skin_test(unsigned long long* __restrict__ disp, double skin, unsigned int* __restrict__ stale,
+                             unsigned int* __restrict__ ring_entry) {
+  if (threadIdx.x != 0) return;
+  const unsigned long long u = *disp;
+  const unsigned long long b = (u >> 63) ? (u & 0x7fffffffffffffffull) : ~u;
+  const double d = __longlong_as_double((long long)b);
+  if (!*stale && 2.0 * d > skin) *stale = 1u;
+  *disp = 0ull;
+  *ring_entry = *stale;
+}
+
+// A kicked evaluation that was skipped still moves the velocities to the
+// other buffer, which the host switches to after every evaluation.
+__global__ void k3_carry_vel(const unsigned int* __restrict__ stale, const double* __restrict__ vel,
+                             double* __restrict__ vel_out, int64_t n3) {
+  if (!*stale) return;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n3) vel_out[k] = vel[k];
 }
 
 // Re-orders the state by the cell order of the last search; afterwards the
@@ -821,6 +860,12 @@ struct tsph_psim {
   int64_t slot_K = 0;
   int64_t ncells_dense = 0, cell_ext[3] = {1, 1, 1}, cell_stride[3] = {0, 0, 0};
   unsigned long long* disp_host = nullptr;   // pinned
+  // lagged skin test: stale word (device), per-step ring of its value (pinned)
+  // and the events that complete each queued step
+  static constexpr int kRingP = 4;
+  DevBuf stale;
+  unsigned int* ring = nullptr;              // pinned, kRingP
+  hipEvent_t ev[kRingP] = {};
   int cur = 0;
   int64_t steps = 0, rebuilds = 0;
   bool built = false, have_acc = false;
@@ -902,6 +947,9 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
   TSPH_TRY(ps->sup_idx.ensure(sizeof(int32_t) * (size_t)((n + 63) / 64) * 64 * (size_t)ps->slot_K));
   TSPH_TRY(ps->sup_cnt.ensure(sizeof(int32_t) * (size_t)n));
   TSPH_CHECK_HIP(hipMemcpyAsync(ps->anchor.ptr, ps->pos.ptr, 24 * n, hipMemcpyDeviceToDevice, stream));
+  // fresh lists: the sweeps run again, the displacement counts from here
+  TSPH_CHECK_HIP(hipMemsetAsync(ps->stale.ptr, 0, sizeof(unsigned int), stream));
+  TSPH_CHECK_HIP(hipMemsetAsync(ps->disp.ptr, 0, sizeof(unsigned long long), stream));
   ps->rebuilds++;
   ps->built = true;
   return TSPH_OK;
@@ -913,6 +961,7 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   a.pos = ps->pos.as<double>();
   a.vel = ps->vel[ps->cur].as<double>();
   a.mass = ps->mass.as<S>();
+  a.skip = ps->stale.as<unsigned int>();
   const NbSlots nb = psim_slots(ps);
   NbSlots sup = nb;
   sup.idx = ps->sup_idx.as<int32_t>();
@@ -921,6 +970,10 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
     TSPH_TRY((threed_sweeps<S, true>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
                                      ps->acc.as<S>(), 0.5 * ps->prm.dt, ps->vel[ps->cur ^ 1].as<double>(), stream,
                                      ps->uniform_mass, &sup)));
+    k3_carry_vel<<<grid_for(3 * ps->n), kBlock, 0, stream>>>(ps->stale.as<unsigned int>(),
+                                                             ps->vel[ps->cur].as<double>(),
+                                                             ps->vel[ps->cur ^ 1].as<double>(), 3 * ps->n);
+    TSPH_CHECK_LAUNCH();
     ps->cur ^= 1;
   } else {
     TSPH_TRY((threed_sweeps<S, false>(a, ps->q, psim_scratch<S>(ps), nb, ps->orig.as<int64_t>(),
@@ -929,22 +982,46 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   return TSPH_OK;
 }
 
+// Leapfrog steps (threed/solver.py:285-299) without a host wait per step:
+// each step is queued as kick-drift, the device skin test, the evaluation
+// and the second half-kick, and an event; the host reads a step's skin flag
+// two steps later.  A step whose drift went past half the skin has its
+// evaluation skipped on the device, and so has every step queued after it;
+// the host then rebuilds the lists from that step's drifted positions, runs
+// its evaluation, and queues the later steps again.  The rebuild schedule is
+// the sequential one's: the lists are rebuilt exactly when a drift goes past
+// half the skin, before that step's evaluation.
 template <typename S>
 int psim_run(tsph_psim* ps, int64_t nsteps, hipStream_t stream) {
   if (!ps->built) TSPH_TRY(psim_rebuild<S>(ps, stream));
   if (!ps->have_acc) { TSPH_TRY(psim_eval<S>(ps, false, stream)); ps->have_acc = true; }
   const double* b = ps->prm.box;
-  for (int64_t k = 0; k < nsteps; ++k) {
-    TSPH_CHECK_HIP(hipMemsetAsync(ps->disp.ptr, 0, 8, stream));
-    k3_kick_drift<S><<<grid_for(ps->n), kBlock, 0, stream>>>(ps->pos.as<double>(), ps->vel[ps->cur].as<double>(),
-        ps->acc.as<S>(), ps->n, ps->prm.dt, b[0], b[1], b[2], ps->anchor.as<double>(),
-        ps->disp.as<unsigned long long>());
-    TSPH_CHECK_LAUNCH();
-    TSPH_CHECK_HIP(hipMemcpyAsync(ps->disp_host, ps->disp.ptr, 8, hipMemcpyDeviceToHost, stream));
-    TSPH_CHECK_HIP(hipStreamSynchronize(stream));
-    if (2.0 * ord_dec_host(*ps->disp_host) > ps->skin) TSPH_TRY(psim_rebuild<S>(ps, stream));
-    TSPH_TRY(psim_eval<S>(ps, true, stream));
+  constexpr int R = tsph_psim::kRingP;
+  const int depth = 2;
+  int64_t next = 0, done = 0;
+  while (done < nsteps) {
+    while (next < nsteps && next - done < depth) {
+      k3_kick_drift<S><<<grid_for(ps->n), kBlock, 0, stream>>>(ps->pos.as<double>(), ps->vel[ps->cur].as<double>(),
+          ps->acc.as<S>(), ps->n, ps->prm.dt, b[0], b[1], b[2], ps->anchor.as<double>(),
+          ps->disp.as<unsigned long long>(), ps->stale.as<unsigned int>());
+      k3_skin_test<<<1, 64, 0, stream>>>(ps->disp.as<unsigned long long>(), ps->skin, ps->stale.as<unsigned int>(),
+                                         ps->ring + next % R);
+      TSPH_CHECK_LAUNCH();
+      TSPH_TRY(psim_eval<S>(ps, true, stream));
+      TSPH_CHECK_HIP(hipEventRecord(ps->ev[next % R], stream));
+      ++next;
+    }
+    TSPH_CHECK_HIP(hipEventSynchronize(ps->ev[done % R]));
+    if (ps->ring[done % R]) {
+      // the steps queued after it did nothing: let them finish, rebuild,
+      // evaluate this step, and queue the rest again
+      for (int64_t k = done + 1; k < next; ++k) TSPH_CHECK_HIP(hipEventSynchronize(ps->ev[k % R]));
+      TSPH_TRY(psim_rebuild<S>(ps, stream));
+      TSPH_TRY(psim_eval<S>(ps, true, stream));
+      next = done + 1;
+    }
     ps->steps++;
+    ++done;
   }
   return TSPH_OK;
 }
@@ -1101,6 +1178,10 @@ int tsph_psim_destroy(tsph_psim* ps) {
                  &ps->cell_R, &ps->entries, &ps->slot_idx, &ps->slot_cnt, &ps->slot_max, &ps->slot_posf,
                  &ps->sup_idx, &ps->sup_cnt, &ps->fourth, &ps->lap1};
   for (DevBuf* x : b) x->release();
+  ps->stale.release();
+  for (int r = 0; r < tsph_psim::kRingP; ++r)
+    if (ps->ev[r]) (void)hipEventDestroy(ps->ev[r]);
+  if (ps->ring) (void)hipHostFree(ps->ring);
   if (ps->disp_host) (void)hipHostFree(ps->disp_host);
   if (ps->ctx) tsph_ctx_destroy(ps->ctx);
   delete ps;
@@ -1143,6 +1224,22 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
   if (rc == TSPH_OK && hipHostMalloc((void**)&ps->disp_host, 8, 0) != hipSuccess) {
     ps->disp_host = nullptr;
     set_error("hipHostMalloc of the displacement word failed");
+    rc = TSPH_ERR_HIP;
+  }
+  if (rc == TSPH_OK) rc = ps->stale.ensure(sizeof(unsigned int));
+  if (rc == TSPH_OK && hipHostMalloc((void**)&ps->ring, sizeof(unsigned int) * tsph_psim::kRingP, 0) != hipSuccess) {
+    ps->ring = nullptr;
+    set_error("hipHostMalloc of the skin-test ring failed");
+    rc = TSPH_ERR_HIP;
+  }
+  for (int r = 0; r < tsph_psim::kRingP && rc == TSPH_OK; ++r)
+    if (hipEventCreateWithFlags(&ps->ev[r], hipEventDisableTiming) != hipSuccess) {
+      ps->ev[r] = nullptr;
+      set_error("hipEventCreate failed");
+      rc = TSPH_ERR_HIP;
+    }
+  if (rc == TSPH_OK && hipMemsetAsync(ps->stale.ptr, 0, sizeof(unsigned int), stream) != hipSuccess) {
+    set_error("hipMemsetAsync failed");
     rc = TSPH_ERR_HIP;
   }
   if (rc != TSPH_OK) { tsph_psim_destroy(ps); return rc; }
