@@ -159,34 +159,30 @@ TSPH_D double sqrt_r2_newton(double r2) {
   return r2 > 1.0e-30 ? r2 * y1 : 0.0;
 }
 
-// W of the cubic spline for r known to FP64 accuracy, branch-free: the two
-// pieces in Horner form, selected by q (both are positive inside their
-// ranges).  Agrees with cubic_wg's W to FP64 rounding; the FP32-compensated
-// density sweeps use it.
+// W of the cubic spline for r known to FP64 accuracy, branch-free: with
+// a = (2 - q)+ and b = (1 - q)+ both pieces are (a^3 - 4 b^3) / 4 (expanding
+// (2 - q)^3 - 4 (1 - q)^3 gives 4 - 6 q^2 + 3 q^3 on q < 1), so no select.
+// Agrees with cubic_wg's W to FP64 rounding; the FP32-compensated wall sweep
+// uses it.
 TSPH_D double cubic_w_select(double r, const KernelConst<double>& k) {
   const double q = r * k.inv_h;
-  const double s1 = fma(q * q, fma(0.75, q, -1.5), 1.0);
-  const double t = 2.0 - q;
-  const double s2 = (0.25 * t) * (t * t);
-  return k.sigma * (q < 1.0 ? s1 : (q < 2.0 ? s2 : 0.0));
+  const double a = fmax(2.0 - q, 0.0), b = fmax(1.0 - q, 0.0);
+  return (0.25 * k.sigma) * fma(a * a, a, -4.0 * (b * b * b));
 }
 
 // W and dW/dr of the cubic spline for r known to FP64 accuracy, branch-free:
 // cubic_w_select's W and the derivative cubic_wg_select forms before its
-// division by r, from one q and one 2 - q.
+// division by r (the same values to FP64 rounding).
 TSPH_D void cubic_w_dw_select(double r, const KernelConst<double>& k, double& w, double& dw) {
+  // with a = (2 - q)+ and b = (1 - q)+ both pieces are one expression:
+  // shape = (a^3 - 4 b^3) / 4 and dshape = 3 b^2 - 3 a^2 / 4 (expanding
+  // (2 - q)^3 - 4 (1 - q)^3 gives 4 - 6 q^2 + 3 q^3 on q < 1), so neither
+  // value needs a select
   const double q = r * k.inv_h;
-  const double s1 = fma(q * q, fma(0.75, q, -1.5), 1.0);
-  const double d1 = q * fma(2.25, q, -3.0);
-  // 2 - q clamped at 0: the outer piece vanishes beyond the support, so one
-  // select per value
-  const double t = fmax(2.0 - q, 0.0);
-  const double t2 = t * t;
-  const double s2 = (0.25 * t) * t2;
-  const double d2 = -0.75 * t2;
-  const bool in1 = q < 1.0;
-  w = k.sigma * (in1 ? s1 : s2);
-  dw = k.sigma_over_h * (in1 ? d1 : d2);
+  const double a = fmax(2.0 - q, 0.0), b = fmax(1.0 - q, 0.0);
+  const double a2 = a * a, b2 = b * b;
+  w = (0.25 * k.sigma) * fma(a2, a, -4.0 * (b2 * b));
+  dw = k.sigma_over_h * fma(3.0, b2, -0.75 * a2);
 }
 
 // Byte offset into an array with a 32-bit index, so the address is the
