@@ -200,6 +200,29 @@ def precision_error(cfg, mass, wall, wmass, state, diffusion=0.0) -> dict:
             "state": "the FP64 run's state after its timed window; one step in each precision"}
 
 
+def common_state_error(make, state) -> dict:
+    """FP32-compensated against FP64 from one common state: make(prec, pos,
+    vel) builds a simulation there; one step in each precision, then the
+    largest difference of density and acceleration over the FP64 field's
+    largest magnitude."""
+    import numpy as np
+    out = {}
+    for prec in ("fp32", "fp64"):
+        sim = make(prec, *state)
+        try:
+            sim.step(1)
+            r = sim.read()
+            out[prec] = {k: r[k].double().cpu().numpy() for k in ("density", "acceleration")}
+        finally:
+            sim.close()
+
+    def scaled(a, b):
+        return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+    return {"density_max_rel": scaled(out["fp32"]["density"], out["fp64"]["density"]),
+            "acceleration_max_over_max": scaled(out["fp32"]["acceleration"], out["fp64"]["acceleration"])}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -360,11 +383,17 @@ def shear_bench(args) -> dict:
     vel = shear_wave_velocity(pos, 1e-3, 2.0 * math.pi, "single")
     mass = torch.full((pos.shape[0],), calibrated_particle_mass(dx, h, rho0), dtype=torch.float64)
     runs = {}
+    make = lambda prec, p0, v0: PeriodicSimulation(p0, v0, mass, h=h, box=1.0, rho0=rho0, sound_speed=c0,
+                                                   viscosity=nu, dt=dt, skin=0.5 * h, renormalise=True,
+                                                   precision=prec, device="cuda")
+    state = None
     for prec in ("fp32", "fp64"):
-        sim = PeriodicSimulation(pos, vel, mass, h=h, box=1.0, rho0=rho0, sound_speed=c0, viscosity=nu,
-                                 dt=dt, skin=0.5 * h, renormalise=True, precision=prec, device="cuda")
+        sim = make(prec, pos, vel)
         wall_s = _timed(sim.step, args.steps, args.warmup)
         st = sim.stats()
+        if prec == "fp64":
+            r = sim.read()
+            state = (r["position"].cpu(), r["velocity"].cpu())
         sim.close()
         runs[prec] = {"wall_s": wall_s, "value": pos.shape[0] * args.steps / wall_s, "pairs": st["pairs"],
                       "rebuilds": st["rebuilds"]}
@@ -386,7 +415,11 @@ def shear_bench(args) -> dict:
                  list_pairs=runs["fp32"]["pairs"], list_rebuilds_in_run=runs["fp32"]["rebuilds"],
                  fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
                        "algorithmic_GBps_step": v64 * b64 / 1e9},
-                 fp32c_over_fp64=v32 / v64, decay=decay)
+                 fp32c_over_fp64=v32 / v64,
+                 fp32c_vs_fp64_error=dict(common_state_error(make, state),
+                                          state="the FP64 run's state after its timed window; one "
+                                                "step in each precision"),
+                 decay=decay)
 
 
 def shear_decay(n: int) -> dict:
@@ -423,6 +456,7 @@ def kleefsman_bench(args) -> dict:
                             obstacle_fill="shell")
     cfg.dt = 0.9 * cfg.dt_from_courant()
     runs = {}
+    state = None
     for prec in ("fp32", "fp64"):
         sim, nf, nw = make_simulation(cfg, prec)
         sim.step(max(args.warmup, 1))
@@ -436,6 +470,9 @@ def kleefsman_bench(args) -> dict:
         sim.set_timing(False)
         loads = sim.loads()
         stats = sim.stats()
+        if prec == "fp64":
+            r = sim.read()
+            state = (r["position"].cpu().numpy(), r["velocity"].cpu().numpy())
         sim.close()
         runs[prec] = {"wall_s": wall_s, "value": nf * args.steps / wall_s,
                       "sweep_ms": [x / max(nsteps, 1) for x in sweep_ms], "rebuilds": stats["rebuilds"],
@@ -470,7 +507,11 @@ def kleefsman_bench(args) -> dict:
                  fp64={"value": v64, "ms_per_step": 1000.0 * runs["fp64"]["wall_s"] / args.steps,
                        "sweep_ms_per_step": dict(zip(("density", "wall", "force_integrate"),
                                                      runs["fp64"]["sweep_ms"]))},
-                 fp32c_over_fp64=v32 / v64, history=history)
+                 fp32c_over_fp64=v32 / v64,
+                 fp32c_vs_fp64_error=dict(common_state_error(
+                     lambda prec, p0, v0: make_simulation(cfg, prec, pos=p0, vel=v0)[0], state),
+                     state="the FP64 run's state after its timed window; one step in each precision"),
+                 history=history)
 
 
 def kleefsman_history(cfg, t_end: float) -> dict:

@@ -366,6 +366,14 @@ int tsph_psim_step(tsph_psim* sim, int64_t nsteps, hipStream_t stream);
 /* The conservation row (TSPH_CONSERVATION_ROW doubles, see
  * tsph_conservation_row) of the current state.  Synchronises. */
 int tsph_psim_diagnostics(tsph_psim* sim, double wavenumber, double* out, hipStream_t stream);
+/* The n masses in the caller's order, FP64 (device).  An FP32-compensated
+ * periodic stepper keeps each as two floats and sums densities with the FP64
+ * value: near rho0 the Tait law turns a density error into
+ * c0^2 rho0 (dm / m), so a mass rounded to FP32 would shift every pressure
+ * by about 7 B 2^-24.  Any time; takes effect at the next evaluation.  The
+ * reference holds masses in the run's dtype (threed/solver.py:84-90); its
+ * FP64 runs see the FP64 masses this keeps. */
+int tsph_psim_set_masses_f64(tsph_psim* sim, const double* mass, hipStream_t stream);
 /* State in the caller's order: pos, vel (n, 3) FP64; rho (n) and acc (n, 3)
  * of the last evaluation in the scalar type.  Any pointer may be NULL. */
 int tsph_psim_read(tsph_psim* sim, double* pos, double* vel, void* rho, void* acc,

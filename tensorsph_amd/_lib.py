@@ -166,6 +166,7 @@ def _declare(L):
                                  _vp],
         "tsph_psim_create": [C.POINTER(ThreedParams), _dbl, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
         "tsph_psim_step": [_vp, _i64, _vp],
+        "tsph_psim_set_masses_f64": [_vp, _vp, _vp],
         "tsph_psim_diagnostics": [_vp, _dbl, _dp, _vp],
         "tsph_psim_read": [_vp, _vp, _vp, _vp, _vp, _vp],
         "tsph_psim_stats": [_vp, _ip],

@@ -44,6 +44,7 @@ template <typename S> struct ThreedArgs {
   const double* pos;
   const double* vel;
   const S* mass;
+  const S* mass_lo;  // resident FP32 stepper: the part of each mass FP32 dropped (NULL elsewhere)
   S* rho;       // summation density (+ diffusion): the returned rho
   S* rho_u;     // density used by the forces (Adami walls: Tait-consistent)
   S* p;
@@ -107,11 +108,13 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
     keep = keep && !(r2 > a.skip2d);
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
-    acc.add_if(keep, (double)a.mass[j] * w);
+    const double mj = a.mass_lo ? (double)a.mass[j] + (double)a.mass_lo[j] : (double)a.mass[j];
+    acc.add_if(keep, mj * w);
     if (sup_idx) sup.push_if(keep, (int32_t)j);
   });
   if (sup_idx) sup_cnt[i] = sup.finish();
-  store_rho<S>(hi(acc) + (double)a.mass[i] * a.kd.w0d, out, a.rho_lo, i);
+  const double mi = a.mass_lo ? (double)a.mass[i] + (double)a.mass_lo[i] : (double)a.mass[i];
+  store_rho<S>(hi(acc) + mi * a.kd.w0d, out, a.rho_lo, i);
 }
 
 // renormalised density gradient over fluid-fluid pairs (density_diffusion.py:95-142)
@@ -718,6 +721,19 @@ __global__ void k3_carry_vel(const unsigned int* __restrict__ stale, const doubl
   if (k < n3) vel_out[k] = vel[k];
 }
 
+// row k's mass from the caller's FP64 masses (row k holds caller row
+// orig[k]): FP32 runs as hi + lo (lo at lo_off), FP64 runs as given
+template <typename S>
+__global__ void k3_set_masses(const double* __restrict__ m, const int64_t* __restrict__ orig, int64_t n,
+                              S* __restrict__ mass, int64_t lo_off) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double v = m[orig[k]];
+  const S h = (S)v;
+  mass[k] = h;
+  if (lo_off) mass[lo_off + k] = (S)(v - (double)h);
+}
+
 // Re-orders the state by the cell order of the last search; afterwards the
 // search's own slot order is the identity.
 template <typename S>
@@ -728,7 +744,7 @@ __global__ void __launch_bounds__(kBlock) k3_permute(const int64_t* __restrict__
                                                      const int64_t* __restrict__ cells, double* __restrict__ pos2,
                                                      double* __restrict__ vel2, S* __restrict__ mass2,
                                                      S* __restrict__ acc2, int64_t* __restrict__ orig2,
-                                                     int64_t* __restrict__ cells2) {
+                                                     int64_t* __restrict__ cells2, int64_t lo_off) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t o = order[k];
@@ -740,6 +756,7 @@ __global__ void __launch_bounds__(kBlock) k3_permute(const int64_t* __restrict__
     cells2[3 * k + d] = cells[3 * o + d];
   }
   mass2[k] = mass[o];
+  if (lo_off) mass2[lo_off + k] = mass[lo_off + o];   // the low parts (FP32 runs)
   orig2[k] = orig[o];
 }
 
@@ -918,7 +935,8 @@ int psim_rebuild(tsph_psim* ps, hipStream_t stream) {
                                           ps->vel[ps->cur].as<double>(), ps->mass.as<S>(), ps->acc.as<S>(),
                                           ps->orig.as<int64_t>(), cs.cells.as<int64_t>(), ps->pos2.as<double>(),
                                           ps->vel2.as<double>(), ps->mass2.as<S>(), ps->acc2.as<S>(),
-                                          ps->orig2.as<int64_t>(), ps->cells2.as<int64_t>());
+                                          ps->orig2.as<int64_t>(), ps->cells2.as<int64_t>(),
+                                          sizeof(S) == 4 ? (int64_t)n : 0);
   k3_order_identity<<<g, kBlock, 0, stream>>>(cs.order.as<int64_t>(), n);
   TSPH_CHECK_LAUNCH();
   std::swap(ps->pos, ps->pos2);
@@ -961,6 +979,7 @@ int psim_eval(tsph_psim* ps, bool kick, hipStream_t stream) {
   a.pos = ps->pos.as<double>();
   a.vel = ps->vel[ps->cur].as<double>();
   a.mass = ps->mass.as<S>();
+  a.mass_lo = sizeof(S) == 4 ? ps->mass.as<S>() + ps->n : nullptr;
   a.skip = ps->stale.as<unsigned int>();
   const NbSlots nb = psim_slots(ps);
   NbSlots sup = nb;
@@ -1215,7 +1234,8 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
                     &ps->part, &ps->diag, &ps->rho_raw, &ps->rho, &ps->rho_u, &ps->p, &ps->moment, &ps->gr,
                     &ps->rho_lo, &ps->pack};
   const size_t sizes[] = {24 * n, 24 * n, 24 * n, 24 * n, 24 * n, 24 * n, 24 * n,
-                          s * (n + 1), s * (n + 1), 3 * s * (n + 1), 3 * s * (n + 1), 8 * n, 8 * n, 64,
+                          (s == 4 ? 2 : 1) * s * (n + 1), (s == 4 ? 2 : 1) * s * (n + 1),
+                          3 * s * (n + 1), 3 * s * (n + 1), 8 * n, 8 * n, 64,
                           8 * DG_N * ((size_t)grid_for(ps->n) + 1), 8 * DG_N, s * (n + 1), s * (n + 1), s * (n + 1),
                           s * (n + 1), s * (n + 1), 3 * s * (n + 1), s * (n + 1), 4 * s * (n + 1)};
   for (size_t k = 0; k < sizeof(bufs) / sizeof(bufs[0]) && rc == TSPH_OK; ++k) rc = bufs[k]->ensure(sizes[k]);
@@ -1246,6 +1266,9 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
   hipError_t e = hipMemcpyAsync(ps->pos.ptr, pos, 24 * n, hipMemcpyDeviceToDevice, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(ps->vel[0].ptr, vel, 24 * n, hipMemcpyDeviceToDevice, stream);
   if (e == hipSuccess) e = hipMemcpyAsync(ps->mass.ptr, mass, s * n, hipMemcpyDeviceToDevice, stream);
+  // FP32 runs: the masses' low parts (the second half of the buffer) start at
+  // zero; tsph_psim_set_masses_f64 fills them
+  if (e == hipSuccess && s == 4) e = hipMemsetAsync(ps->mass.as<float>() + n, 0, s * n, stream);
   if (e == hipSuccess) e = hipMemsetAsync(ps->acc.ptr, 0, 3 * s * n, stream);
   if (e != hipSuccess) {
     set_error("periodic stepper init: %s", hipGetErrorString(e));
@@ -1271,6 +1294,19 @@ int tsph_psim_create(const tsph_threed_params* prm, double skin, const double* p
     return TSPH_ERR_HIP;
   }
   *out = ps;
+  return TSPH_OK;
+}
+
+int tsph_psim_set_masses_f64(tsph_psim* ps, const double* mass, hipStream_t stream) {
+  TSPH_REQUIRE(ps && mass, TSPH_ERR_ARG, "NULL argument");
+  const int64_t n = ps->n;
+  if (ps->s == 8)
+    k3_set_masses<double><<<grid_for(n), kBlock, 0, stream>>>(mass, ps->orig.as<int64_t>(), n,
+                                                              ps->mass.as<double>(), 0);
+  else
+    k3_set_masses<float><<<grid_for(n), kBlock, 0, stream>>>(mass, ps->orig.as<int64_t>(), n,
+                                                             ps->mass.as<float>(), n);
+  TSPH_CHECK_LAUNCH();
   return TSPH_OK;
 }
 

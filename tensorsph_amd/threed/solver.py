@@ -85,6 +85,13 @@ class PeriodicSimulation:
                                                    _lib.ptr(self._v), _lib.ptr(self._m),
                                                    _lib.stream_handle(dev), C.byref(handle)))
         self._h = handle
+        if self.code == _lib.F32:
+            # FP32 steps keep each mass as two floats and sum densities with
+            # the FP64 value (near rho0 a mass rounding shifts p by ~7 B 2^-24)
+            self._m64 = m.to(dev, torch.float64).contiguous()
+            with torch.cuda.device(dev):
+                _lib.check(_lib.lib().tsph_psim_set_masses_f64(handle, _lib.ptr(self._m64),
+                                                               _lib.stream_handle(dev)))
 
     def _stream(self):
         return _lib.stream_handle(self.device)
