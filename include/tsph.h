@@ -370,7 +370,8 @@ int tsph_psim_diagnostics(tsph_psim* sim, double wavenumber, double* out, hipStr
  * periodic stepper keeps each as two floats and sums densities with the FP64
  * value: near rho0 the Tait law turns a density error into
  * c0^2 rho0 (dm / m), so a mass rounded to FP32 would shift every pressure
- * by about 7 B 2^-24.  Any time; takes effect at the next evaluation.  The
+ * by about 7 B 2^-24.  Any time; takes effect at the next evaluation;
+ * synchronises the stream (it re-checks whether every mass is equal).  The
  * reference holds masses in the run's dtype (threed/solver.py:84-90); its
  * FP64 runs see the FP64 masses this keeps. */
 int tsph_psim_set_masses_f64(tsph_psim* sim, const double* mass, hipStream_t stream);

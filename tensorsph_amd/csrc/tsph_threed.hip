@@ -85,7 +85,10 @@ TSPH_D void load3(const double* p, int64_t i, double* x) {
 // summation_density (operators3.py:70-79): pair sums, then the self term
 // With sup_idx the entries within the support are also written, in list
 // order, as the support lists of the later sweeps (NbSlots layout of width K).
-template <typename S, typename NB>
+// MM selects how the masses are read: 0 one gathered value per entry, 1 the
+// two gathered parts (mass + mass_lo) of the FP32 stepper, 2 every particle
+// has the same mass (m_0 times the kernel sum, no gather).
+template <typename S, typename NB, int MM>
 __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* __restrict__ out,
                                                      int32_t* __restrict__ sup_idx, int32_t* __restrict__ sup_cnt,
                                                      int64_t K) {
@@ -108,13 +111,35 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
     keep = keep && !(r2 > a.skip2d);
     double w, g;
     cubic_wg<double>(sqrt(r2), a.kd, w, g);
-    const double mj = a.mass_lo ? (double)a.mass[j] + (double)a.mass_lo[j] : (double)a.mass[j];
-    acc.add_if(keep, mj * w);
+    if constexpr (MM == 2) {
+      acc.add_if(keep, w);
+    } else {
+      const double mj = MM == 1 ? (double)a.mass[j] + (double)a.mass_lo[j] : (double)a.mass[j];
+      acc.add_if(keep, mj * w);
+    }
     if (sup_idx) sup.push_if(keep, (int32_t)j);
   });
   if (sup_idx) sup_cnt[i] = sup.finish();
-  const double mi = a.mass_lo ? (double)a.mass[i] + (double)a.mass_lo[i] : (double)a.mass[i];
-  store_rho<S>(hi(acc) + mi * a.kd.w0d, out, a.rho_lo, i);
+  if constexpr (MM == 2) {
+    const double m0 = a.mass_lo ? (double)a.mass[0] + (double)a.mass_lo[0] : (double)a.mass[0];
+    store_rho<S>(m0 * (hi(acc) + a.kd.w0d), out, a.rho_lo, i);
+  } else {
+    const double mi = MM == 1 ? (double)a.mass[i] + (double)a.mass_lo[i] : (double)a.mass[i];
+    store_rho<S>(hi(acc) + mi * a.kd.w0d, out, a.rho_lo, i);
+  }
+}
+
+// Queues k3_density with the mass read that fits the stepper.
+template <typename S, typename NB>
+void launch_density3(const ThreedArgs<S>& a, NB nb, S* out, int32_t* sup_idx, int32_t* sup_cnt, int64_t K,
+                     bool uniform_mass, hipStream_t stream) {
+  const unsigned g = grid_for(a.n);
+  if (uniform_mass)
+    k3_density<S, NB, 2><<<g, kBlock, 0, stream>>>(a, nb, out, sup_idx, sup_cnt, K);
+  else if (a.mass_lo)
+    k3_density<S, NB, 1><<<g, kBlock, 0, stream>>>(a, nb, out, sup_idx, sup_cnt, K);
+  else
+    k3_density<S, NB, 0><<<g, kBlock, 0, stream>>>(a, nb, out, sup_idx, sup_cnt, K);
 }
 
 // renormalised density gradient over fluid-fluid pairs (density_diffusion.py:95-142)
@@ -517,14 +542,14 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   // every later sweep reads them instead of the build-radius lists
   if constexpr (std::is_same<NB, NbSlots>::value) {
     if (support) {
-      k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, const_cast<int32_t*>(support->idx),
-                                                  const_cast<int32_t*>(support->cnt), nb.K);
+      launch_density3<S, NB>(a, nb, diff ? b.rho_raw : b.rho, const_cast<int32_t*>(support->idx),
+                             const_cast<int32_t*>(support->cnt), nb.K, uniform_mass, stream);
       nb = *support;
     } else {
-      k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0);
+      launch_density3<S, NB>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0, uniform_mass, stream);
     }
   } else {
-    k3_density<S, NB><<<g, kBlock, 0, stream>>>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0);
+    launch_density3<S, NB>(a, nb, diff ? b.rho_raw : b.rho, nullptr, nullptr, 0, uniform_mass, stream);
   }
   if (diff) {
     if (q.diff_form == 1) {
@@ -605,7 +630,7 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
   S* p_in = (S*)p;               // read only
   switch (op) {
     case 0:
-      k3_density<S, NbList><<<g, kBlock, 0, stream>>>(a, nb, (S*)out, nullptr, nullptr, 0);
+      launch_density3<S, NbList>(a, nb, (S*)out, nullptr, nullptr, 0, false, stream);
       break;
     case 1:
       a.rho_u = rho_in; a.p = p_in; a.nu = S(0); a.has_gravity = 0; a.order4 = 0;
@@ -1307,6 +1332,16 @@ int tsph_psim_set_masses_f64(tsph_psim* ps, const double* mass, hipStream_t stre
     k3_set_masses<float><<<grid_for(n), kBlock, 0, stream>>>(mass, ps->orig.as<int64_t>(), n,
                                                              ps->mass.as<float>(), n);
   TSPH_CHECK_LAUNCH();
+  // the equal-mass flag of the sweeps, from the new masses (diag is scratch
+  // until the next diagnostics call writes it)
+  unsigned int* flag = ps->diag.as<unsigned int>();
+  TSPH_CHECK_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned int), stream));
+  k3_mass_differs<double><<<grid_for(n), kBlock, 0, stream>>>(mass, n, flag);
+  TSPH_CHECK_LAUNCH();
+  unsigned int differs = 1;
+  TSPH_CHECK_HIP(hipMemcpyAsync(&differs, flag, sizeof(differs), hipMemcpyDeviceToHost, stream));
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  ps->uniform_mass = differs == 0;
   return TSPH_OK;
 }
 
