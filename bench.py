@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--decay-n", type=int, default=64,
                     help="lattice edge of the full-length decay run of the shear workload (0: skip)")
     ap.add_argument("--fluid3d", type=float, default=20e6, help="fluid particles of the box case")
+    ap.add_argument("--layout3d", choices=("default", "lists", "halo"), default="default",
+                    help="sweep layout of the box case (default: the stepper's own)")
     ap.add_argument("--t-end", type=float, default=0.0,
                     help="box case: also integrate both precisions to this physical time and "
                          "compare the force / moment histories on the box")
@@ -457,8 +459,11 @@ def kleefsman_bench(args) -> dict:
     cfg.dt = 0.9 * cfg.dt_from_courant()
     runs = {}
     state = None
+    layout = getattr(args, "layout3d", "default")
     for prec in ("fp32", "fp64"):
         sim, nf, nw = make_simulation(cfg, prec)
+        if layout != "default":
+            sim.configure(tiles=False, halo=layout == "halo")
         sim.step(max(args.warmup, 1))
         torch.cuda.synchronize()
         sim.set_timing(True)
@@ -470,13 +475,14 @@ def kleefsman_bench(args) -> dict:
         sim.set_timing(False)
         loads = sim.loads()
         stats = sim.stats()
+        lay = sim.layout()
         if prec == "fp64":
             r = sim.read()
             state = (r["position"].cpu().numpy(), r["velocity"].cpu().numpy())
         sim.close()
         runs[prec] = {"wall_s": wall_s, "value": nf * args.steps / wall_s,
                       "sweep_ms": [x / max(nsteps, 1) for x in sweep_ms], "rebuilds": stats["rebuilds"],
-                      "force": loads[-1, 1:4].tolist(), "moment": loads[-1, 4:7].tolist()}
+                      "force": loads[-1, 1:4].tolist(), "moment": loads[-1, 4:7].tolist(), "layout": lay}
     v32, v64 = runs["fp32"]["value"], runs["fp64"]["value"]
     history = kleefsman_history(cfg, args.t_end) if args.t_end > 0 else None
     s = 4
@@ -500,6 +506,7 @@ def kleefsman_bench(args) -> dict:
                            "kernel": "k_step_force (force + viscosity + integrate + box clamp)",
                            "algorithmic_bytes_per_launch": b_force},
                  list_rebuilds_in_run=runs["fp32"]["rebuilds"],
+                 layout={"fp32c": runs["fp32"]["layout"], "fp64": runs["fp64"]["layout"]},
                  box_load_last_step={"force_N_fp32c": f32, "force_N_fp64": f64,
                                      "moment_Nm_fp32c": runs["fp32"]["moment"],
                                      "moment_Nm_fp64": runs["fp64"]["moment"],

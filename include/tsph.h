@@ -469,9 +469,11 @@ int tsph_sim_time(tsph_sim* sim, double* out, hipStream_t stream);
 int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows);
 /* Sweep layout.  tiles = 1 runs the LDS cell-tile sweeps with tile_x x
  * tile_y cells per workgroup ((tile_x+2)(tile_y+2) <= 108, tile_x tile_y <=
- * 64); tiles = 0 runs the neighbour-list sweeps (the 3-D path); tiles = 2
- * (the 2-D default) runs the neighbour-list sweeps with each workgroup's
- * neighbourhood rows staged in LDS (the same bits as tiles = 0).
+ * 64; 2-D only); tiles = 0 runs the neighbour-list sweeps from the global
+ * arrays; tiles = 2 (the default) runs the neighbour-list sweeps with each
+ * 256-row workgroup's neighbourhood rows staged in LDS (up to 1,024 rows in
+ * 2-D, 2,048 in 3-D; a tile with more reads the global arrays), with the same
+ * bits as tiles = 0.
  * Takes effect at the next step, which rebuilds the layout.  No counterpart
  * in the reference, which has one layout (its neighbour pairs). */
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y);

@@ -338,9 +338,9 @@ __global__ void __launch_bounds__(kBlock) k_step_density(WcsphArgs<S> a, NB nbr,
     if (H >= 0) {
       hm = hx + (size_t)H * D;
       // every load of the staging is issued before the first store (up to
-      // kHaloCap / kBlock rows per thread)
-      const uint32_t* hr = hv.rows + tile * kHaloCap;
-      constexpr int QS = kHaloCap / kBlock;
+      // halo_cap / kBlock rows per thread)
+      const uint32_t* hr = hv.rows + tile * halo_cap<D>();
+      constexpr int QS = halo_cap<D>() / kBlock;
       double px[QS][D], pm[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
@@ -738,12 +738,15 @@ TSPH_D double wave_sum(double v) {
   return v;
 }
 
-// One halo row as the force sweep stages it: position, velocity and the
-// (m, p / rho^2, m / rho) record.
+// One halo row as the force sweep stages it: the (m, p / rho^2, m / rho)
+// record, position, and velocity in the sweep's precision (FP32 runs form
+// the viscous velocity difference from the FP32 neighbour velocity in both
+// the halo and the global path): 40 bytes in 2-D and 56 in 3-D for FP32, so
+// a 3-D halo of up to 2,816 rows fits a CU's LDS.
 template <int D, typename S> struct HaloRec {
+  S q[4];                 // not Vec4: its 16-byte alignment would pad the record
   double x[D];
-  double v[D];
-  Vec4<S> q;
+  S v[D];
 };
 
 template <int D, typename S, bool METERS, typename NB, bool HALO = false>
@@ -773,17 +776,18 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
     if (H > hv.max_rows) H = -1;                     // more rows than this launch staged: global path
     if (H >= 0) {
       // every load of the staging is issued before the first store (up to
-      // kHaloCap / kBlock rows per thread)
-      const uint32_t* hr = hv.rows + tile * kHaloCap;
-      constexpr int QS = kHaloCap / kBlock;
-      double rx[QS][D], rv[QS][D];
+      // halo_cap / kBlock rows per thread)
+      const uint32_t* hr = hv.rows + tile * halo_cap<D>();
+      constexpr int QS = halo_cap<D>() / kBlock;
+      double rx[QS][D];
+      S rv[QS][D];
       S r0[QS], r1[QS], r2[QS], r3[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
         const int s = threadIdx.x + q * kBlock;
         const uint32_t j = hr[s < H ? s : 0];
 #pragma unroll
-        for (int d = 0; d < D; ++d) { rx[q][d] = a.pos[(int64_t)j * D + d]; rv[q][d] = a.vel[(int64_t)j * D + d]; }
+        for (int d = 0; d < D; ++d) { rx[q][d] = a.pos[(int64_t)j * D + d]; rv[q][d] = (S)a.vel[(int64_t)j * D + d]; }
         if constexpr (sizeof(S) == 4) {
           const Vec4<S> w = a.aux[j];
           r0[q] = w.x; r1[q] = w.y; r2[q] = w.z; r3[q] = w.w;
@@ -798,7 +802,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
           HaloRec<D, S>& rr = hrec[s];
 #pragma unroll
           for (int d = 0; d < D; ++d) { rr.x[d] = rx[q][d]; rr.v[d] = rv[q][d]; }
-          rr.q = Vec4<S>{r0[q], r1[q], r2[q], r3[q]};
+          rr.q[0] = r0[q]; rr.q[1] = r1[q]; rr.q[2] = r2[q]; rr.q[3] = r3[q];
         }
       }
       __syncthreads();
@@ -872,7 +876,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
           const HaloRec<D, S>& rr = hrec[l4[u]];
 #pragma unroll
           for (int d = 0; d < D; ++d) { pj[d] = rr.x[d]; vj[d] = rr.v[d]; }
-          q = rr.q;
+          q = Vec4<S>{rr.q[0], rr.q[1], rr.q[2], rr.q[3]};
           wall = l4[u] >= Fh;
         });
       });
@@ -883,7 +887,7 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
           const double* xp = at_bytes(a.pos, j * (uint32_t)(D * sizeof(double)));
           const double* vp = at_bytes(a.vel, j * (uint32_t)(D * sizeof(double)));
 #pragma unroll
-          for (int d = 0; d < D; ++d) { pj[d] = xp[d]; vj[d] = vp[d]; }
+          for (int d = 0; d < D; ++d) { pj[d] = xp[d]; vj[d] = (double)(S)vp[d]; }
           if constexpr (sizeof(S) == 4) q = *at_bytes(a.aux, j * (uint32_t)sizeof(Vec4<S>));
           else q = Vec4<S>{a.mass[j], a.pr2[j], a.vol[j], S(0)};
           wall = j >= (uint32_t)a.n;
@@ -1178,9 +1182,9 @@ int build_sim_slots(tsph_sim* sim, const double* npos, double rc, const int64_t*
 }
 template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*, const int64_t*, hipStream_t);
 
-// The 2-D halo layout's lists (tiles == 2): per tile of kHaloRows fluid rows
+// The halo layout's lists (tiles == 2): per tile of kHaloRows fluid rows
 // one workgroup builds the tile's global and halo lists and its halo
-// (k_halo_lists2); the wall rows' lists come from the cell walk; the reduce
+// (k_halo_lists); the wall rows' lists come from the cell walk; the reduce
 // kernel writes the build's report into pinned memory and marks the step
 // flags stale when a list did not fit its slots or a row was binned from
 // outside the grid.  lagged: one pass at the current width, no wait; the
@@ -1190,23 +1194,51 @@ template int build_sim_slots<2>(tsph_sim*, const double*, double, const int64_t*
 // synchronised.  The sweeps stage halos of up to halo_max rows (a tile with a
 // larger halo reads the global arrays): the exact largest halo after a
 // synchronised build, the last report's plus a margin after a lagged one,
-// within kHaloLds bytes of staged rows (rec_bytes each), which keeps the
-// sweeps within the default dynamic LDS limit of a launch.
-constexpr size_t kHaloLds = 60 * 1024;
+// within halo_lds_budget<D>() bytes of staged rows (rec_bytes each): in 2-D the
+// default dynamic LDS limit of a launch (four force workgroups per CU), in
+// 3-D the whole 160 KiB of a CU, which the 3-D halos (1,500 - 2,000 rows)
+// need; the sweeps raise their launch limit to it (allow_halo_lds).
+// (the sweeps' own static LDS, a few hundred bytes of reductions, takes
+// part of the last 4 KiB)
+template <int D> constexpr size_t halo_lds_budget() { return D == 2 ? 60 * 1024 : 156 * 1024; }
 constexpr int kHaloMargin = 64;
+
+// Raises a halo sweep's dynamic LDS limit to halo_lds_budget<3>() once per
+// kernel (its static LDS and the budget must fit the CU's 160 KiB).
+template <auto KERNEL>
+int allow_halo_lds() {
+  static const int status = [] {
+    const void* fn = reinterpret_cast<const void*>(KERNEL);
+    hipFuncAttributes attr{};
+    hipError_t e = hipFuncGetAttributes(&attr, fn);
+    if (e != hipSuccess) { set_error("hipFuncGetAttributes: %s", hipGetErrorString(e)); return TSPH_ERR_HIP; }
+    if (attr.sharedSizeBytes + halo_lds_budget<3>() > 160 * 1024) {
+      set_error("halo sweep: %zu bytes of static LDS leave no room for %zu staged", (size_t)attr.sharedSizeBytes,
+                halo_lds_budget<3>());
+      return TSPH_ERR_HIP;
+    }
+    e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)halo_lds_budget<3>());
+    if (e != hipSuccess) { set_error("hipFuncSetAttribute: %s", hipGetErrorString(e)); return TSPH_ERR_HIP; }
+    return TSPH_OK;
+  }();
+  return status;
+}
+
+template <int D>
 int build_halo_lists(tsph_sim* sim, const double* npos, double rc, size_t rec_bytes, bool lagged,
                      hipStream_t stream) {
   const int64_t n = sim->n, nt = sim->nt;
   sim->halo_ok = false;
-  TSPH_TRY(build_cell_table<2>(sim, sim->bgrid.ext, nullptr, stream));
+  TSPH_TRY(build_cell_table<D>(sim, sim->bgrid.ext, nullptr, stream));
   const int64_t ntiles = (n + kHaloRows - 1) / kHaloRows;
   sim->halo_tiles = ntiles;
   const int64_t groups = (nt + 63) / 64;
-  const int cap = (int)std::min<size_t>((size_t)kHaloCap, kHaloLds / rec_bytes);
+  constexpr int kCap = halo_cap<D>();
+  const int cap = (int)std::min<size_t>((size_t)kCap, halo_lds_budget<D>() / rec_bytes);
   const double rc2 = rc * rc;
   TSPH_TRY(sim->slot_cnt.ensure(sizeof(int32_t) * (size_t)(nt > 0 ? nt : 1)));
   TSPH_TRY(sim->slot_max.ensure(sizeof(unsigned int)));
-  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)(ntiles > 0 ? ntiles : 1) * kHaloCap));
+  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)(ntiles > 0 ? ntiles : 1) * kCap));
   TSPH_TRY(sim->halo_cnt.ensure(sizeof(int2) * (size_t)(ntiles > 0 ? ntiles : 1)));
   TSPH_TRY(sim->halo_hmax.ensure(sizeof(unsigned int) * (2 * (size_t)ntiles + 2)));
   unsigned int* blk = sim->halo_hmax.as<unsigned int>();
@@ -1227,22 +1259,22 @@ int build_halo_lists(tsph_sim* sim, const double* npos, double rc, size_t rec_by
     TSPH_CHECK_HIP(hipMemsetAsync(sim->slot_max.ptr, 0, sizeof(unsigned int), stream));
     if (ntiles > 0) {
       if (K > 0)
-        k_halo_lists2<true><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
+        k_halo_lists<D, true><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
             sim_cells(sim), npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
             sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
       else
-        k_halo_lists2<false><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
+        k_halo_lists<D, false><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
             sim_cells(sim), npos, n, rc2, cap, sim->slot_cnt.as<int32_t>(), gidx, hidx, K,
             sim->halo_rows.as<uint32_t>(), sim->halo_cnt.as<int2>(), blk);
     }
     if (nt > n) {
       const unsigned g = grid_xcd(nt - n);
       if (K > 0)
-        k_slot_lists<2, true, false, false><<<g, kBlock, 0, stream>>>(
+        k_slot_lists<D, true, false, false><<<g, kBlock, 0, stream>>>(
             sim_cells(sim), npos, nullptr, n, n, nt, rc2, 0.f, 0.f, 0.0, 0.0, 0.0, sim->slot_cnt.as<int32_t>(),
             gidx, K, sim->slot_max.as<unsigned int>(), nullptr);
       else
-        k_slot_lists<2, false, false, false><<<g, kBlock, 0, stream>>>(
+        k_slot_lists<D, false, false, false><<<g, kBlock, 0, stream>>>(
             sim_cells(sim), npos, nullptr, n, n, nt, rc2, 0.f, 0.f, 0.0, 0.0, 0.0, sim->slot_cnt.as<int32_t>(),
             gidx, K, sim->slot_max.as<unsigned int>(), nullptr);
     }
@@ -1387,8 +1419,7 @@ template <int D, typename S>
 int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
   const int64_t n = sim->n, nt = sim->nt;
   const double rc = 2.0 * sim->prm.op.h + sim->prm.skin;
-  bool halo_lists = false;
-  if constexpr (D == 2) halo_lists = sim->tiles == 2;
+  const bool halo_lists = sim->tiles == 2;
   if (sim->slot_K_next >= 0) { sim->slot_K = sim->slot_K_next; sim->slot_K_next = -1; }
   const BuildStats prev = *sim->build_stats;
   const bool redo = lagged && halo_lists && sim->built && prev.K > 0 && (prev.kmax > prev.K || prev.outside);
@@ -1427,7 +1458,7 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
     const double* npos = sim->pos[sim->cur].as<double>();
     if (halo_lists) {
       // clears the step flags; a lagged build may mark them stale again
-      TSPH_TRY(build_halo_lists(sim, npos, rc, sizeof(HaloRec<D, S>), lagged, stream));
+      TSPH_TRY(build_halo_lists<D>(sim, npos, rc, sizeof(HaloRec<D, S>), lagged, stream));
       if (lagged && sim->slot_K > 0 && sim->halo_ok) break;
       if (!sim->build_stats->outside) break;
     } else {
@@ -1530,9 +1561,16 @@ int queue_list_sweeps(tsph_sim* sim, int64_t k, bool meters, hipStream_t stream)
   const unsigned gx = grid_xcd(n);
   const int r = (int)(k % kRing);
   if (sim->tstep) TSPH_CHECK_HIP(hipEventRecord(sim->tev[r][0], stream));
-  // halo mode (2-D): the density and force sweeps stage their tiles' halos
-  constexpr bool kHalo2 = D == 2;
-  const bool halo = kHalo2 && sim->tiles == 2 && sim->halo_ok;
+  // halo mode: the density and force sweeps stage their tiles' halos
+  constexpr bool kHalo2 = true;
+  const bool halo = sim->tiles == 2 && sim->halo_ok;
+  if constexpr (D == 3) {
+    if (halo) {
+      TSPH_TRY((allow_halo_lds<k_step_density<D, S, NbSlots, true, false>>()));
+      TSPH_TRY((allow_halo_lds<k_step_force<D, S, true, NbSlots, true>>()));
+      TSPH_TRY((allow_halo_lds<k_step_force<D, S, false, NbSlots, true>>()));
+    }
+  }
   const HaloView hv = halo ? sim_halo(sim) : HaloView{};
   const NbHalo hl = halo ? sim_halo_lists(sim) : NbHalo{};
   const size_t halo_density_bytes = halo ? (size_t)sim->halo_max * (D + 1) * sizeof(double) : 0;
@@ -1891,10 +1929,11 @@ int tsph_sim_create(const tsph_sim_params* prm, const double* pos, const double*
   if (he != hipSuccess) { set_error("sim init: %s", hipGetErrorString(he)); tsph_sim_destroy(sim); return TSPH_ERR_HIP; }
   k_iota<<<grid_for(nt), kBlock, 0, stream>>>(sim->orig.as<int64_t>(), nt);
   k_init_time<<<1, 1, 0, stream>>>(sim->flags.as<SimFlags>(), sim_dtctl(sim));
-  // 2-D runs default to the slot lists with LDS halos (measured fastest on
-  // MI355X at 1.6M fluid rows in FP32 and FP64); the cell tiles stay selectable
+  // runs default to the slot lists with LDS halos (measured fastest on
+  // MI355X: 2-D dam break at 1.6 M fluid rows, 3-D box case at 20 M, FP32
+  // and FP64); the cell tiles (2-D) and the plain lists stay selectable
   sim->tile_dim[0] = 8; sim->tile_dim[1] = 4;
-  if (sim->dim == 2) sim->tiles = 2;
+  sim->tiles = 2;
   *out = sim;
   return TSPH_OK;
 }
@@ -2013,7 +2052,7 @@ int tsph_sim_loads(tsph_sim* sim, double* rows, int64_t max_rows, int64_t* nrows
 int tsph_sim_configure(tsph_sim* sim, int tiles, int tile_x, int tile_y) {
   TSPH_REQUIRE(sim, TSPH_ERR_ARG, "sim must not be NULL");
   TSPH_REQUIRE(tiles >= 0 && tiles <= 2, TSPH_ERR_ARG, "layout must be 0 (lists), 1 (cell tiles) or 2 (halos)");
-  TSPH_REQUIRE(tiles == 0 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile and halo sweeps are 2-D");
+  TSPH_REQUIRE(tiles != 1 || sim->dim == 2, TSPH_ERR_ARG, "cell-tile sweeps are 2-D");
   TSPH_REQUIRE(tiles != 1 || !sim->obstacle, TSPH_ERR_ARG, "obstacle loads run on the slot-list sweeps");
   TSPH_REQUIRE(tiles != 1 || !(sim->prm.op.diffusion_scale > 0.0), TSPH_ERR_ARG,
                "density diffusion runs on the slot-list sweeps");

@@ -1,4 +1,4 @@
-// Neighbourhood halos of the 2-D resident stepper's slot-list sweeps.
+// Neighbourhood halos of the resident stepper's slot-list sweeps (2-D and 3-D).
 //
 // The rows of a workgroup (kHaloRows consecutive fluid rows, a tile) are
 // adjacent in the binning's blocked Morton order, so their neighbour lists
@@ -26,12 +26,14 @@
 namespace tsph {
 
 constexpr int kHaloRows = 256;     // rows per tile: the sweeps' workgroup
-constexpr int kHaloCap = 1024;     // halo rows a tile may stage
+// halo rows a tile may stage: about 3x a 2-D tile's rows, 7-10x a 3-D tile's
+// (the 27 cells around each of a tile's ten or so cells, ~25 rows a cell)
+template <int D> constexpr int halo_cap() { return D == 2 ? 1024 : 2816; }
 
 // Per tile: halo rows, fluid halo rows first (x = all, y = fluid; x = -1: the
 // tile keeps the global path).
 struct HaloView {
-  const uint32_t* rows;            // (tiles, kHaloCap)
+  const uint32_t* rows;            // (tiles, halo_cap<D>())
   const int2* cnt;
   const uint16_t* idx;             // lists of halo positions, NbSlots layout
   int64_t ntiles;
@@ -125,89 +127,121 @@ TSPH_D int halo_floor_search(const int* a, int m, int x) {
 constexpr int kHaloUniq = 512;     // distinct neighbourhood cells a tile may hold
 constexpr int kHaloBox = 1024;     // cells of the box around a tile's cells
 
-// One workgroup per tile of kHaloRows fluid rows (2-D): the tile rows' slot
-// lists at the build radius (rc2) in both forms, global rows (idx, the
-// NbSlots layout) and halo positions (lidx), and the tile's halo.
+// relative positions of the builder's halo rows in LDS
+template <int D> struct HaloPosT;
+template <> struct HaloPosT<2> { using T = float2; };
+template <> struct HaloPosT<3> { using T = float4; };
+
+// One workgroup per tile of kHaloRows fluid rows: the tile rows' slot lists
+// at the build radius (rc2) in both forms, global rows (idx, the NbSlots
+// layout) and halo positions (lidx), and the tile's halo.
 //
 // The tile's distinct cells are the runs of equal cells among its rows
-// (sorted by binning key).  Each marks the 3 x 3 cells around it in a table
+// (sorted by binning key).  Each marks the 3^D cells around it in a table
 // over the box of the tile's cells widened by one cell; the marked cells,
 // numbered in box order, are the neighbourhood, and every (cell, offset)
 // candidate reads its cell's number u from the table.  The halo holds the
 // fluid rows of those cells in that order, then their wall rows, and their
 // positions are staged in LDS.  A row then walks the cells of its own
-// cell's nine candidates, in the cell walk's order (offsets x-major, fluid
-// rows before wall rows), tests each row by the squared distance (an FP32
-// test on tile-relative positions, the FP64 test within its rounding band:
-// the test the other list builds decide by) and appends the accepted rows to
-// both lists.  A tile with too many cells or rows for its LDS walks the
-// global cell table instead and gets no halo (count -1).
+// cell's 3^D candidates, in the cell walk's order (offsets with the last axis
+// fastest, fluid rows before wall rows), tests each row by the squared
+// distance (an FP32 test on tile-relative positions, the FP64 test within
+// its rounding band: the test the other list builds decide by) and appends
+// the accepted rows to both lists.  A tile with too many cells or rows for
+// its LDS walks the global cell table instead and gets no halo (count -1).
 //
 // cnt, idx: per-row count and global lists (K slots); lidx: halo lists, the
 // same layout with 16-bit entries; blk[2 * tile]: the tile's longest list and
 // (halo rows + 1, or 0 without a halo).
-template <bool FILL>
-__global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const double* __restrict__ pos,
-                                                           int64_t n, double rc2, int cap,
-                                                           int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
-                                                           uint16_t* __restrict__ lidx, int64_t K,
-                                                           uint32_t* __restrict__ halo_rows,
-                                                           int2* __restrict__ halo_cnt,
-                                                           unsigned int* __restrict__ blk) {
-  constexpr int D = 2, NOFF = 9;
+template <int D, bool FILL>
+__global__ void __launch_bounds__(kHaloRows) k_halo_lists(NbCells cg, const double* __restrict__ pos,
+                                                          int64_t n, double rc2, int cap,
+                                                          int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
+                                                          uint16_t* __restrict__ lidx, int64_t K,
+                                                          uint32_t* __restrict__ halo_rows,
+                                                          int2* __restrict__ halo_cnt,
+                                                          unsigned int* __restrict__ blk) {
+  constexpr int NOFF = D == 2 ? 9 : 27;
+  constexpr int CAP = halo_cap<D>();
+  using HP = typename HaloPosT<D>::T;
   __shared__ int16_t tbl[kHaloBox];                    // the tile's box of cells: -1, 0 (marked), or the cell's number
   __shared__ int16_t cand_u[kHaloRows * NOFF];
   __shared__ int ufs[kHaloUniq], ufl[kHaloUniq], ufo[kHaloUniq], uws[kHaloUniq], uwl[kHaloUniq], uwo[kHaloUniq];
-  __shared__ float2 hx[kHaloCap];                      // halo positions relative to the tile's first row
+  __shared__ HP hx[CAP];                               // halo positions relative to the tile's first row
   __shared__ int wtot[4];
-  __shared__ int s_box[4][kHaloRows / 64];
+  __shared__ int s_box[2 * D][kHaloRows / 64];
   __shared__ int s_kmax[4];
   __shared__ float s_ext[4];
   const int t = threadIdx.x;
   const int64_t tile = blockIdx.x;
   const int64_t r = tile * kHaloRows + t;
   const bool live = r < n;
-  int c[2] = {0, 0};
+  int c[D];
   bool head = false;
+#pragma unroll
+  for (int d = 0; d < D; ++d) c[d] = live ? (int)cg.cells[r * D + d] : 0;
   if (live) {
-    c[0] = (int)cg.cells[r * D];
-    c[1] = (int)cg.cells[r * D + 1];
-    head = t == 0 || cg.cells[(r - 1) * D] != c[0] || cg.cells[(r - 1) * D + 1] != c[1];
+    head = t == 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) head = head || cg.cells[(r - 1) * D + d] != c[d];
   }
   int nheads;
   const int hpos = halo_block_scan(head ? 1 : 0, wtot, &nheads);
   const int hidx = hpos + (head ? 1 : 0) - 1;          // this row's cell among the tile's
   // the box of the tile's cells widened by one cell, cut to the grid
-  int bb[4] = {live ? c[0] : INT_MAX, live ? c[1] : INT_MAX, live ? c[0] : INT_MIN, live ? c[1] : INT_MIN};
+  int bb[2 * D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) { bb[d] = live ? c[d] : INT_MAX; bb[D + d] = live ? c[d] : INT_MIN; }
   for (int o = 32; o > 0; o >>= 1) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2 * D; ++q) {
       const int y = __shfl_xor(bb[q], o, 64);
-      bb[q] = q < 2 ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
+      bb[q] = q < D ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
     }
   }
   if ((t & 63) == 0)
-    for (int q = 0; q < 4; ++q) s_box[q][t >> 6] = bb[q];
+    for (int q = 0; q < 2 * D; ++q) s_box[q][t >> 6] = bb[q];
   __syncthreads();
   for (int w = 0; w < kHaloRows / 64; ++w)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2 * D; ++q) {
       const int y = s_box[q][w];
-      bb[q] = q < 2 ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
+      bb[q] = q < D ? (y < bb[q] ? y : bb[q]) : (y > bb[q] ? y : bb[q]);
     }
-  const int x0 = bb[0] - 1 > 0 ? bb[0] - 1 : 0, y0 = bb[1] - 1 > 0 ? bb[1] - 1 : 0;
-  const int x1 = bb[2] + 1 < (int)cg.ext[0] - 1 ? bb[2] + 1 : (int)cg.ext[0] - 1;
-  const int y1 = bb[3] + 1 < (int)cg.ext[1] - 1 ? bb[3] + 1 : (int)cg.ext[1] - 1;
-  const int bw = x1 - x0 + 1, bh = y1 - y0 + 1;
-  bool ok = bw * bh <= kHaloBox;                       // block-uniform
+  int b0[D], be[D];                                    // box origin and edge lengths (cells)
+  int64_t nbox = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    b0[d] = bb[d] - 1 > 0 ? bb[d] - 1 : 0;
+    const int top = bb[D + d] + 1 < (int)cg.ext[d] - 1 ? bb[D + d] + 1 : (int)cg.ext[d] - 1;
+    be[d] = top - b0[d] + 1;
+    nbox *= be[d];
+  }
+  // box number of cell c + offset o (o: last axis fastest, as the cell walk),
+  // or -1 outside the box
+  auto box_of = [&](const int* cc, int o) -> int {
+    int e = 0, rem = o;
+    bool in = true;
+    int off[D];
+#pragma unroll
+    for (int d = D - 1; d >= 0; --d) { off[d] = rem % 3 - 1; rem /= 3; }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int v = cc[d] + off[d] - b0[d];
+      in = in && v >= 0 && v < be[d];
+      e = e * be[d] + v;
+    }
+    return in ? e : -1;
+  };
+  bool ok = nbox <= kHaloBox;                          // block-uniform
   int F = 0, W = 0;
   if (ok) {
     for (int e = t; e < kHaloBox; e += kHaloRows) tbl[e] = -1;
     __syncthreads();
     if (head) {
       for (int o = 0; o < NOFF; ++o) {
-        const int nx = c[0] + (o / 3 - 1), ny = c[1] + (o % 3 - 1);
-        if (nx >= x0 && nx <= x1 && ny >= y0 && ny <= y1) tbl[(nx - x0) * bh + (ny - y0)] = 0;
+        const int e = box_of(c, o);
+        if (e >= 0) tbl[e] = 0;
       }
     }
     __syncthreads();
@@ -216,7 +250,7 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int e = 4 * t + q;
-      flag[q] = e < bw * bh && tbl[e] == 0 ? 1 : 0;
+      flag[q] = e < (int)nbox && tbl[e] == 0 ? 1 : 0;
       fsum += flag[q];
     }
     const int ubase = halo_block_scan(fsum, wtot, &nu);
@@ -228,8 +262,14 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
         if (!flag[q]) continue;
         const int e = 4 * t + q;
         tbl[e] = (int16_t)u;
-        const int64_t cx = x0 + e / bh, cy = y0 + e % bh;
-        const int4 R = cg.R[cx * cg.stride[0] + cy * cg.stride[1]];
+        int64_t L = 0;
+        int rem = e;
+#pragma unroll
+        for (int d = D - 1; d >= 0; --d) {
+          L += (int64_t)(b0[d] + rem % be[d]) * cg.stride[d];
+          rem /= be[d];
+        }
+        const int4 R = cg.R[L];
         ufs[u] = R.x; ufl[u] = R.y - R.x; ufo[u] = R.y - R.x;
         uws[u] = R.z; uwl[u] = R.w - R.z; uwo[u] = R.w - R.z;
         ++u;
@@ -238,9 +278,8 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
     __syncthreads();
     if (ok && head) {
       for (int o = 0; o < NOFF; ++o) {
-        const int nx = c[0] + (o / 3 - 1), ny = c[1] + (o % 3 - 1);
-        const bool in = nx >= x0 && nx <= x1 && ny >= y0 && ny <= y1;
-        cand_u[hpos * NOFF + o] = in ? tbl[(nx - x0) * bh + (ny - y0)] : (int16_t)-1;
+        const int e = box_of(c, o);
+        cand_u[hpos * NOFF + o] = e >= 0 ? tbl[e] : (int16_t)-1;
       }
     }
     if (ok) {
@@ -251,10 +290,10 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
     if (ok) {
       // the halo's rows and their positions: fluid rows by cell, then wall
       // rows; slot s of the halo finds its cell by the offsets, and every
-      // thread issues its (up to kHaloCap / kHaloRows) loads together
-      uint32_t* hr = halo_rows + tile * kHaloCap;
+      // thread issues its (up to CAP / kHaloRows) loads together
+      uint32_t* hr = halo_rows + tile * CAP;
       const int H = F + W;
-      constexpr int QS = kHaloCap / kHaloRows;
+      constexpr int QS = CAP / kHaloRows;
       int32_t jq[QS];
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
@@ -278,18 +317,27 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
         }
         jq[q] = j;
       }
-      const double ox = pos[tile * kHaloRows * 2], oy = pos[tile * kHaloRows * 2 + 1];
-      double2 xq[QS];
+      double org[D];
 #pragma unroll
-      for (int q = 0; q < QS; ++q) xq[q] = make_double2(pos[(int64_t)jq[q] * 2], pos[(int64_t)jq[q] * 2 + 1]);
+      for (int d = 0; d < D; ++d) org[d] = pos[tile * kHaloRows * D + d];
+      double xq[QS][D];
+#pragma unroll
+      for (int q = 0; q < QS; ++q)
+#pragma unroll
+        for (int d = 0; d < D; ++d) xq[q][d] = pos[(int64_t)jq[q] * D + d];
       float ext = 0.f;
 #pragma unroll
       for (int q = 0; q < QS; ++q) {
         const int s2 = t + q * kHaloRows;
         if (s2 < H) {
-          const float2 v = make_float2((float)(xq[q].x - ox), (float)(xq[q].y - oy));
-          hx[s2] = v;
-          ext = fmaxf(ext, fmaxf(fabsf(v.x), fabsf(v.y)));
+          float v[D];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            v[d] = (float)(xq[q][d] - org[d]);
+            ext = fmaxf(ext, fabsf(v[d]));
+          }
+          if constexpr (D == 2) hx[s2] = make_float2(v[0], v[1]);
+          else hx[s2] = make_float4(v[0], v[1], v[2], 0.f);
         }
       }
       for (int o = 32; o > 0; o >>= 1) ext = fmaxf(ext, __shfl_xor(ext, o, 64));
@@ -300,7 +348,9 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
   // the lists
   int k = 0;
   if (live) {
-    const double xi[2] = {pos[r * 2], pos[r * 2 + 1]};
+    double xi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xi[d] = pos[r * D + d];
     int4* gp = FILL ? reinterpret_cast<int4*>(idx) + (r >> 6) * 16 * K + (r & 63) : nullptr;
     ushort4* hp = FILL ? reinterpret_cast<ushort4*>(lidx) + (r >> 6) * 16 * K + (r & 63) : nullptr;
     const int chunks = (int)(K >> 2);
@@ -325,44 +375,51 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists2(NbCells cg, const dou
       // the test) is decided by the FP64 test on the global positions.  The
       // relative coordinates carry at most e0 = 2^-24 ext of rounding, a
       // difference at most e = 2 e0 + 2^-24 1.01 rc, and the squared distance
-      // near rc^2 at most 2 sqrt(2) rc e + 2 e^2 + 4.4 2^-24 rc^2; the band
-      // is twice that.
+      // near rc^2 at most 2 sqrt(D) rc e + D e^2 + (2.2 D) 2^-24 rc^2 (the
+      // rounding of the D squares and their sum); the band is twice that.
       const float ext = fmaxf(fmaxf(s_ext[0], s_ext[1]), fmaxf(s_ext[2], s_ext[3]));
       const double rc = sqrt(rc2);
       const double e = 2.0 * ldexp((double)ext, -24) + ldexp(1.01 * rc, -24);
-      const double band = 2.0 * (2.0 * 1.4142135623730951 * rc * e + 2.0 * e * e + 4.4 * ldexp(rc2, -24));
+      const double band = 2.0 * (2.0 * sqrt((double)D) * rc * e + D * e * e + 2.2 * D * ldexp(rc2, -24));
       const float lo2 = (float)(rc2 - band) * (1.0f - 1.2e-7f), hi2 = (float)(rc2 + band) * (1.0f + 1.2e-7f);
-      const float2 xf = make_float2((float)(xi[0] - pos[tile * kHaloRows * 2]),
-                                    (float)(xi[1] - pos[tile * kHaloRows * 2 + 1]));
+      float xf[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xf[d] = (float)(xi[d] - pos[tile * kHaloRows * D + d]);
       for (int o = 0; o < NOFF; ++o) {
         const int u = cand_u[hidx * NOFF + o];
         if (u < 0) continue;
 #pragma unroll
         for (int wl = 0; wl < 2; ++wl) {
-          const int b0 = wl ? uws[u] : ufs[u], len = wl ? uwl[u] : ufl[u], o0 = wl ? F + uwo[u] : ufo[u];
+          const int c0 = wl ? uws[u] : ufs[u], len = wl ? uwl[u] : ufl[u], o0 = wl ? F + uwo[u] : ufo[u];
           for (int q0 = 0; q0 < len; q0 += 4) {
-            float2 cq[4];
+            HP cq[4];
 #pragma unroll
             for (int uu = 0; uu < 4; ++uu) cq[uu] = hx[o0 + (q0 + uu < len ? q0 + uu : len - 1)];
 #pragma unroll
             for (int uu = 0; uu < 4; ++uu) {
               const int q = q0 + uu;
-              const int32_t j = b0 + q;
+              const int32_t j = c0 + q;
               if (q >= len || j == (int32_t)r) continue;
-              const float dx = xf.x - cq[uu].x, dy = xf.y - cq[uu].y;
-              const float d2 = dx * dx + dy * dy;
+              float d2;
+              if constexpr (D == 2) {
+                const float dx = xf[0] - cq[uu].x, dy = xf[1] - cq[uu].y;
+                d2 = dx * dx + dy * dy;
+              } else {
+                const float dx = xf[0] - cq[uu].x, dy = xf[1] - cq[uu].y, dz = xf[2] - cq[uu].z;
+                d2 = dx * dx + dy * dy + dz * dz;
+              }
               bool in = d2 < lo2;
-              if (!in && d2 <= hi2) in = dist2_pt<double, 2, false>(xi, pos + (int64_t)j * 2, nullptr) <= rc2;
+              if (!in && d2 <= hi2) in = dist2_pt<double, D, false>(xi, pos + (int64_t)j * D, nullptr) <= rc2;
               if (in) accept(j, o0 + q);
             }
           }
         }
       }
     } else {
-      cg.ranges<2>(r, true, [&](int32_t b0, int32_t e0) {
-        for (int32_t j = b0; j < e0; ++j) {
+      cg.ranges<D>(r, true, [&](int32_t rb, int32_t re) {
+        for (int32_t j = rb; j < re; ++j) {
           if (j == (int32_t)r) continue;
-          if (dist2_pt<double, 2, false>(xi, pos + (int64_t)j * 2, nullptr) <= rc2) accept(j, 0);
+          if (dist2_pt<double, D, false>(xi, pos + (int64_t)j * D, nullptr) <= rc2) accept(j, 0);
         }
       });
     }

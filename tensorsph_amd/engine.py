@@ -248,10 +248,10 @@ def _sweep_times(self):
 
 
 def _configure(self, tiles: bool = True, tile=(8, 4), halo: bool = False) -> None:
-    """Choose the sweep layout (2-D): LDS cell tiles, or the per-row slot
-    lists (tiles False), evaluated from the global arrays or, with halo True,
-    from each workgroup's neighbourhood rows staged in LDS.  Every layout
-    gives the same bits for the same lists."""
+    """Choose the sweep layout: LDS cell tiles (2-D only), or the per-row
+    slot lists (tiles False), evaluated from the global arrays or, with halo
+    True, from each workgroup's neighbourhood rows staged in LDS (2-D and
+    3-D).  Every layout gives the same bits for the same lists."""
     if halo and tiles:
         raise ValueError("the halo layout runs on the slot lists: pass tiles=False")
     mode = 2 if halo else (1 if tiles else 0)

@@ -68,12 +68,17 @@ def make_simulation(cfg: Kleefsman3DConfig, precision: str = "fp32", device="cud
         raise ValueError(f"state holds {pos.shape} particles, this configuration builds {pos0.shape}")
     h = cfg.h_factor * cfg.dx
     wall_mass = mass_3d(cfg.dx, cfg.h_factor, cfg.rho0, cfg.density_calibration)
+    if cfg.density_diffusion > 0.0 and cfg.density_diffusion_form != "molteni":
+        # rhs_3d takes the Molteni form only (dam_break_3d.py:330-336)
+        raise ValueError("density_diffusion_rate implements the 'molteni' form only; the "
+                         f"configured form is {cfg.density_diffusion_form!r}")
     sim = WcsphSimulation(pos, vel, mass, wall, h=h, rho0=cfg.rho0,
                           sound_speed=cfg.sound_speed, gamma=cfg.gamma, gravity=cfg.gravity,
                           viscosity=cfg.viscosity, dt=cfg.dt, wall_mass=wall_mass,
                           skin=cfg.pair_skin_factor * h, lo=[0.0, 0.0, 0.0],
                           hi=[cfg.pool_width, cfg.tank_top(), cfg.tank_depth], tol=0.25 * cfg.dx,
-                          wall_friction=cfg.wall_friction, precision=precision, device=device,
+                          wall_friction=cfg.wall_friction, density_diffusion=cfg.density_diffusion,
+                          diffusion_form="molteni", precision=precision, device=device,
                           obstacle={"rows": rows, "centre": centre, "box": box, "eps": 0.01 * cfg.dx},
                           adaptive=adaptive)
     return sim, pos.shape[0], wall.shape[0]
