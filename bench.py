@@ -484,7 +484,7 @@ def kleefsman_bench(args) -> dict:
                       "sweep_ms": [x / max(nsteps, 1) for x in sweep_ms], "rebuilds": stats["rebuilds"],
                       "force": loads[-1, 1:4].tolist(), "moment": loads[-1, 4:7].tolist(), "layout": lay}
     v32, v64 = runs["fp32"]["value"], runs["fp64"]["value"]
-    history = kleefsman_history(cfg, args.t_end) if args.t_end > 0 else None
+    history = kleefsman_history(cfg, args.t_end, layout) if args.t_end > 0 else None
     s = 4
     b_force = nf * (48 * 2 + 2 * s) + nw * (24 + 3 * s)
     force_ms = runs["fp32"]["sweep_ms"][2]
@@ -521,7 +521,7 @@ def kleefsman_bench(args) -> dict:
                  history=history)
 
 
-def kleefsman_history(cfg, t_end: float) -> dict:
+def kleefsman_history(cfg, t_end: float, layout: str = "default") -> dict:
     """Both precisions integrated to t_end with the box force and moment of
     every step (kleefsman_obstacle_3d.py:644-706): peak force, impulse, first
     contact and the FP32c - FP64 difference of the histories."""
@@ -533,6 +533,8 @@ def kleefsman_history(cfg, t_end: float) -> dict:
     nf = 0
     for prec in ("fp32", "fp64"):
         sim, nf, _ = make_simulation(cfg, prec)
+        if layout != "default":
+            sim.configure(tiles=False, halo=layout == "halo")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         done = 0

@@ -1302,6 +1302,41 @@ int build_halo_lists(tsph_sim* sim, const double* npos, double rc, size_t rec_by
   return TSPH_OK;
 }
 
+// The 3-D halo layout's halos, over the lists the cell walk just built
+// (build_sim_slots, synchronised): per tile the halo and the halo lists of
+// the global lists (k_halo_lists in MAP form); synchronises for the largest
+// halo.
+template <int D>
+int build_halo_map(tsph_sim* sim, size_t rec_bytes, hipStream_t stream) {
+  const int64_t n = sim->n, nt = sim->nt;
+  sim->halo_ok = false;
+  const int64_t ntiles = (n + kHaloRows - 1) / kHaloRows;
+  sim->halo_tiles = ntiles;
+  const int64_t groups = (nt + 63) / 64;
+  constexpr int kCap = halo_cap<D>();
+  const int cap = (int)std::min<size_t>((size_t)kCap, halo_lds_budget<D>() / rec_bytes);
+  const int64_t K = sim->slot_K;
+  TSPH_TRY(sim->halo_rows.ensure(sizeof(uint32_t) * (size_t)(ntiles > 0 ? ntiles : 1) * kCap));
+  TSPH_TRY(sim->halo_cnt.ensure(sizeof(int2) * (size_t)(ntiles > 0 ? ntiles : 1)));
+  TSPH_TRY(sim->halo_hmax.ensure(sizeof(unsigned int) * (2 * (size_t)ntiles + 2)));
+  TSPH_TRY(sim->halo_idx.ensure(sizeof(uint16_t) * (size_t)groups * 64 * (size_t)K + 64));
+  unsigned int* blk = sim->halo_hmax.as<unsigned int>();
+  SimFlags* fl = sim->flags.as<SimFlags>();
+  if (ntiles > 0)
+    k_halo_lists<D, true, true><<<(unsigned)ntiles, kHaloRows, 0, stream>>>(
+        sim_cells(sim), sim->pos[sim->cur].as<double>(), n, 0.0, cap, sim->slot_cnt.as<int32_t>(),
+        sim->slot_idx.as<int32_t>(), sim->halo_idx.as<uint16_t>(), K, sim->halo_rows.as<uint32_t>(),
+        sim->halo_cnt.as<int2>(), blk);
+  k_halo_lists_reduce<<<1, kHaloRows, 0, stream>>>(blk, ntiles, sim->slot_max.as<unsigned int>(),
+                                                   sim->bin_flag.as<unsigned int>(), K, &fl->stale[0],
+                                                   sim->build_stats);
+  TSPH_CHECK_LAUNCH();
+  TSPH_CHECK_HIP(hipStreamSynchronize(stream));
+  sim->halo_max = (int)sim->build_stats->hmax;
+  sim->halo_ok = true;
+  return TSPH_OK;
+}
+
 static inline HaloView sim_halo(const tsph_sim* sim) {
   HaloView h{};
   h.rows = sim->halo_rows.as<uint32_t>();
@@ -1456,7 +1491,7 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
                                     sizeof(int64_t) * (nt - n), hipMemcpyDeviceToDevice, stream));
     }
     const double* npos = sim->pos[sim->cur].as<double>();
-    if (halo_lists) {
+    if (halo_lists && D == 2) {
       // clears the step flags; a lagged build may mark them stale again
       TSPH_TRY(build_halo_lists<D>(sim, npos, rc, sizeof(HaloRec<D, S>), lagged, stream));
       if (lagged && sim->slot_K > 0 && sim->halo_ok) break;
@@ -1465,7 +1500,10 @@ int sim_rebuild(tsph_sim* sim, hipStream_t stream, bool lagged = false) {
       TSPH_TRY((build_sim_slots<D>(sim, npos, rc, sim->bgrid.ext, nullptr, stream)));  // synchronises
       SimFlags* fl = sim->flags.as<SimFlags>();
       TSPH_CHECK_HIP(hipMemsetAsync(&fl->stale[0], 0, sizeof(unsigned long long) * 2, stream));
-      if (!*sim->bin_outside) break;
+      if (!*sim->bin_outside) {
+        if (halo_lists) TSPH_TRY(build_halo_map<D>(sim, sizeof(HaloRec<D, S>), stream));
+        break;
+      }
     }
     // a fluid row left the grid (a correction moved it past the clamp box):
     // regrid and bin every row again

@@ -153,7 +153,15 @@ template <> struct HaloPosT<3> { using T = float4; };
 // cnt, idx: per-row count and global lists (K slots); lidx: halo lists, the
 // same layout with 16-bit entries; blk[2 * tile]: the tile's longest list and
 // (halo rows + 1, or 0 without a halo).
-template <int D, bool FILL>
+//
+// MAP: the global lists are already built (the cell walk, k_slot_lists) and
+// only the halo and the halo lists are made: every entry j of a tile row's
+// list lies in the 3^D cells around the row's cell, so its halo position is
+// its cell's halo offset plus its place in the cell's fluid or wall run
+// (blk[2 * tile] is 0: the walk reports the longest list).  The 3-D build
+// takes this form: the walk's FP32 prefilter over the global positions
+// fills the 3-D lists in about half the time of the staged-candidate walk.
+template <int D, bool FILL, bool MAP = false>
 __global__ void __launch_bounds__(kHaloRows) k_halo_lists(NbCells cg, const double* __restrict__ pos,
                                                           int64_t n, double rc2, int cap,
                                                           int32_t* __restrict__ cnt, int32_t* __restrict__ idx,
@@ -288,7 +296,7 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists(NbCells cg, const doub
       ok = F + W <= cap;
     }
     if (ok) {
-      // the halo's rows and their positions: fluid rows by cell, then wall
+      // the halo's rows and (unless MAP) their positions: fluid rows by cell, then wall
       // rows; slot s of the halo finds its cell by the offsets, and every
       // thread issues its (up to CAP / kHaloRows) loads together
       uint32_t* hr = halo_rows + tile * CAP;
@@ -317,6 +325,7 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists(NbCells cg, const doub
         }
         jq[q] = j;
       }
+      if constexpr (!MAP) {
       double org[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) org[d] = pos[tile * kHaloRows * D + d];
@@ -343,7 +352,42 @@ __global__ void __launch_bounds__(kHaloRows) k_halo_lists(NbCells cg, const doub
       for (int o = 32; o > 0; o >>= 1) ext = fmaxf(ext, __shfl_xor(ext, o, 64));
       if ((t & 63) == 0) s_ext[t >> 6] = ext;
       __syncthreads();
+      }
     }
+  }
+  if constexpr (MAP) {
+    // the halo positions of the global lists' entries
+    if (live && ok) {
+      const int nr = cnt[r];
+      const int4* gp = reinterpret_cast<const int4*>(idx) + (r >> 6) * 16 * K + (r & 63);
+      ushort4* hp = reinterpret_cast<ushort4*>(lidx) + (r >> 6) * 16 * K + (r & 63);
+      for (int k0 = 0; k0 < nr; k0 += 4) {
+        const int4 q = gp[(k0 >> 2) * 64];
+        const int32_t j4[4] = {q.x, q.y, q.z, q.w};
+        uint16_t l4[4];
+#pragma unroll
+        for (int uu = 0; uu < 4; ++uu) {
+          int l = 0;
+          if (k0 + uu < nr) {
+            const int32_t j = j4[uu];
+            int cj[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) cj[d] = (int)cg.cells[(int64_t)j * D + d];
+            const int e = box_of(cj, NOFF / 2);            // offset NOFF / 2: the cell itself
+            const int u = e >= 0 ? tbl[e] : -1;            // always a marked cell (the walk's)
+            if (u >= 0) l = j < n ? ufo[u] + (j - ufs[u]) : F + uwo[u] + (j - uws[u]);
+          }
+          l4[uu] = (uint16_t)l;
+        }
+        hp[(k0 >> 2) * 64] = make_ushort4(l4[0], l4[1], l4[2], l4[3]);
+      }
+    }
+    if (t == 0) {
+      blk[2 * tile] = 0u;
+      blk[2 * tile + 1] = ok ? (unsigned int)(F + W + 1) : 0u;
+      halo_cnt[tile] = ok ? make_int2(F + W, F) : make_int2(-1, -1);
+    }
+    return;
   }
   // the lists
   int k = 0;
