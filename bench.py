@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 DP_BYTES_PER_UPDATE = 250
 PROFILE = os.path.join(HERE, "profiles", "k_tile_force_dambreak_1p6M_fp32c.md")
 PROFILE_LISTS = os.path.join(HERE, "profiles", "k_step_force_dambreak_1p6M_fp32c.md")
-PROFILE_3D = os.path.join(HERE, "profiles", "k_step_force_kleefsman_5M_fp32c.md")
+PROFILE_3D = os.path.join(HERE, "profiles", "k_step_force_halo_kleefsman_20M_fp32c.md")
 PROFILE_HALO = os.path.join(HERE, "profiles", "k_step_force_halo_dambreak_1p6M_fp32c.md")
 
 
