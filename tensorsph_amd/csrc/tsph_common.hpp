@@ -348,5 +348,12 @@ inline unsigned grid_xcd(int64_t n, int block = kBlock) {
   g = ((g + 7) / 8) * 8;
   return (unsigned)(g > 0 ? g : 8);
 }
+// Row block of the calling workgroup: xcd_block on a grid padded by grid_xcd
+// (neighbouring row blocks share one XCD's L2), the plain block index on any
+// other grid.
+TSPH_D int64_t row_block() {
+  const int64_t nb = gridDim.x;
+  return (nb & 7) ? (int64_t)blockIdx.x : xcd_block(blockIdx.x, nb);
+}
 
 }  // namespace tsph

@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
                                                      int32_t* __restrict__ sup_idx, int32_t* __restrict__ sup_cnt,
                                                      int64_t K) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
@@ -133,7 +133,7 @@ __global__ void __launch_bounds__(kBlock) k3_density(ThreedArgs<S> a, NB nb, S* 
 template <typename S, typename NB>
 void launch_density3(const ThreedArgs<S>& a, NB nb, S* out, int32_t* sup_idx, int32_t* sup_cnt, int64_t K,
                      bool uniform_mass, hipStream_t stream) {
-  const unsigned g = grid_for(a.n);
+  const unsigned g = grid_xcd(a.n);
   if (uniform_mass)
     k3_density<S, NB, 2><<<g, kBlock, 0, stream>>>(a, nb, out, sup_idx, sup_cnt, K);
   else if (a.mass_lo)
@@ -146,7 +146,7 @@ void launch_density3(const ThreedArgs<S>& a, NB nb, S* out, int32_t* sup_idx, in
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_gradient(ThreedArgs<S> a, NB nb, const S* __restrict__ rho) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   if (i >= a.nf) { a.gr[3 * i] = a.gr[3 * i + 1] = a.gr[3 * i + 2] = S(0); return; }
   double xi[3];
@@ -190,7 +190,7 @@ template <typename S, int FORM, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S* __restrict__ rho_in,
                                                  int diffusion) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double rho_d = (double)rho_in[i] + (a.rho_lo ? (double)a.rho_lo[i] : 0.0);
   if (diffusion) {
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(kBlock) k3_eos(ThreedArgs<S> a, NB nb, const S
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
   if (skipped(a)) return;
-  const int64_t w = a.nf + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t w = a.nf + row_block() * blockDim.x + threadIdx.x;
   if (w >= a.n) return;
   double xw[3];
   load3(a.pos, w, xw);
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) k3_adami(ThreedArgs<S> a, NB nb) {
 template <typename S, typename NB>
 __global__ void __launch_bounds__(kBlock) k3_moment(ThreedArgs<S> a, NB nb) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) k3_force(ThreedArgs<S> a, NB nb, const
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const bool visc = a.nu != S(0);
   double xi[3], vi[3] = {0.0, 0.0, 0.0};
@@ -408,7 +408,7 @@ __global__ void __launch_bounds__(kBlock) k3_visc4(ThreedArgs<S> a, NB nb, const
                                                    S* __restrict__ acc_out, double half_dt,
                                                    double* __restrict__ vel_out) {
   if (skipped(a)) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = row_block() * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   double xi[3];
   load3(a.pos, i, xi);
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kBlock) k3_visc4(ThreedArgs<S> a, NB nb, const
 template <typename S, bool KICK, typename NB, int MW, bool PK>
 void queue_force(const ThreedArgs<S>& a, NB nb, const int64_t* orig, S* acc_out, double half_dt,
                  double* vel_out, hipStream_t stream) {
-  const unsigned g = grid_for(a.n);
+  const unsigned g = grid_xcd(a.n);
   if (a.order4) {
     k3_force<S, false, NB, MW, PK><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, 0.0, nullptr);
     k3_visc4<S, KICK, NB, MW><<<g, kBlock, 0, stream>>>(a, nb, orig, acc_out, half_dt, vel_out);
@@ -536,7 +536,7 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   if (n == 0) return TSPH_OK;
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr; a.rho_lo = b.rho_lo;
   a.pack = b.pack; a.fourth = b.fourth; a.lap1 = b.lap1;
-  const unsigned g = grid_for(n);
+  const unsigned g = grid_xcd(n);
   const bool diff = q.diff_scale > 0.0;
   // with support lists (slot lists only) the density sweep writes them and
   // every later sweep reads them instead of the build-radius lists
@@ -561,7 +561,7 @@ int threed_sweeps(ThreedArgs<S> a, const ThreedPrm& q, const ThreedScratch<S>& b
   } else {
     k3_eos<S, 0, NB><<<g, kBlock, 0, stream>>>(a, nb, b.rho, 0);
   }
-  if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_for(n - a.nf), kBlock, 0, stream>>>(a, nb);
+  if (a.nf < n && a.adami) k3_adami<S, NB><<<grid_xcd(n - a.nf), kBlock, 0, stream>>>(a, nb);
   if (a.nu != S(0) && (a.renormalise || a.order4)) k3_moment<S, NB><<<g, kBlock, 0, stream>>>(a, nb);
   if (uniform_mass)
     queue_force<S, KICK, NB, 2, true>(a, nb, orig, acc_out, half_dt, vel_out, stream);
@@ -625,7 +625,7 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
   a.rho = b.rho; a.rho_u = b.rho_u; a.p = b.p; a.moment = b.moment; a.gr = b.gr;
   a.fourth = b.fourth; a.lap1 = b.lap1;
   const NbList nb{ctx->nb_off.as<int64_t>(), ctx->nb_idx.as<int32_t>()};
-  const unsigned g = grid_for(n);
+  const unsigned g = grid_xcd(n);
   S* rho_in = (S*)rho;           // read only
   S* p_in = (S*)p;               // read only
   switch (op) {
@@ -645,7 +645,7 @@ int threed_operator(tsph_ctx* ctx, const tsph_threed_params* prm, int op, const 
       if (nf == n) return TSPH_OK;
       TSPH_CHECK_HIP(hipMemcpyAsync(b.p, p_in, sizeof(S) * n, hipMemcpyDeviceToDevice, stream));
       a.rho = rho_in;
-      k3_adami<S, NbList><<<grid_for(n - nf), kBlock, 0, stream>>>(a, nb);
+      k3_adami<S, NbList><<<grid_xcd(n - nf), kBlock, 0, stream>>>(a, nb);
       TSPH_CHECK_LAUNCH();
       TSPH_CHECK_HIP(hipMemcpyAsync(out, b.p + nf, sizeof(S) * (n - nf), hipMemcpyDeviceToDevice, stream));
       break;
