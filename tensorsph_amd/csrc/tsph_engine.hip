@@ -907,7 +907,9 @@ __global__ void __launch_bounds__(kBlock) k_step_force(WcsphArgs<S> a, NB nbr, S
 #pragma unroll
       for (int d = 0; d < D; ++d) acc_out[i * D + d] = acc[d];
     }
-    mi = (double)a.mass[i]; rhoi = (double)a.rho[i]; pi_ = (double)a.p[i];
+    // the meters and the clamp impulse take the FP32 run's mass as its two parts
+    mi = (double)a.mass[i] + (a.mass_lo ? (double)a.mass_lo[i] : 0.0);
+    rhoi = (double)a.rho[i]; pi_ = (double)a.p[i];
   }
   // ---- integrate + clamp (droplet_impact.py:1745-1756, :1293-1312)
   const double dt = fl->step_dt;

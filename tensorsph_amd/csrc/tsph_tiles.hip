@@ -330,7 +330,8 @@ __global__ void __launch_bounds__(kBlock) k_tile_force(WcsphArgs<S> a, TileGeom 
     const double rhoi = (double)a.rho[r];
     mx_rho = fmax(mx_rho, rhoi);
     if (METERS) {
-      const double mi = (double)a.mass[r], pi_ = (double)a.p[r];
+      // the FP32 run's mass as its two parts
+      const double mi = (double)a.mass[r] + (a.mass_lo ? (double)a.mass_lo[r] : 0.0), pi_ = (double)a.p[r];
       ms[MS_MASS] += mi; ms[MS_MX] += mi * vi0; ms[MS_MY] += mi * vi1;
       ms[MS_KE] += 0.5 * (mi * spd2); ms[MS_RHO] += rhoi; ms[MS_PE] += mi * st.gravity * xi1;
       mn_rho = fmin(mn_rho, rhoi); mx_p = fmax(mx_p, pi_); mn_p = fmin(mn_p, pi_);

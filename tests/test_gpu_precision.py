@@ -100,3 +100,53 @@ def test_fp32c_force_on_deep_column_within_1e5_of_fp64(column, state, layout):
           flush=True)
     assert acc < 1e-5
     assert rho < 2e-7
+
+
+def test_fp32c_run_tracks_the_fp64_run_over_2000_steps():
+    """The dam break of the bench at its smallest size (100k fluid), run from
+    rest for 2,000 steps in each precision on the default layout, with the
+    mass / momentum / energy meters every 20 steps.  Bars: the mass meter
+    equal to 1e-12 relative (the FP32 run meters its masses as two floats);
+    momentum, kinetic and potential energy and the density sum within 1e-6 of
+    their largest magnitude over the run at every metered step (measured on
+    MI355X: at most 8.4e-8); the list-build counts equal; the final positions
+    within 1e-6 of the column height (measured 1.6e-8).  Per-particle final
+    velocities and densities carry 2,000 steps of accumulated FP32 force
+    rounding (about 1e-6 of the largest acceleration per step): bars 2e-4 of
+    the largest speed and 2e-5 of the largest density (measured 7.1e-5 and
+    6.9e-6)."""
+    from tensorsph_amd.engine import METER_KEYS
+    cfg, pos, mass, wall, wmass = _dam_break(1.0e5)
+    print(f"dam break run: {pos.shape[0]} fluid + {wall.shape[0]} wall, 2000 steps per precision", flush=True)
+    runs = {}
+    for prec in ("fp32", "fp64"):
+        sim = _sim(cfg, pos, np.zeros_like(pos), mass, wall, wmass, prec, "halo")
+        try:
+            sim.step(2000, meter_every=20)
+            r = sim.read()
+            runs[prec] = (sim.meters(), {k: r[k].double().cpu().numpy() for k in ("position", "velocity", "density")},
+                          sim.stats())
+        finally:
+            sim.close()
+    m32, s32, st32 = runs["fp32"]
+    m64, s64, st64 = runs["fp64"]
+    assert m32.shape == m64.shape and m32.shape[0] >= 100
+    col = {k: i for i, k in enumerate(METER_KEYS)}
+    assert np.array_equal(m32[:, col["step"]], m64[:, col["step"]])
+    mass_err = float(np.abs(m32[:, col["mass"]] / m64[:, col["mass"]] - 1.0).max())
+    errs = {}
+    for k in ("momentum_x", "momentum_y", "kinetic_energy", "potential_energy", "rho_sum"):
+        a, b = m32[:, col[k]], m64[:, col[k]]
+        errs[k] = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    x_err = float(np.abs(s32["position"] - s64["position"]).max() / cfg.column_height)
+    v_err = _scaled(s32["velocity"], s64["velocity"])
+    r_err = _scaled(s32["density"], s64["density"])
+    print(f"  mass {mass_err:.2e}; " + ", ".join(f"{k} {v:.2e}" for k, v in errs.items())
+          + f"; final x {x_err:.2e}, v {v_err:.2e}, rho {r_err:.2e}; rebuilds {st32['rebuilds']} / {st64['rebuilds']}",
+          flush=True)
+    assert mass_err < 1e-12
+    for k, v in errs.items():
+        assert v < 1e-6, k
+    assert x_err < 1e-6
+    assert v_err < 2e-4 and r_err < 2e-5
+    assert st32["rebuilds"] == st64["rebuilds"]
